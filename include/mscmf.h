@@ -1,0 +1,227 @@
+/*
+ * mscmf.h -- C ABI of the MI355X exact max-flow library (libmscmf.so).
+ *
+ * Plain C types only: pointers, sizes, int32/int64.  Every call returns an
+ * int status (0 = ok, < 0 = error, message from mf_last_error()); nothing
+ * aborts.  The caller owns host arrays (the library copies them); the
+ * library owns device memory.  Handles are independent: separate engines
+ * may be driven from separate threads.
+ *
+ * Citations are to xiyuweiguang/MScResearch,
+ * Region_Aware_PR_with_Pruning_BoundaryCore.cu ("BC:line"), the path these
+ * entry points replace.  The reference has no FFI of its own; its in-process
+ * boundary is run_gpu_static_with_state (BC:2119-2123) and its process
+ * boundary is the `<in> <out> static|dynamic` command line (BC:2561-2933),
+ * which tools/mscmf (mf_cli.cpp) reproduces on top of this ABI.
+ */
+#ifndef MSCMF_H
+#define MSCMF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define MF_OK 0
+#define MF_EINVAL (-1)
+#define MF_ENOMEM (-2)
+#define MF_EHIP (-3)
+#define MF_EIO (-4)
+#define MF_EITER (-5)  /* iteration watchdog (BC:1893-1896 returned silently) */
+#define MF_ECERT (-6)  /* min-cut certificate failed */
+#define MF_ENODEV (-7)
+#define MF_EBUSY (-8)
+
+typedef struct mf_hgraph mf_hgraph; /* host CSR + partition */
+typedef struct mf_edges mf_edges;   /* generated edge list */
+typedef struct mf_graph mf_graph;   /* device-resident region-ordered graph */
+typedef struct mf_engine mf_engine; /* slots + scheduler + stream */
+
+const char* mf_last_error(void);
+int mf_version(void);
+
+/* ----------------------------------------------------------- host graph */
+
+/* Parse graph.txt and build the CSR with reverse entries.
+ * Replaces build_graph (BC:1044-1126): header "n m", up to m lines "u v cap",
+ * stop at the first malformed line, skip out-of-range ids, self-loops and
+ * cap <= 0; forward then reverse entry per line, rev links the pair. */
+int mf_hgraph_load(const char* graph_path, mf_hgraph** out);
+
+/* Same CSR from 0-based edge arrays (no skip rules applied beyond them). */
+int mf_hgraph_from_edges(int n, int64_t cnt, const int32_t* us, const int32_t* vs,
+                         const int64_t* caps, mf_hgraph** out);
+int mf_hgraph_from_edgelist(const mf_edges* e, mf_hgraph** out);
+
+/* Capacity-degree partition + region-ordered relabel, in place.
+ * Replaces build_capacity_degree_partition_from_stats (BC:1130-1178),
+ * partition_and_reorder (BC:1182-1260) and build_region_id (BC:1263-1276).
+ * Score ties are broken by vertex id (the reference used an unstable sort). */
+int mf_hgraph_partition(mf_hgraph* h, int R);
+
+int mf_hgraph_info(const mf_hgraph* h, int* n, int64_t* E, int* R, int64_t* lines);
+/* Copy arrays out; any pointer may be NULL.  Sizes: row_ptr n+1, col/cap/rev E,
+ * new_id/old_id/region_id n, rstart R+1. */
+int mf_hgraph_export(const mf_hgraph* h, int32_t* row_ptr, int32_t* col, int64_t* cap,
+                     int32_t* rev, int32_t* new_id, int32_t* old_id, int32_t* rstart,
+                     int32_t* region_id);
+/* Per-vertex stats of the input lines (BC:1090-1096), in the current order. */
+int mf_hgraph_stats(const mf_hgraph* h, int64_t* sum_out, int64_t* sum_in, int32_t* deg_out,
+                    int32_t* deg_in);
+void mf_hgraph_free(mf_hgraph* h);
+
+/* Stand-alone partition on given stats (BC:1130-1178), for parity tests. */
+int mf_partition(int n, int R, const int64_t* sum_out, const int64_t* sum_in,
+                 const int32_t* deg_out, const int32_t* deg_in, int32_t* part);
+
+/* test.txt reader (BC:2603-2621): 1-based pairs until a read fails; returns
+ * 0-based ids, unvalidated.  *count = pairs read (<= cap). */
+int mf_read_queries(const char* path, int64_t cap, int32_t* s, int32_t* t, int64_t* count);
+
+/* -------------------------------------------------------------- generators
+ * Seeded splitmix64 -> xoshiro256** streams; identical output on any host. */
+int mf_gen_uniform(int n, int64_t m, int64_t cap_max, uint64_t seed, mf_edges** out);
+int mf_gen_grid(int rows, int cols, int64_t cap_max, uint64_t seed, mf_edges** out);
+int mf_gen_rmat(int scale, int64_t samples, double a, double b, double c, int64_t cap_max,
+                uint64_t seed, int threads, mf_edges** out);
+int mf_edges_info(const mf_edges* e, int* n, int64_t* cnt);
+int mf_edges_copy(const mf_edges* e, int32_t* us, int32_t* vs, int64_t* caps);
+int mf_edges_write(const mf_edges* e, const char* graph_path);
+void mf_edges_free(mf_edges* e);
+int mf_gen_queries_uniform(int n, int q, uint64_t seed, int32_t* s, int32_t* t);
+int mf_gen_queries_grid(int rows, int cols, int q, uint64_t seed, int32_t* s, int32_t* t);
+int mf_gen_queries_hub(const mf_edges* e, double top_frac, int q, uint64_t seed, int32_t* s,
+                       int32_t* t);
+int mf_write_queries(const char* path, int q, const int32_t* s, const int32_t* t);
+
+/* ---------------------------------------------------------- device graph */
+
+int mf_device_count(int* count);
+
+/* Upload a (partitioned) host graph; regions are cut into tiles of at most
+ * tile_max vertices (<= 4096), one workgroup per tile in region mode.
+ * Replaces init_global_graph_on_device (BC:1293-1332); residency is keyed by
+ * the handle, never by (V,E,R) sizes. */
+int mf_graph_upload(const mf_hgraph* h, int tile_max, mf_graph** out);
+int mf_graph_info(const mf_graph* g, int* n, int64_t* E, int* ntiles, int64_t* bytes);
+void mf_graph_free(mf_graph* g);
+
+/* ---------------------------------------------------------------- engine */
+
+/* Scheduler parameters; mf_params_default() gives the reference's constants
+ * (BC:1459-1484, BC:1725, BC:1936, BC:2644-2648). */
+typedef struct mf_params {
+    int K;               /* discharge cycles per vertex per pass, 0 = max(4, E/V) */
+    int gr_freq;         /* GR every gr_freq iterations */
+    int gr_freq_min, gr_freq_max;
+    int gr_stall;        /* extra GR after excess(t) stalls this many iterations, 0 = off */
+    int h_cut;
+    int stagnate_rounds;
+    int path_switch, path_len, path_attempts, p_count0;
+    int hot, hot_k, hot_min_active, hot_rounds;
+    int rm, rm_freq0, rm_freq_max, rm_margin;
+    double rm_tau;
+    int tile_rounds;
+    double tile_frac;    /* region mode when active >= tile_frac * n */
+    int cert;            /* certify every answer on the GPU */
+    long long iter_max;
+    double bu_alpha, bu_beta;
+    int prune;           /* per-query reachability pruning + induced subgraph */
+    double prune_ratio;  /* keep the subgraph when kept < ratio * n */
+} mf_params;
+
+void mf_params_default(mf_params* p);
+
+/* Engine with nslots concurrent query slots; per-query residual slabs are
+ * sized for the largest graph added.  Replaces DynamicState /
+ * init_state_for_st (BC:47-113, BC:1336-1372). */
+int mf_engine_create(int nslots, const mf_params* p, mf_engine** out);
+int mf_engine_add_graph(mf_engine* e, mf_graph* g, int* gid);
+void mf_engine_free(mf_engine* e);
+
+/* Result of one query. */
+typedef struct mf_result {
+    int64_t tag;
+    int64_t flow;
+    int status;
+    int cert_ok;
+    int pruned;
+    int n_run;
+    int64_t E_run;
+    int64_t kept;
+    int64_t iters, steps, grs, bfs_levels, drains, rm_fixed;
+    uint64_t edge_scans, vertex_scans, pushes, relabels;
+    double t_arrive, t_start, t_end; /* seconds on the engine clock */
+    /* work per kernel family (init, bfs, active, discharge, rm, cert) x
+     * (edge scans, vertex scans, pushes, relabels + BFS discoveries) */
+    uint64_t work[6][4];
+} mf_result;
+
+/* Plain single query: solve (s,t) (ids of graph gid) to completion.
+ * Replaces run_gpu_static_with_state (BC:2119-2205) including per-query
+ * pruning (BC:2666-2698): the int64 flow is exact and certified. */
+int mf_solve(mf_engine* e, int gid, int s, int t, mf_result* r);
+
+/* Batched / served use: queue queries, then drive the engine.  Freed slots
+ * are refilled from the queue every step (continuous batching). */
+int mf_engine_submit(mf_engine* e, int gid, int s, int t, int64_t tag, double t_arrive);
+/* Run steps until the queue and every slot are empty (max_steps < 0) or for
+ * max_steps steps; finished results are copied to out (up to cap), *nout set. */
+int mf_engine_run(mf_engine* e, int64_t max_steps, mf_result* out, int cap, int* nout);
+int mf_engine_live(const mf_engine* e, int* live, int64_t* queued);
+double mf_engine_now(const mf_engine* e);
+int mf_engine_cache_stats(const mf_engine* e, int64_t* hits, int64_t* misses);
+/* Per kernel family (init, bfs, active, discharge, rm, cert) timing with HIP
+ * events on the engine's stream while on != 0, plus the algorithmic work of
+ * the queries that finished meanwhile.  Turning it on clears the totals.
+ * ms[6], launches[6], work[6*4]. */
+int mf_engine_profile(mf_engine* e, int on);
+int mf_engine_profile_read(const mf_engine* e, double* ms, int64_t* launches, uint64_t* work);
+
+/* ------------------------------------------------------- dynamic mode
+ * Pinned slots keep their residual across capacity updates (the reference's
+ * DynamicState reuse, BC:2230-2335).  Pinned queries run on the full graph. */
+
+/* Pin (s,t) of graph gid to slot `slot`; it is not refilled by the queue. */
+int mf_engine_pin(mf_engine* e, int slot, int gid, int s, int t, int64_t tag);
+/* Solve every pinned slot, concurrently.  warm = 0: from scratch (init +
+ * preflow); warm = 1: from the residual left by the last solve and updates
+ * (preflow again, exact GR, drain of any deficits).  A warm answer that
+ * fails its certificate is re-solved cold (counted in mf_result.drains < 0). */
+int mf_engine_solve_pinned(mf_engine* e, int warm, mf_result* out, int cap, int* nout);
+/* Capacity updates on graph gid.  entries are CSR entry ids of forward
+ * edges (cap > 0), new_cap >= 1, no repeats.  Every pinned residual on gid
+ * is adjusted: an increase adds residual; a decrease below the edge's flow
+ * sends the surplus back (excess at the tail, deficit at the head).
+ * Replaces OnAdd_kernel (BC:959-1000), which handled increases only. */
+int mf_graph_update_caps(mf_engine* e, int gid, int n, const int32_t* entries,
+                         const int64_t* new_cap);
+/* First forward entry u->v (region-ordered ids) of a host graph, or -1. */
+int mf_hgraph_find_entry(const mf_hgraph* h, int u, int v, int32_t* entry);
+
+/* ------------------------------------------------- parity hooks (tests) */
+
+/* Residual init + source preflow on slot 0 (BC:151-200); copies res[E] and
+ * excess[n] out. */
+int mf_dbg_init_preflow(mf_engine* e, int gid, int s, int t, int64_t* res, int64_t* excess);
+/* Exact global relabel on given residual/excess (BC:203-233, BC:322-348);
+ * copies heights out (n = unreached). */
+int mf_dbg_global_relabel(mf_engine* e, int gid, int s, int t, const int64_t* res,
+                          const int64_t* excess, int32_t* height);
+/* Pruning mask on the GPU (BC:1391-1445); mask[n] 0/1, *kept. */
+int mf_dbg_prune_mask(mf_engine* e, int gid, int s, int t, int32_t* mask, int* kept);
+/* Induced subgraph on the GPU for a given mask (BC:2477-2551). */
+int mf_dbg_induced(mf_engine* e, int gid, const int32_t* mask, int* n_out, int64_t* E_out,
+                   int32_t* row_ptr, int32_t* col, int64_t* cap, int32_t* rev,
+                   int32_t* old2sub);
+/* Solve without pruning on slot 0 and copy the final residual, excess and
+ * heights out (for an independent CPU certificate). */
+int mf_dbg_solve_state(mf_engine* e, int gid, int s, int t, mf_result* r, int64_t* res,
+                       int64_t* excess, int32_t* height);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,452 @@
+"""MI355X-native exact integer max flow (push-relabel) behind a C ABI.
+
+Python bindings over ``libmscmf.so`` (include/mscmf.h).  The library is the
+product: graph preparation runs in its C++ host code, every solve runs in the
+HIP kernels on the GPU.  There is no CPU fallback; without the built library
+or without a GPU the calls raise.
+
+Reference interface mirrored: xiyuweiguang/MScResearch,
+Region_Aware_PR_with_Pruning_BoundaryCore.cu (build_graph BC:1044,
+partition BC:1130, partition_and_reorder BC:1182, init_global_graph_on_device
+BC:1293, run_gpu_static_with_state BC:2119, incremental update BC:2273).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libmscmf.so")
+CLI_PATH = os.path.join(_PKG, "bin", "mscmf")
+
+MF_OK, MF_EINVAL, MF_ENOMEM, MF_EHIP, MF_EIO, MF_EITER, MF_ECERT, MF_ENODEV, MF_EBUSY = 0, -1, -2, -3, -4, -5, -6, -7, -8
+
+_lib = None
+
+
+class MaxFlowError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("K", C.c_int), ("gr_freq", C.c_int), ("gr_freq_min", C.c_int), ("gr_freq_max", C.c_int),
+        ("gr_stall", C.c_int), ("h_cut", C.c_int), ("stagnate_rounds", C.c_int),
+        ("path_switch", C.c_int), ("path_len", C.c_int), ("path_attempts", C.c_int), ("p_count0", C.c_int),
+        ("hot", C.c_int), ("hot_k", C.c_int), ("hot_min_active", C.c_int), ("hot_rounds", C.c_int),
+        ("rm", C.c_int), ("rm_freq0", C.c_int), ("rm_freq_max", C.c_int), ("rm_margin", C.c_int),
+        ("rm_tau", C.c_double), ("tile_rounds", C.c_int), ("tile_frac", C.c_double), ("cert", C.c_int),
+        ("iter_max", C.c_longlong), ("bu_alpha", C.c_double), ("bu_beta", C.c_double),
+        ("prune", C.c_int), ("prune_ratio", C.c_double),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("tag", C.c_int64), ("flow", C.c_int64), ("status", C.c_int), ("cert_ok", C.c_int),
+        ("pruned", C.c_int), ("n_run", C.c_int), ("E_run", C.c_int64), ("kept", C.c_int64),
+        ("iters", C.c_int64), ("steps", C.c_int64), ("grs", C.c_int64), ("bfs_levels", C.c_int64),
+        ("drains", C.c_int64), ("rm_fixed", C.c_int64),
+        ("edge_scans", C.c_uint64), ("vertex_scans", C.c_uint64), ("pushes", C.c_uint64),
+        ("relabels", C.c_uint64),
+        ("t_arrive", C.c_double), ("t_start", C.c_double), ("t_end", C.c_double),
+        ("work", (C.c_uint64 * 4) * 6),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "work"}
+        d["work"] = [[int(self.work[f][k]) for k in range(4)] for f in range(6)]
+        return d
+
+
+_P = C.c_void_p
+_I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_I64P = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+# name -> (restype, argtypes); also the list of symbols include/mscmf.h declares
+SIGNATURES = {
+    "mf_last_error": (C.c_char_p, []),
+    "mf_version": (C.c_int, []),
+    "mf_hgraph_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "mf_hgraph_from_edges": (C.c_int, [C.c_int, C.c_int64, _P, _P, _P, C.POINTER(_P)]),
+    "mf_hgraph_from_edgelist": (C.c_int, [_P, C.POINTER(_P)]),
+    "mf_hgraph_partition": (C.c_int, [_P, C.c_int]),
+    "mf_hgraph_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
+    "mf_hgraph_export": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "mf_hgraph_stats": (C.c_int, [_P, _P, _P, _P, _P]),
+    "mf_hgraph_free": (None, [_P]),
+    "mf_hgraph_find_entry": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_int32)]),
+    "mf_partition": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _P]),
+    "mf_read_queries": (C.c_int, [C.c_char_p, C.c_int64, _P, _P, C.POINTER(C.c_int64)]),
+    "mf_gen_uniform": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_uint64, C.POINTER(_P)]),
+    "mf_gen_grid": (C.c_int, [C.c_int, C.c_int, C.c_int64, C.c_uint64, C.POINTER(_P)]),
+    "mf_gen_rmat": (C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_uint64, C.c_int, C.POINTER(_P)]),
+    "mf_edges_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
+    "mf_edges_copy": (C.c_int, [_P, _P, _P, _P]),
+    "mf_edges_write": (C.c_int, [_P, C.c_char_p]),
+    "mf_edges_free": (None, [_P]),
+    "mf_gen_queries_uniform": (C.c_int, [C.c_int, C.c_int, C.c_uint64, _P, _P]),
+    "mf_gen_queries_grid": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, _P, _P]),
+    "mf_gen_queries_hub": (C.c_int, [_P, C.c_double, C.c_int, C.c_uint64, _P, _P]),
+    "mf_write_queries": (C.c_int, [C.c_char_p, C.c_int, _P, _P]),
+    "mf_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "mf_graph_upload": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
+    "mf_graph_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
+    "mf_graph_free": (None, [_P]),
+    "mf_params_default": (None, [C.POINTER(Params)]),
+    "mf_engine_create": (C.c_int, [C.c_int, C.POINTER(Params), C.POINTER(_P)]),
+    "mf_engine_add_graph": (C.c_int, [_P, _P, C.POINTER(C.c_int)]),
+    "mf_engine_free": (None, [_P]),
+    "mf_solve": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(Result)]),
+    "mf_engine_submit": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_double]),
+    "mf_engine_run": (C.c_int, [_P, C.c_int64, C.POINTER(Result), C.c_int, C.POINTER(C.c_int)]),
+    "mf_engine_live": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
+    "mf_engine_now": (C.c_double, [_P]),
+    "mf_engine_cache_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "mf_engine_profile": (C.c_int, [_P, C.c_int]),
+    "mf_engine_profile_read": (C.c_int, [_P, _P, _P, _P]),
+    "mf_engine_pin": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64]),
+    "mf_engine_solve_pinned": (C.c_int, [_P, C.c_int, C.POINTER(Result), C.c_int, C.POINTER(C.c_int)]),
+    "mf_graph_update_caps": (C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
+    "mf_dbg_init_preflow": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P]),
+    "mf_dbg_global_relabel": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
+    "mf_dbg_prune_mask": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.POINTER(C.c_int)]),
+    "mf_dbg_induced": (C.c_int, [_P, C.c_int, _P, C.POINTER(C.c_int), C.POINTER(C.c_int64), _P, _P, _P, _P, _P]),
+    "mf_dbg_solve_state": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(Result), _P, _P, _P]),
+}
+
+
+def lib() -> C.CDLL:
+    """Load libmscmf.so (built in-tree).  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MaxFlowError(MF_EIO, f"{LIB_PATH} not built; run python -m mscresearch_amd.build")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc: int) -> None:
+    if rc != MF_OK:
+        raise MaxFlowError(rc, lib().mf_last_error().decode())
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    _check(lib().mf_device_count(C.byref(c)))
+    return c.value
+
+
+def default_params(**over) -> Params:
+    p = Params()
+    lib().mf_params_default(C.byref(p))
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
+class Edges:
+    """A generated edge list (0-based ids, emission order)."""
+
+    def __init__(self, h):
+        self._h = h
+
+    @staticmethod
+    def uniform(n, m, cap_max, seed):
+        h = _P()
+        _check(lib().mf_gen_uniform(n, m, cap_max, seed, C.byref(h)))
+        return Edges(h)
+
+    @staticmethod
+    def grid(rows, cols, cap_max, seed):
+        h = _P()
+        _check(lib().mf_gen_grid(rows, cols, cap_max, seed, C.byref(h)))
+        return Edges(h)
+
+    @staticmethod
+    def rmat(scale, cap_max, seed, a=0.57, b=0.19, c=0.19, samples=None, threads=8):
+        h = _P()
+        samples = 16 * (1 << scale) if samples is None else samples
+        _check(lib().mf_gen_rmat(scale, samples, a, b, c, cap_max, seed, threads, C.byref(h)))
+        return Edges(h)
+
+    def info(self):
+        n, cnt = C.c_int(0), C.c_int64(0)
+        _check(lib().mf_edges_info(self._h, C.byref(n), C.byref(cnt)))
+        return n.value, cnt.value
+
+    def arrays(self):
+        n, cnt = self.info()
+        us = np.empty(cnt, np.int32)
+        vs = np.empty(cnt, np.int32)
+        cs = np.empty(cnt, np.int64)
+        _check(lib().mf_edges_copy(self._h, _ptr(us), _ptr(vs), _ptr(cs)))
+        return n, us, vs, cs
+
+    def write(self, path):
+        _check(lib().mf_edges_write(self._h, str(path).encode()))
+
+    def hub_queries(self, q, seed, top_frac=0.01):
+        s = np.empty(q, np.int32)
+        t = np.empty(q, np.int32)
+        _check(lib().mf_gen_queries_hub(self._h, top_frac, q, seed, _ptr(s), _ptr(t)))
+        return s, t
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mf_edges_free(self._h)
+            self._h = None
+
+
+def uniform_queries(n, q, seed):
+    s = np.empty(q, np.int32)
+    t = np.empty(q, np.int32)
+    _check(lib().mf_gen_queries_uniform(n, q, seed, _ptr(s), _ptr(t)))
+    return s, t
+
+
+def grid_queries(rows, cols, q, seed):
+    s = np.empty(q, np.int32)
+    t = np.empty(q, np.int32)
+    _check(lib().mf_gen_queries_grid(rows, cols, q, seed, _ptr(s), _ptr(t)))
+    return s, t
+
+
+def write_queries(path, s, t):
+    s = np.ascontiguousarray(s, np.int32)
+    t = np.ascontiguousarray(t, np.int32)
+    _check(lib().mf_write_queries(str(path).encode(), len(s), _ptr(s), _ptr(t)))
+
+
+def read_queries(path):
+    cnt = C.c_int64(0)
+    _check(lib().mf_read_queries(str(path).encode(), 0, None, None, C.byref(cnt)))
+    s = np.empty(cnt.value, np.int32)
+    t = np.empty(cnt.value, np.int32)
+    _check(lib().mf_read_queries(str(path).encode(), cnt.value, _ptr(s), _ptr(t), C.byref(cnt)))
+    return s, t
+
+
+def partition(R, so, si, dout, din):
+    n = len(so)
+    part = np.empty(n, np.int32)
+    so, si = np.ascontiguousarray(so, np.int64), np.ascontiguousarray(si, np.int64)
+    dout, din = np.ascontiguousarray(dout, np.int32), np.ascontiguousarray(din, np.int32)
+    _check(lib().mf_partition(n, R, _ptr(so), _ptr(si), _ptr(dout), _ptr(din), _ptr(part)))
+    return part
+
+
+class HostGraph:
+    """Host CSR with reverse entries (+ region partition once partitioned)."""
+
+    def __init__(self, h):
+        self._h = h
+
+    @staticmethod
+    def load(path):
+        h = _P()
+        _check(lib().mf_hgraph_load(str(path).encode(), C.byref(h)))
+        return HostGraph(h)
+
+    @staticmethod
+    def from_edges(n, us, vs, caps):
+        us = np.ascontiguousarray(us, np.int32)
+        vs = np.ascontiguousarray(vs, np.int32)
+        caps = np.ascontiguousarray(caps, np.int64)
+        h = _P()
+        _check(lib().mf_hgraph_from_edges(n, len(us), _ptr(us), _ptr(vs), _ptr(caps), C.byref(h)))
+        return HostGraph(h)
+
+    @staticmethod
+    def from_edgelist(e: Edges):
+        h = _P()
+        _check(lib().mf_hgraph_from_edgelist(e._h, C.byref(h)))
+        return HostGraph(h)
+
+    def partition(self, R):
+        _check(lib().mf_hgraph_partition(self._h, R))
+        return self
+
+    def info(self):
+        n, E, R, lines = C.c_int(0), C.c_int64(0), C.c_int(0), C.c_int64(0)
+        _check(lib().mf_hgraph_info(self._h, C.byref(n), C.byref(E), C.byref(R), C.byref(lines)))
+        return n.value, E.value, R.value, lines.value
+
+    def export(self):
+        n, E, R, _ = self.info()
+        d = {
+            "row_ptr": np.empty(n + 1, np.int32), "col": np.empty(E, np.int32),
+            "cap": np.empty(E, np.int64), "rev": np.empty(E, np.int32),
+        }
+        if R > 0:
+            d.update(new_id=np.empty(n, np.int32), old_id=np.empty(n, np.int32),
+                     rstart=np.empty(R + 1, np.int32), region_id=np.empty(n, np.int32))
+        _check(lib().mf_hgraph_export(self._h, _ptr(d["row_ptr"]), _ptr(d["col"]), _ptr(d["cap"]),
+                                      _ptr(d["rev"]), _ptr(d.get("new_id")), _ptr(d.get("old_id")),
+                                      _ptr(d.get("rstart")), _ptr(d.get("region_id"))))
+        return d
+
+    def stats(self):
+        n = self.info()[0]
+        so, si = np.empty(n, np.int64), np.empty(n, np.int64)
+        do, di = np.empty(n, np.int32), np.empty(n, np.int32)
+        _check(lib().mf_hgraph_stats(self._h, _ptr(so), _ptr(si), _ptr(do), _ptr(di)))
+        return so, si, do, di
+
+    def find_entry(self, u, v):
+        e = C.c_int32(-1)
+        _check(lib().mf_hgraph_find_entry(self._h, u, v, C.byref(e)))
+        return e.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mf_hgraph_free(self._h)
+            self._h = None
+
+
+class DeviceGraph:
+    """A region-ordered graph resident in HBM."""
+
+    def __init__(self, hg: HostGraph, tile_max: int = 4096):
+        self._h = _P()
+        _check(lib().mf_graph_upload(hg._h, tile_max, C.byref(self._h)))
+
+    def info(self):
+        n, E, T, b = C.c_int(0), C.c_int64(0), C.c_int(0), C.c_int64(0)
+        _check(lib().mf_graph_info(self._h, C.byref(n), C.byref(E), C.byref(T), C.byref(b)))
+        return {"n": n.value, "E": E.value, "ntiles": T.value, "bytes": b.value}
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mf_graph_free(self._h)
+            self._h = None
+
+
+class Engine:
+    """Query slots + scheduler + HIP stream (plain solve = one slot)."""
+
+    def __init__(self, nslots: int = 1, params: Params | None = None):
+        self._h = _P()
+        self._graphs = []
+        p = params if params is not None else default_params()
+        _check(lib().mf_engine_create(nslots, C.byref(p), C.byref(self._h)))
+        self.nslots = nslots
+
+    def add_graph(self, g: DeviceGraph) -> int:
+        gid = C.c_int(0)
+        _check(lib().mf_engine_add_graph(self._h, g._h, C.byref(gid)))
+        self._graphs.append(g)
+        return gid.value
+
+    def solve(self, gid, s, t) -> Result:
+        r = Result()
+        rc = lib().mf_solve(self._h, gid, int(s), int(t), C.byref(r))
+        _check(rc)
+        return r
+
+    def submit(self, gid, s, t, tag, t_arrive=0.0):
+        _check(lib().mf_engine_submit(self._h, gid, int(s), int(t), int(tag), float(t_arrive)))
+
+    def run(self, max_steps=-1, cap=1 << 16):
+        out = (Result * cap)()
+        k = C.c_int(0)
+        _check(lib().mf_engine_run(self._h, max_steps, out, cap, C.byref(k)))
+        return [out[i] for i in range(k.value)]
+
+    def live(self):
+        a, b = C.c_int(0), C.c_int64(0)
+        _check(lib().mf_engine_live(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def now(self):
+        return lib().mf_engine_now(self._h)
+
+    def cache_stats(self):
+        a, b = C.c_int64(0), C.c_int64(0)
+        _check(lib().mf_engine_cache_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def profile(self, on: bool):
+        _check(lib().mf_engine_profile(self._h, 1 if on else 0))
+
+    def profile_read(self):
+        ms = np.zeros(6, np.float64)
+        la = np.zeros(6, np.int64)
+        wk = np.zeros(24, np.uint64)
+        _check(lib().mf_engine_profile_read(self._h, _ptr(ms), _ptr(la), _ptr(wk)))
+        return ms, la, wk.reshape(6, 4)
+
+    def pin(self, slot, gid, s, t, tag):
+        _check(lib().mf_engine_pin(self._h, slot, gid, int(s), int(t), int(tag)))
+
+    def solve_pinned(self, warm: bool):
+        out = (Result * self.nslots)()
+        k = C.c_int(0)
+        _check(lib().mf_engine_solve_pinned(self._h, 1 if warm else 0, out, self.nslots, C.byref(k)))
+        return [out[i] for i in range(k.value)]
+
+    def update_caps(self, gid, entries, new_caps):
+        entries = np.ascontiguousarray(entries, np.int32)
+        new_caps = np.ascontiguousarray(new_caps, np.int64)
+        _check(lib().mf_graph_update_caps(self._h, gid, len(entries), _ptr(entries), _ptr(new_caps)))
+
+    # parity hooks ---------------------------------------------------------
+    def dbg_init_preflow(self, gid, s, t, n, E):
+        res = np.empty(E, np.int64)
+        ex = np.empty(n, np.int64)
+        _check(lib().mf_dbg_init_preflow(self._h, gid, int(s), int(t), _ptr(res), _ptr(ex)))
+        return res, ex
+
+    def dbg_global_relabel(self, gid, s, t, res, excess):
+        res = np.ascontiguousarray(res, np.int64)
+        excess = np.ascontiguousarray(excess, np.int64)
+        h = np.empty(len(excess), np.int32)
+        _check(lib().mf_dbg_global_relabel(self._h, gid, int(s), int(t), _ptr(res), _ptr(excess), _ptr(h)))
+        return h
+
+    def dbg_prune_mask(self, gid, s, t, n):
+        mask = np.empty(n, np.int32)
+        k = C.c_int(0)
+        _check(lib().mf_dbg_prune_mask(self._h, gid, int(s), int(t), _ptr(mask), C.byref(k)))
+        return mask, k.value
+
+    def dbg_induced(self, gid, mask, E_parent):
+        mask = np.ascontiguousarray(mask, np.int32)
+        n = len(mask)
+        nk, Ek = C.c_int(0), C.c_int64(0)
+        rp = np.empty(n + 1, np.int32)
+        col = np.empty(max(E_parent, 1), np.int32)
+        cap = np.empty(max(E_parent, 1), np.int64)
+        rev = np.empty(max(E_parent, 1), np.int32)
+        o2s = np.empty(n, np.int32)
+        _check(lib().mf_dbg_induced(self._h, gid, _ptr(mask), C.byref(nk), C.byref(Ek), _ptr(rp), _ptr(col),
+                                    _ptr(cap), _ptr(rev), _ptr(o2s)))
+        k, m = nk.value, Ek.value
+        return {"n": k, "E": m, "row_ptr": rp[: k + 1].copy(), "col": col[:m].copy(), "cap": cap[:m].copy(),
+                "rev": rev[:m].copy(), "old2sub": o2s}
+
+    def dbg_solve_state(self, gid, s, t, n, E):
+        r = Result()
+        res = np.empty(E, np.int64)
+        ex = np.empty(n, np.int64)
+        h = np.empty(n, np.int32)
+        _check(lib().mf_dbg_solve_state(self._h, gid, int(s), int(t), C.byref(r), _ptr(res), _ptr(ex), _ptr(h)))
+        return r, res, ex, h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mf_engine_free(self._h)
+            self._h = None

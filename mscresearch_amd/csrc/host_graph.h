@@ -1,0 +1,92 @@
+// host_graph.h -- host-side graph preparation for the MI355X max-flow path.
+//
+// Parsing, CSR-with-reverse-entries construction, the capacity-degree region
+// partition and the region-ordered relabelling.  Everything here is exact
+// integer work except the partition score (double, used only to order
+// vertices).  Reference: Region_Aware_PR_with_Pruning_BoundaryCore.cu
+// (BC:1044-1276).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace mf {
+
+struct HostGraph {
+    int n = 0;                 // vertices
+    int64_t lines = 0;         // accepted input lines
+    std::vector<int32_t> row_ptr;  // n+1
+    std::vector<int32_t> col;      // E
+    std::vector<int64_t> cap;      // E
+    std::vector<int32_t> rev;      // E
+    // per-vertex stats over original forward edges (BC:1090-1096)
+    std::vector<int64_t> sum_out, sum_in;
+    std::vector<int32_t> deg_out, deg_in;
+    // partition state; empty until partition_and_reorder() runs
+    int R = 0;
+    std::vector<int32_t> new_id, old_id;   // original <-> region-ordered ids
+    std::vector<int32_t> rstart;           // R+1 region boundaries (region-ordered ids)
+    std::vector<int32_t> region_id;        // n
+    int64_t E() const { return (int64_t)col.size(); }
+};
+
+// Errors are reported as negative codes plus a message in err.
+// graph.txt reader (BC:1044-1076): header "n m", then up to m lines
+// "u v cap"; stops at the first malformed line; skips out-of-range ids,
+// self-loops and cap<=0.
+int parse_graph_file(const std::string& path, std::vector<int32_t>& us, std::vector<int32_t>& vs,
+                     std::vector<int64_t>& caps, int& n, std::string& err);
+
+// test.txt reader (BC:2603-2621): 1-based pairs until a read fails; returns
+// 0-based ids (unmapped, may be out of range).
+int parse_query_file(const std::string& path, std::vector<int32_t>& s, std::vector<int32_t>& t,
+                     std::string& err);
+
+// CSR with one forward (cap c) and one reverse (cap 0) entry per edge, in
+// edge order (BC:1078-1125).
+void build_csr(int n, const std::vector<int32_t>& us, const std::vector<int32_t>& vs,
+               const std::vector<int64_t>& caps, HostGraph& g);
+
+// Stats recomputed from CSR entries with cap>0 (BC:2707-2720).
+void csr_stats(const HostGraph& g, std::vector<int64_t>& so, std::vector<int64_t>& si,
+               std::vector<int32_t>& dout, std::vector<int32_t>& din);
+
+// Capacity-degree partition (BC:1130-1178); ties in score broken by id.
+void capacity_degree_partition(int n, int R, const std::vector<int64_t>& so,
+                               const std::vector<int64_t>& si, const std::vector<int32_t>& dout,
+                               const std::vector<int32_t>& din, std::vector<int32_t>& part);
+
+// Region-contiguous relabel (BC:1182-1276).  Rewrites g in place to the
+// region-ordered CSR and fills new_id/old_id/rstart/region_id.
+void partition_and_reorder(HostGraph& g, int R, const std::vector<int32_t>& part);
+
+// ------------------------------------------------------------ generators
+// Seeded, platform-independent generators (splitmix64 -> xoshiro256**).
+// Each writes an edge list with 0-based ids in emission order.
+void gen_uniform(int n, int64_t m, int64_t cap_max, uint64_t seed, std::vector<int32_t>& us,
+                 std::vector<int32_t>& vs, std::vector<int64_t>& caps);
+void gen_grid(int rows, int cols, int64_t cap_max, uint64_t seed, std::vector<int32_t>& us,
+              std::vector<int32_t>& vs, std::vector<int64_t>& caps);
+// R-MAT (a,b,c,d) with 16*2^scale samples, self-loops dropped, duplicates
+// merged by summing log-uniform [1,cap_max] capacities; output sorted by (u,v).
+void gen_rmat(int scale, int64_t samples, double a, double b, double c, int64_t cap_max,
+              uint64_t seed, int threads, std::vector<int32_t>& us, std::vector<int32_t>& vs,
+              std::vector<int64_t>& caps);
+
+// Query generators (0-based ids).
+void gen_queries_uniform(int n, int q, uint64_t seed, std::vector<int32_t>& s,
+                         std::vector<int32_t>& t);
+void gen_queries_grid(int rows, int cols, int q, uint64_t seed, std::vector<int32_t>& s,
+                      std::vector<int32_t>& t);
+// s from the top `top_frac` of vertices by out-degree (ties by id), t uniform
+// over vertices with in-degree > 0, s != t.
+void gen_queries_hub(int n, const std::vector<int32_t>& us, const std::vector<int32_t>& vs,
+                     double top_frac, int q, uint64_t seed, std::vector<int32_t>& s,
+                     std::vector<int32_t>& t);
+
+int write_graph_file(const std::string& path, int n, const std::vector<int32_t>& us,
+                     const std::vector<int32_t>& vs, const std::vector<int64_t>& caps);
+int write_query_file(const std::string& path, const std::vector<int32_t>& s,
+                     const std::vector<int32_t>& t);
+
+}  // namespace mf

@@ -1,0 +1,82 @@
+// mf_device.h -- device-visible structs shared by the HIP kernels and the
+// host scheduler.  Plain C++ layout (no HIP types) so host code compiled by
+// any compiler can fill them.
+#pragma once
+#include <cstdint>
+
+namespace mf {
+
+// Resident topology of one region-ordered graph (device pointers).
+// row_ptr[n+1], col/rev[E] (int32), cap[E] (int64): the reference's CSR with
+// explicit reverse entries (BC:20-27).  Vertices are region-ordered; the
+// regions are further cut into tiles of at most kTileMax vertices (a tile
+// never spans two regions), tile_start[ntiles+1], tile_of[n].
+struct DGraph {
+    int n = 0;
+    int E = 0;
+    int ntiles = 0;
+    int R = 0;
+    const int32_t* row_ptr = nullptr;
+    const int32_t* col = nullptr;
+    const int64_t* cap = nullptr;
+    const int32_t* rev = nullptr;
+    const int32_t* tile_start = nullptr;
+    const int32_t* tile_of = nullptr;
+};
+
+// Per-slot device counters.  Written by kernels, copied to the host once per
+// scheduler step.  Field order is part of the host/device contract.
+struct Ctr {
+    int32_t fsize[4];        // frontier size of level L lives in fsize[L % 4]
+    int64_t fedges[4];       // row-length sum of the level-L frontier, same ring
+    int32_t level;           // levels completed by the running BFS
+    int32_t visited;         // vertices labelled by the running BFS
+    int32_t active;          // excess>0, h<n, not s/t
+    int32_t active_high;     // ... with h > H_cut
+    int32_t nlight;          // light active-list entries (front of the list)
+    int32_t nheavy;          // heavy entries (back of the list)
+    int64_t pos_excess;      // sum of positive excess off s,t
+    int32_t neg_count;       // deficits (excess<0) off s,t
+    int32_t fixed_edges;     // RM repairs in the last RM pass
+    int64_t neg_sum;         // sum of deficits (negative)
+    int32_t hmin, hmax;      // active height band
+    int32_t cert_bad;        // certificate violations
+    int32_t err;             // device-side error flags
+    int64_t cut;             // certificate: cut(V\T,T)
+    int64_t ex_t;            // excess of t seen by the last active / certificate pass
+    // algorithmic work counters (roofline accounting), per kernel family:
+    // wk[family][0] edge scans, [1] vertex scans, [2] pushes,
+    // [3] relabels + BFS discoveries
+    uint64_t wk[6][4];
+};
+
+// Kernel families of the work counters and of the engine's timing.
+enum WorkFamily : int { WF_INIT = 0, WF_BFS = 1, WF_ACTIVE = 2, WF_DISCHARGE = 3, WF_RM = 4, WF_CERT = 5 };
+constexpr int kFamilies = 6;
+
+// Everything a kernel needs to work on one query.
+struct SlotView {
+    DGraph g;
+    int64_t* res = nullptr;      // E residual
+    int64_t* excess = nullptr;   // n
+    int32_t* h = nullptr;        // n heights (GR writes BFS levels, n = unreached)
+    int32_t* fa = nullptr;       // n frontier / active list A
+    int32_t* fb = nullptr;       // n frontier B
+    int32_t* lock = nullptr;     // n path-push ownership (-1 = free)
+    int32_t* aux = nullptr;      // n scratch (prune masks, compaction)
+    int32_t* tile_cnt = nullptr; // ntiles active counts
+    int32_t* tile_hot = nullptr; // ntiles hot flags
+    const int32_t* mask = nullptr;  // optional vertex mask (nullptr = all)
+    Ctr* ctr = nullptr;
+    int32_t s = 0, t = 0;
+    int32_t K = 4;               // discharge cycles per vertex per pass
+    int32_t H_cut = 100;         // "high" label threshold (BC:1469)
+    int32_t phase = 0;           // 0 = normal (seeds t + deficits), 1 = drain (seeds deficits, t active)
+};
+
+constexpr int kTileMax = 4096;      // vertices staged in LDS per workgroup
+constexpr int kLightDeg = 32;       // rows up to this length are scanned by one lane
+constexpr int kSmallFrontier = 16384;  // single-workgroup BFS keeps going below this
+constexpr int64_t kSmallEdges = 1 << 18; // ... and while the frontier's rows stay below this
+
+}  // namespace mf

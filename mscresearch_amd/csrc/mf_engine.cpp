@@ -1,0 +1,949 @@
+// mf_engine.cpp -- resident graphs, slabs, scheduler and step executor.
+#include "mf_engine.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstring>
+#include <mutex>
+
+#include "mf_kernels.h"
+
+namespace mf {
+
+// ------------------------------------------------------------------ errors
+namespace {
+thread_local std::string g_err;
+}
+void set_error(const std::string& msg) { g_err = msg; }
+const std::string& last_error() { return g_err; }
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                \
+            return e_ == hipErrorOutOfMemory ? MF_ENOMEM : MF_EHIP;                      \
+        }                                                                                \
+    } while (0)
+
+double wall_seconds() {
+    using namespace std::chrono;
+    return duration_cast<duration<double>>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ----------------------------------------------------------- device graph
+DeviceGraph::~DeviceGraph() {
+    (void)hipFree(d_row_ptr); (void)hipFree(d_col); (void)hipFree(d_cap); (void)hipFree(d_rev);
+    (void)hipFree(d_tile_start); (void)hipFree(d_tile_of);
+    (void)hipFree(d_sub2old); (void)hipFree(d_old2sub); (void)hipFree(d_mask);
+}
+
+void make_tiles(const std::vector<int32_t>& rstart, int tile_max, std::vector<int32_t>& ts) {
+    ts.clear();
+    const int R = (int)rstart.size() - 1;
+    for (int r = 0; r < R; ++r)
+        for (int a = rstart[r]; a < rstart[r + 1]; a += tile_max) ts.push_back(a);
+    ts.push_back(R >= 0 ? rstart[R] : 0);
+}
+
+int upload_graph(const HostGraph& hg, int tile_max, DeviceGraph& dg) {
+    if (tile_max < 1 || tile_max > kTileMax) { set_error("tile_max out of range"); return MF_EINVAL; }
+    const int n = hg.n;
+    const int64_t E = hg.E();
+    if (E >= INT32_MAX) { set_error("graph has 2^31 or more CSR entries"); return MF_EINVAL; }
+    std::vector<int32_t> rstart = hg.rstart;
+    if (rstart.empty()) rstart = {0, n};  // unpartitioned: one region
+    std::vector<int32_t> ts;
+    make_tiles(rstart, tile_max, ts);
+    const int ntiles = (int)ts.size() - 1;
+    std::vector<int32_t> tile_of(n);
+    for (int k = 0; k < ntiles; ++k)
+        for (int v = ts[k]; v < ts[k + 1]; ++v) tile_of[v] = k;
+    HIPCHK(hipMalloc(&dg.d_row_ptr, sizeof(int32_t) * ((size_t)n + 1)));
+    HIPCHK(hipMalloc(&dg.d_col, sizeof(int32_t) * std::max<int64_t>(E, 1)));
+    HIPCHK(hipMalloc(&dg.d_cap, sizeof(int64_t) * std::max<int64_t>(E, 1)));
+    HIPCHK(hipMalloc(&dg.d_rev, sizeof(int32_t) * std::max<int64_t>(E, 1)));
+    HIPCHK(hipMalloc(&dg.d_tile_start, sizeof(int32_t) * ((size_t)ntiles + 1)));
+    HIPCHK(hipMalloc(&dg.d_tile_of, sizeof(int32_t) * std::max(n, 1)));
+    HIPCHK(hipMemcpy(dg.d_row_ptr, hg.row_ptr.data(), sizeof(int32_t) * ((size_t)n + 1), hipMemcpyHostToDevice));
+    if (E > 0) {
+        HIPCHK(hipMemcpy(dg.d_col, hg.col.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dg.d_cap, hg.cap.data(), sizeof(int64_t) * E, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dg.d_rev, hg.rev.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(dg.d_tile_start, ts.data(), sizeof(int32_t) * ts.size(), hipMemcpyHostToDevice));
+    if (n > 0) HIPCHK(hipMemcpy(dg.d_tile_of, tile_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    dg.g.n = n;
+    dg.g.E = (int)E;
+    dg.g.ntiles = ntiles;
+    dg.g.R = (int)rstart.size() - 1;
+    dg.g.row_ptr = dg.d_row_ptr;
+    dg.g.col = dg.d_col;
+    dg.g.cap = dg.d_cap;
+    dg.g.rev = dg.d_rev;
+    dg.g.tile_start = dg.d_tile_start;
+    dg.g.tile_of = dg.d_tile_of;
+    dg.tile_max = tile_max;
+    dg.bytes = sizeof(int32_t) * ((size_t)n + 1) + (size_t)E * 16 + sizeof(int32_t) * (ts.size() + n);
+    return MF_OK;
+}
+
+// ------------------------------------------------------------------- slabs
+Slab::~Slab() {
+    (void)hipFree(res); (void)hipFree(excess); (void)hipFree(h); (void)hipFree(fa); (void)hipFree(fb); (void)hipFree(lock);
+    (void)hipFree(aux); (void)hipFree(tile_cnt); (void)hipFree(tile_hot); (void)hipFree(ctr);
+}
+
+size_t Slab::bytes_for(int n, int64_t E, int ntiles) {
+    return (size_t)E * 8 + (size_t)n * (8 + 4 * 5) + (size_t)ntiles * 8 + sizeof(Ctr);
+}
+
+int alloc_slab(int n, int64_t E, int ntiles, Slab& sl) {
+    const size_t nn = (size_t)std::max(n, 1), ee = (size_t)std::max<int64_t>(E, 1),
+                 tt = (size_t)std::max(ntiles, 1);
+    HIPCHK(hipMalloc(&sl.res, 8 * ee));
+    HIPCHK(hipMalloc(&sl.excess, 8 * nn));
+    HIPCHK(hipMalloc(&sl.h, 4 * nn));
+    HIPCHK(hipMalloc(&sl.fa, 4 * nn));
+    HIPCHK(hipMalloc(&sl.fb, 4 * nn));
+    HIPCHK(hipMalloc(&sl.lock, 4 * nn));
+    HIPCHK(hipMalloc(&sl.aux, 4 * nn));
+    HIPCHK(hipMalloc(&sl.tile_cnt, 4 * tt));
+    HIPCHK(hipMalloc(&sl.tile_hot, 4 * tt));
+    HIPCHK(hipMalloc(&sl.ctr, sizeof(Ctr)));
+    HIPCHK(hipMemset(sl.ctr, 0, sizeof(Ctr)));
+    sl.n_cap = n;
+    sl.E_cap = (int)E;
+    sl.tiles_cap = ntiles;
+    sl.bytes = Slab::bytes_for(n, E, ntiles);
+    return MF_OK;
+}
+
+// --------------------------------------------------------------- scheduler
+namespace {
+inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+inline int clampi(int64_t x, int lo, int hi) { return (int)std::max<int64_t>(lo, std::min<int64_t>(hi, x)); }
+}  // namespace
+
+void SlotSched::begin(const Params* P, const DGraph& g, int s, int t, bool prune_first, bool warm) {
+    P_ = P;
+    g_ = g;
+    s_ = s;
+    t_ = t;
+    status_ = MF_OK;
+    err_.clear();
+    phase_ = 0;
+    stats = QueryStats();
+    stats.n_run = g.n;
+    stats.E_run = g.E;
+    const int avg = g.n > 0 ? (int)(g.E / g.n) : 0;
+    K_ = P->K > 0 ? P->K : std::max(4, avg);  // BC:1465
+    GR_FREQ_ = P->gr_freq;
+    RM_FREQ_ = P->rm_freq0;
+    iter_ = 0;
+    last_gr_iter_ = 0;
+    stag_ = no_stag_ = 0;
+    last_pos_ = INT64_MAX;
+    force_gr_ = false;
+    path_stable_ = 0;
+    p_count_ = P->p_count0;
+    path_rounds_ = 0;
+    last_ext_ = -1;
+    stall_ = 0;
+    drain_loops_ = 0;
+    rm_off_ = false;
+    just_gr_ = false;
+    work_reset_ = false;
+    st_ = prune_first ? S_PRUNE_FWD : (warm ? S_WARM_START : S_START);
+}
+
+void SlotSched::resume_after_prune(const DGraph& g, int s, int t) {
+    g_ = g;
+    s_ = s;
+    t_ = t;
+    stats.n_run = g.n;
+    stats.E_run = g.E;
+    const int avg = g.n > 0 ? (int)(g.E / g.n) : 0;
+    K_ = P_->K > 0 ? P_->K : std::max(4, avg);
+    st_ = S_START;
+}
+
+void SlotSched::fail(int code, const std::string& msg) {
+    status_ = code;
+    err_ = msg;
+    st_ = S_ERROR;
+    stats.status = code;
+}
+
+Plan SlotSched::start_gr() {
+    Plan p;
+    p.stages = ST_RESET_PRE | ST_GR_SEED | ST_BFS;
+    p.reset_pre = RST_BFS;
+    p.bfs_kind = BK_SMALL;
+    p.bfs_mode = BFS_GR;
+    bfs_mode_ = BFS_GR;
+    bfs_medges_ = 0;
+    bfs_bottom_up_ = false;
+    last_gr_iter_ = iter_;
+    force_gr_ = false;
+    stall_ = 0;
+    stats.grs++;
+    st_ = S_BFS;
+    return p;
+}
+
+// One BFS level decision.  Direction-optimizing (Beamer): go bottom-up when
+// the frontier's rows outweigh the unvisited rows / alpha, back to top-down
+// when the frontier is below n / beta; small frontiers run many levels in
+// one single-workgroup launch.
+Plan SlotSched::bfs_step(const Ctr& c, int mode) {
+    Plan p;
+    const int L = c.level;
+    const int f = c.fsize[L & 3];
+    const int64_t mf = c.fedges[L & 3];
+    p.stages = ST_BFS;
+    p.bfs_mode = mode;
+    const int64_t mu = std::max<int64_t>(0, (int64_t)g_.E - bfs_medges_);
+    if (!bfs_bottom_up_ && (double)mf > (double)mu / P_->bu_alpha && f > kSmallFrontier)
+        bfs_bottom_up_ = true;
+    else if (bfs_bottom_up_ && (double)f < (double)g_.n / P_->bu_beta)
+        bfs_bottom_up_ = false;
+    if (bfs_bottom_up_) {
+        p.bfs_kind = BK_BU;
+        p.bfs_grid = clampi(cdiv(g_.n, 256), 1, 8192);
+    } else if (f <= kSmallFrontier && mf <= kSmallEdges) {
+        p.bfs_kind = BK_SMALL;
+    } else {
+        p.bfs_kind = BK_TD;
+        p.bfs_grid = std::max(1, cdiv(f, 256));
+    }
+    return p;
+}
+
+Plan SlotSched::plan(const Ctr& c) {
+    Plan p;
+    switch (st_) {
+    case S_PRUNE_FWD: {
+        // forward BFS from s over cap > 0 (BC:1402-1416)
+        p.stages = ST_RESET_PRE | ST_BFS_SEED1 | ST_BFS;
+        p.reset_pre = RST_BFS | RST_WORK | RST_ERR;
+        work_reset_ = true;
+        p.seed_mode = BFS_FWD;
+        p.bfs_kind = BK_SMALL;
+        p.bfs_mode = BFS_FWD;
+        bfs_mode_ = BFS_FWD;
+        bfs_medges_ = 0;
+        bfs_bottom_up_ = false;
+        st_ = S_PRUNE_UND;
+        return p;
+    }
+    case S_PRUNE_UND:
+    case S_PRUNE_MASK: {
+        const int L = c.level;
+        if (c.fsize[L & 3] > 0) {
+            bfs_medges_ += c.fedges[L & 3];
+            return bfs_step(c, bfs_mode_);
+        }
+        if (st_ == S_PRUNE_UND) {
+            // undirected BFS from t (BC:1419-1433)
+            p.stages = ST_RESET_PRE | ST_BFS_SEED1 | ST_BFS;
+            p.reset_pre = RST_BFS;
+            p.seed_mode = BFS_UND;
+            p.bfs_kind = BK_SMALL;
+            p.bfs_mode = BFS_UND;
+            bfs_mode_ = BFS_UND;
+            bfs_medges_ = 0;
+            bfs_bottom_up_ = false;
+            st_ = S_PRUNE_MASK;
+            return p;
+        }
+        p.stages = ST_RESET_PRE | ST_MASK;
+        p.reset_pre = RST_BFS;
+        st_ = S_PRUNE_DECIDE;
+        return p;
+    }
+    case S_PRUNE_DECIDE:
+        // the engine must call resume_after_prune first
+        return p;
+    case S_START:
+    case S_WARM_START: {
+        // cold: residual init (BC:151-170) + preflow (BC:173-200);
+        // warm: the slab keeps the residual, preflow saturates whatever new
+        // residual s has (BC:2304-2322).
+        const bool cold = (st_ == S_START);
+        p = start_gr();
+        p.stages |= ST_PREFLOW | (cold ? ST_INIT : 0u);
+        p.reset_pre = RST_BFS | RST_ACTIVE | RST_CERT | RST_RM | RST_ERR | (work_reset_ ? 0u : (unsigned)RST_WORK);
+        work_reset_ = true;
+        stats.grs = 1;
+        return p;
+    }
+    case S_BFS: {
+        const int L = c.level;
+        if (c.fsize[L & 3] > 0) {
+            bfs_medges_ += c.fedges[L & 3];
+            return bfs_step(c, bfs_mode_);
+        }
+        stats.bfs_levels += L;
+        p.stages = ST_RESET_ACT | ST_ACTIVE;
+        p.reset_act = RST_ACTIVE;
+        just_gr_ = true;
+        st_ = S_COUNTED;
+        return p;
+    }
+    case S_COUNTED: {
+        const bool exact = just_gr_;
+        just_gr_ = false;
+        if (c.err) { fail(MF_EHIP, "device error flag"); return p; }
+        const int active = c.active;
+        if (active == 0) {
+            if (!exact) { force_gr_ = false; return start_gr(); }
+            if (phase_ == 0 && c.neg_count > 0) {
+                // Deficits left (RM or a capacity decrease made them) and no
+                // positive excess can reach t or a deficit: drain them from t.
+                if (++drain_loops_ > 64) { fail(MF_EITER, "deficit drain did not converge"); return p; }
+                phase_ = 1;
+                rm_off_ = true;
+                stats.drains++;
+                return start_gr();
+            }
+            if (phase_ == 1) {
+                if (c.neg_count > 0) { fail(MF_EITER, "deficits unreachable from t"); return p; }
+                phase_ = 0;
+                return start_gr();
+            }
+            // Exact termination: no active vertex can reach t.
+            if (P_->cert) {
+                p.stages = ST_RESET_PRE | ST_CERT;
+                p.reset_pre = RST_CERT;
+                st_ = S_CERT;
+                return p;
+            }
+            stats.flow = c.ex_t;
+            st_ = S_DONE;
+            return p;
+        }
+        if (iter_ > P_->iter_max) { fail(MF_EITER, "iteration watchdog fired"); return p; }
+        // stagnation monitor on the positive excess sum (BC:1851-1891)
+        bool hotspot_triggered = false;
+        if (!exact && iter_ > 0) {
+            if (last_pos_ != INT64_MAX && c.pos_excess >= last_pos_) ++stag_;
+            else stag_ = 0;
+            last_pos_ = c.pos_excess;
+            if (stag_ >= P_->stagnate_rounds) {
+                hotspot_triggered = true;
+                force_gr_ = true;
+                stag_ = 0;
+                GR_FREQ_ = std::min(P_->gr_freq_max, GR_FREQ_ * 2);
+                K_ = std::min(16, K_ * 2);
+                no_stag_ = 0;
+            } else if (++no_stag_ >= P_->stagnate_rounds) {
+                GR_FREQ_ = std::max(P_->gr_freq_min, GR_FREQ_ / 2);
+                K_ = std::max(4, K_ / 2);
+                no_stag_ = 0;
+            }
+            if (c.ex_t == last_ext_) ++stall_;
+            else stall_ = 0;
+            last_ext_ = c.ex_t;
+            // forced GR when most active labels are high (BC:1845-1849)
+            const float frac_high = active > 0 ? (float)c.active_high / (float)active : 0.f;
+            if (frac_high > 0.8f && iter_ % 2048 == 0) force_gr_ = true;
+            if (P_->rm && !rm_off_) {
+                // RM feedback (BC:2083-2093)
+                if (rm_last_) {
+                    const double frac_fix = g_.E > 0 ? (double)c.fixed_edges / (double)g_.E : 0.0;
+                    if (frac_fix < P_->rm_tau) RM_FREQ_ = std::min(RM_FREQ_ * 2, P_->rm_freq_max);
+                    else RM_FREQ_ = std::max(P_->rm_freq0, RM_FREQ_ / 2);
+                    stats.rm_fixed += c.fixed_edges;
+                }
+            }
+        } else {
+            last_pos_ = c.pos_excess;
+            last_ext_ = c.ex_t;
+        }
+        rm_last_ = false;
+        const bool gr_due = force_gr_ || (iter_ - last_gr_iter_ >= GR_FREQ_) ||
+                            (P_->gr_stall > 0 && stall_ >= P_->gr_stall);
+        if (gr_due && !exact) return start_gr();
+        // discharge step
+        if (active > 0 && active <= P_->path_switch) ++path_stable_;
+        else { path_stable_ = 0; p_count_ = P_->p_count0; }
+        const bool use_path = active > 0 && active <= P_->path_switch && path_stable_ >= p_count_;
+        p.stages = ST_DISCHARGE | ST_RESET_ACT | ST_ACTIVE;
+        p.reset_act = RST_ACTIVE;
+        if (use_path) {
+            p.dis_kind = DK_PATH;
+            p.dis_grid = std::max(1, cdiv(active, 4));
+            path_stable_ = 0;
+            if (++path_rounds_ >= 4) { path_rounds_ = 0; p_count_ = 2; }  // BC:1827-1829
+        } else if ((double)active >= P_->tile_frac * (double)g_.n) {
+            p.dis_kind = DK_TILE;
+            p.dis_grid = std::max(1, g_.ntiles);
+        } else {
+            p.dis_kind = DK_LIST;
+            p.dis_grid = std::max(1, cdiv(c.nlight, 256) + cdiv(c.nheavy, 4));
+        }
+        // hot-region extra sweep (BC:1898-1984)
+        if (P_->hot && !use_path && active >= P_->hot_min_active && (iter_ % 2 == 0) &&
+            !hotspot_triggered)
+            p.stages |= ST_HOT;
+        // RM repair (BC:1990-2097)
+        const float frac_high = active > 0 ? (float)c.active_high / (float)active : 0.f;
+        if (P_->rm && !rm_off_ && phase_ == 0 && frac_high >= 0.5f && iter_ != 0 &&
+            (iter_ % RM_FREQ_ == 0)) {
+            p.stages |= ST_RM;
+            p.reset_act |= RST_RM;
+            rm_last_ = true;
+        }
+        ++iter_;
+        stats.iters = iter_;
+        return p;
+    }
+    case S_CERT: {
+        stats.cert_ok = (c.cert_bad == 0 && c.cut == c.ex_t) ? 1 : 0;
+        stats.flow = c.ex_t;
+        if (!stats.cert_ok) {
+            fail(MF_ECERT, "min-cut certificate failed: bad=" + std::to_string(c.cert_bad) +
+                               " cut=" + std::to_string(c.cut) + " excess_t=" + std::to_string(c.ex_t));
+            return p;
+        }
+        st_ = S_DONE;
+        return p;
+    }
+    case S_DONE:
+    case S_ERROR:
+        return p;
+    }
+    return p;
+}
+
+// ------------------------------------------------------------------ engine
+Engine::~Engine() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    slots_.clear();
+    slabs_.clear();
+    cache_.clear();
+    graphs_.clear();
+    (void)hipFree(d_views_);
+    (void)hipFree(d_hash_);
+    if (h_stage_) (void)hipHostFree(h_stage_);
+    if (h_ctr_) (void)hipHostFree(h_ctr_);
+    if (h_hash_) (void)hipHostFree(h_hash_);
+    for (auto& ev : ev_) (void)hipEventDestroy(ev);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+double Engine::now() const { return wall_seconds() - t0_; }
+
+int Engine::init(int nslots, const Params& P) {
+    if (nslots < 1) { set_error("nslots must be >= 1"); return MF_EINVAL; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { set_error("no HIP device"); return MF_ENODEV; }
+    P_ = P;
+    nslots_ = nslots;
+    HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    // views followed by the launch lists (one per stage group, 32 groups)
+    list_cap_ = (size_t)nslots * 32;
+    HIPCHK(hipMalloc(&d_views_, sizeof(SlotView) * nslots + sizeof(int) * list_cap_));
+    HIPCHK(hipMalloc(&d_hash_, sizeof(unsigned long long) * nslots));
+    HIPCHK(hipHostMalloc(&h_stage_, sizeof(SlotView) * nslots + sizeof(int) * list_cap_));
+    HIPCHK(hipHostMalloc((void**)&h_ctr_, sizeof(Ctr) * nslots));
+    HIPCHK(hipHostMalloc((void**)&h_hash_, sizeof(unsigned long long) * nslots));
+    memset(h_ctr_, 0, sizeof(Ctr) * nslots);
+    ev_.resize(64);
+    for (auto& ev : ev_) HIPCHK(hipEventCreate(&ev));
+    slots_.resize(nslots);
+    slabs_.resize(nslots);
+    t0_ = wall_seconds();
+    return MF_OK;
+}
+
+int Engine::add_graph(std::shared_ptr<DeviceGraph> g) {
+    graphs_.push_back(g);
+    // grow slabs to fit the largest graph
+    for (int i = 0; i < nslots_; ++i) {
+        Slab* sl = slabs_[i].get();
+        if (!sl || sl->n_cap < g->g.n || sl->E_cap < g->g.E || sl->tiles_cap < g->g.ntiles) {
+            const int n = std::max(g->g.n, sl ? sl->n_cap : 0);
+            const int E = std::max(g->g.E, sl ? sl->E_cap : 0);
+            const int T = std::max(g->g.ntiles, sl ? sl->tiles_cap : 0);
+            auto ns = std::make_unique<Slab>();
+            slabs_[i].reset();
+            int rc = alloc_slab(n, E, T, *ns);
+            if (rc != MF_OK) return rc;
+            slabs_[i] = std::move(ns);
+        }
+    }
+    return (int)graphs_.size() - 1;
+}
+
+size_t Engine::slab_bytes_total() const {
+    size_t b = 0;
+    for (auto& s : slabs_) if (s) b += s->bytes;
+    return b;
+}
+
+int Engine::submit(const Query& q) {
+    if (q.graph < 0 || q.graph >= (int)graphs_.size()) { set_error("bad graph index"); return MF_EINVAL; }
+    queue_.push_back(q);
+    if (queue_.back().t_arrive <= 0) queue_.back().t_arrive = now();
+    return MF_OK;
+}
+
+int Engine::live() const {
+    int k = 0;
+    for (auto& s : slots_) k += s.live ? 1 : 0;
+    return k;
+}
+
+int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
+    SlotState& S = slots_[i];
+    S.live = true;
+    S.q = q;
+    S.run_graph.reset();
+    const DeviceGraph& G = *graphs_[q.graph];
+    S.parent_n = G.g.n;
+    Slab& sl = *slabs_[i];
+    SlotView& v = S.view;
+    v = SlotView();
+    v.g = G.g;
+    v.res = sl.res; v.excess = sl.excess; v.h = sl.h; v.fa = sl.fa; v.fb = sl.fb;
+    v.lock = sl.lock; v.aux = sl.aux; v.tile_cnt = sl.tile_cnt; v.tile_hot = sl.tile_hot;
+    v.ctr = sl.ctr;
+    v.s = q.s; v.t = q.t;
+    v.H_cut = P_.h_cut;
+    const bool prune = P_.prune != 0 && prune_ok && !warm;
+    S.sched.begin(&P_, G.g, q.s, q.t, prune, warm);
+    S.sched.stats.t_submit = q.t_arrive;
+    S.sched.stats.t_start = now();
+    v.K = S.sched.K();
+    return MF_OK;
+}
+
+// Prune decision (BC:2678-2698): keep the induced subgraph when the mask
+// drops at least (1 - prune_ratio) of the vertices.
+int Engine::prune_decide(int i) {
+    SlotState& S = slots_[i];
+    const DeviceGraph& G = *graphs_[S.q.graph];
+    const Ctr& c = h_ctr_[i];
+    const int kept = c.visited;
+    S.sched.stats.kept = kept;
+    if (kept > 0 && (double)kept < P_.prune_ratio * (double)G.g.n) {
+        const unsigned long long hsh = h_hash_[i];
+        const std::string key = std::to_string(S.q.graph) + ":" + std::to_string(hsh) + ":" + std::to_string(kept);
+        std::shared_ptr<DeviceGraph> run;
+        auto it = cache_.find(key);
+        if (it != cache_.end()) {
+            run = it->second.g;
+            it->second.last_use = ++use_clock_;
+            ++cache_hits;
+        } else {
+            auto ng = std::make_shared<DeviceGraph>();
+            int rc = build_induced_gpu(G, S.view.aux, kept, stream_, *ng);
+            if (rc != MF_OK) return rc;
+            // evict least recently used entries over budget
+            cache_bytes_ += ng->bytes;
+            while (cache_bytes_ > cache_budget && !cache_.empty()) {
+                auto lru = cache_.begin();
+                for (auto jt = cache_.begin(); jt != cache_.end(); ++jt)
+                    if (jt->second.last_use < lru->second.last_use) lru = jt;
+                cache_bytes_ -= lru->second.g->bytes;
+                cache_.erase(lru);
+            }
+            cache_[key] = CacheEntry{ng, ++use_clock_};
+            run = ng;
+            ++cache_misses;
+        }
+        // map s, t into the run graph
+        int ids[2] = {-1, -1};
+        HIPCHK(hipMemcpyAsync(&ids[0], run->d_old2sub + S.q.s, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipMemcpyAsync(&ids[1], run->d_old2sub + S.q.t, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        if (ids[0] >= 0 && ids[1] >= 0) {
+            S.run_graph = run;
+            S.view.g = run->g;
+            S.view.s = ids[0];
+            S.view.t = ids[1];
+            S.sched.stats.pruned = 1;
+            S.sched.resume_after_prune(run->g, ids[0], ids[1]);
+            S.view.K = S.sched.K();
+            return MF_OK;
+        }
+    }
+    S.sched.resume_after_prune(G.g, S.q.s, S.q.t);
+    S.view.K = S.sched.K();
+    return MF_OK;
+}
+
+int Engine::step(std::vector<Result>& out) {
+    // refill free slots (continuous batching)
+    for (int i = 0; i < nslots_ && !queue_.empty(); ++i) {
+        if (slots_[i].live || slots_[i].pinned) continue;
+        Query q = queue_.front();
+        queue_.pop_front();
+        int rc = admit(i, q);
+        if (rc != MF_OK) return rc;
+        memset(&h_ctr_[i], 0, sizeof(Ctr));
+    }
+    // plan
+    std::vector<Plan> plans(nslots_);
+    for (int i = 0; i < nslots_; ++i) {
+        SlotState& S = slots_[i];
+        if (!S.live) continue;
+        if (S.sched.wants_prune_decision()) {
+            int rc = prune_decide(i);
+            if (rc != MF_OK) return rc;
+        }
+        plans[i] = S.sched.plan(h_ctr_[i]);
+        S.view.phase = S.sched.phase();
+        S.view.K = S.sched.K();
+        S.sched.stats.steps++;
+    }
+    // retire finished slots
+    for (int i = 0; i < nslots_; ++i) {
+        SlotState& S = slots_[i];
+        if (!S.live || !S.sched.done()) continue;
+        Result r;
+        r.tag = S.q.tag;
+        r.st = S.sched.stats;
+        r.st.status = S.sched.status();
+        const Ctr& c = h_ctr_[i];
+        r.st.edge_scans = r.st.vertex_scans = r.st.pushes = r.st.relabels = 0;
+        for (int f = 0; f < kFamilies; ++f) {
+            for (int k = 0; k < 4; ++k) {
+                r.st.wk[f][k] = c.wk[f][k];
+                fam_work[f][k] += c.wk[f][k];
+            }
+            r.st.edge_scans += c.wk[f][0];
+            r.st.vertex_scans += c.wk[f][1];
+            r.st.pushes += c.wk[f][2];
+            r.st.relabels += c.wk[f][3];
+        }
+        r.st.t_end = now();
+        if (r.st.status != MF_OK) set_error(S.sched.error());
+        out.push_back(r);
+        S.live = false;
+        S.run_graph.reset();
+        plans[i] = Plan();
+    }
+    int rc = launch_plans(plans);
+    if (rc < 0) return rc;
+    return live();
+}
+
+// Stage every live slot's view, build one launch list per stage group, issue
+// the groups in stage order on the engine stream, copy the counters back and
+// wait.  One host synchronisation per call.
+int Engine::launch_plans(std::vector<Plan>& plans) {
+    // stage the views and lists
+    SlotView* hv = (SlotView*)h_stage_;
+    int* hl = (int*)((char*)h_stage_ + sizeof(SlotView) * nslots_);
+    for (int i = 0; i < nslots_; ++i) hv[i] = slots_[i].view;
+    size_t lpos = 0;
+    struct Group { int off = 0, cnt = 0; };
+    auto make_list = [&](auto pred) {
+        Group g;
+        g.off = (int)lpos;
+        for (int i = 0; i < nslots_; ++i)
+            if (slots_[i].live && pred(i)) hl[lpos++] = i;
+        g.cnt = (int)lpos - g.off;
+        return g;
+    };
+    auto has = [&](int i, unsigned bit) { return (plans[i].stages & bit) != 0; };
+    // reset_pre grouped by distinct mask values
+    std::vector<std::pair<unsigned, Group>> pre_groups;
+    {
+        std::vector<unsigned> vals;
+        for (int i = 0; i < nslots_; ++i)
+            if (slots_[i].live && has(i, ST_RESET_PRE) &&
+                std::find(vals.begin(), vals.end(), plans[i].reset_pre) == vals.end())
+                vals.push_back(plans[i].reset_pre);
+        for (unsigned w : vals)
+            pre_groups.push_back({w, make_list([&](int i) { return has(i, ST_RESET_PRE) && plans[i].reset_pre == w; })});
+    }
+    Group g_init = make_list([&](int i) { return has(i, ST_INIT); });
+    Group g_pre = make_list([&](int i) { return has(i, ST_PREFLOW); });
+    Group g_seed = make_list([&](int i) { return has(i, ST_GR_SEED); });
+    Group g_seed1[3];
+    for (int m = 0; m < 3; ++m) g_seed1[m] = make_list([&](int i) { return has(i, ST_BFS_SEED1) && plans[i].seed_mode == m; });
+    Group g_bfs[3][3];
+    for (int k = 0; k < 3; ++k)
+        for (int m = 0; m < 3; ++m)
+            g_bfs[k][m] = make_list([&](int i) { return has(i, ST_BFS) && plans[i].bfs_kind == k && plans[i].bfs_mode == m; });
+    Group g_mask = make_list([&](int i) { return has(i, ST_MASK); });
+    Group g_dis[3];
+    for (int k = 0; k < 3; ++k) g_dis[k] = make_list([&](int i) { return has(i, ST_DISCHARGE) && plans[i].dis_kind == k; });
+    Group g_hot = make_list([&](int i) { return has(i, ST_HOT); });
+    Group g_rm = make_list([&](int i) { return has(i, ST_RM); });
+    std::vector<std::pair<unsigned, Group>> act_groups;
+    {
+        std::vector<unsigned> vals;
+        for (int i = 0; i < nslots_; ++i)
+            if (slots_[i].live && has(i, ST_RESET_ACT) &&
+                std::find(vals.begin(), vals.end(), plans[i].reset_act) == vals.end())
+                vals.push_back(plans[i].reset_act);
+        for (unsigned w : vals)
+            act_groups.push_back({w, make_list([&](int i) { return has(i, ST_RESET_ACT) && plans[i].reset_act == w; })});
+    }
+    Group g_act = make_list([&](int i) { return has(i, ST_ACTIVE); });
+    Group g_cert = make_list([&](int i) { return has(i, ST_CERT); });
+    if (lpos > list_cap_) { set_error("launch list overflow"); return MF_EINVAL; }
+    const int nlive = live();
+    if (nlive == 0) return 0;
+    HIPCHK(hipMemcpyAsync(d_views_, h_stage_, sizeof(SlotView) * nslots_ + sizeof(int) * lpos,
+                          hipMemcpyHostToDevice, stream_));
+    // the launch lists sit right after the views
+    int* d_list_base = (int*)((char*)d_views_ + sizeof(SlotView) * nslots_);
+    auto L = [&](const Group& g) { return d_list_base + g.off; };
+    auto maxof = [&](const Group& g, auto f) {
+        int64_t m = 1;
+        for (int k = 0; k < g.cnt; ++k) m = std::max<int64_t>(m, f(hl[g.off + k]));
+        return (int)std::min<int64_t>(m, INT32_MAX);
+    };
+    hipStream_t st = stream_;
+    // profiling: an event pair around each kernel family's launches
+    std::vector<std::pair<int, int>> timed;  // (family, event pair index)
+    auto tb = [&](int fam) {
+        if (!profile) return -1;
+        const int k = (int)timed.size();
+        if (2 * k + 1 >= (int)ev_.size()) return -1;
+        (void)hipEventRecord(ev_[2 * k], st);
+        timed.push_back({fam, k});
+        return k;
+    };
+    auto te = [&](int k) {
+        if (k >= 0) (void)hipEventRecord(ev_[2 * k + 1], st);
+    };
+    auto gridN = [&](int i) { return clampi(cdiv(slots_[i].view.g.n, 256), 1, 4096); };
+    auto gridE = [&](int i) { return clampi(cdiv(slots_[i].view.g.E, 1024), 1, 8192); };
+    for (auto& pg : pre_groups) launch_reset(st, d_views_, L(pg.second), pg.second.cnt, pg.first);
+    if (g_init.cnt) {
+        const int k = tb(WF_INIT);
+        launch_init(st, d_views_, L(g_init), g_init.cnt, maxof(g_init, [&](int i) { return std::max(gridE(i), gridN(i)); }));
+        te(k);
+    }
+    if (g_pre.cnt) { const int k = tb(WF_INIT); launch_preflow(st, d_views_, L(g_pre), g_pre.cnt); te(k); }
+    if (g_seed.cnt) { const int k = tb(WF_BFS); launch_gr_seed(st, d_views_, L(g_seed), g_seed.cnt, maxof(g_seed, gridN)); te(k); }
+    for (int m = 0; m < 3; ++m)
+        if (g_seed1[m].cnt) {
+            const int k = tb(WF_BFS);
+            launch_bfs_seed1(st, d_views_, L(g_seed1[m]), g_seed1[m].cnt, maxof(g_seed1[m], gridN), m);
+            te(k);
+        }
+    for (int m = 0; m < 3; ++m) {
+        if (g_bfs[BK_SMALL][m].cnt) {
+            const int k = tb(WF_BFS);
+            launch_bfs_small(st, d_views_, L(g_bfs[BK_SMALL][m]), g_bfs[BK_SMALL][m].cnt, m, P_.small_levels);
+            te(k);
+        }
+        if (g_bfs[BK_TD][m].cnt) {
+            const int k = tb(WF_BFS);
+            launch_bfs_td(st, d_views_, L(g_bfs[BK_TD][m]), g_bfs[BK_TD][m].cnt, maxof(g_bfs[BK_TD][m], [&](int i) { return plans[i].bfs_grid; }), m);
+            te(k);
+        }
+        if (g_bfs[BK_BU][m].cnt) {
+            const int k = tb(WF_BFS);
+            launch_bfs_bu(st, d_views_, L(g_bfs[BK_BU][m]), g_bfs[BK_BU][m].cnt, maxof(g_bfs[BK_BU][m], [&](int i) { return plans[i].bfs_grid; }), m);
+            te(k);
+        }
+    }
+    if (g_mask.cnt) {
+        HIPCHK(hipMemsetAsync(d_hash_, 0, sizeof(unsigned long long) * nslots_, st));
+        const int k = tb(WF_BFS);
+        launch_mask_combine(st, d_views_, L(g_mask), g_mask.cnt, maxof(g_mask, gridN), d_hash_);
+        te(k);
+    }
+    if (g_dis[DK_TILE].cnt) {
+        const int k = tb(WF_DISCHARGE);
+        launch_discharge_tile(st, d_views_, L(g_dis[DK_TILE]), g_dis[DK_TILE].cnt, maxof(g_dis[DK_TILE], [&](int i) { return plans[i].dis_grid; }), 0, P_.tile_rounds);
+        te(k);
+    }
+    if (g_dis[DK_LIST].cnt) {
+        const int k = tb(WF_DISCHARGE);
+        launch_discharge_list(st, d_views_, L(g_dis[DK_LIST]), g_dis[DK_LIST].cnt, maxof(g_dis[DK_LIST], [&](int i) { return plans[i].dis_grid; }));
+        te(k);
+    }
+    if (g_dis[DK_PATH].cnt) {
+        const int k = tb(WF_DISCHARGE);
+        launch_path_push(st, d_views_, L(g_dis[DK_PATH]), g_dis[DK_PATH].cnt, maxof(g_dis[DK_PATH], [&](int i) { return plans[i].dis_grid; }), P_.path_len * P_.path_attempts);
+        te(k);
+    }
+    if (g_hot.cnt) {
+        const int k = tb(WF_DISCHARGE);
+        launch_hot_select(st, d_views_, L(g_hot), g_hot.cnt, P_.hot_k);
+        launch_discharge_tile(st, d_views_, L(g_hot), g_hot.cnt, maxof(g_hot, [&](int i) { return slots_[i].view.g.ntiles; }), 1, P_.hot_rounds);
+        te(k);
+    }
+    if (g_rm.cnt) {
+        const int k = tb(WF_RM);
+        launch_rm(st, d_views_, L(g_rm), g_rm.cnt, maxof(g_rm, [&](int i) { return slots_[i].view.g.ntiles; }), P_.rm_margin);
+        te(k);
+    }
+    for (auto& ag : act_groups) launch_reset(st, d_views_, L(ag.second), ag.second.cnt, ag.first);
+    if (g_act.cnt) { const int k = tb(WF_ACTIVE); launch_active(st, d_views_, L(g_act), g_act.cnt, maxof(g_act, gridN)); te(k); }
+    if (g_cert.cnt) { const int k = tb(WF_CERT); launch_cert(st, d_views_, L(g_cert), g_cert.cnt, maxof(g_cert, gridN)); te(k); }
+    HIPCHK(hipGetLastError());
+    for (int i = 0; i < nslots_; ++i)
+        if (slots_[i].live)
+            HIPCHK(hipMemcpyAsync(&h_ctr_[i], slots_[i].view.ctr, sizeof(Ctr), hipMemcpyDeviceToHost, st));
+    if (g_mask.cnt)
+        HIPCHK(hipMemcpyAsync(h_hash_, d_hash_, sizeof(unsigned long long) * nslots_, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (auto& pr : timed) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev_[2 * pr.second], ev_[2 * pr.second + 1]) == hipSuccess) {
+            fam_ms[pr.first] += ms;
+            fam_launches[pr.first] += 1;
+        }
+    }
+    return MF_OK;
+}
+
+void Engine::prof_reset() {
+    for (int f = 0; f < kFamilies; ++f) {
+        fam_ms[f] = 0;
+        fam_launches[f] = 0;
+        for (int k = 0; k < 4; ++k) fam_work[f][k] = 0;
+    }
+}
+
+int Engine::pin(int slot, const Query& q) {
+    if (slot < 0 || slot >= nslots_) { set_error("bad slot"); return MF_EINVAL; }
+    if (q.graph < 0 || q.graph >= (int)graphs_.size()) { set_error("bad graph index"); return MF_EINVAL; }
+    if (slots_[slot].live) { set_error("slot busy"); return MF_EBUSY; }
+    const int n = graphs_[q.graph]->g.n;
+    if (q.s < 0 || q.t < 0 || q.s >= n || q.t >= n || q.s == q.t) { set_error("bad query"); return MF_EINVAL; }
+    slots_[slot].pinned = true;
+    slots_[slot].solved = false;
+    slots_[slot].q = q;
+    return MF_OK;
+}
+
+int Engine::solve_pinned(bool warm, std::vector<Result>& out) {
+    std::vector<int> mine;
+    for (int i = 0; i < nslots_; ++i) {
+        SlotState& S = slots_[i];
+        if (!S.pinned) continue;
+        const bool w = warm && S.solved;
+        int rc = admit(i, S.q, w, false);
+        if (rc != MF_OK) return rc;
+        memset(&h_ctr_[i], 0, sizeof(Ctr));
+        mine.push_back(i);
+    }
+    std::vector<Result> got;
+    std::vector<char> retried(nslots_, 0);
+    while (true) {
+        bool any = false;
+        for (int i : mine) any = any || slots_[i].live;
+        if (!any) break;
+        const size_t before = got.size();
+        int rc = step(got);
+        if (rc < 0) return rc;
+        // a warm answer that fails its certificate is re-solved from scratch
+        for (size_t k = before; k < got.size(); ++k) {
+            Result& r = got[k];
+            int i = -1;
+            for (int j : mine) if (slots_[j].q.tag == r.tag && !slots_[j].live) i = j;
+            if (i >= 0 && r.st.status != MF_OK && warm && !retried[i]) {
+                retried[i] = 1;
+                int rc2 = admit(i, slots_[i].q, false, false);
+                if (rc2 != MF_OK) return rc2;
+                memset(&h_ctr_[i], 0, sizeof(Ctr));
+                r.tag = INT64_MIN;  // drop this attempt
+            }
+        }
+    }
+    for (auto& r : got) {
+        if (r.tag == INT64_MIN) continue;
+        for (int j : mine)
+            if (slots_[j].q.tag == r.tag) {
+                slots_[j].solved = (r.st.status == MF_OK);
+                if (retried[j]) r.st.drains = -1 - r.st.drains;
+            }
+        out.push_back(r);
+    }
+    return MF_OK;
+}
+
+int Engine::update_caps(int gid, const int32_t* entries, const int64_t* new_cap, int n) {
+    if (gid < 0 || gid >= (int)graphs_.size()) { set_error("bad graph index"); return MF_EINVAL; }
+    if (n <= 0) return MF_OK;
+    DeviceGraph& G = *graphs_[gid];
+    for (int i = 0; i < n; ++i) {
+        if (entries[i] < 0 || entries[i] >= G.g.E) { set_error("update entry out of range"); return MF_EINVAL; }
+        if (new_cap[i] < 1) { set_error("updated capacity must be >= 1"); return MF_EINVAL; }
+    }
+    int32_t* d_e = nullptr;
+    int64_t *d_new = nullptr, *d_old = nullptr;
+    HIPCHK(hipMalloc(&d_e, sizeof(int32_t) * n));
+    HIPCHK(hipMalloc(&d_new, sizeof(int64_t) * n));
+    HIPCHK(hipMalloc(&d_old, sizeof(int64_t) * n));
+    HIPCHK(hipMemcpyAsync(d_e, entries, sizeof(int32_t) * n, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyAsync(d_new, new_cap, sizeof(int64_t) * n, hipMemcpyHostToDevice, stream_));
+    launch_cap_scatter(stream_, G.d_cap, d_e, d_new, d_old, n);
+    // adjust every pinned residual on this graph
+    SlotView* hv = (SlotView*)h_stage_;
+    int* hl = (int*)((char*)h_stage_ + sizeof(SlotView) * nslots_);
+    int cnt = 0;
+    for (int i = 0; i < nslots_; ++i) {
+        hv[i] = slots_[i].view;
+        if (slots_[i].pinned && slots_[i].solved && slots_[i].q.graph == gid) hl[cnt++] = i;
+    }
+    if (cnt) {
+        HIPCHK(hipMemcpyAsync(d_views_, h_stage_, sizeof(SlotView) * nslots_ + sizeof(int) * cnt,
+                              hipMemcpyHostToDevice, stream_));
+        int* d_list = (int*)((char*)d_views_ + sizeof(SlotView) * nslots_);
+        launch_apply_updates(stream_, d_views_, d_list, cnt, d_e, d_old, d_new, n);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
+    (void)hipFree(d_e); (void)hipFree(d_new); (void)hipFree(d_old);
+    // cached induced subgraphs of this graph carry old capacities
+    const std::string pre = std::to_string(gid) + ":";
+    for (auto it = cache_.begin(); it != cache_.end();) {
+        if (it->first.compare(0, pre.size(), pre) == 0) { cache_bytes_ -= it->second.g->bytes; it = cache_.erase(it); }
+        else ++it;
+    }
+    return MF_OK;
+}
+
+int Engine::dbg_begin(int gid, int s, int t) {
+    if (gid < 0 || gid >= (int)graphs_.size()) { set_error("bad graph index"); return MF_EINVAL; }
+    for (auto& S : slots_) if (S.live) { set_error("engine busy"); return MF_EBUSY; }
+    Query q;
+    q.graph = gid; q.s = s; q.t = t;
+    int rc = admit(0, q);
+    if (rc != MF_OK) return rc;
+    memset(&h_ctr_[0], 0, sizeof(Ctr));
+    return MF_OK;
+}
+
+int Engine::dbg_plan(const Plan& p) {
+    std::vector<Plan> plans(nslots_);
+    plans[0] = p;
+    slots_[0].view.phase = 0;
+    return launch_plans(plans);
+}
+
+void Engine::dbg_end() { slots_[0].live = false; slots_[0].run_graph.reset(); }
+
+const Ctr& Engine::ctr(int i) const { return h_ctr_[i]; }
+SlotView& Engine::view(int i) { return slots_[i].view; }
+
+int Engine::run(std::vector<Result>& out, int64_t max_steps) {
+    int64_t k = 0;
+    for (;;) {
+        const int before = live();
+        if (before == 0 && queue_.empty()) break;
+        int rc = step(out);
+        if (rc < 0) return rc;
+        if (max_steps >= 0 && ++k >= max_steps) break;
+    }
+    // flush slots that finished on the last step
+    return MF_OK;
+}
+
+}  // namespace mf

@@ -1,0 +1,299 @@
+// mf_engine.h -- resident graphs, per-query slabs, the per-query scheduler
+// and the step executor that drives many queries through the kernels at once.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <deque>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/mscmf.h"
+#include "host_graph.h"
+#include "mf_device.h"
+
+namespace mf {
+
+// ------------------------------------------------------------------ errors
+// Status codes are the ABI's (MF_OK, MF_EINVAL, ... in include/mscmf.h).
+void set_error(const std::string& msg);
+const std::string& last_error();
+
+// -------------------------------------------------------------- parameters
+// Defaults reproduce the reference's constants (BC:1459-1484, 1725, 1936).
+struct Params {
+    int K = 0;                 // cycles per vertex per discharge pass; 0 = max(4, E/V)
+    int gr_freq = 128;         // GR every gr_freq iterations (BC:1466)
+    int gr_freq_min = 128, gr_freq_max = 256;
+    int gr_stall = 0;          // also GR after excess[t] stalls this many iterations (0 = off)
+    int h_cut = 100;           // BC:1469
+    int stagnate_rounds = 128; // BC:1472
+    int path_switch = 8;       // BC:1480
+    int path_len = 256;        // BC:1481
+    int path_attempts = 4;     // BC:895
+    int p_count0 = 64;         // BC:1483
+    int hot = 1, hot_k = 32, hot_min_active = 64, hot_rounds = 1;  // BC:1899, 1936
+    int rm = 1, rm_freq0 = 64, rm_freq_max = 256, rm_margin = 4;    // BC:1459, 2023, 2086
+    double rm_tau = 1e-5;
+    int tile_rounds = 1;       // local cycles per region pass (LDS-staged)
+    double tile_frac = 1.0 / 64.0;  // region mode when active >= tile_frac * n
+    int cert = 1;              // certify every answer on the GPU
+    long long iter_max = 20000000;  // BC:1476, but reported as an error
+    int small_levels = 1 << 30;
+    double bu_alpha = 14.0, bu_beta = 24.0;  // direction-optimizing BFS thresholds
+    int prune = 1;             // per-query reachability pruning (BC:2678)
+    double prune_ratio = 0.995;  // BC:2647
+};
+
+// ---------------------------------------------------------- device memory
+struct DeviceGraph {
+    DGraph g;
+    int32_t* d_row_ptr = nullptr;
+    int32_t* d_col = nullptr;
+    int64_t* d_cap = nullptr;
+    int32_t* d_rev = nullptr;
+    int32_t* d_tile_start = nullptr;
+    int32_t* d_tile_of = nullptr;
+    int32_t* d_sub2old = nullptr;  // pruned run graphs: run id -> parent id
+    int32_t* d_old2sub = nullptr;  // pruned run graphs: parent id -> run id (-1 = dropped)
+    uint8_t* d_mask = nullptr;     // pruned run graphs: parent mask (cache check)
+    size_t bytes = 0;
+    int tile_max = kTileMax;
+    ~DeviceGraph();
+};
+
+// Tiles: each region [rstart[r], rstart[r+1]) cut into pieces of at most tile_max.
+void make_tiles(const std::vector<int32_t>& rstart, int tile_max, std::vector<int32_t>& tile_start);
+
+int upload_graph(const HostGraph& hg, int tile_max, DeviceGraph& dg);
+
+struct Slab {
+    int n_cap = 0, E_cap = 0, tiles_cap = 0;
+    int64_t* res = nullptr;
+    int64_t* excess = nullptr;
+    int32_t* h = nullptr;
+    int32_t* fa = nullptr;
+    int32_t* fb = nullptr;
+    int32_t* lock = nullptr;
+    int32_t* aux = nullptr;
+    int32_t* tile_cnt = nullptr;
+    int32_t* tile_hot = nullptr;
+    Ctr* ctr = nullptr;
+    size_t bytes = 0;
+    ~Slab();
+    static size_t bytes_for(int n, int64_t E, int ntiles);
+};
+
+int alloc_slab(int n, int64_t E, int ntiles, Slab& sl);
+
+// --------------------------------------------------------------- scheduler
+enum StageBit : unsigned {
+    ST_RESET_PRE = 1u << 0,
+    ST_INIT = 1u << 1,
+    ST_PREFLOW = 1u << 2,
+    ST_GR_SEED = 1u << 3,
+    ST_BFS_SEED1 = 1u << 4,
+    ST_BFS = 1u << 5,
+    ST_MASK = 1u << 6,
+    ST_DISCHARGE = 1u << 7,
+    ST_HOT = 1u << 8,
+    ST_RM = 1u << 9,
+    ST_RESET_ACT = 1u << 10,
+    ST_ACTIVE = 1u << 11,
+    ST_CERT = 1u << 12,
+};
+
+enum BfsKind : int { BK_SMALL = 0, BK_TD = 1, BK_BU = 2 };
+enum DisKind : int { DK_TILE = 0, DK_LIST = 1, DK_PATH = 2 };
+
+struct Plan {
+    unsigned stages = 0;
+    unsigned reset_pre = 0;
+    unsigned reset_act = 0;
+    int bfs_kind = BK_SMALL, bfs_mode = 0, bfs_grid = 1;
+    int dis_kind = DK_LIST, dis_grid = 1;
+    int seed_mode = 0;
+};
+
+// Per-query statistics (host side).
+struct QueryStats {
+    int64_t flow = 0;
+    int status = MF_OK;
+    int n_run = 0;
+    int64_t E_run = 0;
+    int pruned = 0;           // ran on an induced subgraph
+    int kept = 0;             // vertices kept by the pruning mask
+    int64_t iters = 0;        // discharge iterations
+    int64_t steps = 0;        // scheduler steps (host syncs)
+    int64_t grs = 0;          // exact global relabels
+    int64_t bfs_levels = 0;
+    int64_t drains = 0;       // deficit drain phases
+    int64_t rm_fixed = 0;
+    int cert_ok = 0;
+    uint64_t edge_scans = 0, vertex_scans = 0, pushes = 0, relabels = 0;
+    uint64_t wk[kFamilies][4] = {};  // per kernel family (see WorkFamily)
+    double t_submit = 0, t_start = 0, t_end = 0;  // seconds, engine clock
+};
+
+class SlotSched {
+public:
+    // warm: keep the residual and excess already in the slab (dynamic mode)
+    void begin(const Params* P, const DGraph& g, int s, int t, bool prune_first, bool warm = false);
+    // Decide the next step given the counters of the last one.
+    Plan plan(const Ctr& c);
+    bool done() const { return st_ == S_DONE || st_ == S_ERROR; }
+    int status() const { return status_; }
+    const std::string& error() const { return err_; }
+    int phase() const { return phase_; }
+    int K() const { return K_; }
+    QueryStats stats;
+    // pruning hand-off: after S_MASK the engine swaps graphs and calls resume()
+    bool wants_prune_decision() const { return st_ == S_PRUNE_DECIDE; }
+    void resume_after_prune(const DGraph& g, int s, int t);
+
+private:
+    enum St { S_PRUNE_FWD, S_PRUNE_UND, S_PRUNE_MASK, S_PRUNE_DECIDE, S_START, S_WARM_START, S_BFS,
+              S_COUNTED, S_CERT, S_DONE, S_ERROR };
+    Plan start_gr();
+    Plan bfs_step(const Ctr& c, int mode);
+    void fail(int code, const std::string& msg);
+
+    const Params* P_ = nullptr;
+    DGraph g_;
+    int s_ = 0, t_ = 0;
+    St st_ = S_START;
+    int status_ = MF_OK;
+    std::string err_;
+    int phase_ = 0;
+    int bfs_mode_ = 0;
+    int64_t bfs_medges_ = 0;   // edges of visited vertices (direction heuristic)
+    bool bfs_bottom_up_ = false;
+    bool just_gr_ = false;
+    int64_t iter_ = 0, last_gr_iter_ = 0;
+    int GR_FREQ_ = 128, K_ = 4, RM_FREQ_ = 64;
+    int stag_ = 0, no_stag_ = 0;
+    int64_t last_pos_ = INT64_MAX;
+    bool force_gr_ = false;
+    int path_stable_ = 0, p_count_ = 64, path_rounds_ = 0;
+    int64_t last_ext_ = -1;
+    int stall_ = 0;
+    int drain_loops_ = 0;
+    bool rm_off_ = false;
+    bool rm_last_ = false;
+    bool work_reset_ = false;
+};
+
+// ------------------------------------------------------------------ engine
+// A query submitted to the engine.
+struct Query {
+    int graph = 0;     // index into the engine's graph table
+    int s = 0, t = 0;  // ids of that graph
+    int64_t tag = 0;
+    double t_arrive = 0;  // engine clock; 0 = time of submit()
+};
+
+struct Result {
+    int64_t tag = 0;
+    QueryStats st;
+};
+
+class Engine {
+public:
+    Engine() = default;
+    ~Engine();
+    // nslots concurrent queries; each slot's slab is sized for the largest
+    // registered graph it may run.
+    int init(int nslots, const Params& P);
+    int add_graph(std::shared_ptr<DeviceGraph> g);  // returns graph index
+    int submit(const Query& q);                       // queued
+    // Run scheduler steps until the queue and all slots are empty, or until
+    // max_steps.  Finished results are appended to out.
+    int run(std::vector<Result>& out, int64_t max_steps = -1);
+    // One step; returns number of live slots after it.
+    int step(std::vector<Result>& out);
+    size_t queued() const { return queue_.size(); }
+    int live() const;
+    const Params& params() const { return P_; }
+    Params& params_mut() { return P_; }
+    double now() const;
+    hipStream_t stream() const { return stream_; }
+    // Device pointers of slot i's slab (debug / parity hooks).
+    Slab* slab(int i) { return slabs_[i].get(); }
+    int nslots() const { return nslots_; }
+    size_t slab_bytes_total() const;
+    // Single-slot manual control for parity hooks: admit (gid, s, t) into
+    // slot 0 without scheduling, launch hand-built plans, release.
+    int dbg_begin(int gid, int s, int t);
+    int dbg_plan(const Plan& p);
+    void dbg_end();
+    const Ctr& ctr(int i) const;
+    SlotView& view(int i);
+    DeviceGraph* graph(int gid) { return graphs_[gid].get(); }
+    int ngraphs() const { return (int)graphs_.size(); }
+    // Dynamic mode: pinned slots keep their residual across updates.
+    int pin(int slot, const Query& q);
+    int solve_pinned(bool warm, std::vector<Result>& out);
+    int update_caps(int gid, const int32_t* entries, const int64_t* new_cap, int n);
+
+private:
+    struct SlotState {
+        bool live = false;
+        bool pinned = false;
+        bool solved = false;   // pinned slot holds a residual for its query
+        Query q;
+        SlotSched sched;
+        SlotView view;
+        std::shared_ptr<DeviceGraph> run_graph;  // pruned subgraph when used
+        int parent_n = 0;
+    };
+    int admit(int slot, const Query& q, bool warm = false, bool prune = true);
+    int prune_decide(int slot);
+    int launch_plans(std::vector<Plan>& plans);
+
+    Params P_;
+    int nslots_ = 0;
+    hipStream_t stream_ = nullptr;
+    std::vector<std::shared_ptr<DeviceGraph>> graphs_;
+    std::vector<std::unique_ptr<Slab>> slabs_;
+    std::vector<SlotState> slots_;
+    std::deque<Query> queue_;
+    // device copies of views and launch lists; pinned staging
+    SlotView* d_views_ = nullptr;
+    unsigned long long* d_hash_ = nullptr;
+    void* h_stage_ = nullptr;  // pinned: views + lists
+    Ctr* h_ctr_ = nullptr;     // pinned: per-slot counter snapshots
+    unsigned long long* h_hash_ = nullptr;
+    size_t list_cap_ = 0;
+    double t0_ = 0;
+    std::vector<hipEvent_t> ev_;  // profiling event pairs
+    // induced-subgraph cache: key = (graph index, mask hash, kept count)
+    struct CacheEntry {
+        std::shared_ptr<DeviceGraph> g;
+        uint64_t last_use = 0;
+    };
+    std::unordered_map<std::string, CacheEntry> cache_;
+    uint64_t use_clock_ = 0;
+    size_t cache_bytes_ = 0;
+public:
+    // Per kernel family timing with HIP events on the engine stream, and the
+    // algorithmic work of retired queries.
+    bool profile = false;
+    double fam_ms[kFamilies] = {};
+    int64_t fam_launches[kFamilies] = {};
+    uint64_t fam_work[kFamilies][4] = {};
+    void prof_reset();
+    size_t cache_budget = (size_t)64 << 30;
+    int64_t cache_hits = 0, cache_misses = 0;
+};
+
+// Induced subgraph of parent on the slot's mask (aux: 0/1), built on the GPU
+// (BC:2477-2551).  Vertex order, and so region/tile membership, is inherited
+// from the parent.
+int build_induced_gpu(const DeviceGraph& parent, const int32_t* d_mask, int kept,
+                      hipStream_t st, DeviceGraph& out);
+
+double wall_seconds();
+
+}  // namespace mf

@@ -1,0 +1,82 @@
+// mf_kernels.h -- launch wrappers for the CDNA4 push-relabel kernels.
+//
+// Every kernel works on a list of slots (one query each): blockIdx.y selects
+// list[blockIdx.y], blockIdx.x splits that slot's work.  A plain solve is a
+// list of one slot.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include "mf_device.h"
+
+namespace mf {
+
+// Counter reset bits for launch_reset.
+enum : unsigned {
+    RST_BFS = 1u << 0,     // fsize/fedges ring, level, visited
+    RST_ACTIVE = 1u << 1,  // active counters, excess sums, band, tile_cnt
+    RST_CERT = 1u << 2,    // cert_bad, cut
+    RST_RM = 1u << 3,      // fixed_edges
+    RST_WORK = 1u << 4,    // work counters
+    RST_ERR = 1u << 5,
+};
+
+enum BfsMode : int { BFS_GR = 0, BFS_FWD = 1, BFS_UND = 2 };
+
+void launch_reset(hipStream_t st, const SlotView* slots, const int* list, int nlist, unsigned what);
+void launch_init(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);
+void launch_preflow(hipStream_t st, const SlotView* slots, const int* list, int nlist);
+
+// Global-relabel seeding: h = 0 on seeds, n elsewhere; seeds go to fa.
+void launch_gr_seed(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);
+// Single-source BFS seeding for pruning: lvl (h for FWD, aux for UND) = n,
+// lvl[src] = 0, frontier {src}.
+void launch_bfs_seed1(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                      int mode);
+// One top-down level (level index taken from each slot's ctr->level).
+void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                   int mode);
+// One bottom-up level.
+void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                   int mode);
+// Single-workgroup multi-level top-down BFS while the frontier stays small.
+void launch_bfs_small(hipStream_t st, const SlotView* slots, const int* list, int nlist, int mode,
+                      int max_levels);
+
+// Active list (light entries at fa[0..), heavy at fa[n-1..]) and counters.
+void launch_active(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);
+// Discharge of the active list: lanes for light rows, waves for heavy rows.
+void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* list, int nlist,
+                           int gridx);
+// One workgroup per tile, tile heights and pending excess staged in LDS.
+void launch_discharge_tile(hipStream_t st, const SlotView* slots, const int* list, int nlist,
+                           int gridx, int hot_only, int rounds);
+// Owner-exclusive path push from the active list.
+void launch_path_push(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                      int max_hops);
+// Hot tiles: the top `k` tiles by active count get tile_hot = 1.
+void launch_hot_select(hipStream_t st, const SlotView* slots, const int* list, int nlist, int k);
+// RemoveInvalidEdges repair over tiles holding active vertices, band
+// [ctr->hmin - margin, ctr->hmax + margin].
+void launch_rm(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+               int margin);
+// Min-cut certificate over the final residual and heights.
+void launch_cert(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);
+
+// ---------------------------------------------------------------- pruning
+// aux <- (h != n && aux != n) || v == s || v == t   (after FWD into h, UND into aux)
+// ctr->visited <- kept count; hash of the mask into *hash_out (per list entry).
+void launch_mask_combine(hipStream_t st, const SlotView* slots, const int* list, int nlist,
+                         int gridx, unsigned long long* hash_out);
+
+// Dynamic capacity updates on a residual: for each (entry e, new cap c):
+// capacity increase adds to res[e]; a decrease below the current flow moves
+// the excess flow back (excess at the tail, deficit at the head).
+void launch_apply_updates(hipStream_t st, const SlotView* slots, const int* list, int nlist,
+                          const int32_t* entries, const int64_t* old_cap, const int64_t* new_cap,
+                          int nupd);
+
+// cap[entries[i]] <- new_cap[i], old value to old_cap[i].
+void launch_cap_scatter(hipStream_t st, int64_t* cap, const int32_t* entries, const int64_t* new_cap,
+                        int64_t* old_cap, int n);
+
+}  // namespace mf

@@ -1,0 +1,1274 @@
+// mf_kernels.hip -- CDNA4 (gfx950) kernels for exact integer push-relabel.
+//
+// Design notes (the reference kernels are cited as BC:line of
+// Region_Aware_PR_with_Pruning_BoundaryCore.cu):
+//  * Wave64 everywhere: frontier / active-list compaction by ballot + LDS
+//    staging (one global atomic per workgroup), rows longer than kLightDeg
+//    are scanned by a whole wave with coalesced col/res loads.
+//  * All flow arithmetic is int64 with native global 64-bit atomics.  Only
+//    the owner of a vertex (one lane, one wave, or one path-push wave holding
+//    its lock) takes excess or residual out of it; everybody else only adds.
+//    The owner reads excess and residuals with relaxed agent-scope atomic
+//    loads, so what it sees is a lower bound and a push never overdraws.
+//  * Exactness does not rest on heights: termination is decided by an exact
+//    BFS (global relabel) and every answer can be certified by launch_cert.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdint>
+
+#include "mf_kernels.h"
+
+namespace mf {
+
+namespace {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ int64_t ld64(const int64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int64_t add64(int64_t* p, int64_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int64_t add64_lds(int64_t* p, int64_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ int32_t add32(int32_t* p, int32_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t addu64(uint64_t* p, uint64_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld32(const int32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool cas32(int32_t* p, int32_t expect, int32_t want) {
+    return __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED,
+                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
+
+__device__ __forceinline__ int wave_min_i32(int x) {
+    for (int o = WAVE / 2; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, WAVE));
+    return x;
+}
+__device__ __forceinline__ int wave_max_i32(int x) {
+    for (int o = WAVE / 2; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, WAVE));
+    return x;
+}
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t x) {
+    for (int o = WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, WAVE);
+    return x;
+}
+__device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t x) {
+    const int lane = lane_id();
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int64_t y = __shfl_up(x, o, WAVE);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// Block-wide sum of a per-thread value, result valid in thread 0.  `red`
+// must hold blockDim.x / WAVE entries.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* red) {
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    const int w = threadIdx.x / WAVE;
+    if (lane_id() == 0) red[w] = v;
+    __syncthreads();
+    T s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x / WAVE); ++i) s += red[i];
+    __syncthreads();
+    return s;
+}
+
+__device__ __forceinline__ SlotView load_slot(const SlotView* slots, const int* list) {
+    return slots[list[blockIdx.y]];
+}
+
+// LDS-staged append buffer: entries are collected with LDS atomics and
+// flushed with one global atomic per workgroup.  Overflow goes straight to
+// global memory.
+template <int CAP>
+struct Stage {
+    int32_t buf[CAP];
+    int64_t deg_sum;
+    int cnt;
+    int base;
+};
+
+template <int CAP>
+__device__ __forceinline__ void stage_init(Stage<CAP>& st) {
+    if (threadIdx.x == 0) { st.cnt = 0; st.deg_sum = 0; }
+}
+
+template <int CAP>
+__device__ __forceinline__ void stage_push(Stage<CAP>& st, int32_t item, int deg, int32_t* gbuf,
+                                           int32_t* gcnt, int64_t* gdeg) {
+    const int pos = atomicAdd(&st.cnt, 1);
+    if (pos < CAP) {
+        st.buf[pos] = item;
+        add64_lds(&st.deg_sum, (int64_t)deg);
+    } else {
+        const int g = add32(gcnt, 1);
+        gbuf[g] = item;
+        if (gdeg) add64(gdeg, (int64_t)deg);
+    }
+}
+
+template <int CAP>
+__device__ __forceinline__ void stage_flush(Stage<CAP>& st, int32_t* gbuf, int32_t* gcnt,
+                                            int64_t* gdeg) {
+    __syncthreads();
+    const int n = min(st.cnt, CAP);
+    if (threadIdx.x == 0) {
+        st.base = n > 0 ? add32(gcnt, n) : 0;
+        if (gdeg && st.deg_sum) add64(gdeg, st.deg_sum);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) gbuf[st.base + i] = st.buf[i];
+}
+
+struct Work {
+    uint64_t edges = 0, verts = 0, pushes = 0, relabels = 0;
+};
+
+__device__ __forceinline__ void work_flush(const SlotView& S, Work w, uint64_t* red, int fam) {
+    // Four block sums, one atomic each from thread 0.
+    const uint64_t e = block_sum<uint64_t>(w.edges, red);
+    const uint64_t v = block_sum<uint64_t>(w.verts, red);
+    const uint64_t p = block_sum<uint64_t>(w.pushes, red);
+    const uint64_t r = block_sum<uint64_t>(w.relabels, red);
+    if (threadIdx.x == 0) {
+        uint64_t* c = S.ctr->wk[fam];
+        if (e) addu64(&c[0], e);
+        if (v) addu64(&c[1], v);
+        if (p) addu64(&c[2], p);
+        if (r) addu64(&c[3], r);
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ resets
+
+__global__ void k_reset(const SlotView* __restrict__ slots, const int* __restrict__ list,
+                        unsigned what) {
+    const SlotView S = load_slot(slots, list);
+    Ctr* c = S.ctr;
+    if (threadIdx.x == 0) {
+        if (what & RST_BFS) {
+            for (int i = 0; i < 4; ++i) { c->fsize[i] = 0; c->fedges[i] = 0; }
+            c->level = 0;
+            c->visited = 0;
+        }
+        if (what & RST_ACTIVE) {
+            c->active = 0; c->active_high = 0; c->nlight = 0; c->nheavy = 0;
+            c->pos_excess = 0; c->neg_count = 0; c->neg_sum = 0;
+            c->hmin = INT_MAX; c->hmax = -1;
+        }
+        if (what & RST_CERT) { c->cert_bad = 0; c->cut = 0; }
+        if (what & RST_RM) c->fixed_edges = 0;
+        if (what & RST_WORK)
+            for (int f = 0; f < kFamilies; ++f)
+                for (int k = 0; k < 4; ++k) c->wk[f][k] = 0;
+        if (what & RST_ERR) c->err = 0;
+    }
+    if (what & RST_ACTIVE)
+        for (int i = threadIdx.x; i < S.g.ntiles; i += blockDim.x) { S.tile_cnt[i] = 0; S.tile_hot[i] = 0; }
+}
+
+void launch_reset(hipStream_t st, const SlotView* slots, const int* list, int nlist, unsigned what) {
+    hipLaunchKernelGGL(k_reset, dim3(1, nlist), dim3(256), 0, st, slots, list, what);
+}
+
+// -------------------------------------------------------- residual init (a10)
+// BC:151-170: res = cap, excess = 0.  Edge-parallel copy; locks freed.
+
+__global__ __launch_bounds__(256) void k_init(const SlotView* __restrict__ slots,
+                                              const int* __restrict__ list) {
+    const SlotView S = load_slot(slots, list);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t e = i0; e < S.g.E; e += stride) S.res[e] = S.g.cap[e];
+    for (int64_t v = i0; v < S.g.n; v += stride) {
+        S.excess[v] = 0;
+        S.h[v] = 0;
+        S.lock[v] = -1;
+    }
+    if (i0 == 0) {
+        addu64(&S.ctr->wk[WF_INIT][0], (uint64_t)S.g.E);
+        addu64(&S.ctr->wk[WF_INIT][1], (uint64_t)S.g.n);
+    }
+}
+
+void launch_init(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx) {
+    hipLaunchKernelGGL(k_init, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+}
+
+// ---------------------------------------------------------------- preflow (a11)
+// BC:173-200: saturate every entry of s with positive residual.  One
+// workgroup strides the row of s instead of one thread.
+
+__global__ __launch_bounds__(256) void k_preflow(const SlotView* __restrict__ slots,
+                                                 const int* __restrict__ list) {
+    const SlotView S = load_slot(slots, list);
+    __shared__ int64_t red[256 / WAVE];
+    const int s = S.s;
+    const int rb = S.g.row_ptr[s], re = S.g.row_ptr[s + 1];
+    int64_t sum = 0;
+    for (int e = rb + threadIdx.x; e < re; e += blockDim.x) {
+        const int64_t d = ld64(&S.res[e]);
+        if (d > 0) {
+            add64(&S.res[e], -d);
+            add64(&S.res[S.g.rev[e]], d);
+            add64(&S.excess[S.g.col[e]], d);
+            sum += d;
+        }
+    }
+    const int64_t tot = block_sum<int64_t>(sum, red);
+    if (threadIdx.x == 0 && tot) add64(&S.excess[s], -tot);
+}
+
+void launch_preflow(hipStream_t st, const SlotView* slots, const int* list, int nlist) {
+    hipLaunchKernelGGL(k_preflow, dim3(1, nlist), dim3(256), 0, st, slots, list);
+}
+
+// ------------------------------------------------------------- BFS (a12/a13)
+// Level-L frontier lives in fa when L is even, fb when odd; its size in
+// ctr->fsize[L % 4].  Level L writes level L+1 and clears ring slot L+2.
+//
+//   BFS_GR : backwards over residual edges (w joins via entry v->w when
+//            res[rev] > 0), visited marker h == n, s never joins, optional
+//            vertex mask (BC:203-233, BC:322-348).
+//   BFS_FWD: forwards over entries with cap > 0 (pruning, BC:1402-1416);
+//            visited marker h == n.
+//   BFS_UND: over every entry (undirected component, BC:1419-1433);
+//            visited marker aux == n.
+
+template <int MODE>
+__device__ __forceinline__ int32_t* bfs_lvl(const SlotView& S) {
+    return MODE == BFS_UND ? S.aux : S.h;
+}
+
+// May frontier vertex v (entry e: v -> w) discover w?
+template <int MODE>
+__device__ __forceinline__ bool bfs_edge_ok(const SlotView& S, int e, int w) {
+    if (MODE == BFS_GR) {
+        if (w == S.s) return false;
+        if (S.mask && S.mask[w] == 0) return false;
+        return ld64(&S.res[S.g.rev[e]]) > 0;
+    } else if (MODE == BFS_FWD) {
+        return S.g.cap[e] > 0;
+    }
+    return true;
+}
+
+// Bottom-up: may unvisited v join level L+1 through entry e: v -> x where x
+// is on level L?  GR: v can push to x (res[e] > 0).  FWD: x -> v carries
+// capacity, i.e. e is a reverse entry (cap[e] == 0; forward entries always
+// have cap > 0 and reverse entries cap 0).  UND: any entry.
+template <int MODE>
+__device__ __forceinline__ bool bfs_bu_edge_ok(const SlotView& S, int e) {
+    if (MODE == BFS_GR) return ld64(&S.res[e]) > 0;
+    if (MODE == BFS_FWD) return S.g.cap[e] == 0;
+    return true;
+}
+
+template <int MODE>
+__device__ __forceinline__ bool bfs_vertex_ok(const SlotView& S, int v) {
+    if (MODE == BFS_GR) {
+        if (v == S.s) return false;
+        if (S.mask && S.mask[v] == 0) return false;
+    }
+    return true;
+}
+
+// GR seeding (a12): h = n everywhere, level 0 = t plus deficits (phase 0) or
+// deficits only (phase 1, draining from t), under the mask; s stays n.
+__global__ __launch_bounds__(256) void k_gr_seed(const SlotView* __restrict__ slots,
+                                                 const int* __restrict__ list) {
+    const SlotView S = load_slot(slots, list);
+    __shared__ Stage<1024> st;
+    __shared__ uint64_t red[256 / WAVE];
+    stage_init(st);
+    __syncthreads();
+    const int n = S.g.n;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    Work wk;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int v = (int)i;
+        bool seed = false;
+        if (v != S.s && (S.mask == nullptr || S.mask[v] != 0 || v == S.t)) {
+            if (S.phase == 0) seed = (v == S.t) || (S.excess[v] < 0);
+            else seed = (v != S.t) && (S.excess[v] < 0);
+        }
+        S.h[v] = seed ? 0 : n;
+        wk.verts++;
+        if (seed) {
+            const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
+            stage_push(st, v, deg, S.fa, &S.ctr->fsize[0], &S.ctr->fedges[0]);
+            wk.relabels++;
+        }
+    }
+    stage_flush(st, S.fa, &S.ctr->fsize[0], &S.ctr->fedges[0]);
+    const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
+    if (threadIdx.x == 0 && nv) add32(&S.ctr->visited, (int32_t)nv);
+    work_flush(S, wk, red, WF_BFS);
+}
+
+void launch_gr_seed(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx) {
+    hipLaunchKernelGGL(k_gr_seed, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+}
+
+// Single-source seeding for the pruning BFS: lvl = n, lvl[src] = 0.
+// FWD starts at s, UND at t.
+__global__ __launch_bounds__(256) void k_bfs_seed1(const SlotView* __restrict__ slots,
+                                                   const int* __restrict__ list, int mode) {
+    const SlotView S = load_slot(slots, list);
+    int32_t* lvl = (mode == BFS_UND) ? S.aux : S.h;
+    const int src = (mode == BFS_UND) ? S.t : S.s;
+    const int n = S.g.n;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        lvl[i] = (i == src) ? 0 : n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S.fa[0] = src;
+        S.ctr->fsize[0] = 1;
+        S.ctr->fedges[0] = S.g.row_ptr[src + 1] - S.g.row_ptr[src];
+        S.ctr->visited = 1;
+    }
+}
+
+void launch_bfs_seed1(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                      int mode) {
+    hipLaunchKernelGGL(k_bfs_seed1, dim3(gridx, nlist), dim3(256), 0, st, slots, list, mode);
+}
+
+// Top-down expansion of one frontier vertex by one lane.
+template <int MODE, int CAP>
+__device__ __forceinline__ void td_lane(const SlotView& S, int v, int L, int32_t* lvl,
+                                        Stage<CAP>& st, int32_t* out, int32_t* ocnt,
+                                        int64_t* odeg, Work& wk) {
+    const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+    const int n = S.g.n;
+    wk.edges += (uint64_t)(re - rb);
+    for (int e = rb; e < re; ++e) {
+        const int w = S.g.col[e];
+        if (lvl[w] != n) continue;
+        if (!bfs_edge_ok<MODE>(S, e, w)) continue;
+        if (cas32(&lvl[w], n, L + 1)) {
+            const int dw = S.g.row_ptr[w + 1] - S.g.row_ptr[w];
+            stage_push(st, w, dw, out, ocnt, odeg);
+            wk.relabels++;
+        }
+    }
+}
+
+// Top-down expansion of one (heavy) frontier vertex by the whole workgroup.
+template <int MODE, int CAP>
+__device__ __forceinline__ void td_block(const SlotView& S, int v, int L, int32_t* lvl,
+                                         Stage<CAP>& st, int32_t* out, int32_t* ocnt,
+                                         int64_t* odeg, Work& wk) {
+    const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+    const int n = S.g.n;
+    if (threadIdx.x == 0) wk.edges += (uint64_t)(re - rb);
+    for (int e = rb + (int)threadIdx.x; e < re; e += blockDim.x) {
+        const int w = S.g.col[e];
+        if (lvl[w] != n) continue;
+        if (!bfs_edge_ok<MODE>(S, e, w)) continue;
+        if (cas32(&lvl[w], n, L + 1)) {
+            const int dw = S.g.row_ptr[w + 1] - S.g.row_ptr[w];
+            stage_push(st, w, dw, out, ocnt, odeg);
+            wk.relabels++;
+        }
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bfs_td(const SlotView* __restrict__ slots,
+                                                const int* __restrict__ list) {
+    const SlotView S = load_slot(slots, list);
+    __shared__ Stage<2048> st;
+    __shared__ int heavy[256];
+    __shared__ int nheavy;
+    __shared__ uint64_t red[256 / WAVE];
+    const int L = S.ctr->level;
+    const int fsz = S.ctr->fsize[L & 3];
+    if ((int)(blockIdx.x * blockDim.x) >= fsz) return;  // uniform per block
+    int32_t* lvl = bfs_lvl<MODE>(S);
+    const int32_t* in = (L & 1) ? S.fb : S.fa;
+    int32_t* out = (L & 1) ? S.fa : S.fb;
+    int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
+    int64_t* odeg = &S.ctr->fedges[(L + 1) & 3];
+    if (threadIdx.x == 0) nheavy = 0;
+    stage_init(st);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S.ctr->fsize[(L + 2) & 3] = 0;
+        S.ctr->fedges[(L + 2) & 3] = 0;
+    }
+    __syncthreads();
+    Work wk;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < fsz) {
+        const int v = in[i];
+        wk.verts++;
+        const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
+        if (deg <= kLightDeg) td_lane<MODE>(S, v, L, lvl, st, out, ocnt, odeg, wk);
+        else heavy[atomicAdd(&nheavy, 1)] = v;
+    }
+    __syncthreads();
+    // heavy rows: the whole workgroup strides one row after another
+    const int nh = nheavy;
+    for (int k = 0; k < nh; ++k)
+        td_block<MODE>(S, heavy[k], L, lvl, st, out, ocnt, odeg, wk);
+    stage_flush(st, out, ocnt, odeg);
+    const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
+    if (threadIdx.x == 0 && nv) add32(&S.ctr->visited, (int32_t)nv);
+    work_flush(S, wk, red, WF_BFS);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slots,
+                                                const int* __restrict__ list) {
+    const SlotView S = load_slot(slots, list);
+    __shared__ Stage<2048> st;
+    __shared__ uint64_t red[256 / WAVE];
+    const int L = S.ctr->level;
+    int32_t* lvl = bfs_lvl<MODE>(S);
+    int32_t* out = (L & 1) ? S.fa : S.fb;
+    int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
+    int64_t* odeg = &S.ctr->fedges[(L + 1) & 3];
+    stage_init(st);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S.ctr->fsize[(L + 2) & 3] = 0;
+        S.ctr->fedges[(L + 2) & 3] = 0;
+    }
+    __syncthreads();
+    const int n = S.g.n;
+    Work wk;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int v = (int)i;
+        if (lvl[v] != n) continue;
+        if (!bfs_vertex_ok<MODE>(S, v)) continue;
+        wk.verts++;
+        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+        int e = rb;
+        for (; e < re; ++e) {
+            if (lvl[S.g.col[e]] != L) continue;
+            if (bfs_bu_edge_ok<MODE>(S, e)) break;
+        }
+        wk.edges += (uint64_t)(min(e + 1, re) - rb);
+        if (e < re) {
+            lvl[v] = L + 1;  // only this lane writes v
+            stage_push(st, v, re - rb, out, ocnt, odeg);
+            wk.relabels++;
+        }
+    }
+    stage_flush(st, out, ocnt, odeg);
+    const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
+    if (threadIdx.x == 0 && nv) add32(&S.ctr->visited, (int32_t)nv);
+    work_flush(S, wk, red, WF_BFS);
+}
+
+// Level bookkeeping after a multi-workgroup level: level += 1.
+__global__ void k_bfs_advance(const SlotView* __restrict__ slots, const int* __restrict__ list) {
+    const SlotView S = load_slot(slots, list);
+    if (threadIdx.x == 0) S.ctr->level = S.ctr->level + 1;
+}
+
+void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                   int mode) {
+    if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_td<BFS_GR>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_td<BFS_FWD>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    else hipLaunchKernelGGL(k_bfs_td<BFS_UND>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_bfs_advance, dim3(1, nlist), dim3(64), 0, st, slots, list);
+}
+
+void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                   int mode) {
+    if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_bu<BFS_GR>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_bu<BFS_FWD>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    else hipLaunchKernelGGL(k_bfs_bu<BFS_UND>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_bfs_advance, dim3(1, nlist), dim3(64), 0, st, slots, list);
+}
+
+// Single-workgroup BFS: runs levels back to back with only workgroup
+// barriers while the frontier holds at most kSmallFrontier vertices and at
+// most max_levels levels.  This is the deep-and-narrow case (grids), where a
+// launch per level would cost more than the level itself.
+template <int MODE>
+__global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__ slots,
+                                                    const int* __restrict__ list, int max_levels) {
+    const SlotView S = load_slot(slots, list);
+    __shared__ int heavy[1024];
+    __shared__ int nheavy;
+    __shared__ int next_cnt;
+    __shared__ int64_t next_deg;
+    __shared__ uint64_t red[1024 / WAVE];
+    int32_t* lvl = bfs_lvl<MODE>(S);
+    const int n = S.g.n;
+    int L = S.ctr->level;
+    int fsz = S.ctr->fsize[L & 3];
+    Work wk;
+    int done_levels = 0;
+    int64_t fdeg = S.ctr->fedges[L & 3];
+    while (fsz > 0 && fsz <= kSmallFrontier && fdeg <= kSmallEdges && done_levels < max_levels) {
+        const int32_t* in = (L & 1) ? S.fb : S.fa;
+        int32_t* out = (L & 1) ? S.fa : S.fb;
+        if (threadIdx.x == 0) { nheavy = 0; next_cnt = 0; next_deg = 0; }
+        __syncthreads();
+        for (int i = threadIdx.x; i < fsz; i += blockDim.x) {
+            const int v = in[i];
+            wk.verts++;
+            const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+            if (re - rb > kLightDeg) {
+                const int k = atomicAdd(&nheavy, 1);
+                if (k < 1024) { heavy[k] = v; continue; }
+            }
+            wk.edges += (uint64_t)(re - rb);
+            for (int e = rb; e < re; ++e) {
+                const int w = S.g.col[e];
+                if (lvl[w] != n) continue;
+                if (!bfs_edge_ok<MODE>(S, e, w)) continue;
+                if (cas32(&lvl[w], n, L + 1)) {
+                    const int p = atomicAdd(&next_cnt, 1);
+                    out[p] = w;
+                    add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
+                    wk.relabels++;
+                }
+            }
+        }
+        __syncthreads();
+        const int nh = min(nheavy, 1024);
+        for (int k = 0; k < nh; ++k) {
+            const int v = heavy[k];
+            const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+            if (threadIdx.x == 0) wk.edges += (uint64_t)(re - rb);
+            for (int e = rb + (int)threadIdx.x; e < re; e += blockDim.x) {
+                const int w = S.g.col[e];
+                if (lvl[w] != n) continue;
+                if (!bfs_edge_ok<MODE>(S, e, w)) continue;
+                if (cas32(&lvl[w], n, L + 1)) {
+                    const int p = atomicAdd(&next_cnt, 1);
+                    out[p] = w;
+                    add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
+                    wk.relabels++;
+                }
+            }
+        }
+        __syncthreads();
+        fsz = next_cnt;
+        fdeg = next_deg;
+        ++L;
+        ++done_levels;
+        if (threadIdx.x == 0) {
+            S.ctr->fsize[L & 3] = fsz;
+            S.ctr->fedges[L & 3] = next_deg;
+            // the next level (run here or by a multi-workgroup launch) appends
+            // into ring slot L+1, which must start empty
+            S.ctr->fsize[(L + 1) & 3] = 0;
+            S.ctr->fedges[(L + 1) & 3] = 0;
+            S.ctr->visited = S.ctr->visited + fsz;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) S.ctr->level = L;
+    work_flush(S, wk, red, WF_BFS);
+}
+
+void launch_bfs_small(hipStream_t st, const SlotView* slots, const int* list, int nlist, int mode,
+                      int max_levels) {
+    if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_small<BFS_GR>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
+    else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_small<BFS_FWD>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
+    else hipLaunchKernelGGL(k_bfs_small<BFS_UND>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
+}
+
+// ------------------------------------------------ active list + counters (a15, a18)
+// BC:664-695 (active list, counts), BC:559-573 (positive excess sum),
+// BC:283-318 (per-region active counts), BC:519-536 (active height band).
+// Active: excess > 0, h < n, v != s and (phase 0) v != t.  Light rows go to
+// fa[0..nlight), heavy rows to fa[n-1], fa[n-2], ...
+
+__global__ __launch_bounds__(256) void k_active(const SlotView* __restrict__ slots,
+                                                const int* __restrict__ list) {
+    const SlotView S = load_slot(slots, list);
+    __shared__ Stage<1024> st;
+    __shared__ int64_t red64[256 / WAVE];
+    __shared__ uint64_t redu[256 / WAVE];
+    __shared__ int bmin, bmax;
+    stage_init(st);
+    if (threadIdx.x == 0) { bmin = INT_MAX; bmax = -1; }
+    __syncthreads();
+    const int n = S.g.n;
+    int64_t pos = 0, negs = 0;
+    uint64_t act = 0, high = 0, negc = 0, nv = 0;
+    int hmin = INT_MAX, hmax = -1;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int v = (int)i;
+        ++nv;
+        if (v == S.s) continue;
+        const int64_t ex = S.excess[v];
+        if (v == S.t) S.ctr->ex_t = ex;
+        if (v == S.t && S.phase == 0) continue;
+        if (ex > 0) pos += ex;
+        if (ex < 0) { negs += ex; ++negc; }
+        if (ex <= 0) continue;
+        const int hv = S.h[v];
+        if (hv >= n) continue;
+        ++act;
+        if (hv > S.H_cut) ++high;
+        hmin = min(hmin, hv);
+        hmax = max(hmax, hv);
+        atomicAdd(&S.tile_cnt[S.g.tile_of[v]], 1);
+        const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
+        if (deg <= kLightDeg) {
+            stage_push(st, v, 0, S.fa, &S.ctr->nlight, (int64_t*)nullptr);
+        } else {
+            const int k = add32(&S.ctr->nheavy, 1);
+            S.fa[n - 1 - k] = v;
+        }
+    }
+    stage_flush(st, S.fa, &S.ctr->nlight, (int64_t*)nullptr);
+    hmin = wave_min_i32(hmin);
+    hmax = wave_max_i32(hmax);
+    if (lane_id() == 0) { atomicMin(&bmin, hmin); atomicMax(&bmax, hmax); }
+    const int64_t bpos = block_sum<int64_t>(pos, red64);
+    const int64_t bneg = block_sum<int64_t>(negs, red64);
+    const uint64_t bact = block_sum<uint64_t>(act, redu);
+    const uint64_t bhigh = block_sum<uint64_t>(high, redu);
+    const uint64_t bnegc = block_sum<uint64_t>(negc, redu);
+    const uint64_t bnv = block_sum<uint64_t>(nv, redu);
+    if (threadIdx.x == 0) {
+        if (bpos) add64(&S.ctr->pos_excess, bpos);
+        if (bneg) add64(&S.ctr->neg_sum, bneg);
+        if (bact) add32(&S.ctr->active, (int32_t)bact);
+        if (bhigh) add32(&S.ctr->active_high, (int32_t)bhigh);
+        if (bnegc) add32(&S.ctr->neg_count, (int32_t)bnegc);
+        if (bnv) addu64(&S.ctr->wk[WF_ACTIVE][1], bnv);
+        if (bmax >= 0) { atomicMin(&S.ctr->hmin, bmin); atomicMax(&S.ctr->hmax, bmax); }
+    }
+}
+
+void launch_active(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx) {
+    hipLaunchKernelGGL(k_active, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+}
+
+// ------------------------------------------------------------- discharge (a16)
+// One discharge cycle of the reference (BC:467-511): find the lowest
+// neighbour over residual entries; if it is lower than v, push; else
+// relabel to lowest + 1.  The push sends min(excess, res) to every entry at
+// that lowest height in row order until the excess is gone, which is a
+// sequence of the reference's single pushes done in one scan.
+
+struct TileCtx {
+    int a = 0, b = 0;          // staged tile [a, b); b == a means list mode
+    int32_t* lh = nullptr;     // LDS heights of the tile
+    int64_t* ldx = nullptr;    // LDS pending excess deltas of the tile
+};
+
+__device__ __forceinline__ int nbr_h(const SlotView& S, const TileCtx& tc, int w) {
+    return (w >= tc.a && w < tc.b) ? tc.lh[w - tc.a] : S.h[w];
+}
+
+__device__ __forceinline__ void push_into(const SlotView& S, const TileCtx& tc, int w, int64_t d) {
+    if (w >= tc.a && w < tc.b) add64_lds(&tc.ldx[w - tc.a], d);
+    else add64(&S.excess[w], d);
+}
+
+// Owner view of v's excess.
+__device__ __forceinline__ int64_t own_excess(const SlotView& S, const TileCtx& tc, int v) {
+    int64_t ex = ld64(&S.excess[v]);
+    if (v >= tc.a && v < tc.b) ex += tc.ldx[v - tc.a];
+    return ex;
+}
+
+// Take tot out of v; returns the owner's new view.
+__device__ __forceinline__ int64_t take_excess(const SlotView& S, const TileCtx& tc, int v,
+                                               int64_t ex_view, int64_t tot) {
+    if (v >= tc.a && v < tc.b) {
+        add64_lds(&tc.ldx[v - tc.a], -tot);
+        return ex_view - tot;
+    }
+    return add64(&S.excess[v], -tot) - tot;
+}
+
+__device__ __forceinline__ int own_h(const SlotView& S, const TileCtx& tc, int v) {
+    return (v >= tc.a && v < tc.b) ? tc.lh[v - tc.a] : S.h[v];
+}
+
+__device__ __forceinline__ void set_h(const SlotView& S, const TileCtx& tc, int v, int hv) {
+    if (v >= tc.a && v < tc.b) tc.lh[v - tc.a] = hv;
+    else S.h[v] = hv;
+}
+
+// Light row: one lane, `cycles` discharge cycles.
+__device__ void lane_discharge(const SlotView& S, const TileCtx& tc, int v, int cycles, Work& wk) {
+    const int n = S.g.n;
+    const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+    int hv = own_h(S, tc, v);
+    int64_t ex = own_excess(S, tc, v);
+    for (int c = 0; c < cycles; ++c) {
+        if (ex <= 0 || hv >= n) break;
+        int hmin = INT_MAX;
+        for (int e = rb; e < re; ++e) {
+            if (ld64(&S.res[e]) > 0) hmin = min(hmin, nbr_h(S, tc, S.g.col[e]));
+        }
+        wk.edges += (uint64_t)(re - rb);
+        if (hmin >= n) {  // nothing below n is reachable from v
+            hv = n;
+            set_h(S, tc, v, hv);
+            wk.relabels++;
+            break;
+        }
+        if (hv > hmin) {
+            int64_t rem = ex;
+            for (int e = rb; e < re && rem > 0; ++e) {
+                const int64_t r = ld64(&S.res[e]);
+                if (r <= 0) continue;
+                const int w = S.g.col[e];
+                if (nbr_h(S, tc, w) != hmin) continue;
+                const int64_t d = rem < r ? rem : r;
+                add64(&S.res[e], -d);
+                add64(&S.res[S.g.rev[e]], d);
+                push_into(S, tc, w, d);
+                rem -= d;
+                wk.pushes++;
+            }
+            const int64_t tot = ex - rem;
+            if (tot > 0) ex = take_excess(S, tc, v, ex, tot);
+        } else {
+            hv = hmin + 1;
+            set_h(S, tc, v, hv);
+            wk.relabels++;
+        }
+    }
+}
+
+// Heavy row: one wave, `cycles` discharge cycles; wave-uniform control.
+__device__ void wave_discharge(const SlotView& S, const TileCtx& tc, int v, int cycles, Work& wk) {
+    const int n = S.g.n;
+    const int lane = lane_id();
+    const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+    int hv = __shfl(own_h(S, tc, v), 0, WAVE);
+    int64_t ex = __shfl(own_excess(S, tc, v), 0, WAVE);
+    for (int c = 0; c < cycles; ++c) {
+        if (ex <= 0 || hv >= n) break;
+        int hmin = INT_MAX;
+        for (int e = rb + lane; e < re; e += WAVE)
+            if (ld64(&S.res[e]) > 0) hmin = min(hmin, nbr_h(S, tc, S.g.col[e]));
+        hmin = wave_min_i32(hmin);
+        if (lane == 0) wk.edges += (uint64_t)(re - rb);
+        if (hmin >= n) {
+            hv = n;
+            if (lane == 0) { set_h(S, tc, v, hv); wk.relabels++; }
+            break;
+        }
+        if (hv > hmin) {
+            int64_t rem = ex;
+            for (int base = rb; base < re && rem > 0; base += WAVE) {
+                const int e = base + lane;
+                int64_t r = 0;
+                if (e < re) {
+                    r = ld64(&S.res[e]);
+                    if (r > 0 && nbr_h(S, tc, S.g.col[e]) != hmin) r = 0;
+                    if (r < 0) r = 0;
+                }
+                const int64_t incl = wave_incl_scan_i64(r);
+                const int64_t excl = incl - r;
+                const int64_t d = (r > 0 && rem > excl) ? ((rem - excl) < r ? (rem - excl) : r) : 0;
+                if (d > 0) {
+                    add64(&S.res[e], -d);
+                    add64(&S.res[S.g.rev[e]], d);
+                    push_into(S, tc, S.g.col[e], d);
+                    wk.pushes++;
+                }
+                const int64_t chunk = __shfl(incl, WAVE - 1, WAVE);
+                rem -= (chunk < rem ? chunk : rem);
+            }
+            const int64_t tot = ex - rem;
+            if (tot > 0) {
+                int64_t nex = 0;
+                if (lane == 0) nex = take_excess(S, tc, v, ex, tot);
+                ex = __shfl(nex, 0, WAVE);
+            }
+        } else {
+            hv = hmin + 1;
+            if (lane == 0) { set_h(S, tc, v, hv); wk.relabels++; }
+        }
+    }
+}
+
+// Active-list discharge: light entries one lane each, heavy entries one
+// wave each.  Blocks [0, light_blocks) take light entries.
+__global__ __launch_bounds__(256) void k_discharge_list(const SlotView* __restrict__ slots,
+                                                        const int* __restrict__ list) {
+    const SlotView S = load_slot(slots, list);
+    __shared__ uint64_t red[256 / WAVE];
+    const int nl = S.ctr->nlight, nh = S.ctr->nheavy;
+    const int lb = (nl + 255) / 256;
+    const int hb = (nh + 3) / 4;
+    if ((int)blockIdx.x >= lb + hb) return;
+    TileCtx tc;
+    Work wk;
+    if ((int)blockIdx.x < lb) {
+        const int i = blockIdx.x * 256 + threadIdx.x;
+        if (i < nl) lane_discharge(S, tc, S.fa[i], S.K, wk);
+    } else {
+        const int k = ((int)blockIdx.x - lb) * 4 + (int)(threadIdx.x / WAVE);
+        if (k < nh) wave_discharge(S, tc, S.fa[S.g.n - 1 - k], S.K, wk);
+    }
+    work_flush(S, wk, red, WF_DISCHARGE);
+}
+
+void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* list, int nlist,
+                           int gridx) {
+    hipLaunchKernelGGL(k_discharge_list, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+}
+
+// Region (tile) discharge (BC:432-515, one block per region): the tile's
+// heights and pending excess live in LDS for the whole launch; pushes to
+// vertices of the same tile are LDS atomics, other pushes global atomics.
+// `rounds` rebuilds of the tile's active list, one cycle per vertex each.
+__global__ __launch_bounds__(512) void k_discharge_tile(const SlotView* __restrict__ slots,
+                                                        const int* __restrict__ list, int hot_only,
+                                                        int rounds) {
+    const SlotView S = load_slot(slots, list);
+    const int tile = blockIdx.x;
+    if (tile >= S.g.ntiles) return;
+    if (hot_only && S.tile_hot[tile] == 0) return;
+    __shared__ int32_t lh[kTileMax];
+    __shared__ int64_t ldx[kTileMax];
+    __shared__ int32_t alist[kTileMax];
+    __shared__ int nlight, nheavy;
+    __shared__ uint64_t red[512 / WAVE];
+    const int a = S.g.tile_start[tile], b = S.g.tile_start[tile + 1];
+    const int size = b - a;  // <= kTileMax by construction of tile_start
+    const int n = S.g.n;
+    for (int i = threadIdx.x; i < size; i += blockDim.x) { lh[i] = S.h[a + i]; ldx[i] = 0; }
+    TileCtx tc;
+    tc.a = a; tc.b = b; tc.lh = lh; tc.ldx = ldx;
+    Work wk;
+    for (int r = 0; r < rounds; ++r) {
+        if (threadIdx.x == 0) { nlight = 0; nheavy = 0; }
+        __syncthreads();
+        for (int i = threadIdx.x; i < size; i += blockDim.x) {
+            const int v = a + i;
+            if (v == S.s) continue;
+            if (v == S.t && S.phase == 0) continue;
+            if (lh[i] >= n) continue;
+            const int64_t ex = ld64(&S.excess[v]) + ldx[i];
+            if (ex <= 0) continue;
+            const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
+            if (deg <= kLightDeg) alist[atomicAdd(&nlight, 1)] = i;
+            else alist[size - 1 - atomicAdd(&nheavy, 1)] = i;
+        }
+        if (threadIdx.x == 0) wk.verts += (uint64_t)size;
+        __syncthreads();
+        const int nl = nlight, nh = nheavy;
+        if (nl + nh == 0) break;
+        for (int k = threadIdx.x; k < nl; k += blockDim.x) lane_discharge(S, tc, a + alist[k], 1, wk);
+        for (int k = threadIdx.x / WAVE; k < nh; k += blockDim.x / WAVE)
+            wave_discharge(S, tc, a + alist[size - 1 - k], 1, wk);
+        __syncthreads();
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < size; i += blockDim.x) {
+        S.h[a + i] = lh[i];
+        if (ldx[i] != 0) add64(&S.excess[a + i], ldx[i]);
+    }
+    work_flush(S, wk, red, WF_DISCHARGE);
+}
+
+void launch_discharge_tile(hipStream_t st, const SlotView* slots, const int* list, int nlist,
+                           int gridx, int hot_only, int rounds) {
+    hipLaunchKernelGGL(k_discharge_tile, dim3(gridx, nlist), dim3(512), 0, st, slots, list, hot_only,
+                       rounds);
+}
+
+// ------------------------------------------------------------ path push (a17)
+// BC:773-955, made owner-exclusive: a wave walks from an active vertex down
+// admissible entries (h[w] == h[cur] - 1), holding lock[cur] while it takes
+// excess out of cur, and hands the lock over hop by hop.  A dead end
+// relabels cur to lowest residual neighbour + 1 (n when none is below n-1).
+
+__global__ __launch_bounds__(256) void k_path_push(const SlotView* __restrict__ slots,
+                                                   const int* __restrict__ list, int max_hops) {
+    const SlotView S = load_slot(slots, list);
+    __shared__ uint64_t red[256 / WAVE];
+    const int nl = S.ctr->nlight, nh = S.ctr->nheavy;
+    const int k = blockIdx.x * 4 + (int)(threadIdx.x / WAVE);
+    const int lane = lane_id();
+    const int n = S.g.n;
+    Work wk;
+    if (k < nl + nh) {
+        const int v = (k < nl) ? S.fa[k] : S.fa[n - 1 - (k - nl)];
+        const int tag = v;
+        int got = 0;
+        if (lane == 0) got = cas32(&S.lock[v], -1, tag) ? 1 : 0;
+        got = __shfl(got, 0, WAVE);
+        if (got) {
+            int cur = v;
+            int hknown = -1;  // height this wave last wrote for cur (lane 0 stores it)
+            for (int hop = 0; hop < max_hops; ++hop) {
+                int64_t ex = 0;
+                if (lane == 0) ex = ld64(&S.excess[cur]);
+                ex = __shfl(ex, 0, WAVE);
+                if (ex <= 0) break;
+                if (cur == S.t && S.phase == 0) break;
+                int hc = hknown;
+                if (hc < 0) hc = __shfl(S.h[cur], 0, WAVE);
+                if (hc >= n) break;
+                const int rb = S.g.row_ptr[cur], re = S.g.row_ptr[cur + 1];
+                if (lane == 0) wk.edges += (uint64_t)(re - rb);
+                int best = -1;
+                for (int base = rb; base < re; base += WAVE) {
+                    const int e = base + lane;
+                    bool ok = false;
+                    if (e < re) ok = (ld64(&S.res[e]) > 0) && (S.h[S.g.col[e]] == hc - 1);
+                    const unsigned long long m = __ballot(ok);
+                    if (m) { best = base + (int)__ffsll((long long)m) - 1; break; }
+                }
+                if (best >= 0) {
+                    const int w = S.g.col[best];
+                    int moved = 0;
+                    if (lane == 0) {
+                        const int64_t r = ld64(&S.res[best]);
+                        const int64_t d = ex < r ? ex : r;
+                        if (d > 0) {
+                            add64(&S.res[best], -d);
+                            add64(&S.res[S.g.rev[best]], d);
+                            add64(&S.excess[w], d);
+                            add64(&S.excess[cur], -d);
+                            wk.pushes++;
+                        }
+                        // hand the lock to w unless w is a terminal or taken
+                        if (w != S.s && !(w == S.t && S.phase == 0) && cas32(&S.lock[w], -1, tag)) {
+                            __hip_atomic_store(&S.lock[cur], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            moved = 1;
+                        }
+                    }
+                    moved = __shfl(moved, 0, WAVE);
+                    if (!moved) break;
+                    cur = w;
+                    hknown = -1;
+                } else {
+                    int hmin = INT_MAX;
+                    for (int e = rb + lane; e < re; e += WAVE)
+                        if (ld64(&S.res[e]) > 0) hmin = min(hmin, S.h[S.g.col[e]]);
+                    hmin = wave_min_i32(hmin);
+                    const int hnew = (hmin == INT_MAX || hmin >= n - 1) ? n : hmin + 1;
+                    if (lane == 0) {
+                        S.h[cur] = hnew;
+                        wk.relabels++;
+                    }
+                    hknown = hnew;
+                    if (hnew >= n) break;
+                }
+            }
+            if (lane == 0) __hip_atomic_store(&S.lock[cur], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    work_flush(S, wk, red, WF_DISCHARGE);
+}
+
+void launch_path_push(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                      int max_hops) {
+    hipLaunchKernelGGL(k_path_push, dim3(gridx, nlist), dim3(256), 0, st, slots, list, max_hops);
+}
+
+// ------------------------------------------------------------- hot tiles (a19)
+// BC:1899-1946: the top-k regions by active count (ties by lower index, only
+// regions with at least one active vertex).  One workgroup per slot: each
+// tile's rank is the number of tiles that beat it.
+
+__global__ __launch_bounds__(1024) void k_hot_select(const SlotView* __restrict__ slots,
+                                                     const int* __restrict__ list, int k) {
+    const SlotView S = load_slot(slots, list);
+    const int T = S.g.ntiles;
+    // Threshold search: count tiles with cnt > x by bisection on x.
+    __shared__ int cnt_gt;
+    __shared__ int lo_s, hi_s;
+    if (threadIdx.x == 0) { lo_s = 0; hi_s = S.g.n; }
+    __syncthreads();
+    // smallest x with |{cnt > x}| < k  ->  tiles with cnt > x are hot, ties at
+    // cnt == x filled by lowest index.
+    while (true) {
+        const int lo = lo_s, hi = hi_s;
+        if (lo >= hi) break;
+        const int mid = lo + (hi - lo) / 2;
+        if (threadIdx.x == 0) cnt_gt = 0;
+        __syncthreads();
+        int c = 0;
+        for (int i = threadIdx.x; i < T; i += blockDim.x) c += (S.tile_cnt[i] > mid) ? 1 : 0;
+        atomicAdd(&cnt_gt, c);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (cnt_gt < k) hi_s = mid; else lo_s = mid + 1;
+        }
+        __syncthreads();
+    }
+    const int x = lo_s;
+    __shared__ int above, ties;
+    if (threadIdx.x == 0) { above = 0; ties = 0; }
+    __syncthreads();
+    int c = 0;
+    for (int i = threadIdx.x; i < T; i += blockDim.x) c += (S.tile_cnt[i] > x) ? 1 : 0;
+    atomicAdd(&above, c);
+    __syncthreads();
+    const int room = k - above;
+    // Ties at cnt == x (x >= 1): lowest indices first; serial over chunks keeps order.
+    for (int base = 0; base < T; base += blockDim.x) {
+        const int i = base + threadIdx.x;
+        int hot = 0;
+        bool tie = false;
+        if (i < T) {
+            const int ci = S.tile_cnt[i];
+            hot = (ci > x) ? 1 : 0;
+            tie = (ci == x && x >= 1);
+        }
+        // rank ties within the chunk in index order
+        __shared__ int tie_base;
+        __shared__ int wave_ties[1024 / WAVE];
+        const unsigned long long m = __ballot(tie);
+        const int w = threadIdx.x / WAVE;
+        if (lane_id() == 0) wave_ties[w] = __popcll(m);
+        if (threadIdx.x == 0) tie_base = ties;
+        __syncthreads();
+        int before = tie_base;
+        for (int j = 0; j < w; ++j) before += wave_ties[j];
+        before += __popcll(m & ((1ull << lane_id()) - 1ull));
+        if (tie && before < room) hot = 1;
+        if (i < T) S.tile_hot[i] = hot;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int j = 0; j < (int)(blockDim.x / WAVE); ++j) tot += wave_ties[j];
+            ties = tie_base + tot;
+        }
+        __syncthreads();
+    }
+}
+
+void launch_hot_select(hipStream_t st, const SlotView* slots, const int* list, int nlist, int k) {
+    hipLaunchKernelGGL(k_hot_select, dim3(1, nlist), dim3(1024), 0, st, slots, list, k);
+}
+
+// ---------------------------------------------------------------- RM (a20)
+// BC:605-657: for v in a tile holding active vertices, with h(v) in the band
+// and v not a terminal, every entry v->w with h(w) > h(v)+1 and excess(w) <= 0
+// has its residual w->v pushed from w to v (w may go into deficit).  The
+// condition is exclusive per entry pair, so each pair has one writer.
+
+__global__ __launch_bounds__(256) void k_rm(const SlotView* __restrict__ slots,
+                                            const int* __restrict__ list, int margin) {
+    const SlotView S = load_slot(slots, list);
+    __shared__ uint64_t red[256 / WAVE];
+    const int tile = blockIdx.x;
+    if (tile >= S.g.ntiles) return;
+    if (S.tile_cnt[tile] == 0) return;
+    const int n = S.g.n;
+    const int hmin = S.ctr->hmin, hmax = S.ctr->hmax;
+    int lo = 0, hi = n - 1;
+    if (hmax >= 0 && hmin <= hmax) { lo = max(0, hmin - margin); hi = min(n - 1, hmax + margin); }
+    const int a = S.g.tile_start[tile], b = S.g.tile_start[tile + 1];
+    uint64_t fixed = 0;
+    Work wk;
+    for (int v = a + threadIdx.x; v < b; v += blockDim.x) {
+        const int hv = S.h[v];
+        if (hv < lo || hv > hi) continue;
+        if (v == S.s || v == S.t) continue;
+        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+        wk.edges += (uint64_t)(re - rb);
+        int64_t gained = 0;
+        for (int e = rb; e < re; ++e) {
+            const int w = S.g.col[e];
+            if (S.h[w] <= hv + 1) continue;
+            if (ld64(&S.excess[w]) > 0) continue;
+            const int re_ = S.g.rev[e];
+            const int64_t d = ld64(&S.res[re_]);
+            if (d <= 0) continue;
+            add64(&S.res[re_], -d);
+            add64(&S.res[e], d);
+            add64(&S.excess[w], -d);
+            gained += d;
+            ++fixed;
+            wk.pushes++;
+        }
+        if (gained) add64(&S.excess[v], gained);
+    }
+    const uint64_t bf = block_sum<uint64_t>(fixed, red);
+    if (threadIdx.x == 0 && bf) add32(&S.ctr->fixed_edges, (int32_t)bf);
+    work_flush(S, wk, red, WF_RM);
+}
+
+void launch_rm(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+               int margin) {
+    hipLaunchKernelGGL(k_rm, dim3(gridx, nlist), dim3(256), 0, st, slots, list, margin);
+}
+
+// ------------------------------------------------------------ certificate
+// T = {v : h(v) < n} from the final exact BFS.  Violations counted: negative
+// residual, pair sum != cap pair sum, stored excess != excess recomputed from
+// residuals, deficit off s, excess inside T off t, t outside T, s inside T,
+// a residual entry from outside T into T.  cut = sum of cap over entries
+// from outside T into T.
+
+__global__ __launch_bounds__(256) void k_cert(const SlotView* __restrict__ slots,
+                                              const int* __restrict__ list) {
+    const SlotView S = load_slot(slots, list);
+    __shared__ int64_t red64[256 / WAVE];
+    __shared__ uint64_t redu[256 / WAVE];
+    const int n = S.g.n;
+    int64_t cut = 0;
+    uint64_t bad = 0, edges = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int v = (int)i;
+        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+        const bool inT = S.h[v] < n;
+        int64_t ex = 0;
+        edges += (uint64_t)(re - rb);
+        for (int e = rb; e < re; ++e) {
+            const int64_t r = S.res[e];
+            const int rv = S.g.rev[e];
+            if (r < 0) ++bad;
+            if (r + S.res[rv] != S.g.cap[e] + S.g.cap[rv]) ++bad;
+            ex += r - S.g.cap[e];
+            if (!inT) {
+                const int w = S.g.col[e];
+                if (S.h[w] < n) {
+                    cut += S.g.cap[e];
+                    if (r > 0) ++bad;
+                }
+            }
+        }
+        if (ex != S.excess[v]) ++bad;
+        if (v != S.s && ex < 0) ++bad;
+        if (inT && v != S.t && ex != 0) ++bad;
+        if (v == S.t) S.ctr->ex_t = S.excess[v];
+        if (v == S.t && !inT) ++bad;
+        if (v == S.s && inT) ++bad;
+    }
+    const int64_t bc = block_sum<int64_t>(cut, red64);
+    const uint64_t bb = block_sum<uint64_t>(bad, redu);
+    const uint64_t be = block_sum<uint64_t>(edges, redu);
+    if (threadIdx.x == 0) {
+        if (bc) add64(&S.ctr->cut, bc);
+        if (bb) add32(&S.ctr->cert_bad, (int32_t)bb);
+        if (be) addu64(&S.ctr->wk[WF_CERT][0], be);
+    }
+}
+
+void launch_cert(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx) {
+    hipLaunchKernelGGL(k_cert, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+}
+
+// ------------------------------------------------------------- prune mask (a6)
+// After FWD into h and UND into aux: aux = keep flag.  The kept count goes to
+// ctr->visited and an order-independent 64-bit hash of the kept set to
+// hash_out[list index].
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_mask_combine(const SlotView* __restrict__ slots,
+                                                      const int* __restrict__ list,
+                                                      unsigned long long* hash_out) {
+    const SlotView S = load_slot(slots, list);
+    __shared__ uint64_t redu[256 / WAVE];
+    const int n = S.g.n;
+    uint64_t kept = 0, hsh = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int v = (int)i;
+        const bool keep = (S.h[v] != n && S.aux[v] != n) || v == S.s || v == S.t;
+        S.aux[v] = keep ? 1 : 0;
+        if (keep) { ++kept; hsh += mix64((uint64_t)v + 0x9E3779B97F4A7C15ULL); }
+    }
+    const uint64_t bk = block_sum<uint64_t>(kept, redu);
+    const uint64_t bh = block_sum<uint64_t>(hsh, redu);
+    if (threadIdx.x == 0) {
+        if (bk) add32(&S.ctr->visited, (int32_t)bk);
+        if (bh) addu64((uint64_t*)&hash_out[list[blockIdx.y]], bh);
+    }
+}
+
+void launch_mask_combine(hipStream_t st, const SlotView* slots, const int* list, int nlist,
+                         int gridx, unsigned long long* hash_out) {
+    hipLaunchKernelGGL(k_mask_combine, dim3(gridx, nlist), dim3(256), 0, st, slots, list, hash_out);
+}
+
+// ------------------------------------------------------- capacity updates (a24)
+// For entry e (u -> w) with capacity c -> c': the flow on the pair is
+// f = cap[e] - res[e] (negative when it runs w -> u).  Setting cap[e] = c'
+// keeps f when f <= c': res[e] += c' - c.  When f > c' the excess flow
+// f - c' is sent back: res[e] = 0, res[rev] += c - f ... i.e. the pair
+// carries exactly c' from u to w, u gains f - c', w loses f - c'.
+// Entries are distinct within a batch (the host merges duplicates), so each
+// pair has one writer.  The graph's cap array itself is updated by the host.
+
+__global__ __launch_bounds__(256) void k_apply_updates(const SlotView* __restrict__ slots,
+                                                       const int* __restrict__ list,
+                                                       const int32_t* __restrict__ entries,
+                                                       const int64_t* __restrict__ old_cap,
+                                                       const int64_t* __restrict__ new_cap, int nupd) {
+    const SlotView S = load_slot(slots, list);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nupd; i += gridDim.x * blockDim.x) {
+        const int e = entries[i];
+        const int r = S.g.rev[e];
+        const int64_t c = old_cap[i], c2 = new_cap[i];
+        const int64_t res = S.res[e];
+        const int64_t f = c - res;  // flow u -> w on this pair (cap[rev] stays)
+        if (f <= c2) {
+            S.res[e] = res + (c2 - c);
+        } else {
+            const int64_t back = f - c2;  // flow that no longer fits
+            S.res[e] = 0;
+            S.res[r] = S.res[r] - back;
+            // u = owner of entry e gets the flow back, w loses it
+            const int w = S.g.col[e];
+            const int u = S.g.col[r];
+            add64(&S.excess[u], back);
+            add64(&S.excess[w], -back);
+        }
+    }
+}
+
+void launch_apply_updates(hipStream_t st, const SlotView* slots, const int* list, int nlist,
+                          const int32_t* entries, const int64_t* old_cap, const int64_t* new_cap,
+                          int nupd) {
+    const int gx = (nupd + 255) / 256 > 0 ? (nupd + 255) / 256 : 1;
+    hipLaunchKernelGGL(k_apply_updates, dim3(gx < 4096 ? gx : 4096, nlist), dim3(256), 0, st, slots,
+                       list, entries, old_cap, new_cap, nupd);
+}
+
+__global__ void k_cap_scatter(int64_t* __restrict__ cap, const int32_t* __restrict__ entries,
+                              const int64_t* __restrict__ new_cap, int64_t* __restrict__ old_cap, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int e = entries[i];
+        old_cap[i] = cap[e];
+        cap[e] = new_cap[i];
+    }
+}
+
+void launch_cap_scatter(hipStream_t st, int64_t* cap, const int32_t* entries, const int64_t* new_cap,
+                        int64_t* old_cap, int n) {
+    const int gx = std::max(1, std::min(4096, (n + 255) / 256));
+    hipLaunchKernelGGL(k_cap_scatter, dim3(gx), dim3(256), 0, st, cap, entries, new_cap, old_cap, n);
+}
+
+}  // namespace mf

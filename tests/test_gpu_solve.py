@@ -1,0 +1,161 @@
+"""End-to-end GPU flows, bit-exact against the oracle and the golden fixtures.
+
+Discharge, path push, RM and the hot sweep are nondeterministic, so they are
+pinned through the final int64 flow (unique), the GPU's own min-cut
+certificate and an independent CPU certificate on the downloaded residual.
+"""
+import numpy as np
+import pytest
+
+import mscresearch_amd as M
+import oracle as O
+from gpu_util import Case, engine_with, log
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1():
+    log("case: uniform n=10000 m=100000 seed 1001, R=128")
+    return Case(M.Edges.uniform(10000, 100000, 100, 1001), R=128)
+
+
+def test_golden_flows(golden):
+    from test_oracle import regenerate
+
+    for fx in golden:
+        log(f"golden {fx['name']}")
+        e = regenerate(fx)
+        case = Case(e, R=min(128, fx["n"]))
+        eng, gid = engine_with(case)
+        for (a, b), f in zip(fx["queries"], fx["flows"]):
+            s, t = case.q(a, b)
+            r = eng.solve(gid, s, t)
+            assert r.flow == f, (fx["name"], a, b, r.flow, f)
+            assert r.cert_ok == 1
+
+
+def test_c1_flows_and_cpu_certificate(c1):
+    eng, gid = engine_with(c1)
+    qs, qt = M.uniform_queries(c1.n, 64, 1002)
+    for i, (a, b) in enumerate(zip(qs, qt)):
+        s, t = c1.q(a, b)
+        r, res, ex, h = eng.dbg_solve_state(gid, s, t, c1.n, c1.E)
+        f = c1.oracle_flow(s, t)
+        assert r.flow == f, (i, r.flow, f)
+        assert O.certify(c1.og, res, s, t, r.flow, excess=ex) == 0
+        # the GPU's final heights are the exact reach-to-t set of the residual
+        hb = O.bfs_heights(c1.og, res, np.zeros(c1.n, np.int64), s, t)
+        assert np.array_equal(h < c1.n, hb < c1.n)
+
+
+@pytest.mark.parametrize("rows,cols", [(64, 64), (128, 96)])
+def test_grid_flows(rows, cols):
+    log(f"grid {rows}x{cols}")
+    case = Case(M.Edges.grid(rows, cols, 1000, 2001), R=64, tile_max=512)
+    eng, gid = engine_with(case)
+    qs, qt = M.grid_queries(rows, cols, 8, 2002)
+    for a, b in zip(qs, qt):
+        s, t = case.q(a, b)
+        r = eng.solve(gid, s, t)
+        assert r.flow == case.oracle_flow(s, t)
+        assert r.cert_ok == 1
+
+
+@pytest.mark.parametrize("scale", [12, 14])
+def test_rmat_flows_pruned(scale):
+    log(f"rmat scale {scale}")
+    case = Case(M.Edges.rmat(scale, 10**6, 3000 + scale), R=128)
+    eng, gid = engine_with(case)
+    qs, qt = case.edges.hub_queries(24, 3100 + scale)
+    pruned = 0
+    for a, b in zip(qs, qt):
+        s, t = case.q(a, b)
+        r = eng.solve(gid, s, t)
+        assert r.flow == case.oracle_flow(s, t), (a, b)
+        assert r.cert_ok == 1
+        pruned += r.pruned
+    assert pruned > 0  # R-MAT has isolated vertices: pruning shrinks the graph
+    hits, misses = eng.cache_stats()
+    assert hits + misses == pruned
+
+
+def test_edge_cases():
+    # 0: s, 5: t; parallel and antiparallel lines; an isolated vertex 6;
+    # 7 -> 8 unreachable component
+    lines = [(0, 1, 3), (0, 1, 4), (1, 0, 2), (1, 5, 5), (0, 2, 9), (2, 5, 1), (2, 3, 6), (3, 5, 2),
+             (5, 3, 7), (7, 8, 5), (4, 0, 3)]
+    n = 9
+    us = np.array([l[0] for l in lines], np.int32)
+    vs = np.array([l[1] for l in lines], np.int32)
+    cs = np.array([l[2] for l in lines], np.int64)
+    hg = M.HostGraph.from_edges(n, us, vs, cs).partition(2)
+    d = hg.export()
+    og = O.Graph(n, d["row_ptr"], d["col"], d["cap"], d["rev"])
+    dg = M.DeviceGraph(hg, 4)
+    eng = M.Engine(1)
+    gid = eng.add_graph(dg)
+    nid = d["new_id"]
+    pairs = [(0, 5), (0, 8), (7, 8), (5, 0), (6, 5), (4, 5), (0, 3), (8, 7), (1, 2)]
+    for a, b in pairs:
+        s, t = int(nid[a]), int(nid[b])
+        r = eng.solve(gid, s, t)
+        assert r.flow == O.maxflow(og, s, t), (a, b)
+    with pytest.raises(M.MaxFlowError):
+        eng.solve(gid, 1, 1)
+    with pytest.raises(M.MaxFlowError):
+        eng.solve(gid, 0, n)
+
+
+@pytest.mark.parametrize("variant", [
+    dict(tile_frac=0.0),                        # region (tile) discharge every iteration
+    dict(tile_frac=1e9),                        # active-list discharge every iteration
+    dict(path_switch=1 << 20, p_count0=1),      # path push whenever anything is active
+    dict(h_cut=0, rm_freq0=1, rm_freq_max=1),   # RM repair on every iteration
+    dict(hot_min_active=1, hot_k=4),            # hot sweep on every even iteration
+    dict(gr_freq=1, gr_freq_min=1, gr_freq_max=1),
+    dict(K=1, gr_freq=1000, gr_freq_min=1000, gr_freq_max=1000),
+    dict(prune=0),
+])
+def test_scheduler_variants_stay_exact(c1, variant):
+    log(f"variant {variant}")
+    eng, gid = engine_with(c1, **variant)
+    qs, qt = M.uniform_queries(c1.n, 12, 1005)
+    for a, b in zip(qs, qt):
+        s, t = c1.q(a, b)
+        r = eng.solve(gid, s, t)
+        assert r.flow == c1.oracle_flow(s, t), (variant, a, b)
+        assert r.cert_ok == 1
+
+
+def test_batched_equals_alone(c1):
+    """Each result is the same alone and in the mix (continuous batching)."""
+    qs, qt = M.uniform_queries(c1.n, 64, 1002)
+    q = [c1.q(a, b) for a, b in zip(qs, qt)]
+    alone_eng, gid = engine_with(c1)
+    alone = [alone_eng.solve(gid, s, t).flow for s, t in q]
+    mix_eng, gid2 = engine_with(c1, nslots=24)
+    for i, (s, t) in enumerate(q):
+        mix_eng.submit(gid2, s, t, i)
+    res = mix_eng.run()
+    assert len(res) == len(q)
+    got = {r.tag: r.flow for r in res}
+    assert [got[i] for i in range(len(q))] == alone
+    assert all(r.cert_ok == 1 and r.status == 0 for r in res)
+
+
+def test_mixed_graphs_in_one_engine():
+    g1 = Case(M.Edges.grid(40, 40, 1000, 4001), R=32, tile_max=256)
+    g2 = Case(M.Edges.rmat(11, 10**6, 4002), R=64)
+    eng = M.Engine(8)
+    a = eng.add_graph(g1.dg)
+    b = eng.add_graph(g2.dg)
+    qa = [g1.q(x, y) for x, y in zip(*M.grid_queries(40, 40, 10, 4003))]
+    qb = [g2.q(x, y) for x, y in zip(*g2.edges.hub_queries(10, 4004))]
+    for i in range(10):
+        eng.submit(a, *qa[i], 2 * i)
+        eng.submit(b, *qb[i], 2 * i + 1)
+    res = {r.tag: r for r in eng.run()}
+    for i in range(10):
+        assert res[2 * i].flow == g1.oracle_flow(*qa[i])
+        assert res[2 * i + 1].flow == g2.oracle_flow(*qb[i])

@@ -1,0 +1,426 @@
+"""Pins the CPU oracle against independent checks (no GPU needed).
+
+Each oracle function is checked against something that does not retype its
+formula: golden flows from networkx/scipy, closed forms, brute force over all
+cuts, scipy shortest paths, hand-written fixtures, and invariants chosen so a
+plausible slip (dropped term, wrong index, transposed operand) fails.
+"""
+import itertools
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy.sparse import csr_matrix
+from scipy.sparse.csgraph import shortest_path
+
+import mscresearch_amd as M
+import oracle as O
+
+
+def write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def small_graph(n, edges):
+    us = np.array([e[0] for e in edges], np.int32)
+    vs = np.array([e[1] for e in edges], np.int32)
+    cs = np.array([e[2] for e in edges], np.int64)
+    return O.build_csr(n, us, vs, cs)
+
+
+# ----------------------------------------------------------------- parsing
+
+def test_parse_skip_and_stop_rules(tmp_path):
+    # header n=4 m=9; lines: ok, out of range (5), self-loop, cap 0, cap -3,
+    # ok, id 0 (out of range), ok, then a malformed line stops the parse
+    text = "4 9\n1 2 5\n1 5 3\n2 2 4\n2 3 0\n3 4 -3\n3 4 7\n0 1 2\n4 1 1\n2 x 9\n1 3 8\n"
+    n, us, vs, cs = O.parse_graph(write(tmp_path, "graph.txt", text))
+    assert n == 4
+    assert us.tolist() == [0, 2, 3] and vs.tolist() == [1, 3, 0] and cs.tolist() == [5, 7, 1]
+
+
+def test_parse_header_m_limits_lines(tmp_path):
+    text = "3 2\n1 2 1\n2 3 1\n1 3 1\n"
+    n, us, _, _ = O.parse_graph(write(tmp_path, "graph.txt", text))
+    assert len(us) == 2
+    text = "3 10\n1 2 1\n2 3 1\n"  # fewer lines than m: stop at EOF
+    _, us, _, _ = O.parse_graph(write(tmp_path, "g2.txt", text))
+    assert len(us) == 2
+
+
+def test_parse_bad_header(tmp_path):
+    with pytest.raises(IOError):
+        O.parse_graph(write(tmp_path, "graph.txt", "x y\n"))
+    with pytest.raises(IOError):
+        O.parse_graph(str(tmp_path / "missing.txt"))
+
+
+def test_parse_queries(tmp_path):
+    s, t = O.parse_queries(write(tmp_path, "test.txt", "1 2\n3 4\n5\n"))
+    assert s.tolist() == [0, 2] and t.tolist() == [1, 3]
+
+
+# --------------------------------------------------------------------- CSR
+
+def test_csr_layout_by_hand():
+    # lines: 0->1 (5), 2->0 (3), 0->2 (4)
+    g = small_graph(3, [(0, 1, 5), (2, 0, 3), (0, 2, 4)])
+    # row 0: fwd 0->1, rev of 2->0, fwd 0->2 ; row 1: rev of 0->1 ; row 2: fwd 2->0, rev of 0->2
+    assert g.row_ptr.tolist() == [0, 3, 4, 6]
+    assert g.col.tolist() == [1, 2, 2, 0, 0, 0]
+    assert g.cap.tolist() == [5, 0, 4, 0, 3, 0]
+    assert g.rev.tolist() == [3, 4, 5, 0, 1, 2]
+    so, si, do, di = g.stats
+    assert so.tolist() == [9, 0, 3] and si.tolist() == [3, 5, 4]
+    assert do.tolist() == [2, 0, 1] and di.tolist() == [1, 1, 1]
+
+
+def test_csr_invariants_random():
+    e = M.Edges.uniform(500, 4000, 50, 11)
+    n, us, vs, cs = e.arrays()
+    g = O.build_csr(n, us, vs, cs)
+    assert g.E == 2 * len(us)
+    E = np.arange(g.E)
+    assert np.array_equal(g.rev[g.rev], E)
+    owner = np.repeat(np.arange(n), np.diff(g.row_ptr))
+    assert np.array_equal(g.col[g.rev], owner)
+    assert np.all((g.cap > 0) ^ (g.cap[g.rev] > 0))  # exactly one side carries capacity
+    assert g.cap.sum() == cs.sum()
+
+
+# --------------------------------------------------------------- partition
+
+def test_partition_scores_closed_form():
+    so = np.array([10, 0, 7, 0], np.int64)
+    si = np.array([0, 4, 2, 0], np.int64)
+    do = np.array([2, 0, 1, 0], np.int32)
+    di = np.array([0, 1, 3, 0], np.int32)
+    part, score = O.partition(4, 2, so, si, do, di)
+    # v0: do*di = 0 -> score 1; v1: 0 -> 1; v3: 1
+    # v2: (ln(7*1+1) + ln(2*3+1)) / ln(sqrt(3)+1)
+    expect2 = (math.log(8.0) + math.log(7.0)) / math.log(math.sqrt(3.0) + 1.0)
+    assert score[[0, 1, 3]].tolist() == [1.0, 1.0, 1.0]
+    assert abs(score[2] - expect2) < 1e-12
+    # order: v2 (largest), then v0, v1, v3 (ties by id); greedy least-loaded:
+    # v2 -> r0 (load e2), v0 -> r1 (1), v1 -> r1 (2), v3 -> r1 if 2 < e2 else r0
+    assert part[2] == 0 and part[0] == 1 and part[1] == 1
+    assert part[3] == (1 if 2.0 < expect2 else 0)
+
+
+def test_partition_greedy_min_load_property():
+    e = M.Edges.uniform(3000, 20000, 100, 12)
+    n, us, vs, cs = e.arrays()
+    g = O.build_csr(n, us, vs, cs)
+    R = 16
+    part, score = O.partition(n, R, *g.stats)
+    order = sorted(range(n), key=lambda v: (-score[v], v))
+    load = [0.0] * R
+    for v in order:  # each vertex lands on a region whose load was minimal
+        assert load[part[v]] == min(load)
+        assert part[v] == load.index(min(load))
+        load[part[v]] += score[v]
+    assert np.bincount(part, minlength=R).min() > 0
+
+
+# ----------------------------------------------------------------- reorder
+
+def test_reorder_invariants():
+    e = M.Edges.uniform(800, 6000, 30, 13)
+    n, us, vs, cs = e.arrays()
+    g = O.build_csr(n, us, vs, cs)
+    R = 8
+    part, _ = O.partition(n, R, *g.stats)
+    g2, new_id, old_id, rs, re_, rid = O.reorder(g, R, part)
+    assert sorted(new_id.tolist()) == list(range(n))
+    assert np.array_equal(old_id[new_id], np.arange(n))
+    for r in range(R):
+        members = old_id[rs[r]:re_[r]]
+        assert np.all(part[members] == r)
+        assert np.all(np.diff(members) > 0)  # original order inside a region
+        assert np.all(rid[rs[r]:re_[r]] == r)
+    assert rs[0] == 0 and re_[-1] == n and np.array_equal(rs[1:], re_[:-1])
+    # isomorphism: entry multisets per vertex map through new_id, rev consistent
+    for v in range(0, n, 37):
+        a = sorted(zip(new_id[g.col[g.row_ptr[v]:g.row_ptr[v + 1]]].tolist(), g.cap[g.row_ptr[v]:g.row_ptr[v + 1]].tolist()))
+        w = new_id[v]
+        b = sorted(zip(g2.col[g2.row_ptr[w]:g2.row_ptr[w + 1]].tolist(), g2.cap[g2.row_ptr[w]:g2.row_ptr[w + 1]].tolist()))
+        assert a == b
+    owner = np.repeat(np.arange(n), np.diff(g2.row_ptr))
+    assert np.array_equal(g2.rev[g2.rev], np.arange(g2.E))
+    assert np.array_equal(g2.col[g2.rev], owner)
+
+
+# ---------------------------------------------------------------- pruning
+
+def reach(n, edges, src):
+    A = np.zeros((n, n), bool)
+    for u, v in edges:
+        A[u, v] = True
+    R = np.eye(n, dtype=bool)
+    for _ in range(n):
+        R = R | (R.astype(np.int32) @ A.astype(np.int32) > 0)
+    return R[src]
+
+
+def test_prune_mask_brute_force():
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        n = 9
+        edges = [(int(a), int(b), int(c)) for a, b, c in zip(rng.integers(0, n, 14), rng.integers(0, n, 14), rng.integers(1, 5, 14)) if a != b]
+        g = small_graph(n, edges)
+        s, t = 0, n - 1
+        fwd = reach(n, [(u, v) for u, v, _ in edges], s)
+        und = reach(n, [(u, v) for u, v, _ in edges] + [(v, u) for u, v, _ in edges], t)
+        expect = (fwd & und).astype(np.int32)
+        expect[s] = expect[t] = 1
+        assert np.array_equal(O.prune_mask(g, s, t), expect), trial
+
+
+def test_prune_keeps_every_st_path_vertex():
+    e = M.Edges.rmat(9, 1000, 21)
+    n, us, vs, cs = e.arrays()
+    g = O.build_csr(n, us, vs, cs)
+    s, t = e.hub_queries(4, 22)
+    for a, b in zip(s, t):
+        mask = O.prune_mask(g, a, b)
+        fwd = O.bfs_heights  # noqa: F841 (documentation only)
+        # any vertex on an s->t path is forward-reachable from s and reaches t
+        f_all, res = O.maxflow(g, a, b, want_residual=True)
+        flow_vertices = set()
+        for v in range(n):
+            for ee in range(g.row_ptr[v], g.row_ptr[v + 1]):
+                if g.cap[ee] > 0 and res[ee] < g.cap[ee]:
+                    flow_vertices.update([v, int(g.col[ee])])
+        assert all(mask[v] for v in flow_vertices)
+        # and the flow on the induced subgraph is the same
+        sub, o2s, _ = O.induced(g, mask)
+        assert O.maxflow(sub, o2s[a], o2s[b]) == f_all
+
+
+def test_induced_brute_force():
+    rng = np.random.default_rng(6)
+    n = 30
+    edges = [(int(a), int(b), int(c)) for a, b, c in zip(rng.integers(0, n, 90), rng.integers(0, n, 90), rng.integers(1, 9, 90)) if a != b]
+    g = small_graph(n, edges)
+    mask = (rng.random(n) < 0.6).astype(np.int32)
+    sub, o2s, s2o = O.induced(g, mask)
+    kept = [v for v in range(n) if mask[v]]
+    assert s2o.tolist() == kept
+    assert o2s.tolist() == [kept.index(v) if mask[v] else -1 for v in range(n)]
+    for i, v in enumerate(kept):
+        row = [(int(o2s[g.col[e]]), int(g.cap[e])) for e in range(g.row_ptr[v], g.row_ptr[v + 1]) if mask[g.col[e]]]
+        got = list(zip(sub.col[sub.row_ptr[i]:sub.row_ptr[i + 1]].tolist(), sub.cap[sub.row_ptr[i]:sub.row_ptr[i + 1]].tolist()))
+        assert got == row
+    assert np.array_equal(sub.rev[sub.rev], np.arange(sub.E))
+    owner = np.repeat(np.arange(sub.n), np.diff(sub.row_ptr))
+    assert np.array_equal(sub.col[sub.rev], owner)
+
+
+def test_core_mask_properties():
+    e = M.Edges.uniform(400, 1500, 20, 31)
+    n, us, vs, cs = e.arrays()
+    g = O.build_csr(n, us, vs, cs)
+    R = 8
+    part, _ = O.partition(n, R, *g.stats)
+    g2, _, _, rs, re_, rid = O.reorder(g, R, part)
+    s, t = 3, 250
+    mask, kept = O.core_mask(g2, R, rid, s, t, hop_limit=2)
+    assert mask[s] == 1 and mask[t] == 1
+    boundary = np.zeros(n, bool)
+    for v in range(n):
+        cols = g2.col[g2.row_ptr[v]:g2.row_ptr[v + 1]]
+        boundary[v] = np.any(rid[cols] != rid[v])
+    # random regions: every region connects, all are kept, boundary vertices kept
+    assert kept == R
+    assert np.all(mask[boundary] == 1)
+    assert np.all(mask[rid == rid[s]] == 1) and np.all(mask[rid == rid[t]] == 1)
+    # with a huge hop limit the core is every vertex of a kept region
+    big, _ = O.core_mask(g2, R, rid, s, t, hop_limit=10**6)
+    assert np.all(big[mask == 1] == 1)
+
+
+def test_core_mask_by_hand():
+    # two regions {0,1,2} and {3,4,5}; chain 0-1-2 inside r0, 2->3 crosses, 3-4-5 in r1
+    edges = [(0, 1, 1), (1, 2, 1), (2, 3, 1), (3, 4, 1), (4, 5, 1)]
+    g = small_graph(6, edges)
+    rid = np.array([0, 0, 0, 1, 1, 1], np.int32)
+    mask, kept = O.core_mask(g, 2, rid, 0, 5, hop_limit=0)
+    # s region and t region keep all their vertices
+    assert kept == 2 and mask.tolist() == [1, 1, 1, 1, 1, 1]
+    # s and t in the same region 0: region 1 is reachable but no region path returns -> only r0 kept
+    g3 = small_graph(6, [(0, 1, 1), (1, 2, 1), (2, 3, 1), (3, 4, 1), (4, 5, 1)])
+    m3, k3 = O.core_mask(g3, 2, rid, 0, 2, hop_limit=0)
+    # undirected entries make r0 <-> r1 both ways, so both regions stay
+    assert k3 == 2 and m3[0] == 1 and m3[2] == 1 and m3[3] == 1  # 3 is a boundary vertex
+
+
+# -------------------------------------------------------- preflow and BFS
+
+def test_init_preflow_closed_form():
+    e = M.Edges.uniform(200, 1500, 40, 41)
+    n, us, vs, cs = e.arrays()
+    g = O.build_csr(n, us, vs, cs)
+    s = int(us[0])
+    res, ex = O.init_preflow(g, s)
+    out_cap = cs[us == s].sum()
+    assert ex[s] == -out_cap
+    for w in range(n):
+        if w != s:
+            assert ex[w] == cs[(us == s) & (vs == w)].sum()
+    row = slice(g.row_ptr[s], g.row_ptr[s + 1])
+    assert np.all(res[row] == 0)
+    assert np.array_equal(res + res[g.rev], g.cap + g.cap[g.rev])
+
+
+def test_bfs_heights_vs_scipy_shortest_paths():
+    rng = np.random.default_rng(8)
+    for trial in range(10):
+        e = M.Edges.uniform(60, 240, 10, 100 + trial)
+        n, us, vs, cs = e.arrays()
+        g = O.build_csr(n, us, vs, cs)
+        s, t = 0, 1
+        res, ex = O.init_preflow(g, s)
+        ex = ex.copy()
+        deficits = rng.choice(np.arange(2, n), 3, replace=False)
+        ex[deficits] = -1
+        h = O.bfs_heights(g, res, ex, s, t)
+        # distance from v to the seed set over residual entries v->w (res>0), s removed
+        rows, cols = [], []
+        for v in range(n):
+            for ee in range(g.row_ptr[v], g.row_ptr[v + 1]):
+                w = int(g.col[ee])
+                if res[ee] > 0 and v != s and w != s:
+                    rows.append(v)
+                    cols.append(w)
+        A = csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(n, n))
+        D = shortest_path(A, unweighted=True, directed=True)
+        seeds = [t] + [int(d) for d in deficits]
+        dist = D[:, seeds].min(axis=1)
+        expect = np.where(np.isinf(dist), n, dist).astype(np.int64)
+        expect[s] = n
+        assert np.array_equal(h.astype(np.int64), expect), trial
+
+
+# ------------------------------------------------------------------ Dinic
+
+def brute_min_cut(n, edges, s, t):
+    best = None
+    others = [v for v in range(n) if v not in (s, t)]
+    for bits in itertools.product([0, 1], repeat=len(others)):
+        S = {s} | {v for v, b in zip(others, bits) if b}
+        c = sum(cap for u, v, cap in edges if u in S and v not in S)
+        best = c if best is None else min(best, c)
+    return best
+
+
+def test_dinic_equals_brute_force_min_cut():
+    rng = np.random.default_rng(9)
+    for trial in range(25):
+        n = int(rng.integers(4, 9))
+        m = int(rng.integers(n, 3 * n))
+        edges = [(int(a), int(b), int(c)) for a, b, c in zip(rng.integers(0, n, m), rng.integers(0, n, m), rng.integers(1, 20, m)) if a != b]
+        g = small_graph(n, edges)
+        f, res = O.maxflow(g, 0, n - 1, want_residual=True)
+        assert f == brute_min_cut(n, edges, 0, n - 1), trial
+        assert O.certify(g, res, 0, n - 1, f) == 0
+
+
+def test_dinic_closed_forms():
+    # series chain: min capacity
+    g = small_graph(5, [(0, 1, 7), (1, 2, 3), (2, 3, 9), (3, 4, 5)])
+    assert O.maxflow(g, 0, 4) == 3
+    # parallel disjoint paths: sum
+    g = small_graph(6, [(0, 1, 4), (1, 5, 4), (0, 2, 6), (2, 5, 6), (0, 3, 1), (3, 5, 2), (5, 4, 1)])
+    assert O.maxflow(g, 0, 5) == 11
+    # disconnected: 0
+    g = small_graph(4, [(0, 1, 5), (2, 3, 5)])
+    assert O.maxflow(g, 0, 3) == 0
+    # complete bipartite a x b with unit middle edges, big source/sink caps: a*b
+    a, b = 4, 5
+    edges = [(0, 1 + i, 100) for i in range(a)] + [(1 + i, 1 + a + j, 1) for i in range(a) for j in range(b)] + [(1 + a + j, 1 + a + b, 100) for j in range(b)]
+    g = small_graph(a + b + 2, edges)
+    assert O.maxflow(g, 0, a + b + 1) == a * b
+    # duplicate and antiparallel lines are kept separate but sum correctly
+    g = small_graph(3, [(0, 1, 2), (0, 1, 3), (1, 0, 4), (1, 2, 10)])
+    assert O.maxflow(g, 0, 2) == 5
+
+
+def test_dinic_grid_bound():
+    e = M.Edges.grid(12, 20, 1000, 51)
+    n, us, vs, cs = e.arrays()
+    g = O.build_csr(n, us, vs, cs)
+    s, t = M.grid_queries(12, 20, 6, 52)
+    for a, b in zip(s, t):
+        f = O.maxflow(g, a, b)
+        assert 0 < f <= min(cs[us == a].sum(), cs[vs == b].sum())
+
+
+def test_dinic_matches_golden(golden):
+    for fx in golden:
+        if fx["n"] > 5000:
+            continue  # the 1e4-vertex fixture is checked on the GPU side and in test_golden_large
+        e = regenerate(fx)
+        n, us, vs, cs = e.arrays()
+        g = O.build_csr(n, us, vs, cs)
+        for (a, b), f in zip(fx["queries"], fx["flows"]):
+            assert O.maxflow(g, a, b) == f, (fx["name"], a, b)
+
+
+def test_golden_large(golden):
+    fx = [x for x in golden if x["n"] == 10000][0]
+    e = regenerate(fx)
+    n, us, vs, cs = e.arrays()
+    g = O.build_csr(n, us, vs, cs)
+    qs = np.array([q[0] for q in fx["queries"]], np.int32)
+    qt = np.array([q[1] for q in fx["queries"]], np.int32)
+    flows, _ = O.maxflow_many(g, qs, qt, threads=4)
+    assert flows.tolist() == fx["flows"]
+
+
+def regenerate(fx):
+    import hashlib
+    import tempfile
+
+    name = fx["name"]
+    if name.startswith("uniform"):
+        n = fx["n"]
+        m = int(name.split("_m")[1])
+        seeds = {12: 7000, 40: 7001, 150: 7002, 300: 7003, 10000: 1001}
+        caps = {12: 9, 40: 20, 150: 100, 300: 100, 10000: 100}
+        e = M.Edges.uniform(n, m, caps[n], seeds[n])
+    elif name.startswith("grid"):
+        r, c = map(int, name.split("_")[1].split("x"))
+        seed = {(4, 6): 7200, (16, 16): 7201, (32, 48): 7202}[(r, c)]
+        e = M.Edges.grid(r, c, 1000, seed)
+    else:
+        sc = int(name.split("_s")[1])
+        e = M.Edges.rmat(sc, 10**6, {8: 7400, 10: 7401}[sc])
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "graph.txt")
+        e.write(p)
+        assert hashlib.sha256(open(p, "rb").read()).hexdigest() == fx["graph_txt_sha256"], name
+    return e
+
+
+# ------------------------------------------------------------ certificate
+
+def test_certificate_catches_planted_mistakes():
+    e = M.Edges.uniform(120, 900, 30, 61)
+    n, us, vs, cs = e.arrays()
+    g = O.build_csr(n, us, vs, cs)
+    s, t = int(us[3]), int(vs[7])
+    f, res = O.maxflow(g, s, t, want_residual=True)
+    assert O.certify(g, res, s, t, f) == 0
+    assert O.certify(g, res, s, t, f + 1) == 7          # wrong value claimed
+    assert O.certify(g, g.cap.copy(), s, t, 0) in (4,)  # zero flow: s still reaches t
+    bad = res.copy()
+    k = int(np.nonzero(res > 0)[0][0])
+    bad[k] += 1                                           # breaks the pair sum
+    assert O.certify(g, bad, s, t, f) == 2
+    bad = res.copy()
+    bad[k] = -1
+    bad[g.rev[k]] = res[k] + res[g.rev[k]] + 1           # negative residual, pair kept
+    assert O.certify(g, bad, s, t, f) == 1
