@@ -1,0 +1,249 @@
+"""Benchmark: exact max-flow queries/sec on one MI355X, batched vs plain.
+
+Workload (default): R-MAT scale 22 (a,b,c,d = .57,.19,.19,.05; 16*2^22
+samples, duplicates merged by summing log-uniform [1,1e6] capacities, graph
+seed 3001), 1,024 queries with s from the top 1% of vertices by out-degree
+and t uniform over vertices with in-degree > 0 (query seed 3002).  A step is
+one batch of `--batch` queries run to completion by the batched engine
+(`--batch` concurrent slots, continuous refill); --steps batches are timed.
+Every GPU flow carries the GPU's min-cut certificate; a sample is also
+compared bit for bit with the CPU oracle (Dinic), whose time on the same
+sample is the cpu_baseline.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import mscresearch_amd as M  # noqa: E402
+
+METRIC = "max-flow queries/sec + p50/p99 latency, served/batched vs plain, bit-exact"
+HBM_PEAK = 8.0e12
+# algorithmic bytes per unit: edge scan, vertex scan, push, relabel/discovery
+UNIT_BYTES = np.array([16.0, 16.0, 64.0, 8.0])
+FAMILIES = ["init", "bfs", "active", "discharge", "rm", "cert"]
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def pct(v, p):
+    v = sorted(v)
+    if not v:
+        return 0.0
+    k = max(1, min(len(v), int(np.ceil(p * len(v)))))
+    return float(v[k - 1])
+
+
+def build_workload(args):
+    t0 = time.time()
+    if args.workload == "rmat":
+        e = M.Edges.rmat(args.scale, 10**6, args.graph_seed, threads=min(16, os.cpu_count() or 4))
+        qs, qt = e.hub_queries(args.queries + args.batch * args.warmup, args.query_seed)
+        name = f"rmat{args.scale}_hub_queries"
+        desc = (f"R-MAT scale {args.scale} (a=.57,b=.19,c=.19), 16*2^{args.scale} samples, duplicates merged, "
+                f"caps log-uniform [1,1e6]; s in top 1% out-degree, t in-degree>0")
+    elif args.workload == "grid":
+        side = args.grid
+        e = M.Edges.grid(side, side, 1000, args.graph_seed)
+        qs, qt = M.grid_queries(side, side, args.queries + args.batch * args.warmup, args.query_seed)
+        name = f"grid{side}x{side}_left_right"
+        desc = f"{side}x{side} 4-neighbour grid, both directions, caps U[1,1000]; s col 0, t col {side - 1}"
+    else:
+        e = M.Edges.uniform(10000, 100000, 100, args.graph_seed)
+        qs, qt = M.uniform_queries(10000, args.queries + args.batch * args.warmup, args.query_seed)
+        name = "uniform_n10000_m100000"
+        desc = "uniform n=1e4 m=1e5 caps U[1,100]"
+    n, cnt = e.info()
+    log(f"generated {name}: n={n} lines={cnt} in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    hg = M.HostGraph.from_edgelist(e).partition(args.R)
+    log(f"CSR + partition R={args.R} in {time.time() - t0:.1f}s")
+    d = hg.export()
+    nid = d["new_id"]
+    qs2 = nid[qs].astype(np.int32)
+    qt2 = nid[qt].astype(np.int32)
+    return e, hg, d, qs2, qt2, name, desc
+
+
+def run_batches(eng, gid, qs, qt, batch, nbatches, offset):
+    lat, flows, res_all = [], {}, []
+    for b in range(nbatches):
+        lo = offset + b * batch
+        for i in range(lo, lo + batch):
+            eng.submit(gid, qs[i], qt[i], i)
+        rs = eng.run()
+        res_all.extend(rs)
+    for r in res_all:
+        if r.status != 0 or r.cert_ok != 1:
+            raise RuntimeError(f"query {r.tag}: status {r.status} cert {r.cert_ok}: {M.lib().mf_last_error().decode()}")
+        lat.append(r.t_end - r.t_arrive)
+        flows[r.tag] = r.flow
+    return res_all, lat, flows
+
+
+def cpu_baseline(d, qs, qt, target_s, threads):
+    """Oracle Dinic on a bounded sample of the same queries (CPU baseline)."""
+    import oracle as O
+
+    og = O.Graph(len(d["new_id"]), d["row_ptr"], d["col"], d["cap"], d["rev"])
+    t0 = time.time()
+    f0, sec0 = O.maxflow_many(og, qs[:1], qt[:1], threads=1)
+    per = max(sec0, 1e-3)
+    k = int(max(threads, min(len(qs), threads * max(1, int(target_s / per)))))
+    k = min(k, len(qs))
+    flows, sec = O.maxflow_many(og, qs[:k], qt[:k], threads=threads)
+    log(f"cpu oracle: 1 query {sec0:.2f}s single-thread; {k} queries on {threads} threads in {sec:.2f}s")
+    return flows, sec, k, sec0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16, help="timed batches")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed batches (distinct queries)")
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--workload", choices=["rmat", "grid", "uniform"], default="rmat")
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--grid", type=int, default=1024)
+    ap.add_argument("--R", type=int, default=512)
+    ap.add_argument("--graph-seed", type=int, default=3001)
+    ap.add_argument("--query-seed", type=int, default=3002)
+    ap.add_argument("--plain", type=int, default=64, help="queries timed on the plain (one-at-a-time) solver")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one MI355X and this path shards by query "
+              "(see DESIGN.md); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    args.queries = args.steps * args.batch
+    if M.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(3)
+
+    e, hg, d, qs, qt, name, desc = build_workload(args)
+    n = len(d["new_id"])
+    E = len(d["col"])
+    t0 = time.time()
+    dg = M.DeviceGraph(hg)
+    log(f"upload {dg.info()} in {time.time() - t0:.1f}s")
+    timed_q = slice(args.batch * args.warmup, args.batch * args.warmup + args.queries)
+    tq_s, tq_t = qs[timed_q], qt[timed_q]
+
+    # ---- batched (the measured value)
+    eng = M.Engine(args.batch)
+    gid = eng.add_graph(dg)
+    if args.warmup:
+        run_batches(eng, gid, qs, qt, args.batch, args.warmup, 0)
+    eng.profile(True)
+    t0 = time.perf_counter()
+    res, lat, flows = run_batches(eng, gid, qs, qt, args.batch, args.steps, args.batch * args.warmup)
+    wall = time.perf_counter() - t0
+    ms, launches, work = eng.profile_read()
+    eng.profile(False)
+    qps = args.queries / wall
+    e_run = sum(r.E_run for r in res)
+    hits, misses = eng.cache_stats()
+    log(f"batched: {args.queries} queries in {wall:.3f}s = {qps:.1f} q/s; cache hits {hits} misses {misses}")
+    st = {k: float(np.mean([getattr(r, k) for r in res])) for k in
+          ("iters", "steps", "grs", "bfs_levels", "drains", "pruned", "n_run", "E_run", "kept")}
+    log("per-query means: " + ", ".join(f"{k}={v:.1f}" for k, v in st.items()))
+    log("family ms: " + ", ".join(f"{FAMILIES[f]}={ms[f]:.1f}/{launches[f]}" for f in range(6)))
+
+    # roofline of the dominant kernel family (HIP events on the engine stream)
+    fam = int(np.argmax(ms))
+    algo_bytes = float((work[fam].astype(np.float64) * UNIT_BYTES).sum())
+    achieved = algo_bytes / (ms[fam] * 1e-3) if ms[fam] > 0 else 0.0
+    fam_table = {FAMILIES[f]: {"ms": round(float(ms[f]), 3), "launches": int(launches[f]),
+                               "alg_bytes": float((work[f].astype(np.float64) * UNIT_BYTES).sum())}
+                 for f in range(6)}
+
+    # ---- plain: one query at a time on one slot
+    plain_lat = []
+    plain_flows = {}
+    if args.plain > 0:
+        peng = M.Engine(1)
+        pgid = peng.add_graph(dg)
+        k = min(args.plain, args.queries)
+        peng.solve(pgid, tq_s[0], tq_t[0])  # warm the subgraph cache like the batched warmup
+        peng.profile(True)
+        t0 = time.perf_counter()
+        for i in range(k):
+            r = peng.solve(pgid, tq_s[i], tq_t[i])
+            plain_lat.append(r.t_end - r.t_start)
+            plain_flows[i] = r.flow
+        pwall = time.perf_counter() - t0
+        pms, pla, _ = peng.profile_read()
+        plain_qps = k / pwall
+        del peng
+        log(f"plain: {k} queries in {pwall:.3f}s = {plain_qps:.1f} q/s, p50 {pct(plain_lat, .5) * 1e3:.2f} ms")
+        log("plain family ms: " + ", ".join(f"{FAMILIES[f]}={pms[f]:.1f}/{pla[f]}" for f in range(6)))
+        for i in range(k):
+            assert plain_flows[i] == flows[args.batch * args.warmup + i], "plain and batched flows differ"
+    else:
+        plain_qps = None
+
+    # ---- CPU oracle baseline on a bounded sample; bit-exact check of the sample
+    cpu = None
+    exact = None
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        oflows, osec, k, sec1 = cpu_baseline(d, tq_s, tq_t, args.cpu_seconds, threads)
+        mism = sum(int(oflows[i] != flows[args.batch * args.warmup + i]) for i in range(k))
+        exact = {"checked": k, "mismatches": mism}
+        cpu = {"value": round(k / osec, 3), "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": f"first {k} timed queries of the same workload, oracle Dinic, one query per thread; "
+                         f"single-thread {1.0 / max(sec1, 1e-9):.3f} q/s",
+               "model": open("/proc/cpuinfo").read().split("model name")[1].split("\n")[0].strip(": ")
+               if os.path.exists("/proc/cpuinfo") and "model name" in open("/proc/cpuinfo").read() else "unknown"}
+        if mism:
+            log(f"EXACTNESS FAILURE: {mism} of {k} flows differ from the oracle")
+
+    out = {
+        "metric": METRIC,
+        "value": round(qps, 3),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": {"workload": name, "graph": desc, "n": n, "csr_entries": E, "queries": args.queries,
+                   "batch": args.batch, "regions": args.R, "parallelism": "one GPU, batched slots"},
+        "plain_qps": round(plain_qps, 3) if plain_qps else None,
+        "speedup_vs_plain": round(qps / plain_qps, 3) if plain_qps else None,
+        "p50_ms": round(pct(lat, .5) * 1e3, 3),
+        "p99_ms": round(pct(lat, .99) * 1e3, 3),
+        "plain_p50_ms": round(pct(plain_lat, .5) * 1e3, 3) if plain_lat else None,
+        "plain_p99_ms": round(pct(plain_lat, .99) * 1e3, 3) if plain_lat else None,
+        "residual_edges_per_s": round(e_run / wall, 1),
+        "certified": sum(int(r.cert_ok) for r in res),
+        "exact_sample": exact,
+        "subgraph_cache": {"hits": hits, "misses": misses},
+        "kernel_families": fam_table,
+        "roofline": {"bound": "hbm", "kernel_family": FAMILIES[fam], "achieved": round(achieved / 1e9, 3),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5),
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

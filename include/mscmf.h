@@ -72,6 +72,12 @@ int mf_hgraph_stats(const mf_hgraph* h, int64_t* sum_out, int64_t* sum_in, int32
                     int32_t* deg_in);
 void mf_hgraph_free(mf_hgraph* h);
 
+/* Reachability classes behind the per-query pruning mask (BC:1391-1445):
+ * scc[n] = strongly connected component over entries with cap > 0, wcc[n] =
+ * weakly connected component.  The mask of (s,t) with t reachable from s is
+ * {v : v reachable from scc[s]} AND {v : wcc[v] == wcc[t]}. */
+int mf_hgraph_components(const mf_hgraph* h, int32_t* scc, int32_t* wcc);
+
 /* Stand-alone partition on given stats (BC:1130-1178), for parity tests. */
 int mf_partition(int n, int R, const int64_t* sum_out, const int64_t* sum_in,
                  const int32_t* deg_out, const int32_t* deg_in, int32_t* part);
@@ -130,6 +136,8 @@ typedef struct mf_params {
     double bu_alpha, bu_beta;
     int prune;           /* per-query reachability pruning + induced subgraph */
     double prune_ratio;  /* keep the subgraph when kept < ratio * n */
+    double gr_work;      /* also GR after gr_work * E discharge scans since the last one, 0 = off */
+    int K_heavy;         /* discharge cycles per pass for wave-scanned (long) rows */
 } mf_params;
 
 void mf_params_default(mf_params* p);

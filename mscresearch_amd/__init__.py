@@ -41,7 +41,7 @@ class Params(C.Structure):
         ("rm", C.c_int), ("rm_freq0", C.c_int), ("rm_freq_max", C.c_int), ("rm_margin", C.c_int),
         ("rm_tau", C.c_double), ("tile_rounds", C.c_int), ("tile_frac", C.c_double), ("cert", C.c_int),
         ("iter_max", C.c_longlong), ("bu_alpha", C.c_double), ("bu_beta", C.c_double),
-        ("prune", C.c_int), ("prune_ratio", C.c_double),
+        ("prune", C.c_int), ("prune_ratio", C.c_double), ("gr_work", C.c_double), ("K_heavy", C.c_int),
     ]
 
 
@@ -80,6 +80,7 @@ SIGNATURES = {
     "mf_hgraph_stats": (C.c_int, [_P, _P, _P, _P, _P]),
     "mf_hgraph_free": (None, [_P]),
     "mf_hgraph_find_entry": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_int32)]),
+    "mf_hgraph_components": (C.c_int, [_P, _P, _P]),
     "mf_partition": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     "mf_read_queries": (C.c_int, [C.c_char_p, C.c_int64, _P, _P, C.POINTER(C.c_int64)]),
     "mf_gen_uniform": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_uint64, C.POINTER(_P)]),
@@ -305,6 +306,13 @@ class HostGraph:
         do, di = np.empty(n, np.int32), np.empty(n, np.int32)
         _check(lib().mf_hgraph_stats(self._h, _ptr(so), _ptr(si), _ptr(do), _ptr(di)))
         return so, si, do, di
+
+    def components(self):
+        n = self.info()[0]
+        scc = np.empty(n, np.int32)
+        wcc = np.empty(n, np.int32)
+        _check(lib().mf_hgraph_components(self._h, _ptr(scc), _ptr(wcc)))
+        return scc, wcc
 
     def find_entry(self, u, v):
         e = C.c_int32(-1)

@@ -262,6 +262,82 @@ void partition_and_reorder(HostGraph& g, int R, const std::vector<int32_t>& part
     g.rev.swap(rev2);
 }
 
+// ------------------------------------------------------------- components
+
+void strong_components(const HostGraph& g, std::vector<int32_t>& comp) {
+    const int n = g.n;
+    comp.assign(n, -1);
+    std::vector<int32_t> index(n, -1), low(n, 0), stk;
+    std::vector<char> on(n, 0);
+    std::vector<std::pair<int32_t, int32_t>> call;  // (vertex, next entry)
+    int32_t idx = 0, nc = 0;
+    stk.reserve(1024);
+    call.reserve(1024);
+    for (int r = 0; r < n; ++r) {
+        if (index[r] != -1) continue;
+        index[r] = low[r] = idx++;
+        stk.push_back(r);
+        on[r] = 1;
+        call.push_back({r, g.row_ptr[r]});
+        while (!call.empty()) {
+            const int v = call.back().first;
+            int e = call.back().second;
+            const int end = g.row_ptr[v + 1];
+            bool descended = false;
+            for (; e < end; ++e) {
+                if (g.cap[e] <= 0) continue;
+                const int w = g.col[e];
+                if (index[w] == -1) {
+                    call.back().second = e + 1;
+                    index[w] = low[w] = idx++;
+                    stk.push_back(w);
+                    on[w] = 1;
+                    call.push_back({w, g.row_ptr[w]});
+                    descended = true;
+                    break;
+                }
+                if (on[w] && index[w] < low[v]) low[v] = index[w];
+            }
+            if (descended) continue;
+            if (low[v] == index[v]) {
+                for (;;) {
+                    const int x = stk.back();
+                    stk.pop_back();
+                    on[x] = 0;
+                    comp[x] = nc;
+                    if (x == v) break;
+                }
+                ++nc;
+            }
+            call.pop_back();
+            if (!call.empty()) {
+                const int u = call.back().first;
+                if (low[v] < low[u]) low[u] = low[v];
+            }
+        }
+    }
+}
+
+void weak_components(const HostGraph& g, std::vector<int32_t>& comp) {
+    const int n = g.n;
+    std::vector<int32_t> par(n);
+    std::iota(par.begin(), par.end(), 0);
+    auto find = [&](int x) {
+        while (par[x] != x) {
+            par[x] = par[par[x]];
+            x = par[x];
+        }
+        return x;
+    };
+    for (int v = 0; v < n; ++v)
+        for (int e = g.row_ptr[v]; e < g.row_ptr[v + 1]; ++e) {
+            int a = find(v), b = find(g.col[e]);
+            if (a != b) { if (a < b) par[b] = a; else par[a] = b; }
+        }
+    comp.assign(n, 0);
+    for (int v = 0; v < n; ++v) comp[v] = find(v);
+}
+
 // --------------------------------------------------------------- generators
 
 namespace {

@@ -60,6 +60,14 @@ void capacity_degree_partition(int n, int R, const std::vector<int64_t>& so,
 // region-ordered CSR and fills new_id/old_id/rstart/region_id.
 void partition_and_reorder(HostGraph& g, int R, const std::vector<int32_t>& part);
 
+// Strongly connected components over entries with cap > 0 (iterative
+// Tarjan) and weakly connected components over all entries (union-find).
+// Two sources in one SCC reach the same vertex set, and the undirected
+// component of t is its WCC, so the pruning mask of BC:1391-1445 is a
+// function of (scc[s], wcc[t]).
+void strong_components(const HostGraph& g, std::vector<int32_t>& comp);
+void weak_components(const HostGraph& g, std::vector<int32_t>& comp);
+
 // ------------------------------------------------------------ generators
 // Seeded, platform-independent generators (splitmix64 -> xoshiro256**).
 // Each writes an edge list with 0-based ids in emission order.

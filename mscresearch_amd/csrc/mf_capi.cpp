@@ -98,6 +98,8 @@ Params from_c(const mf_params* p) {
     P.bu_beta = p->bu_beta;
     P.prune = p->prune;
     P.prune_ratio = p->prune_ratio;
+    P.gr_work = p->gr_work;
+    P.K_heavy = p->K_heavy;
     return P;
 }
 
@@ -213,6 +215,14 @@ int mf_hgraph_find_entry(const mf_hgraph* h, int u, int v, int32_t* entry) {
     if (u < 0 || u >= h->g.n || v < 0 || v >= h->g.n) return fail(MF_EINVAL, "vertex out of range");
     for (int e = h->g.row_ptr[u]; e < h->g.row_ptr[u + 1]; ++e)
         if (h->g.col[e] == v && h->g.cap[e] > 0) { *entry = e; break; }
+    return MF_OK;
+}
+
+int mf_hgraph_components(const mf_hgraph* h, int32_t* scc, int32_t* wcc) {
+    if (!h) return fail(MF_EINVAL, "null graph");
+    std::vector<int32_t> a;
+    if (scc) { strong_components(h->g, a); std::copy(a.begin(), a.end(), scc); }
+    if (wcc) { weak_components(h->g, a); std::copy(a.begin(), a.end(), wcc); }
     return MF_OK;
 }
 
@@ -393,6 +403,8 @@ void mf_params_default(mf_params* p) {
     p->bu_beta = P.bu_beta;
     p->prune = P.prune;
     p->prune_ratio = P.prune_ratio;
+    p->gr_work = P.gr_work;
+    p->K_heavy = P.K_heavy;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {
@@ -564,7 +576,7 @@ static int dbg_bfs(mf_engine* e, Plan first, int mode) {
         p.stages = ST_BFS;
         p.bfs_mode = mode;
         const int64_t mu = std::max<int64_t>(0, (int64_t)g.E - medges);
-        if (!bu && (double)c.fedges[L & 3] > (double)mu / P.bu_alpha && f > kSmallFrontier) bu = true;
+        if (!bu && (double)c.fedges[L & 3] > (double)mu / P.bu_alpha) bu = true;
         else if (bu && (double)f < (double)g.n / P.bu_beta) bu = false;
         if (bu) { p.bfs_kind = BK_BU; p.bfs_grid = std::max(1, std::min(8192, (g.n + 255) / 256)); }
         else if (f <= kSmallFrontier && c.fedges[L & 3] <= kSmallEdges) p.bfs_kind = BK_SMALL;

@@ -31,6 +31,7 @@ struct Ctr {
     int64_t fedges[4];       // row-length sum of the level-L frontier, same ring
     int32_t level;           // levels completed by the running BFS
     int32_t visited;         // vertices labelled by the running BFS
+    int32_t nhl;             // entries in the heavy-row work list
     int32_t active;          // excess>0, h<n, not s/t
     int32_t active_high;     // ... with h > H_cut
     int32_t nlight;          // light active-list entries (front of the list)
@@ -64,18 +65,21 @@ struct SlotView {
     int32_t* fb = nullptr;       // n frontier B
     int32_t* lock = nullptr;     // n path-push ownership (-1 = free)
     int32_t* aux = nullptr;      // n scratch (prune masks, compaction)
+    int32_t* hl = nullptr;       // n heavy-row work list (BFS levels, certificate)
     int32_t* tile_cnt = nullptr; // ntiles active counts
     int32_t* tile_hot = nullptr; // ntiles hot flags
     const int32_t* mask = nullptr;  // optional vertex mask (nullptr = all)
     Ctr* ctr = nullptr;
     int32_t s = 0, t = 0;
-    int32_t K = 4;               // discharge cycles per vertex per pass
+    int32_t K = 4;               // discharge cycles per light vertex per pass
+    int32_t Kh = 1;              // ... per heavy (wave-scanned) vertex per pass
     int32_t H_cut = 100;         // "high" label threshold (BC:1469)
     int32_t phase = 0;           // 0 = normal (seeds t + deficits), 1 = drain (seeds deficits, t active)
 };
 
 constexpr int kTileMax = 4096;      // vertices staged in LDS per workgroup
 constexpr int kLightDeg = 32;       // rows up to this length are scanned by one lane
+constexpr int kGiantDeg = 2048;     // rows longer than this are scanned by a whole workgroup
 constexpr int kSmallFrontier = 16384;  // single-workgroup BFS keeps going below this
 constexpr int64_t kSmallEdges = 1 << 18; // ... and while the frontier's rows stay below this
 

@@ -39,7 +39,10 @@ DeviceGraph::~DeviceGraph() {
     (void)hipFree(d_row_ptr); (void)hipFree(d_col); (void)hipFree(d_cap); (void)hipFree(d_rev);
     (void)hipFree(d_tile_start); (void)hipFree(d_tile_of);
     (void)hipFree(d_sub2old); (void)hipFree(d_old2sub); (void)hipFree(d_mask);
+    (void)hipFree(d_wcc);
 }
+
+DeviceGraph::FwdSet::~FwdSet() { (void)hipFree(d_mask); }
 
 void make_tiles(const std::vector<int32_t>& rstart, int tile_max, std::vector<int32_t>& ts) {
     ts.clear();
@@ -88,17 +91,24 @@ int upload_graph(const HostGraph& hg, int tile_max, DeviceGraph& dg) {
     dg.g.tile_of = dg.d_tile_of;
     dg.tile_max = tile_max;
     dg.bytes = sizeof(int32_t) * ((size_t)n + 1) + (size_t)E * 16 + sizeof(int32_t) * (ts.size() + n);
+    // reachability classes for pruning
+    strong_components(hg, dg.scc);
+    weak_components(hg, dg.wcc);
+    HIPCHK(hipMalloc(&dg.d_wcc, sizeof(int32_t) * std::max(n, 1)));
+    if (n > 0) HIPCHK(hipMemcpy(dg.d_wcc, dg.wcc.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    dg.classes = true;
+    dg.bytes += sizeof(int32_t) * (size_t)n;
     return MF_OK;
 }
 
 // ------------------------------------------------------------------- slabs
 Slab::~Slab() {
     (void)hipFree(res); (void)hipFree(excess); (void)hipFree(h); (void)hipFree(fa); (void)hipFree(fb); (void)hipFree(lock);
-    (void)hipFree(aux); (void)hipFree(tile_cnt); (void)hipFree(tile_hot); (void)hipFree(ctr);
+    (void)hipFree(aux); (void)hipFree(hl); (void)hipFree(tile_cnt); (void)hipFree(tile_hot); (void)hipFree(ctr);
 }
 
 size_t Slab::bytes_for(int n, int64_t E, int ntiles) {
-    return (size_t)E * 8 + (size_t)n * (8 + 4 * 5) + (size_t)ntiles * 8 + sizeof(Ctr);
+    return (size_t)E * 8 + (size_t)n * (8 + 4 * 6) + (size_t)ntiles * 8 + sizeof(Ctr);
 }
 
 int alloc_slab(int n, int64_t E, int ntiles, Slab& sl) {
@@ -111,6 +121,7 @@ int alloc_slab(int n, int64_t E, int ntiles, Slab& sl) {
     HIPCHK(hipMalloc(&sl.fb, 4 * nn));
     HIPCHK(hipMalloc(&sl.lock, 4 * nn));
     HIPCHK(hipMalloc(&sl.aux, 4 * nn));
+    HIPCHK(hipMalloc(&sl.hl, 4 * nn));
     HIPCHK(hipMalloc(&sl.tile_cnt, 4 * tt));
     HIPCHK(hipMalloc(&sl.tile_hot, 4 * tt));
     HIPCHK(hipMalloc(&sl.ctr, sizeof(Ctr)));
@@ -157,6 +168,7 @@ void SlotSched::begin(const Params* P, const DGraph& g, int s, int t, bool prune
     rm_off_ = false;
     just_gr_ = false;
     work_reset_ = false;
+    fwd_only_ = false;
     st_ = prune_first ? S_PRUNE_FWD : (warm ? S_WARM_START : S_START);
 }
 
@@ -171,6 +183,14 @@ void SlotSched::resume_after_prune(const DGraph& g, int s, int t) {
     st_ = S_START;
 }
 
+void SlotSched::finish_trivial(int64_t flow) {
+    stats.flow = flow;
+    stats.cert_ok = 1;
+    stats.n_run = 0;
+    stats.E_run = 0;
+    st_ = S_DONE;
+}
+
 void SlotSched::fail(int code, const std::string& msg) {
     status_ = code;
     err_ = msg;
@@ -178,8 +198,9 @@ void SlotSched::fail(int code, const std::string& msg) {
     stats.status = code;
 }
 
-Plan SlotSched::start_gr() {
+Plan SlotSched::start_gr(const Ctr* c) {
     Plan p;
+    scans_at_gr_ = c ? c->wk[WF_DISCHARGE][0] : 0;
     p.stages = ST_RESET_PRE | ST_GR_SEED | ST_BFS;
     p.reset_pre = RST_BFS;
     p.bfs_kind = BK_SMALL;
@@ -207,7 +228,7 @@ Plan SlotSched::bfs_step(const Ctr& c, int mode) {
     p.stages = ST_BFS;
     p.bfs_mode = mode;
     const int64_t mu = std::max<int64_t>(0, (int64_t)g_.E - bfs_medges_);
-    if (!bfs_bottom_up_ && (double)mf > (double)mu / P_->bu_alpha && f > kSmallFrontier)
+    if (!bfs_bottom_up_ && (double)mf > (double)mu / P_->bu_alpha)
         bfs_bottom_up_ = true;
     else if (bfs_bottom_up_ && (double)f < (double)g_.n / P_->bu_beta)
         bfs_bottom_up_ = false;
@@ -246,6 +267,10 @@ Plan SlotSched::plan(const Ctr& c) {
         if (c.fsize[L & 3] > 0) {
             bfs_medges_ += c.fedges[L & 3];
             return bfs_step(c, bfs_mode_);
+        }
+        if (st_ == S_PRUNE_UND && fwd_only_) {
+            st_ = S_PRUNE_DECIDE;  // the engine combines h with t's WCC
+            return p;
         }
         if (st_ == S_PRUNE_UND) {
             // undirected BFS from t (BC:1419-1433)
@@ -300,7 +325,7 @@ Plan SlotSched::plan(const Ctr& c) {
         if (c.err) { fail(MF_EHIP, "device error flag"); return p; }
         const int active = c.active;
         if (active == 0) {
-            if (!exact) { force_gr_ = false; return start_gr(); }
+            if (!exact) { force_gr_ = false; return start_gr(&c); }
             if (phase_ == 0 && c.neg_count > 0) {
                 // Deficits left (RM or a capacity decrease made them) and no
                 // positive excess can reach t or a deficit: drain them from t.
@@ -308,12 +333,12 @@ Plan SlotSched::plan(const Ctr& c) {
                 phase_ = 1;
                 rm_off_ = true;
                 stats.drains++;
-                return start_gr();
+                return start_gr(&c);
             }
             if (phase_ == 1) {
                 if (c.neg_count > 0) { fail(MF_EITER, "deficits unreachable from t"); return p; }
                 phase_ = 0;
-                return start_gr();
+                return start_gr(&c);
             }
             // Exact termination: no active vertex can reach t.
             if (P_->cert) {
@@ -365,9 +390,13 @@ Plan SlotSched::plan(const Ctr& c) {
             last_ext_ = c.ex_t;
         }
         rm_last_ = false;
+        // GR when due by count (BC:1496), forced, when the flow into t has
+        // stalled, or when the discharge has done about a GR's worth of scans
+        const uint64_t scans = c.wk[WF_DISCHARGE][0] - scans_at_gr_;
         const bool gr_due = force_gr_ || (iter_ - last_gr_iter_ >= GR_FREQ_) ||
-                            (P_->gr_stall > 0 && stall_ >= P_->gr_stall);
-        if (gr_due && !exact) return start_gr();
+                            (P_->gr_stall > 0 && stall_ >= P_->gr_stall) ||
+                            (P_->gr_work > 0 && (double)scans >= P_->gr_work * (double)g_.E);
+        if (gr_due && !exact) return start_gr(&c);
         // discharge step
         if (active > 0 && active <= P_->path_switch) ++path_stable_;
         else { path_stable_ = 0; p_count_ = P_->p_count0; }
@@ -511,15 +540,105 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
     v = SlotView();
     v.g = G.g;
     v.res = sl.res; v.excess = sl.excess; v.h = sl.h; v.fa = sl.fa; v.fb = sl.fb;
-    v.lock = sl.lock; v.aux = sl.aux; v.tile_cnt = sl.tile_cnt; v.tile_hot = sl.tile_hot;
+    v.lock = sl.lock; v.aux = sl.aux; v.hl = sl.hl; v.tile_cnt = sl.tile_cnt; v.tile_hot = sl.tile_hot;
     v.ctr = sl.ctr;
     v.s = q.s; v.t = q.t;
     v.H_cut = P_.h_cut;
+    v.Kh = std::max(1, P_.K_heavy);
     const bool prune = P_.prune != 0 && prune_ok && !warm;
     S.sched.begin(&P_, G.g, q.s, q.t, prune, warm);
     S.sched.stats.t_submit = q.t_arrive;
     S.sched.stats.t_start = now();
     v.K = S.sched.K();
+    if (prune && G.classes) {
+        // with the forward set of s's SCC at hand the prune needs no BFS
+        if (G.fwd_cache.count(G.scc[q.s])) return class_decide(i);
+        S.sched.set_fwd_only();
+    }
+    return MF_OK;
+}
+
+// Pruning from reachability classes: mask = fwd(scc(s)) AND wcc == wcc(t)
+// is the mask of BC:1391-1445 whenever t is reachable from s (then s and t
+// are in it); when t is not reachable the flow is 0 (fwd(s) is a closed set
+// holding s but not t: an s-t cut of capacity 0).
+int Engine::class_decide(int i) {
+    SlotState& S = slots_[i];
+    DeviceGraph& G = *graphs_[S.q.graph];
+    const int cls = G.scc[S.q.s];
+    auto it = G.fwd_cache.find(cls);
+    std::shared_ptr<DeviceGraph::FwdSet> fwd;
+    if (it == G.fwd_cache.end()) {
+        // the slot has just run BFS_FWD from s: h < n is fwd(s)
+        fwd = std::make_shared<DeviceGraph::FwdSet>();
+        const int n = G.g.n;
+        HIPCHK(hipMalloc(&fwd->d_mask, std::max(n, 1)));
+        launch_fwd_from_h(stream_, S.view.h, n, fwd->d_mask);
+        fwd->mask.resize(n);
+        if (n > 0) HIPCHK(hipMemcpyAsync(fwd->mask.data(), fwd->d_mask, n, hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        int64_t cnt = 0;
+        for (int v = 0; v < n; ++v) cnt += fwd->mask[v];
+        fwd->count = cnt;
+        G.fwd_cache[cls] = fwd;
+    } else {
+        fwd = it->second;
+    }
+    if (!fwd->mask[S.q.t]) {
+        S.sched.finish_trivial(0);
+        return MF_OK;
+    }
+    const int wt = G.wcc[S.q.t];
+    const std::string key = std::to_string(S.q.graph) + ":c" + std::to_string(cls) + ":" + std::to_string(wt);
+    std::shared_ptr<DeviceGraph> run;
+    bool known = false;
+    auto ct = cache_.find(key);
+    if (ct != cache_.end()) {
+        run = ct->second.g;
+        ct->second.last_use = ++use_clock_;
+        known = true;
+        ++cache_hits;
+        S.sched.stats.kept = run ? run->g.n : G.g.n;
+    }
+    if (!known) {
+        ++cache_misses;
+        const int n = G.g.n;
+        int64_t kept = 0;
+        for (int v = 0; v < n; ++v) kept += (fwd->mask[v] && G.wcc[v] == wt) ? 1 : 0;
+        S.sched.stats.kept = (int)kept;
+        if (kept > 0 && (double)kept < P_.prune_ratio * (double)n) {
+            launch_class_mask(stream_, fwd->d_mask, G.d_wcc, wt, n, S.view.aux);
+            auto ng = std::make_shared<DeviceGraph>();
+            int rc = build_induced_gpu(G, S.view.aux, (int)kept, stream_, *ng);
+            if (rc != MF_OK) return rc;
+            run = ng;
+            cache_bytes_ += ng->bytes;
+        }
+        while (cache_bytes_ > cache_budget && !cache_.empty()) {
+            auto lru = cache_.begin();
+            for (auto jt = cache_.begin(); jt != cache_.end(); ++jt)
+                if (jt->second.last_use < lru->second.last_use) lru = jt;
+            if (lru->second.g) cache_bytes_ -= lru->second.g->bytes;
+            cache_.erase(lru);
+        }
+        cache_[key] = CacheEntry{run, ++use_clock_};
+    }
+    if (run) {
+        const int s = run->old2sub_host.empty() ? -1 : run->old2sub_host[S.q.s];
+        const int t = run->old2sub_host.empty() ? -1 : run->old2sub_host[S.q.t];
+        if (s >= 0 && t >= 0) {
+            S.run_graph = run;
+            S.view.g = run->g;
+            S.view.s = s;
+            S.view.t = t;
+            S.sched.stats.pruned = 1;
+            S.sched.resume_after_prune(run->g, s, t);
+            S.view.K = S.sched.K();
+            return MF_OK;
+        }
+    }
+    S.sched.resume_after_prune(G.g, S.q.s, S.q.t);
+    S.view.K = S.sched.K();
     return MF_OK;
 }
 
@@ -527,6 +646,7 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
 // drops at least (1 - prune_ratio) of the vertices.
 int Engine::prune_decide(int i) {
     SlotState& S = slots_[i];
+    if (graphs_[S.q.graph]->classes) return class_decide(i);
     const DeviceGraph& G = *graphs_[S.q.graph];
     const Ctr& c = h_ctr_[i];
     const int kept = c.visited;

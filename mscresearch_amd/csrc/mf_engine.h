@@ -27,7 +27,7 @@ struct Params {
     int K = 0;                 // cycles per vertex per discharge pass; 0 = max(4, E/V)
     int gr_freq = 128;         // GR every gr_freq iterations (BC:1466)
     int gr_freq_min = 128, gr_freq_max = 256;
-    int gr_stall = 0;          // also GR after excess[t] stalls this many iterations (0 = off)
+    int gr_stall = 3;          // also GR after excess[t] stalls this many iterations (0 = off)
     int h_cut = 100;           // BC:1469
     int stagnate_rounds = 128; // BC:1472
     int path_switch = 8;       // BC:1480
@@ -45,6 +45,12 @@ struct Params {
     double bu_alpha = 14.0, bu_beta = 24.0;  // direction-optimizing BFS thresholds
     int prune = 1;             // per-query reachability pruning (BC:2678)
     double prune_ratio = 0.995;  // BC:2647
+    // also GR once the discharge has scanned gr_work x E entries since the
+    // last GR (a GR costs about E scans; 0 = off)
+    double gr_work = 1.0;
+    // discharge cycles per pass for rows scanned by a whole wave (light rows
+    // use K): one full scan of a hub row is already a long pass
+    int K_heavy = 1;
 };
 
 // ---------------------------------------------------------- device memory
@@ -61,8 +67,28 @@ struct DeviceGraph {
     uint8_t* d_mask = nullptr;     // pruned run graphs: parent mask (cache check)
     size_t bytes = 0;
     int tile_max = kTileMax;
+    // Reachability classes for pruning: scc (over cap > 0 entries) and wcc
+    // ids on the host, wcc ids on the device, and the forward-reach set of
+    // each SCC once some query has needed it.
+    bool classes = false;
+    std::vector<int32_t> scc, wcc;
+    int32_t* d_wcc = nullptr;
+    struct FwdSet {
+        std::vector<uint8_t> mask;  // host copy
+        uint8_t* d_mask = nullptr;
+        int64_t count = 0;
+        ~FwdSet();
+    };
+    std::unordered_map<int32_t, std::shared_ptr<FwdSet>> fwd_cache;
+    std::vector<int32_t> old2sub_host;  // pruned run graphs: host copy of d_old2sub
     ~DeviceGraph();
 };
+
+// mask8[v] = (h[v] < n), the forward-reach set left by a BFS_FWD in h.
+void launch_fwd_from_h(hipStream_t st, const int32_t* h, int n, uint8_t* mask8);
+// mask[v] = fwd[v] && wcc[v] == wt  (int32 0/1)
+void launch_class_mask(hipStream_t st, const uint8_t* fwd, const int32_t* wcc, int32_t wt, int n,
+                       int32_t* mask);
 
 // Tiles: each region [rstart[r], rstart[r+1]) cut into pieces of at most tile_max.
 void make_tiles(const std::vector<int32_t>& rstart, int tile_max, std::vector<int32_t>& tile_start);
@@ -78,6 +104,7 @@ struct Slab {
     int32_t* fb = nullptr;
     int32_t* lock = nullptr;
     int32_t* aux = nullptr;
+    int32_t* hl = nullptr;
     int32_t* tile_cnt = nullptr;
     int32_t* tile_hot = nullptr;
     Ctr* ctr = nullptr;
@@ -152,11 +179,15 @@ public:
     // pruning hand-off: after S_MASK the engine swaps graphs and calls resume()
     bool wants_prune_decision() const { return st_ == S_PRUNE_DECIDE; }
     void resume_after_prune(const DGraph& g, int s, int t);
+    // Only the forward BFS is needed (the engine has the WCC of t).
+    void set_fwd_only() { fwd_only_ = true; }
+    // The answer is known without a solve (t not reachable from s).
+    void finish_trivial(int64_t flow);
 
 private:
     enum St { S_PRUNE_FWD, S_PRUNE_UND, S_PRUNE_MASK, S_PRUNE_DECIDE, S_START, S_WARM_START, S_BFS,
               S_COUNTED, S_CERT, S_DONE, S_ERROR };
-    Plan start_gr();
+    Plan start_gr(const Ctr* c = nullptr);
     Plan bfs_step(const Ctr& c, int mode);
     void fail(int code, const std::string& msg);
 
@@ -178,11 +209,13 @@ private:
     bool force_gr_ = false;
     int path_stable_ = 0, p_count_ = 64, path_rounds_ = 0;
     int64_t last_ext_ = -1;
+    uint64_t scans_at_gr_ = 0;  // discharge edge scans when the last GR started
     int stall_ = 0;
     int drain_loops_ = 0;
     bool rm_off_ = false;
     bool rm_last_ = false;
     bool work_reset_ = false;
+    bool fwd_only_ = false;
 };
 
 // ------------------------------------------------------------------ engine
@@ -250,6 +283,7 @@ private:
     };
     int admit(int slot, const Query& q, bool warm = false, bool prune = true);
     int prune_decide(int slot);
+    int class_decide(int slot);
     int launch_plans(std::vector<Plan>& plans);
 
     Params P_;

@@ -119,6 +119,17 @@ __global__ void k_tiles(const int32_t* __restrict__ ptile_start, const uint32_t*
         tile_start2[j] = (int32_t)vscan[ptile_start[j]];
 }
 
+__global__ void k_fwd_from_h(const int32_t* __restrict__ h, int n, uint8_t* __restrict__ mask8) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+        mask8[v] = h[v] < n ? 1 : 0;
+}
+
+__global__ void k_class_mask(const uint8_t* __restrict__ fwd, const int32_t* __restrict__ wcc, int32_t wt,
+                             int n, int32_t* __restrict__ mask) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+        mask[v] = (fwd[v] && wcc[v] == wt) ? 1 : 0;
+}
+
 inline int grid_for(int64_t items, int per_block) {
     int64_t g = (items + per_block - 1) / per_block;
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, 16384));
@@ -134,6 +145,15 @@ inline int grid_for(int64_t items, int per_block) {
             return e_ == hipErrorOutOfMemory ? MF_ENOMEM : MF_EHIP;                      \
         }                                                                                \
     } while (0)
+
+void launch_fwd_from_h(hipStream_t st, const int32_t* h, int n, uint8_t* mask8) {
+    hipLaunchKernelGGL(k_fwd_from_h, dim3(grid_for(n, 256)), dim3(256), 0, st, h, n, mask8);
+}
+
+void launch_class_mask(hipStream_t st, const uint8_t* fwd, const int32_t* wcc, int32_t wt, int n,
+                       int32_t* mask) {
+    hipLaunchKernelGGL(k_class_mask, dim3(grid_for(n, 256)), dim3(256), 0, st, fwd, wcc, wt, n, mask);
+}
 
 int build_induced_gpu(const DeviceGraph& P, const int32_t* d_mask, int kept, hipStream_t st,
                       DeviceGraph& out) {
@@ -188,6 +208,9 @@ int build_induced_gpu(const DeviceGraph& P, const int32_t* d_mask, int kept, hip
     hipLaunchKernelGGL(k_tiles, dim3(grid_for(ntiles + 1, 256)), dim3(256), 0, st, P.g.tile_start, vscan,
                        ntiles, out.d_tile_start);
     HIPCHK(hipGetLastError());
+    out.old2sub_host.resize(n);
+    if (n > 0)
+        HIPCHK(hipMemcpyAsync(out.old2sub_host.data(), out.d_old2sub, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     (void)hipFree(vflag); (void)hipFree(vscan); (void)hipFree(kdeg); (void)hipFree(escan); (void)hipFree(e2s); (void)hipFree(tmp);
     out.g.n = k;

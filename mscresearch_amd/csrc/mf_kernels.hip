@@ -26,27 +26,37 @@ namespace {
 
 constexpr int WAVE = 64;
 
-__device__ __forceinline__ int64_t ld64(const int64_t* p) {
+template <typename P>
+__device__ __forceinline__ int64_t ld64(P p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ int64_t add64(int64_t* p, int64_t v) {
+template <typename P>
+__device__ __forceinline__ int64_t add64(P p, int64_t v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ int64_t add64_lds(int64_t* p, int64_t v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-__device__ __forceinline__ int32_t add32(int32_t* p, int32_t v) {
+template <typename P>
+__device__ __forceinline__ int32_t add32(P p, int32_t v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ uint64_t addu64(uint64_t* p, uint64_t v) {
+template <typename P>
+__device__ __forceinline__ uint64_t addu64(P p, uint64_t v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ int ld32(const int32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+template <typename P>
+__device__ __forceinline__ bool cas32(P p, int32_t expect, int32_t want) {
+    return __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ bool cas32(int32_t* p, int32_t expect, int32_t want) {
-    return __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED,
-                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+template <typename P>
+__device__ __forceinline__ void min32(P p, int32_t v) {
+    __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename P>
+__device__ __forceinline__ void max32(P p, int32_t v) {
+    __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
@@ -87,8 +97,76 @@ __device__ __forceinline__ T block_sum(T v, T* red) {
     return s;
 }
 
-__device__ __forceinline__ SlotView load_slot(const SlotView* slots, const int* list) {
-    return slots[list[blockIdx.y]];
+// Every pointer of a slot is device global memory.  Kernels see the slot
+// through GSlot, whose pointers are typed address_space(1): the compiler then
+// emits global (not flat) loads, stores and atomics, without the lgkmcnt wait
+// each flat access carries in an inner loop.
+#define GAS __attribute__((address_space(1)))
+
+struct GGraph {
+    int n, E, ntiles, R;
+    const GAS int32_t* row_ptr;
+    const GAS int32_t* col;
+    const GAS int64_t* cap;
+    const GAS int32_t* rev;
+    const GAS int32_t* tile_start;
+    const GAS int32_t* tile_of;
+};
+
+struct GSlot {
+    GGraph g;
+    GAS int64_t* res;
+    GAS int64_t* excess;
+    GAS int32_t* h;
+    GAS int32_t* fa;
+    GAS int32_t* fb;
+    GAS int32_t* lock;
+    GAS int32_t* aux;
+    GAS int32_t* hl;
+    GAS int32_t* tile_cnt;
+    GAS int32_t* tile_hot;
+    const GAS int32_t* mask;
+    GAS Ctr* ctr;
+    int32_t s, t, K, Kh, H_cut, phase;
+};
+
+template <typename T>
+__device__ __forceinline__ GAS T* as_global(T* p) {
+    return (GAS T*)p;
+}
+
+__device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* list) {
+    const SlotView v = slots[list[blockIdx.y]];
+    GSlot S;
+    S.g.n = v.g.n;
+    S.g.E = v.g.E;
+    S.g.ntiles = v.g.ntiles;
+    S.g.R = v.g.R;
+    S.g.row_ptr = as_global(v.g.row_ptr);
+    S.g.col = as_global(v.g.col);
+    S.g.cap = as_global(v.g.cap);
+    S.g.rev = as_global(v.g.rev);
+    S.g.tile_start = as_global(v.g.tile_start);
+    S.g.tile_of = as_global(v.g.tile_of);
+    S.res = as_global(v.res);
+    S.excess = as_global(v.excess);
+    S.h = as_global(v.h);
+    S.fa = as_global(v.fa);
+    S.fb = as_global(v.fb);
+    S.lock = as_global(v.lock);
+    S.aux = as_global(v.aux);
+    S.hl = as_global(v.hl);
+    S.tile_cnt = as_global(v.tile_cnt);
+    S.tile_hot = as_global(v.tile_hot);
+    S.mask = as_global(v.mask);
+    S.ctr = as_global(v.ctr);
+    S.s = v.s;
+    S.t = v.t;
+    S.K = v.K;
+    S.Kh = v.Kh;
+    S.H_cut = v.H_cut;
+    S.phase = v.phase;
+    return S;
 }
 
 // LDS-staged append buffer: entries are collected with LDS atomics and
@@ -108,8 +186,8 @@ __device__ __forceinline__ void stage_init(Stage<CAP>& st) {
 }
 
 template <int CAP>
-__device__ __forceinline__ void stage_push(Stage<CAP>& st, int32_t item, int deg, int32_t* gbuf,
-                                           int32_t* gcnt, int64_t* gdeg) {
+__device__ __forceinline__ void stage_push(Stage<CAP>& st, int32_t item, int deg, GAS int32_t* gbuf,
+                                           GAS int32_t* gcnt, GAS int64_t* gdeg) {
     const int pos = atomicAdd(&st.cnt, 1);
     if (pos < CAP) {
         st.buf[pos] = item;
@@ -122,8 +200,8 @@ __device__ __forceinline__ void stage_push(Stage<CAP>& st, int32_t item, int deg
 }
 
 template <int CAP>
-__device__ __forceinline__ void stage_flush(Stage<CAP>& st, int32_t* gbuf, int32_t* gcnt,
-                                            int64_t* gdeg) {
+__device__ __forceinline__ void stage_flush(Stage<CAP>& st, GAS int32_t* gbuf, GAS int32_t* gcnt,
+                                            GAS int64_t* gdeg) {
     __syncthreads();
     const int n = min(st.cnt, CAP);
     if (threadIdx.x == 0) {
@@ -138,14 +216,14 @@ struct Work {
     uint64_t edges = 0, verts = 0, pushes = 0, relabels = 0;
 };
 
-__device__ __forceinline__ void work_flush(const SlotView& S, Work w, uint64_t* red, int fam) {
+__device__ __forceinline__ void work_flush(const GSlot& S, Work w, uint64_t* red, int fam) {
     // Four block sums, one atomic each from thread 0.
     const uint64_t e = block_sum<uint64_t>(w.edges, red);
     const uint64_t v = block_sum<uint64_t>(w.verts, red);
     const uint64_t p = block_sum<uint64_t>(w.pushes, red);
     const uint64_t r = block_sum<uint64_t>(w.relabels, red);
     if (threadIdx.x == 0) {
-        uint64_t* c = S.ctr->wk[fam];
+        GAS uint64_t* c = S.ctr->wk[fam];
         if (e) addu64(&c[0], e);
         if (v) addu64(&c[1], v);
         if (p) addu64(&c[2], p);
@@ -159,12 +237,13 @@ __device__ __forceinline__ void work_flush(const SlotView& S, Work w, uint64_t* 
 
 __global__ void k_reset(const SlotView* __restrict__ slots, const int* __restrict__ list,
                         unsigned what) {
-    const SlotView S = load_slot(slots, list);
-    Ctr* c = S.ctr;
+    const GSlot S = load_slot(slots, list);
+    GAS Ctr* c = S.ctr;
     if (threadIdx.x == 0) {
         if (what & RST_BFS) {
             for (int i = 0; i < 4; ++i) { c->fsize[i] = 0; c->fedges[i] = 0; }
             c->level = 0;
+            c->nhl = 0;
             c->visited = 0;
         }
         if (what & RST_ACTIVE) {
@@ -172,7 +251,7 @@ __global__ void k_reset(const SlotView* __restrict__ slots, const int* __restric
             c->pos_excess = 0; c->neg_count = 0; c->neg_sum = 0;
             c->hmin = INT_MAX; c->hmax = -1;
         }
-        if (what & RST_CERT) { c->cert_bad = 0; c->cut = 0; }
+        if (what & RST_CERT) { c->cert_bad = 0; c->cut = 0; c->nhl = 0; }
         if (what & RST_RM) c->fixed_edges = 0;
         if (what & RST_WORK)
             for (int f = 0; f < kFamilies; ++f)
@@ -192,7 +271,7 @@ void launch_reset(hipStream_t st, const SlotView* slots, const int* list, int nl
 
 __global__ __launch_bounds__(256) void k_init(const SlotView* __restrict__ slots,
                                               const int* __restrict__ list) {
-    const SlotView S = load_slot(slots, list);
+    const GSlot S = load_slot(slots, list);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (int64_t e = i0; e < S.g.E; e += stride) S.res[e] = S.g.cap[e];
@@ -217,7 +296,7 @@ void launch_init(hipStream_t st, const SlotView* slots, const int* list, int nli
 
 __global__ __launch_bounds__(256) void k_preflow(const SlotView* __restrict__ slots,
                                                  const int* __restrict__ list) {
-    const SlotView S = load_slot(slots, list);
+    const GSlot S = load_slot(slots, list);
     __shared__ int64_t red[256 / WAVE];
     const int s = S.s;
     const int rb = S.g.row_ptr[s], re = S.g.row_ptr[s + 1];
@@ -252,13 +331,13 @@ void launch_preflow(hipStream_t st, const SlotView* slots, const int* list, int 
 //            visited marker aux == n.
 
 template <int MODE>
-__device__ __forceinline__ int32_t* bfs_lvl(const SlotView& S) {
+__device__ __forceinline__ GAS int32_t* bfs_lvl(const GSlot& S) {
     return MODE == BFS_UND ? S.aux : S.h;
 }
 
 // May frontier vertex v (entry e: v -> w) discover w?
 template <int MODE>
-__device__ __forceinline__ bool bfs_edge_ok(const SlotView& S, int e, int w) {
+__device__ __forceinline__ bool bfs_edge_ok(const GSlot& S, int e, int w) {
     if (MODE == BFS_GR) {
         if (w == S.s) return false;
         if (S.mask && S.mask[w] == 0) return false;
@@ -274,14 +353,14 @@ __device__ __forceinline__ bool bfs_edge_ok(const SlotView& S, int e, int w) {
 // capacity, i.e. e is a reverse entry (cap[e] == 0; forward entries always
 // have cap > 0 and reverse entries cap 0).  UND: any entry.
 template <int MODE>
-__device__ __forceinline__ bool bfs_bu_edge_ok(const SlotView& S, int e) {
+__device__ __forceinline__ bool bfs_bu_edge_ok(const GSlot& S, int e) {
     if (MODE == BFS_GR) return ld64(&S.res[e]) > 0;
     if (MODE == BFS_FWD) return S.g.cap[e] == 0;
     return true;
 }
 
 template <int MODE>
-__device__ __forceinline__ bool bfs_vertex_ok(const SlotView& S, int v) {
+__device__ __forceinline__ bool bfs_vertex_ok(const GSlot& S, int v) {
     if (MODE == BFS_GR) {
         if (v == S.s) return false;
         if (S.mask && S.mask[v] == 0) return false;
@@ -293,7 +372,7 @@ __device__ __forceinline__ bool bfs_vertex_ok(const SlotView& S, int v) {
 // deficits only (phase 1, draining from t), under the mask; s stays n.
 __global__ __launch_bounds__(256) void k_gr_seed(const SlotView* __restrict__ slots,
                                                  const int* __restrict__ list) {
-    const SlotView S = load_slot(slots, list);
+    const GSlot S = load_slot(slots, list);
     __shared__ Stage<1024> st;
     __shared__ uint64_t red[256 / WAVE];
     stage_init(st);
@@ -330,8 +409,8 @@ void launch_gr_seed(hipStream_t st, const SlotView* slots, const int* list, int 
 // FWD starts at s, UND at t.
 __global__ __launch_bounds__(256) void k_bfs_seed1(const SlotView* __restrict__ slots,
                                                    const int* __restrict__ list, int mode) {
-    const SlotView S = load_slot(slots, list);
-    int32_t* lvl = (mode == BFS_UND) ? S.aux : S.h;
+    const GSlot S = load_slot(slots, list);
+    GAS int32_t* lvl = (mode == BFS_UND) ? S.aux : S.h;
     const int src = (mode == BFS_UND) ? S.t : S.s;
     const int n = S.g.n;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -352,9 +431,9 @@ void launch_bfs_seed1(hipStream_t st, const SlotView* slots, const int* list, in
 
 // Top-down expansion of one frontier vertex by one lane.
 template <int MODE, int CAP>
-__device__ __forceinline__ void td_lane(const SlotView& S, int v, int L, int32_t* lvl,
-                                        Stage<CAP>& st, int32_t* out, int32_t* ocnt,
-                                        int64_t* odeg, Work& wk) {
+__device__ __forceinline__ void td_lane(const GSlot& S, int v, int L, GAS int32_t* lvl,
+                                        Stage<CAP>& st, GAS int32_t* out, GAS int32_t* ocnt,
+                                        GAS int64_t* odeg, Work& wk) {
     const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
     const int n = S.g.n;
     wk.edges += (uint64_t)(re - rb);
@@ -372,9 +451,9 @@ __device__ __forceinline__ void td_lane(const SlotView& S, int v, int L, int32_t
 
 // Top-down expansion of one (heavy) frontier vertex by the whole workgroup.
 template <int MODE, int CAP>
-__device__ __forceinline__ void td_block(const SlotView& S, int v, int L, int32_t* lvl,
-                                         Stage<CAP>& st, int32_t* out, int32_t* ocnt,
-                                         int64_t* odeg, Work& wk) {
+__device__ __forceinline__ void td_block(const GSlot& S, int v, int L, GAS int32_t* lvl,
+                                         Stage<CAP>& st, GAS int32_t* out, GAS int32_t* ocnt,
+                                         GAS int64_t* odeg, Work& wk) {
     const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
     const int n = S.g.n;
     if (threadIdx.x == 0) wk.edges += (uint64_t)(re - rb);
@@ -393,20 +472,17 @@ __device__ __forceinline__ void td_block(const SlotView& S, int v, int L, int32_
 template <int MODE>
 __global__ __launch_bounds__(256) void k_bfs_td(const SlotView* __restrict__ slots,
                                                 const int* __restrict__ list) {
-    const SlotView S = load_slot(slots, list);
+    const GSlot S = load_slot(slots, list);
     __shared__ Stage<2048> st;
-    __shared__ int heavy[256];
-    __shared__ int nheavy;
     __shared__ uint64_t red[256 / WAVE];
     const int L = S.ctr->level;
     const int fsz = S.ctr->fsize[L & 3];
     if ((int)(blockIdx.x * blockDim.x) >= fsz) return;  // uniform per block
-    int32_t* lvl = bfs_lvl<MODE>(S);
-    const int32_t* in = (L & 1) ? S.fb : S.fa;
-    int32_t* out = (L & 1) ? S.fa : S.fb;
-    int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
-    int64_t* odeg = &S.ctr->fedges[(L + 1) & 3];
-    if (threadIdx.x == 0) nheavy = 0;
+    GAS int32_t* lvl = bfs_lvl<MODE>(S);
+    const GAS int32_t* in = (L & 1) ? S.fb : S.fa;
+    GAS int32_t* out = (L & 1) ? S.fa : S.fb;
+    GAS int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
+    GAS int64_t* odeg = &S.ctr->fedges[(L + 1) & 3];
     stage_init(st);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         S.ctr->fsize[(L + 2) & 3] = 0;
@@ -420,13 +496,34 @@ __global__ __launch_bounds__(256) void k_bfs_td(const SlotView* __restrict__ slo
         wk.verts++;
         const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
         if (deg <= kLightDeg) td_lane<MODE>(S, v, L, lvl, st, out, ocnt, odeg, wk);
-        else heavy[atomicAdd(&nheavy, 1)] = v;
+        else S.hl[add32(&S.ctr->nhl, 1)] = v;  // k_bfs_td_heavy spreads these over the grid
     }
+    stage_flush(st, out, ocnt, odeg);
+    const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
+    if (threadIdx.x == 0 && nv) add32(&S.ctr->visited, (int32_t)nv);
+    work_flush(S, wk, red, WF_BFS);
+}
+
+// Heavy rows of a top-down level: one row per workgroup at a time, rows taken
+// grid-stride from the level's heavy list.  Hubs sit together at region
+// starts, so handing them out across the grid is what balances the level.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bfs_td_heavy(const SlotView* __restrict__ slots,
+                                                      const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ Stage<2048> st;
+    __shared__ uint64_t red[256 / WAVE];
+    const int nh = S.ctr->nhl;
+    if ((int)blockIdx.x >= nh) return;  // uniform per block
+    const int L = S.ctr->level;
+    GAS int32_t* lvl = bfs_lvl<MODE>(S);
+    GAS int32_t* out = (L & 1) ? S.fa : S.fb;
+    GAS int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
+    GAS int64_t* odeg = &S.ctr->fedges[(L + 1) & 3];
+    stage_init(st);
     __syncthreads();
-    // heavy rows: the whole workgroup strides one row after another
-    const int nh = nheavy;
-    for (int k = 0; k < nh; ++k)
-        td_block<MODE>(S, heavy[k], L, lvl, st, out, ocnt, odeg, wk);
+    Work wk;
+    for (int k = blockIdx.x; k < nh; k += gridDim.x) td_block<MODE>(S, S.hl[k], L, lvl, st, out, ocnt, odeg, wk);
     stage_flush(st, out, ocnt, odeg);
     const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
     if (threadIdx.x == 0 && nv) add32(&S.ctr->visited, (int32_t)nv);
@@ -436,14 +533,14 @@ __global__ __launch_bounds__(256) void k_bfs_td(const SlotView* __restrict__ slo
 template <int MODE>
 __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slots,
                                                 const int* __restrict__ list) {
-    const SlotView S = load_slot(slots, list);
+    const GSlot S = load_slot(slots, list);
     __shared__ Stage<2048> st;
     __shared__ uint64_t red[256 / WAVE];
     const int L = S.ctr->level;
-    int32_t* lvl = bfs_lvl<MODE>(S);
-    int32_t* out = (L & 1) ? S.fa : S.fb;
-    int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
-    int64_t* odeg = &S.ctr->fedges[(L + 1) & 3];
+    GAS int32_t* lvl = bfs_lvl<MODE>(S);
+    GAS int32_t* out = (L & 1) ? S.fa : S.fb;
+    GAS int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
+    GAS int64_t* odeg = &S.ctr->fedges[(L + 1) & 3];
     stage_init(st);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         S.ctr->fsize[(L + 2) & 3] = 0;
@@ -459,6 +556,10 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
         if (!bfs_vertex_ok<MODE>(S, v)) continue;
         wk.verts++;
         const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+        if (re - rb > kGiantDeg) {  // k_bfs_bu_heavy scans it with a whole workgroup
+            S.hl[add32(&S.ctr->nhl, 1)] = v;
+            continue;
+        }
         int e = rb;
         for (; e < re; ++e) {
             if (lvl[S.g.col[e]] != L) continue;
@@ -477,25 +578,92 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
     work_flush(S, wk, red, WF_BFS);
 }
 
+// Giant unvisited rows of a bottom-up level: a workgroup scans the row in
+// chunks and stops at the first chunk holding a parent.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bfs_bu_heavy(const SlotView* __restrict__ slots,
+                                                      const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ Stage<256> st;
+    __shared__ uint64_t red[256 / WAVE];
+    __shared__ int found;
+    const int nh = S.ctr->nhl;
+    if ((int)blockIdx.x >= nh) return;  // uniform per block
+    const int L = S.ctr->level;
+    GAS int32_t* lvl = bfs_lvl<MODE>(S);
+    GAS int32_t* out = (L & 1) ? S.fa : S.fb;
+    GAS int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
+    GAS int64_t* odeg = &S.ctr->fedges[(L + 1) & 3];
+    stage_init(st);
+    Work wk;
+    for (int k = blockIdx.x; k < nh; k += gridDim.x) {
+        const int v = S.hl[k];
+        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+        if (threadIdx.x == 0) found = 0;
+        __syncthreads();
+        int base = rb;
+        for (; base < re; base += blockDim.x) {
+            const int e = base + (int)threadIdx.x;
+            if (e < re && lvl[S.g.col[e]] == L && bfs_bu_edge_ok<MODE>(S, e)) found = 1;
+            __syncthreads();
+            const int f = found;
+            __syncthreads();
+            if (f) break;
+        }
+        if (threadIdx.x == 0) {
+            wk.edges += (uint64_t)(min(base + (int)blockDim.x, re) - rb);
+            if (found) {
+                lvl[v] = L + 1;
+                stage_push(st, v, re - rb, out, ocnt, odeg);
+                wk.relabels++;
+            }
+        }
+        __syncthreads();
+    }
+    stage_flush(st, out, ocnt, odeg);
+    const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
+    if (threadIdx.x == 0 && nv) add32(&S.ctr->visited, (int32_t)nv);
+    work_flush(S, wk, red, WF_BFS);
+}
+
 // Level bookkeeping after a multi-workgroup level: level += 1.
 __global__ void k_bfs_advance(const SlotView* __restrict__ slots, const int* __restrict__ list) {
-    const SlotView S = load_slot(slots, list);
-    if (threadIdx.x == 0) S.ctr->level = S.ctr->level + 1;
+    const GSlot S = load_slot(slots, list);
+    if (threadIdx.x == 0) {
+        S.ctr->level = S.ctr->level + 1;
+        S.ctr->nhl = 0;
+    }
 }
 
 void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
                    int mode) {
-    if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_td<BFS_GR>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
-    else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_td<BFS_FWD>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
-    else hipLaunchKernelGGL(k_bfs_td<BFS_UND>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    const int gh = gridx < 2048 ? gridx : 2048;
+    if (mode == BFS_GR) {
+        hipLaunchKernelGGL(k_bfs_td<BFS_GR>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_td_heavy<BFS_GR>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
+    } else if (mode == BFS_FWD) {
+        hipLaunchKernelGGL(k_bfs_td<BFS_FWD>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_td_heavy<BFS_FWD>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
+    } else {
+        hipLaunchKernelGGL(k_bfs_td<BFS_UND>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_td_heavy<BFS_UND>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
+    }
     hipLaunchKernelGGL(k_bfs_advance, dim3(1, nlist), dim3(64), 0, st, slots, list);
 }
 
 void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
                    int mode) {
-    if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_bu<BFS_GR>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
-    else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_bu<BFS_FWD>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
-    else hipLaunchKernelGGL(k_bfs_bu<BFS_UND>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    const int gh = gridx < 2048 ? gridx : 2048;
+    if (mode == BFS_GR) {
+        hipLaunchKernelGGL(k_bfs_bu<BFS_GR>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_bu_heavy<BFS_GR>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
+    } else if (mode == BFS_FWD) {
+        hipLaunchKernelGGL(k_bfs_bu<BFS_FWD>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_bu_heavy<BFS_FWD>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
+    } else {
+        hipLaunchKernelGGL(k_bfs_bu<BFS_UND>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_bu_heavy<BFS_UND>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
+    }
     hipLaunchKernelGGL(k_bfs_advance, dim3(1, nlist), dim3(64), 0, st, slots, list);
 }
 
@@ -506,13 +674,13 @@ void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int n
 template <int MODE>
 __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__ slots,
                                                     const int* __restrict__ list, int max_levels) {
-    const SlotView S = load_slot(slots, list);
+    const GSlot S = load_slot(slots, list);
     __shared__ int heavy[1024];
     __shared__ int nheavy;
     __shared__ int next_cnt;
     __shared__ int64_t next_deg;
     __shared__ uint64_t red[1024 / WAVE];
-    int32_t* lvl = bfs_lvl<MODE>(S);
+    GAS int32_t* lvl = bfs_lvl<MODE>(S);
     const int n = S.g.n;
     int L = S.ctr->level;
     int fsz = S.ctr->fsize[L & 3];
@@ -520,8 +688,8 @@ __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__
     int done_levels = 0;
     int64_t fdeg = S.ctr->fedges[L & 3];
     while (fsz > 0 && fsz <= kSmallFrontier && fdeg <= kSmallEdges && done_levels < max_levels) {
-        const int32_t* in = (L & 1) ? S.fb : S.fa;
-        int32_t* out = (L & 1) ? S.fa : S.fb;
+        const GAS int32_t* in = (L & 1) ? S.fb : S.fa;
+        GAS int32_t* out = (L & 1) ? S.fa : S.fb;
         if (threadIdx.x == 0) { nheavy = 0; next_cnt = 0; next_deg = 0; }
         __syncthreads();
         for (int i = threadIdx.x; i < fsz; i += blockDim.x) {
@@ -598,7 +766,7 @@ void launch_bfs_small(hipStream_t st, const SlotView* slots, const int* list, in
 
 __global__ __launch_bounds__(256) void k_active(const SlotView* __restrict__ slots,
                                                 const int* __restrict__ list) {
-    const SlotView S = load_slot(slots, list);
+    const GSlot S = load_slot(slots, list);
     __shared__ Stage<1024> st;
     __shared__ int64_t red64[256 / WAVE];
     __shared__ uint64_t redu[256 / WAVE];
@@ -627,16 +795,16 @@ __global__ __launch_bounds__(256) void k_active(const SlotView* __restrict__ slo
         if (hv > S.H_cut) ++high;
         hmin = min(hmin, hv);
         hmax = max(hmax, hv);
-        atomicAdd(&S.tile_cnt[S.g.tile_of[v]], 1);
+        add32(&S.tile_cnt[S.g.tile_of[v]], 1);
         const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
         if (deg <= kLightDeg) {
-            stage_push(st, v, 0, S.fa, &S.ctr->nlight, (int64_t*)nullptr);
+            stage_push(st, v, 0, S.fa, &S.ctr->nlight, (GAS int64_t*)nullptr);
         } else {
             const int k = add32(&S.ctr->nheavy, 1);
             S.fa[n - 1 - k] = v;
         }
     }
-    stage_flush(st, S.fa, &S.ctr->nlight, (int64_t*)nullptr);
+    stage_flush(st, S.fa, &S.ctr->nlight, (GAS int64_t*)nullptr);
     hmin = wave_min_i32(hmin);
     hmax = wave_max_i32(hmax);
     if (lane_id() == 0) { atomicMin(&bmin, hmin); atomicMax(&bmax, hmax); }
@@ -653,7 +821,7 @@ __global__ __launch_bounds__(256) void k_active(const SlotView* __restrict__ slo
         if (bhigh) add32(&S.ctr->active_high, (int32_t)bhigh);
         if (bnegc) add32(&S.ctr->neg_count, (int32_t)bnegc);
         if (bnv) addu64(&S.ctr->wk[WF_ACTIVE][1], bnv);
-        if (bmax >= 0) { atomicMin(&S.ctr->hmin, bmin); atomicMax(&S.ctr->hmax, bmax); }
+        if (bmax >= 0) { min32(&S.ctr->hmin, bmin); max32(&S.ctr->hmax, bmax); }
     }
 }
 
@@ -662,11 +830,12 @@ void launch_active(hipStream_t st, const SlotView* slots, const int* list, int n
 }
 
 // ------------------------------------------------------------- discharge (a16)
-// One discharge cycle of the reference (BC:467-511): find the lowest
-// neighbour over residual entries; if it is lower than v, push; else
-// relabel to lowest + 1.  The push sends min(excess, res) to every entry at
-// that lowest height in row order until the excess is gone, which is a
-// sequence of the reference's single pushes done in one scan.
+// One discharge cycle (the reference's cycle is BC:467-511): find the lowest
+// neighbour over residual entries; if v is not above it, relabel v to lowest
+// + 1 (the reference's relabel); then push min(excess, res) to every entry at
+// that lowest height in row order until the excess is gone.  A cycle is a
+// relabel followed by the reference's single pushes, done in two row scans,
+// so excess moves on the cycle that relabels instead of the next one.
 
 struct TileCtx {
     int a = 0, b = 0;          // staged tile [a, b); b == a means list mode
@@ -674,24 +843,24 @@ struct TileCtx {
     int64_t* ldx = nullptr;    // LDS pending excess deltas of the tile
 };
 
-__device__ __forceinline__ int nbr_h(const SlotView& S, const TileCtx& tc, int w) {
+__device__ __forceinline__ int nbr_h(const GSlot& S, const TileCtx& tc, int w) {
     return (w >= tc.a && w < tc.b) ? tc.lh[w - tc.a] : S.h[w];
 }
 
-__device__ __forceinline__ void push_into(const SlotView& S, const TileCtx& tc, int w, int64_t d) {
+__device__ __forceinline__ void push_into(const GSlot& S, const TileCtx& tc, int w, int64_t d) {
     if (w >= tc.a && w < tc.b) add64_lds(&tc.ldx[w - tc.a], d);
     else add64(&S.excess[w], d);
 }
 
 // Owner view of v's excess.
-__device__ __forceinline__ int64_t own_excess(const SlotView& S, const TileCtx& tc, int v) {
+__device__ __forceinline__ int64_t own_excess(const GSlot& S, const TileCtx& tc, int v) {
     int64_t ex = ld64(&S.excess[v]);
     if (v >= tc.a && v < tc.b) ex += tc.ldx[v - tc.a];
     return ex;
 }
 
 // Take tot out of v; returns the owner's new view.
-__device__ __forceinline__ int64_t take_excess(const SlotView& S, const TileCtx& tc, int v,
+__device__ __forceinline__ int64_t take_excess(const GSlot& S, const TileCtx& tc, int v,
                                                int64_t ex_view, int64_t tot) {
     if (v >= tc.a && v < tc.b) {
         add64_lds(&tc.ldx[v - tc.a], -tot);
@@ -700,17 +869,17 @@ __device__ __forceinline__ int64_t take_excess(const SlotView& S, const TileCtx&
     return add64(&S.excess[v], -tot) - tot;
 }
 
-__device__ __forceinline__ int own_h(const SlotView& S, const TileCtx& tc, int v) {
+__device__ __forceinline__ int own_h(const GSlot& S, const TileCtx& tc, int v) {
     return (v >= tc.a && v < tc.b) ? tc.lh[v - tc.a] : S.h[v];
 }
 
-__device__ __forceinline__ void set_h(const SlotView& S, const TileCtx& tc, int v, int hv) {
+__device__ __forceinline__ void set_h(const GSlot& S, const TileCtx& tc, int v, int hv) {
     if (v >= tc.a && v < tc.b) tc.lh[v - tc.a] = hv;
     else S.h[v] = hv;
 }
 
 // Light row: one lane, `cycles` discharge cycles.
-__device__ void lane_discharge(const SlotView& S, const TileCtx& tc, int v, int cycles, Work& wk) {
+__device__ void lane_discharge(const GSlot& S, const TileCtx& tc, int v, int cycles, Work& wk) {
     const int n = S.g.n;
     const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
     int hv = own_h(S, tc, v);
@@ -728,7 +897,12 @@ __device__ void lane_discharge(const SlotView& S, const TileCtx& tc, int v, int 
             wk.relabels++;
             break;
         }
-        if (hv > hmin) {
+        if (hv <= hmin) {  // relabel (BC:504-507), then push at the new height
+            hv = hmin + 1;
+            set_h(S, tc, v, hv);
+            wk.relabels++;
+        }
+        {
             int64_t rem = ex;
             for (int e = rb; e < re && rem > 0; ++e) {
                 const int64_t r = ld64(&S.res[e]);
@@ -744,16 +918,12 @@ __device__ void lane_discharge(const SlotView& S, const TileCtx& tc, int v, int 
             }
             const int64_t tot = ex - rem;
             if (tot > 0) ex = take_excess(S, tc, v, ex, tot);
-        } else {
-            hv = hmin + 1;
-            set_h(S, tc, v, hv);
-            wk.relabels++;
         }
     }
 }
 
 // Heavy row: one wave, `cycles` discharge cycles; wave-uniform control.
-__device__ void wave_discharge(const SlotView& S, const TileCtx& tc, int v, int cycles, Work& wk) {
+__device__ void wave_discharge(const GSlot& S, const TileCtx& tc, int v, int cycles, Work& wk) {
     const int n = S.g.n;
     const int lane = lane_id();
     const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
@@ -771,7 +941,11 @@ __device__ void wave_discharge(const SlotView& S, const TileCtx& tc, int v, int 
             if (lane == 0) { set_h(S, tc, v, hv); wk.relabels++; }
             break;
         }
-        if (hv > hmin) {
+        if (hv <= hmin) {  // relabel, then push at the new height
+            hv = hmin + 1;
+            if (lane == 0) { set_h(S, tc, v, hv); wk.relabels++; }
+        }
+        {
             int64_t rem = ex;
             for (int base = rb; base < re && rem > 0; base += WAVE) {
                 const int e = base + lane;
@@ -799,9 +973,95 @@ __device__ void wave_discharge(const SlotView& S, const TileCtx& tc, int v, int 
                 if (lane == 0) nex = take_excess(S, tc, v, ex, tot);
                 ex = __shfl(nex, 0, WAVE);
             }
-        } else {
+        }
+    }
+}
+
+// Block-wide helpers (blockDim a multiple of 64, scratch holds blockDim/64
+// entries).  Results are valid in every thread.
+__device__ __forceinline__ int block_min_i32(int x, int* scratch) {
+    x = wave_min_i32(x);
+    const int w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    if (lane_id() == 0) scratch[w] = x;
+    __syncthreads();
+    int m = scratch[0];
+    for (int j = 1; j < nw; ++j) m = min(m, scratch[j]);
+    __syncthreads();
+    return m;
+}
+
+// Inclusive scan of x over the workgroup; *total gets the block sum.
+__device__ __forceinline__ int64_t block_incl_scan_i64(int64_t x, int64_t* scratch, int64_t* total) {
+    const int w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    const int64_t incl = wave_incl_scan_i64(x);
+    if (lane_id() == WAVE - 1) scratch[w] = incl;
+    __syncthreads();
+    int64_t before = 0, all = 0;
+    for (int j = 0; j < nw; ++j) {
+        const int64_t t = scratch[j];
+        if (j < w) before += t;
+        all += t;
+    }
+    __syncthreads();
+    *total = all;
+    return incl + before;
+}
+
+// Giant row: the whole workgroup runs the discharge cycle of lane_discharge /
+// wave_discharge (lowest residual neighbour, relabel, pushes in row order),
+// with a block min-reduction and a block prefix sum over residuals.
+__device__ void block_discharge(const GSlot& S, const TileCtx& tc, int v, int cycles, Work& wk,
+                                int* iscr, int64_t* lscr, int64_t* bcast) {
+    const int n = S.g.n;
+    const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+    if (threadIdx.x == 0) { bcast[0] = own_excess(S, tc, v); bcast[1] = own_h(S, tc, v); }
+    __syncthreads();
+    int64_t ex = bcast[0];
+    int hv = (int)bcast[1];
+    __syncthreads();
+    for (int c = 0; c < cycles; ++c) {
+        if (ex <= 0 || hv >= n) break;
+        int hmin = INT_MAX;
+        for (int e = rb + (int)threadIdx.x; e < re; e += blockDim.x)
+            if (ld64(&S.res[e]) > 0) hmin = min(hmin, nbr_h(S, tc, S.g.col[e]));
+        hmin = block_min_i32(hmin, iscr);
+        if (threadIdx.x == 0) wk.edges += (uint64_t)(re - rb);
+        if (hmin >= n) {
+            hv = n;
+            if (threadIdx.x == 0) { set_h(S, tc, v, hv); wk.relabels++; }
+            break;
+        }
+        if (hv <= hmin) {
             hv = hmin + 1;
-            if (lane == 0) { set_h(S, tc, v, hv); wk.relabels++; }
+            if (threadIdx.x == 0) { set_h(S, tc, v, hv); wk.relabels++; }
+        }
+        int64_t rem = ex;
+        for (int base = rb; base < re && rem > 0; base += blockDim.x) {
+            const int e = base + (int)threadIdx.x;
+            int64_t r = 0;
+            if (e < re) {
+                r = ld64(&S.res[e]);
+                if (r > 0 && nbr_h(S, tc, S.g.col[e]) != hmin) r = 0;
+                if (r < 0) r = 0;
+            }
+            int64_t chunk = 0;
+            const int64_t incl = block_incl_scan_i64(r, lscr, &chunk);
+            const int64_t excl = incl - r;
+            const int64_t d = (r > 0 && rem > excl) ? ((rem - excl) < r ? (rem - excl) : r) : 0;
+            if (d > 0) {
+                add64(&S.res[e], -d);
+                add64(&S.res[S.g.rev[e]], d);
+                push_into(S, tc, S.g.col[e], d);
+                wk.pushes++;
+            }
+            rem -= (chunk < rem ? chunk : rem);
+        }
+        const int64_t tot = ex - rem;
+        if (tot > 0) {
+            if (threadIdx.x == 0) bcast[0] = take_excess(S, tc, v, ex, tot);
+            __syncthreads();
+            ex = bcast[0];
+            __syncthreads();
         }
     }
 }
@@ -810,7 +1070,7 @@ __device__ void wave_discharge(const SlotView& S, const TileCtx& tc, int v, int 
 // wave each.  Blocks [0, light_blocks) take light entries.
 __global__ __launch_bounds__(256) void k_discharge_list(const SlotView* __restrict__ slots,
                                                         const int* __restrict__ list) {
-    const SlotView S = load_slot(slots, list);
+    const GSlot S = load_slot(slots, list);
     __shared__ uint64_t red[256 / WAVE];
     const int nl = S.ctr->nlight, nh = S.ctr->nheavy;
     const int lb = (nl + 255) / 256;
@@ -823,7 +1083,28 @@ __global__ __launch_bounds__(256) void k_discharge_list(const SlotView* __restri
         if (i < nl) lane_discharge(S, tc, S.fa[i], S.K, wk);
     } else {
         const int k = ((int)blockIdx.x - lb) * 4 + (int)(threadIdx.x / WAVE);
-        if (k < nh) wave_discharge(S, tc, S.fa[S.g.n - 1 - k], S.K, wk);
+        if (k < nh) {
+            const int v = S.fa[S.g.n - 1 - k];
+            if (S.g.row_ptr[v + 1] - S.g.row_ptr[v] <= kGiantDeg) wave_discharge(S, tc, v, S.Kh, wk);
+        }
+    }
+    work_flush(S, wk, red, WF_DISCHARGE);
+}
+
+// Giant rows of the active list: one workgroup per row, rows grid-stride.
+__global__ __launch_bounds__(1024) void k_discharge_giant(const SlotView* __restrict__ slots,
+                                                          const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ uint64_t red[1024 / WAVE];
+    __shared__ int iscr[1024 / WAVE];
+    __shared__ int64_t lscr[1024 / WAVE];
+    __shared__ int64_t bcast[2];
+    const int nh = S.ctr->nheavy;
+    TileCtx tc;
+    Work wk;
+    for (int k = blockIdx.x; k < nh; k += gridDim.x) {
+        const int v = S.fa[S.g.n - 1 - k];
+        if (S.g.row_ptr[v + 1] - S.g.row_ptr[v] > kGiantDeg) block_discharge(S, tc, v, S.Kh, wk, iscr, lscr, bcast);
     }
     work_flush(S, wk, red, WF_DISCHARGE);
 }
@@ -831,6 +1112,7 @@ __global__ __launch_bounds__(256) void k_discharge_list(const SlotView* __restri
 void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* list, int nlist,
                            int gridx) {
     hipLaunchKernelGGL(k_discharge_list, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_discharge_giant, dim3(256, nlist), dim3(1024), 0, st, slots, list);
 }
 
 // Region (tile) discharge (BC:432-515, one block per region): the tile's
@@ -840,7 +1122,7 @@ void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* lis
 __global__ __launch_bounds__(512) void k_discharge_tile(const SlotView* __restrict__ slots,
                                                         const int* __restrict__ list, int hot_only,
                                                         int rounds) {
-    const SlotView S = load_slot(slots, list);
+    const GSlot S = load_slot(slots, list);
     const int tile = blockIdx.x;
     if (tile >= S.g.ntiles) return;
     if (hot_only && S.tile_hot[tile] == 0) return;
@@ -849,6 +1131,9 @@ __global__ __launch_bounds__(512) void k_discharge_tile(const SlotView* __restri
     __shared__ int32_t alist[kTileMax];
     __shared__ int nlight, nheavy;
     __shared__ uint64_t red[512 / WAVE];
+    __shared__ int giscr[512 / WAVE];
+    __shared__ int64_t glscr[512 / WAVE];
+    __shared__ int64_t gbcast[2];
     const int a = S.g.tile_start[tile], b = S.g.tile_start[tile + 1];
     const int size = b - a;  // <= kTileMax by construction of tile_start
     const int n = S.g.n;
@@ -875,8 +1160,15 @@ __global__ __launch_bounds__(512) void k_discharge_tile(const SlotView* __restri
         const int nl = nlight, nh = nheavy;
         if (nl + nh == 0) break;
         for (int k = threadIdx.x; k < nl; k += blockDim.x) lane_discharge(S, tc, a + alist[k], 1, wk);
-        for (int k = threadIdx.x / WAVE; k < nh; k += blockDim.x / WAVE)
-            wave_discharge(S, tc, a + alist[size - 1 - k], 1, wk);
+        for (int k = threadIdx.x / WAVE; k < nh; k += blockDim.x / WAVE) {
+            const int v = a + alist[size - 1 - k];
+            if (S.g.row_ptr[v + 1] - S.g.row_ptr[v] <= kGiantDeg) wave_discharge(S, tc, v, 1, wk);
+        }
+        __syncthreads();
+        for (int k = 0; k < nh; ++k) {  // giant rows: the whole workgroup
+            const int v = a + alist[size - 1 - k];
+            if (S.g.row_ptr[v + 1] - S.g.row_ptr[v] > kGiantDeg) block_discharge(S, tc, v, 1, wk, giscr, glscr, gbcast);
+        }
         __syncthreads();
     }
     __syncthreads();
@@ -901,7 +1193,7 @@ void launch_discharge_tile(hipStream_t st, const SlotView* slots, const int* lis
 
 __global__ __launch_bounds__(256) void k_path_push(const SlotView* __restrict__ slots,
                                                    const int* __restrict__ list, int max_hops) {
-    const SlotView S = load_slot(slots, list);
+    const GSlot S = load_slot(slots, list);
     __shared__ uint64_t red[256 / WAVE];
     const int nl = S.ctr->nlight, nh = S.ctr->nheavy;
     const int k = blockIdx.x * 4 + (int)(threadIdx.x / WAVE);
@@ -991,7 +1283,7 @@ void launch_path_push(hipStream_t st, const SlotView* slots, const int* list, in
 
 __global__ __launch_bounds__(1024) void k_hot_select(const SlotView* __restrict__ slots,
                                                      const int* __restrict__ list, int k) {
-    const SlotView S = load_slot(slots, list);
+    const GSlot S = load_slot(slots, list);
     const int T = S.g.ntiles;
     // Threshold search: count tiles with cnt > x by bisection on x.
     __shared__ int cnt_gt;
@@ -1069,7 +1361,7 @@ void launch_hot_select(hipStream_t st, const SlotView* slots, const int* list, i
 
 __global__ __launch_bounds__(256) void k_rm(const SlotView* __restrict__ slots,
                                             const int* __restrict__ list, int margin) {
-    const SlotView S = load_slot(slots, list);
+    const GSlot S = load_slot(slots, list);
     __shared__ uint64_t red[256 / WAVE];
     const int tile = blockIdx.x;
     if (tile >= S.g.ntiles) return;
@@ -1121,42 +1413,45 @@ void launch_rm(hipStream_t st, const SlotView* slots, const int* list, int nlist
 // a residual entry from outside T into T.  cut = sum of cap over entries
 // from outside T into T.
 
-__global__ __launch_bounds__(256) void k_cert(const SlotView* __restrict__ slots,
-                                              const int* __restrict__ list) {
-    const SlotView S = load_slot(slots, list);
-    __shared__ int64_t red64[256 / WAVE];
-    __shared__ uint64_t redu[256 / WAVE];
-    const int n = S.g.n;
-    int64_t cut = 0;
-    uint64_t bad = 0, edges = 0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int v = (int)i;
-        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
-        const bool inT = S.h[v] < n;
-        int64_t ex = 0;
-        edges += (uint64_t)(re - rb);
-        for (int e = rb; e < re; ++e) {
-            const int64_t r = S.res[e];
-            const int rv = S.g.rev[e];
-            if (r < 0) ++bad;
-            if (r + S.res[rv] != S.g.cap[e] + S.g.cap[rv]) ++bad;
-            ex += r - S.g.cap[e];
-            if (!inT) {
-                const int w = S.g.col[e];
-                if (S.h[w] < n) {
-                    cut += S.g.cap[e];
-                    if (r > 0) ++bad;
-                }
-            }
-        }
-        if (ex != S.excess[v]) ++bad;
-        if (v != S.s && ex < 0) ++bad;
-        if (inT && v != S.t && ex != 0) ++bad;
-        if (v == S.t) S.ctr->ex_t = S.excess[v];
-        if (v == S.t && !inT) ++bad;
-        if (v == S.s && inT) ++bad;
+// Row terms of the certificate.  Every entry pair has exactly one entry with
+// capacity (forward) and one without (reverse), so pair sums are checked once,
+// from the forward side.  Rows of T also check closedness and sum the cut
+// from T's side: entry e = w -> u with u outside T closes pair u -> w.
+__device__ __forceinline__ void cert_entry(const GSlot& S, int e, bool inT, int n, int64_t& ex,
+                                           uint64_t& bad, int64_t& cut) {
+    const int64_t r = S.res[e];
+    const int64_t c = S.g.cap[e];
+    if (r < 0) ++bad;
+    ex += r - c;
+    if (c > 0) {  // forward entry; its reverse entry has capacity 0 by construction
+        const int rv = S.g.rev[e];
+        if (r + S.res[rv] != c) ++bad;
     }
+    if (inT) {
+        if (S.h[S.g.col[e]] >= n) {  // col[e] outside T: its pair entry leaves V\T into T
+            const int rv = S.g.rev[e];
+            cut += S.g.cap[rv];
+            if (S.res[rv] > 0) ++bad;
+        }
+    }
+}
+
+__device__ __forceinline__ void cert_vertex(const GSlot& S, int v, int64_t ex, bool inT, uint64_t& bad) {
+    if (ex != S.excess[v]) ++bad;
+    if (v != S.s && ex < 0) ++bad;
+    if (inT && v != S.t && ex != 0) ++bad;
+    if (v == S.t) S.ctr->ex_t = S.excess[v];
+    if (v == S.t && !inT) ++bad;
+    if (v == S.s && inT) ++bad;
+}
+
+// Light rows: one lane each; medium rows: one wave each; giant rows go to
+// the slot's heavy list, which k_cert_heavy hands out grid-stride, one row
+// per workgroup with a block reduction of the row's excess (hub rows sit
+// together at region starts, so they are spread over the whole grid instead
+// of the block that met them).
+__device__ __forceinline__ void cert_flush(const GSlot& S, int64_t cut, uint64_t bad, uint64_t edges,
+                                           int64_t* red64, uint64_t* redu) {
     const int64_t bc = block_sum<int64_t>(cut, red64);
     const uint64_t bb = block_sum<uint64_t>(bad, redu);
     const uint64_t be = block_sum<uint64_t>(edges, redu);
@@ -1167,8 +1462,83 @@ __global__ __launch_bounds__(256) void k_cert(const SlotView* __restrict__ slots
     }
 }
 
+__global__ __launch_bounds__(256) void k_cert(const SlotView* __restrict__ slots,
+                                              const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ int64_t red64[256 / WAVE];
+    __shared__ uint64_t redu[256 / WAVE];
+    __shared__ int med[256];
+    __shared__ int nmed;
+    const int n = S.g.n;
+    int64_t cut = 0;
+    uint64_t bad = 0, edges = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+        if (threadIdx.x == 0) nmed = 0;
+        __syncthreads();
+        const int64_t vi = base + threadIdx.x;
+        if (vi < n) {
+            const int v = (int)vi;
+            const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+            if (re - rb <= kLightDeg) {  // one lane
+                const bool inT = S.h[v] < n;
+                int64_t ex = 0;
+                for (int e = rb; e < re; ++e) cert_entry(S, e, inT, n, ex, bad, cut);
+                edges += (uint64_t)(re - rb);
+                cert_vertex(S, v, ex, inT, bad);
+            } else if (re - rb <= kGiantDeg) {  // one wave, below
+                med[atomicAdd(&nmed, 1)] = v;
+            } else {  // k_cert_heavy
+                S.hl[add32(&S.ctr->nhl, 1)] = v;
+            }
+        }
+        __syncthreads();
+        const int nm = nmed;
+        for (int k = threadIdx.x / WAVE; k < nm; k += blockDim.x / WAVE) {
+            const int v = med[k];
+            const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+            const bool inT = S.h[v] < n;
+            int64_t ex = 0;
+            for (int e = rb + lane_id(); e < re; e += WAVE) cert_entry(S, e, inT, n, ex, bad, cut);
+            ex = wave_sum_i64(ex);
+            if (lane_id() == 0) {
+                edges += (uint64_t)(re - rb);
+                cert_vertex(S, v, ex, inT, bad);
+            }
+        }
+        __syncthreads();
+    }
+    cert_flush(S, cut, bad, edges, red64, redu);
+}
+
+__global__ __launch_bounds__(256) void k_cert_heavy(const SlotView* __restrict__ slots,
+                                                    const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ int64_t red64[256 / WAVE];
+    __shared__ uint64_t redu[256 / WAVE];
+    const int nh = S.ctr->nhl;
+    if ((int)blockIdx.x >= nh) return;  // uniform per block
+    const int n = S.g.n;
+    int64_t cut = 0;
+    uint64_t bad = 0, edges = 0;
+    for (int k = blockIdx.x; k < nh; k += gridDim.x) {
+        const int v = S.hl[k];
+        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+        const bool inT = S.h[v] < n;
+        int64_t ex = 0;
+        for (int e = rb + (int)threadIdx.x; e < re; e += blockDim.x) cert_entry(S, e, inT, n, ex, bad, cut);
+        const int64_t tot = block_sum<int64_t>(ex, red64);
+        if (threadIdx.x == 0) {
+            edges += (uint64_t)(re - rb);
+            cert_vertex(S, v, tot, inT, bad);
+        }
+    }
+    cert_flush(S, cut, bad, edges, red64, redu);
+}
+
 void launch_cert(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx) {
     hipLaunchKernelGGL(k_cert, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_cert_heavy, dim3(2048, nlist), dim3(256), 0, st, slots, list);
 }
 
 // ------------------------------------------------------------- prune mask (a6)
@@ -1188,7 +1558,7 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
 __global__ __launch_bounds__(256) void k_mask_combine(const SlotView* __restrict__ slots,
                                                       const int* __restrict__ list,
                                                       unsigned long long* hash_out) {
-    const SlotView S = load_slot(slots, list);
+    const GSlot S = load_slot(slots, list);
     __shared__ uint64_t redu[256 / WAVE];
     const int n = S.g.n;
     uint64_t kept = 0, hsh = 0;
@@ -1226,7 +1596,7 @@ __global__ __launch_bounds__(256) void k_apply_updates(const SlotView* __restric
                                                        const int32_t* __restrict__ entries,
                                                        const int64_t* __restrict__ old_cap,
                                                        const int64_t* __restrict__ new_cap, int nupd) {
-    const SlotView S = load_slot(slots, list);
+    const GSlot S = load_slot(slots, list);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nupd; i += gridDim.x * blockDim.x) {
         const int e = entries[i];
         const int r = S.g.rev[e];

@@ -159,3 +159,28 @@ def test_mixed_graphs_in_one_engine():
     for i in range(10):
         assert res[2 * i].flow == g1.oracle_flow(*qa[i])
         assert res[2 * i + 1].flow == g2.oracle_flow(*qb[i])
+
+
+def test_giant_rows():
+    """Rows longer than the giant threshold (2048) go through the workgroup
+    paths: top-down/bottom-up heavy BFS, block discharge, heavy certificate."""
+    rng = np.random.default_rng(99)
+    L = 6000
+    s, hub, t = 0, 1, 2
+    leaves = np.arange(3, 3 + L)
+    us = [s, s] + [hub] * L + leaves.tolist() + leaves[:L // 2].tolist()
+    vs = [hub, 3] + leaves.tolist() + [t] * L + [hub] * (L // 2)
+    cs = [10**9, 7] + rng.integers(1, 50, L).tolist() + rng.integers(1, 50, L).tolist() + rng.integers(1, 9, L // 2).tolist()
+    n = 3 + L
+    hg = M.HostGraph.from_edges(n, us, vs, cs).partition(4)
+    d = hg.export()
+    og = O.Graph(n, d["row_ptr"], d["col"], d["cap"], d["rev"])
+    dg = M.DeviceGraph(hg, 1024)
+    for variant in (dict(), dict(tile_frac=0.0), dict(tile_frac=1e9), dict(prune=0)):
+        eng = M.Engine(1, M.default_params(**variant))
+        gid = eng.add_graph(dg)
+        for a, b in ((s, t), (hub, t), (s, int(leaves[5]))):
+            sa, tb = int(d["new_id"][a]), int(d["new_id"][b])
+            r = eng.solve(gid, sa, tb)
+            assert r.flow == O.maxflow(og, sa, tb), (variant, a, b)
+            assert r.cert_ok == 1

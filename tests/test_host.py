@@ -168,3 +168,57 @@ def test_find_entry():
     e = hg.find_entry(int(nid[0]), int(nid[1]))
     assert e >= 0 and d["cap"][e] == 5 and d["col"][e] == nid[1]
     assert hg.find_entry(int(nid[1]), int(nid[0])) == -1  # reverse entry only (cap 0)
+
+
+def _reach_sets(n, us, vs):
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+
+    A = sp.coo_matrix((np.ones(len(us)), (us, vs)), shape=(n, n)).tocsr()
+    ns, scc = connected_components(A, directed=True, connection="strong")
+    nw, wcc = connected_components(A, directed=False)
+    return scc, wcc
+
+
+@pytest.mark.parametrize("gen", ["uniform", "rmat", "grid"])
+def test_components_match_scipy(gen):
+    if gen == "uniform":
+        e = M.Edges.uniform(3000, 4000, 10, 41)  # sparse: many small components
+    elif gen == "rmat":
+        e = M.Edges.rmat(12, 10**6, 42)
+    else:
+        e = M.Edges.grid(30, 30, 1000, 43)
+    n, us, vs, cs = e.arrays()
+    hg = M.HostGraph.from_edgelist(e)
+    scc, wcc = hg.components()
+    ref_s, ref_w = _reach_sets(n, us, vs)
+    # same partitions of the vertex set (labels may differ)
+    for mine, ref in ((scc, ref_s), (wcc, ref_w)):
+        pairs = set(zip(mine.tolist(), ref.tolist()))
+        assert len(pairs) == len(set(mine.tolist())) == len(set(ref.tolist()))
+
+
+def test_class_mask_equals_prune_mask():
+    """fwd(scc(s)) AND wcc == wcc(t) is the mask of BC:1391-1445 whenever t is
+    reachable from s; otherwise t is outside fwd(s) and the flow is 0."""
+    e = M.Edges.rmat(11, 10**6, 44)
+    n, us, vs, cs = e.arrays()
+    og = O.build_csr(n, us, vs, cs)
+    hg = M.HostGraph.from_edgelist(e)
+    scc, wcc = hg.components()
+    rng = np.random.default_rng(45)
+    hub_s, hub_t = e.hub_queries(20, 46)
+    pairs = list(zip(hub_s.tolist(), hub_t.tolist())) + [tuple(x) for x in rng.integers(0, n, (20, 2)).tolist()]
+    for s, t in pairs:
+        if s == t:
+            continue
+        mask = O.prune_mask(og, s, t)
+        # forward set of another member of s's SCC; prune_mask(r, r) is
+        # fwd(r) AND wcc(r) = fwd(r) because reachable vertices share r's WCC
+        rep = int(np.nonzero(scc == scc[s])[0][0])
+        fwd = O.prune_mask(og, rep, rep).astype(bool)
+        if not fwd[t]:
+            assert O.maxflow(og, s, t) == 0
+            continue
+        cls = fwd & (wcc == wcc[t])
+        assert np.array_equal(cls.astype(np.int32), mask), (s, t)
