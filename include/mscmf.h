@@ -138,6 +138,8 @@ typedef struct mf_params {
     double prune_ratio;  /* keep the subgraph when kept < ratio * n */
     double gr_work;      /* also GR after gr_work * E discharge scans since the last one, 0 = off */
     int K_heavy;         /* discharge cycles per pass for wave-scanned (long) rows */
+    int local_relabel;   /* 1 = BFS relabel of active tiles between global relabels */
+    int local_band;      /* ... when the active height band is at most this wide */
 } mf_params;
 
 void mf_params_default(mf_params* p);
@@ -224,6 +226,13 @@ int mf_dbg_prune_mask(mf_engine* e, int gid, int s, int t, int32_t* mask, int* k
 int mf_dbg_induced(mf_engine* e, int gid, const int32_t* mask, int* n_out, int64_t* E_out,
                    int32_t* row_ptr, int32_t* col, int64_t* cap, int32_t* rev,
                    int32_t* old2sub);
+/* Local relabel (BC:236-279, BC:1582-1721) on given residual, excess and
+ * heights, with the tiles flagged in tile_active[ntiles] as the active
+ * region and level cap lcap (1..256).  Heights are updated in place; *level
+ * gets the first level the bounded BFS did not complete. */
+int mf_dbg_local_relabel(mf_engine* e, int gid, int s, int t, const int64_t* res,
+                         const int64_t* excess, int32_t* height, const int32_t* tile_active,
+                         int lcap, int* level);
 /* Solve without pruning on slot 0 and copy the final residual, excess and
  * heights out (for an independent CPU certificate). */
 int mf_dbg_solve_state(mf_engine* e, int gid, int s, int t, mf_result* r, int64_t* res,

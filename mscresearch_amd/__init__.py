@@ -42,6 +42,7 @@ class Params(C.Structure):
         ("rm_tau", C.c_double), ("tile_rounds", C.c_int), ("tile_frac", C.c_double), ("cert", C.c_int),
         ("iter_max", C.c_longlong), ("bu_alpha", C.c_double), ("bu_beta", C.c_double),
         ("prune", C.c_int), ("prune_ratio", C.c_double), ("gr_work", C.c_double), ("K_heavy", C.c_int),
+        ("local_relabel", C.c_int), ("local_band", C.c_int),
     ]
 
 
@@ -115,6 +116,7 @@ SIGNATURES = {
     "mf_graph_update_caps": (C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
     "mf_dbg_init_preflow": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P]),
     "mf_dbg_global_relabel": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
+    "mf_dbg_local_relabel": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, C.c_int, _P]),
     "mf_dbg_prune_mask": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.POINTER(C.c_int)]),
     "mf_dbg_induced": (C.c_int, [_P, C.c_int, _P, C.POINTER(C.c_int), C.POINTER(C.c_int64), _P, _P, _P, _P, _P]),
     "mf_dbg_solve_state": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(Result), _P, _P, _P]),
@@ -424,6 +426,17 @@ class Engine:
         h = np.empty(len(excess), np.int32)
         _check(lib().mf_dbg_global_relabel(self._h, gid, int(s), int(t), _ptr(res), _ptr(excess), _ptr(h)))
         return h
+
+    def dbg_local_relabel(self, gid, s, t, res, excess, h, tile_active, lcap):
+        """Local relabel on the given state; returns (new heights, level)."""
+        res = np.ascontiguousarray(res, np.int64)
+        excess = np.ascontiguousarray(excess, np.int64)
+        h = np.array(h, np.int32)
+        ta = np.ascontiguousarray(tile_active, np.int32)
+        lvl = C.c_int(0)
+        _check(lib().mf_dbg_local_relabel(self._h, gid, int(s), int(t), _ptr(res), _ptr(excess), _ptr(h),
+                                          _ptr(ta), int(lcap), C.byref(lvl)))
+        return h, lvl.value
 
     def dbg_prune_mask(self, gid, s, t, n):
         mask = np.empty(n, np.int32)

@@ -100,6 +100,8 @@ Params from_c(const mf_params* p) {
     P.prune_ratio = p->prune_ratio;
     P.gr_work = p->gr_work;
     P.K_heavy = p->K_heavy;
+    P.local_relabel = p->local_relabel;
+    P.local_band = p->local_band;
     return P;
 }
 
@@ -405,6 +407,8 @@ void mf_params_default(mf_params* p) {
     p->prune_ratio = P.prune_ratio;
     p->gr_work = P.gr_work;
     p->K_heavy = P.K_heavy;
+    p->local_relabel = P.local_relabel;
+    p->local_band = P.local_band;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {
@@ -605,6 +609,33 @@ int mf_dbg_global_relabel(mf_engine* e, int gid, int s, int t, const int64_t* re
     p.bfs_mode = BFS_GR;
     rc = dbg_bfs(e, p, BFS_GR);
     if (rc == MF_OK) HIPCK(hipMemcpy(height, v.h, sizeof(int32_t) * g.n, hipMemcpyDeviceToHost));
+    e->eng.dbg_end();
+    return rc;
+}
+
+int mf_dbg_local_relabel(mf_engine* e, int gid, int s, int t, const int64_t* res, const int64_t* excess,
+                         int32_t* height, const int32_t* tile_active, int lcap, int* level) {
+    int rc = check_query(e, gid, s, t);
+    if (rc != MF_OK) return rc;
+    if (!res || !excess || !height || !tile_active || !level) return fail(MF_EINVAL, "null argument");
+    if (lcap < 1 || lcap > kLocalLevels) return fail(MF_EINVAL, "lcap out of range");
+    rc = e->eng.dbg_begin(gid, s, t);
+    if (rc != MF_OK) return rc;
+    SlotView& v = e->eng.view(0);
+    const DGraph& g = v.g;
+    HIPCK(hipMemcpy(v.res, res, sizeof(int64_t) * g.E, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(v.excess, excess, sizeof(int64_t) * g.n, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(v.h, height, sizeof(int32_t) * g.n, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(v.tile_cnt, tile_active, sizeof(int32_t) * g.ntiles, hipMemcpyHostToDevice));
+    Plan p;
+    p.stages = ST_RESET_PRE | ST_LOCAL;
+    p.reset_pre = RST_BFS | RST_WORK | RST_ERR;
+    p.local_cap = lcap;
+    rc = e->eng.dbg_plan(p);
+    if (rc == MF_OK) {
+        HIPCK(hipMemcpy(height, v.h, sizeof(int32_t) * g.n, hipMemcpyDeviceToHost));
+        *level = e->eng.ctr(0).level;
+    }
     e->eng.dbg_end();
     return rc;
 }

@@ -68,6 +68,7 @@ struct SlotView {
     int32_t* hl = nullptr;       // n heavy-row work list (BFS levels, certificate)
     int32_t* tile_cnt = nullptr; // ntiles active counts
     int32_t* tile_hot = nullptr; // ntiles hot flags
+    int32_t* lbkt = nullptr;     // 2 * kLocalLevels local-relabel bucket counts and cursors
     const int32_t* mask = nullptr;  // optional vertex mask (nullptr = all)
     Ctr* ctr = nullptr;
     int32_t s = 0, t = 0;
@@ -82,5 +83,6 @@ constexpr int kLightDeg = 32;       // rows up to this length are scanned by one
 constexpr int kGiantDeg = 2048;     // rows longer than this are scanned by a whole workgroup
 constexpr int kSmallFrontier = 16384;  // single-workgroup BFS keeps going below this
 constexpr int64_t kSmallEdges = 1 << 18; // ... and while the frontier's rows stay below this
+constexpr int kLocalLevels = 256;   // level cap of the local relabel
 
 }  // namespace mf

@@ -104,11 +104,13 @@ int upload_graph(const HostGraph& hg, int tile_max, DeviceGraph& dg) {
 // ------------------------------------------------------------------- slabs
 Slab::~Slab() {
     (void)hipFree(res); (void)hipFree(excess); (void)hipFree(h); (void)hipFree(fa); (void)hipFree(fb); (void)hipFree(lock);
-    (void)hipFree(aux); (void)hipFree(hl); (void)hipFree(tile_cnt); (void)hipFree(tile_hot); (void)hipFree(ctr);
+    (void)hipFree(aux); (void)hipFree(hl); (void)hipFree(tile_cnt); (void)hipFree(tile_hot); (void)hipFree(lbkt);
+    (void)hipFree(ctr);
 }
 
 size_t Slab::bytes_for(int n, int64_t E, int ntiles) {
-    return (size_t)E * 8 + (size_t)n * (8 + 4 * 6) + (size_t)ntiles * 8 + sizeof(Ctr);
+    return (size_t)E * 8 + (size_t)n * (8 + 4 * 6) + (size_t)ntiles * 8 + sizeof(int32_t) * 2 * kLocalLevels +
+           sizeof(Ctr);
 }
 
 int alloc_slab(int n, int64_t E, int ntiles, Slab& sl) {
@@ -124,6 +126,7 @@ int alloc_slab(int n, int64_t E, int ntiles, Slab& sl) {
     HIPCHK(hipMalloc(&sl.hl, 4 * nn));
     HIPCHK(hipMalloc(&sl.tile_cnt, 4 * tt));
     HIPCHK(hipMalloc(&sl.tile_hot, 4 * tt));
+    HIPCHK(hipMalloc(&sl.lbkt, sizeof(int32_t) * 2 * kLocalLevels));
     HIPCHK(hipMalloc(&sl.ctr, sizeof(Ctr)));
     HIPCHK(hipMemset(sl.ctr, 0, sizeof(Ctr)));
     sl.n_cap = n;
@@ -167,6 +170,7 @@ void SlotSched::begin(const Params* P, const DGraph& g, int s, int t, bool prune
     drain_loops_ = 0;
     rm_off_ = false;
     just_gr_ = false;
+    after_local_ = false;
     work_reset_ = false;
     fwd_only_ = false;
     st_ = prune_first ? S_PRUNE_FWD : (warm ? S_WARM_START : S_START);
@@ -322,6 +326,9 @@ Plan SlotSched::plan(const Ctr& c) {
     case S_COUNTED: {
         const bool exact = just_gr_;
         just_gr_ = false;
+        // counts right after a local relabel follow no discharge: they do not
+        // feed the stagnation and stall monitors
+        const bool was_local = after_local_;
         if (c.err) { fail(MF_EHIP, "device error flag"); return p; }
         const int active = c.active;
         if (active == 0) {
@@ -354,7 +361,7 @@ Plan SlotSched::plan(const Ctr& c) {
         if (iter_ > P_->iter_max) { fail(MF_EITER, "iteration watchdog fired"); return p; }
         // stagnation monitor on the positive excess sum (BC:1851-1891)
         bool hotspot_triggered = false;
-        if (!exact && iter_ > 0) {
+        if (!exact && !was_local && iter_ > 0) {
             if (last_pos_ != INT64_MAX && c.pos_excess >= last_pos_) ++stag_;
             else stag_ = 0;
             last_pos_ = c.pos_excess;
@@ -385,7 +392,7 @@ Plan SlotSched::plan(const Ctr& c) {
                     stats.rm_fixed += c.fixed_edges;
                 }
             }
-        } else {
+        } else if (!was_local) {
             last_pos_ = c.pos_excess;
             last_ext_ = c.ex_t;
         }
@@ -397,6 +404,22 @@ Plan SlotSched::plan(const Ctr& c) {
                             (P_->gr_stall > 0 && stall_ >= P_->gr_stall) ||
                             (P_->gr_work > 0 && (double)scans >= P_->gr_work * (double)g_.E);
         if (gr_due && !exact) return start_gr(&c);
+        // local relabel (BC:1582-1721): BFS from t and the deficits through
+        // the tiles holding active vertices only, for at most band + 2 levels,
+        // when the active heights span a narrow band; vertices it does not
+        // reach keep their height.  The discharge follows.
+        const int lcap = std::min(c.hmax + 2, std::min(kLocalLevels, g_.n));
+        if (P_->local_relabel && !was_local && !exact && c.hmax >= c.hmin &&
+            c.hmax - c.hmin + 1 <= P_->local_band && c.hmax + 2 <= lcap) {
+            p.stages = ST_RESET_PRE | ST_LOCAL | ST_RESET_ACT | ST_ACTIVE;
+            p.reset_pre = RST_BFS;
+            p.reset_act = RST_ACTIVE;
+            p.local_cap = lcap;
+            after_local_ = true;
+            stats.local_relabels++;
+            return p;
+        }
+        after_local_ = false;
         // discharge step
         if (active > 0 && active <= P_->path_switch) ++path_stable_;
         else { path_stable_ = 0; p_count_ = P_->p_count0; }
@@ -541,6 +564,7 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
     v.g = G.g;
     v.res = sl.res; v.excess = sl.excess; v.h = sl.h; v.fa = sl.fa; v.fb = sl.fb;
     v.lock = sl.lock; v.aux = sl.aux; v.hl = sl.hl; v.tile_cnt = sl.tile_cnt; v.tile_hot = sl.tile_hot;
+    v.lbkt = sl.lbkt;
     v.ctr = sl.ctr;
     v.s = q.s; v.t = q.t;
     v.H_cut = P_.h_cut;
@@ -793,6 +817,17 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     for (int k = 0; k < 3; ++k)
         for (int m = 0; m < 3; ++m)
             g_bfs[k][m] = make_list([&](int i) { return has(i, ST_BFS) && plans[i].bfs_kind == k && plans[i].bfs_mode == m; });
+    // local relabels, one group per distinct level cap
+    std::vector<std::pair<int, Group>> local_groups;
+    {
+        std::vector<int> caps;
+        for (int i = 0; i < nslots_; ++i)
+            if (slots_[i].live && has(i, ST_LOCAL) &&
+                std::find(caps.begin(), caps.end(), plans[i].local_cap) == caps.end())
+                caps.push_back(plans[i].local_cap);
+        for (int c : caps)
+            local_groups.push_back({c, make_list([&](int i) { return has(i, ST_LOCAL) && plans[i].local_cap == c; })});
+    }
     Group g_mask = make_list([&](int i) { return has(i, ST_MASK); });
     Group g_dis[3];
     for (int k = 0; k < 3; ++k) g_dis[k] = make_list([&](int i) { return has(i, ST_DISCHARGE) && plans[i].dis_kind == k; });
@@ -847,6 +882,19 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     }
     if (g_pre.cnt) { const int k = tb(WF_INIT); launch_preflow(st, d_views_, L(g_pre), g_pre.cnt); te(k); }
     if (g_seed.cnt) { const int k = tb(WF_BFS); launch_gr_seed(st, d_views_, L(g_seed), g_seed.cnt, maxof(g_seed, gridN)); te(k); }
+    if (!local_groups.empty()) {
+        const int k = tb(WF_BFS);
+        for (auto& lg : local_groups) {
+            const Group& g = lg.second;
+            const int gx = maxof(g, gridN);
+            launch_local_mark(st, d_views_, L(g), g.cnt, gx, lg.first);
+            launch_local_bound(st, d_views_, L(g), g.cnt, gx, lg.first);
+            launch_local_bucket(st, d_views_, L(g), g.cnt, gx, lg.first);
+            launch_local_bfs(st, d_views_, L(g), g.cnt, lg.first);
+            launch_local_finish(st, d_views_, L(g), g.cnt, gx);
+        }
+        te(k);
+    }
     for (int m = 0; m < 3; ++m)
         if (g_seed1[m].cnt) {
             const int k = tb(WF_BFS);

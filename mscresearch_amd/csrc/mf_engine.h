@@ -51,6 +51,13 @@ struct Params {
     // discharge cycles per pass for rows scanned by a whole wave (light rows
     // use K): one full scan of a hub row is already a long pass
     int K_heavy = 1;
+    // local relabel (BC:1582-1721) before a discharge when the active height
+    // band is at most local_band wide.  Off by default: on R-MAT hub queries
+    // the tiles holding active vertices span most of the graph, so each local
+    // relabel costs about a global one (plain solve 52.6 -> 11 queries/s at
+    // R-MAT scale 22)
+    int local_relabel = 0;
+    int local_band = 128;
 };
 
 // ---------------------------------------------------------- device memory
@@ -107,6 +114,7 @@ struct Slab {
     int32_t* hl = nullptr;
     int32_t* tile_cnt = nullptr;
     int32_t* tile_hot = nullptr;
+    int32_t* lbkt = nullptr;
     Ctr* ctr = nullptr;
     size_t bytes = 0;
     ~Slab();
@@ -130,6 +138,7 @@ enum StageBit : unsigned {
     ST_RESET_ACT = 1u << 10,
     ST_ACTIVE = 1u << 11,
     ST_CERT = 1u << 12,
+    ST_LOCAL = 1u << 13,
 };
 
 enum BfsKind : int { BK_SMALL = 0, BK_TD = 1, BK_BU = 2 };
@@ -142,6 +151,7 @@ struct Plan {
     int bfs_kind = BK_SMALL, bfs_mode = 0, bfs_grid = 1;
     int dis_kind = DK_LIST, dis_grid = 1;
     int seed_mode = 0;
+    int local_cap = 0;        // ST_LOCAL: level cap of the local relabel
 };
 
 // Per-query statistics (host side).
@@ -157,6 +167,7 @@ struct QueryStats {
     int64_t grs = 0;          // exact global relabels
     int64_t bfs_levels = 0;
     int64_t drains = 0;       // deficit drain phases
+    int64_t local_relabels = 0;
     int64_t rm_fixed = 0;
     int cert_ok = 0;
     uint64_t edge_scans = 0, vertex_scans = 0, pushes = 0, relabels = 0;
@@ -216,6 +227,7 @@ private:
     bool rm_last_ = false;
     bool work_reset_ = false;
     bool fwd_only_ = false;
+    bool after_local_ = false;
 };
 
 // ------------------------------------------------------------------ engine

@@ -28,6 +28,24 @@ void launch_preflow(hipStream_t st, const SlotView* slots, const int* list, int 
 
 // Global-relabel seeding: h = 0 on seeds, n elsewhere; seeds go to fa.
 void launch_gr_seed(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);
+// Local relabel of the tiles holding active vertices (region R), levels below
+// lcap <= kLocalLevels.  Only raises heights, so a valid labelling stays valid:
+//  mark   : aux = h on R; R' = {v in R : h < lcap}; seeds of R' (h == 0) start
+//           the frontier, the rest of R' is marked n + 1;
+//  bound  : b(v) = 1 + min h(x) over residual v -> x with x outside R' (the
+//           heights around R' enter the BFS as sources at level b(v));
+//  bucket : R' vertices with b(v) < lcap grouped by b(v) into hl;
+//  bfs    : one workgroup, levels 0.. lcap-1, bucket L joins level L;
+//  finish : h = max(aux, level) on reached vertices, max(aux, Lf) on the
+//           rest, Lf being the first level the BFS did not complete.
+void launch_local_mark(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                       int lcap);
+void launch_local_bound(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                        int lcap);
+void launch_local_bucket(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                         int lcap);
+void launch_local_bfs(hipStream_t st, const SlotView* slots, const int* list, int nlist, int lcap);
+void launch_local_finish(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);
 // Single-source BFS seeding for pruning: lvl (h for FWD, aux for UND) = n,
 // lvl[src] = 0, frontier {src}.
 void launch_bfs_seed1(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,

@@ -125,6 +125,7 @@ struct GSlot {
     GAS int32_t* hl;
     GAS int32_t* tile_cnt;
     GAS int32_t* tile_hot;
+    GAS int32_t* lbkt;
     const GAS int32_t* mask;
     GAS Ctr* ctr;
     int32_t s, t, K, Kh, H_cut, phase;
@@ -158,6 +159,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.hl = as_global(v.hl);
     S.tile_cnt = as_global(v.tile_cnt);
     S.tile_hot = as_global(v.tile_hot);
+    S.lbkt = as_global(v.lbkt);
     S.mask = as_global(v.mask);
     S.ctr = as_global(v.ctr);
     S.s = v.s;
@@ -756,6 +758,270 @@ void launch_bfs_small(hipStream_t st, const SlotView* slots, const int* list, in
     if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_small<BFS_GR>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
     else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_small<BFS_FWD>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
     else hipLaunchKernelGGL(k_bfs_small<BFS_UND>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
+}
+
+// ------------------------------------------------------- local relabel (a14)
+// BC:236-279 and BC:1582-1721 relabel the tiles that hold active vertices by
+// a BFS from t and the deficits inside them, for at most band + 2 levels, and
+// overwrite the heights the BFS reaches (which may lower them).  Here the
+// same bounded BFS over the active tiles only ever raises heights: the
+// heights around the relabelled set R' enter as sources, so the result is the
+// largest valid labelling of R' below lcap and RM never has invalid entries
+// to repair.  Vertices of R' are marked n + 1 while the BFS runs.
+
+__device__ __forceinline__ bool local_in_region(const GSlot& S, int v) {
+    return S.tile_cnt[S.g.tile_of[v]] > 0;
+}
+
+__global__ __launch_bounds__(256) void k_local_mark(const SlotView* __restrict__ slots,
+                                                    const int* __restrict__ list, int lcap) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ Stage<1024> st;
+    __shared__ uint64_t red[256 / WAVE];
+    stage_init(st);
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < 2 * kLocalLevels; i += blockDim.x) S.lbkt[i] = 0;
+    __syncthreads();
+    const int n = S.g.n;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    Work wk;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int v = (int)i;
+        if (!local_in_region(S, v)) continue;
+        const int hv = S.h[v];
+        S.aux[v] = hv;
+        wk.verts++;
+        if (hv >= lcap || v == S.s || (S.mask != nullptr && S.mask[v] == 0 && v != S.t)) continue;
+        bool seed = false;
+        if (hv == 0) {
+            if (S.phase == 0) seed = (v == S.t) || (S.excess[v] < 0);
+            else seed = (v != S.t) && (S.excess[v] < 0);
+        }
+        if (seed) {
+            stage_push(st, v, S.g.row_ptr[v + 1] - S.g.row_ptr[v], S.fa, &S.ctr->fsize[0], &S.ctr->fedges[0]);
+        } else {
+            S.h[v] = n + 1;
+        }
+    }
+    stage_flush(st, S.fa, &S.ctr->fsize[0], &S.ctr->fedges[0]);
+    work_flush(S, wk, red, WF_BFS);
+}
+
+void launch_local_mark(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                       int lcap) {
+    hipLaunchKernelGGL(k_local_mark, dim3(gridx, nlist), dim3(256), 0, st, slots, list, lcap);
+}
+
+// b(v) for marked v into fb[v]; counts per b below lcap into lbkt[0..lcap).
+// Rows up to kLightDeg by one lane, longer rows by a wave.
+__global__ __launch_bounds__(256) void k_local_bound(const SlotView* __restrict__ slots,
+                                                     const int* __restrict__ list, int lcap) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ int heavy[256];
+    __shared__ int nheavy;
+    __shared__ uint64_t red[256 / WAVE];
+    const int n = S.g.n;
+    const int mark = n + 1;
+    const int lane = lane_id();
+    const int wave = threadIdx.x / WAVE;
+    Work wk;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+        if (threadIdx.x == 0) nheavy = 0;
+        __syncthreads();
+        const int v = (int)(base + threadIdx.x);
+        if (v < n && S.h[v] == mark) {
+            const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+            if (re - rb > kLightDeg) {
+                heavy[atomicAdd(&nheavy, 1)] = v;
+            } else {
+                int b = INT_MAX;
+                wk.edges += (uint64_t)(re - rb);
+                for (int e = rb; e < re; ++e) {
+                    if (ld64(&S.res[e]) <= 0) continue;
+                    const int hx = S.h[S.g.col[e]];
+                    if (hx != mark) b = min(b, hx + 1);
+                }
+                const int bb = b < lcap ? b : lcap;
+                S.fb[v] = bb;
+                if (bb < lcap) add32(&S.lbkt[bb], 1);
+            }
+        }
+        __syncthreads();
+        const int nh = nheavy;
+        for (int k = wave; k < nh; k += (int)(blockDim.x / WAVE)) {
+            const int u = heavy[k];
+            const int rb = S.g.row_ptr[u], re = S.g.row_ptr[u + 1];
+            if (lane == 0) wk.edges += (uint64_t)(re - rb);
+            int b = INT_MAX;
+            for (int e = rb + lane; e < re; e += WAVE) {
+                if (ld64(&S.res[e]) <= 0) continue;
+                const int hx = S.h[S.g.col[e]];
+                if (hx != mark) b = min(b, hx + 1);
+            }
+            b = wave_min_i32(b);
+            if (lane == 0) {
+                const int bb = b < lcap ? b : lcap;
+                S.fb[u] = bb;
+                if (bb < lcap) add32(&S.lbkt[bb], 1);
+            }
+        }
+        __syncthreads();
+    }
+    work_flush(S, wk, red, WF_BFS);
+}
+
+void launch_local_bound(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                        int lcap) {
+    hipLaunchKernelGGL(k_local_bound, dim3(gridx, nlist), dim3(256), 0, st, slots, list, lcap);
+}
+
+// Exclusive offsets of the lcap bucket counts into off[0..lcap], by one thread.
+__device__ __forceinline__ void local_offsets(const GSlot& S, int lcap, int* off) {
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int b = 0; b < lcap; ++b) { off[b] = acc; acc += S.lbkt[b]; }
+        off[lcap] = acc;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_local_bucket(const SlotView* __restrict__ slots,
+                                                      const int* __restrict__ list, int lcap) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ int off[kLocalLevels + 1];
+    local_offsets(S, lcap, off);
+    const int n = S.g.n;
+    const int mark = n + 1;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int v = (int)i;
+        if (S.h[v] != mark) continue;
+        const int b = S.fb[v];
+        if (b >= lcap) continue;
+        const int pos = off[b] + add32(&S.lbkt[kLocalLevels + b], 1);
+        S.hl[pos] = v;
+    }
+}
+
+void launch_local_bucket(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                         int lcap) {
+    hipLaunchKernelGGL(k_local_bucket, dim3(gridx, nlist), dim3(256), 0, st, slots, list, lcap);
+}
+
+// One workgroup per slot.  Level L: bucket L joins the frontier, then the
+// frontier is expanded backwards over residual entries into marked vertices.
+// Stops early (ctr->level = L + 1) when the frontier outgrows one workgroup;
+// otherwise ctr->level = lcap.
+__global__ __launch_bounds__(1024) void k_local_bfs(const SlotView* __restrict__ slots,
+                                                    const int* __restrict__ list, int lcap) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ int off[kLocalLevels + 1];
+    __shared__ int heavy[1024];
+    __shared__ int nheavy;
+    __shared__ int next_cnt;
+    __shared__ int64_t next_deg;
+    __shared__ uint64_t red[1024 / WAVE];
+    local_offsets(S, lcap, off);
+    const int n = S.g.n;
+    const int mark = n + 1;
+    int fsz = S.ctr->fsize[0];
+    int64_t fdeg = S.ctr->fedges[0];
+    GAS int32_t* in = S.fa;
+    GAS int32_t* out = S.fb;
+    Work wk;
+    int Lf = lcap;
+    for (int L = 0; L < lcap; ++L) {
+        // bucket L joins level L
+        if (threadIdx.x == 0) { next_cnt = 0; next_deg = 0; }
+        __syncthreads();
+        for (int i = off[L] + (int)threadIdx.x; i < off[L + 1]; i += blockDim.x) {
+            const int u = S.hl[i];
+            if (cas32(&S.h[u], mark, L)) {
+                in[fsz + atomicAdd(&next_cnt, 1)] = u;
+                add64_lds(&next_deg, (int64_t)(S.g.row_ptr[u + 1] - S.g.row_ptr[u]));
+                wk.relabels++;
+            }
+        }
+        __syncthreads();
+        fsz += next_cnt;
+        fdeg += next_deg;
+        __syncthreads();  // everybody has read next_cnt before it is reset
+        if (fsz > kSmallFrontier || fdeg > kSmallEdges) { Lf = L + 1; break; }
+        if (L + 1 >= lcap) break;
+        if (fsz == 0) continue;
+        if (threadIdx.x == 0) { nheavy = 0; next_cnt = 0; next_deg = 0; }
+        __syncthreads();
+        for (int i = threadIdx.x; i < fsz; i += blockDim.x) {
+            const int v = in[i];
+            wk.verts++;
+            const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+            if (re - rb > kLightDeg) {
+                const int k = atomicAdd(&nheavy, 1);
+                if (k < 1024) { heavy[k] = v; continue; }
+            }
+            wk.edges += (uint64_t)(re - rb);
+            for (int e = rb; e < re; ++e) {
+                const int w = S.g.col[e];
+                if (S.h[w] != mark) continue;
+                if (ld64(&S.res[S.g.rev[e]]) <= 0) continue;
+                if (cas32(&S.h[w], mark, L + 1)) {
+                    out[atomicAdd(&next_cnt, 1)] = w;
+                    add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
+                    wk.relabels++;
+                }
+            }
+        }
+        __syncthreads();
+        const int nh = min(nheavy, 1024);
+        for (int k = 0; k < nh; ++k) {
+            const int v = heavy[k];
+            const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+            if (threadIdx.x == 0) wk.edges += (uint64_t)(re - rb);
+            for (int e = rb + (int)threadIdx.x; e < re; e += blockDim.x) {
+                const int w = S.g.col[e];
+                if (S.h[w] != mark) continue;
+                if (ld64(&S.res[S.g.rev[e]]) <= 0) continue;
+                if (cas32(&S.h[w], mark, L + 1)) {
+                    out[atomicAdd(&next_cnt, 1)] = w;
+                    add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
+                    wk.relabels++;
+                }
+            }
+        }
+        __syncthreads();
+        fsz = next_cnt;
+        fdeg = next_deg;
+        GAS int32_t* tmp = in;
+        in = out;
+        out = tmp;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) S.ctr->level = Lf;
+    work_flush(S, wk, red, WF_BFS);
+}
+
+void launch_local_bfs(hipStream_t st, const SlotView* slots, const int* list, int nlist, int lcap) {
+    hipLaunchKernelGGL(k_local_bfs, dim3(1, nlist), dim3(1024), 0, st, slots, list, lcap);
+}
+
+__global__ __launch_bounds__(256) void k_local_finish(const SlotView* __restrict__ slots,
+                                                      const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    const int n = S.g.n;
+    const int mark = n + 1;
+    const int Lf = S.ctr->level;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int v = (int)i;
+        if (!local_in_region(S, v)) continue;
+        const int hv = S.h[v];
+        const int ho = S.aux[v];
+        S.h[v] = max(ho, hv == mark ? Lf : hv);
+    }
+}
+
+void launch_local_finish(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx) {
+    hipLaunchKernelGGL(k_local_finish, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
 }
 
 // ------------------------------------------------ active list + counters (a15, a18)

@@ -1,0 +1,176 @@
+"""Dynamic mode and the drop-in command line on the GPU.
+
+Dynamic (reference: OnAdd_kernel BC:959-1000, incremental update
+BC:2273-2335; the reference handles capacity increases only and never calls
+its update path from main): pinned queries keep their residual, capacity
+updates (increases and decreases) are applied to it, and the warm re-solve
+must give exactly the flow of a cold solve on the updated graph and of the
+oracle.
+
+CLI (reference main, BC:2561-2933): same arguments, inputs and res_mt.txt /
+res_dynamic.txt lines; flows compared with the oracle on the original ids.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import mscresearch_amd as M
+import oracle as O
+from gpu_util import log
+
+pytestmark = pytest.mark.gpu
+
+
+def updated_round(rng, cap, fwd_entries, frac):
+    k = max(1, int(frac * len(fwd_entries)))
+    pick = rng.choice(fwd_entries, k, replace=False)
+    factor = rng.uniform(0.5, 2.0, k)
+    new = np.maximum(1, np.rint(cap[pick] * factor)).astype(np.int64)
+    return pick.astype(np.int32), new
+
+
+@pytest.mark.parametrize("gen", ["rmat", "grid", "uniform"])
+def test_warm_resolve_matches_cold_and_oracle(gen):
+    rng = np.random.default_rng(7)
+    if gen == "rmat":
+        e = M.Edges.rmat(12, 10**6, 5001)
+        qs, qt = e.hub_queries(8, 5002)
+    elif gen == "grid":
+        e = M.Edges.grid(40, 40, 1000, 5003)
+        qs, qt = M.grid_queries(40, 40, 8, 5004)
+    else:
+        e = M.Edges.uniform(3000, 30000, 100, 5005)
+        qs, qt = M.uniform_queries(3000, 8, 5006)
+    hg = M.HostGraph.from_edgelist(e).partition(64)
+    d = hg.export()
+    n = len(d["new_id"])
+    cap = d["cap"].copy()
+    fwd = np.nonzero(cap > 0)[0]
+    dg = M.DeviceGraph(hg, 512)
+    eng = M.Engine(8)
+    gid = eng.add_graph(dg)
+    q = [(int(d["new_id"][a]), int(d["new_id"][b])) for a, b in zip(qs, qt)]
+    for i, (s, t) in enumerate(q):
+        eng.pin(i, gid, s, t, i)
+    res = {r.tag: r for r in eng.solve_pinned(warm=False)}
+    og = O.Graph(n, d["row_ptr"], d["col"], cap, d["rev"])
+    for i, (s, t) in enumerate(q):
+        assert res[i].flow == O.maxflow(og, s, t)
+    for rnd in range(4):
+        ent, new = updated_round(rng, cap, fwd, 0.05)
+        eng.update_caps(gid, ent, new)
+        cap[ent] = new
+        og = O.Graph(n, d["row_ptr"], d["col"], cap, d["rev"])
+        warm = {r.tag: r for r in eng.solve_pinned(warm=True)}
+        # cold: a fresh upload of the updated graph, solved from scratch
+        hg2 = M.HostGraph.from_edges(n, *edges_from_csr(d, cap)).partition(64)
+        d2 = hg2.export()
+        ceng = M.Engine(1)
+        cgid = ceng.add_graph(M.DeviceGraph(hg2, 512))
+        for i, (s, t) in enumerate(q):
+            want = O.maxflow(og, s, t)
+            log(f"{gen} round {rnd} q{i}: warm={warm[i].flow} oracle={want} drains={warm[i].drains}")
+            assert warm[i].flow == want, (gen, rnd, i)
+            assert warm[i].cert_ok == 1
+            cold = ceng.solve(cgid, int(d2["new_id"][s]), int(d2["new_id"][t]))
+            assert cold.flow == want
+
+
+def edges_from_csr(d, cap):
+    """Forward entries of a CSR as an edge list (region-ordered ids)."""
+    rp, col = d["row_ptr"], d["col"]
+    owner = np.repeat(np.arange(len(rp) - 1), np.diff(rp)).astype(np.int32)
+    fwd = cap > 0
+    return owner[fwd], col[fwd].astype(np.int32), cap[fwd]
+
+
+def test_cold_solve_after_update_equals_oracle():
+    """Queries submitted after an update (not pinned) see the new capacities."""
+    rng = np.random.default_rng(11)
+    e = M.Edges.uniform(2000, 16000, 100, 5101)
+    hg = M.HostGraph.from_edgelist(e).partition(32)
+    d = hg.export()
+    n = len(d["new_id"])
+    cap = d["cap"].copy()
+    dg = M.DeviceGraph(hg, 512)
+    eng = M.Engine(4)
+    gid = eng.add_graph(dg)
+    ent, new = updated_round(rng, cap, np.nonzero(cap > 0)[0], 0.1)
+    eng.update_caps(gid, ent, new)
+    cap[ent] = new
+    og = O.Graph(n, d["row_ptr"], d["col"], cap, d["rev"])
+    qs, qt = M.uniform_queries(n, 10, 5102)
+    for a, b in zip(qs, qt):
+        s, t = int(d["new_id"][a]), int(d["new_id"][b])
+        assert eng.solve(gid, s, t).flow == O.maxflow(og, s, t)
+
+
+def run_cli(tmp_path, mode, env=None):
+    out = tmp_path / ("out_" + mode)
+    r = subprocess.run([M.CLI_PATH, str(tmp_path / "in"), str(out), mode], capture_output=True, text=True,
+                       env={**os.environ, **(env or {})}, timeout=300)
+    return r, out
+
+
+def write_inputs(tmp_path, e, qs, qt, extra=""):
+    (tmp_path / "in").mkdir(exist_ok=True)
+    e.write(str(tmp_path / "in" / "graph.txt"))
+    lines = "".join(f"{a + 1} {b + 1}\n" for a, b in zip(qs, qt)) + extra
+    (tmp_path / "in" / "test.txt").write_text(lines)
+
+
+@pytest.mark.parametrize("slots", ["1", "8"])
+def test_cli_static_matches_oracle(tmp_path, slots):
+    e = M.Edges.uniform(10000, 100000, 100, 1001)
+    qs, qt = M.uniform_queries(10000, 24, 1002)
+    # plus invalid queries: out of range, id 0, s == t
+    write_inputs(tmp_path, e, qs, qt, extra="10001 5\n0 3\n7 7\n")
+    r, out = run_cli(tmp_path, "static", {"MSCMF_SLOTS": slots})
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = (out / "res_mt.txt").read_text().splitlines()
+    assert len(lines) == 27
+    n, us, vs, cs = O.parse_graph(str(tmp_path / "in" / "graph.txt"))
+    og = O.build_csr(n, us, vs, cs)
+    for i, (a, b) in enumerate(zip(qs, qt)):
+        flow, ms = map(int, lines[i].split())
+        assert flow == O.maxflow(og, a, b), i
+        assert ms >= 0
+    assert lines[24:] == ["0 0", "0 0", "0 0"]
+    assert (out / "metrics.json").exists()
+    assert "Total time:" in r.stderr
+
+
+def test_cli_dynamic_with_updates(tmp_path):
+    rng = np.random.default_rng(3)
+    e = M.Edges.rmat(11, 10**6, 5201)
+    qs, qt = e.hub_queries(12, 5202)
+    write_inputs(tmp_path, e, qs, qt)
+    n, us, vs, cs = e.arrays()
+    cap = cs.copy()
+    upd = []
+    rounds = 3
+    caps_after = {}
+    for rnd in range(1, rounds + 1):
+        pick = rng.choice(len(us), max(1, len(us) // 50), replace=False)
+        newc = np.maximum(1, np.rint(cap[pick] * rng.uniform(0.5, 2.0, len(pick)))).astype(np.int64)
+        for k, c in zip(pick, newc):
+            upd.append(f"{rnd} {us[k] + 1} {vs[k] + 1} {c}\n")
+        cap[pick] = newc
+        caps_after[rnd] = cap.copy()
+    (tmp_path / "in" / "updates.txt").write_text("".join(upd))
+    r, out = run_cli(tmp_path, "dynamic")
+    assert r.returncode == 0, r.stderr[-2000:]
+    base = (out / "res_dynamic.txt").read_text().splitlines()
+    og0 = O.build_csr(n, us, vs, cs)
+    for i, (a, b) in enumerate(zip(qs, qt)):
+        assert int(base[i].split()[0]) == O.maxflow(og0, a, b)
+    rows = [list(map(int, l.split())) for l in (out / "res_dynamic_rounds.txt").read_text().splitlines()]
+    # windows of 10 queries each restart from graph.txt and replay the rounds
+    assert len(rows) == rounds * len(qs)
+    for rnd, want_caps in caps_after.items():
+        og = O.build_csr(n, us, vs, want_caps)
+        for rd, qi, flow, ms in rows:
+            if rd == rnd:
+                assert flow == O.maxflow(og, qs[qi - 1], qt[qi - 1]), (rnd, qi)

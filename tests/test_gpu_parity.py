@@ -101,3 +101,106 @@ def test_induced_subgraph_bit_exact(rmat12):
         for k in ("row_ptr", "col", "cap", "rev"):
             assert np.array_equal(got[k], getattr(sub, k)), k
         assert np.array_equal(got["old2sub"], o2s)
+
+
+def tile_of(case, tile_max):
+    """Tiles as the upload cuts them: each region in runs of tile_max."""
+    rs = case.d["rstart"]
+    starts = np.concatenate([np.arange(rs[r], rs[r + 1], tile_max) for r in range(len(rs) - 1)])
+    return np.searchsorted(starts, np.arange(case.n), side="right") - 1, len(starts)
+
+
+def local_relabel_expected(og, res, ex, h, tof, active, s, t, lcap):
+    """Largest valid labelling of R' = {v in active tiles, h(v) < lcap, v != s}
+    below lcap, heights outside R' fixed, by Dijkstra with unit weights from
+    the seeds (h == 0: t or a deficit) and from the fixed heights around R'."""
+    import heapq
+
+    n = og.n
+    rp, col, rev = og.row_ptr, og.col, og.rev
+    inR = active[tof] > 0
+    Rp = inR & (h < lcap)
+    Rp[s] = False
+    seed = Rp & (h == 0) & ((np.arange(n) == t) | (ex < 0))
+    marked = Rp & ~seed
+    dist = np.full(n, np.iinfo(np.int64).max, np.int64)
+    heap = []
+    for u in np.nonzero(marked)[0]:
+        best = dist[u]
+        for e in range(rp[u], rp[u + 1]):
+            x = col[e]
+            if res[e] > 0 and not marked[x]:
+                best = min(best, int(h[x]) + 1)
+        dist[u] = best
+        if best < lcap:
+            heap.append((best, int(u)))
+    for u in np.nonzero(seed)[0]:
+        dist[u] = 0
+        heap.append((0, int(u)))
+    heapq.heapify(heap)
+    while heap:
+        d, v = heapq.heappop(heap)
+        if d != dist[v] or d + 1 >= lcap:
+            continue
+        for e in range(rp[v], rp[v + 1]):
+            w = col[e]
+            if marked[w] and res[rev[e]] > 0 and dist[w] > d + 1:
+                dist[w] = d + 1
+                heapq.heappush(heap, (d + 1, int(w)))
+    out = h.astype(np.int64).copy()
+    m = np.nonzero(marked)[0]
+    out[m] = np.maximum(h[m], np.minimum(dist[m], lcap))
+    return out.astype(np.int32), marked
+
+
+@pytest.mark.parametrize("which", ["c1", "rmat12", "grid"])
+def test_local_relabel_bit_exact(which, request):
+    """The local relabel only raises heights, keeps a valid labelling valid
+    and gives the largest valid labelling of the relabelled set below lcap."""
+    case = request.getfixturevalue(which)
+    tmax = 256 if which == "grid" else 4096
+    tof, ntiles = tile_of(case, tmax)
+    assert ntiles == case.dg.info()["ntiles"]
+    e, gid = engine_with(case)
+    rng = np.random.default_rng(91)
+    if which == "rmat12":
+        s_, t_ = case.edges.hub_queries(3, 1204)
+    elif which == "grid":
+        s_, t_ = M.grid_queries(48, 64, 3, 1304)
+    else:
+        s_, t_ = M.uniform_queries(case.n, 3, 1004)
+    n = case.n
+    og = case.og
+    owner = np.repeat(np.arange(n), np.diff(og.row_ptr))
+    for a, b in zip(s_, t_):
+        s, t = case.q(a, b)
+        for state in ("preflow", "maxflow"):
+            if state == "preflow":
+                res, ex = O.init_preflow(og, s)
+            else:
+                _, res = O.maxflow(og, s, t, want_residual=True)
+                ex = np.zeros(n, np.int64)
+                ex[rng.choice(n, 4, replace=False)] = -2
+            hgr = O.bfs_heights(og, res, ex, s, t)
+            for shrink in ("half", "clip"):
+                h0 = (hgr // 2) if shrink == "half" else np.minimum(hgr, 3)
+                h0 = h0.astype(np.int32)
+                h0[s] = n
+                for act, lcap in (("some", 5), ("some", min(256, n)), ("all", min(256, n))):
+                    active = (rng.random(ntiles) < 0.6).astype(np.int32) if act == "some" else np.ones(ntiles, np.int32)
+                    log(f"local {which} s={s} t={t} {state} {shrink} {act} lcap={lcap}")
+                    h1, lvl = e.dbg_local_relabel(gid, s, t, res, ex, h0, active, lcap)
+                    assert lvl == lcap
+                    want, marked = local_relabel_expected(og, res, ex, h0, tof, active, s, t, lcap)
+                    assert np.array_equal(h1, want)
+                    assert np.all(h1 >= h0)
+                    # validity: every residual entry u -> v off s has h(u) <= h(v) + 1
+                    live = (res > 0) & (owner != s)
+                    assert np.all(h1[owner[live]] <= h1[og.col[live]] + 1)
+                    # fixpoint of the relabelled vertices
+                    lo = np.full(n, np.iinfo(np.int64).max, np.int64)
+                    np.minimum.at(lo, owner[res > 0], h1[og.col[res > 0]].astype(np.int64) + 1)
+                    m = np.nonzero(marked)[0]
+                    assert np.array_equal(h1[m], np.maximum(h0[m], np.minimum(lo[m], lcap)))
+                    if act == "all":  # one region: the exact global relabel, capped
+                        assert np.array_equal(h1[m], np.minimum(hgr[m], lcap))

@@ -116,6 +116,9 @@ def test_edge_cases():
     dict(gr_freq=1, gr_freq_min=1, gr_freq_max=1),
     dict(K=1, gr_freq=1000, gr_freq_min=1000, gr_freq_max=1000),
     dict(prune=0),
+    dict(local_relabel=1),
+    dict(local_relabel=1, local_band=1 << 30, gr_stall=0, gr_work=0.0),  # local relabel between all discharges
+    dict(local_relabel=1, h_cut=0, rm_freq0=1, rm_freq_max=1),       # ... and RM on every iteration
 ])
 def test_scheduler_variants_stay_exact(c1, variant):
     log(f"variant {variant}")
