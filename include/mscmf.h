@@ -12,7 +12,7 @@
  * entry points replace.  The reference has no FFI of its own; its in-process
  * boundary is run_gpu_static_with_state (BC:2119-2123) and its process
  * boundary is the `<in> <out> static|dynamic` command line (BC:2561-2933),
- * which tools/mscmf (mf_cli.cpp) reproduces on top of this ABI.
+ * which the mscmf binary (mf_cli.cpp) reproduces on top of this ABI.
  */
 #ifndef MSCMF_H
 #define MSCMF_H

@@ -31,11 +31,13 @@ struct Ctr {
     int64_t fedges[4];       // row-length sum of the level-L frontier, same ring
     int32_t level;           // levels completed by the running BFS
     int32_t visited;         // vertices labelled by the running BFS
-    int32_t nhl;             // entries in the heavy-row work list
+    int32_t nhl;             // entries in the heavy-row work list (front of hl)
+    int32_t nhg;             // giant rows of a top-down level (back of hl)
     int32_t active;          // excess>0, h<n, not s/t
     int32_t active_high;     // ... with h > H_cut
     int32_t nlight;          // light active-list entries (front of the list)
     int32_t nheavy;          // heavy entries (back of the list)
+    int32_t ngiant;          // active rows longer than kGiantDeg, listed in hl
     int64_t pos_excess;      // sum of positive excess off s,t
     int32_t neg_count;       // deficits (excess<0) off s,t
     int32_t fixed_edges;     // RM repairs in the last RM pass

@@ -246,10 +246,11 @@ __global__ void k_reset(const SlotView* __restrict__ slots, const int* __restric
             for (int i = 0; i < 4; ++i) { c->fsize[i] = 0; c->fedges[i] = 0; }
             c->level = 0;
             c->nhl = 0;
+            c->nhg = 0;
             c->visited = 0;
         }
         if (what & RST_ACTIVE) {
-            c->active = 0; c->active_high = 0; c->nlight = 0; c->nheavy = 0;
+            c->active = 0; c->active_high = 0; c->nlight = 0; c->nheavy = 0; c->ngiant = 0;
             c->pos_excess = 0; c->neg_count = 0; c->neg_sum = 0;
             c->hmin = INT_MAX; c->hmax = -1;
         }
@@ -497,8 +498,12 @@ __global__ __launch_bounds__(256) void k_bfs_td(const SlotView* __restrict__ slo
         const int v = in[i];
         wk.verts++;
         const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
+        // k_bfs_td_heavy spreads the longer rows over the grid: medium rows
+        // from the front of hl (a wave each), giant rows from the back (a
+        // workgroup each)
         if (deg <= kLightDeg) td_lane<MODE>(S, v, L, lvl, st, out, ocnt, odeg, wk);
-        else S.hl[add32(&S.ctr->nhl, 1)] = v;  // k_bfs_td_heavy spreads these over the grid
+        else if (deg <= kGiantDeg) S.hl[add32(&S.ctr->nhl, 1)] = v;
+        else S.hl[S.g.n - 1 - add32(&S.ctr->nhg, 1)] = v;
     }
     stage_flush(st, out, ocnt, odeg);
     const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
@@ -506,17 +511,40 @@ __global__ __launch_bounds__(256) void k_bfs_td(const SlotView* __restrict__ slo
     work_flush(S, wk, red, WF_BFS);
 }
 
-// Heavy rows of a top-down level: one row per workgroup at a time, rows taken
-// grid-stride from the level's heavy list.  Hubs sit together at region
-// starts, so handing them out across the grid is what balances the level.
+// Top-down expansion of one (medium) frontier vertex by one wave.
+template <int MODE, int CAP>
+__device__ __forceinline__ void td_wave(const GSlot& S, int v, int L, GAS int32_t* lvl,
+                                        Stage<CAP>& st, GAS int32_t* out, GAS int32_t* ocnt,
+                                        GAS int64_t* odeg, Work& wk) {
+    const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+    const int n = S.g.n;
+    if (lane_id() == 0) wk.edges += (uint64_t)(re - rb);
+    for (int e = rb + lane_id(); e < re; e += WAVE) {
+        const int w = S.g.col[e];
+        if (lvl[w] != n) continue;
+        if (!bfs_edge_ok<MODE>(S, e, w)) continue;
+        if (cas32(&lvl[w], n, L + 1)) {
+            const int dw = S.g.row_ptr[w + 1] - S.g.row_ptr[w];
+            stage_push(st, w, dw, out, ocnt, odeg);
+            wk.relabels++;
+        }
+    }
+}
+
+// Long rows of a top-down level, taken grid-stride from the level's lists:
+// giant rows (back of hl) a workgroup each, then medium rows (front of hl) a
+// wave each.  Hubs sit together at region starts, so handing them out across
+// the grid is what balances the level.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_bfs_td_heavy(const SlotView* __restrict__ slots,
                                                       const int* __restrict__ list) {
     const GSlot S = load_slot(slots, list);
     __shared__ Stage<2048> st;
     __shared__ uint64_t red[256 / WAVE];
-    const int nh = S.ctr->nhl;
-    if ((int)blockIdx.x >= nh) return;  // uniform per block
+    const int nm = S.ctr->nhl;
+    const int ng = S.ctr->nhg;
+    const int wpb = (int)(blockDim.x / WAVE);
+    if ((int)blockIdx.x >= ng && (int)blockIdx.x * wpb >= nm) return;  // uniform per block
     const int L = S.ctr->level;
     GAS int32_t* lvl = bfs_lvl<MODE>(S);
     GAS int32_t* out = (L & 1) ? S.fa : S.fb;
@@ -525,7 +553,10 @@ __global__ __launch_bounds__(256) void k_bfs_td_heavy(const SlotView* __restrict
     stage_init(st);
     __syncthreads();
     Work wk;
-    for (int k = blockIdx.x; k < nh; k += gridDim.x) td_block<MODE>(S, S.hl[k], L, lvl, st, out, ocnt, odeg, wk);
+    for (int k = blockIdx.x; k < ng; k += gridDim.x)
+        td_block<MODE>(S, S.hl[S.g.n - 1 - k], L, lvl, st, out, ocnt, odeg, wk);
+    const int gw = (int)blockIdx.x * wpb + (int)(threadIdx.x / WAVE);
+    for (int k = gw; k < nm; k += (int)gridDim.x * wpb) td_wave<MODE>(S, S.hl[k], L, lvl, st, out, ocnt, odeg, wk);
     stage_flush(st, out, ocnt, odeg);
     const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
     if (threadIdx.x == 0 && nv) add32(&S.ctr->visited, (int32_t)nv);
@@ -538,6 +569,8 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
     const GSlot S = load_slot(slots, list);
     __shared__ Stage<2048> st;
     __shared__ uint64_t red[256 / WAVE];
+    __shared__ int med[256];
+    __shared__ int nmed;
     const int L = S.ctr->level;
     GAS int32_t* lvl = bfs_lvl<MODE>(S);
     GAS int32_t* out = (L & 1) ? S.fa : S.fb;
@@ -551,28 +584,60 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
     __syncthreads();
     const int n = S.g.n;
     Work wk;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int v = (int)i;
-        if (lvl[v] != n) continue;
-        if (!bfs_vertex_ok<MODE>(S, v)) continue;
-        wk.verts++;
-        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
-        if (re - rb > kGiantDeg) {  // k_bfs_bu_heavy scans it with a whole workgroup
-            S.hl[add32(&S.ctr->nhl, 1)] = v;
-            continue;
+    const int lane = lane_id();
+    const int wpb = (int)(blockDim.x / WAVE);
+    // chunks of blockDim vertices: rows up to kLightDeg by their lane, longer
+    // rows by a wave of the block (ballot, stop at the first parent), giant
+    // rows by k_bfs_bu_heavy (a whole workgroup)
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+        if (threadIdx.x == 0) nmed = 0;
+        __syncthreads();
+        const int v = (int)(base + threadIdx.x);
+        if (v < n && lvl[v] == n && bfs_vertex_ok<MODE>(S, v)) {
+            wk.verts++;
+            const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+            if (re - rb > kGiantDeg) {
+                S.hl[add32(&S.ctr->nhl, 1)] = v;
+            } else if (re - rb > kLightDeg) {
+                med[atomicAdd(&nmed, 1)] = v;
+            } else {
+                int e = rb;
+                for (; e < re; ++e) {
+                    if (lvl[S.g.col[e]] != L) continue;
+                    if (bfs_bu_edge_ok<MODE>(S, e)) break;
+                }
+                wk.edges += (uint64_t)(min(e + 1, re) - rb);
+                if (e < re) {
+                    lvl[v] = L + 1;  // only this lane writes v
+                    stage_push(st, v, re - rb, out, ocnt, odeg);
+                    wk.relabels++;
+                }
+            }
         }
-        int e = rb;
-        for (; e < re; ++e) {
-            if (lvl[S.g.col[e]] != L) continue;
-            if (bfs_bu_edge_ok<MODE>(S, e)) break;
+        __syncthreads();
+        const int nm = nmed;
+        for (int k = (int)(threadIdx.x / WAVE); k < nm; k += wpb) {
+            const int u = med[k];
+            const int rb = S.g.row_ptr[u], re = S.g.row_ptr[u + 1];
+            bool hit = false;
+            int scanned = 0;
+            for (int e0 = rb; e0 < re; e0 += WAVE) {
+                const int e = e0 + lane;
+                bool ok = false;
+                if (e < re) ok = (lvl[S.g.col[e]] == L) && bfs_bu_edge_ok<MODE>(S, e);
+                scanned += min(WAVE, re - e0);
+                if (__ballot(ok)) { hit = true; break; }
+            }
+            if (lane == 0) {
+                wk.edges += (uint64_t)scanned;
+                if (hit) {
+                    lvl[u] = L + 1;  // only this wave writes u
+                    stage_push(st, u, re - rb, out, ocnt, odeg);
+                    wk.relabels++;
+                }
+            }
         }
-        wk.edges += (uint64_t)(min(e + 1, re) - rb);
-        if (e < re) {
-            lvl[v] = L + 1;  // only this lane writes v
-            stage_push(st, v, re - rb, out, ocnt, odeg);
-            wk.relabels++;
-        }
+        __syncthreads();  // med / nmed are reused by the next chunk
     }
     stage_flush(st, out, ocnt, odeg);
     const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
@@ -634,12 +699,15 @@ __global__ void k_bfs_advance(const SlotView* __restrict__ slots, const int* __r
     if (threadIdx.x == 0) {
         S.ctr->level = S.ctr->level + 1;
         S.ctr->nhl = 0;
+        S.ctr->nhg = 0;
     }
 }
 
 void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
                    int mode) {
-    const int gh = gridx < 2048 ? gridx : 2048;
+    // gridx = frontier / 256 workgroups; the long rows (at most the frontier)
+    // get up to 2048 workgroups of 4 waves
+    const int gh = gridx * 64 < 2048 ? gridx * 64 : 2048;
     if (mode == BFS_GR) {
         hipLaunchKernelGGL(k_bfs_td<BFS_GR>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
         hipLaunchKernelGGL(k_bfs_td_heavy<BFS_GR>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
@@ -1068,6 +1136,11 @@ __global__ __launch_bounds__(256) void k_active(const SlotView* __restrict__ slo
         } else {
             const int k = add32(&S.ctr->nheavy, 1);
             S.fa[n - 1 - k] = v;
+            if (deg > kGiantDeg) {  // also to the giant list: row minimum slot starts empty
+                const int g = add32(&S.ctr->ngiant, 1);
+                S.hl[g] = v;
+                S.hl[n - 1 - g] = INT_MAX;
+            }
         }
     }
     stage_flush(st, S.fa, &S.ctr->nlight, (GAS int64_t*)nullptr);
@@ -1273,65 +1346,6 @@ __device__ __forceinline__ int64_t block_incl_scan_i64(int64_t x, int64_t* scrat
     return incl + before;
 }
 
-// Giant row: the whole workgroup runs the discharge cycle of lane_discharge /
-// wave_discharge (lowest residual neighbour, relabel, pushes in row order),
-// with a block min-reduction and a block prefix sum over residuals.
-__device__ void block_discharge(const GSlot& S, const TileCtx& tc, int v, int cycles, Work& wk,
-                                int* iscr, int64_t* lscr, int64_t* bcast) {
-    const int n = S.g.n;
-    const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
-    if (threadIdx.x == 0) { bcast[0] = own_excess(S, tc, v); bcast[1] = own_h(S, tc, v); }
-    __syncthreads();
-    int64_t ex = bcast[0];
-    int hv = (int)bcast[1];
-    __syncthreads();
-    for (int c = 0; c < cycles; ++c) {
-        if (ex <= 0 || hv >= n) break;
-        int hmin = INT_MAX;
-        for (int e = rb + (int)threadIdx.x; e < re; e += blockDim.x)
-            if (ld64(&S.res[e]) > 0) hmin = min(hmin, nbr_h(S, tc, S.g.col[e]));
-        hmin = block_min_i32(hmin, iscr);
-        if (threadIdx.x == 0) wk.edges += (uint64_t)(re - rb);
-        if (hmin >= n) {
-            hv = n;
-            if (threadIdx.x == 0) { set_h(S, tc, v, hv); wk.relabels++; }
-            break;
-        }
-        if (hv <= hmin) {
-            hv = hmin + 1;
-            if (threadIdx.x == 0) { set_h(S, tc, v, hv); wk.relabels++; }
-        }
-        int64_t rem = ex;
-        for (int base = rb; base < re && rem > 0; base += blockDim.x) {
-            const int e = base + (int)threadIdx.x;
-            int64_t r = 0;
-            if (e < re) {
-                r = ld64(&S.res[e]);
-                if (r > 0 && nbr_h(S, tc, S.g.col[e]) != hmin) r = 0;
-                if (r < 0) r = 0;
-            }
-            int64_t chunk = 0;
-            const int64_t incl = block_incl_scan_i64(r, lscr, &chunk);
-            const int64_t excl = incl - r;
-            const int64_t d = (r > 0 && rem > excl) ? ((rem - excl) < r ? (rem - excl) : r) : 0;
-            if (d > 0) {
-                add64(&S.res[e], -d);
-                add64(&S.res[S.g.rev[e]], d);
-                push_into(S, tc, S.g.col[e], d);
-                wk.pushes++;
-            }
-            rem -= (chunk < rem ? chunk : rem);
-        }
-        const int64_t tot = ex - rem;
-        if (tot > 0) {
-            if (threadIdx.x == 0) bcast[0] = take_excess(S, tc, v, ex, tot);
-            __syncthreads();
-            ex = bcast[0];
-            __syncthreads();
-        }
-    }
-}
-
 // Active-list discharge: light entries one lane each, heavy entries one
 // wave each.  Blocks [0, light_blocks) take light entries.
 __global__ __launch_bounds__(256) void k_discharge_list(const SlotView* __restrict__ slots,
@@ -1357,28 +1371,148 @@ __global__ __launch_bounds__(256) void k_discharge_list(const SlotView* __restri
     work_flush(S, wk, red, WF_DISCHARGE);
 }
 
-// Giant rows of the active list: one workgroup per row, rows grid-stride.
-__global__ __launch_bounds__(1024) void k_discharge_giant(const SlotView* __restrict__ slots,
-                                                          const int* __restrict__ list) {
+// Giant rows (longer than kGiantDeg) of the active list, spread over the
+// whole grid in chunks of kGiantChunk entries: k_giant_min takes the lowest
+// height over residual entries of every chunk into hl[n-1-g]; k_giant_push
+// relabels (lowest + 1, or n) and has every chunk claim excess from the row's
+// vertex with a compare-and-swap that never overdraws, then push the claimed
+// amount to the chunk's entries at the lowest height in prefix-sum order.
+// Only the row's chunks take excess or residual out of the row, so the
+// lower-bound reads of the single-owner scheme still hold.  Chunks are dealt
+// round-robin across workgroups, continuing from the previous row's last.
+constexpr int kGiantChunk = 2048;
+
+template <typename P>
+__device__ __forceinline__ bool cas64(P p, int64_t& expect, int64_t want) {
+    return __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(256) void k_giant_min(const SlotView* __restrict__ slots,
+                                                   const int* __restrict__ list) {
     const GSlot S = load_slot(slots, list);
-    __shared__ uint64_t red[1024 / WAVE];
-    __shared__ int iscr[1024 / WAVE];
-    __shared__ int64_t lscr[1024 / WAVE];
-    __shared__ int64_t bcast[2];
-    const int nh = S.ctr->nheavy;
-    TileCtx tc;
+    __shared__ int iscr[256 / WAVE];
+    __shared__ uint64_t red[256 / WAVE];
+    const int ng = S.ctr->ngiant;
+    if (ng == 0) return;
+    const int n = S.g.n;
+    const int G = (int)gridDim.x;
     Work wk;
-    for (int k = blockIdx.x; k < nh; k += gridDim.x) {
-        const int v = S.fa[S.g.n - 1 - k];
-        if (S.g.row_ptr[v + 1] - S.g.row_ptr[v] > kGiantDeg) block_discharge(S, tc, v, S.Kh, wk, iscr, lscr, bcast);
+    int off = 0;
+    for (int k = 0; k < ng; ++k) {
+        const int v = S.hl[k];
+        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+        const int nch = (re - rb + kGiantChunk - 1) / kGiantChunk;
+        const int c0 = ((int)blockIdx.x - off % G + G) % G;
+        off += nch;
+        // nothing pushes during this kernel, so every workgroup sees the same excess
+        if (ld64(&S.excess[v]) <= 0 || S.h[v] >= n) continue;
+        for (int c = c0; c < nch; c += G) {
+            const int e0 = rb + c * kGiantChunk;
+            const int e1 = min(re, e0 + kGiantChunk);
+            int m = INT_MAX;
+            for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x)
+                if (ld64(&S.res[e]) > 0) m = min(m, S.h[S.g.col[e]]);
+            m = block_min_i32(m, iscr);
+            if (threadIdx.x == 0) {
+                // INT_MAX - 1: the row was scanned and has no residual entry here
+                min32(&S.hl[n - 1 - k], m == INT_MAX ? INT_MAX - 1 : m);
+                wk.edges += (uint64_t)(e1 - e0);
+            }
+        }
     }
     work_flush(S, wk, red, WF_DISCHARGE);
+}
+
+__global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__ slots,
+                                                    const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ int64_t lscr[256 / WAVE];
+    __shared__ int64_t bcast;
+    __shared__ uint64_t red[256 / WAVE];
+    constexpr int PER = kGiantChunk / 256;
+    const int ng = S.ctr->ngiant;
+    if (ng == 0) return;
+    const int n = S.g.n;
+    const int G = (int)gridDim.x;
+    Work wk;
+    int off = 0;
+    for (int k = 0; k < ng; ++k) {
+        const int v = S.hl[k];
+        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+        const int nch = (re - rb + kGiantChunk - 1) / kGiantChunk;
+        const int c0 = ((int)blockIdx.x - off % G + G) % G;
+        off += nch;
+        const int hmin = S.hl[n - 1 - k];
+        if (hmin == INT_MAX) continue;  // k_giant_min skipped the row: inactive
+        if (hmin >= n - 1) {            // no residual entry leads below n - 1: v cannot reach t
+            if (c0 == 0 && threadIdx.x == 0 && S.h[v] < n) { S.h[v] = n; wk.relabels++; }
+            continue;
+        }
+        if (c0 == 0 && threadIdx.x == 0 && S.h[v] <= hmin) { S.h[v] = hmin + 1; wk.relabels++; }
+        for (int c = c0; c < nch; c += G) {
+            // stop once the row's vertex has no excess left to claim
+            if (threadIdx.x == 0) bcast = ld64(&S.excess[v]);
+            __syncthreads();
+            const bool dry = bcast <= 0;
+            __syncthreads();
+            if (dry) break;
+            const int e0 = rb + c * kGiantChunk;
+            const int e1 = min(re, e0 + kGiantChunk);
+            int64_t r[PER];
+            int64_t tsum = 0;
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
+                r[j] = 0;
+                if (e < e1) {
+                    const int64_t x = ld64(&S.res[e]);
+                    if (x > 0 && S.h[S.g.col[e]] == hmin) r[j] = x;
+                }
+                tsum += r[j];
+            }
+            int64_t tot = 0;
+            const int64_t incl = block_incl_scan_i64(tsum, lscr, &tot);
+            if (threadIdx.x == 0) wk.edges += (uint64_t)(e1 - e0);
+            if (tot == 0) continue;
+            if (threadIdx.x == 0) {
+                int64_t cur = ld64(&S.excess[v]);
+                int64_t a = 0;
+                while (cur > 0) {
+                    a = cur < tot ? cur : tot;
+                    if (cas64(&S.excess[v], cur, cur - a)) break;
+                }
+                bcast = cur > 0 ? a : 0;
+            }
+            __syncthreads();
+            int64_t rem = bcast - (incl - tsum);
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                if (rem <= 0) break;
+                if (r[j] <= 0) continue;
+                const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
+                const int64_t d = rem < r[j] ? rem : r[j];
+                add64(&S.res[e], -d);
+                add64(&S.res[S.g.rev[e]], d);
+                add64(&S.excess[S.g.col[e]], d);
+                rem -= d;
+                wk.pushes++;
+            }
+        }
+    }
+    work_flush(S, wk, red, WF_DISCHARGE);
+}
+
+void launch_giant(hipStream_t st, const SlotView* slots, const int* list, int nlist) {
+    hipLaunchKernelGGL(k_giant_min, dim3(512, nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_giant_push, dim3(512, nlist), dim3(256), 0, st, slots, list);
 }
 
 void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* list, int nlist,
                            int gridx) {
     hipLaunchKernelGGL(k_discharge_list, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
-    hipLaunchKernelGGL(k_discharge_giant, dim3(256, nlist), dim3(1024), 0, st, slots, list);
+    launch_giant(st, slots, list, nlist);
 }
 
 // Region (tile) discharge (BC:432-515, one block per region): the tile's
@@ -1397,9 +1531,6 @@ __global__ __launch_bounds__(512) void k_discharge_tile(const SlotView* __restri
     __shared__ int32_t alist[kTileMax];
     __shared__ int nlight, nheavy;
     __shared__ uint64_t red[512 / WAVE];
-    __shared__ int giscr[512 / WAVE];
-    __shared__ int64_t glscr[512 / WAVE];
-    __shared__ int64_t gbcast[2];
     const int a = S.g.tile_start[tile], b = S.g.tile_start[tile + 1];
     const int size = b - a;  // <= kTileMax by construction of tile_start
     const int n = S.g.n;
@@ -1419,22 +1550,16 @@ __global__ __launch_bounds__(512) void k_discharge_tile(const SlotView* __restri
             if (ex <= 0) continue;
             const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
             if (deg <= kLightDeg) alist[atomicAdd(&nlight, 1)] = i;
-            else alist[size - 1 - atomicAdd(&nheavy, 1)] = i;
+            else if (deg <= kGiantDeg) alist[size - 1 - atomicAdd(&nheavy, 1)] = i;
+            // giant rows: k_giant_min / k_giant_push after the tile pass
         }
         if (threadIdx.x == 0) wk.verts += (uint64_t)size;
         __syncthreads();
         const int nl = nlight, nh = nheavy;
         if (nl + nh == 0) break;
         for (int k = threadIdx.x; k < nl; k += blockDim.x) lane_discharge(S, tc, a + alist[k], 1, wk);
-        for (int k = threadIdx.x / WAVE; k < nh; k += blockDim.x / WAVE) {
-            const int v = a + alist[size - 1 - k];
-            if (S.g.row_ptr[v + 1] - S.g.row_ptr[v] <= kGiantDeg) wave_discharge(S, tc, v, 1, wk);
-        }
-        __syncthreads();
-        for (int k = 0; k < nh; ++k) {  // giant rows: the whole workgroup
-            const int v = a + alist[size - 1 - k];
-            if (S.g.row_ptr[v + 1] - S.g.row_ptr[v] > kGiantDeg) block_discharge(S, tc, v, 1, wk, giscr, glscr, gbcast);
-        }
+        for (int k = threadIdx.x / WAVE; k < nh; k += blockDim.x / WAVE)
+            wave_discharge(S, tc, a + alist[size - 1 - k], 1, wk);
         __syncthreads();
     }
     __syncthreads();
@@ -1449,6 +1574,7 @@ void launch_discharge_tile(hipStream_t st, const SlotView* slots, const int* lis
                            int gridx, int hot_only, int rounds) {
     hipLaunchKernelGGL(k_discharge_tile, dim3(gridx, nlist), dim3(512), 0, st, slots, list, hot_only,
                        rounds);
+    if (!hot_only) launch_giant(st, slots, list, nlist);
 }
 
 // ------------------------------------------------------------ path push (a17)
