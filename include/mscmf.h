@@ -233,6 +233,12 @@ int mf_dbg_induced(mf_engine* e, int gid, const int32_t* mask, int* n_out, int64
 int mf_dbg_local_relabel(mf_engine* e, int gid, int s, int t, const int64_t* res,
                          const int64_t* excess, int32_t* height, const int32_t* tile_active,
                          int lcap, int* level);
+/* Min-cut certificate (mode 1 fingerprint, 2 entry by entry) of a given
+ * residual, excess and heights (T = {h < n}) on slot 0: violation count,
+ * cut(V\T, T), excess(t) and whether the pair-sum fingerprint is 0. */
+int mf_dbg_certify(mf_engine* e, int gid, int s, int t, const int64_t* res, const int64_t* excess,
+                   const int32_t* height, int mode, int64_t* bad, int64_t* cut, int64_t* excess_t,
+                   int* fp_zero);
 /* Solve without pruning on slot 0 and copy the final residual, excess and
  * heights out (for an independent CPU certificate). */
 int mf_dbg_solve_state(mf_engine* e, int gid, int s, int t, mf_result* r, int64_t* res,

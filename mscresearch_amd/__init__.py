@@ -117,6 +117,7 @@ SIGNATURES = {
     "mf_dbg_init_preflow": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P]),
     "mf_dbg_global_relabel": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
     "mf_dbg_local_relabel": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, C.c_int, _P]),
+    "mf_dbg_certify": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, C.c_int, _P, _P, _P, _P]),
     "mf_dbg_prune_mask": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.POINTER(C.c_int)]),
     "mf_dbg_induced": (C.c_int, [_P, C.c_int, _P, C.POINTER(C.c_int), C.POINTER(C.c_int64), _P, _P, _P, _P, _P]),
     "mf_dbg_solve_state": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(Result), _P, _P, _P]),
@@ -437,6 +438,16 @@ class Engine:
         _check(lib().mf_dbg_local_relabel(self._h, gid, int(s), int(t), _ptr(res), _ptr(excess), _ptr(h),
                                           _ptr(ta), int(lcap), C.byref(lvl)))
         return h, lvl.value
+
+    def dbg_certify(self, gid, s, t, res, excess, h, mode):
+        """GPU certificate of a given state: (bad, cut, excess_t, fingerprint_zero)."""
+        res = np.ascontiguousarray(res, np.int64)
+        excess = np.ascontiguousarray(excess, np.int64)
+        h = np.ascontiguousarray(h, np.int32)
+        bad, cut, ext, fz = C.c_int64(0), C.c_int64(0), C.c_int64(0), C.c_int(0)
+        _check(lib().mf_dbg_certify(self._h, gid, int(s), int(t), _ptr(res), _ptr(excess), _ptr(h), int(mode),
+                                    C.byref(bad), C.byref(cut), C.byref(ext), C.byref(fz)))
+        return bad.value, cut.value, ext.value, bool(fz.value)
 
     def dbg_prune_mask(self, gid, s, t, n):
         mask = np.empty(n, np.int32)

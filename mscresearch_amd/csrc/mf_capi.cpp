@@ -640,6 +640,38 @@ int mf_dbg_local_relabel(mf_engine* e, int gid, int s, int t, const int64_t* res
     return rc;
 }
 
+int mf_dbg_certify(mf_engine* e, int gid, int s, int t, const int64_t* res, const int64_t* excess,
+                   const int32_t* height, int mode, int64_t* bad, int64_t* cut, int64_t* excess_t,
+                   int* fp_zero) {
+    int rc = check_query(e, gid, s, t);
+    if (rc != MF_OK) return rc;
+    if (!res || !excess || !height || !bad || !cut || !excess_t || !fp_zero) return fail(MF_EINVAL, "null argument");
+    if (mode != 1 && mode != 2) return fail(MF_EINVAL, "mode must be 1 or 2");
+    rc = e->eng.dbg_begin(gid, s, t);
+    if (rc != MF_OK) return rc;
+    SlotView& v = e->eng.view(0);
+    const DGraph& g = v.g;
+    HIPCK(hipMemcpy(v.res, res, sizeof(int64_t) * g.E, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(v.excess, excess, sizeof(int64_t) * g.n, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(v.h, height, sizeof(int32_t) * g.n, hipMemcpyHostToDevice));
+    const int saved = e->eng.params().cert;
+    e->eng.set_cert_mode(mode);
+    Plan p;
+    p.stages = ST_RESET_PRE | ST_CERT;
+    p.reset_pre = RST_CERT | RST_WORK | RST_ERR;
+    rc = e->eng.dbg_plan(p);
+    e->eng.set_cert_mode(saved);
+    if (rc == MF_OK) {
+        const Ctr& c = e->eng.ctr(0);
+        *bad = c.cert_bad;
+        *cut = c.cut;
+        *excess_t = c.ex_t;
+        *fp_zero = c.fp == 0 ? 1 : 0;
+    }
+    e->eng.dbg_end();
+    return rc;
+}
+
 int mf_dbg_prune_mask(mf_engine* e, int gid, int s, int t, int32_t* mask, int* kept) {
     int rc = check_query(e, gid, s, t);
     if (rc != MF_OK) return rc;

@@ -22,6 +22,12 @@ struct DGraph {
     const int32_t* rev = nullptr;
     const int32_t* tile_start = nullptr;
     const int32_t* tile_of = nullptr;
+    // certificate constants, kept current by capacity updates: capsum[v] =
+    // sum of cap over v's row; *cert_fp = sum over entries of
+    // pair_weight(pair, cert_seed) * cap (mod 2^64)
+    int64_t* capsum = nullptr;
+    uint64_t* cert_fp = nullptr;
+    uint64_t cert_seed = 0;
 };
 
 // Per-slot device counters.  Written by kernels, copied to the host once per
@@ -37,7 +43,8 @@ struct Ctr {
     int32_t active_high;     // ... with h > H_cut
     int32_t nlight;          // light active-list entries (front of the list)
     int32_t nheavy;          // heavy entries (back of the list)
-    int32_t ngiant;          // active rows longer than kGiantDeg, listed in hl
+    int32_t ngiant;          // rows longer than kGiantDeg in the giant list
+    int32_t ngchunks;        // kGiantChunk-entry tasks over those rows
     int64_t pos_excess;      // sum of positive excess off s,t
     int32_t neg_count;       // deficits (excess<0) off s,t
     int32_t fixed_edges;     // RM repairs in the last RM pass
@@ -46,6 +53,7 @@ struct Ctr {
     int32_t cert_bad;        // certificate violations
     int32_t err;             // device-side error flags
     int64_t cut;             // certificate: cut(V\T,T)
+    uint64_t fp;             // certificate: pair-sum fingerprint (0 when every pair sums to its capacity)
     int64_t ex_t;            // excess of t seen by the last active / certificate pass
     // algorithmic work counters (roofline accounting), per kernel family:
     // wk[family][0] edge scans, [1] vertex scans, [2] pushes,
@@ -71,6 +79,12 @@ struct SlotView {
     int32_t* tile_cnt = nullptr; // ntiles active counts
     int32_t* tile_hot = nullptr; // ntiles hot flags
     int32_t* lbkt = nullptr;     // 2 * kLocalLevels local-relabel bucket counts and cursors
+    // giant-row list (rows longer than kGiantDeg; at most E / (kGiantDeg+1)
+    // of them, gcap): ids [0, gcap), per-row minimum [gcap, 2 gcap), chunk
+    // prefix [2 gcap, 3 gcap]; gacc[gcap] per-row int64 sums
+    int32_t* giant = nullptr;
+    int64_t* gacc = nullptr;
+    int32_t gcap = 0;
     const int32_t* mask = nullptr;  // optional vertex mask (nullptr = all)
     Ctr* ctr = nullptr;
     int32_t s = 0, t = 0;
@@ -86,5 +100,6 @@ constexpr int kGiantDeg = 2048;     // rows longer than this are scanned by a wh
 constexpr int kSmallFrontier = 16384;  // single-workgroup BFS keeps going below this
 constexpr int64_t kSmallEdges = 1 << 18; // ... and while the frontier's rows stay below this
 constexpr int kLocalLevels = 256;   // level cap of the local relabel
+constexpr int kGiantChunk = 2048;   // entries per workgroup task on a giant row
 
 }  // namespace mf

@@ -40,6 +40,7 @@ DeviceGraph::~DeviceGraph() {
     (void)hipFree(d_tile_start); (void)hipFree(d_tile_of);
     (void)hipFree(d_sub2old); (void)hipFree(d_old2sub); (void)hipFree(d_mask);
     (void)hipFree(d_wcc);
+    (void)hipFree(d_capsum); (void)hipFree(d_cert_fp);
 }
 
 DeviceGraph::FwdSet::~FwdSet() { (void)hipFree(d_mask); }
@@ -98,6 +99,22 @@ int upload_graph(const HostGraph& hg, int tile_max, DeviceGraph& dg) {
     if (n > 0) HIPCHK(hipMemcpy(dg.d_wcc, dg.wcc.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     dg.classes = true;
     dg.bytes += sizeof(int32_t) * (size_t)n;
+    return init_cert_consts(dg, nullptr);
+}
+
+int init_cert_consts(DeviceGraph& dg, hipStream_t st) {
+    static uint64_t graphs_made = 0;
+    const int n = dg.g.n;
+    HIPCHK(hipMalloc(&dg.d_capsum, sizeof(int64_t) * std::max(n, 1)));
+    HIPCHK(hipMalloc(&dg.d_cert_fp, sizeof(uint64_t)));
+    HIPCHK(hipMemsetAsync(dg.d_cert_fp, 0, sizeof(uint64_t), st));
+    dg.g.capsum = dg.d_capsum;
+    dg.g.cert_fp = dg.d_cert_fp;
+    dg.g.cert_seed = 0x9e3779b97f4a7c15ULL * (++graphs_made) ^ ((uint64_t)(uint32_t)n << 32) ^ (uint64_t)(uint32_t)dg.g.E;
+    launch_cert_consts(st, dg.g, dg.d_capsum, dg.d_cert_fp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    dg.bytes += sizeof(int64_t) * (size_t)n + sizeof(uint64_t);
     return MF_OK;
 }
 
@@ -105,12 +122,13 @@ int upload_graph(const HostGraph& hg, int tile_max, DeviceGraph& dg) {
 Slab::~Slab() {
     (void)hipFree(res); (void)hipFree(excess); (void)hipFree(h); (void)hipFree(fa); (void)hipFree(fb); (void)hipFree(lock);
     (void)hipFree(aux); (void)hipFree(hl); (void)hipFree(tile_cnt); (void)hipFree(tile_hot); (void)hipFree(lbkt);
-    (void)hipFree(ctr);
+    (void)hipFree(giant); (void)hipFree(gacc);
 }
 
 size_t Slab::bytes_for(int n, int64_t E, int ntiles) {
+    const size_t gcap = (size_t)E / (kGiantDeg + 1) + 1;
     return (size_t)E * 8 + (size_t)n * (8 + 4 * 6) + (size_t)ntiles * 8 + sizeof(int32_t) * 2 * kLocalLevels +
-           sizeof(Ctr);
+           gcap * (4 * 3 + 4 + 8);
 }
 
 int alloc_slab(int n, int64_t E, int ntiles, Slab& sl) {
@@ -127,8 +145,10 @@ int alloc_slab(int n, int64_t E, int ntiles, Slab& sl) {
     HIPCHK(hipMalloc(&sl.tile_cnt, 4 * tt));
     HIPCHK(hipMalloc(&sl.tile_hot, 4 * tt));
     HIPCHK(hipMalloc(&sl.lbkt, sizeof(int32_t) * 2 * kLocalLevels));
-    HIPCHK(hipMalloc(&sl.ctr, sizeof(Ctr)));
-    HIPCHK(hipMemset(sl.ctr, 0, sizeof(Ctr)));
+    // rows longer than kGiantDeg: at most E / (kGiantDeg + 1) of them
+    sl.gcap = (int)(ee / (kGiantDeg + 1) + 1);
+    HIPCHK(hipMalloc(&sl.giant, sizeof(int32_t) * (3 * (size_t)sl.gcap + 1)));
+    HIPCHK(hipMalloc(&sl.gacc, sizeof(int64_t) * (size_t)sl.gcap));
     sl.n_cap = n;
     sl.E_cap = (int)E;
     sl.tiles_cap = ntiles;
@@ -455,11 +475,12 @@ Plan SlotSched::plan(const Ctr& c) {
         return p;
     }
     case S_CERT: {
-        stats.cert_ok = (c.cert_bad == 0 && c.cut == c.ex_t) ? 1 : 0;
+        stats.cert_ok = (c.cert_bad == 0 && c.fp == 0 && c.cut == c.ex_t) ? 1 : 0;
         stats.flow = c.ex_t;
         if (!stats.cert_ok) {
             fail(MF_ECERT, "min-cut certificate failed: bad=" + std::to_string(c.cert_bad) +
-                               " cut=" + std::to_string(c.cut) + " excess_t=" + std::to_string(c.ex_t));
+                               " fingerprint=" + std::to_string(c.fp) + " cut=" + std::to_string(c.cut) +
+                               " excess_t=" + std::to_string(c.ex_t));
             return p;
         }
         st_ = S_DONE;
@@ -481,6 +502,7 @@ Engine::~Engine() {
     graphs_.clear();
     (void)hipFree(d_views_);
     (void)hipFree(d_hash_);
+    (void)hipFree(d_ctrs_);
     if (h_stage_) (void)hipHostFree(h_stage_);
     if (h_ctr_) (void)hipHostFree(h_ctr_);
     if (h_hash_) (void)hipHostFree(h_hash_);
@@ -503,6 +525,8 @@ int Engine::init(int nslots, const Params& P) {
     HIPCHK(hipMalloc(&d_hash_, sizeof(unsigned long long) * nslots));
     HIPCHK(hipHostMalloc(&h_stage_, sizeof(SlotView) * nslots + sizeof(int) * list_cap_));
     HIPCHK(hipHostMalloc((void**)&h_ctr_, sizeof(Ctr) * nslots));
+    HIPCHK(hipMalloc((void**)&d_ctrs_, sizeof(Ctr) * nslots));
+    HIPCHK(hipMemset(d_ctrs_, 0, sizeof(Ctr) * nslots));
     HIPCHK(hipHostMalloc((void**)&h_hash_, sizeof(unsigned long long) * nslots));
     memset(h_ctr_, 0, sizeof(Ctr) * nslots);
     ev_.resize(64);
@@ -565,7 +589,8 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
     v.res = sl.res; v.excess = sl.excess; v.h = sl.h; v.fa = sl.fa; v.fb = sl.fb;
     v.lock = sl.lock; v.aux = sl.aux; v.hl = sl.hl; v.tile_cnt = sl.tile_cnt; v.tile_hot = sl.tile_hot;
     v.lbkt = sl.lbkt;
-    v.ctr = sl.ctr;
+    v.giant = sl.giant; v.gacc = sl.gacc; v.gcap = sl.gcap;
+    v.ctr = d_ctrs_ + i;
     v.s = q.s; v.t = q.t;
     v.H_cut = P_.h_cut;
     v.Kh = std::max(1, P_.K_heavy);
@@ -952,11 +977,9 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     }
     for (auto& ag : act_groups) launch_reset(st, d_views_, L(ag.second), ag.second.cnt, ag.first);
     if (g_act.cnt) { const int k = tb(WF_ACTIVE); launch_active(st, d_views_, L(g_act), g_act.cnt, maxof(g_act, gridN)); te(k); }
-    if (g_cert.cnt) { const int k = tb(WF_CERT); launch_cert(st, d_views_, L(g_cert), g_cert.cnt, maxof(g_cert, gridN)); te(k); }
+    if (g_cert.cnt) { const int k = tb(WF_CERT); launch_cert(st, d_views_, L(g_cert), g_cert.cnt, maxof(g_cert, gridN), P_.cert); te(k); }
     HIPCHK(hipGetLastError());
-    for (int i = 0; i < nslots_; ++i)
-        if (slots_[i].live)
-            HIPCHK(hipMemcpyAsync(&h_ctr_[i], slots_[i].view.ctr, sizeof(Ctr), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(h_ctr_, d_ctrs_, sizeof(Ctr) * nslots_, hipMemcpyDeviceToHost, st));
     if (g_mask.cnt)
         HIPCHK(hipMemcpyAsync(h_hash_, d_hash_, sizeof(unsigned long long) * nslots_, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -1051,7 +1074,7 @@ int Engine::update_caps(int gid, const int32_t* entries, const int64_t* new_cap,
     HIPCHK(hipMalloc(&d_old, sizeof(int64_t) * n));
     HIPCHK(hipMemcpyAsync(d_e, entries, sizeof(int32_t) * n, hipMemcpyHostToDevice, stream_));
     HIPCHK(hipMemcpyAsync(d_new, new_cap, sizeof(int64_t) * n, hipMemcpyHostToDevice, stream_));
-    launch_cap_scatter(stream_, G.d_cap, d_e, d_new, d_old, n);
+    launch_cap_scatter(stream_, G.g, G.d_cap, d_e, d_new, d_old, n);
     // adjust every pinned residual on this graph
     SlotView* hv = (SlotView*)h_stage_;
     int* hl = (int*)((char*)h_stage_ + sizeof(SlotView) * nslots_);

@@ -39,7 +39,7 @@ struct Params {
     double rm_tau = 1e-5;
     int tile_rounds = 1;       // local cycles per region pass (LDS-staged)
     double tile_frac = 1.0 / 64.0;  // region mode when active >= tile_frac * n
-    int cert = 1;              // certify every answer on the GPU
+    int cert = 1;              // certify every answer on the GPU: 1 pair sums by fingerprint, 2 entry by entry, 0 off
     long long iter_max = 20000000;  // BC:1476, but reported as an error
     int small_levels = 1 << 30;
     double bu_alpha = 14.0, bu_beta = 24.0;  // direction-optimizing BFS thresholds
@@ -80,6 +80,8 @@ struct DeviceGraph {
     bool classes = false;
     std::vector<int32_t> scc, wcc;
     int32_t* d_wcc = nullptr;
+    int64_t* d_capsum = nullptr;   // certificate constants (DGraph::capsum, cert_fp)
+    uint64_t* d_cert_fp = nullptr;
     struct FwdSet {
         std::vector<uint8_t> mask;  // host copy
         uint8_t* d_mask = nullptr;
@@ -101,6 +103,8 @@ void launch_class_mask(hipStream_t st, const uint8_t* fwd, const int32_t* wcc, i
 void make_tiles(const std::vector<int32_t>& rstart, int tile_max, std::vector<int32_t>& tile_start);
 
 int upload_graph(const HostGraph& hg, int tile_max, DeviceGraph& dg);
+// Allocate and compute dg's certificate constants (after its CSR is set).
+int init_cert_consts(DeviceGraph& dg, hipStream_t st);
 
 struct Slab {
     int n_cap = 0, E_cap = 0, tiles_cap = 0;
@@ -115,7 +119,9 @@ struct Slab {
     int32_t* tile_cnt = nullptr;
     int32_t* tile_hot = nullptr;
     int32_t* lbkt = nullptr;
-    Ctr* ctr = nullptr;
+    int32_t* giant = nullptr;
+    int64_t* gacc = nullptr;
+    int gcap = 0;
     size_t bytes = 0;
     ~Slab();
     static size_t bytes_for(int n, int64_t E, int ntiles);
@@ -261,6 +267,7 @@ public:
     size_t queued() const { return queue_.size(); }
     int live() const;
     const Params& params() const { return P_; }
+    void set_cert_mode(int m) { P_.cert = m; }
     Params& params_mut() { return P_; }
     double now() const;
     hipStream_t stream() const { return stream_; }
@@ -310,6 +317,7 @@ private:
     unsigned long long* d_hash_ = nullptr;
     void* h_stage_ = nullptr;  // pinned: views + lists
     Ctr* h_ctr_ = nullptr;     // pinned: per-slot counter snapshots
+    Ctr* d_ctrs_ = nullptr;    // per-slot device counters, contiguous: one copy per step
     unsigned long long* h_hash_ = nullptr;
     size_t list_cap_ = 0;
     double t0_ = 0;

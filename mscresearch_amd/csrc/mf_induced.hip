@@ -225,7 +225,7 @@ int build_induced_gpu(const DeviceGraph& P, const int32_t* d_mask, int kept, hip
     out.g.tile_of = out.d_tile_of;
     out.tile_max = P.tile_max;
     out.bytes = sizeof(int32_t) * ((size_t)k + 1) + (size_t)Ek * 16 + sizeof(int32_t) * (ntiles + 1 + 2 * (size_t)k + n) + n;
-    return MF_OK;
+    return init_cert_consts(out, st);
 }
 
 }  // namespace mf

@@ -77,8 +77,9 @@ void launch_hot_select(hipStream_t st, const SlotView* slots, const int* list, i
 // [ctr->hmin - margin, ctr->hmax + margin].
 void launch_rm(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
                int margin);
-// Min-cut certificate over the final residual and heights.
-void launch_cert(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);
+// Min-cut certificate over the final residual and heights.  mode 1: pair
+// sums by fingerprint (coalesced); mode 2: pair sums entry by entry.
+void launch_cert(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx, int mode);
 
 // ---------------------------------------------------------------- pruning
 // aux <- (h != n && aux != n) || v == s || v == t   (after FWD into h, UND into aux)
@@ -93,8 +94,11 @@ void launch_apply_updates(hipStream_t st, const SlotView* slots, const int* list
                           const int32_t* entries, const int64_t* old_cap, const int64_t* new_cap,
                           int nupd);
 
-// cap[entries[i]] <- new_cap[i], old value to old_cap[i].
-void launch_cap_scatter(hipStream_t st, int64_t* cap, const int32_t* entries, const int64_t* new_cap,
-                        int64_t* old_cap, int n);
+// cap[entries[i]] <- new_cap[i], old value to old_cap[i]; g.capsum and
+// *g.cert_fp follow.  Entries must not repeat.
+void launch_cap_scatter(hipStream_t st, const DGraph& g, int64_t* cap, const int32_t* entries,
+                        const int64_t* new_cap, int64_t* old_cap, int n);
+// g.capsum (n) and *fp_out (+=, zero it first) from g's capacities and seed.
+void launch_cert_consts(hipStream_t st, const DGraph& g, int64_t* capsum, uint64_t* fp_out);
 
 }  // namespace mf

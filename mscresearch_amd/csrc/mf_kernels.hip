@@ -111,6 +111,9 @@ struct GGraph {
     const GAS int32_t* rev;
     const GAS int32_t* tile_start;
     const GAS int32_t* tile_of;
+    GAS int64_t* capsum;
+    GAS uint64_t* cert_fp;
+    uint64_t cert_seed;
 };
 
 struct GSlot {
@@ -126,6 +129,9 @@ struct GSlot {
     GAS int32_t* tile_cnt;
     GAS int32_t* tile_hot;
     GAS int32_t* lbkt;
+    GAS int32_t* giant;
+    GAS int64_t* gacc;
+    int32_t gcap;
     const GAS int32_t* mask;
     GAS Ctr* ctr;
     int32_t s, t, K, Kh, H_cut, phase;
@@ -149,6 +155,9 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.g.rev = as_global(v.g.rev);
     S.g.tile_start = as_global(v.g.tile_start);
     S.g.tile_of = as_global(v.g.tile_of);
+    S.g.capsum = as_global(v.g.capsum);
+    S.g.cert_fp = as_global(v.g.cert_fp);
+    S.g.cert_seed = v.g.cert_seed;
     S.res = as_global(v.res);
     S.excess = as_global(v.excess);
     S.h = as_global(v.h);
@@ -160,6 +169,9 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.tile_cnt = as_global(v.tile_cnt);
     S.tile_hot = as_global(v.tile_hot);
     S.lbkt = as_global(v.lbkt);
+    S.giant = as_global(v.giant);
+    S.gacc = as_global(v.gacc);
+    S.gcap = v.gcap;
     S.mask = as_global(v.mask);
     S.ctr = as_global(v.ctr);
     S.s = v.s;
@@ -254,7 +266,7 @@ __global__ void k_reset(const SlotView* __restrict__ slots, const int* __restric
             c->pos_excess = 0; c->neg_count = 0; c->neg_sum = 0;
             c->hmin = INT_MAX; c->hmax = -1;
         }
-        if (what & RST_CERT) { c->cert_bad = 0; c->cut = 0; c->nhl = 0; }
+        if (what & RST_CERT) { c->cert_bad = 0; c->cut = 0; c->fp = 0; c->nhl = 0; c->ngiant = 0; }
         if (what & RST_RM) c->fixed_edges = 0;
         if (what & RST_WORK)
             for (int f = 0; f < kFamilies; ++f)
@@ -1138,8 +1150,8 @@ __global__ __launch_bounds__(256) void k_active(const SlotView* __restrict__ slo
             S.fa[n - 1 - k] = v;
             if (deg > kGiantDeg) {  // also to the giant list: row minimum slot starts empty
                 const int g = add32(&S.ctr->ngiant, 1);
-                S.hl[g] = v;
-                S.hl[n - 1 - g] = INT_MAX;
+                S.giant[g] = v;
+                S.giant[S.gcap + g] = INT_MAX;
             }
         }
     }
@@ -1371,17 +1383,80 @@ __global__ __launch_bounds__(256) void k_discharge_list(const SlotView* __restri
     work_flush(S, wk, red, WF_DISCHARGE);
 }
 
-// Giant rows (longer than kGiantDeg) of the active list, spread over the
-// whole grid in chunks of kGiantChunk entries: k_giant_min takes the lowest
-// height over residual entries of every chunk into hl[n-1-g]; k_giant_push
-// relabels (lowest + 1, or n) and has every chunk claim excess from the row's
-// vertex with a compare-and-swap that never overdraws, then push the claimed
-// amount to the chunk's entries at the lowest height in prefix-sum order.
-// Only the row's chunks take excess or residual out of the row, so the
-// lower-bound reads of the single-owner scheme still hold.  Chunks are dealt
-// round-robin across workgroups, continuing from the previous row's last.
-constexpr int kGiantChunk = 2048;
+// Giant rows (longer than kGiantDeg), spread over the whole grid in tasks of
+// kGiantChunk entries.  k_giant_prep turns the slot's giant list into a
+// prefix of task counts (one workgroup per slot); a workgroup maps a task to
+// (row, chunk) by binary search.
+__global__ __launch_bounds__(1024) void k_giant_prep(const SlotView* __restrict__ slots,
+                                                     const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ int wsum[1024 / WAVE];
+    __shared__ int carry;
+    const int ng = S.ctr->ngiant;
+    GAS int32_t* pre = S.giant + 2 * S.gcap;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < ng; base += blockDim.x) {
+        const int g = base + (int)threadIdx.x;
+        int c = 0;
+        if (g < ng) {
+            const int v = S.giant[g];
+            c = (S.g.row_ptr[v + 1] - S.g.row_ptr[v] + kGiantChunk - 1) / kGiantChunk;
+            S.gacc[g] = 0;
+        }
+        // block exclusive scan of c
+        int x = c;
+        const int lane = lane_id();
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const int y = __shfl_up(x, o, WAVE);
+            if (lane >= o) x += y;
+        }
+        const int w = threadIdx.x / WAVE;
+        if (lane == WAVE - 1) wsum[w] = x;
+        __syncthreads();
+        int before = carry;
+        int all = 0;
+        for (int j = 0; j < (int)(blockDim.x / WAVE); ++j) {
+            if (j < w) before += wsum[j];
+            all += wsum[j];
+        }
+        if (g < ng) pre[g] = before + x - c;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += all;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        pre[ng] = carry;
+        S.ctr->ngchunks = carry;
+    }
+}
 
+// Task -> (row k, chunk c) of the slot's giant list, found by thread 0 and
+// shared through LDS.
+__device__ __forceinline__ void giant_task(const GSlot& S, int task, int ng, int* sh) {
+    __syncthreads();  // the previous task's reads of sh are done
+    if (threadIdx.x == 0) {
+        const GAS int32_t* pre = S.giant + 2 * S.gcap;
+        int lo = 0, hi = ng - 1;  // last k with pre[k] <= task
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= task) lo = mid; else hi = mid - 1;
+        }
+        sh[0] = lo;
+        sh[1] = task - pre[lo];
+        sh[2] = S.giant[lo];
+    }
+    __syncthreads();
+}
+
+// k_giant_min: lowest height over the residual entries of each chunk into
+// the row's minimum slot (INT_MAX - 1 when the chunk has none).  k_giant_push
+// relabels the row (lowest + 1, or n) from its first chunk and has every
+// chunk claim excess from the row's vertex with a compare-and-swap that never
+// overdraws, then push the claimed amount to the chunk's entries at the
+// lowest height in prefix-sum order.  Only the row's chunks take excess or
+// residual out of the row, so the lower-bound reads of the single-owner
+// scheme still hold.
 template <typename P>
 __device__ __forceinline__ bool cas64(P p, int64_t& expect, int64_t want) {
     return __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
@@ -1393,32 +1468,30 @@ __global__ __launch_bounds__(256) void k_giant_min(const SlotView* __restrict__ 
     const GSlot S = load_slot(slots, list);
     __shared__ int iscr[256 / WAVE];
     __shared__ uint64_t red[256 / WAVE];
+    __shared__ int sh[4];
     const int ng = S.ctr->ngiant;
-    if (ng == 0) return;
+    const int T = S.ctr->ngchunks;
+    if ((int)blockIdx.x >= T) return;  // uniform per block
     const int n = S.g.n;
-    const int G = (int)gridDim.x;
     Work wk;
-    int off = 0;
-    for (int k = 0; k < ng; ++k) {
-        const int v = S.hl[k];
+    for (int task = blockIdx.x; task < T; task += gridDim.x) {
+        giant_task(S, task, ng, sh);
+        const int k = sh[0], c = sh[1], v = sh[2];
+        if (threadIdx.x == 0) sh[3] = (ld64(&S.excess[v]) > 0 && S.h[v] < n) ? 1 : 0;
+        __syncthreads();
+        const bool live = sh[3] != 0;
+        __syncthreads();
+        if (!live) continue;
         const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
-        const int nch = (re - rb + kGiantChunk - 1) / kGiantChunk;
-        const int c0 = ((int)blockIdx.x - off % G + G) % G;
-        off += nch;
-        // nothing pushes during this kernel, so every workgroup sees the same excess
-        if (ld64(&S.excess[v]) <= 0 || S.h[v] >= n) continue;
-        for (int c = c0; c < nch; c += G) {
-            const int e0 = rb + c * kGiantChunk;
-            const int e1 = min(re, e0 + kGiantChunk);
-            int m = INT_MAX;
-            for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x)
-                if (ld64(&S.res[e]) > 0) m = min(m, S.h[S.g.col[e]]);
-            m = block_min_i32(m, iscr);
-            if (threadIdx.x == 0) {
-                // INT_MAX - 1: the row was scanned and has no residual entry here
-                min32(&S.hl[n - 1 - k], m == INT_MAX ? INT_MAX - 1 : m);
-                wk.edges += (uint64_t)(e1 - e0);
-            }
+        const int e0 = rb + c * kGiantChunk;
+        const int e1 = min(re, e0 + kGiantChunk);
+        int m = INT_MAX;
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x)
+            if (ld64(&S.res[e]) > 0) m = min(m, S.h[S.g.col[e]]);
+        m = block_min_i32(m, iscr);
+        if (threadIdx.x == 0) {
+            min32(&S.giant[S.gcap + k], m == INT_MAX ? INT_MAX - 1 : m);
+            wk.edges += (uint64_t)(e1 - e0);
         }
     }
     work_flush(S, wk, red, WF_DISCHARGE);
@@ -1430,81 +1503,78 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
     __shared__ int64_t lscr[256 / WAVE];
     __shared__ int64_t bcast;
     __shared__ uint64_t red[256 / WAVE];
+    __shared__ int sh[4];
     constexpr int PER = kGiantChunk / 256;
     const int ng = S.ctr->ngiant;
-    if (ng == 0) return;
+    const int T = S.ctr->ngchunks;
+    if ((int)blockIdx.x >= T) return;  // uniform per block
     const int n = S.g.n;
-    const int G = (int)gridDim.x;
     Work wk;
-    int off = 0;
-    for (int k = 0; k < ng; ++k) {
-        const int v = S.hl[k];
-        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
-        const int nch = (re - rb + kGiantChunk - 1) / kGiantChunk;
-        const int c0 = ((int)blockIdx.x - off % G + G) % G;
-        off += nch;
-        const int hmin = S.hl[n - 1 - k];
+    for (int task = blockIdx.x; task < T; task += gridDim.x) {
+        giant_task(S, task, ng, sh);
+        const int k = sh[0], c = sh[1], v = sh[2];
+        const int hmin = S.giant[S.gcap + k];
         if (hmin == INT_MAX) continue;  // k_giant_min skipped the row: inactive
         if (hmin >= n - 1) {            // no residual entry leads below n - 1: v cannot reach t
-            if (c0 == 0 && threadIdx.x == 0 && S.h[v] < n) { S.h[v] = n; wk.relabels++; }
+            if (c == 0 && threadIdx.x == 0 && S.h[v] < n) { S.h[v] = n; wk.relabels++; }
             continue;
         }
-        if (c0 == 0 && threadIdx.x == 0 && S.h[v] <= hmin) { S.h[v] = hmin + 1; wk.relabels++; }
-        for (int c = c0; c < nch; c += G) {
-            // stop once the row's vertex has no excess left to claim
-            if (threadIdx.x == 0) bcast = ld64(&S.excess[v]);
-            __syncthreads();
-            const bool dry = bcast <= 0;
-            __syncthreads();
-            if (dry) break;
-            const int e0 = rb + c * kGiantChunk;
-            const int e1 = min(re, e0 + kGiantChunk);
-            int64_t r[PER];
-            int64_t tsum = 0;
+        if (c == 0 && threadIdx.x == 0 && S.h[v] <= hmin) { S.h[v] = hmin + 1; wk.relabels++; }
+        // nothing to do once the row's vertex has no excess left to claim
+        if (threadIdx.x == 0) bcast = ld64(&S.excess[v]);
+        __syncthreads();
+        const bool dry = bcast <= 0;
+        __syncthreads();
+        if (dry) continue;
+        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+        const int e0 = rb + c * kGiantChunk;
+        const int e1 = min(re, e0 + kGiantChunk);
+        int64_t r[PER];
+        int64_t tsum = 0;
 #pragma unroll
-            for (int j = 0; j < PER; ++j) {
-                const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
-                r[j] = 0;
-                if (e < e1) {
-                    const int64_t x = ld64(&S.res[e]);
-                    if (x > 0 && S.h[S.g.col[e]] == hmin) r[j] = x;
-                }
-                tsum += r[j];
+        for (int j = 0; j < PER; ++j) {
+            const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
+            r[j] = 0;
+            if (e < e1) {
+                const int64_t x = ld64(&S.res[e]);
+                if (x > 0 && S.h[S.g.col[e]] == hmin) r[j] = x;
             }
-            int64_t tot = 0;
-            const int64_t incl = block_incl_scan_i64(tsum, lscr, &tot);
-            if (threadIdx.x == 0) wk.edges += (uint64_t)(e1 - e0);
-            if (tot == 0) continue;
-            if (threadIdx.x == 0) {
-                int64_t cur = ld64(&S.excess[v]);
-                int64_t a = 0;
-                while (cur > 0) {
-                    a = cur < tot ? cur : tot;
-                    if (cas64(&S.excess[v], cur, cur - a)) break;
-                }
-                bcast = cur > 0 ? a : 0;
+            tsum += r[j];
+        }
+        int64_t tot = 0;
+        const int64_t incl = block_incl_scan_i64(tsum, lscr, &tot);
+        if (threadIdx.x == 0) wk.edges += (uint64_t)(e1 - e0);
+        if (tot == 0) continue;
+        if (threadIdx.x == 0) {
+            int64_t cur = ld64(&S.excess[v]);
+            int64_t a = 0;
+            while (cur > 0) {
+                a = cur < tot ? cur : tot;
+                if (cas64(&S.excess[v], cur, cur - a)) break;
             }
-            __syncthreads();
-            int64_t rem = bcast - (incl - tsum);
-            __syncthreads();
+            bcast = cur > 0 ? a : 0;
+        }
+        __syncthreads();
+        int64_t rem = bcast - (incl - tsum);
+        __syncthreads();
 #pragma unroll
-            for (int j = 0; j < PER; ++j) {
-                if (rem <= 0) break;
-                if (r[j] <= 0) continue;
-                const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
-                const int64_t d = rem < r[j] ? rem : r[j];
-                add64(&S.res[e], -d);
-                add64(&S.res[S.g.rev[e]], d);
-                add64(&S.excess[S.g.col[e]], d);
-                rem -= d;
-                wk.pushes++;
-            }
+        for (int j = 0; j < PER; ++j) {
+            if (rem <= 0) break;
+            if (r[j] <= 0) continue;
+            const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
+            const int64_t d = rem < r[j] ? rem : r[j];
+            add64(&S.res[e], -d);
+            add64(&S.res[S.g.rev[e]], d);
+            add64(&S.excess[S.g.col[e]], d);
+            rem -= d;
+            wk.pushes++;
         }
     }
     work_flush(S, wk, red, WF_DISCHARGE);
 }
 
 void launch_giant(hipStream_t st, const SlotView* slots, const int* list, int nlist) {
+    hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
     hipLaunchKernelGGL(k_giant_min, dim3(512, nlist), dim3(256), 0, st, slots, list);
     hipLaunchKernelGGL(k_giant_push, dim3(512, nlist), dim3(256), 0, st, slots, list);
 }
@@ -1806,18 +1876,42 @@ void launch_rm(hipStream_t st, const SlotView* slots, const int* list, int nlist
 // from outside T into T.
 
 // Row terms of the certificate.  Every entry pair has exactly one entry with
-// capacity (forward) and one without (reverse), so pair sums are checked once,
-// from the forward side.  Rows of T also check closedness and sum the cut
-// from T's side: entry e = w -> u with u outside T closes pair u -> w.
+// capacity (forward) and one without (reverse).  Pair sums, two ways:
+//  FULL: from the forward side, res[e] + res[rev e] == cap[e] (a gather);
+//  FP  : every entry adds w(pair) * (res[e] - cap[e]) to a 64-bit sum, w an
+//        odd hash of the pair's lower entry id and the graph's seed; pairs
+//        that sum to their capacity add 0, a single wrong pair can never
+//        give 0 (odd w, |defect| < 2^63), several cancel with chance about
+//        2^-64.  The cap terms are a graph constant (g.cert_fp, subtracted
+//        once by k_cert_fin) and so are the row sums of cap (g.capsum), so
+//        an entry costs res + rev, both coalesced.
+// Rows of T also check closedness and sum the cut from T's side: entry
+// e = w -> u with u outside T closes pair u -> w.
+enum CertMode : int { CERT_FP = 1, CERT_FULL = 2 };
+
+__device__ __forceinline__ uint64_t pair_weight(uint64_t pair, uint64_t seed) {
+    uint64_t x = pair + seed + 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return (x ^ (x >> 31)) | 1ULL;
+}
+
+template <int MODE>
 __device__ __forceinline__ void cert_entry(const GSlot& S, int e, bool inT, int n, int64_t& ex,
-                                           uint64_t& bad, int64_t& cut) {
+                                           uint64_t& bad, int64_t& cut, uint64_t& fp) {
     const int64_t r = S.res[e];
-    const int64_t c = S.g.cap[e];
     if (r < 0) ++bad;
-    ex += r - c;
-    if (c > 0) {  // forward entry; its reverse entry has capacity 0 by construction
+    if (MODE == CERT_FULL) {
+        const int64_t c = S.g.cap[e];
+        ex += r - c;
+        if (c > 0) {  // forward entry; its reverse entry has capacity 0 by construction
+            const int rv = S.g.rev[e];
+            if (r + S.res[rv] != c) ++bad;
+        }
+    } else {
+        ex += r;  // k_cert_fin / cert_vertex subtract the row's capacity sum
         const int rv = S.g.rev[e];
-        if (r + S.res[rv] != c) ++bad;
+        fp += pair_weight((uint64_t)min(e, rv), S.g.cert_seed) * (uint64_t)r;
     }
     if (inT) {
         if (S.h[S.g.col[e]] >= n) {  // col[e] outside T: its pair entry leaves V\T into T
@@ -1828,7 +1922,9 @@ __device__ __forceinline__ void cert_entry(const GSlot& S, int e, bool inT, int 
     }
 }
 
+template <int MODE>
 __device__ __forceinline__ void cert_vertex(const GSlot& S, int v, int64_t ex, bool inT, uint64_t& bad) {
+    if (MODE == CERT_FP) ex -= S.g.capsum[v];
     if (ex != S.excess[v]) ++bad;
     if (v != S.s && ex < 0) ++bad;
     if (inT && v != S.t && ex != 0) ++bad;
@@ -1838,22 +1934,24 @@ __device__ __forceinline__ void cert_vertex(const GSlot& S, int v, int64_t ex, b
 }
 
 // Light rows: one lane each; medium rows: one wave each; giant rows go to
-// the slot's heavy list, which k_cert_heavy hands out grid-stride, one row
-// per workgroup with a block reduction of the row's excess (hub rows sit
-// together at region starts, so they are spread over the whole grid instead
-// of the block that met them).
+// the slot's giant list, whose rows k_cert_chunks cuts into kGiantChunk-entry
+// tasks over the whole grid (hub rows sit together at region starts, so they
+// are spread over the grid instead of the block that met them).
 __device__ __forceinline__ void cert_flush(const GSlot& S, int64_t cut, uint64_t bad, uint64_t edges,
-                                           int64_t* red64, uint64_t* redu) {
+                                           uint64_t fp, int64_t* red64, uint64_t* redu) {
     const int64_t bc = block_sum<int64_t>(cut, red64);
     const uint64_t bb = block_sum<uint64_t>(bad, redu);
     const uint64_t be = block_sum<uint64_t>(edges, redu);
+    const uint64_t bf = block_sum<uint64_t>(fp, redu);  // wraps mod 2^64, order-free
     if (threadIdx.x == 0) {
+        if (bf) addu64(&S.ctr->fp, bf);
         if (bc) add64(&S.ctr->cut, bc);
         if (bb) add32(&S.ctr->cert_bad, (int32_t)bb);
         if (be) addu64(&S.ctr->wk[WF_CERT][0], be);
     }
 }
 
+template <int MODE>
 __global__ __launch_bounds__(256) void k_cert(const SlotView* __restrict__ slots,
                                               const int* __restrict__ list) {
     const GSlot S = load_slot(slots, list);
@@ -1863,7 +1961,7 @@ __global__ __launch_bounds__(256) void k_cert(const SlotView* __restrict__ slots
     __shared__ int nmed;
     const int n = S.g.n;
     int64_t cut = 0;
-    uint64_t bad = 0, edges = 0;
+    uint64_t bad = 0, edges = 0, fp = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
         if (threadIdx.x == 0) nmed = 0;
@@ -1875,13 +1973,13 @@ __global__ __launch_bounds__(256) void k_cert(const SlotView* __restrict__ slots
             if (re - rb <= kLightDeg) {  // one lane
                 const bool inT = S.h[v] < n;
                 int64_t ex = 0;
-                for (int e = rb; e < re; ++e) cert_entry(S, e, inT, n, ex, bad, cut);
+                for (int e = rb; e < re; ++e) cert_entry<MODE>(S, e, inT, n, ex, bad, cut, fp);
                 edges += (uint64_t)(re - rb);
-                cert_vertex(S, v, ex, inT, bad);
+                cert_vertex<MODE>(S, v, ex, inT, bad);
             } else if (re - rb <= kGiantDeg) {  // one wave, below
                 med[atomicAdd(&nmed, 1)] = v;
-            } else {  // k_cert_heavy
-                S.hl[add32(&S.ctr->nhl, 1)] = v;
+            } else {  // giant list: k_cert_chunks, k_cert_fin
+                S.giant[add32(&S.ctr->ngiant, 1)] = v;
             }
         }
         __syncthreads();
@@ -1891,46 +1989,78 @@ __global__ __launch_bounds__(256) void k_cert(const SlotView* __restrict__ slots
             const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
             const bool inT = S.h[v] < n;
             int64_t ex = 0;
-            for (int e = rb + lane_id(); e < re; e += WAVE) cert_entry(S, e, inT, n, ex, bad, cut);
+            for (int e = rb + lane_id(); e < re; e += WAVE) cert_entry<MODE>(S, e, inT, n, ex, bad, cut, fp);
             ex = wave_sum_i64(ex);
             if (lane_id() == 0) {
                 edges += (uint64_t)(re - rb);
-                cert_vertex(S, v, ex, inT, bad);
+                cert_vertex<MODE>(S, v, ex, inT, bad);
             }
         }
         __syncthreads();
     }
-    cert_flush(S, cut, bad, edges, red64, redu);
+    cert_flush(S, cut, bad, edges, fp, red64, redu);
 }
 
-__global__ __launch_bounds__(256) void k_cert_heavy(const SlotView* __restrict__ slots,
-                                                    const int* __restrict__ list) {
+// Giant rows of the certificate in kGiantChunk-entry tasks over the grid:
+// each chunk adds its part of the row's recomputed excess into gacc[row];
+// k_cert_fin then checks each giant vertex.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_cert_chunks(const SlotView* __restrict__ slots,
+                                                     const int* __restrict__ list) {
     const GSlot S = load_slot(slots, list);
     __shared__ int64_t red64[256 / WAVE];
     __shared__ uint64_t redu[256 / WAVE];
-    const int nh = S.ctr->nhl;
-    if ((int)blockIdx.x >= nh) return;  // uniform per block
+    __shared__ int sh[4];
+    const int ng = S.ctr->ngiant;
+    const int T = S.ctr->ngchunks;
+    if ((int)blockIdx.x >= T) return;  // uniform per block
     const int n = S.g.n;
     int64_t cut = 0;
-    uint64_t bad = 0, edges = 0;
-    for (int k = blockIdx.x; k < nh; k += gridDim.x) {
-        const int v = S.hl[k];
+    uint64_t bad = 0, edges = 0, fp = 0;
+    for (int task = blockIdx.x; task < T; task += gridDim.x) {
+        giant_task(S, task, ng, sh);
+        const int k = sh[0], c = sh[1], v = sh[2];
         const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+        const int e0 = rb + c * kGiantChunk;
+        const int e1 = min(re, e0 + kGiantChunk);
         const bool inT = S.h[v] < n;
         int64_t ex = 0;
-        for (int e = rb + (int)threadIdx.x; e < re; e += blockDim.x) cert_entry(S, e, inT, n, ex, bad, cut);
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) cert_entry<MODE>(S, e, inT, n, ex, bad, cut, fp);
         const int64_t tot = block_sum<int64_t>(ex, red64);
         if (threadIdx.x == 0) {
-            edges += (uint64_t)(re - rb);
-            cert_vertex(S, v, tot, inT, bad);
+            if (tot) add64(&S.gacc[k], tot);
+            edges += (uint64_t)(e1 - e0);
         }
     }
-    cert_flush(S, cut, bad, edges, red64, redu);
+    cert_flush(S, cut, bad, edges, fp, red64, redu);
 }
 
-void launch_cert(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx) {
-    hipLaunchKernelGGL(k_cert, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
-    hipLaunchKernelGGL(k_cert_heavy, dim3(2048, nlist), dim3(256), 0, st, slots, list);
+template <int MODE>
+__global__ __launch_bounds__(256) void k_cert_fin(const SlotView* __restrict__ slots,
+                                                  const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ int64_t red64[256 / WAVE];
+    __shared__ uint64_t redu[256 / WAVE];
+    const int ng = S.ctr->ngiant;
+    const int n = S.g.n;
+    uint64_t bad = 0;
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
+        const int v = S.giant[g];
+        cert_vertex<MODE>(S, v, ld64(&S.gacc[g]), S.h[v] < n, bad);
+    }
+    // every k_cert / k_cert_chunks block has added its part by now (stream order)
+    if (MODE == CERT_FP && blockIdx.x == 0 && threadIdx.x == 0) addu64(&S.ctr->fp, 0ULL - *S.g.cert_fp);
+    cert_flush(S, 0, bad, 0, 0, red64, redu);
+}
+
+void launch_cert(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx, int mode) {
+    if (mode == CERT_FULL) hipLaunchKernelGGL(k_cert<CERT_FULL>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    else hipLaunchKernelGGL(k_cert<CERT_FP>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
+    if (mode == CERT_FULL) hipLaunchKernelGGL(k_cert_chunks<CERT_FULL>, dim3(1024, nlist), dim3(256), 0, st, slots, list);
+    else hipLaunchKernelGGL(k_cert_chunks<CERT_FP>, dim3(1024, nlist), dim3(256), 0, st, slots, list);
+    if (mode == CERT_FULL) hipLaunchKernelGGL(k_cert_fin<CERT_FULL>, dim3(4, nlist), dim3(256), 0, st, slots, list);
+    else hipLaunchKernelGGL(k_cert_fin<CERT_FP>, dim3(4, nlist), dim3(256), 0, st, slots, list);
 }
 
 // ------------------------------------------------------------- prune mask (a6)
@@ -2018,19 +2148,48 @@ void launch_apply_updates(hipStream_t st, const SlotView* slots, const int* list
                        list, entries, old_cap, new_cap, nupd);
 }
 
-__global__ void k_cap_scatter(int64_t* __restrict__ cap, const int32_t* __restrict__ entries,
+__global__ void k_cap_scatter(DGraph g, int64_t* __restrict__ cap, const int32_t* __restrict__ entries,
                               const int64_t* __restrict__ new_cap, int64_t* __restrict__ old_cap, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int e = entries[i];
-        old_cap[i] = cap[e];
+        const int64_t was = cap[e];
+        old_cap[i] = was;
         cap[e] = new_cap[i];
+        const int64_t d = new_cap[i] - was;
+        const int rv = g.rev[e];
+        add64(&g.capsum[g.col[rv]], d);  // col[rev e] owns e
+        addu64(g.cert_fp, pair_weight((uint64_t)min(e, rv), g.cert_seed) * (uint64_t)d);
     }
 }
 
-void launch_cap_scatter(hipStream_t st, int64_t* cap, const int32_t* entries, const int64_t* new_cap,
-                        int64_t* old_cap, int n) {
+void launch_cap_scatter(hipStream_t st, const DGraph& g, int64_t* cap, const int32_t* entries,
+                        const int64_t* new_cap, int64_t* old_cap, int n) {
     const int gx = std::max(1, std::min(4096, (n + 255) / 256));
-    hipLaunchKernelGGL(k_cap_scatter, dim3(gx), dim3(256), 0, st, cap, entries, new_cap, old_cap, n);
+    hipLaunchKernelGGL(k_cap_scatter, dim3(gx), dim3(256), 0, st, g, cap, entries, new_cap, old_cap, n);
+}
+
+// Certificate constants of a graph: capsum[v] and cert_fp (see DGraph).
+__global__ __launch_bounds__(256) void k_cert_consts(DGraph g, int64_t* __restrict__ capsum,
+                                                     uint64_t* __restrict__ fp_out) {
+    __shared__ uint64_t red[256 / WAVE];
+    uint64_t fp = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int v = (int)i;
+        int64_t cs = 0;
+        for (int e = g.row_ptr[v]; e < g.row_ptr[v + 1]; ++e) {
+            const int64_t c = g.cap[e];
+            cs += c;
+            fp += pair_weight((uint64_t)min(e, g.rev[e]), g.cert_seed) * (uint64_t)c;
+        }
+        capsum[v] = cs;
+    }
+    const uint64_t b = block_sum<uint64_t>(fp, red);
+    if (threadIdx.x == 0 && b) addu64(fp_out, b);
+}
+
+void launch_cert_consts(hipStream_t st, const DGraph& g, int64_t* capsum, uint64_t* fp_out) {
+    const int gx = std::max(1, std::min(8192, (g.n + 255) / 256));
+    hipLaunchKernelGGL(k_cert_consts, dim3(gx), dim3(256), 0, st, g, capsum, fp_out);
 }
 
 }  // namespace mf

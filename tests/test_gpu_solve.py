@@ -119,6 +119,7 @@ def test_edge_cases():
     dict(local_relabel=1),
     dict(local_relabel=1, local_band=1 << 30, gr_stall=0, gr_work=0.0),  # local relabel between all discharges
     dict(local_relabel=1, h_cut=0, rm_freq0=1, rm_freq_max=1),       # ... and RM on every iteration
+    dict(cert=2),                               # certificate pair sums entry by entry
 ])
 def test_scheduler_variants_stay_exact(c1, variant):
     log(f"variant {variant}")
@@ -187,3 +188,50 @@ def test_giant_rows():
             r = eng.solve(gid, sa, tb)
             assert r.flow == O.maxflow(og, sa, tb), (variant, a, b)
             assert r.cert_ok == 1
+
+
+def test_gpu_certificate_catches_planted_mistakes(c1):
+    """The GPU certificate (fingerprint and entry-by-entry pair sums) accepts
+    a correct final state and rejects planted mistakes, like the oracle's."""
+    eng, gid = engine_with(c1)
+    qs, qt = M.uniform_queries(c1.n, 4, 1007)
+    og = c1.og
+    n = c1.n
+    owner = np.repeat(np.arange(n), np.diff(og.row_ptr))
+    rng = np.random.default_rng(5)
+    for a, b in zip(qs, qt):
+        s, t = c1.q(a, b)
+        r, res, ex, h = eng.dbg_solve_state(gid, s, t, c1.n, c1.E)
+        assert O.certify(og, res, s, t, r.flow, excess=ex) == 0
+        for mode in (1, 2):
+            bad, cut, ext, fz = eng.dbg_certify(gid, s, t, res, ex, h, mode)
+            assert (bad, fz, cut, ext) == (0, True, r.flow, r.flow), mode
+        # (a) two pair sums off by +1 / -1 inside one row: the recomputed
+        # excess is unchanged, only the pair-sum check can see it
+        fwd = np.nonzero((og.cap > 0) & (res > 0) & (owner != s) & (owner != t))[0]
+        rows, cnt = np.unique(owner[fwd], return_counts=True)
+        v = rng.choice(rows[cnt >= 2])
+        e1, e2 = fwd[owner[fwd] == v][:2]
+        bad_res = res.copy()
+        bad_res[e1] += 1
+        bad_res[e2] -= 1
+        assert O.certify(og, bad_res, s, t, r.flow, excess=ex) != 0
+        assert not eng.dbg_certify(gid, s, t, bad_res, ex, h, 1)[3]
+        assert eng.dbg_certify(gid, s, t, bad_res, ex, h, 2)[0] > 0
+        # (b) stored excess off by one at an inner vertex
+        w = int(rng.choice(np.setdiff1d(np.arange(n), [s, t])))
+        bad_ex = ex.copy()
+        bad_ex[w] += 1
+        assert O.certify(og, res, s, t, r.flow, excess=bad_ex) != 0
+        for mode in (1, 2):
+            assert eng.dbg_certify(gid, s, t, res, bad_ex, h, mode)[0] > 0
+        # (c) a negative residual
+        bad_res = res.copy()
+        bad_res[int(rng.choice(np.nonzero(res > 0)[0]))] = -1
+        for mode in (1, 2):
+            assert eng.dbg_certify(gid, s, t, bad_res, ex, h, mode)[0] > 0
+        # (d) heights claiming s reaches t (s inside T)
+        bad_h = h.copy()
+        bad_h[s] = 0
+        for mode in (1, 2):
+            assert eng.dbg_certify(gid, s, t, res, ex, bad_h, mode)[0] > 0
