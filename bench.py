@@ -109,6 +109,75 @@ def cpu_baseline(d, qs, qt, target_s, threads):
     return flows, sec, k, sec0
 
 
+def serve_bench(dg22, hg22, e22, args):
+    """Serving loop: grid 1024x1024, R-MAT 20 and R-MAT 22 resident, one engine
+    lane each (own host thread and HIP stream, slab arena sized in slabs of
+    its graph); Poisson arrivals at each offered load, graph chosen 0.3 /
+    0.4 / 0.3; each lane admits by slab bytes and batches continuously."""
+    t0 = time.time()
+    grid = M.Edges.grid(1024, 1024, 1000, 2001)
+    hgg = M.HostGraph.from_edgelist(grid).partition(512)
+    dgg = M.DeviceGraph(hgg)
+    e20 = M.Edges.rmat(20, 10**6, 4001, threads=min(16, os.cpu_count() or 4))
+    hg20 = M.HostGraph.from_edgelist(e20).partition(512)
+    dg20 = M.DeviceGraph(hg20)
+    log(f"serving graphs ready in {time.time() - t0:.1f}s")
+    lanes = [("grid1024", dgg, hgg, 32, M.default_params(fuse=32)),
+             ("rmat20", dg20, hg20, 32, M.default_params()),
+             ("rmat22", dg22, hg22, 48, M.default_params())]
+    engines, budgets = [], []
+    for name, dg, hg, slots, prm in lanes:
+        eng = M.Engine(slots, prm)
+        eng.add_graph(dg)
+        info = dg.info()
+        slab = 8 * info["E"] + 32 * info["n"]
+        nslab = args.serve_slabs if name == "rmat22" else slots
+        eng.set_budget(int(nslab * slab))
+        engines.append(eng)
+        budgets.append(int(nslab * slab))
+    nid = [hgg.export()["new_id"], hg20.export()["new_id"], hg22.export()["new_id"]]
+    nmax = 4096
+    pools = [M.grid_queries(1024, 1024, nmax, 2002), e20.hub_queries(nmax, 4002), e22.hub_queries(nmax, 4003)]
+    pools = [(nid[k][a].astype(np.int32), nid[k][b].astype(np.int32)) for k, (a, b) in enumerate(pools)]
+    rng = np.random.default_rng(4004)
+    # warm: reachability classes and pruned subgraphs of the first queries
+    wq = [(k, j) for k in range(3) for j in range(4)]
+    M.serve_lanes(engines, np.array([k for k, _ in wq]), np.zeros(len(wq)), np.array([pools[k][0][j] for k, j in wq]),
+                  np.array([pools[k][1][j] for k, j in wq]), np.zeros(len(wq)))
+    levels = []
+    used = [4, 4, 4]
+    for qps in args.serve_qps:
+        nq = int(min(args.serve_max, max(20, qps * args.serve_seconds)))
+        g = rng.choice(3, nq, p=[0.3, 0.4, 0.3])
+        arrive = np.cumsum(rng.exponential(1.0 / qps, nq))
+        s, t = np.empty(nq, np.int32), np.empty(nq, np.int32)
+        for i in range(nq):
+            k = g[i]
+            j = used[k] % nmax
+            used[k] += 1
+            s[i], t[i] = pools[k][0][j], pools[k][1][j]
+        before = [e.mem_stats()["admit_waits"] for e in engines]
+        res = M.serve_lanes(engines, g, np.zeros(nq), s, t, arrive)
+        lat = [r.t_end - r.t_arrive for r in res]
+        span = max(r.t_end for r in res) - min(r.t_arrive for r in res)
+        bad = sum(1 for r in res if r.status != 0 or r.cert_ok != 1)
+        per = {}
+        for k, (nm, *_rest) in enumerate(lanes):
+            lk = [r.t_end - r.t_arrive for r in res if g[r.tag] == k]
+            per[nm] = {"n": len(lk), "p50_ms": round(pct(lk, .5) * 1e3, 3), "p99_ms": round(pct(lk, .99) * 1e3, 3),
+                       "admission_waits": engines[k].mem_stats()["admit_waits"] - before[k]}
+        lv = {"offered_qps": qps, "queries": nq, "sustained_qps": round(nq / span, 2),
+              "p50_ms": round(pct(lat, .5) * 1e3, 3), "p99_ms": round(pct(lat, .99) * 1e3, 3),
+              "failed_or_uncertified": bad, "per_graph": per}
+        log(f"serve {qps} q/s offered: {lv}")
+        levels.append(lv)
+    return {"graphs": ["grid 1024x1024 caps U[1,1000] seed 2001", "R-MAT 20 seed 4001", "R-MAT 22 (the main workload)"],
+            "mix": [0.3, 0.4, 0.3], "lanes": [{"graph": nm, "slots": sl, "slab_budget_bytes": b,
+                                             "slab_peak_bytes": e.mem_stats()["peak"]}
+                                            for (nm, _d, _h, sl, _p), b, e in zip(lanes, budgets, engines)],
+            "levels": levels}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +193,12 @@ def main():
     ap.add_argument("--plain", type=int, default=64, help="queries timed on the plain (one-at-a-time) solver")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-qps", type=float, nargs="+", default=[50.0, 200.0, 800.0])
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-max", type=int, default=1200)
+    ap.add_argument("--serve-slabs", type=float, default=24.0,
+                    help="slab arena budget in full R-MAT 22 slabs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X and this path shards by query "
@@ -197,6 +272,13 @@ def main():
     else:
         plain_qps = None
 
+    del eng  # its slab arena is not needed any more
+    serving = None
+    if not args.no_serve and args.workload == "rmat" and args.scale == 22:
+        serving = serve_bench(dg, hg, e, args)
+        if plain_lat:
+            serving["plain_p50_ms_rmat22"] = round(pct(plain_lat, .5) * 1e3, 3)
+
     # ---- CPU oracle baseline on a bounded sample; bit-exact check of the sample
     cpu = None
     exact = None
@@ -247,6 +329,7 @@ def main():
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5),
                      "traffic": None},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -140,6 +140,7 @@ typedef struct mf_params {
     int K_heavy;         /* discharge cycles per pass for wave-scanned (long) rows */
     int local_relabel;   /* 1 = BFS relabel of active tiles between global relabels */
     int local_band;      /* ... when the active height band is at most this wide */
+    int fuse;            /* discharge + count cycles per host round trip on deep graphs (1 = off) */
 } mf_params;
 
 void mf_params_default(mf_params* p);
@@ -181,6 +182,32 @@ int mf_engine_submit(mf_engine* e, int gid, int s, int t, int64_t tag, double t_
  * max_steps steps; finished results are copied to out (up to cap), *nout set. */
 int mf_engine_run(mf_engine* e, int64_t max_steps, mf_result* out, int cap, int* nout);
 int mf_engine_live(const mf_engine* e, int* live, int64_t* queued);
+/* Slab arena of the engine: per-query device arrays (residual, excess,
+ * heights, lists) are carved from one HBM block of `bytes` (0 = default:
+ * nslots full slabs of the largest graph).  A queued query is admitted when
+ * a slot is free and its run graph's slab fits: the pruned subgraph's when
+ * that is already cached, else the full graph's (the reference sized
+ * DynamicState per query, BC:1336-1372).  Only while no slot is in use. */
+int mf_engine_set_budget(mf_engine* e, int64_t bytes);
+/* Arena size, bytes in use, peak, and steps whose queue head waited for room. */
+int mf_engine_mem_stats(const mf_engine* e, int64_t* budget, int64_t* in_use, int64_t* peak,
+                        int64_t* admit_waits);
+/* Serving loop: query i (graph gid[i], s[i], t[i]) arrives arrive[i] seconds
+ * after the call (ascending).  Arrivals are queued when their time comes,
+ * admitted by slab bytes as slots free up (continuous batching) and run
+ * concurrently; the call returns when every query has finished.  Results in
+ * completion order, tag = i, t_arrive/t_start/t_end on the engine clock.
+ * The reference answered one query at a time (BC:2628-2843). */
+int mf_engine_serve(mf_engine* e, int nq, const int32_t* gid, const int32_t* s, const int32_t* t,
+                    const double* arrive, mf_result* out, int* nout);
+/* Serving over several engines ("lanes"), typically one per resident graph
+ * class: query i goes to engines[lane[i]] (graph gid[i] of that engine).
+ * Each lane runs mf_engine_serve's loop on its own host thread and HIP
+ * stream, so the lanes' kernels overlap and a deep query class (grids) never
+ * holds back a shallow one.  Arrival offsets share one start; results in
+ * completion order, tag = i, times on the clock all engines share. */
+int mf_serve_lanes(int nengines, mf_engine* const* engines, int nq, const int32_t* lane, const int32_t* gid,
+                   const int32_t* s, const int32_t* t, const double* arrive, mf_result* out, int* nout);
 double mf_engine_now(const mf_engine* e);
 int mf_engine_cache_stats(const mf_engine* e, int64_t* hits, int64_t* misses);
 /* Per kernel family (init, bfs, active, discharge, rm, cert) timing with HIP

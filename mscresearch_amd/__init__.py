@@ -42,7 +42,7 @@ class Params(C.Structure):
         ("rm_tau", C.c_double), ("tile_rounds", C.c_int), ("tile_frac", C.c_double), ("cert", C.c_int),
         ("iter_max", C.c_longlong), ("bu_alpha", C.c_double), ("bu_beta", C.c_double),
         ("prune", C.c_int), ("prune_ratio", C.c_double), ("gr_work", C.c_double), ("K_heavy", C.c_int),
-        ("local_relabel", C.c_int), ("local_band", C.c_int),
+        ("local_relabel", C.c_int), ("local_band", C.c_int), ("fuse", C.c_int),
     ]
 
 
@@ -107,6 +107,10 @@ SIGNATURES = {
     "mf_engine_submit": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_double]),
     "mf_engine_run": (C.c_int, [_P, C.c_int64, C.POINTER(Result), C.c_int, C.POINTER(C.c_int)]),
     "mf_engine_live": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
+    "mf_engine_set_budget": (C.c_int, [_P, C.c_int64]),
+    "mf_engine_mem_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "mf_engine_serve": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, C.POINTER(Result), C.POINTER(C.c_int)]),
+    "mf_serve_lanes": (C.c_int, [C.c_int, _P, C.c_int, _P, _P, _P, _P, _P, C.POINTER(Result), C.POINTER(C.c_int)]),
     "mf_engine_now": (C.c_double, [_P]),
     "mf_engine_cache_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mf_engine_profile": (C.c_int, [_P, C.c_int]),
@@ -213,6 +217,25 @@ class Edges:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.mf_edges_free(self._h)
             self._h = None
+
+
+def serve_lanes(engines, lane, gids, s, t, arrive):
+    """Serving over several engines (one host thread and HIP stream each):
+    query i goes to engines[lane[i]], graph gids[i] of that engine, arriving
+    arrive[i] s after the call (ascending).  Results (tag = i) in completion
+    order."""
+    arr = (C.c_void_p * len(engines))(*[e._h.value if isinstance(e._h, C.c_void_p) else e._h for e in engines])
+    lane = np.ascontiguousarray(lane, np.int32)
+    gids = np.ascontiguousarray(gids, np.int32)
+    s = np.ascontiguousarray(s, np.int32)
+    t = np.ascontiguousarray(t, np.int32)
+    arrive = np.ascontiguousarray(arrive, np.float64)
+    nq = len(s)
+    out = (Result * max(nq, 1))()
+    k = C.c_int(0)
+    _check(lib().mf_serve_lanes(len(engines), C.cast(arr, C.c_void_p), nq, _ptr(lane), _ptr(gids), _ptr(s), _ptr(t),
+                                _ptr(arrive), out, C.byref(k)))
+    return [out[i] for i in range(k.value)]
 
 
 def uniform_queries(n, q, seed):
@@ -375,6 +398,28 @@ class Engine:
         out = (Result * cap)()
         k = C.c_int(0)
         _check(lib().mf_engine_run(self._h, max_steps, out, cap, C.byref(k)))
+        return [out[i] for i in range(k.value)]
+
+    def set_budget(self, nbytes: int):
+        """Slab arena size in bytes (0 = nslots full slabs of the largest graph)."""
+        _check(lib().mf_engine_set_budget(self._h, int(nbytes)))
+
+    def mem_stats(self):
+        v = [C.c_int64(0) for _ in range(4)]
+        _check(lib().mf_engine_mem_stats(self._h, *[C.byref(x) for x in v]))
+        return {"budget": v[0].value, "in_use": v[1].value, "peak": v[2].value, "admit_waits": v[3].value}
+
+    def serve(self, gids, s, t, arrive):
+        """Serving loop: query i arrives arrive[i] s after the call (ascending);
+        returns the results (tag = i) in completion order."""
+        gids = np.ascontiguousarray(gids, np.int32)
+        s = np.ascontiguousarray(s, np.int32)
+        t = np.ascontiguousarray(t, np.int32)
+        arrive = np.ascontiguousarray(arrive, np.float64)
+        nq = len(s)
+        out = (Result * max(nq, 1))()
+        k = C.c_int(0)
+        _check(lib().mf_engine_serve(self._h, nq, _ptr(gids), _ptr(s), _ptr(t), _ptr(arrive), out, C.byref(k)))
         return [out[i] for i in range(k.value)]
 
     def live(self):

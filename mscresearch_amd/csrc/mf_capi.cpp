@@ -6,6 +6,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mscmf.h"
@@ -102,6 +103,7 @@ Params from_c(const mf_params* p) {
     P.K_heavy = p->K_heavy;
     P.local_relabel = p->local_relabel;
     P.local_band = p->local_band;
+    P.fuse = p->fuse;
     return P;
 }
 
@@ -409,6 +411,7 @@ void mf_params_default(mf_params* p) {
     p->K_heavy = P.K_heavy;
     p->local_relabel = P.local_relabel;
     p->local_band = P.local_band;
+    p->fuse = P.fuse;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {
@@ -476,6 +479,106 @@ int mf_engine_run(mf_engine* e, int64_t max_steps, mf_result* out, int cap, int*
     }
     if (nout) *nout = k;
     return rc;
+}
+
+int mf_engine_set_budget(mf_engine* e, int64_t bytes) {
+    if (!e) return fail(MF_EINVAL, "null engine");
+    if (bytes < 0) return fail(MF_EINVAL, "negative budget");
+    return e->eng.set_budget((size_t)bytes);
+}
+
+int mf_engine_mem_stats(const mf_engine* e, int64_t* budget, int64_t* in_use, int64_t* peak,
+                        int64_t* admit_waits) {
+    if (!e) return fail(MF_EINVAL, "null engine");
+    size_t b = 0, u = 0, p = 0;
+    int64_t w = 0;
+    e->eng.mem_stats(&b, &u, &p, &w);
+    if (budget) *budget = (int64_t)b;
+    if (in_use) *in_use = (int64_t)u;
+    if (peak) *peak = (int64_t)p;
+    if (admit_waits) *admit_waits = w;
+    return MF_OK;
+}
+
+int mf_engine_serve(mf_engine* e, int nq, const int32_t* gid, const int32_t* s, const int32_t* t,
+                    const double* arrive, mf_result* out, int* nout) {
+    if (!e) return fail(MF_EINVAL, "null engine");
+    if (nq < 0 || (nq > 0 && (!gid || !s || !t || !arrive || !out))) return fail(MF_EINVAL, "bad arguments");
+    std::vector<Query> qs(nq);
+    for (int i = 0; i < nq; ++i) {
+        if (gid[i] < 0 || gid[i] >= e->eng.ngraphs()) return fail(MF_EINVAL, "bad graph index");
+        const int n = e->eng.graph(gid[i])->g.n;
+        if (s[i] < 0 || t[i] < 0 || s[i] >= n || t[i] >= n || s[i] == t[i]) return fail(MF_EINVAL, "bad query");
+        qs[i].graph = gid[i];
+        qs[i].s = s[i];
+        qs[i].t = t[i];
+        qs[i].tag = i;
+        qs[i].t_arrive = arrive[i];
+    }
+    std::vector<Result> got;
+    const int rc = e->eng.serve(std::move(qs), got);
+    int k = 0;
+    for (auto& r : got) {
+        if (k >= nq) break;
+        to_c(r, &out[k++]);
+    }
+    if (nout) *nout = k;
+    return rc;
+}
+
+int mf_serve_lanes(int nengines, mf_engine* const* engines, int nq, const int32_t* lane, const int32_t* gid,
+                   const int32_t* s, const int32_t* t, const double* arrive, mf_result* out, int* nout) {
+    if (nengines < 1 || !engines) return fail(MF_EINVAL, "no engines");
+    for (int l = 0; l < nengines; ++l) {
+        if (!engines[l]) return fail(MF_EINVAL, "null engine");
+        for (int k = 0; k < l; ++k)
+            if (engines[k] == engines[l]) return fail(MF_EINVAL, "an engine serves one lane");
+    }
+    if (nq < 0 || (nq > 0 && (!lane || !gid || !s || !t || !arrive || !out))) return fail(MF_EINVAL, "bad arguments");
+    std::vector<std::vector<Query>> per(nengines);
+    for (int i = 0; i < nq; ++i) {
+        if (lane[i] < 0 || lane[i] >= nengines) return fail(MF_EINVAL, "bad lane");
+        Engine& eng = engines[lane[i]]->eng;
+        if (gid[i] < 0 || gid[i] >= eng.ngraphs()) return fail(MF_EINVAL, "bad graph index");
+        const int n = eng.graph(gid[i])->g.n;
+        if (s[i] < 0 || t[i] < 0 || s[i] >= n || t[i] >= n || s[i] == t[i]) return fail(MF_EINVAL, "bad query");
+        if (i > 0 && arrive[i] < arrive[i - 1]) return fail(MF_EINVAL, "arrivals must be ascending");
+        Query q;
+        q.graph = gid[i];
+        q.s = s[i];
+        q.t = t[i];
+        q.tag = i;
+        q.t_arrive = arrive[i];
+        per[lane[i]].push_back(q);
+    }
+    // one host thread and one HIP stream per lane: the lanes' kernels overlap
+    // on the GPU and a lane of shallow queries never waits for a deep one
+    std::vector<std::vector<Result>> got(nengines);
+    std::vector<int> rc(nengines, MF_OK);
+    std::vector<std::string> err(nengines);
+    const double T0 = engines[0]->eng.now() + 1e-3;
+    std::vector<std::thread> th;
+    for (int l = 0; l < nengines; ++l) {
+        if (per[l].empty()) continue;
+        th.emplace_back([&, l]() {
+            rc[l] = engines[l]->eng.serve(per[l], got[l], T0);
+            if (rc[l] != MF_OK) err[l] = last_error();
+        });
+    }
+    for (auto& x : th) x.join();
+    std::vector<const Result*> all;
+    for (auto& g : got)
+        for (auto& r : g) all.push_back(&r);
+    std::sort(all.begin(), all.end(), [](const Result* a, const Result* b) { return a->st.t_end < b->st.t_end; });
+    int k = 0;
+    for (auto* r : all) {
+        if (k >= nq) break;
+        to_c(*r, &out[k++]);
+    }
+    if (nout) *nout = k;
+    for (int l = 0; l < nengines; ++l)
+        if (rc[l] != MF_OK) return fail(rc[l], err[l]);
+    return MF_OK;
 }
 
 int mf_engine_live(const mf_engine* e, int* live, int64_t* queued) {

@@ -16,6 +16,7 @@ struct DGraph {
     int E = 0;
     int ntiles = 0;
     int R = 0;
+    int max_deg = 0;  // longest row (an upper bound for induced subgraphs)
     const int32_t* row_ptr = nullptr;
     const int32_t* col = nullptr;
     const int64_t* cap = nullptr;
@@ -45,6 +46,11 @@ struct Ctr {
     int32_t nheavy;          // heavy entries (back of the list)
     int32_t ngiant;          // rows longer than kGiantDeg in the giant list
     int32_t ngchunks;        // kGiantChunk-entry tasks over those rows
+    // sparse counting of fused cycles: vertices pushed into since the last
+    // touch reset are listed in hl (deduplicated by stamps in aux)
+    int32_t ntouched;
+    int32_t epoch;           // touch epoch: stamps -(2 epoch + 1) touched, -(2 epoch + 2) counted
+    int32_t prev_nlight, prev_nheavy;  // active list being replaced by a sparse count
     int64_t pos_excess;      // sum of positive excess off s,t
     int32_t neg_count;       // deficits (excess<0) off s,t
     int32_t fixed_edges;     // RM repairs in the last RM pass

@@ -4,10 +4,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <climits>
 #include <cstring>
 #include <mutex>
+#include <thread>
 
 #include "mf_kernels.h"
 
@@ -32,6 +34,13 @@ const std::string& last_error() { return g_err; }
 double wall_seconds() {
     using namespace std::chrono;
     return duration_cast<duration<double>>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Every engine's clock starts at the same instant, so times from engines
+// serving side by side compare directly.
+static double clock_epoch() {
+    static const double t0 = wall_seconds();
+    return t0;
 }
 
 // ----------------------------------------------------------- device graph
@@ -83,6 +92,7 @@ int upload_graph(const HostGraph& hg, int tile_max, DeviceGraph& dg) {
     dg.g.n = n;
     dg.g.E = (int)E;
     dg.g.ntiles = ntiles;
+    for (int v = 0; v < n; ++v) dg.g.max_deg = std::max(dg.g.max_deg, hg.row_ptr[v + 1] - hg.row_ptr[v]);
     dg.g.R = (int)rstart.size() - 1;
     dg.g.row_ptr = dg.d_row_ptr;
     dg.g.col = dg.d_col;
@@ -119,41 +129,101 @@ int init_cert_consts(DeviceGraph& dg, hipStream_t st) {
 }
 
 // ------------------------------------------------------------------- slabs
-Slab::~Slab() {
-    (void)hipFree(res); (void)hipFree(excess); (void)hipFree(h); (void)hipFree(fa); (void)hipFree(fb); (void)hipFree(lock);
-    (void)hipFree(aux); (void)hipFree(hl); (void)hipFree(tile_cnt); (void)hipFree(tile_hot); (void)hipFree(lbkt);
-    (void)hipFree(giant); (void)hipFree(gacc);
-}
-
-size_t Slab::bytes_for(int n, int64_t E, int ntiles) {
-    const size_t gcap = (size_t)E / (kGiantDeg + 1) + 1;
-    return (size_t)E * 8 + (size_t)n * (8 + 4 * 6) + (size_t)ntiles * 8 + sizeof(int32_t) * 2 * kLocalLevels +
-           gcap * (4 * 3 + 4 + 8);
-}
-
-int alloc_slab(int n, int64_t E, int ntiles, Slab& sl) {
+size_t Slab::layout(int n, int64_t E, int ntiles, char* base, Slab* sl) {
     const size_t nn = (size_t)std::max(n, 1), ee = (size_t)std::max<int64_t>(E, 1),
                  tt = (size_t)std::max(ntiles, 1);
-    HIPCHK(hipMalloc(&sl.res, 8 * ee));
-    HIPCHK(hipMalloc(&sl.excess, 8 * nn));
-    HIPCHK(hipMalloc(&sl.h, 4 * nn));
-    HIPCHK(hipMalloc(&sl.fa, 4 * nn));
-    HIPCHK(hipMalloc(&sl.fb, 4 * nn));
-    HIPCHK(hipMalloc(&sl.lock, 4 * nn));
-    HIPCHK(hipMalloc(&sl.aux, 4 * nn));
-    HIPCHK(hipMalloc(&sl.hl, 4 * nn));
-    HIPCHK(hipMalloc(&sl.tile_cnt, 4 * tt));
-    HIPCHK(hipMalloc(&sl.tile_hot, 4 * tt));
-    HIPCHK(hipMalloc(&sl.lbkt, sizeof(int32_t) * 2 * kLocalLevels));
     // rows longer than kGiantDeg: at most E / (kGiantDeg + 1) of them
-    sl.gcap = (int)(ee / (kGiantDeg + 1) + 1);
-    HIPCHK(hipMalloc(&sl.giant, sizeof(int32_t) * (3 * (size_t)sl.gcap + 1)));
-    HIPCHK(hipMalloc(&sl.gacc, sizeof(int64_t) * (size_t)sl.gcap));
-    sl.n_cap = n;
-    sl.E_cap = (int)E;
-    sl.tiles_cap = ntiles;
-    sl.bytes = Slab::bytes_for(n, E, ntiles);
+    const size_t gcap = ee / (kGiantDeg + 1) + 1;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return base ? base + at : nullptr;
+    };
+    char* p_res = take(8 * ee);
+    char* p_ex = take(8 * nn);
+    char* p_h = take(4 * nn);
+    char* p_fa = take(4 * nn);
+    char* p_fb = take(4 * nn);
+    char* p_lock = take(4 * nn);
+    char* p_aux = take(4 * nn);
+    char* p_hl = take(4 * nn);
+    char* p_tc = take(4 * tt);
+    char* p_th = take(4 * tt);
+    char* p_lb = take(sizeof(int32_t) * 2 * kLocalLevels);
+    char* p_gi = take(sizeof(int32_t) * (3 * gcap + 1));
+    char* p_ga = take(sizeof(int64_t) * gcap);
+    if (sl) {
+        sl->res = (int64_t*)p_res;
+        sl->excess = (int64_t*)p_ex;
+        sl->h = (int32_t*)p_h;
+        sl->fa = (int32_t*)p_fa;
+        sl->fb = (int32_t*)p_fb;
+        sl->lock = (int32_t*)p_lock;
+        sl->aux = (int32_t*)p_aux;
+        sl->hl = (int32_t*)p_hl;
+        sl->tile_cnt = (int32_t*)p_tc;
+        sl->tile_hot = (int32_t*)p_th;
+        sl->lbkt = (int32_t*)p_lb;
+        sl->giant = (int32_t*)p_gi;
+        sl->gacc = (int64_t*)p_ga;
+        sl->gcap = (int)gcap;
+        sl->n_cap = n;
+        sl->E_cap = (int)E;
+        sl->tiles_cap = ntiles;
+    }
+    return off;
+}
+
+int SlabArena::create(size_t bytes) {
+    destroy();
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (bytes == 0) bytes = 256;
+    HIPCHK(hipMalloc(&base, bytes));
+    size = bytes;
+    used = peak = 0;
+    free_.clear();
+    free_[0] = bytes;
     return MF_OK;
+}
+
+void SlabArena::destroy() {
+    if (base) (void)hipFree(base);
+    base = nullptr;
+    size = used = 0;
+    free_.clear();
+}
+
+bool SlabArena::alloc(size_t bytes, size_t* off) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < bytes) continue;
+        *off = it->first;
+        const size_t rest = it->second - bytes;
+        const size_t at = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[at] = rest;
+        used += bytes;
+        peak = std::max(peak, used);
+        return true;
+    }
+    return false;
+}
+
+void SlabArena::release(size_t off, size_t bytes) {
+    used -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            free_.erase(it);
+        }
+    }
 }
 
 // --------------------------------------------------------------- scheduler
@@ -179,6 +249,7 @@ void SlotSched::begin(const Params* P, const DGraph& g, int s, int t, bool prune
     RM_FREQ_ = P->rm_freq0;
     iter_ = 0;
     last_gr_iter_ = 0;
+    gr_depth_ = 0;
     stag_ = no_stag_ = 0;
     last_pos_ = INT64_MAX;
     force_gr_ = false;
@@ -337,6 +408,7 @@ Plan SlotSched::plan(const Ctr& c) {
             return bfs_step(c, bfs_mode_);
         }
         stats.bfs_levels += L;
+        if (bfs_mode_ == BFS_GR) gr_depth_ = L;
         p.stages = ST_RESET_ACT | ST_ACTIVE;
         p.reset_act = RST_ACTIVE;
         just_gr_ = true;
@@ -418,10 +490,13 @@ Plan SlotSched::plan(const Ctr& c) {
         }
         rm_last_ = false;
         // GR when due by count (BC:1496), forced, when the flow into t has
-        // stalled, or when the discharge has done about a GR's worth of scans
+        // stalled, or when the discharge has done about a GR's worth of scans.
+        // A stall only counts once the excess has had a quarter as many
+        // iterations as the last GR had levels: excess moves about one hop
+        // per iteration, so on deep graphs (grids) it is still in transit.
         const uint64_t scans = c.wk[WF_DISCHARGE][0] - scans_at_gr_;
-        const bool gr_due = force_gr_ || (iter_ - last_gr_iter_ >= GR_FREQ_) ||
-                            (P_->gr_stall > 0 && stall_ >= P_->gr_stall) ||
+        const bool gr_due = force_gr_ || (iter_ - last_gr_iter_ >= std::max<int64_t>(GR_FREQ_, gr_depth_ / 4)) ||
+                            (P_->gr_stall > 0 && stall_ >= P_->gr_stall && 4 * (iter_ - last_gr_iter_) >= gr_depth_) ||
                             (P_->gr_work > 0 && (double)scans >= P_->gr_work * (double)g_.E);
         if (gr_due && !exact) return start_gr(&c);
         // local relabel (BC:1582-1721): BFS from t and the deficits through
@@ -470,7 +545,13 @@ Plan SlotSched::plan(const Ctr& c) {
             p.reset_act |= RST_RM;
             rm_last_ = true;
         }
-        ++iter_;
+        // deep graphs: several cycles per host round trip, up to the next
+        // count-due global relabel
+        if (!use_path && P_->fuse > 1 && gr_depth_ >= 64) {
+            const int64_t left = std::max<int64_t>(1, std::max<int64_t>(GR_FREQ_, gr_depth_ / 4) - (iter_ - last_gr_iter_));
+            p.cycles = (int)std::min<int64_t>(P_->fuse, left);
+        }
+        iter_ += p.cycles;
         stats.iters = iter_;
         return p;
     }
@@ -497,7 +578,7 @@ Plan SlotSched::plan(const Ctr& c) {
 Engine::~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     slots_.clear();
-    slabs_.clear();
+    arena_.destroy();
     cache_.clear();
     graphs_.clear();
     (void)hipFree(d_views_);
@@ -508,6 +589,9 @@ Engine::~Engine() {
     if (h_hash_) (void)hipHostFree(h_hash_);
     for (auto& ev : ev_) (void)hipEventDestroy(ev);
     if (stream_) (void)hipStreamDestroy(stream_);
+    if (side_) (void)hipStreamDestroy(side_);
+    if (ev_up_) (void)hipEventDestroy(ev_up_);
+    if (ev_side_) (void)hipEventDestroy(ev_side_);
 }
 
 double Engine::now() const { return wall_seconds() - t0_; }
@@ -519,8 +603,12 @@ int Engine::init(int nslots, const Params& P) {
     P_ = P;
     nslots_ = nslots;
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    // views followed by the launch lists (one per stage group, 32 groups)
-    list_cap_ = (size_t)nslots * 32;
+    HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ev_up_, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_side_, hipEventDisableTiming));
+    // views followed by the launch lists: a slot sits in at most one group per
+    // stage plus three per extra fused cycle
+    list_cap_ = (size_t)nslots * (32 + 3 * (size_t)std::max(1, P.fuse));
     HIPCHK(hipMalloc(&d_views_, sizeof(SlotView) * nslots + sizeof(int) * list_cap_));
     HIPCHK(hipMalloc(&d_hash_, sizeof(unsigned long long) * nslots));
     HIPCHK(hipHostMalloc(&h_stage_, sizeof(SlotView) * nslots + sizeof(int) * list_cap_));
@@ -529,38 +617,101 @@ int Engine::init(int nslots, const Params& P) {
     HIPCHK(hipMemset(d_ctrs_, 0, sizeof(Ctr) * nslots));
     HIPCHK(hipHostMalloc((void**)&h_hash_, sizeof(unsigned long long) * nslots));
     memset(h_ctr_, 0, sizeof(Ctr) * nslots);
-    ev_.resize(64);
+    ev_.resize(256);  // timed launch ranges per step (profiling)
     for (auto& ev : ev_) HIPCHK(hipEventCreate(&ev));
     slots_.resize(nslots);
-    slabs_.resize(nslots);
-    t0_ = wall_seconds();
+    t0_ = clock_epoch();
     return MF_OK;
 }
 
 int Engine::add_graph(std::shared_ptr<DeviceGraph> g) {
     graphs_.push_back(g);
-    // grow slabs to fit the largest graph
-    for (int i = 0; i < nslots_; ++i) {
-        Slab* sl = slabs_[i].get();
-        if (!sl || sl->n_cap < g->g.n || sl->E_cap < g->g.E || sl->tiles_cap < g->g.ntiles) {
-            const int n = std::max(g->g.n, sl ? sl->n_cap : 0);
-            const int E = std::max(g->g.E, sl ? sl->E_cap : 0);
-            const int T = std::max(g->g.ntiles, sl ? sl->tiles_cap : 0);
-            auto ns = std::make_unique<Slab>();
-            slabs_[i].reset();
-            int rc = alloc_slab(n, E, T, *ns);
-            if (rc != MF_OK) return rc;
-            slabs_[i] = std::move(ns);
-        }
-    }
+    if (arena_.base && budget_ == 0) arena_stale_ = true;  // resized when nothing holds a slab
     return (int)graphs_.size() - 1;
 }
 
-size_t Engine::slab_bytes_total() const {
-    size_t b = 0;
-    for (auto& s : slabs_) if (s) b += s->bytes;
-    return b;
+int Engine::make_arena() {
+    size_t bytes = budget_;
+    if (bytes == 0) {
+        size_t full = 0;
+        for (auto& g : graphs_) full = std::max(full, Slab::bytes_for(g->g.n, g->g.E, g->g.ntiles));
+        bytes = full * (size_t)nslots_;
+    }
+    arena_stale_ = false;
+    return arena_.create(bytes);
 }
+
+int Engine::set_budget(size_t bytes) {
+    for (auto& S : slots_)
+        if (S.live || S.pinned) { set_error("set_budget: slots are in use"); return MF_EBUSY; }
+    for (int i = 0; i < nslots_; ++i) drop_slab(i);
+    arena_.destroy();
+    budget_ = bytes;
+    return MF_OK;
+}
+
+void Engine::mem_stats(size_t* budget, size_t* in_use, size_t* peak, int64_t* waits) const {
+    if (budget) *budget = arena_.size;
+    if (in_use) *in_use = arena_.used;
+    if (peak) *peak = arena_.peak;
+    if (waits) *waits = admit_waits_;
+}
+
+void Engine::drop_slab(int i) {
+    Slab& sl = slots_[i].slab;
+    if (sl.held) arena_.release(sl.off, sl.bytes);
+    sl = Slab();
+}
+
+int Engine::ensure_slab(int i, int n, int64_t E, int ntiles) {
+    if (slots_[i].slab.fits(n, E, ntiles)) return MF_OK;
+    drop_slab(i);
+    if (arena_stale_) {
+        bool busy = false;
+        for (auto& S : slots_) busy = busy || S.live || S.pinned;
+        if (!busy) {
+            for (int j = 0; j < nslots_; ++j) drop_slab(j);
+            arena_.destroy();
+        }
+    }
+    if (!arena_.base) {
+        int rc = make_arena();
+        if (rc != MF_OK) return rc;
+    }
+    const size_t need = Slab::bytes_for(n, E, ntiles);
+    size_t off = 0;
+    if (!arena_.alloc(need, &off)) {
+        // slabs kept by idle slots go back first
+        for (int j = 0; j < nslots_; ++j)
+            if (j != i && !slots_[j].live && !slots_[j].pinned) drop_slab(j);
+        if (!arena_.alloc(need, &off)) return kNoRoom;
+    }
+    Slab& sl = slots_[i].slab;
+    Slab::layout(n, E, ntiles, arena_.base + off, &sl);
+    sl.off = off;
+    sl.bytes = need;
+    sl.held = true;
+    return MF_OK;
+}
+
+void Engine::run_dims(const Query& q, int* n, int64_t* E, int* ntiles) const {
+    const DeviceGraph& G = *graphs_[q.graph];
+    *n = G.g.n;
+    *E = G.g.E;
+    *ntiles = G.g.ntiles;
+    if (!P_.prune || !G.classes) return;
+    const int cls = G.scc[q.s];
+    auto it = G.fwd_cache.find(cls);
+    if (it == G.fwd_cache.end() || !it->second->mask[q.t]) return;
+    const std::string key = std::to_string(q.graph) + ":c" + std::to_string(cls) + ":" + std::to_string(G.wcc[q.t]);
+    auto ct = cache_.find(key);
+    if (ct == cache_.end() || !ct->second.g) return;
+    const DGraph& r = ct->second.g->g;
+    *n = r.n;
+    *E = r.E;
+    *ntiles = r.ntiles;
+}
+
 
 int Engine::submit(const Query& q) {
     if (q.graph < 0 || q.graph >= (int)graphs_.size()) { set_error("bad graph index"); return MF_EINVAL; }
@@ -582,7 +733,7 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
     S.run_graph.reset();
     const DeviceGraph& G = *graphs_[q.graph];
     S.parent_n = G.g.n;
-    Slab& sl = *slabs_[i];
+    Slab& sl = S.slab;
     SlotView& v = S.view;
     v = SlotView();
     v.g = G.g;
@@ -748,12 +899,37 @@ int Engine::prune_decide(int i) {
 }
 
 int Engine::step(std::vector<Result>& out) {
-    // refill free slots (continuous batching)
+    // refill free slots (continuous batching), first come first served, each
+    // query admitted when its slab fits in the arena
     for (int i = 0; i < nslots_ && !queue_.empty(); ++i) {
         if (slots_[i].live || slots_[i].pinned) continue;
-        Query q = queue_.front();
+        const Query q = queue_.front();
+        int n = 0, ntiles = 0;
+        int64_t E = 0;
+        run_dims(q, &n, &E, &ntiles);
+        int rc = ensure_slab(i, n, E, ntiles);
+        if (rc == kNoRoom) {
+            bool busy = false;
+            for (auto& S : slots_) busy = busy || S.live || S.pinned;
+            if (busy) {  // wait for running queries to give bytes back
+                ++admit_waits_;
+                break;
+            }
+            queue_.pop_front();  // cannot fit even in an empty arena
+            Result r;
+            r.tag = q.tag;
+            r.st.status = MF_ENOMEM;
+            r.st.t_submit = q.t_arrive;
+            r.st.t_start = r.st.t_end = now();
+            set_error("query needs " + std::to_string(Slab::bytes_for(n, E, ntiles)) +
+                      " slab bytes, more than the arena holds (" + std::to_string(arena_.size) + ")");
+            out.push_back(r);
+            --i;
+            continue;
+        }
+        if (rc != MF_OK) return rc;
         queue_.pop_front();
-        int rc = admit(i, q);
+        rc = admit(i, q);
         if (rc != MF_OK) return rc;
         memset(&h_ctr_[i], 0, sizeof(Ctr));
     }
@@ -817,11 +993,47 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         Group g;
         g.off = (int)lpos;
         for (int i = 0; i < nslots_; ++i)
-            if (slots_[i].live && pred(i)) hl[lpos++] = i;
+            if (slots_[i].live && pred(i)) {
+                if (lpos < list_cap_) hl[lpos] = i;  // overflow is reported below
+                ++lpos;
+            }
         g.cnt = (int)lpos - g.off;
         return g;
     };
-    auto has = [&](int i, unsigned bit) { return (plans[i].stages & bit) != 0; };
+    // Slots whose step is only a single-workgroup BFS (a GR or pruning BFS
+    // under way) run on the side stream: a few workgroups for many levels,
+    // overlapped with the other slots' bandwidth-bound kernels.
+    std::vector<char> side(nslots_, 0);
+    {
+        const unsigned bfs_only = ST_RESET_PRE | ST_GR_SEED | ST_BFS_SEED1 | ST_BFS;
+        int nside = 0, nmain = 0;
+        for (int i = 0; i < nslots_; ++i) {
+            if (!slots_[i].live || !plans[i].stages) continue;
+            const Plan& p = plans[i];
+            side[i] = (p.stages & ~bfs_only) == 0 && (p.stages & ST_BFS) && p.bfs_kind == BK_SMALL;
+            nside += side[i];
+            nmain += !side[i];
+        }
+        if (nside == 0 || nmain == 0) std::fill(side.begin(), side.end(), 0);
+    }
+    auto has = [&](int i, unsigned bit) { return !side[i] && (plans[i].stages & bit) != 0; };
+    auto hs = [&](int i, unsigned bit) { return side[i] && (plans[i].stages & bit) != 0; };
+    std::vector<std::pair<unsigned, Group>> side_pre;
+    Group s_seed, s_seed1[3], s_bfs[3];
+    {
+        std::vector<unsigned> vals;
+        for (int i = 0; i < nslots_; ++i)
+            if (slots_[i].live && hs(i, ST_RESET_PRE) &&
+                std::find(vals.begin(), vals.end(), plans[i].reset_pre) == vals.end())
+                vals.push_back(plans[i].reset_pre);
+        for (unsigned w : vals)
+            side_pre.push_back({w, make_list([&](int i) { return hs(i, ST_RESET_PRE) && plans[i].reset_pre == w; })});
+        s_seed = make_list([&](int i) { return hs(i, ST_GR_SEED); });
+        for (int m = 0; m < 3; ++m) {
+            s_seed1[m] = make_list([&](int i) { return hs(i, ST_BFS_SEED1) && plans[i].seed_mode == m; });
+            s_bfs[m] = make_list([&](int i) { return hs(i, ST_BFS) && plans[i].bfs_mode == m; });
+        }
+    }
     // reset_pre grouped by distinct mask values
     std::vector<std::pair<unsigned, Group>> pre_groups;
     {
@@ -870,6 +1082,17 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     }
     Group g_act = make_list([&](int i) { return has(i, ST_ACTIVE); });
     Group g_cert = make_list([&](int i) { return has(i, ST_CERT); });
+    // slots of fused steps: per extra cycle c, discharge lists by kind and the
+    // count list
+    int max_cycles = 1;
+    for (int i = 0; i < nslots_; ++i)
+        if (slots_[i].live && has(i, ST_DISCHARGE)) max_cycles = std::max(max_cycles, plans[i].cycles);
+    std::vector<std::array<Group, 3>> g_cyc(max_cycles);
+    for (int c = 1; c < max_cycles; ++c) {
+        for (int kind = DK_TILE; kind <= DK_LIST; ++kind)
+            g_cyc[c][kind] = make_list([&](int i) { return has(i, ST_DISCHARGE) && plans[i].cycles > c && plans[i].dis_kind == kind; });
+        g_cyc[c][2] = make_list([&](int i) { return has(i, ST_DISCHARGE) && plans[i].cycles > c; });
+    }
     if (lpos > list_cap_) { set_error("launch list overflow"); return MF_EINVAL; }
     const int nlive = live();
     if (nlive == 0) return 0;
@@ -886,19 +1109,41 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     hipStream_t st = stream_;
     // profiling: an event pair around each kernel family's launches
     std::vector<std::pair<int, int>> timed;  // (family, event pair index)
-    auto tb = [&](int fam) {
+    auto tbs = [&](int fam, hipStream_t s) {
         if (!profile) return -1;
         const int k = (int)timed.size();
         if (2 * k + 1 >= (int)ev_.size()) return -1;
-        (void)hipEventRecord(ev_[2 * k], st);
+        (void)hipEventRecord(ev_[2 * k], s);
         timed.push_back({fam, k});
         return k;
     };
-    auto te = [&](int k) {
-        if (k >= 0) (void)hipEventRecord(ev_[2 * k + 1], st);
+    auto tes = [&](int k, hipStream_t s) {
+        if (k >= 0) (void)hipEventRecord(ev_[2 * k + 1], s);
     };
+    auto tb = [&](int fam) { return tbs(fam, st); };
+    auto te = [&](int k) { tes(k, st); };
     auto gridN = [&](int i) { return clampi(cdiv(slots_[i].view.g.n, 256), 1, 4096); };
     auto gridE = [&](int i) { return clampi(cdiv(slots_[i].view.g.E, 1024), 1, 8192); };
+    auto giants = [&](const Group& g) {
+        for (int k = 0; k < g.cnt; ++k)
+            if (slots_[hl[g.off + k]].view.g.max_deg > kGiantDeg) return 1;
+        return 0;
+    };
+    // side stream: starts after the view / list upload on the main stream
+    const bool use_side = !side_pre.empty() || s_seed.cnt || s_bfs[0].cnt || s_bfs[1].cnt || s_bfs[2].cnt;
+    if (use_side) {
+        HIPCHK(hipEventRecord(ev_up_, st));
+        HIPCHK(hipStreamWaitEvent(side_, ev_up_, 0));
+        const int k = tbs(WF_BFS, side_);
+        for (auto& pg : side_pre) launch_reset(side_, d_views_, L(pg.second), pg.second.cnt, pg.first);
+        if (s_seed.cnt) launch_gr_seed(side_, d_views_, L(s_seed), s_seed.cnt, maxof(s_seed, gridN));
+        for (int m = 0; m < 3; ++m)
+            if (s_seed1[m].cnt) launch_bfs_seed1(side_, d_views_, L(s_seed1[m]), s_seed1[m].cnt, maxof(s_seed1[m], gridN), m);
+        for (int m = 0; m < 3; ++m)
+            if (s_bfs[m].cnt) launch_bfs_small(side_, d_views_, L(s_bfs[m]), s_bfs[m].cnt, m, P_.small_levels);
+        tes(k, side_);
+        HIPCHK(hipEventRecord(ev_side_, side_));
+    }
     for (auto& pg : pre_groups) launch_reset(st, d_views_, L(pg.second), pg.second.cnt, pg.first);
     if (g_init.cnt) {
         const int k = tb(WF_INIT);
@@ -951,12 +1196,12 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     }
     if (g_dis[DK_TILE].cnt) {
         const int k = tb(WF_DISCHARGE);
-        launch_discharge_tile(st, d_views_, L(g_dis[DK_TILE]), g_dis[DK_TILE].cnt, maxof(g_dis[DK_TILE], [&](int i) { return plans[i].dis_grid; }), 0, P_.tile_rounds);
+        launch_discharge_tile(st, d_views_, L(g_dis[DK_TILE]), g_dis[DK_TILE].cnt, maxof(g_dis[DK_TILE], [&](int i) { return plans[i].dis_grid; }), 0, P_.tile_rounds, giants(g_dis[DK_TILE]));
         te(k);
     }
     if (g_dis[DK_LIST].cnt) {
         const int k = tb(WF_DISCHARGE);
-        launch_discharge_list(st, d_views_, L(g_dis[DK_LIST]), g_dis[DK_LIST].cnt, maxof(g_dis[DK_LIST], [&](int i) { return plans[i].dis_grid; }));
+        launch_discharge_list(st, d_views_, L(g_dis[DK_LIST]), g_dis[DK_LIST].cnt, maxof(g_dis[DK_LIST], [&](int i) { return plans[i].dis_grid; }), giants(g_dis[DK_LIST]));
         te(k);
     }
     if (g_dis[DK_PATH].cnt) {
@@ -967,7 +1212,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     if (g_hot.cnt) {
         const int k = tb(WF_DISCHARGE);
         launch_hot_select(st, d_views_, L(g_hot), g_hot.cnt, P_.hot_k);
-        launch_discharge_tile(st, d_views_, L(g_hot), g_hot.cnt, maxof(g_hot, [&](int i) { return slots_[i].view.g.ntiles; }), 1, P_.hot_rounds);
+        launch_discharge_tile(st, d_views_, L(g_hot), g_hot.cnt, maxof(g_hot, [&](int i) { return slots_[i].view.g.ntiles; }), 1, P_.hot_rounds, 0);
         te(k);
     }
     if (g_rm.cnt) {
@@ -977,8 +1222,30 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     }
     for (auto& ag : act_groups) launch_reset(st, d_views_, L(ag.second), ag.second.cnt, ag.first);
     if (g_act.cnt) { const int k = tb(WF_ACTIVE); launch_active(st, d_views_, L(g_act), g_act.cnt, maxof(g_act, gridN)); te(k); }
+    // further discharge + count cycles of fused steps (deep graphs): the
+    // count after each only looks at the previous list and the vertices the
+    // discharge pushed into
+    for (int c = 1; c < max_cycles; ++c) {
+        launch_reset(st, d_views_, L(g_cyc[c][2]), g_cyc[c][2].cnt, RST_TOUCH);
+        for (int kind = DK_TILE; kind <= DK_LIST; ++kind) {
+            const Group& g = g_cyc[c][kind];
+            if (!g.cnt) continue;
+            const int k = tb(WF_DISCHARGE);
+            if (kind == DK_TILE)
+                launch_discharge_tile(st, d_views_, L(g), g.cnt, maxof(g, [&](int i) { return plans[i].dis_grid; }), 0, P_.tile_rounds, giants(g));
+            else
+                launch_discharge_list(st, d_views_, L(g), g.cnt, maxof(g, [&](int i) { return std::max(plans[i].dis_grid, 64); }), giants(g));
+            te(k);
+        }
+        const Group& ga = g_cyc[c][2];
+        launch_reset(st, d_views_, L(ga), ga.cnt, RST_SPARSE);
+        const int k = tb(WF_ACTIVE);
+        launch_active_sparse(st, d_views_, L(ga), ga.cnt, maxof(ga, [&](int i) { return clampi(cdiv(h_ctr_[i].active, 256), 8, 1024); }));
+        te(k);
+    }
     if (g_cert.cnt) { const int k = tb(WF_CERT); launch_cert(st, d_views_, L(g_cert), g_cert.cnt, maxof(g_cert, gridN), P_.cert); te(k); }
     HIPCHK(hipGetLastError());
+    if (use_side) HIPCHK(hipStreamWaitEvent(st, ev_side_, 0));
     HIPCHK(hipMemcpyAsync(h_ctr_, d_ctrs_, sizeof(Ctr) * nslots_, hipMemcpyDeviceToHost, st));
     if (g_mask.cnt)
         HIPCHK(hipMemcpyAsync(h_hash_, d_hash_, sizeof(unsigned long long) * nslots_, hipMemcpyDeviceToHost, st));
@@ -1007,6 +1274,10 @@ int Engine::pin(int slot, const Query& q) {
     if (slots_[slot].live) { set_error("slot busy"); return MF_EBUSY; }
     const int n = graphs_[q.graph]->g.n;
     if (q.s < 0 || q.t < 0 || q.s >= n || q.t >= n || q.s == q.t) { set_error("bad query"); return MF_EINVAL; }
+    const DGraph& g = graphs_[q.graph]->g;
+    int rc = ensure_slab(slot, g.n, g.E, g.ntiles);  // pinned queries run on the full graph
+    if (rc == kNoRoom) { set_error("pin: no room in the slab arena"); return MF_ENOMEM; }
+    if (rc != MF_OK) return rc;
     slots_[slot].pinned = true;
     slots_[slot].solved = false;
     slots_[slot].q = q;
@@ -1106,7 +1377,11 @@ int Engine::dbg_begin(int gid, int s, int t) {
     for (auto& S : slots_) if (S.live) { set_error("engine busy"); return MF_EBUSY; }
     Query q;
     q.graph = gid; q.s = s; q.t = t;
-    int rc = admit(0, q);
+    const DGraph& g = graphs_[gid]->g;
+    int rc = ensure_slab(0, g.n, g.E, g.ntiles);
+    if (rc == kNoRoom) { set_error("no room in the slab arena"); return MF_ENOMEM; }
+    if (rc != MF_OK) return rc;
+    rc = admit(0, q);
     if (rc != MF_OK) return rc;
     memset(&h_ctr_[0], 0, sizeof(Ctr));
     return MF_OK;
@@ -1123,6 +1398,34 @@ void Engine::dbg_end() { slots_[0].live = false; slots_[0].run_graph.reset(); }
 
 const Ctr& Engine::ctr(int i) const { return h_ctr_[i]; }
 SlotView& Engine::view(int i) { return slots_[i].view; }
+
+int Engine::serve(std::vector<Query> qs, std::vector<Result>& out, double T0_in) {
+    for (size_t i = 1; i < qs.size(); ++i)
+        if (qs[i].t_arrive < qs[i - 1].t_arrive) { set_error("serve: arrivals must be ascending"); return MF_EINVAL; }
+    for (auto& q : qs)
+        if (q.graph < 0 || q.graph >= (int)graphs_.size()) { set_error("serve: bad graph index"); return MF_EINVAL; }
+    const double T0 = T0_in < 0 ? now() : T0_in;
+    size_t next = 0;
+    const size_t first = out.size();
+    while (out.size() - first < qs.size()) {
+        const double tn = now() - T0;
+        while (next < qs.size() && qs[next].t_arrive <= tn) {
+            Query q = qs[next++];
+            q.t_arrive += T0;
+            queue_.push_back(q);
+        }
+        if (live() == 0 && queue_.empty()) {
+            if (next >= qs.size()) break;
+            // idle until the next arrival
+            const double wait = qs[next].t_arrive - (now() - T0);
+            if (wait > 0) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(std::min(wait, 1e-3) * 1e6)));
+            continue;
+        }
+        int rc = step(out);
+        if (rc < 0) return rc;
+    }
+    return MF_OK;
+}
 
 int Engine::run(std::vector<Result>& out, int64_t max_steps) {
     int64_t k = 0;

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <deque>
+#include <map>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -41,7 +42,9 @@ struct Params {
     double tile_frac = 1.0 / 64.0;  // region mode when active >= tile_frac * n
     int cert = 1;              // certify every answer on the GPU: 1 pair sums by fingerprint, 2 entry by entry, 0 off
     long long iter_max = 20000000;  // BC:1476, but reported as an error
-    int small_levels = 1 << 30;
+    // levels one single-workgroup BFS launch may run: bounds a step's length
+    // on deep graphs so batched slots keep moving while one relabels
+    int small_levels = 256;
     double bu_alpha = 14.0, bu_beta = 24.0;  // direction-optimizing BFS thresholds
     int prune = 1;             // per-query reachability pruning (BC:2678)
     double prune_ratio = 0.995;  // BC:2647
@@ -58,6 +61,9 @@ struct Params {
     // R-MAT scale 22)
     int local_relabel = 0;
     int local_band = 128;
+    // discharge + count cycles per engine step (one host round trip) when the
+    // last global relabel was at least 64 levels deep (grids); 1 = off
+    int fuse = 8;
 };
 
 // ---------------------------------------------------------- device memory
@@ -106,6 +112,8 @@ int upload_graph(const HostGraph& hg, int tile_max, DeviceGraph& dg);
 // Allocate and compute dg's certificate constants (after its CSR is set).
 int init_cert_consts(DeviceGraph& dg, hipStream_t st);
 
+// Per-query device arrays (the reference's DynamicState, BC:47-113), carved
+// from the engine's slab arena for the run graph's (n, E, ntiles).
 struct Slab {
     int n_cap = 0, E_cap = 0, tiles_cap = 0;
     int64_t* res = nullptr;
@@ -122,12 +130,28 @@ struct Slab {
     int32_t* giant = nullptr;
     int64_t* gacc = nullptr;
     int gcap = 0;
-    size_t bytes = 0;
-    ~Slab();
-    static size_t bytes_for(int n, int64_t E, int ntiles);
+    size_t off = 0, bytes = 0;  // arena block
+    bool held = false;
+    bool fits(int n, int64_t E, int ntiles) const {
+        return held && n <= n_cap && E <= (int64_t)E_cap && ntiles <= tiles_cap;
+    }
+    // Bytes of a slab for (n, E, ntiles), every array 256-byte aligned; with
+    // base != nullptr the arrays of sl are carved from base.
+    static size_t layout(int n, int64_t E, int ntiles, char* base, Slab* sl);
+    static size_t bytes_for(int n, int64_t E, int ntiles) { return layout(n, E, ntiles, nullptr, nullptr); }
 };
 
-int alloc_slab(int n, int64_t E, int ntiles, Slab& sl);
+// One device allocation that slabs are carved from: first fit by address,
+// free neighbours coalesced.
+struct SlabArena {
+    char* base = nullptr;
+    size_t size = 0, used = 0, peak = 0;
+    std::map<size_t, size_t> free_;  // offset -> bytes
+    int create(size_t bytes);
+    void destroy();
+    bool alloc(size_t bytes, size_t* off);
+    void release(size_t off, size_t bytes);
+};
 
 // --------------------------------------------------------------- scheduler
 enum StageBit : unsigned {
@@ -158,6 +182,7 @@ struct Plan {
     int dis_kind = DK_LIST, dis_grid = 1;
     int seed_mode = 0;
     int local_cap = 0;        // ST_LOCAL: level cap of the local relabel
+    int cycles = 1;           // ST_DISCHARGE: discharge + count cycles in this step
 };
 
 // Per-query statistics (host side).
@@ -220,6 +245,7 @@ private:
     bool bfs_bottom_up_ = false;
     bool just_gr_ = false;
     int64_t iter_ = 0, last_gr_iter_ = 0;
+    int gr_depth_ = 0;  // levels of the last exact global relabel
     int GR_FREQ_ = 128, K_ = 4, RM_FREQ_ = 64;
     int stag_ = 0, no_stag_ = 0;
     int64_t last_pos_ = INT64_MAX;
@@ -272,9 +298,19 @@ public:
     double now() const;
     hipStream_t stream() const { return stream_; }
     // Device pointers of slot i's slab (debug / parity hooks).
-    Slab* slab(int i) { return slabs_[i].get(); }
+    Slab* slab(int i) { return &slots_[i].slab; }
     int nslots() const { return nslots_; }
-    size_t slab_bytes_total() const;
+    // Slab arena: budget in bytes (0 = nslots full slabs of the largest
+    // graph).  A queued query is admitted when a slot is free and its run
+    // graph's slab fits; it waits otherwise (first come, first served).
+    int set_budget(size_t bytes);
+    void mem_stats(size_t* budget, size_t* in_use, size_t* peak, int64_t* waits) const;
+    // Serve queries arriving at offsets qs[i].t_arrive (seconds from the call,
+    // ascending): each is submitted when its time comes, the engine steps
+    // while anything is queued or live and idles until the next arrival.
+    // T0 < 0: the call's start; else an instant on the engine clock shared by
+    // engines serving side by side.
+    int serve(std::vector<Query> qs, std::vector<Result>& out, double T0 = -1.0);
     // Single-slot manual control for parity hooks: admit (gid, s, t) into
     // slot 0 without scheduling, launch hand-built plans, release.
     int dbg_begin(int gid, int s, int t);
@@ -299,8 +335,18 @@ private:
         SlotView view;
         std::shared_ptr<DeviceGraph> run_graph;  // pruned subgraph when used
         int parent_n = 0;
+        Slab slab;  // kept after the query retires, reused when the next one fits
     };
     int admit(int slot, const Query& q, bool warm = false, bool prune = true);
+    // (n, E, ntiles) of the graph q will run on: the cached pruned subgraph of
+    // its reachability class when there is one, else its full graph.
+    void run_dims(const Query& q, int* n, int64_t* E, int* ntiles) const;
+    // Give slot i a slab for (n, E, ntiles): MF_OK, kNoRoom (the arena is full
+    // for now) or an error.
+    static constexpr int kNoRoom = 1;
+    int ensure_slab(int i, int n, int64_t E, int ntiles);
+    void drop_slab(int i);
+    int make_arena();
     int prune_decide(int slot);
     int class_decide(int slot);
     int launch_plans(std::vector<Plan>& plans);
@@ -308,9 +354,14 @@ private:
     Params P_;
     int nslots_ = 0;
     hipStream_t stream_ = nullptr;
+    hipStream_t side_ = nullptr;          // single-workgroup BFS steps, overlapped with the rest
+    hipEvent_t ev_up_ = nullptr, ev_side_ = nullptr;
     std::vector<std::shared_ptr<DeviceGraph>> graphs_;
-    std::vector<std::unique_ptr<Slab>> slabs_;
     std::vector<SlotState> slots_;
+    SlabArena arena_;
+    size_t budget_ = 0;         // 0 = nslots full slabs of the largest graph
+    bool arena_stale_ = false;  // default-sized arena smaller than the graphs now need
+    int64_t admit_waits_ = 0;   // steps whose queue head waited for slab bytes
     std::deque<Query> queue_;
     // device copies of views and launch lists; pinned staging
     SlotView* d_views_ = nullptr;

@@ -215,6 +215,7 @@ int build_induced_gpu(const DeviceGraph& P, const int32_t* d_mask, int kept, hip
     (void)hipFree(vflag); (void)hipFree(vscan); (void)hipFree(kdeg); (void)hipFree(escan); (void)hipFree(e2s); (void)hipFree(tmp);
     out.g.n = k;
     out.g.E = (int)Ek;
+    out.g.max_deg = P.g.max_deg;
     out.g.ntiles = ntiles;
     out.g.R = P.g.R;
     out.g.row_ptr = out.d_row_ptr;

@@ -18,6 +18,8 @@ enum : unsigned {
     RST_RM = 1u << 3,      // fixed_edges
     RST_WORK = 1u << 4,    // work counters
     RST_ERR = 1u << 5,
+    RST_TOUCH = 1u << 6,   // new touch epoch, touched list emptied
+    RST_SPARSE = 1u << 7,  // like RST_ACTIVE, keeping the touched list for a sparse count
 };
 
 enum BfsMode : int { BFS_GR = 0, BFS_FWD = 1, BFS_UND = 2 };
@@ -62,12 +64,16 @@ void launch_bfs_small(hipStream_t st, const SlotView* slots, const int* list, in
 
 // Active list (light entries at fa[0..), heavy at fa[n-1..]) and counters.
 void launch_active(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);
-// Discharge of the active list: lanes for light rows, waves for heavy rows.
+// The same from the previous list and the vertices touched since the last
+// RST_TOUCH only (after a RST_SPARSE reset); gridx any size.
+void launch_active_sparse(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);
+// Discharge of the active list: lanes for light rows, waves for heavy rows;
+// giants != 0: rows longer than kGiantDeg over the whole grid after it.
 void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* list, int nlist,
-                           int gridx);
+                           int gridx, int giants);
 // One workgroup per tile, tile heights and pending excess staged in LDS.
 void launch_discharge_tile(hipStream_t st, const SlotView* slots, const int* list, int nlist,
-                           int gridx, int hot_only, int rounds);
+                           int gridx, int hot_only, int rounds, int giants);
 // Owner-exclusive path push from the active list.
 void launch_path_push(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
                       int max_hops);

@@ -135,6 +135,7 @@ struct GSlot {
     const GAS int32_t* mask;
     GAS Ctr* ctr;
     int32_t s, t, K, Kh, H_cut, phase;
+    int32_t tstamp;  // this cycle's touch stamp (negative)
 };
 
 template <typename T>
@@ -180,6 +181,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.Kh = v.Kh;
     S.H_cut = v.H_cut;
     S.phase = v.phase;
+    S.tstamp = -(2 * S.ctr->epoch + 1);
     return S;
 }
 
@@ -261,7 +263,15 @@ __global__ void k_reset(const SlotView* __restrict__ slots, const int* __restric
             c->nhg = 0;
             c->visited = 0;
         }
-        if (what & RST_ACTIVE) {
+        if (what & (RST_ACTIVE | RST_SPARSE)) {
+            c->prev_nlight = c->nlight;
+            c->prev_nheavy = c->nheavy;
+        }
+        if (what & (RST_ACTIVE | RST_TOUCH)) {
+            c->ntouched = 0;
+            c->epoch = c->epoch + 1;
+        }
+        if (what & (RST_ACTIVE | RST_SPARSE)) {
             c->active = 0; c->active_high = 0; c->nlight = 0; c->nheavy = 0; c->ngiant = 0;
             c->pos_excess = 0; c->neg_count = 0; c->neg_sum = 0;
             c->hmin = INT_MAX; c->hmax = -1;
@@ -273,7 +283,7 @@ __global__ void k_reset(const SlotView* __restrict__ slots, const int* __restric
                 for (int k = 0; k < 4; ++k) c->wk[f][k] = 0;
         if (what & RST_ERR) c->err = 0;
     }
-    if (what & RST_ACTIVE)
+    if (what & (RST_ACTIVE | RST_SPARSE))
         for (int i = threadIdx.x; i < S.g.ntiles; i += blockDim.x) { S.tile_cnt[i] = 0; S.tile_hot[i] = 0; }
 }
 
@@ -768,6 +778,7 @@ __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__
     int fsz = S.ctr->fsize[L & 3];
     Work wk;
     int done_levels = 0;
+    int visited = 0;
     int64_t fdeg = S.ctr->fedges[L & 3];
     while (fsz > 0 && fsz <= kSmallFrontier && fdeg <= kSmallEdges && done_levels < max_levels) {
         const GAS int32_t* in = (L & 1) ? S.fb : S.fa;
@@ -818,18 +829,19 @@ __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__
         fdeg = next_deg;
         ++L;
         ++done_levels;
-        if (threadIdx.x == 0) {
-            S.ctr->fsize[L & 3] = fsz;
-            S.ctr->fedges[L & 3] = next_deg;
-            // the next level (run here or by a multi-workgroup launch) appends
-            // into ring slot L+1, which must start empty
-            S.ctr->fsize[(L + 1) & 3] = 0;
-            S.ctr->fedges[(L + 1) & 3] = 0;
-            S.ctr->visited = S.ctr->visited + fsz;
-        }
-        __syncthreads();
+        visited += fsz;
+        __syncthreads();  // next_cnt / next_deg are reset by the next level
     }
-    if (threadIdx.x == 0) S.ctr->level = L;
+    if (threadIdx.x == 0 && done_levels > 0) {
+        // the ring as a multi-workgroup continuation expects it: level L's
+        // frontier in slot L, slot L+1 empty for the next level's appends
+        S.ctr->fsize[L & 3] = fsz;
+        S.ctr->fedges[L & 3] = fdeg;
+        S.ctr->fsize[(L + 1) & 3] = 0;
+        S.ctr->fedges[(L + 1) & 3] = 0;
+        S.ctr->visited = S.ctr->visited + visited;
+        S.ctr->level = L;
+    }
     work_flush(S, wk, red, WF_BFS);
 }
 
@@ -1180,6 +1192,93 @@ void launch_active(hipStream_t st, const SlotView* slots, const int* list, int n
     hipLaunchKernelGGL(k_active, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
 }
 
+// Sparse count of a fused cycle: the candidates are the active list being
+// replaced (fa, prev_nlight / prev_nheavy) and the vertices pushed into this
+// cycle (hl, ntouched); each is counted once (exchange on its aux stamp).  The
+// new list goes to fb (light at the front, heavy at the back) and
+// k_list_copy moves it to fa.  Same counters as k_active, except that excess
+// nobody pushed to or discharged is not re-summed (deficits are counted by the
+// full pass that follows every global relabel).
+__global__ __launch_bounds__(256) void k_active_sparse(const SlotView* __restrict__ slots,
+                                                       const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ Stage<1024> st;
+    __shared__ int64_t red64[256 / WAVE];
+    __shared__ uint64_t redu[256 / WAVE];
+    __shared__ int bmin, bmax;
+    stage_init(st);
+    if (threadIdx.x == 0) { bmin = INT_MAX; bmax = -1; }
+    __syncthreads();
+    const int n = S.g.n;
+    const int pl = S.ctr->prev_nlight, ph = S.ctr->prev_nheavy, nt = S.ctr->ntouched;
+    const int cstamp = S.tstamp - 1;  // -(2 epoch + 2)
+    const int total = pl + ph + nt;
+    int64_t pos = 0;
+    uint64_t act = 0, high = 0, nv = 0;
+    int hmin = INT_MAX, hmax = -1;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int v = i < pl ? S.fa[i] : (i < pl + ph ? S.fa[n - 1 - (i - pl)] : S.hl[i - pl - ph]);
+        if (__hip_atomic_exchange(&S.aux[v], cstamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == cstamp) continue;
+        ++nv;
+        if (v == S.s) continue;
+        const int64_t ex = ld64(&S.excess[v]);
+        if (v == S.t) S.ctr->ex_t = ex;
+        if (v == S.t && S.phase == 0) continue;
+        if (ex <= 0) continue;
+        pos += ex;
+        const int hv = S.h[v];
+        if (hv >= n) continue;
+        ++act;
+        if (hv > S.H_cut) ++high;
+        hmin = min(hmin, hv);
+        hmax = max(hmax, hv);
+        add32(&S.tile_cnt[S.g.tile_of[v]], 1);
+        const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
+        if (deg <= kLightDeg) {
+            stage_push(st, v, 0, S.fb, &S.ctr->nlight, (GAS int64_t*)nullptr);
+        } else {
+            const int k = add32(&S.ctr->nheavy, 1);
+            S.fb[n - 1 - k] = v;
+            if (deg > kGiantDeg) {
+                const int g = add32(&S.ctr->ngiant, 1);
+                S.giant[g] = v;
+                S.giant[S.gcap + g] = INT_MAX;
+            }
+        }
+    }
+    stage_flush(st, S.fb, &S.ctr->nlight, (GAS int64_t*)nullptr);
+    hmin = wave_min_i32(hmin);
+    hmax = wave_max_i32(hmax);
+    if (lane_id() == 0) { atomicMin(&bmin, hmin); atomicMax(&bmax, hmax); }
+    const int64_t bpos = block_sum<int64_t>(pos, red64);
+    const uint64_t bact = block_sum<uint64_t>(act, redu);
+    const uint64_t bhigh = block_sum<uint64_t>(high, redu);
+    const uint64_t bnv = block_sum<uint64_t>(nv, redu);
+    if (threadIdx.x == 0) {
+        if (bpos) add64(&S.ctr->pos_excess, bpos);
+        if (bact) add32(&S.ctr->active, (int32_t)bact);
+        if (bhigh) add32(&S.ctr->active_high, (int32_t)bhigh);
+        if (bnv) addu64(&S.ctr->wk[WF_ACTIVE][1], bnv);
+        if (bmax >= 0) { min32(&S.ctr->hmin, bmin); max32(&S.ctr->hmax, bmax); }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_list_copy(const SlotView* __restrict__ slots,
+                                                   const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    const int n = S.g.n;
+    const int nl = S.ctr->nlight, nh = S.ctr->nheavy;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nl + nh; i += gridDim.x * blockDim.x) {
+        const int j = i < nl ? i : n - 1 - (i - nl);
+        S.fa[j] = S.fb[j];
+    }
+}
+
+void launch_active_sparse(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx) {
+    hipLaunchKernelGGL(k_active_sparse, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_list_copy, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+}
+
 // ------------------------------------------------------------- discharge (a16)
 // One discharge cycle (the reference's cycle is BC:467-511): find the lowest
 // neighbour over residual entries; if v is not above it, relabel v to lowest
@@ -1198,9 +1297,19 @@ __device__ __forceinline__ int nbr_h(const GSlot& S, const TileCtx& tc, int w) {
     return (w >= tc.a && w < tc.b) ? tc.lh[w - tc.a] : S.h[w];
 }
 
+// w received excess this cycle: listed once per touch epoch for a sparse count.
+__device__ __forceinline__ void touch(const GSlot& S, int w) {
+    if (__hip_atomic_exchange(&S.aux[w], S.tstamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != S.tstamp)
+        S.hl[add32(&S.ctr->ntouched, 1)] = w;
+}
+
 __device__ __forceinline__ void push_into(const GSlot& S, const TileCtx& tc, int w, int64_t d) {
-    if (w >= tc.a && w < tc.b) add64_lds(&tc.ldx[w - tc.a], d);
-    else add64(&S.excess[w], d);
+    if (w >= tc.a && w < tc.b) {
+        add64_lds(&tc.ldx[w - tc.a], d);  // touched when the tile flushes
+    } else {
+        add64(&S.excess[w], d);
+        touch(S, w);
+    }
 }
 
 // Owner view of v's excess.
@@ -1370,14 +1479,18 @@ __global__ __launch_bounds__(256) void k_discharge_list(const SlotView* __restri
     if ((int)blockIdx.x >= lb + hb) return;
     TileCtx tc;
     Work wk;
-    if ((int)blockIdx.x < lb) {
-        const int i = blockIdx.x * 256 + threadIdx.x;
-        if (i < nl) lane_discharge(S, tc, S.fa[i], S.K, wk);
-    } else {
-        const int k = ((int)blockIdx.x - lb) * 4 + (int)(threadIdx.x / WAVE);
-        if (k < nh) {
-            const int v = S.fa[S.g.n - 1 - k];
-            if (S.g.row_ptr[v + 1] - S.g.row_ptr[v] <= kGiantDeg) wave_discharge(S, tc, v, S.Kh, wk);
+    // work blocks [0, lb) light, [lb, lb + hb) heavy, dealt grid-stride so any
+    // grid covers the whole list (fused cycles launch before the count is known)
+    for (int b = blockIdx.x; b < lb + hb; b += gridDim.x) {
+        if (b < lb) {
+            const int i = b * 256 + threadIdx.x;
+            if (i < nl) lane_discharge(S, tc, S.fa[i], S.K, wk);
+        } else {
+            const int k = (b - lb) * 4 + (int)(threadIdx.x / WAVE);
+            if (k < nh) {
+                const int v = S.fa[S.g.n - 1 - k];
+                if (S.g.row_ptr[v + 1] - S.g.row_ptr[v] <= kGiantDeg) wave_discharge(S, tc, v, S.Kh, wk);
+            }
         }
     }
     work_flush(S, wk, red, WF_DISCHARGE);
@@ -1566,6 +1679,7 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
             add64(&S.res[e], -d);
             add64(&S.res[S.g.rev[e]], d);
             add64(&S.excess[S.g.col[e]], d);
+            touch(S, S.g.col[e]);
             rem -= d;
             wk.pushes++;
         }
@@ -1580,9 +1694,9 @@ void launch_giant(hipStream_t st, const SlotView* slots, const int* list, int nl
 }
 
 void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* list, int nlist,
-                           int gridx) {
+                           int gridx, int giants) {
     hipLaunchKernelGGL(k_discharge_list, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
-    launch_giant(st, slots, list, nlist);
+    if (giants) launch_giant(st, slots, list, nlist);
 }
 
 // Region (tile) discharge (BC:432-515, one block per region): the tile's
@@ -1596,6 +1710,9 @@ __global__ __launch_bounds__(512) void k_discharge_tile(const SlotView* __restri
     const int tile = blockIdx.x;
     if (tile >= S.g.ntiles) return;
     if (hot_only && S.tile_hot[tile] == 0) return;
+    // no active vertex at the last count: nothing to do (vertices pushed into
+    // since are counted next cycle)
+    if (S.tile_cnt[tile] == 0) return;
     __shared__ int32_t lh[kTileMax];
     __shared__ int64_t ldx[kTileMax];
     __shared__ int32_t alist[kTileMax];
@@ -1635,16 +1752,19 @@ __global__ __launch_bounds__(512) void k_discharge_tile(const SlotView* __restri
     __syncthreads();
     for (int i = threadIdx.x; i < size; i += blockDim.x) {
         S.h[a + i] = lh[i];
-        if (ldx[i] != 0) add64(&S.excess[a + i], ldx[i]);
+        if (ldx[i] != 0) {
+            add64(&S.excess[a + i], ldx[i]);
+            if (ldx[i] > 0) touch(S, a + i);
+        }
     }
     work_flush(S, wk, red, WF_DISCHARGE);
 }
 
 void launch_discharge_tile(hipStream_t st, const SlotView* slots, const int* list, int nlist,
-                           int gridx, int hot_only, int rounds) {
+                           int gridx, int hot_only, int rounds, int giants) {
     hipLaunchKernelGGL(k_discharge_tile, dim3(gridx, nlist), dim3(512), 0, st, slots, list, hot_only,
                        rounds);
-    if (!hot_only) launch_giant(st, slots, list, nlist);
+    if (!hot_only && giants) launch_giant(st, slots, list, nlist);
 }
 
 // ------------------------------------------------------------ path push (a17)

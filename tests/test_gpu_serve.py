@@ -1,0 +1,116 @@
+"""Serving loop on the GPU: several resident graphs, queries arriving over
+time, admission by slab bytes, concurrent execution (continuous batching).
+
+Each served flow must equal the same query solved alone and the oracle's
+(the max-flow value is unique, so "the same alone and in the mix" is exact
+equality).  Admission must never put more slab bytes in use than the budget.
+"""
+import numpy as np
+import pytest
+
+import mscresearch_amd as M
+import oracle as O
+from gpu_util import Case, log
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def graphs():
+    log("serving graphs: grid 40x40, rmat 12, uniform 3000/30000")
+    return [
+        Case(M.Edges.grid(40, 40, 1000, 6001), R=16, tile_max=256),
+        Case(M.Edges.rmat(12, 10**6, 6002), R=32),
+        Case(M.Edges.uniform(3000, 30000, 100, 6003), R=16),
+    ]
+
+
+def stream(graphs, nq, qps, seed):
+    rng = np.random.default_rng(seed)
+    gsel = rng.choice(3, nq, p=[0.3, 0.4, 0.3])
+    arrive = np.cumsum(rng.exponential(1.0 / qps, nq))
+    qs, qt = [], []
+    pools = [M.grid_queries(40, 40, nq, seed + 1), graphs[1].edges.hub_queries(nq, seed + 2),
+             M.uniform_queries(3000, nq, seed + 3)]
+    for i, g in enumerate(gsel):
+        s, t = graphs[g].q(int(pools[g][0][i]), int(pools[g][1][i]))
+        qs.append(s)
+        qt.append(t)
+    return gsel.astype(np.int32), np.array(qs, np.int32), np.array(qt, np.int32), arrive
+
+
+def slab_bytes(case):
+    d = case.dg.info()
+    n, E, T = d["n"], d["E"], d["ntiles"]
+    return 8 * E + 32 * n + 8 * T + 2 * 1024 + (E // 2049 + 1) * 20 + 14 * 256
+
+
+@pytest.mark.parametrize("budget_slabs", [0, 2.5])
+def test_serve_mixed_graphs_exact_and_within_budget(graphs, budget_slabs):
+    eng = M.Engine(8)
+    gids = [eng.add_graph(c.dg) for c in graphs]
+    big = max(slab_bytes(c) for c in graphs)
+    if budget_slabs:
+        eng.set_budget(int(budget_slabs * big))
+    g, s, t, arrive = stream(graphs, 60, 400.0, 6100)
+    res = eng.serve(np.array(gids, np.int32)[g], s, t, arrive)
+    assert len(res) == 60
+    by = {r.tag: r for r in res}
+    for i in range(60):
+        r = by[i]
+        want = graphs[g[i]].oracle_flow(int(s[i]), int(t[i]))
+        assert r.status == 0 and r.cert_ok == 1, (i, r.status)
+        assert r.flow == want, (i, r.flow, want)
+        assert r.t_start >= r.t_arrive - 1e-9 and r.t_end >= r.t_start
+    st = eng.mem_stats()
+    log(f"budget {budget_slabs} slabs: {st}")
+    assert st["peak"] <= st["budget"]
+    if budget_slabs:
+        assert st["budget"] <= int(budget_slabs * big) + 256
+    # alone: the same queries one at a time on a fresh engine give the same flows
+    alone = M.Engine(1)
+    agids = [alone.add_graph(c.dg) for c in graphs]
+    for i in range(0, 60, 7):
+        assert alone.solve(agids[g[i]], int(s[i]), int(t[i])).flow == by[i].flow
+
+
+def test_serve_reports_queries_that_can_never_fit(graphs):
+    eng = M.Engine(4)
+    gids = [eng.add_graph(c.dg) for c in graphs]
+    room = max(slab_bytes(graphs[0]), slab_bytes(graphs[2]))
+    need_rmat = slab_bytes(graphs[1])
+    assert need_rmat > 1.2 * room
+    eng.set_budget(int(room * 1.2))  # room for a grid or a uniform slab, not for the R-MAT one
+    hs, ht = graphs[1].edges.hub_queries(1, 7)
+    pairs = [graphs[0].q(0, 39), graphs[1].q(int(hs[0]), int(ht[0])), graphs[2].q(5, 9), graphs[0].q(40, 79)]
+    g = np.array([0, 1, 2, 0], np.int32)
+    s = np.array([p[0] for p in pairs], np.int32)
+    t = np.array([p[1] for p in pairs], np.int32)
+    res = {r.tag: r for r in eng.serve(np.array(gids, np.int32)[g], s, t, np.zeros(4))}
+    assert len(res) == 4
+    assert res[1].status == M.MF_ENOMEM
+    for i in (0, 2, 3):
+        assert res[i].status == 0
+        assert res[i].flow == graphs[g[i]].oracle_flow(int(s[i]), int(t[i]))
+
+
+def test_serve_lanes_one_engine_per_graph(graphs):
+    """One engine lane per graph (own thread and stream): the same flows as
+    the oracle, every lane's slab use within its budget."""
+    engines = []
+    for c in graphs:
+        eng = M.Engine(4, M.default_params(fuse=16))
+        eng.add_graph(c.dg)
+        eng.set_budget(int(2.5 * slab_bytes(c)))
+        engines.append(eng)
+    g, s, t, arrive = stream(graphs, 60, 300.0, 6200)
+    res = M.serve_lanes(engines, g, np.zeros(len(g)), s, t, arrive)
+    assert len(res) == 60
+    assert all(res[k].t_end <= res[k + 1].t_end for k in range(59))
+    for r in res:
+        i = r.tag
+        assert r.status == 0 and r.cert_ok == 1
+        assert r.flow == graphs[g[i]].oracle_flow(int(s[i]), int(t[i])), i
+    for e in engines:
+        st = e.mem_stats()
+        assert st["peak"] <= st["budget"]

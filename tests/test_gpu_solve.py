@@ -49,11 +49,12 @@ def test_c1_flows_and_cpu_certificate(c1):
         assert np.array_equal(h < c1.n, hb < c1.n)
 
 
-@pytest.mark.parametrize("rows,cols", [(64, 64), (128, 96)])
-def test_grid_flows(rows, cols):
-    log(f"grid {rows}x{cols}")
+@pytest.mark.parametrize("rows,cols,fuse", [(64, 64, 8), (128, 96, 8), (128, 96, 1)])
+def test_grid_flows(rows, cols, fuse):
+    """Grids are deep (BFS depth ~ rows + cols): fused discharge cycles run."""
+    log(f"grid {rows}x{cols} fuse {fuse}")
     case = Case(M.Edges.grid(rows, cols, 1000, 2001), R=64, tile_max=512)
-    eng, gid = engine_with(case)
+    eng, gid = engine_with(case, fuse=fuse)
     qs, qt = M.grid_queries(rows, cols, 8, 2002)
     for a, b in zip(qs, qt):
         s, t = case.q(a, b)
