@@ -32,7 +32,7 @@ METRIC = "max-flow queries/sec + p50/p99 latency, served/batched vs plain, bit-e
 HBM_PEAK = 8.0e12
 # algorithmic bytes per unit: edge scan, vertex scan, push, relabel/discovery
 UNIT_BYTES = np.array([16.0, 16.0, 64.0, 8.0])
-FAMILIES = ["init", "bfs", "active", "discharge", "rm", "cert"]
+FAMILIES = ["init", "bfs", "active", "discharge", "rm", "cert", "bfs_bu"]
 
 
 def log(msg):
@@ -199,7 +199,21 @@ def main():
     ap.add_argument("--serve-max", type=int, default=1200)
     ap.add_argument("--serve-slabs", type=float, default=24.0,
                     help="slab arena budget in full R-MAT 22 slabs")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run 3 device-to-device copies of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        a = torch.empty(args.calibrate_copy, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy, "copies": 3}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X and this path shards by query "
               "(see DESIGN.md); run with --gpus 1.", file=sys.stderr)
@@ -237,15 +251,20 @@ def main():
     st = {k: float(np.mean([getattr(r, k) for r in res])) for k in
           ("iters", "steps", "grs", "bfs_levels", "drains", "pruned", "n_run", "E_run", "kept")}
     log("per-query means: " + ", ".join(f"{k}={v:.1f}" for k, v in st.items()))
-    log("family ms: " + ", ".join(f"{FAMILIES[f]}={ms[f]:.1f}/{launches[f]}" for f in range(6)))
+    log("family ms: " + ", ".join(f"{FAMILIES[f]}={ms[f]:.1f}/{launches[f]}" for f in range(len(FAMILIES))))
 
     # roofline of the dominant kernel family (HIP events on the engine stream)
     fam = int(np.argmax(ms))
     algo_bytes = float((work[fam].astype(np.float64) * UNIT_BYTES).sum())
     achieved = algo_bytes / (ms[fam] * 1e-3) if ms[fam] > 0 else 0.0
+    alg_per_launch = algo_bytes / max(1, int(launches[fam]))
+    from mscresearch_amd.profiling import traffic_for
+
+    traffic = traffic_for(name, FAMILIES[fam])
+    traffic = round(traffic, 1) if traffic is not None else None
     fam_table = {FAMILIES[f]: {"ms": round(float(ms[f]), 3), "launches": int(launches[f]),
                                "alg_bytes": float((work[f].astype(np.float64) * UNIT_BYTES).sum())}
-                 for f in range(6)}
+                 for f in range(len(FAMILIES))}
 
     # ---- plain: one query at a time on one slot
     plain_lat = []
@@ -266,7 +285,7 @@ def main():
         plain_qps = k / pwall
         del peng
         log(f"plain: {k} queries in {pwall:.3f}s = {plain_qps:.1f} q/s, p50 {pct(plain_lat, .5) * 1e3:.2f} ms")
-        log("plain family ms: " + ", ".join(f"{FAMILIES[f]}={pms[f]:.1f}/{pla[f]}" for f in range(6)))
+        log("plain family ms: " + ", ".join(f"{FAMILIES[f]}={pms[f]:.1f}/{pla[f]}" for f in range(len(FAMILIES))))
         for i in range(k):
             assert plain_flows[i] == flows[args.batch * args.warmup + i], "plain and batched flows differ"
     else:
@@ -327,7 +346,10 @@ def main():
         "kernel_families": fam_table,
         "roofline": {"bound": "hbm", "kernel_family": FAMILIES[fam], "achieved": round(achieved / 1e9, 3),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5),
-                     "traffic": None},
+                     "traffic": traffic, "alg_bytes_per_launch": round(alg_per_launch, 1),
+                     "ms_per_launch": round(float(ms[fam]) / max(1, int(launches[fam])), 4),
+                     "traffic_source": f"profiles/{name}_hbm_traffic.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, "
+                                       "copy-calibrated)" if traffic is not None else None},
         "cpu_baseline": cpu,
         "serving": serving,
     }

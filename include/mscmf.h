@@ -34,6 +34,9 @@ extern "C" {
 #define MF_ENODEV (-7)
 #define MF_EBUSY (-8)
 
+/* kernel families of the work counters and the profile */
+#define MF_FAMILIES 7
+
 typedef struct mf_hgraph mf_hgraph; /* host CSR + partition */
 typedef struct mf_edges mf_edges;   /* generated edge list */
 typedef struct mf_graph mf_graph;   /* device-resident region-ordered graph */
@@ -41,6 +44,9 @@ typedef struct mf_engine mf_engine; /* slots + scheduler + stream */
 
 const char* mf_last_error(void);
 int mf_version(void);
+/* sizeof(mf_params) and sizeof(mf_result) as this library was built: a binding
+ * checks its own layouts against them. */
+void mf_struct_sizes(int64_t* params_bytes, int64_t* result_bytes);
 
 /* ----------------------------------------------------------- host graph */
 
@@ -165,9 +171,11 @@ typedef struct mf_result {
     int64_t iters, steps, grs, bfs_levels, drains, rm_fixed;
     uint64_t edge_scans, vertex_scans, pushes, relabels;
     double t_arrive, t_start, t_end; /* seconds on the engine clock */
-    /* work per kernel family (init, bfs, active, discharge, rm, cert) x
-     * (edge scans, vertex scans, pushes, relabels + BFS discoveries) */
-    uint64_t work[6][4];
+    /* work per kernel family (init, bfs, active, discharge, rm, cert,
+     * bfs_bu) x (edge scans, vertex scans, pushes, relabels + BFS
+     * discoveries); bfs = seeding, single-workgroup and top-down levels,
+     * bfs_bu = bottom-up levels */
+    uint64_t work[MF_FAMILIES][4];
 } mf_result;
 
 /* Plain single query: solve (s,t) (ids of graph gid) to completion.
@@ -210,10 +218,10 @@ int mf_serve_lanes(int nengines, mf_engine* const* engines, int nq, const int32_
                    const int32_t* s, const int32_t* t, const double* arrive, mf_result* out, int* nout);
 double mf_engine_now(const mf_engine* e);
 int mf_engine_cache_stats(const mf_engine* e, int64_t* hits, int64_t* misses);
-/* Per kernel family (init, bfs, active, discharge, rm, cert) timing with HIP
+/* Per kernel family (init, bfs, active, discharge, rm, cert, bfs_bu) timing with HIP
  * events on the engine's stream while on != 0, plus the algorithmic work of
  * the queries that finished meanwhile.  Turning it on clears the totals.
- * ms[6], launches[6], work[6*4]. */
+ * ms[MF_FAMILIES], launches[MF_FAMILIES], work[MF_FAMILIES*4]. */
 int mf_engine_profile(mf_engine* e, int on);
 int mf_engine_profile_read(const mf_engine* e, double* ms, int64_t* launches, uint64_t* work);
 

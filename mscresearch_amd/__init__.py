@@ -55,12 +55,12 @@ class Result(C.Structure):
         ("edge_scans", C.c_uint64), ("vertex_scans", C.c_uint64), ("pushes", C.c_uint64),
         ("relabels", C.c_uint64),
         ("t_arrive", C.c_double), ("t_start", C.c_double), ("t_end", C.c_double),
-        ("work", (C.c_uint64 * 4) * 6),
+        ("work", (C.c_uint64 * 4) * 7),
     ]
 
     def as_dict(self) -> dict:
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "work"}
-        d["work"] = [[int(self.work[f][k]) for k in range(4)] for f in range(6)]
+        d["work"] = [[int(self.work[f][k]) for k in range(4)] for f in range(7)]
         return d
 
 
@@ -72,6 +72,7 @@ _I64P = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 SIGNATURES = {
     "mf_last_error": (C.c_char_p, []),
     "mf_version": (C.c_int, []),
+    "mf_struct_sizes": (None, [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mf_hgraph_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
     "mf_hgraph_from_edges": (C.c_int, [C.c_int, C.c_int64, _P, _P, _P, C.POINTER(_P)]),
     "mf_hgraph_from_edgelist": (C.c_int, [_P, C.POINTER(_P)]),
@@ -439,11 +440,11 @@ class Engine:
         _check(lib().mf_engine_profile(self._h, 1 if on else 0))
 
     def profile_read(self):
-        ms = np.zeros(6, np.float64)
-        la = np.zeros(6, np.int64)
-        wk = np.zeros(24, np.uint64)
+        ms = np.zeros(7, np.float64)
+        la = np.zeros(7, np.int64)
+        wk = np.zeros(28, np.uint64)
         _check(lib().mf_engine_profile_read(self._h, _ptr(ms), _ptr(la), _ptr(wk)))
-        return ms, la, wk.reshape(6, 4)
+        return ms, la, wk.reshape(7, 4)
 
     def pin(self, slot, gid, s, t, tag):
         _check(lib().mf_engine_pin(self._h, slot, gid, int(s), int(t), int(tag)))

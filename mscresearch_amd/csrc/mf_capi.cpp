@@ -124,6 +124,11 @@ const char* mf_last_error(void) {
 
 int mf_version(void) { return 1; }
 
+void mf_struct_sizes(int64_t* params_bytes, int64_t* result_bytes) {
+    if (params_bytes) *params_bytes = (int64_t)sizeof(mf_params);
+    if (result_bytes) *result_bytes = (int64_t)sizeof(mf_result);
+}
+
 // ------------------------------------------------------------- host graph
 
 int mf_hgraph_load(const char* graph_path, mf_hgraph** out) {

@@ -64,12 +64,16 @@ struct Ctr {
     // algorithmic work counters (roofline accounting), per kernel family:
     // wk[family][0] edge scans, [1] vertex scans, [2] pushes,
     // [3] relabels + BFS discoveries
-    uint64_t wk[6][4];
+    uint64_t wk[7][4];
 };
 
 // Kernel families of the work counters and of the engine's timing.
-enum WorkFamily : int { WF_INIT = 0, WF_BFS = 1, WF_ACTIVE = 2, WF_DISCHARGE = 3, WF_RM = 4, WF_CERT = 5 };
-constexpr int kFamilies = 6;
+// WF_BFS: seeding, single-workgroup and top-down levels, local relabel;
+// WF_BFS_BU: bottom-up levels.
+enum WorkFamily : int {
+    WF_INIT = 0, WF_BFS = 1, WF_ACTIVE = 2, WF_DISCHARGE = 3, WF_RM = 4, WF_CERT = 5, WF_BFS_BU = 6
+};
+constexpr int kFamilies = 7;
 
 // Everything a kernel needs to work on one query.
 struct SlotView {

@@ -1183,7 +1183,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
             te(k);
         }
         if (g_bfs[BK_BU][m].cnt) {
-            const int k = tb(WF_BFS);
+            const int k = tb(WF_BFS_BU);
             launch_bfs_bu(st, d_views_, L(g_bfs[BK_BU][m]), g_bfs[BK_BU][m].cnt, maxof(g_bfs[BK_BU][m], [&](int i) { return plans[i].bfs_grid; }), m);
             te(k);
         }

@@ -624,9 +624,11 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
                 med[atomicAdd(&nmed, 1)] = v;
             } else {
                 int e = rb;
+                // the coalesced residual test first: most entries fail it
+                // (reverse entries without flow), saving the level gather
                 for (; e < re; ++e) {
-                    if (lvl[S.g.col[e]] != L) continue;
-                    if (bfs_bu_edge_ok<MODE>(S, e)) break;
+                    if (!bfs_bu_edge_ok<MODE>(S, e)) continue;
+                    if (lvl[S.g.col[e]] == L) break;
                 }
                 wk.edges += (uint64_t)(min(e + 1, re) - rb);
                 if (e < re) {
@@ -646,7 +648,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
             for (int e0 = rb; e0 < re; e0 += WAVE) {
                 const int e = e0 + lane;
                 bool ok = false;
-                if (e < re) ok = (lvl[S.g.col[e]] == L) && bfs_bu_edge_ok<MODE>(S, e);
+                if (e < re) ok = bfs_bu_edge_ok<MODE>(S, e) && (lvl[S.g.col[e]] == L);
                 scanned += min(WAVE, re - e0);
                 if (__ballot(ok)) { hit = true; break; }
             }
@@ -664,7 +666,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
     stage_flush(st, out, ocnt, odeg);
     const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
     if (threadIdx.x == 0 && nv) add32(&S.ctr->visited, (int32_t)nv);
-    work_flush(S, wk, red, WF_BFS);
+    work_flush(S, wk, red, WF_BFS_BU);
 }
 
 // Giant unvisited rows of a bottom-up level: a workgroup scans the row in
@@ -693,7 +695,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu_heavy(const SlotView* __restrict
         int base = rb;
         for (; base < re; base += blockDim.x) {
             const int e = base + (int)threadIdx.x;
-            if (e < re && lvl[S.g.col[e]] == L && bfs_bu_edge_ok<MODE>(S, e)) found = 1;
+            if (e < re && bfs_bu_edge_ok<MODE>(S, e) && lvl[S.g.col[e]] == L) found = 1;
             __syncthreads();
             const int f = found;
             __syncthreads();
@@ -712,7 +714,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu_heavy(const SlotView* __restrict
     stage_flush(st, out, ocnt, odeg);
     const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
     if (threadIdx.x == 0 && nv) add32(&S.ctr->visited, (int32_t)nv);
-    work_flush(S, wk, red, WF_BFS);
+    work_flush(S, wk, red, WF_BFS_BU);
 }
 
 // Level bookkeeping after a multi-workgroup level: level += 1.
@@ -2289,26 +2291,29 @@ void launch_cap_scatter(hipStream_t st, const DGraph& g, int64_t* cap, const int
 }
 
 // Certificate constants of a graph: capsum[v] and cert_fp (see DGraph).
+// One wave per row (rows of any length stay balanced), rows grid-stride.
 __global__ __launch_bounds__(256) void k_cert_consts(DGraph g, int64_t* __restrict__ capsum,
                                                      uint64_t* __restrict__ fp_out) {
     __shared__ uint64_t red[256 / WAVE];
     uint64_t fp = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int v = (int)i;
+    const int lane = lane_id();
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / WAVE);
+    for (int64_t v = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; v < g.n; v += waves) {
         int64_t cs = 0;
-        for (int e = g.row_ptr[v]; e < g.row_ptr[v + 1]; ++e) {
+        for (int e = g.row_ptr[v] + lane; e < g.row_ptr[v + 1]; e += WAVE) {
             const int64_t c = g.cap[e];
             cs += c;
             fp += pair_weight((uint64_t)min(e, g.rev[e]), g.cert_seed) * (uint64_t)c;
         }
-        capsum[v] = cs;
+        cs = wave_sum_i64(cs);
+        if (lane == 0) capsum[v] = cs;
     }
     const uint64_t b = block_sum<uint64_t>(fp, red);
     if (threadIdx.x == 0 && b) addu64(fp_out, b);
 }
 
 void launch_cert_consts(hipStream_t st, const DGraph& g, int64_t* capsum, uint64_t* fp_out) {
-    const int gx = std::max(1, std::min(8192, (g.n + 255) / 256));
+    const int gx = std::max(1, std::min(16384, (g.n + 3) / 4));
     hipLaunchKernelGGL(k_cert_consts, dim3(gx), dim3(256), 0, st, g, capsum, fp_out);
 }
 

@@ -1,5 +1,6 @@
 """The C-ABI library loads and exports every symbol include/mscmf.h declares;
 error paths that need no GPU behave (CPU only, no compute calls)."""
+import ctypes as C
 import os
 import re
 import subprocess
@@ -80,3 +81,9 @@ def test_cli_usage_and_errors(tmp_path):
     r = subprocess.run([cli, str(tmp_path / "nope"), str(tmp_path / "out")], capture_output=True, text=True)
     assert r.returncode == 1
     assert (tmp_path / "out").is_dir()  # output dir created, as the reference does
+
+
+def test_struct_layouts_match_the_library():
+    p, r = C.c_int64(0), C.c_int64(0)
+    M.lib().mf_struct_sizes(C.byref(p), C.byref(r))
+    assert (p.value, r.value) == (C.sizeof(M.Params), C.sizeof(M.Result))
