@@ -147,6 +147,7 @@ typedef struct mf_params {
     int local_relabel;   /* 1 = BFS relabel of active tiles between global relabels */
     int local_band;      /* ... when the active height band is at most this wide */
     int fuse;            /* discharge + count cycles per host round trip on deep graphs (1 = off) */
+    int graphs;          /* 1 = replay repeated fused cycles as captured HIP graphs */
 } mf_params;
 
 void mf_params_default(mf_params* p);
@@ -218,12 +219,34 @@ int mf_serve_lanes(int nengines, mf_engine* const* engines, int nq, const int32_
                    const int32_t* s, const int32_t* t, const double* arrive, mf_result* out, int* nout);
 double mf_engine_now(const mf_engine* e);
 int mf_engine_cache_stats(const mf_engine* e, int64_t* hits, int64_t* misses);
+/* Fused cycles replayed from captured HIP graphs, and graphs captured. */
+int mf_engine_graph_stats(const mf_engine* e, int64_t* captures, int64_t* replays);
 /* Per kernel family (init, bfs, active, discharge, rm, cert, bfs_bu) timing with HIP
  * events on the engine's stream while on != 0, plus the algorithmic work of
  * the queries that finished meanwhile.  Turning it on clears the totals.
  * ms[MF_FAMILIES], launches[MF_FAMILIES], work[MF_FAMILIES*4]. */
 int mf_engine_profile(mf_engine* e, int on);
 int mf_engine_profile_read(const mf_engine* e, double* ms, int64_t* launches, uint64_t* work);
+
+/* ------------------------------------------------------------- autotuner
+ * The reference uses R = 128, K = max(4, E/V) and GR_FREQ0 = 128 for every
+ * graph (BC:2583, BC:1465-1466).  mf_autotune times the plain solver on
+ * sample queries over R in {128, 256, 512, 1024}, tile size in {1024, 2048,
+ * 4096} (vertices one discharge workgroup stages in LDS), K in {auto, 1, 2,
+ * 4, 8} and GR_FREQ in {64, 128, 256, 512}, one coordinate-descent pass from
+ * (R0, tile0, base->K, base->gr_freq); every configuration must reproduce the
+ * first one's flows.  h: the host graph before partitioning; s, t: sample
+ * queries in its ids.  cache_path (or NULL): per graph class
+ * ("n<log2 n>-d<log2 avg degree>-s<log2 skew>-k<log2 BFS depth>") results
+ * are read from and appended to this text file. */
+typedef struct mf_tune_result {
+    int R, tile_max, K, gr_freq;
+    double ms_per_query, default_ms_per_query;
+    int evaluations; /* 0 = found in the cache */
+    char graph_class[64];
+} mf_tune_result;
+int mf_autotune(const mf_hgraph* h, int nq, const int32_t* s, const int32_t* t, const mf_params* base, int R0,
+                int tile0, const char* cache_path, mf_tune_result* out);
 
 /* ------------------------------------------------------- dynamic mode
  * Pinned slots keep their residual across capacity updates (the reference's

@@ -42,8 +42,14 @@ class Params(C.Structure):
         ("rm_tau", C.c_double), ("tile_rounds", C.c_int), ("tile_frac", C.c_double), ("cert", C.c_int),
         ("iter_max", C.c_longlong), ("bu_alpha", C.c_double), ("bu_beta", C.c_double),
         ("prune", C.c_int), ("prune_ratio", C.c_double), ("gr_work", C.c_double), ("K_heavy", C.c_int),
-        ("local_relabel", C.c_int), ("local_band", C.c_int), ("fuse", C.c_int),
+        ("local_relabel", C.c_int), ("local_band", C.c_int), ("fuse", C.c_int), ("graphs", C.c_int),
     ]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("R", C.c_int), ("tile_max", C.c_int), ("K", C.c_int), ("gr_freq", C.c_int),
+                ("ms_per_query", C.c_double), ("default_ms_per_query", C.c_double), ("evaluations", C.c_int),
+                ("graph_class", C.c_char * 64)]
 
 
 class Result(C.Structure):
@@ -115,6 +121,8 @@ SIGNATURES = {
     "mf_engine_now": (C.c_double, [_P]),
     "mf_engine_cache_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mf_engine_profile": (C.c_int, [_P, C.c_int]),
+    "mf_engine_graph_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "mf_autotune": (C.c_int, [_P, C.c_int, _P, _P, C.POINTER(Params), C.c_int, C.c_int, C.c_char_p, C.POINTER(TuneResult)]),
     "mf_engine_profile_read": (C.c_int, [_P, _P, _P, _P]),
     "mf_engine_pin": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64]),
     "mf_engine_solve_pinned": (C.c_int, [_P, C.c_int, C.POINTER(Result), C.c_int, C.POINTER(C.c_int)]),
@@ -218,6 +226,21 @@ class Edges:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.mf_edges_free(self._h)
             self._h = None
+
+
+def autotune(hg_unpartitioned, s, t, params=None, R=128, tile=4096, cache=None) -> dict:
+    """Pick R, tile size, K and GR_FREQ for this graph's class by timing the
+    plain solver on sample queries (ids of the unpartitioned graph); every
+    configuration must give the same flows.  Returns the chosen values."""
+    s = np.ascontiguousarray(s, np.int32)
+    t = np.ascontiguousarray(t, np.int32)
+    p = params if params is not None else default_params()
+    out = TuneResult()
+    _check(lib().mf_autotune(hg_unpartitioned._h, len(s), _ptr(s), _ptr(t), C.byref(p), int(R), int(tile),
+                             cache.encode() if cache else None, C.byref(out)))
+    return {"R": out.R, "tile_max": out.tile_max, "K": out.K, "gr_freq": out.gr_freq,
+            "ms_per_query": out.ms_per_query, "default_ms_per_query": out.default_ms_per_query,
+            "evaluations": out.evaluations, "graph_class": out.graph_class.decode()}
 
 
 def serve_lanes(engines, lane, gids, s, t, arrive):
@@ -400,6 +423,12 @@ class Engine:
         k = C.c_int(0)
         _check(lib().mf_engine_run(self._h, max_steps, out, cap, C.byref(k)))
         return [out[i] for i in range(k.value)]
+
+    def graph_stats(self):
+        """(graphs captured, fused steps replayed from a captured graph)."""
+        a, b = C.c_int64(0), C.c_int64(0)
+        _check(lib().mf_engine_graph_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def set_budget(self, nbytes: int):
         """Slab arena size in bytes (0 = nslots full slabs of the largest graph)."""

@@ -13,6 +13,7 @@
 #include "host_graph.h"
 #include "mf_engine.h"
 #include "mf_kernels.h"
+#include "mf_tune.h"
 
 using namespace mf;
 
@@ -104,6 +105,7 @@ Params from_c(const mf_params* p) {
     P.local_relabel = p->local_relabel;
     P.local_band = p->local_band;
     P.fuse = p->fuse;
+    P.graphs = p->graphs;
     return P;
 }
 
@@ -417,6 +419,7 @@ void mf_params_default(mf_params* p) {
     p->local_relabel = P.local_relabel;
     p->local_band = P.local_band;
     p->fuse = P.fuse;
+    p->graphs = P.graphs;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {
@@ -599,6 +602,32 @@ int mf_engine_cache_stats(const mf_engine* e, int64_t* hits, int64_t* misses) {
     if (!e) return fail(MF_EINVAL, "null engine");
     if (hits) *hits = e->eng.cache_hits;
     if (misses) *misses = e->eng.cache_misses;
+    return MF_OK;
+}
+
+int mf_autotune(const mf_hgraph* h, int nq, const int32_t* s, const int32_t* t, const mf_params* base, int R0,
+                int tile0, const char* cache_path, mf_tune_result* out) {
+    if (!h || nq < 1 || !s || !t || !out) return fail(MF_EINVAL, "bad argument");
+    if (R0 < 1 || tile0 < 1 || tile0 > kTileMax) return fail(MF_EINVAL, "bad R0 or tile0");
+    std::vector<int32_t> vs(s, s + nq), vt(t, t + nq);
+    TuneResult r;
+    const int rc = autotune(h->g, vs, vt, from_c(base), R0, tile0, cache_path ? cache_path : "", r);
+    if (rc != MF_OK) return rc;
+    out->R = r.R;
+    out->tile_max = r.tile_max;
+    out->K = r.K;
+    out->gr_freq = r.gr_freq;
+    out->ms_per_query = r.ms;
+    out->default_ms_per_query = r.ms_default;
+    out->evaluations = r.evaluations;
+    std::snprintf(out->graph_class, sizeof(out->graph_class), "%s", r.cls.c_str());
+    return MF_OK;
+}
+
+int mf_engine_graph_stats(const mf_engine* e, int64_t* captures, int64_t* replays) {
+    if (!e) return fail(MF_EINVAL, "null engine");
+    if (captures) *captures = e->eng.graph_captures;
+    if (replays) *replays = e->eng.graph_replays;
     return MF_OK;
 }
 

@@ -584,6 +584,13 @@ Engine::~Engine() {
     (void)hipFree(d_views_);
     (void)hipFree(d_hash_);
     (void)hipFree(d_ctrs_);
+    for (auto& kv : graph_cache_) {
+        (void)hipGraphExecDestroy(kv.second.x);
+        (void)hipGraphDestroy(kv.second.g);
+    }
+    graph_cache_.clear();
+    (void)hipFree(cyc_d_);
+    if (cyc_h_) (void)hipHostFree(cyc_h_);
     if (h_stage_) (void)hipHostFree(h_stage_);
     if (h_ctr_) (void)hipHostFree(h_ctr_);
     if (h_hash_) (void)hipHostFree(h_hash_);
@@ -616,6 +623,11 @@ int Engine::init(int nslots, const Params& P) {
     HIPCHK(hipMalloc((void**)&d_ctrs_, sizeof(Ctr) * nslots));
     HIPCHK(hipMemset(d_ctrs_, 0, sizeof(Ctr) * nslots));
     HIPCHK(hipHostMalloc((void**)&h_hash_, sizeof(unsigned long long) * nslots));
+    {
+        const size_t cyc = (size_t)std::max(1, P.fuse) * 3 * (size_t)nslots;
+        HIPCHK(hipMalloc(&cyc_d_, sizeof(int32_t) * cyc));
+        HIPCHK(hipHostMalloc((void**)&cyc_h_, sizeof(int32_t) * cyc));
+    }
     memset(h_ctr_, 0, sizeof(Ctr) * nslots);
     ev_.resize(256);  // timed launch ranges per step (profiling)
     for (auto& ev : ev_) HIPCHK(hipEventCreate(&ev));
@@ -1082,17 +1094,11 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     }
     Group g_act = make_list([&](int i) { return has(i, ST_ACTIVE); });
     Group g_cert = make_list([&](int i) { return has(i, ST_CERT); });
-    // slots of fused steps: per extra cycle c, discharge lists by kind and the
-    // count list
+    // slots of fused steps (see the fused-cycle launches below)
     int max_cycles = 1;
     for (int i = 0; i < nslots_; ++i)
-        if (slots_[i].live && has(i, ST_DISCHARGE)) max_cycles = std::max(max_cycles, plans[i].cycles);
-    std::vector<std::array<Group, 3>> g_cyc(max_cycles);
-    for (int c = 1; c < max_cycles; ++c) {
-        for (int kind = DK_TILE; kind <= DK_LIST; ++kind)
-            g_cyc[c][kind] = make_list([&](int i) { return has(i, ST_DISCHARGE) && plans[i].cycles > c && plans[i].dis_kind == kind; });
-        g_cyc[c][2] = make_list([&](int i) { return has(i, ST_DISCHARGE) && plans[i].cycles > c; });
-    }
+        if (slots_[i].live && has(i, ST_DISCHARGE)) max_cycles = std::max(max_cycles, std::min(plans[i].cycles, std::max(1, P_.fuse)));
+    max_cycles = std::min(max_cycles, 64);
     if (lpos > list_cap_) { set_error("launch list overflow"); return MF_EINVAL; }
     const int nlive = live();
     if (nlive == 0) return 0;
@@ -1224,24 +1230,86 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     if (g_act.cnt) { const int k = tb(WF_ACTIVE); launch_active(st, d_views_, L(g_act), g_act.cnt, maxof(g_act, gridN)); te(k); }
     // further discharge + count cycles of fused steps (deep graphs): the
     // count after each only looks at the previous list and the vertices the
-    // discharge pushed into
-    for (int c = 1; c < max_cycles; ++c) {
-        launch_reset(st, d_views_, L(g_cyc[c][2]), g_cyc[c][2].cnt, RST_TOUCH);
-        for (int kind = DK_TILE; kind <= DK_LIST; ++kind) {
-            const Group& g = g_cyc[c][kind];
-            if (!g.cnt) continue;
-            const int k = tb(WF_DISCHARGE);
-            if (kind == DK_TILE)
-                launch_discharge_tile(st, d_views_, L(g), g.cnt, maxof(g, [&](int i) { return plans[i].dis_grid; }), 0, P_.tile_rounds, giants(g));
-            else
-                launch_discharge_list(st, d_views_, L(g), g.cnt, maxof(g, [&](int i) { return std::max(plans[i].dis_grid, 64); }), giants(g));
-            te(k);
+    // discharge pushed into.  The slot lists sit at fixed places of cyc_d_
+    // and the grids are fixed (the kernels are grid-stride), so a step whose
+    // fused slots repeat replays a captured graph of all its cycles.
+    if (max_cycles > 1) {
+        const int ns = nslots_;
+        std::vector<int32_t> key;
+        key.push_back(max_cycles);
+        int tiles = 1, gi = 0;
+        int cnt[64][3] = {};
+        for (int c = 1; c < max_cycles; ++c)
+            for (int kind = 0; kind < 3; ++kind) {
+                int32_t* lst = cyc_h_ + ((size_t)(c - 1) * 3 + kind) * ns;
+                for (int i = 0; i < ns; ++i) {
+                    if (!slots_[i].live || !has(i, ST_DISCHARGE) || plans[i].cycles <= c) continue;
+                    if (kind < 2 && plans[i].dis_kind != kind) continue;
+                    lst[cnt[c][kind]++] = i;
+                    tiles = std::max(tiles, slots_[i].view.g.ntiles);
+                    gi |= slots_[i].view.g.max_deg > kGiantDeg ? 1 : 0;
+                }
+                key.push_back(cnt[c][kind]);
+                key.insert(key.end(), lst, lst + cnt[c][kind]);
+            }
+        key.push_back(tiles);
+        key.push_back(gi);
+        HIPCHK(hipMemcpyAsync(cyc_d_, cyc_h_, sizeof(int32_t) * (size_t)(max_cycles - 1) * 3 * ns,
+                              hipMemcpyHostToDevice, st));
+        auto LC = [&](int c, int kind) { return cyc_d_ + ((size_t)(c - 1) * 3 + kind) * ns; };
+        auto emit = [&](bool timed) {
+            for (int c = 1; c < max_cycles; ++c) {
+                launch_reset(st, d_views_, LC(c, 2), cnt[c][2], RST_TOUCH);
+                if (cnt[c][DK_TILE]) {
+                    const int k = timed ? tb(WF_DISCHARGE) : -1;
+                    launch_discharge_tile(st, d_views_, LC(c, DK_TILE), cnt[c][DK_TILE], tiles, 0, P_.tile_rounds, gi);
+                    te(k);
+                }
+                if (cnt[c][DK_LIST]) {
+                    const int k = timed ? tb(WF_DISCHARGE) : -1;
+                    launch_discharge_list(st, d_views_, LC(c, DK_LIST), cnt[c][DK_LIST], 1024, gi);
+                    te(k);
+                }
+                launch_reset(st, d_views_, LC(c, 2), cnt[c][2], RST_SPARSE);
+                const int k = timed ? tb(WF_ACTIVE) : -1;
+                launch_active_sparse(st, d_views_, LC(c, 2), cnt[c][2], 512);
+                te(k);
+            }
+        };
+        auto it = graph_cache_.end();
+        if (P_.graphs && !profile) it = graph_cache_.find(key);
+        if (it != graph_cache_.end()) {
+            it->second.last = ++graph_clock_;
+            HIPCHK(hipGraphLaunch(it->second.x, st));
+            ++graph_replays;
+        } else if (P_.graphs && !profile && graph_seen_.count(key)) {
+            // second time this set of fused slots comes round: capture
+            GraphEntry ge;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            emit(false);
+            const hipError_t ce = hipStreamEndCapture(st, &ge.g);
+            if (ce != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(ce)); return MF_EHIP; }
+            HIPCHK(hipGraphInstantiate(&ge.x, ge.g, nullptr, nullptr, 0));
+            ge.last = ++graph_clock_;
+            if (graph_cache_.size() >= 32) {
+                auto lru = graph_cache_.begin();
+                for (auto jt = graph_cache_.begin(); jt != graph_cache_.end(); ++jt)
+                    if (jt->second.last < lru->second.last) lru = jt;
+                (void)hipGraphExecDestroy(lru->second.x);
+                (void)hipGraphDestroy(lru->second.g);
+                graph_cache_.erase(lru);
+            }
+            auto ins = graph_cache_.emplace(key, ge).first;
+            graph_seen_.erase(key);
+            HIPCHK(hipGraphLaunch(ins->second.x, st));
+            ++graph_captures;
+        } else {
+            if (P_.graphs && !profile) {
+                if (graph_seen_.size() > 256) graph_seen_.clear();
+                graph_seen_.insert(key);
+            }
+            emit(true);
         }
-        const Group& ga = g_cyc[c][2];
-        launch_reset(st, d_views_, L(ga), ga.cnt, RST_SPARSE);
-        const int k = tb(WF_ACTIVE);
-        launch_active_sparse(st, d_views_, L(ga), ga.cnt, maxof(ga, [&](int i) { return clampi(cdiv(h_ctr_[i].active, 256), 8, 1024); }));
-        te(k);
     }
     if (g_cert.cnt) { const int k = tb(WF_CERT); launch_cert(st, d_views_, L(g_cert), g_cert.cnt, maxof(g_cert, gridN), P_.cert); te(k); }
     HIPCHK(hipGetLastError());

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <deque>
 #include <map>
+#include <set>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -64,6 +65,9 @@ struct Params {
     // discharge + count cycles per engine step (one host round trip) when the
     // last global relabel was at least 64 levels deep (grids); 1 = off
     int fuse = 8;
+    // replay the fused cycles of a step as a captured HIP graph when the
+    // same set of slots fuses again (not while profiling)
+    int graphs = 1;
 };
 
 // ---------------------------------------------------------- device memory
@@ -359,6 +363,21 @@ private:
     std::vector<std::shared_ptr<DeviceGraph>> graphs_;
     std::vector<SlotState> slots_;
     SlabArena arena_;
+    // fused cycles: slot lists at fixed places (cycle, kind) so that a
+    // captured graph of the cycles can be replayed; graphs keyed by the lists
+    int32_t* cyc_d_ = nullptr;
+    int32_t* cyc_h_ = nullptr;
+    struct GraphEntry {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        uint64_t last = 0;
+    };
+    std::map<std::vector<int32_t>, GraphEntry> graph_cache_;
+    std::set<std::vector<int32_t>> graph_seen_;
+    uint64_t graph_clock_ = 0;
+public:
+    int64_t graph_replays = 0, graph_captures = 0;
+private:
     size_t budget_ = 0;         // 0 = nslots full slabs of the largest graph
     bool arena_stale_ = false;  // default-sized arena smaller than the graphs now need
     int64_t admit_waits_ = 0;   // steps whose queue head waited for slab bytes

@@ -1,0 +1,29 @@
+// mf_tune.h -- autotuner of R, tile size, K and GR_FREQ per graph class.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "host_graph.h"
+#include "mf_engine.h"
+
+namespace mf {
+
+struct TuneResult {
+    int R = 0, tile_max = 0, K = 0, gr_freq = 0;
+    double ms = 0, ms_default = 0;  // plain solver ms per sample query: tuned, defaults
+    int evaluations = 0;            // configurations timed (0: taken from the cache)
+    std::string cls;
+};
+
+// "n<log2 n>-d<log2 avg degree>-s<log2 max/avg degree>-k<log2 BFS depth>"
+std::string graph_class(const HostGraph& g);
+
+// g: the graph before partitioning; s, t: sample queries (its ids).  One
+// coordinate-descent pass over R, tile size, K and GR_FREQ from (R0, tile0,
+// base.K, base.gr_freq); every configuration must give the first one's
+// flows.  cache_path (may be empty): "class R tile K gr_freq ms ms_default"
+// lines, read first and appended to.
+int autotune(const HostGraph& g, const std::vector<int32_t>& s, const std::vector<int32_t>& t, const Params& base,
+             int R0, int tile0, const std::string& cache_path, TuneResult& out);
+
+}  // namespace mf

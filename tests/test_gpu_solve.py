@@ -236,3 +236,30 @@ def test_gpu_certificate_catches_planted_mistakes(c1):
         bad_h[s] = 0
         for mode in (1, 2):
             assert eng.dbg_certify(gid, s, t, res, ex, bad_h, mode)[0] > 0
+
+
+def test_fused_cycles_replay_captured_graphs():
+    """Deep graphs fuse discharge cycles; a repeated set of fused slots is
+    replayed from a captured HIP graph.  Flows equal the oracle's and the
+    same solve without graphs or without fusing."""
+    case = Case(M.Edges.grid(96, 96, 1000, 2101), R=32, tile_max=512)
+    qs, qt = M.grid_queries(96, 96, 4, 2102)
+    q = [case.q(a, b) for a, b in zip(qs, qt)]
+    want = [case.oracle_flow(s, t) for s, t in q]
+    for prm in (dict(), dict(graphs=0), dict(fuse=1)):
+        eng, gid = engine_with(case, **prm)
+        got = [eng.solve(gid, s, t) for s, t in q]
+        assert [r.flow for r in got] == want, prm
+        assert all(r.cert_ok == 1 for r in got)
+        cap, rep = eng.graph_stats()
+        log(f"{prm}: graphs captured {cap}, replays {rep}")
+        if not prm:
+            assert rep > 0
+        else:
+            assert rep == 0
+    # batched: several slots fusing together
+    eng, gid = engine_with(case, nslots=4)
+    for i, (s, t) in enumerate(q):
+        eng.submit(gid, s, t, i)
+    res = {r.tag: r.flow for r in eng.run()}
+    assert [res[i] for i in range(4)] == want
