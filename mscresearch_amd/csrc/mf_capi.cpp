@@ -738,6 +738,7 @@ int mf_dbg_global_relabel(mf_engine* e, int gid, int s, int t, const int64_t* re
     SlotView& v = e->eng.view(0);
     const DGraph& g = v.g;
     HIPCK(hipMemcpy(v.res, res, sizeof(int64_t) * g.E, hipMemcpyHostToDevice));
+    e->eng.slab(0)->res_uid = 0;  // res no longer follows the slab's marks
     HIPCK(hipMemcpy(v.excess, excess, sizeof(int64_t) * g.n, hipMemcpyHostToDevice));
     Plan p;
     p.stages = ST_RESET_PRE | ST_GR_SEED | ST_BFS;
@@ -761,6 +762,7 @@ int mf_dbg_local_relabel(mf_engine* e, int gid, int s, int t, const int64_t* res
     SlotView& v = e->eng.view(0);
     const DGraph& g = v.g;
     HIPCK(hipMemcpy(v.res, res, sizeof(int64_t) * g.E, hipMemcpyHostToDevice));
+    e->eng.slab(0)->res_uid = 0;  // res no longer follows the slab's marks
     HIPCK(hipMemcpy(v.excess, excess, sizeof(int64_t) * g.n, hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(v.h, height, sizeof(int32_t) * g.n, hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(v.tile_cnt, tile_active, sizeof(int32_t) * g.ntiles, hipMemcpyHostToDevice));
@@ -789,6 +791,7 @@ int mf_dbg_certify(mf_engine* e, int gid, int s, int t, const int64_t* res, cons
     SlotView& v = e->eng.view(0);
     const DGraph& g = v.g;
     HIPCK(hipMemcpy(v.res, res, sizeof(int64_t) * g.E, hipMemcpyHostToDevice));
+    e->eng.slab(0)->res_uid = 0;  // res no longer follows the slab's marks
     HIPCK(hipMemcpy(v.excess, excess, sizeof(int64_t) * g.n, hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(v.h, height, sizeof(int32_t) * g.n, hipMemcpyHostToDevice));
     const int saved = e->eng.params().cert;

@@ -95,6 +95,7 @@ struct SlotView {
     int32_t* giant = nullptr;
     int64_t* gacc = nullptr;
     int32_t gcap = 0;
+    uint8_t* emark = nullptr;    // E: entry's pair changed since the residual was last reset
     const int32_t* mask = nullptr;  // optional vertex mask (nullptr = all)
     Ctr* ctr = nullptr;
     int32_t s = 0, t = 0;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <chrono>
 #include <climits>
@@ -44,6 +45,11 @@ static double clock_epoch() {
 }
 
 // ----------------------------------------------------------- device graph
+uint64_t next_graph_uid() {
+    static std::atomic<uint64_t> next{1};
+    return next.fetch_add(1);
+}
+
 DeviceGraph::~DeviceGraph() {
     (void)hipFree(d_row_ptr); (void)hipFree(d_col); (void)hipFree(d_cap); (void)hipFree(d_rev);
     (void)hipFree(d_tile_start); (void)hipFree(d_tile_of);
@@ -153,6 +159,7 @@ size_t Slab::layout(int n, int64_t E, int ntiles, char* base, Slab* sl) {
     char* p_lb = take(sizeof(int32_t) * 2 * kLocalLevels);
     char* p_gi = take(sizeof(int32_t) * (3 * gcap + 1));
     char* p_ga = take(sizeof(int64_t) * gcap);
+    char* p_em = take(ee);
     if (sl) {
         sl->res = (int64_t*)p_res;
         sl->excess = (int64_t*)p_ex;
@@ -167,6 +174,8 @@ size_t Slab::layout(int n, int64_t E, int ntiles, char* base, Slab* sl) {
         sl->lbkt = (int32_t*)p_lb;
         sl->giant = (int32_t*)p_gi;
         sl->gacc = (int64_t*)p_ga;
+        sl->emark = (uint8_t*)p_em;
+        sl->res_uid = 0;
         sl->gcap = (int)gcap;
         sl->n_cap = n;
         sl->E_cap = (int)E;
@@ -753,6 +762,7 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
     v.lock = sl.lock; v.aux = sl.aux; v.hl = sl.hl; v.tile_cnt = sl.tile_cnt; v.tile_hot = sl.tile_hot;
     v.lbkt = sl.lbkt;
     v.giant = sl.giant; v.gacc = sl.gacc; v.gcap = sl.gcap;
+    v.emark = sl.emark;
     v.ctr = d_ctrs_ + i;
     v.s = q.s; v.t = q.t;
     v.H_cut = P_.h_cut;
@@ -1057,7 +1067,12 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         for (unsigned w : vals)
             pre_groups.push_back({w, make_list([&](int i) { return has(i, ST_RESET_PRE) && plans[i].reset_pre == w; })});
     }
-    Group g_init = make_list([&](int i) { return has(i, ST_INIT); });
+    // a slab still holding a residual of the run graph resets marked pairs only
+    auto run_uid = [&](int i) { return slots_[i].run_graph ? slots_[i].run_graph->uid : graphs_[slots_[i].q.graph]->uid; };
+    Group g_init = make_list([&](int i) { return has(i, ST_INIT) && slots_[i].slab.res_uid != run_uid(i); });
+    Group g_init_m = make_list([&](int i) { return has(i, ST_INIT) && slots_[i].slab.res_uid == run_uid(i); });
+    for (int i = 0; i < nslots_; ++i)
+        if (slots_[i].live && has(i, ST_INIT)) slots_[i].slab.res_uid = run_uid(i);
     Group g_pre = make_list([&](int i) { return has(i, ST_PREFLOW); });
     Group g_seed = make_list([&](int i) { return has(i, ST_GR_SEED); });
     Group g_seed1[3];
@@ -1154,6 +1169,11 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     if (g_init.cnt) {
         const int k = tb(WF_INIT);
         launch_init(st, d_views_, L(g_init), g_init.cnt, maxof(g_init, [&](int i) { return std::max(gridE(i), gridN(i)); }));
+        te(k);
+    }
+    if (g_init_m.cnt) {
+        const int k = tb(WF_INIT);
+        launch_init_marked(st, d_views_, L(g_init_m), g_init_m.cnt, maxof(g_init_m, gridN));
         te(k);
     }
     if (g_pre.cnt) { const int k = tb(WF_INIT); launch_preflow(st, d_views_, L(g_pre), g_pre.cnt); te(k); }
@@ -1431,6 +1451,11 @@ int Engine::update_caps(int gid, const int32_t* entries, const int64_t* new_cap,
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
     (void)hipFree(d_e); (void)hipFree(d_new); (void)hipFree(d_old);
+    // residuals of this graph kept by other slots predate the new capacities:
+    // their next query on it resets every entry
+    for (int i = 0; i < nslots_; ++i)
+        if (!(slots_[i].pinned && slots_[i].solved && slots_[i].q.graph == gid) && slots_[i].slab.res_uid == G.uid)
+            slots_[i].slab.res_uid = 0;
     // cached induced subgraphs of this graph carry old capacities
     const std::string pre = std::to_string(gid) + ":";
     for (auto it = cache_.begin(); it != cache_.end();) {

@@ -71,8 +71,11 @@ struct Params {
 };
 
 // ---------------------------------------------------------- device memory
+uint64_t next_graph_uid();
+
 struct DeviceGraph {
     DGraph g;
+    uint64_t uid = next_graph_uid();  // identifies the topology a residual belongs to
     int32_t* d_row_ptr = nullptr;
     int32_t* d_col = nullptr;
     int64_t* d_cap = nullptr;
@@ -134,6 +137,8 @@ struct Slab {
     int32_t* giant = nullptr;
     int64_t* gacc = nullptr;
     int gcap = 0;
+    uint8_t* emark = nullptr;   // E marks of changed entry pairs
+    uint64_t res_uid = 0;       // graph whose residual res holds (0 = none)
     size_t off = 0, bytes = 0;  // arena block
     bool held = false;
     bool fits(int n, int64_t E, int ntiles) const {

@@ -26,6 +26,9 @@ enum BfsMode : int { BFS_GR = 0, BFS_FWD = 1, BFS_UND = 2 };
 
 void launch_reset(hipStream_t st, const SlotView* slots, const int* list, int nlist, unsigned what);
 void launch_init(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);
+// The same for a slab whose residual belongs to the same graph: only pairs
+// of entries marked since the last reset are restored.
+void launch_init_marked(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);
 void launch_preflow(hipStream_t st, const SlotView* slots, const int* list, int nlist);
 
 // Global-relabel seeding: h = 0 on seeds, n elsewhere; seeds go to fa.

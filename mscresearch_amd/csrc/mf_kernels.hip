@@ -132,6 +132,7 @@ struct GSlot {
     GAS int32_t* giant;
     GAS int64_t* gacc;
     int32_t gcap;
+    GAS uint8_t* emark;
     const GAS int32_t* mask;
     GAS Ctr* ctr;
     int32_t s, t, K, Kh, H_cut, phase;
@@ -173,6 +174,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.giant = as_global(v.giant);
     S.gacc = as_global(v.gacc);
     S.gcap = v.gcap;
+    S.emark = as_global(v.emark);
     S.mask = as_global(v.mask);
     S.ctr = as_global(v.ctr);
     S.s = v.s;
@@ -231,6 +233,14 @@ __device__ __forceinline__ void stage_flush(Stage<CAP>& st, GAS int32_t* gbuf, G
 struct Work {
     uint64_t edges = 0, verts = 0, pushes = 0, relabels = 0;
 };
+
+// Move d units of residual across entry e (its pair rev[e] gains them) and
+// mark the entry, so the next query on this slab resets only marked pairs.
+__device__ __forceinline__ void move_res(const GSlot& S, int e, int64_t d) {
+    __hip_atomic_fetch_add(&S.res[e], -d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(&S.res[S.g.rev[e]], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    S.emark[e] = 1;
+}
 
 __device__ __forceinline__ void work_flush(const GSlot& S, Work w, uint64_t* red, int fam) {
     // Four block sums, one atomic each from thread 0.
@@ -299,7 +309,10 @@ __global__ __launch_bounds__(256) void k_init(const SlotView* __restrict__ slots
     const GSlot S = load_slot(slots, list);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (int64_t e = i0; e < S.g.E; e += stride) S.res[e] = S.g.cap[e];
+    for (int64_t e = i0; e < S.g.E; e += stride) {
+        S.res[e] = S.g.cap[e];
+        S.emark[e] = 0;
+    }
     for (int64_t v = i0; v < S.g.n; v += stride) {
         S.excess[v] = 0;
         S.h[v] = 0;
@@ -313,6 +326,53 @@ __global__ __launch_bounds__(256) void k_init(const SlotView* __restrict__ slots
 
 void launch_init(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx) {
     hipLaunchKernelGGL(k_init, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+}
+
+// Reset of a slab that last held a residual of the same graph: only entry
+// pairs marked by a push (or an update) since go back to their capacities;
+// the mark map is read 8 entries per load.
+__global__ __launch_bounds__(256) void k_init_marked(const SlotView* __restrict__ slots,
+                                                     const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ uint64_t red[256 / WAVE];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t E = S.g.E;
+    const int64_t nw = E / 8;
+    const GAS uint64_t* m8 = (const GAS uint64_t*)S.emark;  // slab arrays are 256-byte aligned
+    Work wk;
+    for (int64_t w = i0; w < nw; w += stride) {
+        const uint64_t m = m8[w];
+        if (m == 0) continue;
+        for (int b = 0; b < 8; ++b)
+            if ((m >> (8 * b)) & 0xff) {
+                const int e = (int)(w * 8 + b);
+                const int r = S.g.rev[e];
+                S.res[e] = S.g.cap[e];
+                S.res[r] = S.g.cap[r];
+                S.emark[e] = 0;
+                wk.edges++;
+            }
+    }
+    for (int64_t e = nw * 8 + i0; e < E; e += stride)  // tail
+        if (S.emark[e]) {
+            const int r = S.g.rev[(int)e];
+            S.res[e] = S.g.cap[e];
+            S.res[r] = S.g.cap[r];
+            S.emark[e] = 0;
+            wk.edges++;
+        }
+    for (int64_t v = i0; v < S.g.n; v += stride) {
+        S.excess[v] = 0;
+        S.h[v] = 0;
+        S.lock[v] = -1;
+    }
+    if (i0 == 0) wk.verts += (uint64_t)S.g.n;
+    work_flush(S, wk, red, WF_INIT);
+}
+
+void launch_init_marked(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx) {
+    hipLaunchKernelGGL(k_init_marked, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
 }
 
 // ---------------------------------------------------------------- preflow (a11)
@@ -329,8 +389,7 @@ __global__ __launch_bounds__(256) void k_preflow(const SlotView* __restrict__ sl
     for (int e = rb + threadIdx.x; e < re; e += blockDim.x) {
         const int64_t d = ld64(&S.res[e]);
         if (d > 0) {
-            add64(&S.res[e], -d);
-            add64(&S.res[S.g.rev[e]], d);
+            move_res(S, e, d);
             add64(&S.excess[S.g.col[e]], d);
             sum += d;
         }
@@ -1372,8 +1431,7 @@ __device__ void lane_discharge(const GSlot& S, const TileCtx& tc, int v, int cyc
                 const int w = S.g.col[e];
                 if (nbr_h(S, tc, w) != hmin) continue;
                 const int64_t d = rem < r ? rem : r;
-                add64(&S.res[e], -d);
-                add64(&S.res[S.g.rev[e]], d);
+                move_res(S, e, d);
                 push_into(S, tc, w, d);
                 rem -= d;
                 wk.pushes++;
@@ -1421,8 +1479,7 @@ __device__ void wave_discharge(const GSlot& S, const TileCtx& tc, int v, int cyc
                 const int64_t excl = incl - r;
                 const int64_t d = (r > 0 && rem > excl) ? ((rem - excl) < r ? (rem - excl) : r) : 0;
                 if (d > 0) {
-                    add64(&S.res[e], -d);
-                    add64(&S.res[S.g.rev[e]], d);
+                    move_res(S, e, d);
                     push_into(S, tc, S.g.col[e], d);
                     wk.pushes++;
                 }
@@ -1678,8 +1735,7 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
             if (r[j] <= 0) continue;
             const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
             const int64_t d = rem < r[j] ? rem : r[j];
-            add64(&S.res[e], -d);
-            add64(&S.res[S.g.rev[e]], d);
+            move_res(S, e, d);
             add64(&S.excess[S.g.col[e]], d);
             touch(S, S.g.col[e]);
             rem -= d;
@@ -1819,8 +1875,7 @@ __global__ __launch_bounds__(256) void k_path_push(const SlotView* __restrict__ 
                         const int64_t r = ld64(&S.res[best]);
                         const int64_t d = ex < r ? ex : r;
                         if (d > 0) {
-                            add64(&S.res[best], -d);
-                            add64(&S.res[S.g.rev[best]], d);
+                            move_res(S, best, d);
                             add64(&S.excess[w], d);
                             add64(&S.excess[cur], -d);
                             wk.pushes++;
@@ -1971,8 +2026,7 @@ __global__ __launch_bounds__(256) void k_rm(const SlotView* __restrict__ slots,
             const int re_ = S.g.rev[e];
             const int64_t d = ld64(&S.res[re_]);
             if (d <= 0) continue;
-            add64(&S.res[re_], -d);
-            add64(&S.res[e], d);
+            move_res(S, re_, d);
             add64(&S.excess[w], -d);
             gained += d;
             ++fixed;
@@ -2248,6 +2302,7 @@ __global__ __launch_bounds__(256) void k_apply_updates(const SlotView* __restric
         const int64_t res = S.res[e];
         const int64_t f = c - res;  // flow u -> w on this pair (cap[rev] stays)
         if (f <= c2) {
+            S.emark[e] = 1;
             S.res[e] = res + (c2 - c);
         } else {
             const int64_t back = f - c2;  // flow that no longer fits
