@@ -106,6 +106,7 @@ Params from_c(const mf_params* p) {
     P.local_band = p->local_band;
     P.fuse = p->fuse;
     P.graphs = p->graphs;
+    P.spec_cert = p->spec_cert;
     return P;
 }
 
@@ -420,6 +421,7 @@ void mf_params_default(mf_params* p) {
     p->local_band = P.local_band;
     p->fuse = P.fuse;
     p->graphs = P.graphs;
+    p->spec_cert = P.spec_cert;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

@@ -259,6 +259,8 @@ void SlotSched::begin(const Params* P, const DGraph& g, int s, int t, bool prune
     iter_ = 0;
     last_gr_iter_ = 0;
     gr_depth_ = 0;
+    spec_ = false;
+    spec_failed_ = false;
     stag_ = no_stag_ = 0;
     last_pos_ = INT64_MAX;
     force_gr_ = false;
@@ -433,7 +435,20 @@ Plan SlotSched::plan(const Ctr& c) {
         if (c.err) { fail(MF_EHIP, "device error flag"); return p; }
         const int active = c.active;
         if (active == 0) {
-            if (!exact) { force_gr_ = false; return start_gr(&c); }
+            if (!exact) {
+                // The certificate proves optimality for any closed T = {h < n}:
+                // try it on the discharge's labels first; a failure falls back
+                // to the exact global relabel.
+                if (P_->cert && P_->spec_cert && phase_ == 0 && !spec_failed_ && c.neg_count == 0) {
+                    p.stages = ST_RESET_PRE | ST_CERT;
+                    p.reset_pre = RST_CERT;
+                    spec_ = true;
+                    st_ = S_CERT;
+                    return p;
+                }
+                force_gr_ = false;
+                return start_gr(&c);
+            }
             if (phase_ == 0 && c.neg_count > 0) {
                 // Deficits left (RM or a capacity decrease made them) and no
                 // positive excess can reach t or a deficit: drain them from t.
@@ -567,6 +582,13 @@ Plan SlotSched::plan(const Ctr& c) {
     case S_CERT: {
         stats.cert_ok = (c.cert_bad == 0 && c.fp == 0 && c.cut == c.ex_t) ? 1 : 0;
         stats.flow = c.ex_t;
+        if (!stats.cert_ok && spec_) {  // labels not exact enough: relabel and go on
+            spec_ = false;
+            spec_failed_ = true;
+            stats.spec_fails++;
+            return start_gr(&c);
+        }
+        spec_ = false;
         if (!stats.cert_ok) {
             fail(MF_ECERT, "min-cut certificate failed: bad=" + std::to_string(c.cert_bad) +
                                " fingerprint=" + std::to_string(c.fp) + " cut=" + std::to_string(c.cut) +
@@ -1068,11 +1090,20 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
             pre_groups.push_back({w, make_list([&](int i) { return has(i, ST_RESET_PRE) && plans[i].reset_pre == w; })});
     }
     // a slab still holding a residual of the run graph resets marked pairs only
-    auto run_uid = [&](int i) { return slots_[i].run_graph ? slots_[i].run_graph->uid : graphs_[slots_[i].q.graph]->uid; };
-    Group g_init = make_list([&](int i) { return has(i, ST_INIT) && slots_[i].slab.res_uid != run_uid(i); });
-    Group g_init_m = make_list([&](int i) { return has(i, ST_INIT) && slots_[i].slab.res_uid == run_uid(i); });
+    // (same topology and capacity version; engines sharing a graph see each
+    // other's capacity updates through the version)
+    auto run_g = [&](int i) { return slots_[i].run_graph ? slots_[i].run_graph.get() : graphs_[slots_[i].q.graph].get(); };
+    auto marked_ok = [&](int i) {
+        const Slab& sl = slots_[i].slab;
+        return sl.res_uid == run_g(i)->uid && sl.res_version == run_g(i)->cap_version;
+    };
+    Group g_init = make_list([&](int i) { return has(i, ST_INIT) && !marked_ok(i); });
+    Group g_init_m = make_list([&](int i) { return has(i, ST_INIT) && marked_ok(i); });
     for (int i = 0; i < nslots_; ++i)
-        if (slots_[i].live && has(i, ST_INIT)) slots_[i].slab.res_uid = run_uid(i);
+        if (slots_[i].live && has(i, ST_INIT)) {
+            slots_[i].slab.res_uid = run_g(i)->uid;
+            slots_[i].slab.res_version = run_g(i)->cap_version;
+        }
     Group g_pre = make_list([&](int i) { return has(i, ST_PREFLOW); });
     Group g_seed = make_list([&](int i) { return has(i, ST_GR_SEED); });
     Group g_seed1[3];
@@ -1451,11 +1482,13 @@ int Engine::update_caps(int gid, const int32_t* entries, const int64_t* new_cap,
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
     (void)hipFree(d_e); (void)hipFree(d_new); (void)hipFree(d_old);
-    // residuals of this graph kept by other slots predate the new capacities:
-    // their next query on it resets every entry
+    // residuals of this graph kept elsewhere predate the new capacities: the
+    // next query on it (in any engine) resets every entry; pinned residuals
+    // were adjusted above and stay current
+    ++G.cap_version;
     for (int i = 0; i < nslots_; ++i)
-        if (!(slots_[i].pinned && slots_[i].solved && slots_[i].q.graph == gid) && slots_[i].slab.res_uid == G.uid)
-            slots_[i].slab.res_uid = 0;
+        if (slots_[i].pinned && slots_[i].solved && slots_[i].q.graph == gid && slots_[i].slab.res_uid == G.uid)
+            slots_[i].slab.res_version = G.cap_version;
     // cached induced subgraphs of this graph carry old capacities
     const std::string pre = std::to_string(gid) + ":";
     for (auto it = cache_.begin(); it != cache_.end();) {

@@ -68,6 +68,9 @@ struct Params {
     // replay the fused cycles of a step as a captured HIP graph when the
     // same set of slots fuses again (not while profiling)
     int graphs = 1;
+    // when the discharge leaves no active vertex, certify with its labels
+    // (T = {h < n}) before spending an exact global relabel on termination
+    int spec_cert = 1;
 };
 
 // ---------------------------------------------------------- device memory
@@ -76,6 +79,7 @@ uint64_t next_graph_uid();
 struct DeviceGraph {
     DGraph g;
     uint64_t uid = next_graph_uid();  // identifies the topology a residual belongs to
+    uint64_t cap_version = 0;         // bumped by every capacity update
     int32_t* d_row_ptr = nullptr;
     int32_t* d_col = nullptr;
     int64_t* d_cap = nullptr;
@@ -139,6 +143,7 @@ struct Slab {
     int gcap = 0;
     uint8_t* emark = nullptr;   // E marks of changed entry pairs
     uint64_t res_uid = 0;       // graph whose residual res holds (0 = none)
+    uint64_t res_version = 0;   // ... at this capacity version
     size_t off = 0, bytes = 0;  // arena block
     bool held = false;
     bool fits(int n, int64_t E, int ntiles) const {
@@ -208,6 +213,7 @@ struct QueryStats {
     int64_t bfs_levels = 0;
     int64_t drains = 0;       // deficit drain phases
     int64_t local_relabels = 0;
+    int64_t spec_fails = 0;   // certificates on discharge labels that failed
     int64_t rm_fixed = 0;
     int cert_ok = 0;
     uint64_t edge_scans = 0, vertex_scans = 0, pushes = 0, relabels = 0;
@@ -268,6 +274,8 @@ private:
     bool rm_last_ = false;
     bool work_reset_ = false;
     bool fwd_only_ = false;
+    bool spec_ = false;         // the certificate under way follows discharge labels, not a GR
+    bool spec_failed_ = false;
     bool after_local_ = false;
 };
 

@@ -174,3 +174,31 @@ def test_cli_dynamic_with_updates(tmp_path):
         for rd, qi, flow, ms in rows:
             if rd == rnd:
                 assert flow == O.maxflow(og, qs[qi - 1], qt[qi - 1]), (rnd, qi)
+
+
+def test_engines_sharing_a_graph_see_capacity_updates():
+    """Engine B keeps residuals of a graph in its slabs; engine A updates the
+    graph's capacities; B's next solves must start from the new capacities."""
+    rng = np.random.default_rng(13)
+    e = M.Edges.uniform(3000, 30000, 100, 5301)
+    hg = M.HostGraph.from_edgelist(e).partition(32)
+    d = hg.export()
+    n = len(d["new_id"])
+    cap = d["cap"].copy()
+    dg = M.DeviceGraph(hg, 512)
+    a, b = M.Engine(2), M.Engine(2, M.default_params(prune=0))
+    ga, gb = a.add_graph(dg), b.add_graph(dg)
+    qs, qt = M.uniform_queries(n, 6, 5302)
+    q = [(int(d["new_id"][x]), int(d["new_id"][y])) for x, y in zip(qs, qt)]
+    og = O.Graph(n, d["row_ptr"], d["col"], cap, d["rev"])
+    for s, t in q:
+        assert b.solve(gb, s, t).flow == O.maxflow(og, s, t)
+    for rnd in range(2):
+        ent, new = updated_round(rng, cap, np.nonzero(cap > 0)[0], 0.2)
+        a.update_caps(ga, ent, new)
+        cap[ent] = new
+        og = O.Graph(n, d["row_ptr"], d["col"], cap, d["rev"])
+        for s, t in q:
+            r = b.solve(gb, s, t)
+            assert r.flow == O.maxflow(og, s, t), (rnd, s, t)
+            assert r.cert_ok == 1

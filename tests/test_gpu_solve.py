@@ -44,9 +44,15 @@ def test_c1_flows_and_cpu_certificate(c1):
         f = c1.oracle_flow(s, t)
         assert r.flow == f, (i, r.flow, f)
         assert O.certify(c1.og, res, s, t, r.flow, excess=ex) == 0
-        # the GPU's final heights are the exact reach-to-t set of the residual
+        # the GPU's T = {h < n} is a closed set holding t and not s, so it
+        # holds every vertex with a residual path to t
+        T = h < c1.n
+        assert T[t] and not T[s]
+        owner = np.repeat(np.arange(c1.n), np.diff(c1.og.row_ptr))
+        into_T = (res > 0) & ~T[owner] & T[c1.og.col]
+        assert not into_T.any()
         hb = O.bfs_heights(c1.og, res, np.zeros(c1.n, np.int64), s, t)
-        assert np.array_equal(h < c1.n, hb < c1.n)
+        assert np.all(T[hb < c1.n])
 
 
 @pytest.mark.parametrize("rows,cols,fuse", [(64, 64, 8), (128, 96, 8), (128, 96, 1)])
@@ -121,6 +127,7 @@ def test_edge_cases():
     dict(local_relabel=1, local_band=1 << 30, gr_stall=0, gr_work=0.0),  # local relabel between all discharges
     dict(local_relabel=1, h_cut=0, rm_freq0=1, rm_freq_max=1),       # ... and RM on every iteration
     dict(cert=2),                               # certificate pair sums entry by entry
+    dict(spec_cert=0),                          # terminate through an exact global relabel only
 ])
 def test_scheduler_variants_stay_exact(c1, variant):
     log(f"variant {variant}")
