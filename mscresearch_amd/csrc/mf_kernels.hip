@@ -271,6 +271,7 @@ __global__ void k_reset(const SlotView* __restrict__ slots, const int* __restric
             c->level = 0;
             c->nhl = 0;
             c->nhg = 0;
+            c->ngiant = 0;
             c->visited = 0;
         }
         if (what & (RST_ACTIVE | RST_SPARSE)) {
@@ -579,12 +580,12 @@ __global__ __launch_bounds__(256) void k_bfs_td(const SlotView* __restrict__ slo
         const int v = in[i];
         wk.verts++;
         const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
-        // k_bfs_td_heavy spreads the longer rows over the grid: medium rows
-        // from the front of hl (a wave each), giant rows from the back (a
-        // workgroup each)
+        // longer rows go over the grid: medium rows to hl (k_bfs_td_heavy,
+        // a wave each), giant rows to the giant list (k_bfs_td_giant, in
+        // kGiantChunk-entry tasks)
         if (deg <= kLightDeg) td_lane<MODE>(S, v, L, lvl, st, out, ocnt, odeg, wk);
         else if (deg <= kGiantDeg) S.hl[add32(&S.ctr->nhl, 1)] = v;
-        else S.hl[S.g.n - 1 - add32(&S.ctr->nhg, 1)] = v;
+        else S.giant[add32(&S.ctr->ngiant, 1)] = v;
     }
     stage_flush(st, out, ocnt, odeg);
     const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
@@ -612,10 +613,9 @@ __device__ __forceinline__ void td_wave(const GSlot& S, int v, int L, GAS int32_
     }
 }
 
-// Long rows of a top-down level, taken grid-stride from the level's lists:
-// giant rows (back of hl) a workgroup each, then medium rows (front of hl) a
-// wave each.  Hubs sit together at region starts, so handing them out across
-// the grid is what balances the level.
+// Medium rows of a top-down level, a wave each, taken grid-stride from the
+// level's list.  Hubs sit together at region starts, so handing them out
+// across the grid is what balances the level.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_bfs_td_heavy(const SlotView* __restrict__ slots,
                                                       const int* __restrict__ list) {
@@ -623,9 +623,8 @@ __global__ __launch_bounds__(256) void k_bfs_td_heavy(const SlotView* __restrict
     __shared__ Stage<2048> st;
     __shared__ uint64_t red[256 / WAVE];
     const int nm = S.ctr->nhl;
-    const int ng = S.ctr->nhg;
     const int wpb = (int)(blockDim.x / WAVE);
-    if ((int)blockIdx.x >= ng && (int)blockIdx.x * wpb >= nm) return;  // uniform per block
+    if ((int)blockIdx.x * wpb >= nm) return;  // uniform per block
     const int L = S.ctr->level;
     GAS int32_t* lvl = bfs_lvl<MODE>(S);
     GAS int32_t* out = (L & 1) ? S.fa : S.fb;
@@ -634,8 +633,6 @@ __global__ __launch_bounds__(256) void k_bfs_td_heavy(const SlotView* __restrict
     stage_init(st);
     __syncthreads();
     Work wk;
-    for (int k = blockIdx.x; k < ng; k += gridDim.x)
-        td_block<MODE>(S, S.hl[S.g.n - 1 - k], L, lvl, st, out, ocnt, odeg, wk);
     const int gw = (int)blockIdx.x * wpb + (int)(threadIdx.x / WAVE);
     for (int k = gw; k < nm; k += (int)gridDim.x * wpb) td_wave<MODE>(S, S.hl[k], L, lvl, st, out, ocnt, odeg, wk);
     stage_flush(st, out, ocnt, odeg);
@@ -783,25 +780,8 @@ __global__ void k_bfs_advance(const SlotView* __restrict__ slots, const int* __r
         S.ctr->level = S.ctr->level + 1;
         S.ctr->nhl = 0;
         S.ctr->nhg = 0;
+        S.ctr->ngiant = 0;
     }
-}
-
-void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
-                   int mode) {
-    // gridx = frontier / 256 workgroups; the long rows (at most the frontier)
-    // get up to 2048 workgroups of 4 waves
-    const int gh = gridx * 64 < 2048 ? gridx * 64 : 2048;
-    if (mode == BFS_GR) {
-        hipLaunchKernelGGL(k_bfs_td<BFS_GR>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
-        hipLaunchKernelGGL(k_bfs_td_heavy<BFS_GR>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
-    } else if (mode == BFS_FWD) {
-        hipLaunchKernelGGL(k_bfs_td<BFS_FWD>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
-        hipLaunchKernelGGL(k_bfs_td_heavy<BFS_FWD>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
-    } else {
-        hipLaunchKernelGGL(k_bfs_td<BFS_UND>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
-        hipLaunchKernelGGL(k_bfs_td_heavy<BFS_UND>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
-    }
-    hipLaunchKernelGGL(k_bfs_advance, dim3(1, nlist), dim3(64), 0, st, slots, list);
 }
 
 void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
@@ -1743,6 +1723,73 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
         }
     }
     work_flush(S, wk, red, WF_DISCHARGE);
+}
+
+// Giant rows of a top-down level in kGiantChunk-entry tasks over the grid
+// (the task prefix comes from k_giant_prep over the level's giant list).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bfs_td_giant(const SlotView* __restrict__ slots,
+                                                      const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ Stage<2048> st;
+    __shared__ uint64_t red[256 / WAVE];
+    __shared__ int sh[4];
+    const int ng = S.ctr->ngiant;
+    const int T = S.ctr->ngchunks;
+    if ((int)blockIdx.x >= T) return;  // uniform per block
+    const int L = S.ctr->level;
+    const int n = S.g.n;
+    GAS int32_t* lvl = bfs_lvl<MODE>(S);
+    GAS int32_t* out = (L & 1) ? S.fa : S.fb;
+    GAS int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
+    GAS int64_t* odeg = &S.ctr->fedges[(L + 1) & 3];
+    stage_init(st);
+    Work wk;
+    for (int task = blockIdx.x; task < T; task += gridDim.x) {
+        giant_task(S, task, ng, sh);
+        const int c = sh[1], v = sh[2];
+        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+        const int e0 = rb + c * kGiantChunk;
+        const int e1 = min(re, e0 + kGiantChunk);
+        if (threadIdx.x == 0) wk.edges += (uint64_t)(e1 - e0);
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
+            const int w = S.g.col[e];
+            if (lvl[w] != n) continue;
+            if (!bfs_edge_ok<MODE>(S, e, w)) continue;
+            if (cas32(&lvl[w], n, L + 1)) {
+                stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
+                wk.relabels++;
+            }
+        }
+    }
+    stage_flush(st, out, ocnt, odeg);
+    const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
+    if (threadIdx.x == 0 && nv) add32(&S.ctr->visited, (int32_t)nv);
+    work_flush(S, wk, red, WF_BFS);
+}
+
+void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
+                   int mode) {
+    // gridx = frontier / 256 workgroups; medium rows get up to 2048
+    // workgroups of 4 waves, giant rows 1024 workgroups of chunk tasks
+    const int gh = gridx * 64 < 2048 ? gridx * 64 : 2048;
+    if (mode == BFS_GR) {
+        hipLaunchKernelGGL(k_bfs_td<BFS_GR>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_td_giant<BFS_GR>, dim3(1024, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_td_heavy<BFS_GR>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
+    } else if (mode == BFS_FWD) {
+        hipLaunchKernelGGL(k_bfs_td<BFS_FWD>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_td_giant<BFS_FWD>, dim3(1024, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_td_heavy<BFS_FWD>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
+    } else {
+        hipLaunchKernelGGL(k_bfs_td<BFS_UND>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_td_giant<BFS_UND>, dim3(1024, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_td_heavy<BFS_UND>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
+    }
+    hipLaunchKernelGGL(k_bfs_advance, dim3(1, nlist), dim3(64), 0, st, slots, list);
 }
 
 void launch_giant(hipStream_t st, const SlotView* slots, const int* list, int nlist) {

@@ -21,7 +21,7 @@ from collections import defaultdict
 # kernel name (demangled, template arguments dropped) -> engine family
 FAMILY_OF = {
     "k_init": "init", "k_preflow": "init",
-    "k_gr_seed": "bfs", "k_bfs_small": "bfs", "k_bfs_td": "bfs", "k_bfs_td_heavy": "bfs",
+    "k_gr_seed": "bfs", "k_bfs_small": "bfs", "k_bfs_td": "bfs", "k_bfs_td_heavy": "bfs", "k_bfs_td_giant": "bfs",
     "k_bfs_advance": "bfs", "k_bfs_seed1": "bfs", "k_local_mark": "bfs", "k_local_bound": "bfs",
     "k_local_bucket": "bfs", "k_local_bfs": "bfs", "k_local_finish": "bfs", "k_mask_combine": "bfs",
     "k_bfs_bu": "bfs_bu", "k_bfs_bu_heavy": "bfs_bu",
