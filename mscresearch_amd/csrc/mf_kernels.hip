@@ -1601,58 +1601,27 @@ __device__ __forceinline__ void giant_task(const GSlot& S, int task, int ng, int
     __syncthreads();
 }
 
-// k_giant_min: lowest height over the residual entries of each chunk into
-// the row's minimum slot (INT_MAX - 1 when the chunk has none).  k_giant_push
-// relabels the row (lowest + 1, or n) from its first chunk and has every
-// chunk claim excess from the row's vertex with a compare-and-swap that never
-// overdraws, then push the claimed amount to the chunk's entries at the
-// lowest height in prefix-sum order.  Only the row's chunks take excess or
-// residual out of the row, so the lower-bound reads of the single-owner
-// scheme still hold.
+// k_giant_push: one pass over each chunk of a giant row.  The chunk's
+// entries go to registers; the chunk claims min(excess, residual into lower
+// entries) from the row's vertex with a compare-and-swap that never
+// overdraws and pushes it to the lower entries in prefix-sum order.  The
+// lowest height over the entries that keep residual goes to the row's
+// minimum slot, and gacc counts the chunks that reported; k_giant_relabel
+// raises a vertex that still has excess to that minimum + 1 (n when it is
+// n - 1 or more) once every chunk of its row reported.  Only the row's
+// chunks take excess or residual out of the row, so the lower-bound reads of
+// the single-owner scheme still hold.
 template <typename P>
 __device__ __forceinline__ bool cas64(P p, int64_t& expect, int64_t want) {
     return __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ __launch_bounds__(256) void k_giant_min(const SlotView* __restrict__ slots,
-                                                   const int* __restrict__ list) {
-    const GSlot S = load_slot(slots, list);
-    __shared__ int iscr[256 / WAVE];
-    __shared__ uint64_t red[256 / WAVE];
-    __shared__ int sh[4];
-    const int ng = S.ctr->ngiant;
-    const int T = S.ctr->ngchunks;
-    if ((int)blockIdx.x >= T) return;  // uniform per block
-    const int n = S.g.n;
-    Work wk;
-    for (int task = blockIdx.x; task < T; task += gridDim.x) {
-        giant_task(S, task, ng, sh);
-        const int k = sh[0], c = sh[1], v = sh[2];
-        if (threadIdx.x == 0) sh[3] = (ld64(&S.excess[v]) > 0 && S.h[v] < n) ? 1 : 0;
-        __syncthreads();
-        const bool live = sh[3] != 0;
-        __syncthreads();
-        if (!live) continue;
-        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
-        const int e0 = rb + c * kGiantChunk;
-        const int e1 = min(re, e0 + kGiantChunk);
-        int m = INT_MAX;
-        for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x)
-            if (ld64(&S.res[e]) > 0) m = min(m, S.h[S.g.col[e]]);
-        m = block_min_i32(m, iscr);
-        if (threadIdx.x == 0) {
-            min32(&S.giant[S.gcap + k], m == INT_MAX ? INT_MAX - 1 : m);
-            wk.edges += (uint64_t)(e1 - e0);
-        }
-    }
-    work_flush(S, wk, red, WF_DISCHARGE);
-}
-
 __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__ slots,
                                                     const int* __restrict__ list) {
     const GSlot S = load_slot(slots, list);
     __shared__ int64_t lscr[256 / WAVE];
+    __shared__ int iscr[256 / WAVE];
     __shared__ int64_t bcast;
     __shared__ uint64_t red[256 / WAVE];
     __shared__ int sh[4];
@@ -1665,61 +1634,100 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
     for (int task = blockIdx.x; task < T; task += gridDim.x) {
         giant_task(S, task, ng, sh);
         const int k = sh[0], c = sh[1], v = sh[2];
-        const int hmin = S.giant[S.gcap + k];
-        if (hmin == INT_MAX) continue;  // k_giant_min skipped the row: inactive
-        if (hmin >= n - 1) {            // no residual entry leads below n - 1: v cannot reach t
-            if (c == 0 && threadIdx.x == 0 && S.h[v] < n) { S.h[v] = n; wk.relabels++; }
-            continue;
+        if (threadIdx.x == 0) {
+            bcast = ld64(&S.excess[v]);
+            sh[3] = S.h[v];
         }
-        if (c == 0 && threadIdx.x == 0 && S.h[v] <= hmin) { S.h[v] = hmin + 1; wk.relabels++; }
-        // nothing to do once the row's vertex has no excess left to claim
-        if (threadIdx.x == 0) bcast = ld64(&S.excess[v]);
         __syncthreads();
-        const bool dry = bcast <= 0;
+        const int64_t ex0 = bcast;
+        const int hv = sh[3];
         __syncthreads();
-        if (dry) continue;
+        if (ex0 <= 0 || hv >= n) continue;  // dry or cut off: no report, no relabel
         const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
         const int e0 = rb + c * kGiantChunk;
         const int e1 = min(re, e0 + kGiantChunk);
         int64_t r[PER];
+        int hw[PER], cw[PER];
         int64_t tsum = 0;
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
             r[j] = 0;
+            hw[j] = INT_MAX;
+            cw[j] = 0;
             if (e < e1) {
                 const int64_t x = ld64(&S.res[e]);
-                if (x > 0 && S.h[S.g.col[e]] == hmin) r[j] = x;
+                if (x > 0) {
+                    r[j] = x;
+                    cw[j] = S.g.col[e];
+                    hw[j] = S.h[cw[j]];
+                    if (hw[j] < hv) tsum += x;
+                }
             }
-            tsum += r[j];
         }
         int64_t tot = 0;
         const int64_t incl = block_incl_scan_i64(tsum, lscr, &tot);
         if (threadIdx.x == 0) wk.edges += (uint64_t)(e1 - e0);
-        if (tot == 0) continue;
-        if (threadIdx.x == 0) {
-            int64_t cur = ld64(&S.excess[v]);
-            int64_t a = 0;
-            while (cur > 0) {
-                a = cur < tot ? cur : tot;
-                if (cas64(&S.excess[v], cur, cur - a)) break;
+        int64_t got = 0;
+        if (tot > 0) {  // block-uniform
+            if (threadIdx.x == 0) {
+                int64_t cur = ld64(&S.excess[v]);
+                int64_t a = 0;
+                while (cur > 0) {
+                    a = cur < tot ? cur : tot;
+                    if (cas64(&S.excess[v], cur, cur - a)) break;
+                }
+                bcast = cur > 0 ? a : 0;
             }
-            bcast = cur > 0 ? a : 0;
+            __syncthreads();
+            got = bcast;
+            __syncthreads();
         }
-        __syncthreads();
-        int64_t rem = bcast - (incl - tsum);
-        __syncthreads();
+        int64_t rem = got - (incl - tsum);
+        int m = INT_MAX - 1;
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            if (rem <= 0) break;
             if (r[j] <= 0) continue;
-            const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
-            const int64_t d = rem < r[j] ? rem : r[j];
-            move_res(S, e, d);
-            add64(&S.excess[S.g.col[e]], d);
-            touch(S, S.g.col[e]);
-            rem -= d;
-            wk.pushes++;
+            if (hw[j] < hv && rem > 0) {
+                const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
+                const int64_t d = rem < r[j] ? rem : r[j];
+                move_res(S, e, d);
+                add64(&S.excess[cw[j]], d);
+                touch(S, cw[j]);
+                rem -= d;
+                r[j] -= d;
+                wk.pushes++;
+            }
+            if (r[j] > 0) m = min(m, hw[j]);
+        }
+        m = block_min_i32(m, iscr);
+        if (threadIdx.x == 0) {
+            min32(&S.giant[S.gcap + k], m);
+            add64(&S.gacc[k], 1);
+        }
+    }
+    work_flush(S, wk, red, WF_DISCHARGE);
+}
+
+__global__ __launch_bounds__(256) void k_giant_relabel(const SlotView* __restrict__ slots,
+                                                       const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ uint64_t red[256 / WAVE];
+    const int ng = S.ctr->ngiant;
+    const int n = S.g.n;
+    Work wk;
+    for (int k = threadIdx.x; k < ng; k += blockDim.x) {
+        const int v = S.giant[k];
+        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+        const int nch = (re - rb + kGiantChunk - 1) / kGiantChunk;
+        if (S.gacc[k] != (int64_t)nch) continue;  // a chunk found the vertex dry
+        if (ld64(&S.excess[v]) <= 0) continue;
+        const int hv = S.h[v];
+        const int m = S.giant[S.gcap + k];
+        const int nh = m >= n - 1 ? n : m + 1;
+        if (nh > hv) {
+            S.h[v] = nh;
+            wk.relabels++;
         }
     }
     work_flush(S, wk, red, WF_DISCHARGE);
@@ -1794,8 +1802,8 @@ void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int n
 
 void launch_giant(hipStream_t st, const SlotView* slots, const int* list, int nlist) {
     hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
-    hipLaunchKernelGGL(k_giant_min, dim3(512, nlist), dim3(256), 0, st, slots, list);
     hipLaunchKernelGGL(k_giant_push, dim3(512, nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_giant_relabel, dim3(1, nlist), dim3(256), 0, st, slots, list);
 }
 
 void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* list, int nlist,
@@ -1843,7 +1851,7 @@ __global__ __launch_bounds__(512) void k_discharge_tile(const SlotView* __restri
             const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
             if (deg <= kLightDeg) alist[atomicAdd(&nlight, 1)] = i;
             else if (deg <= kGiantDeg) alist[size - 1 - atomicAdd(&nheavy, 1)] = i;
-            // giant rows: k_giant_min / k_giant_push after the tile pass
+            // giant rows: k_giant_push / k_giant_relabel after the tile pass
         }
         if (threadIdx.x == 0) wk.verts += (uint64_t)size;
         __syncthreads();

@@ -27,7 +27,7 @@ FAMILY_OF = {
     "k_bfs_bu": "bfs_bu", "k_bfs_bu_heavy": "bfs_bu",
     "k_active": "active", "k_active_sparse": "active", "k_list_copy": "active",
     "k_discharge_list": "discharge", "k_discharge_tile": "discharge", "k_giant_prep": "discharge",
-    "k_giant_min": "discharge", "k_giant_push": "discharge", "k_path_push": "discharge",
+    "k_giant_push": "discharge", "k_giant_relabel": "discharge", "k_path_push": "discharge",
     "k_hot_select": "discharge",
     "k_rm": "rm",
     "k_cert": "cert", "k_cert_chunks": "cert", "k_cert_fin": "cert",
