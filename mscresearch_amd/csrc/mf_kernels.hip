@@ -2232,6 +2232,96 @@ __global__ __launch_bounds__(256) void k_cert(const SlotView* __restrict__ slots
     cert_flush(S, cut, bad, edges, fp, red64, redu);
 }
 
+// Fingerprint mode, entry-parallel.  A workgroup takes 256 consecutive
+// vertices: their row starts, heights and vertex weights go to LDS and the
+// entries of their rows are read coalesced (giant rows excepted), the row of
+// an entry found by binary search over the LDS row starts.  Each entry adds
+// (w(pair) + u(row)) * res to the fingerprint and each vertex adds
+// -u(v) * (capsum(v) + stored excess(v)), u an odd hash of the vertex id
+// under a second seed.  The u terms sum to zero exactly when every stored
+// excess equals its row's residual sum minus capacity sum, so one 64-bit sum
+// checks pair sums and stored excesses together (a single wrong term can
+// never give zero, several cancel with chance about 2^-64); the vertex
+// checks then read the stored excess.  Giant rows go to the giant list;
+// k_cert_chunks / k_cert_fin check them with exact row sums.
+__device__ __forceinline__ uint64_t vertex_weight(int v, uint64_t seed) {
+    return pair_weight((uint64_t)(uint32_t)v, seed ^ 0x6a09e667f3bcc909ULL);
+}
+
+__global__ __launch_bounds__(256) void k_cert_fp(const SlotView* __restrict__ slots,
+                                                 const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ int64_t red64[256 / WAVE];
+    __shared__ uint64_t redu[256 / WAVE];
+    __shared__ int rp[257];
+    __shared__ int hh[256];
+    __shared__ uint64_t uw[256];
+    __shared__ unsigned char isg[256];
+    const int n = S.g.n;
+    const uint64_t seed = S.g.cert_seed;
+    int64_t cut = 0;
+    uint64_t bad = 0, edges = 0, fp = 0;
+    const int nblk = (n + 255) / 256;
+    for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
+        const int v0 = b * 256;
+        const int nv = min(256, n - v0);
+        const int i = (int)threadIdx.x;
+        __syncthreads();  // the previous block's LDS reads are done
+        if (i < nv) {
+            const int v = v0 + i;
+            const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+            rp[i] = rb;
+            if (i == nv - 1) rp[nv] = re;
+            const int hv = S.h[v];
+            hh[i] = hv;
+            const uint64_t u = vertex_weight(v, seed);
+            uw[i] = u;
+            const bool giant = re - rb > kGiantDeg;
+            isg[i] = giant ? 1 : 0;
+            const bool inT = hv < n;
+            if (giant) {
+                S.giant[add32(&S.ctr->ngiant, 1)] = v;
+            } else {
+                const int64_t x = ld64(&S.excess[v]);
+                fp -= u * (uint64_t)(S.g.capsum[v] + x);
+                if (v != S.s && x < 0) ++bad;
+                if (inT && v != S.t && x != 0) ++bad;
+                if (v == S.t) {
+                    S.ctr->ex_t = x;
+                    if (!inT) ++bad;
+                }
+                if (v == S.s && inT) ++bad;
+                edges += (uint64_t)(re - rb);
+            }
+        }
+        __syncthreads();
+        const int E0 = rp[0], E1 = rp[nv];
+        int lo = 0;  // row of the previous entry: rows only grow along e
+        for (int e = E0 + i; e < E1; e += (int)blockDim.x) {
+            int hi = nv - 1;  // last row r with rp[r] <= e
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (rp[mid] <= e) lo = mid; else hi = mid - 1;
+            }
+            const int r = lo;
+            if (isg[r]) {  // skip the giant row, keeping e's residue mod blockDim
+                const int past = rp[r + 1] - e;
+                e += ((past + (int)blockDim.x - 1) / (int)blockDim.x - 1) * (int)blockDim.x;
+                continue;
+            }
+            const int64_t x = S.res[e];
+            if (x < 0) ++bad;
+            const int rv = S.g.rev[e];
+            fp += (pair_weight((uint64_t)min(e, rv), seed) + uw[r]) * (uint64_t)x;
+            if (hh[r] < n && S.h[S.g.col[e]] >= n) {  // pair col[e] -> row enters T
+                cut += S.g.cap[rv];
+                if (S.res[rv] > 0) ++bad;
+            }
+        }
+    }
+    cert_flush(S, cut, bad, edges, fp, red64, redu);
+}
+
 // Giant rows of the certificate in kGiantChunk-entry tasks over the grid:
 // each chunk adds its part of the row's recomputed excess into gacc[row];
 // k_cert_fin then checks each giant vertex.
@@ -2286,7 +2376,7 @@ __global__ __launch_bounds__(256) void k_cert_fin(const SlotView* __restrict__ s
 
 void launch_cert(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx, int mode) {
     if (mode == CERT_FULL) hipLaunchKernelGGL(k_cert<CERT_FULL>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
-    else hipLaunchKernelGGL(k_cert<CERT_FP>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    else hipLaunchKernelGGL(k_cert_fp, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
     hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
     if (mode == CERT_FULL) hipLaunchKernelGGL(k_cert_chunks<CERT_FULL>, dim3(1024, nlist), dim3(256), 0, st, slots, list);
     else hipLaunchKernelGGL(k_cert_chunks<CERT_FP>, dim3(1024, nlist), dim3(256), 0, st, slots, list);

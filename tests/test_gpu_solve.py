@@ -196,6 +196,33 @@ def test_giant_rows():
             r = eng.solve(gid, sa, tb)
             assert r.flow == O.maxflow(og, sa, tb), (variant, a, b)
             assert r.cert_ok == 1
+    # planted mistakes in and beside the giant hub row: the fingerprint mode
+    # (giant rows by exact row sums, the others by vertex terms) rejects each
+    eng = M.Engine(1)
+    gid = eng.add_graph(dg)
+    sa, tb = int(d["new_id"][s]), int(d["new_id"][t])
+    hb = int(d["new_id"][hub])
+    r, res, ex, h = eng.dbg_solve_state(gid, sa, tb, n, len(d["col"]))
+    for mode in (1, 2):
+        assert eng.dbg_certify(gid, sa, tb, res, ex, h, mode) == (0, r.flow, r.flow, True)
+    rp = d["row_ptr"]
+    assert rp[hb + 1] - rp[hb] > 2048
+    e_hub = rp[hb] + int(np.nonzero(res[rp[hb]:rp[hb + 1]] > 0)[0][0])
+    leaf = int(d["col"][e_hub])
+    assert rp[leaf + 1] - rp[leaf] <= 32
+    e_leaf = int(rp[leaf])
+    for e_bad, row in ((e_hub, hb), (e_leaf, leaf)):
+        # pair sum off, row sum kept: only the pair side of the fingerprint
+        bad_res, bad_ex = res.copy(), ex.copy()
+        bad_res[e_bad] += 1
+        bad_ex[row] += 1
+        b1 = eng.dbg_certify(gid, sa, tb, bad_res, bad_ex, h, 1)
+        assert b1[0] > 0 or not b1[3], row
+        # stored excess off alone
+        bad_ex = ex.copy()
+        bad_ex[row] -= 1
+        b1 = eng.dbg_certify(gid, sa, tb, res, bad_ex, h, 1)
+        assert b1[0] > 0 or not b1[3], row
 
 
 def test_gpu_certificate_catches_planted_mistakes(c1):
@@ -226,13 +253,27 @@ def test_gpu_certificate_catches_planted_mistakes(c1):
         assert O.certify(og, bad_res, s, t, r.flow, excess=ex) != 0
         assert not eng.dbg_certify(gid, s, t, bad_res, ex, h, 1)[3]
         assert eng.dbg_certify(gid, s, t, bad_res, ex, h, 2)[0] > 0
-        # (b) stored excess off by one at an inner vertex
+        # (b) stored excess off by one at an inner vertex: the full mode
+        # counts it; the fingerprint mode's vertex terms leave a nonzero sum
+        # (or count it, for a vertex of T or of a giant row)
         w = int(rng.choice(np.setdiff1d(np.arange(n), [s, t])))
         bad_ex = ex.copy()
         bad_ex[w] += 1
         assert O.certify(og, res, s, t, r.flow, excess=bad_ex) != 0
-        for mode in (1, 2):
-            assert eng.dbg_certify(gid, s, t, res, bad_ex, h, mode)[0] > 0
+        assert eng.dbg_certify(gid, s, t, res, bad_ex, h, 2)[0] > 0
+        b1 = eng.dbg_certify(gid, s, t, res, bad_ex, h, 1)
+        assert b1[0] > 0 or not b1[3]
+        # (b') a residual off by one with the stored excess of its row
+        # following it: pair sums and row sums disagree, only the pair side
+        # of the fingerprint sees it
+        e3 = int(fwd[owner[fwd] == v][0])
+        bad_res = res.copy()
+        bad_res[e3] += 1
+        bad_ex = ex.copy()
+        bad_ex[owner[e3]] += 1
+        b1 = eng.dbg_certify(gid, s, t, bad_res, bad_ex, h, 1)
+        assert b1[0] > 0 or not b1[3]
+        assert eng.dbg_certify(gid, s, t, bad_res, bad_ex, h, 2)[0] > 0
         # (c) a negative residual
         bad_res = res.copy()
         bad_res[int(rng.choice(np.nonzero(res > 0)[0]))] = -1
