@@ -2106,16 +2106,15 @@ void launch_rm(hipStream_t st, const SlotView* slots, const int* list, int nlist
 // a residual entry from outside T into T.  cut = sum of cap over entries
 // from outside T into T.
 
-// Row terms of the certificate.  Every entry pair has exactly one entry with
-// capacity (forward) and one without (reverse).  Pair sums, two ways:
-//  FULL: from the forward side, res[e] + res[rev e] == cap[e] (a gather);
-//  FP  : every entry adds w(pair) * (res[e] - cap[e]) to a 64-bit sum, w an
-//        odd hash of the pair's lower entry id and the graph's seed; pairs
-//        that sum to their capacity add 0, a single wrong pair can never
-//        give 0 (odd w, |defect| < 2^63), several cancel with chance about
-//        2^-64.  The cap terms are a graph constant (g.cert_fp, subtracted
-//        once by k_cert_fin) and so are the row sums of cap (g.capsum), so
-//        an entry costs res + rev, both coalesced.
+// Every entry pair has exactly one entry with capacity (forward) and one
+// without (reverse).  Two modes:
+//  FULL (k_cert, k_cert_chunks, k_cert_fin): pair sums from the forward
+//        side, res[e] + res[rev e] == cap[e] (a gather), and each stored
+//        excess against its row's recomputed sum, exactly;
+//  FP   (k_cert_fp): pair sums and stored excesses through one 64-bit
+//        fingerprint of odd weights (w per pair, u per vertex); the cap
+//        terms are graph constants (g.cert_fp, g.capsum), so an entry costs
+//        res + rev, both coalesced.
 // Rows of T also check closedness and sum the cut from T's side: entry
 // e = w -> u with u outside T closes pair u -> w.
 enum CertMode : int { CERT_FP = 1, CERT_FULL = 2 };
@@ -2127,22 +2126,15 @@ __device__ __forceinline__ uint64_t pair_weight(uint64_t pair, uint64_t seed) {
     return (x ^ (x >> 31)) | 1ULL;
 }
 
-template <int MODE>
 __device__ __forceinline__ void cert_entry(const GSlot& S, int e, bool inT, int n, int64_t& ex,
-                                           uint64_t& bad, int64_t& cut, uint64_t& fp) {
+                                           uint64_t& bad, int64_t& cut) {
     const int64_t r = S.res[e];
     if (r < 0) ++bad;
-    if (MODE == CERT_FULL) {
-        const int64_t c = S.g.cap[e];
-        ex += r - c;
-        if (c > 0) {  // forward entry; its reverse entry has capacity 0 by construction
-            const int rv = S.g.rev[e];
-            if (r + S.res[rv] != c) ++bad;
-        }
-    } else {
-        ex += r;  // k_cert_fin / cert_vertex subtract the row's capacity sum
+    const int64_t c = S.g.cap[e];
+    ex += r - c;
+    if (c > 0) {  // forward entry; its reverse entry has capacity 0 by construction
         const int rv = S.g.rev[e];
-        fp += pair_weight((uint64_t)min(e, rv), S.g.cert_seed) * (uint64_t)r;
+        if (r + S.res[rv] != c) ++bad;
     }
     if (inT) {
         if (S.h[S.g.col[e]] >= n) {  // col[e] outside T: its pair entry leaves V\T into T
@@ -2153,9 +2145,7 @@ __device__ __forceinline__ void cert_entry(const GSlot& S, int e, bool inT, int 
     }
 }
 
-template <int MODE>
 __device__ __forceinline__ void cert_vertex(const GSlot& S, int v, int64_t ex, bool inT, uint64_t& bad) {
-    if (MODE == CERT_FP) ex -= S.g.capsum[v];
     if (ex != S.excess[v]) ++bad;
     if (v != S.s && ex < 0) ++bad;
     if (inT && v != S.t && ex != 0) ++bad;
@@ -2182,7 +2172,6 @@ __device__ __forceinline__ void cert_flush(const GSlot& S, int64_t cut, uint64_t
     }
 }
 
-template <int MODE>
 __global__ __launch_bounds__(256) void k_cert(const SlotView* __restrict__ slots,
                                               const int* __restrict__ list) {
     const GSlot S = load_slot(slots, list);
@@ -2192,7 +2181,7 @@ __global__ __launch_bounds__(256) void k_cert(const SlotView* __restrict__ slots
     __shared__ int nmed;
     const int n = S.g.n;
     int64_t cut = 0;
-    uint64_t bad = 0, edges = 0, fp = 0;
+    uint64_t bad = 0, edges = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
         if (threadIdx.x == 0) nmed = 0;
@@ -2204,9 +2193,9 @@ __global__ __launch_bounds__(256) void k_cert(const SlotView* __restrict__ slots
             if (re - rb <= kLightDeg) {  // one lane
                 const bool inT = S.h[v] < n;
                 int64_t ex = 0;
-                for (int e = rb; e < re; ++e) cert_entry<MODE>(S, e, inT, n, ex, bad, cut, fp);
+                for (int e = rb; e < re; ++e) cert_entry(S, e, inT, n, ex, bad, cut);
                 edges += (uint64_t)(re - rb);
-                cert_vertex<MODE>(S, v, ex, inT, bad);
+                cert_vertex(S, v, ex, inT, bad);
             } else if (re - rb <= kGiantDeg) {  // one wave, below
                 med[atomicAdd(&nmed, 1)] = v;
             } else {  // giant list: k_cert_chunks, k_cert_fin
@@ -2220,52 +2209,53 @@ __global__ __launch_bounds__(256) void k_cert(const SlotView* __restrict__ slots
             const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
             const bool inT = S.h[v] < n;
             int64_t ex = 0;
-            for (int e = rb + lane_id(); e < re; e += WAVE) cert_entry<MODE>(S, e, inT, n, ex, bad, cut, fp);
+            for (int e = rb + lane_id(); e < re; e += WAVE) cert_entry(S, e, inT, n, ex, bad, cut);
             ex = wave_sum_i64(ex);
             if (lane_id() == 0) {
                 edges += (uint64_t)(re - rb);
-                cert_vertex<MODE>(S, v, ex, inT, bad);
+                cert_vertex(S, v, ex, inT, bad);
             }
         }
         __syncthreads();
     }
-    cert_flush(S, cut, bad, edges, fp, red64, redu);
+    cert_flush(S, cut, bad, edges, 0, red64, redu);
 }
 
-// Fingerprint mode, entry-parallel.  A workgroup takes 256 consecutive
-// vertices: their row starts, heights and vertex weights go to LDS and the
-// entries of their rows are read coalesced (giant rows excepted), the row of
-// an entry found by binary search over the LDS row starts.  Each entry adds
-// (w(pair) + u(row)) * res to the fingerprint and each vertex adds
-// -u(v) * (capsum(v) + stored excess(v)), u an odd hash of the vertex id
-// under a second seed.  The u terms sum to zero exactly when every stored
-// excess equals its row's residual sum minus capacity sum, so one 64-bit sum
-// checks pair sums and stored excesses together (a single wrong term can
-// never give zero, several cancel with chance about 2^-64); the vertex
-// checks then read the stored excess.  Giant rows go to the giant list;
-// k_cert_chunks / k_cert_fin check them with exact row sums.
+// Fingerprint mode, entry-parallel, one kernel.  A workgroup takes 256
+// consecutive vertices: their row starts, heights and vertex weights go to
+// LDS and the entries of their rows are read coalesced, four per thread in
+// flight, the row of an entry found by binary lifting over the LDS row
+// starts.  Each entry adds (w(pair) + u(row)) * res to the fingerprint and
+// each vertex adds -u(v) * (capsum(v) + stored excess(v)), u an odd hash of
+// the vertex id under a second seed.  The u terms sum to zero exactly when
+// every stored excess equals its row's residual sum minus capacity sum, so
+// one 64-bit sum checks pair sums and stored excesses together (a single
+// wrong term can never give zero, several cancel with chance about 2^-64);
+// the vertex checks then read the stored excess.  A hub row is read by its
+// vertex's workgroup like any other: at most ~10^5 coalesced entries.
 __device__ __forceinline__ uint64_t vertex_weight(int v, uint64_t seed) {
     return pair_weight((uint64_t)(uint32_t)v, seed ^ 0x6a09e667f3bcc909ULL);
 }
 
 __global__ __launch_bounds__(256) void k_cert_fp(const SlotView* __restrict__ slots,
                                                  const int* __restrict__ list) {
+    constexpr int U = 4;  // entries per thread in flight
     const GSlot S = load_slot(slots, list);
     __shared__ int64_t red64[256 / WAVE];
     __shared__ uint64_t redu[256 / WAVE];
     __shared__ int rp[257];
     __shared__ int hh[256];
     __shared__ uint64_t uw[256];
-    __shared__ unsigned char isg[256];
     const int n = S.g.n;
     const uint64_t seed = S.g.cert_seed;
     int64_t cut = 0;
     uint64_t bad = 0, edges = 0, fp = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) fp = 0ULL - *S.g.cert_fp;  // the pairs' capacity terms
     const int nblk = (n + 255) / 256;
+    const int i = (int)threadIdx.x;
     for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
         const int v0 = b * 256;
         const int nv = min(256, n - v0);
-        const int i = (int)threadIdx.x;
         __syncthreads();  // the previous block's LDS reads are done
         if (i < nv) {
             const int v = v0 + i;
@@ -2276,46 +2266,52 @@ __global__ __launch_bounds__(256) void k_cert_fp(const SlotView* __restrict__ sl
             hh[i] = hv;
             const uint64_t u = vertex_weight(v, seed);
             uw[i] = u;
-            const bool giant = re - rb > kGiantDeg;
-            isg[i] = giant ? 1 : 0;
             const bool inT = hv < n;
-            if (giant) {
-                S.giant[add32(&S.ctr->ngiant, 1)] = v;
-            } else {
-                const int64_t x = ld64(&S.excess[v]);
-                fp -= u * (uint64_t)(S.g.capsum[v] + x);
-                if (v != S.s && x < 0) ++bad;
-                if (inT && v != S.t && x != 0) ++bad;
-                if (v == S.t) {
-                    S.ctr->ex_t = x;
-                    if (!inT) ++bad;
-                }
-                if (v == S.s && inT) ++bad;
-                edges += (uint64_t)(re - rb);
+            const int64_t x = ld64(&S.excess[v]);
+            fp -= u * (uint64_t)(S.g.capsum[v] + x);
+            if (v != S.s && x < 0) ++bad;
+            if (inT && v != S.t && x != 0) ++bad;
+            if (v == S.t) {
+                S.ctr->ex_t = x;
+                if (!inT) ++bad;
             }
+            if (v == S.s && inT) ++bad;
+            edges += (uint64_t)(re - rb);
         }
         __syncthreads();
-        const int E0 = rp[0], E1 = rp[nv];
-        int lo = 0;  // row of the previous entry: rows only grow along e
-        for (int e = E0 + i; e < E1; e += (int)blockDim.x) {
-            int hi = nv - 1;  // last row r with rp[r] <= e
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (rp[mid] <= e) lo = mid; else hi = mid - 1;
+        const int E1 = rp[nv];
+        int lo = 0;  // row of this thread's previous entry: rows only grow along e
+        for (int eb = rp[0] + i; eb < E1; eb += U * 256) {
+            int r[U], rv[U];
+            int64_t x[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const int e = eb + j * 256;
+                int rr = lo;  // last row with rp[row] <= e, within lo + 255
+#pragma unroll
+                for (int step = 128; step > 0; step >>= 1) {
+                    const int c = rr + step;
+                    if (c < nv && rp[c] <= e) rr = c;
+                }
+                r[j] = rr;
+                x[j] = 0;
+                rv[j] = 0;
+                if (e < E1) {
+                    x[j] = S.res[e];
+                    rv[j] = S.g.rev[e];
+                }
             }
-            const int r = lo;
-            if (isg[r]) {  // skip the giant row, keeping e's residue mod blockDim
-                const int past = rp[r + 1] - e;
-                e += ((past + (int)blockDim.x - 1) / (int)blockDim.x - 1) * (int)blockDim.x;
-                continue;
-            }
-            const int64_t x = S.res[e];
-            if (x < 0) ++bad;
-            const int rv = S.g.rev[e];
-            fp += (pair_weight((uint64_t)min(e, rv), seed) + uw[r]) * (uint64_t)x;
-            if (hh[r] < n && S.h[S.g.col[e]] >= n) {  // pair col[e] -> row enters T
-                cut += S.g.cap[rv];
-                if (S.res[rv] > 0) ++bad;
+            lo = r[U - 1];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const int e = eb + j * 256;
+                if (e >= E1) break;
+                if (x[j] < 0) ++bad;
+                fp += (pair_weight((uint64_t)min(e, rv[j]), seed) + uw[r[j]]) * (uint64_t)x[j];
+                if (hh[r[j]] < n && S.h[S.g.col[e]] >= n) {  // pair col[e] -> row enters T
+                    cut += S.g.cap[rv[j]];
+                    if (S.res[rv[j]] > 0) ++bad;
+                }
             }
         }
     }
@@ -2325,7 +2321,6 @@ __global__ __launch_bounds__(256) void k_cert_fp(const SlotView* __restrict__ sl
 // Giant rows of the certificate in kGiantChunk-entry tasks over the grid:
 // each chunk adds its part of the row's recomputed excess into gacc[row];
 // k_cert_fin then checks each giant vertex.
-template <int MODE>
 __global__ __launch_bounds__(256) void k_cert_chunks(const SlotView* __restrict__ slots,
                                                      const int* __restrict__ list) {
     const GSlot S = load_slot(slots, list);
@@ -2337,7 +2332,7 @@ __global__ __launch_bounds__(256) void k_cert_chunks(const SlotView* __restrict_
     if ((int)blockIdx.x >= T) return;  // uniform per block
     const int n = S.g.n;
     int64_t cut = 0;
-    uint64_t bad = 0, edges = 0, fp = 0;
+    uint64_t bad = 0, edges = 0;
     for (int task = blockIdx.x; task < T; task += gridDim.x) {
         giant_task(S, task, ng, sh);
         const int k = sh[0], c = sh[1], v = sh[2];
@@ -2346,17 +2341,16 @@ __global__ __launch_bounds__(256) void k_cert_chunks(const SlotView* __restrict_
         const int e1 = min(re, e0 + kGiantChunk);
         const bool inT = S.h[v] < n;
         int64_t ex = 0;
-        for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) cert_entry<MODE>(S, e, inT, n, ex, bad, cut, fp);
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) cert_entry(S, e, inT, n, ex, bad, cut);
         const int64_t tot = block_sum<int64_t>(ex, red64);
         if (threadIdx.x == 0) {
             if (tot) add64(&S.gacc[k], tot);
             edges += (uint64_t)(e1 - e0);
         }
     }
-    cert_flush(S, cut, bad, edges, fp, red64, redu);
+    cert_flush(S, cut, bad, edges, 0, red64, redu);
 }
 
-template <int MODE>
 __global__ __launch_bounds__(256) void k_cert_fin(const SlotView* __restrict__ slots,
                                                   const int* __restrict__ list) {
     const GSlot S = load_slot(slots, list);
@@ -2367,21 +2361,21 @@ __global__ __launch_bounds__(256) void k_cert_fin(const SlotView* __restrict__ s
     uint64_t bad = 0;
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
         const int v = S.giant[g];
-        cert_vertex<MODE>(S, v, ld64(&S.gacc[g]), S.h[v] < n, bad);
+        cert_vertex(S, v, ld64(&S.gacc[g]), S.h[v] < n, bad);
     }
     // every k_cert / k_cert_chunks block has added its part by now (stream order)
-    if (MODE == CERT_FP && blockIdx.x == 0 && threadIdx.x == 0) addu64(&S.ctr->fp, 0ULL - *S.g.cert_fp);
     cert_flush(S, 0, bad, 0, 0, red64, redu);
 }
 
 void launch_cert(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx, int mode) {
-    if (mode == CERT_FULL) hipLaunchKernelGGL(k_cert<CERT_FULL>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
-    else hipLaunchKernelGGL(k_cert_fp, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    if (mode != CERT_FULL) {
+        hipLaunchKernelGGL(k_cert_fp, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        return;
+    }
+    hipLaunchKernelGGL(k_cert, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
     hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
-    if (mode == CERT_FULL) hipLaunchKernelGGL(k_cert_chunks<CERT_FULL>, dim3(1024, nlist), dim3(256), 0, st, slots, list);
-    else hipLaunchKernelGGL(k_cert_chunks<CERT_FP>, dim3(1024, nlist), dim3(256), 0, st, slots, list);
-    if (mode == CERT_FULL) hipLaunchKernelGGL(k_cert_fin<CERT_FULL>, dim3(4, nlist), dim3(256), 0, st, slots, list);
-    else hipLaunchKernelGGL(k_cert_fin<CERT_FP>, dim3(4, nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_cert_chunks, dim3(1024, nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_cert_fin, dim3(4, nlist), dim3(256), 0, st, slots, list);
 }
 
 // ------------------------------------------------------------- prune mask (a6)
