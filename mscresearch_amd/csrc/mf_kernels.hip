@@ -420,19 +420,6 @@ __device__ __forceinline__ GAS int32_t* bfs_lvl(const GSlot& S) {
     return MODE == BFS_UND ? S.aux : S.h;
 }
 
-// May frontier vertex v (entry e: v -> w) discover w?
-template <int MODE>
-__device__ __forceinline__ bool bfs_edge_ok(const GSlot& S, int e, int w) {
-    if (MODE == BFS_GR) {
-        if (w == S.s) return false;
-        if (S.mask && S.mask[w] == 0) return false;
-        return ld64(&S.res[S.g.rev[e]]) > 0;
-    } else if (MODE == BFS_FWD) {
-        return S.g.cap[e] > 0;
-    }
-    return true;
-}
-
 // Bottom-up: may unvisited v join level L+1 through entry e: v -> x where x
 // is on level L?  GR: v can push to x (res[e] > 0).  FWD: x -> v carries
 // capacity, i.e. e is a reverse entry (cap[e] == 0; forward entries always
@@ -451,6 +438,94 @@ __device__ __forceinline__ bool bfs_vertex_ok(const GSlot& S, int v) {
         if (S.mask && S.mask[v] == 0) return false;
     }
     return true;
+}
+
+// May frontier vertex v (entry e: v -> w) discover w?  GR: w != s, w in
+// the mask, and w can push to v (res[rev e] > 0); FWD: e carries capacity;
+// UND: always.  Split in two so that the entry-side load is issued with the
+// column load: aux = edge_aux(e) (GR: rev[e]; FWD: cap[e] > 0), then
+// edge_ok_aux(aux, w).
+template <int MODE>
+__device__ __forceinline__ int edge_aux(const GSlot& S, int e) {
+    if (MODE == BFS_GR) return S.g.rev[e];
+    if (MODE == BFS_FWD) return S.g.cap[e] > 0 ? 1 : 0;
+    return 1;
+}
+
+template <int MODE>
+__device__ __forceinline__ bool edge_ok_aux(const GSlot& S, int aux, int w) {
+    if (MODE == BFS_GR) {
+        if (w == S.s) return false;
+        if (S.mask && S.mask[w] == 0) return false;
+        return ld64(&S.res[aux]) > 0;
+    }
+    return aux != 0;
+}
+
+// Top-down expansion of entries e0, e0 + step, ... below e1 by one thread,
+// U entries at a time: their column and entry-side loads, then their level
+// gathers, then their residual gathers, each issued together (one memory
+// round trip per stage instead of one per entry and stage).  claim(w) runs
+// for each vertex this thread moved from n to level L + 1.
+template <int MODE, int U, typename Claim>
+__device__ __forceinline__ void td_entries(const GSlot& S, int e0, int e1, int step, int L,
+                                           GAS int32_t* lvl, Claim claim) {
+    const int n = S.g.n;
+    for (int e = e0; e < e1; e += U * step) {
+        int w[U], ax[U], l[U];
+        bool ok[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int ek = e + k * step;
+            w[k] = -1;
+            ax[k] = 0;
+            if (ek < e1) {
+                w[k] = S.g.col[ek];
+                ax[k] = edge_aux<MODE>(S, ek);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) l[k] = w[k] >= 0 ? lvl[w[k]] : -1;
+#pragma unroll
+        for (int k = 0; k < U; ++k) ok[k] = l[k] == n && edge_ok_aux<MODE>(S, ax[k], w[k]);
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            if (ok[k] && cas32(&lvl[w[k]], n, L + 1)) claim(w[k]);
+    }
+}
+
+// Bottom-up test of row [rb, re) by one thread, U entries at a time (their
+// residual/capacity and column loads together, then their level gathers):
+// does an entry lead to level L?  `scanned` = entries up to the end of the
+// group holding the first such entry.
+template <int MODE, int U>
+__device__ __forceinline__ bool bu_row_lane(const GSlot& S, int rb, int re, int L, const GAS int32_t* lvl,
+                                            int& scanned) {
+    for (int e = rb; e < re; e += U) {
+        bool ok[U];
+        int c[U], l[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int ek = e + k;
+            ok[k] = false;
+            c[k] = 0;
+            if (ek < re) {
+                ok[k] = bfs_bu_edge_ok<MODE>(S, ek);
+                c[k] = S.g.col[ek];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) l[k] = ok[k] ? lvl[c[k]] : -1;
+        bool hit = false;
+#pragma unroll
+        for (int k = 0; k < U; ++k) hit |= l[k] == L;
+        if (hit) {
+            scanned = min(e + U, re) - rb;
+            return true;
+        }
+    }
+    scanned = re - rb;
+    return false;
 }
 
 // GR seeding (a12): h = n everywhere, level 0 = t plus deficits (phase 0) or
@@ -520,18 +595,11 @@ __device__ __forceinline__ void td_lane(const GSlot& S, int v, int L, GAS int32_
                                         Stage<CAP>& st, GAS int32_t* out, GAS int32_t* ocnt,
                                         GAS int64_t* odeg, Work& wk) {
     const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
-    const int n = S.g.n;
     wk.edges += (uint64_t)(re - rb);
-    for (int e = rb; e < re; ++e) {
-        const int w = S.g.col[e];
-        if (lvl[w] != n) continue;
-        if (!bfs_edge_ok<MODE>(S, e, w)) continue;
-        if (cas32(&lvl[w], n, L + 1)) {
-            const int dw = S.g.row_ptr[w + 1] - S.g.row_ptr[w];
-            stage_push(st, w, dw, out, ocnt, odeg);
-            wk.relabels++;
-        }
-    }
+    td_entries<MODE, 4>(S, rb, re, 1, L, lvl, [&](int w) {
+        stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
+        wk.relabels++;
+    });
 }
 
 // Top-down expansion of one (heavy) frontier vertex by the whole workgroup.
@@ -540,18 +608,11 @@ __device__ __forceinline__ void td_block(const GSlot& S, int v, int L, GAS int32
                                          Stage<CAP>& st, GAS int32_t* out, GAS int32_t* ocnt,
                                          GAS int64_t* odeg, Work& wk) {
     const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
-    const int n = S.g.n;
     if (threadIdx.x == 0) wk.edges += (uint64_t)(re - rb);
-    for (int e = rb + (int)threadIdx.x; e < re; e += blockDim.x) {
-        const int w = S.g.col[e];
-        if (lvl[w] != n) continue;
-        if (!bfs_edge_ok<MODE>(S, e, w)) continue;
-        if (cas32(&lvl[w], n, L + 1)) {
-            const int dw = S.g.row_ptr[w + 1] - S.g.row_ptr[w];
-            stage_push(st, w, dw, out, ocnt, odeg);
-            wk.relabels++;
-        }
-    }
+    td_entries<MODE, 2>(S, rb + (int)threadIdx.x, re, (int)blockDim.x, L, lvl, [&](int w) {
+        stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
+        wk.relabels++;
+    });
 }
 
 template <int MODE>
@@ -599,18 +660,11 @@ __device__ __forceinline__ void td_wave(const GSlot& S, int v, int L, GAS int32_
                                         Stage<CAP>& st, GAS int32_t* out, GAS int32_t* ocnt,
                                         GAS int64_t* odeg, Work& wk) {
     const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
-    const int n = S.g.n;
     if (lane_id() == 0) wk.edges += (uint64_t)(re - rb);
-    for (int e = rb + lane_id(); e < re; e += WAVE) {
-        const int w = S.g.col[e];
-        if (lvl[w] != n) continue;
-        if (!bfs_edge_ok<MODE>(S, e, w)) continue;
-        if (cas32(&lvl[w], n, L + 1)) {
-            const int dw = S.g.row_ptr[w + 1] - S.g.row_ptr[w];
-            stage_push(st, w, dw, out, ocnt, odeg);
-            wk.relabels++;
-        }
-    }
+    td_entries<MODE, 2>(S, rb + lane_id(), re, WAVE, L, lvl, [&](int w) {
+        stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
+        wk.relabels++;
+    });
 }
 
 // Medium rows of a top-down level, a wave each, taken grid-stride from the
@@ -679,15 +733,12 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
             } else if (re - rb > kLightDeg) {
                 med[atomicAdd(&nmed, 1)] = v;
             } else {
-                int e = rb;
-                // the coalesced residual test first: most entries fail it
-                // (reverse entries without flow), saving the level gather
-                for (; e < re; ++e) {
-                    if (!bfs_bu_edge_ok<MODE>(S, e)) continue;
-                    if (lvl[S.g.col[e]] == L) break;
-                }
-                wk.edges += (uint64_t)(min(e + 1, re) - rb);
-                if (e < re) {
+                // the level gather only for entries that pass the residual
+                // test (most reverse entries carry no flow)
+                int scanned = 0;
+                const bool hit = bu_row_lane<MODE, 4>(S, rb, re, L, lvl, scanned);
+                wk.edges += (uint64_t)scanned;
+                if (hit) {
                     lvl[v] = L + 1;  // only this lane writes v
                     stage_push(st, v, re - rb, out, ocnt, odeg);
                     wk.relabels++;
@@ -704,7 +755,11 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
             for (int e0 = rb; e0 < re; e0 += WAVE) {
                 const int e = e0 + lane;
                 bool ok = false;
-                if (e < re) ok = bfs_bu_edge_ok<MODE>(S, e) && (lvl[S.g.col[e]] == L);
+                if (e < re) {
+                    const bool r = bfs_bu_edge_ok<MODE>(S, e);
+                    const int c = S.g.col[e];  // issued with the residual load
+                    ok = r && lvl[c] == L;
+                }
                 scanned += min(WAVE, re - e0);
                 if (__ballot(ok)) { hit = true; break; }
             }
@@ -751,7 +806,11 @@ __global__ __launch_bounds__(256) void k_bfs_bu_heavy(const SlotView* __restrict
         int base = rb;
         for (; base < re; base += blockDim.x) {
             const int e = base + (int)threadIdx.x;
-            if (e < re && bfs_bu_edge_ok<MODE>(S, e) && lvl[S.g.col[e]] == L) found = 1;
+            if (e < re) {
+                const bool r = bfs_bu_edge_ok<MODE>(S, e);
+                const int c = S.g.col[e];  // issued with the residual load
+                if (r && lvl[c] == L) found = 1;
+            }
             __syncthreads();
             const int f = found;
             __syncthreads();
@@ -814,7 +873,6 @@ __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__
     __shared__ int64_t next_deg;
     __shared__ uint64_t red[1024 / WAVE];
     GAS int32_t* lvl = bfs_lvl<MODE>(S);
-    const int n = S.g.n;
     int L = S.ctr->level;
     int fsz = S.ctr->fsize[L & 3];
     Work wk;
@@ -835,17 +893,12 @@ __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__
                 if (k < 1024) { heavy[k] = v; continue; }
             }
             wk.edges += (uint64_t)(re - rb);
-            for (int e = rb; e < re; ++e) {
-                const int w = S.g.col[e];
-                if (lvl[w] != n) continue;
-                if (!bfs_edge_ok<MODE>(S, e, w)) continue;
-                if (cas32(&lvl[w], n, L + 1)) {
-                    const int p = atomicAdd(&next_cnt, 1);
-                    out[p] = w;
-                    add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
-                    wk.relabels++;
-                }
-            }
+            td_entries<MODE, 4>(S, rb, re, 1, L, lvl, [&](int w) {
+                const int p = atomicAdd(&next_cnt, 1);
+                out[p] = w;
+                add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
+                wk.relabels++;
+            });
         }
         __syncthreads();
         const int nh = min(nheavy, 1024);
@@ -853,17 +906,12 @@ __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__
             const int v = heavy[k];
             const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
             if (threadIdx.x == 0) wk.edges += (uint64_t)(re - rb);
-            for (int e = rb + (int)threadIdx.x; e < re; e += blockDim.x) {
-                const int w = S.g.col[e];
-                if (lvl[w] != n) continue;
-                if (!bfs_edge_ok<MODE>(S, e, w)) continue;
-                if (cas32(&lvl[w], n, L + 1)) {
-                    const int p = atomicAdd(&next_cnt, 1);
-                    out[p] = w;
-                    add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
-                    wk.relabels++;
-                }
-            }
+            td_entries<MODE, 2>(S, rb + (int)threadIdx.x, re, (int)blockDim.x, L, lvl, [&](int w) {
+                const int p = atomicAdd(&next_cnt, 1);
+                out[p] = w;
+                add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
+                wk.relabels++;
+            });
         }
         __syncthreads();
         fsz = next_cnt;
@@ -1746,7 +1794,6 @@ __global__ __launch_bounds__(256) void k_bfs_td_giant(const SlotView* __restrict
     const int T = S.ctr->ngchunks;
     if ((int)blockIdx.x >= T) return;  // uniform per block
     const int L = S.ctr->level;
-    const int n = S.g.n;
     GAS int32_t* lvl = bfs_lvl<MODE>(S);
     GAS int32_t* out = (L & 1) ? S.fa : S.fb;
     GAS int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
@@ -1760,15 +1807,10 @@ __global__ __launch_bounds__(256) void k_bfs_td_giant(const SlotView* __restrict
         const int e0 = rb + c * kGiantChunk;
         const int e1 = min(re, e0 + kGiantChunk);
         if (threadIdx.x == 0) wk.edges += (uint64_t)(e1 - e0);
-        for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
-            const int w = S.g.col[e];
-            if (lvl[w] != n) continue;
-            if (!bfs_edge_ok<MODE>(S, e, w)) continue;
-            if (cas32(&lvl[w], n, L + 1)) {
-                stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
-                wk.relabels++;
-            }
-        }
+        td_entries<MODE, 4>(S, e0 + (int)threadIdx.x, e1, (int)blockDim.x, L, lvl, [&](int w) {
+            stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
+            wk.relabels++;
+        });
     }
     stage_flush(st, out, ocnt, odeg);
     const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
