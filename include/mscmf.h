@@ -149,6 +149,7 @@ typedef struct mf_params {
     int fuse;            /* discharge + count cycles per host round trip on deep graphs (1 = off) */
     int graphs;          /* 1 = replay repeated fused cycles as captured HIP graphs */
     int spec_cert;       /* 1 = certify on the discharge's labels before a terminating exact GR */
+    int sparse_count;    /* 1 = count actives from the previous list and the vertices pushed into */
 } mf_params;
 
 void mf_params_default(mf_params* p);

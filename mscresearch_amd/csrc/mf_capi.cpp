@@ -107,6 +107,7 @@ Params from_c(const mf_params* p) {
     P.fuse = p->fuse;
     P.graphs = p->graphs;
     P.spec_cert = p->spec_cert;
+    P.sparse_count = p->sparse_count;
     return P;
 }
 
@@ -422,6 +423,7 @@ void mf_params_default(mf_params* p) {
     p->fuse = P.fuse;
     p->graphs = P.graphs;
     p->spec_cert = P.spec_cert;
+    p->sparse_count = P.sparse_count;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

@@ -263,6 +263,8 @@ void SlotSched::begin(const Params* P, const DGraph& g, int s, int t, bool prune
     spec_failed_ = false;
     stag_ = no_stag_ = 0;
     last_pos_ = INT64_MAX;
+    pos_full_ = ext_full_ = 0;
+    counted_sparse_ = false;
     force_gr_ = false;
     path_stable_ = 0;
     p_count_ = P->p_count0;
@@ -429,6 +431,15 @@ Plan SlotSched::plan(const Ctr& c) {
     case S_COUNTED: {
         const bool exact = just_gr_;
         just_gr_ = false;
+        const bool sparse = counted_sparse_;
+        counted_sparse_ = false;
+        int64_t pos = c.pos_excess;
+        if (sparse) {
+            pos = pos_full_ - (c.ex_t - ext_full_);
+        } else {
+            pos_full_ = c.pos_excess;
+            ext_full_ = c.ex_t;
+        }
         // counts right after a local relabel follow no discharge: they do not
         // feed the stagnation and stall monitors
         const bool was_local = after_local_;
@@ -478,9 +489,9 @@ Plan SlotSched::plan(const Ctr& c) {
         // stagnation monitor on the positive excess sum (BC:1851-1891)
         bool hotspot_triggered = false;
         if (!exact && !was_local && iter_ > 0) {
-            if (last_pos_ != INT64_MAX && c.pos_excess >= last_pos_) ++stag_;
+            if (last_pos_ != INT64_MAX && pos >= last_pos_) ++stag_;
             else stag_ = 0;
-            last_pos_ = c.pos_excess;
+            last_pos_ = pos;
             if (stag_ >= P_->stagnate_rounds) {
                 hotspot_triggered = true;
                 force_gr_ = true;
@@ -509,7 +520,7 @@ Plan SlotSched::plan(const Ctr& c) {
                 }
             }
         } else if (!was_local) {
-            last_pos_ = c.pos_excess;
+            last_pos_ = pos;
             last_ext_ = c.ex_t;
         }
         rm_last_ = false;
@@ -568,6 +579,16 @@ Plan SlotSched::plan(const Ctr& c) {
             p.stages |= ST_RM;
             p.reset_act |= RST_RM;
             rm_last_ = true;
+        }
+        // sparse count: every kernel of this step that pushes records the
+        // vertices it pushes into, so those and the previous active list hold
+        // every vertex that can be active after it
+        if (P_->sparse_count && !use_path && !(p.stages & ST_RM) && phase_ == 0 && c.neg_count == 0) {
+            p.sparse = true;
+            p.stages |= ST_RESET_PRE;
+            p.reset_pre |= RST_TOUCH;
+            p.reset_act = RST_SPARSE;
+            counted_sparse_ = true;
         }
         // deep graphs: several cycles per host round trip, up to the next
         // count-due global relabel
@@ -1138,7 +1159,8 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         for (unsigned w : vals)
             act_groups.push_back({w, make_list([&](int i) { return has(i, ST_RESET_ACT) && plans[i].reset_act == w; })});
     }
-    Group g_act = make_list([&](int i) { return has(i, ST_ACTIVE); });
+    Group g_act = make_list([&](int i) { return has(i, ST_ACTIVE) && !plans[i].sparse; });
+    Group g_act_sp = make_list([&](int i) { return has(i, ST_ACTIVE) && plans[i].sparse; });
     Group g_cert = make_list([&](int i) { return has(i, ST_CERT); });
     // slots of fused steps (see the fused-cycle launches below)
     int max_cycles = 1;
@@ -1279,6 +1301,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     }
     for (auto& ag : act_groups) launch_reset(st, d_views_, L(ag.second), ag.second.cnt, ag.first);
     if (g_act.cnt) { const int k = tb(WF_ACTIVE); launch_active(st, d_views_, L(g_act), g_act.cnt, maxof(g_act, gridN)); te(k); }
+    if (g_act_sp.cnt) { const int k = tb(WF_ACTIVE); launch_active_sparse(st, d_views_, L(g_act_sp), g_act_sp.cnt, 512); te(k); }
     // further discharge + count cycles of fused steps (deep graphs): the
     // count after each only looks at the previous list and the vertices the
     // discharge pushed into.  The slot lists sit at fixed places of cyc_d_

@@ -71,6 +71,10 @@ struct Params {
     // when the discharge leaves no active vertex, certify with its labels
     // (T = {h < n}) before spending an exact global relabel on termination
     int spec_cert = 1;
+    // count the active vertices after a discharge from the previous active
+    // list and the vertices the discharge pushed into, instead of a pass
+    // over every vertex (not after path pushes or RM, which record nothing)
+    int sparse_count = 1;
 };
 
 // ---------------------------------------------------------- device memory
@@ -197,6 +201,7 @@ struct Plan {
     int seed_mode = 0;
     int local_cap = 0;        // ST_LOCAL: level cap of the local relabel
     int cycles = 1;           // ST_DISCHARGE: discharge + count cycles in this step
+    bool sparse = false;      // ST_ACTIVE: count the previous list and the touched vertices only
 };
 
 // Per-query statistics (host side).
@@ -264,6 +269,11 @@ private:
     int GR_FREQ_ = 128, K_ = 4, RM_FREQ_ = 64;
     int stag_ = 0, no_stag_ = 0;
     int64_t last_pos_ = INT64_MAX;
+    // the last full count's positive excess and flow into t; a sparse count
+    // sees only its candidates, and without deficits the positive excess
+    // off s and t is -excess(s) - excess(t), excess(s) fixed in phase 0
+    int64_t pos_full_ = 0, ext_full_ = 0;
+    bool counted_sparse_ = false;
     bool force_gr_ = false;
     int path_stable_ = 0, p_count_ = 64, path_rounds_ = 0;
     int64_t last_ext_ = -1;

@@ -65,6 +65,9 @@ def test_params_defaults_are_the_reference_constants():
     assert (p.hot_k, p.hot_min_active, p.rm_freq0, p.rm_margin, p.iter_max) == (32, 64, 64, 4, 20000000)
     assert p.prune == 1 and abs(p.prune_ratio - 0.995) < 1e-12 and p.cert == 1
     assert (p.local_relabel, p.local_band) == (0, 128)  # band limit of BC:1611; off by default
+    # additions without a reference constant: fused cycles, graph replay,
+    # speculative certificate, sparse counts
+    assert (p.fuse, p.graphs, p.spec_cert, p.sparse_count) == (8, 1, 1, 1)
 
 
 def test_cli_usage_and_errors(tmp_path):

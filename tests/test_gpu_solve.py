@@ -128,6 +128,8 @@ def test_edge_cases():
     dict(local_relabel=1, h_cut=0, rm_freq0=1, rm_freq_max=1),       # ... and RM on every iteration
     dict(cert=2),                               # certificate pair sums entry by entry
     dict(spec_cert=0),                          # terminate through an exact global relabel only
+    dict(sparse_count=0),                       # count actives over every vertex after each discharge
+    dict(sparse_count=1, hot_min_active=1, hot_k=4, tile_frac=0.0),  # sparse counts after tile + hot sweeps
 ])
 def test_scheduler_variants_stay_exact(c1, variant):
     log(f"variant {variant}")
