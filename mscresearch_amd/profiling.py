@@ -30,7 +30,7 @@ FAMILY_OF = {
     "k_giant_push": "discharge", "k_giant_relabel": "discharge", "k_path_push": "discharge",
     "k_hot_select": "discharge",
     "k_rm": "rm",
-    "k_cert": "cert", "k_cert_chunks": "cert", "k_cert_fin": "cert",
+    "k_cert": "cert", "k_cert_fp": "cert", "k_cert_chunks": "cert", "k_cert_fin": "cert",
 }
 
 
