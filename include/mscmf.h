@@ -33,6 +33,7 @@ extern "C" {
 #define MF_ECERT (-6)  /* min-cut certificate failed */
 #define MF_ENODEV (-7)
 #define MF_EBUSY (-8)
+#define MF_ESTALE (-9) /* a binary graph cache no longer matches its text file or region count */
 
 /* kernel families of the work counters and the profile */
 #define MF_FAMILIES 7
@@ -77,6 +78,18 @@ int mf_hgraph_export(const mf_hgraph* h, int32_t* row_ptr, int32_t* col, int64_t
 int mf_hgraph_stats(const mf_hgraph* h, int64_t* sum_out, int64_t* sum_in, int32_t* deg_out,
                     int32_t* deg_in);
 void mf_hgraph_free(mf_hgraph* h);
+
+/* Binary graph cache (no counterpart in the reference, which parses
+ * graph.txt and partitions on every run, BC:1044-1276): the prepared graph
+ * (CSR, stats, partition and id maps) as one file of 64-byte aligned
+ * sections behind a 256-byte header, read back through mmap.
+ * mf_hgraph_save records source_path's size and modification time when it
+ * is not NULL.  mf_hgraph_open returns MF_ESTALE when source_path is given
+ * and differs from the recorded one, or R >= 0 differs from the recorded
+ * region count; MF_EIO on a missing, truncated or corrupted file (a 64-bit
+ * hash of the sections is checked). */
+int mf_hgraph_save(const mf_hgraph* h, const char* path, const char* source_path);
+int mf_hgraph_open(const char* path, const char* source_path, int R, mf_hgraph** out);
 
 /* Reachability classes behind the per-query pruning mask (BC:1391-1445):
  * scc[n] = strongly connected component over entries with cap > 0, wcc[n] =
@@ -299,6 +312,11 @@ int mf_dbg_local_relabel(mf_engine* e, int gid, int s, int t, const int64_t* res
 int mf_dbg_certify(mf_engine* e, int gid, int s, int t, const int64_t* res, const int64_t* excess,
                    const int32_t* height, int mode, int64_t* bad, int64_t* cut, int64_t* excess_t,
                    int* fp_zero);
+/* Hot-region choice of the hot sweep (BC:1931-1946: the k regions with the
+ * most active vertices) on given per-tile active counts (ntiles of the
+ * graph): tile_hot[i] = 1 for the k tiles first in (count descending, index
+ * ascending) order among tiles with a nonzero count. */
+int mf_dbg_hot_select(mf_engine* e, int gid, const int32_t* tile_cnt, int k, int32_t* tile_hot);
 /* Solve without pruning on slot 0 and copy the final residual, excess and
  * heights out (for an independent CPU certificate). */
 int mf_dbg_solve_state(mf_engine* e, int gid, int s, int t, mf_result* r, int64_t* res,

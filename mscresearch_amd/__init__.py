@@ -21,7 +21,8 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libmscmf.so")
 CLI_PATH = os.path.join(_PKG, "bin", "mscmf")
 
-MF_OK, MF_EINVAL, MF_ENOMEM, MF_EHIP, MF_EIO, MF_EITER, MF_ECERT, MF_ENODEV, MF_EBUSY = 0, -1, -2, -3, -4, -5, -6, -7, -8
+MF_OK, MF_EINVAL, MF_ENOMEM, MF_EHIP, MF_EIO, MF_EITER, MF_ECERT, MF_ENODEV, MF_EBUSY, MF_ESTALE = (
+    0, -1, -2, -3, -4, -5, -6, -7, -8, -9)
 
 _lib = None
 
@@ -84,6 +85,8 @@ SIGNATURES = {
     "mf_hgraph_from_edges": (C.c_int, [C.c_int, C.c_int64, _P, _P, _P, C.POINTER(_P)]),
     "mf_hgraph_from_edgelist": (C.c_int, [_P, C.POINTER(_P)]),
     "mf_hgraph_partition": (C.c_int, [_P, C.c_int]),
+    "mf_hgraph_save": (C.c_int, [_P, C.c_char_p, C.c_char_p]),
+    "mf_hgraph_open": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(_P)]),
     "mf_hgraph_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
     "mf_hgraph_export": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "mf_hgraph_stats": (C.c_int, [_P, _P, _P, _P, _P]),
@@ -133,6 +136,7 @@ SIGNATURES = {
     "mf_dbg_local_relabel": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, C.c_int, _P]),
     "mf_dbg_certify": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, C.c_int, _P, _P, _P, _P]),
     "mf_dbg_prune_mask": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.POINTER(C.c_int)]),
+    "mf_dbg_hot_select": (C.c_int, [_P, C.c_int, _P, C.c_int, _P]),
     "mf_dbg_induced": (C.c_int, [_P, C.c_int, _P, C.POINTER(C.c_int), C.POINTER(C.c_int64), _P, _P, _P, _P, _P]),
     "mf_dbg_solve_state": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(Result), _P, _P, _P]),
 }
@@ -328,6 +332,19 @@ class HostGraph:
         _check(lib().mf_hgraph_from_edgelist(e._h, C.byref(h)))
         return HostGraph(h)
 
+    @staticmethod
+    def open(path, source=None, R=-1):
+        """Load a binary graph cache; MaxFlowError(MF_ESTALE) when `source`
+        (the text file) or `R` no longer match it."""
+        h = _P()
+        _check(lib().mf_hgraph_open(str(path).encode(), None if source is None else str(source).encode(), int(R),
+                                    C.byref(h)))
+        return HostGraph(h)
+
+    def save(self, path, source=None):
+        _check(lib().mf_hgraph_save(self._h, str(path).encode(), None if source is None else str(source).encode()))
+        return self
+
     def partition(self, R):
         _check(lib().mf_hgraph_partition(self._h, R))
         return self
@@ -514,6 +531,13 @@ class Engine:
         _check(lib().mf_dbg_local_relabel(self._h, gid, int(s), int(t), _ptr(res), _ptr(excess), _ptr(h),
                                           _ptr(ta), int(lcap), C.byref(lvl)))
         return h, lvl.value
+
+    def dbg_hot_select(self, gid, tile_cnt, k):
+        """The hot sweep's choice of k tiles on given per-tile active counts."""
+        tile_cnt = np.ascontiguousarray(tile_cnt, np.int32)
+        hot = np.empty(len(tile_cnt), np.int32)
+        _check(lib().mf_dbg_hot_select(self._h, gid, _ptr(tile_cnt), int(k), _ptr(hot)))
+        return hot
 
     def dbg_certify(self, gid, s, t, res, excess, h, mode):
         """GPU certificate of a given state: (bad, cut, excess_t, fingerprint_zero)."""

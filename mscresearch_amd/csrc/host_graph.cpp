@@ -15,6 +15,7 @@
 #include <numeric>
 #include <queue>
 #include <thread>
+#include <type_traits>
 
 namespace mf {
 
@@ -533,6 +534,209 @@ int write_query_file(const std::string& path, const std::vector<int32_t>& s,
     if (!f) return -1;
     for (size_t i = 0; i < s.size(); ++i) fprintf(f, "%d %d\n", s[i] + 1, t[i] + 1);
     fclose(f);
+    return 0;
+}
+
+// ------------------------------------------------------- binary graph cache
+
+namespace {
+
+constexpr char kBinMagic[8] = {'M', 'S', 'C', 'M', 'F', 'G', 'R', '1'};
+constexpr int kBinSections = 12;
+
+struct BinHeader {
+    char magic[8];
+    uint32_t version;
+    uint32_t header_bytes;
+    int32_t n, R;
+    int64_t E, lines;
+    int64_t src_size, src_mtime_ns;  // -1 when no source file was named
+    uint64_t hash;                   // of every section's bytes, in order
+    int64_t off[kBinSections], len[kBinSections];  // bytes
+};
+static_assert(sizeof(BinHeader) == 256, "binary graph header is 256 bytes");
+
+// Four independent multiply-xorshift lanes over 8-byte words, then a final
+// mix; the tail shorter than 32 bytes goes byte by byte.  Chained across
+// sections through the seed.
+uint64_t hash_bytes(const void* p, size_t nbytes, uint64_t seed) {
+    const uint64_t K = 0x9fb21c651e98df25ULL;
+    uint64_t a[4] = {seed ^ 0x243f6a8885a308d3ULL, seed ^ 0x13198a2e03707344ULL, seed ^ 0xa4093822299f31d0ULL,
+                     seed ^ 0x082efa98ec4e6c89ULL};
+    const unsigned char* b = (const unsigned char*)p;
+    size_t i = 0;
+    for (; i + 32 <= nbytes; i += 32)
+        for (int k = 0; k < 4; ++k) {
+            uint64_t w;
+            memcpy(&w, b + i + 8 * k, 8);
+            a[k] = (a[k] ^ w) * K;
+            a[k] ^= a[k] >> 29;
+        }
+    uint64_t h = a[0] ^ (a[1] * 3) ^ (a[2] * 5) ^ (a[3] * 7) ^ (uint64_t)nbytes;
+    for (; i < nbytes; ++i) h = (h ^ b[i]) * 0x100000001b3ULL;
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdULL;
+    h ^= h >> 33;
+    return h;
+}
+
+bool source_identity(const std::string& src, int64_t& size, int64_t& mtime_ns) {
+    struct stat st;
+    if (src.empty() || stat(src.c_str(), &st) != 0) return false;
+    size = (int64_t)st.st_size;
+    mtime_ns = (int64_t)st.st_mtim.tv_sec * 1000000000LL + (int64_t)st.st_mtim.tv_nsec;
+    return true;
+}
+
+template <typename T>
+void section(std::vector<std::pair<const void*, size_t>>& out, const std::vector<T>& v) {
+    out.push_back({v.data(), v.size() * sizeof(T)});
+}
+
+}  // namespace
+
+int save_graph_binary(const HostGraph& g, const std::string& path, const std::string& source,
+                      std::string& err) {
+    std::vector<std::pair<const void*, size_t>> secs;
+    section(secs, g.row_ptr);
+    section(secs, g.col);
+    section(secs, g.cap);
+    section(secs, g.rev);
+    section(secs, g.sum_out);
+    section(secs, g.sum_in);
+    section(secs, g.deg_out);
+    section(secs, g.deg_in);
+    section(secs, g.new_id);
+    section(secs, g.old_id);
+    section(secs, g.rstart);
+    section(secs, g.region_id);
+    BinHeader hd;
+    memset(&hd, 0, sizeof(hd));
+    memcpy(hd.magic, kBinMagic, 8);
+    hd.version = 1;
+    hd.header_bytes = sizeof(BinHeader);
+    hd.n = g.n;
+    hd.R = g.R;
+    hd.E = g.E();
+    hd.lines = g.lines;
+    hd.src_size = hd.src_mtime_ns = -1;
+    if (!source.empty() && !source_identity(source, hd.src_size, hd.src_mtime_ns)) {
+        err = "cannot stat " + source;
+        return -1;
+    }
+    int64_t off = sizeof(BinHeader);
+    uint64_t h = 0;
+    for (int k = 0; k < kBinSections; ++k) {
+        hd.off[k] = off;
+        hd.len[k] = (int64_t)secs[k].second;
+        off += (hd.len[k] + 63) & ~(int64_t)63;
+        h = hash_bytes(secs[k].first, secs[k].second, h);
+    }
+    hd.hash = h;
+    // write beside the target, then rename: a reader never sees half a file
+    const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+    FILE* f = fopen(tmp.c_str(), "wb");
+    if (!f) {
+        err = "cannot write " + tmp;
+        return -1;
+    }
+    static const char zeros[64] = {0};
+    bool ok = fwrite(&hd, sizeof(hd), 1, f) == 1;
+    for (int k = 0; k < kBinSections && ok; ++k) {
+        if (secs[k].second) ok = fwrite(secs[k].first, 1, secs[k].second, f) == secs[k].second;
+        const size_t padb = (size_t)(((hd.len[k] + 63) & ~(int64_t)63) - hd.len[k]);
+        if (ok && padb) ok = fwrite(zeros, 1, padb, f) == padb;
+    }
+    ok = (fclose(f) == 0) && ok;
+    if (!ok || rename(tmp.c_str(), path.c_str()) != 0) {
+        unlink(tmp.c_str());
+        err = "cannot write " + path;
+        return -1;
+    }
+    return 0;
+}
+
+int load_graph_binary(const std::string& path, const std::string& source, int R, HostGraph& g,
+                      std::string& err) {
+    const int fd = open(path.c_str(), O_RDONLY);
+    if (fd < 0) {
+        err = "cannot open " + path;
+        return -1;
+    }
+    struct stat st;
+    if (fstat(fd, &st) != 0 || st.st_size < (off_t)sizeof(BinHeader)) {
+        close(fd);
+        err = "truncated graph cache " + path;
+        return -1;
+    }
+    const size_t fsize = (size_t)st.st_size;
+    void* base = mmap(nullptr, fsize, PROT_READ, MAP_PRIVATE, fd, 0);
+    close(fd);
+    if (base == MAP_FAILED) {
+        err = "cannot map " + path;
+        return -1;
+    }
+    struct Unmap {
+        void* p;
+        size_t n;
+        ~Unmap() { munmap(p, n); }
+    } um{base, fsize};
+    BinHeader hd;
+    memcpy(&hd, base, sizeof(hd));
+    if (memcmp(hd.magic, kBinMagic, 8) != 0 || hd.version != 1 || hd.header_bytes != sizeof(BinHeader) ||
+        hd.n < 0 || hd.E < 0 || hd.R < 0) {
+        err = "not a graph cache: " + path;
+        return -1;
+    }
+    const int64_t n = hd.n, E = hd.E, Rg = hd.R;
+    const int64_t want[kBinSections] = {
+        4 * (n + 1), 4 * E, 8 * E, 4 * E, 8 * n, 8 * n, 4 * n, 4 * n,
+        Rg ? 4 * n : 0, Rg ? 4 * n : 0, Rg ? 4 * (Rg + 1) : 0, Rg ? 4 * n : 0};
+    for (int k = 0; k < kBinSections; ++k)
+        if ((hd.len[k] != want[k] && !(k >= 4 && k <= 7 && hd.len[k] == 0)) ||  // stats may be absent
+            hd.off[k] < (int64_t)sizeof(BinHeader) || hd.off[k] % 64 != 0 ||
+            hd.off[k] + hd.len[k] > (int64_t)fsize) {
+            err = "truncated or inconsistent graph cache " + path;
+            return -1;
+        }
+    if (!source.empty()) {
+        int64_t sz = 0, mt = 0;
+        if (!source_identity(source, sz, mt) || sz != hd.src_size || mt != hd.src_mtime_ns) return 1;
+    }
+    if (R >= 0 && R != hd.R) return 1;
+    const char* b = (const char*)base;
+    uint64_t h = 0;
+    for (int k = 0; k < kBinSections; ++k) h = hash_bytes(b + hd.off[k], (size_t)hd.len[k], h);
+    if (h != hd.hash) {
+        err = "graph cache hash mismatch: " + path;
+        return -1;
+    }
+    auto take = [&](int k, auto& v) {
+        using T = typename std::decay_t<decltype(v)>::value_type;
+        const T* p = (const T*)(b + hd.off[k]);
+        v.assign(p, p + hd.len[k] / (int64_t)sizeof(T));
+    };
+    HostGraph out;
+    out.n = hd.n;
+    out.lines = hd.lines;
+    out.R = hd.R;
+    take(0, out.row_ptr);
+    take(1, out.col);
+    take(2, out.cap);
+    take(3, out.rev);
+    take(4, out.sum_out);
+    take(5, out.sum_in);
+    take(6, out.deg_out);
+    take(7, out.deg_in);
+    take(8, out.new_id);
+    take(9, out.old_id);
+    take(10, out.rstart);
+    take(11, out.region_id);
+    if (out.row_ptr.empty() || out.row_ptr[0] != 0 || out.row_ptr[n] != (int32_t)E) {
+        err = "inconsistent graph cache " + path;
+        return -1;
+    }
+    g = std::move(out);
     return 0;
 }
 

@@ -92,6 +92,19 @@ void gen_queries_hub(int n, const std::vector<int32_t>& us, const std::vector<in
                      double top_frac, int q, uint64_t seed, std::vector<int32_t>& s,
                      std::vector<int32_t>& t);
 
+// ------------------------------------------------------- binary graph cache
+// The prepared graph (CSR, stats, partition) as one file: a 256-byte header,
+// then 64-byte aligned sections read back through mmap.  The header records
+// the text file it was built from (size, modification time in ns) and a
+// 64-bit hash of the sections.  save_graph_binary: 0 or -1 (err).
+// load_graph_binary: 0; 1 when `source` is given and its size or time differ,
+// or R >= 0 differs from the recorded region count (stale); -1 on a missing,
+// truncated or corrupted file (err).
+int save_graph_binary(const HostGraph& g, const std::string& path, const std::string& source,
+                      std::string& err);
+int load_graph_binary(const std::string& path, const std::string& source, int R, HostGraph& g,
+                      std::string& err);
+
 int write_graph_file(const std::string& path, int n, const std::vector<int32_t>& us,
                      const std::vector<int32_t>& vs, const std::vector<int64_t>& caps);
 int write_query_file(const std::string& path, const std::vector<int32_t>& s,

@@ -175,6 +175,29 @@ int mf_hgraph_from_edgelist(const mf_edges* e, mf_hgraph** out) {
     return MF_OK;
 }
 
+int mf_hgraph_save(const mf_hgraph* h, const char* path, const char* source_path) {
+    if (!h || !path) return fail(MF_EINVAL, "null argument");
+    std::string err;
+    if (save_graph_binary(h->g, path, source_path ? source_path : "", err) != 0) return fail(MF_EIO, err);
+    return MF_OK;
+}
+
+int mf_hgraph_open(const char* path, const char* source_path, int R, mf_hgraph** out) {
+    if (!path || !out) return fail(MF_EINVAL, "null argument");
+    *out = nullptr;
+    auto* h = new (std::nothrow) mf_hgraph;
+    if (!h) return fail(MF_ENOMEM, "out of host memory");
+    std::string err;
+    const int rc = load_graph_binary(path, source_path ? source_path : "", R, h->g, err);
+    if (rc != 0) {
+        delete h;
+        if (rc > 0) return fail(MF_ESTALE, std::string("graph cache is stale: ") + path);
+        return fail(MF_EIO, err);
+    }
+    *out = h;
+    return MF_OK;
+}
+
 int mf_hgraph_partition(mf_hgraph* h, int R) {
     if (!h || R < 1) return fail(MF_EINVAL, "bad argument");
     if (!h->g.rstart.empty()) return fail(MF_EINVAL, "graph already partitioned");
@@ -812,6 +835,28 @@ int mf_dbg_certify(mf_engine* e, int gid, int s, int t, const int64_t* res, cons
         *excess_t = c.ex_t;
         *fp_zero = c.fp == 0 ? 1 : 0;
     }
+    e->eng.dbg_end();
+    return rc;
+}
+
+int mf_dbg_hot_select(mf_engine* e, int gid, const int32_t* tile_cnt, int k, int32_t* tile_hot) {
+    if (!e || !tile_cnt || !tile_hot || k < 1) return fail(MF_EINVAL, "bad argument");
+    int rc = check_query(e, gid, 0, 1);
+    if (rc != MF_OK) return rc;
+    rc = e->eng.dbg_begin(gid, 0, 1);
+    if (rc != MF_OK) return rc;
+    SlotView& v = e->eng.view(0);
+    const DGraph& g = v.g;
+    // no excess anywhere: the hot sweep's tile pass after the choice moves nothing
+    HIPCK(hipMemset(v.excess, 0, sizeof(int64_t) * g.n));
+    HIPCK(hipMemcpy(v.tile_cnt, tile_cnt, sizeof(int32_t) * g.ntiles, hipMemcpyHostToDevice));
+    const int saved = e->eng.params().hot_k;
+    e->eng.params_mut().hot_k = k;
+    Plan p;
+    p.stages = ST_HOT;
+    rc = e->eng.dbg_plan(p);
+    e->eng.params_mut().hot_k = saved;
+    if (rc == MF_OK) HIPCK(hipMemcpy(tile_hot, v.tile_hot, sizeof(int32_t) * g.ntiles, hipMemcpyDeviceToHost));
     e->eng.dbg_end();
     return rc;
 }

@@ -11,7 +11,9 @@
 // <out>/res_dynamic_rounds.txt ("round query flow time_ms").
 // Environment: MSCMF_R (regions, default 128 as BC:2583), MSCMF_SLOTS
 // (concurrent queries, default 1 = one query at a time like the reference),
-// MSCMF_TILE (tile size), MSCMF_PRUNE (0/1).
+// MSCMF_TILE (tile size), MSCMF_PRUNE (0/1), MSCMF_GRAPH_CACHE (path of a
+// binary cache of the prepared graph: loaded while it matches graph.txt and
+// MSCMF_R, else rebuilt and written).
 #include <sys/stat.h>
 
 #include <algorithm>
@@ -68,12 +70,30 @@ int main(int argc, char** argv) {
     mkdir(out.c_str(), 0755);  // errors ignored, as BC:2555-2557
 
     mf_hgraph* h = nullptr;
-    if (mf_hgraph_load((in + "/graph.txt").c_str(), &h) != MF_OK) {
-        fprintf(stderr, "%s\n", mf_last_error());
-        exit(1);
-    }
     const int R = env_int("MSCMF_R", 128);
-    mf_hgraph_partition(h, R);
+    const std::string gpath = in + "/graph.txt";
+    // MSCMF_GRAPH_CACHE: binary cache of the prepared graph, used while it
+    // matches graph.txt (size, modification time) and MSCMF_R, else rebuilt
+    const char* cache = getenv("MSCMF_GRAPH_CACHE");
+    if (cache && *cache) {
+        const auto t0 = std::chrono::steady_clock::now();
+        if (mf_hgraph_open(cache, gpath.c_str(), R, &h) == MF_OK)
+            fprintf(stderr, "[GraphCache] loaded %s in %.0f ms\n", cache,
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        else
+            fprintf(stderr, "[GraphCache] %s; rebuilding from %s\n", mf_last_error(), gpath.c_str());
+    }
+    if (!h) {
+        if (mf_hgraph_load(gpath.c_str(), &h) != MF_OK) {
+            fprintf(stderr, "%s\n", mf_last_error());
+            exit(1);
+        }
+        mf_hgraph_partition(h, R);
+        if (cache && *cache) {
+            if (mf_hgraph_save(h, cache, gpath.c_str()) == MF_OK) fprintf(stderr, "[GraphCache] wrote %s\n", cache);
+            else fprintf(stderr, "[GraphCache] not written: %s\n", mf_last_error());
+        }
+    }
     int n = 0, Rg = 0;
     int64_t E = 0, lines = 0;
     mf_hgraph_info(h, &n, &E, &Rg, &lines);

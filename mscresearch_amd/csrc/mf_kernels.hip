@@ -69,6 +69,10 @@ __device__ __forceinline__ int wave_max_i32(int x) {
     for (int o = WAVE / 2; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, WAVE));
     return x;
 }
+__device__ __forceinline__ int wave_sum_i32(int x) {
+    for (int o = WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, WAVE);
+    return x;
+}
 __device__ __forceinline__ int64_t wave_sum_i64(int64_t x) {
     for (int o = WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, WAVE);
     return x;
@@ -2019,15 +2023,30 @@ void launch_path_push(hipStream_t st, const SlotView* slots, const int* list, in
 
 __global__ __launch_bounds__(1024) void k_hot_select(const SlotView* __restrict__ slots,
                                                      const int* __restrict__ list, int k) {
+    constexpr int kLds = 8192;  // tile counts kept in LDS up to this many tiles
     const GSlot S = load_slot(slots, list);
     const int T = S.g.ntiles;
-    // Threshold search: count tiles with cnt > x by bisection on x.
-    __shared__ int cnt_gt;
+    __shared__ int lc[kLds];
+    __shared__ int cnt_gt, maxc;
     __shared__ int lo_s, hi_s;
-    if (threadIdx.x == 0) { lo_s = 0; hi_s = S.g.n; }
+    const bool in_lds = T <= kLds;
+    auto tc = [&](int i) { return in_lds ? lc[i] : S.tile_cnt[i]; };
+    if (threadIdx.x == 0) maxc = 0;
     __syncthreads();
-    // smallest x with |{cnt > x}| < k  ->  tiles with cnt > x are hot, ties at
-    // cnt == x filled by lowest index.
+    int m = 0;
+    for (int i = threadIdx.x; i < T; i += blockDim.x) {
+        const int c = S.tile_cnt[i];
+        if (in_lds) lc[i] = c;
+        m = max(m, c);
+    }
+    m = wave_max_i32(m);
+    if (lane_id() == 0) atomicMax(&maxc, m);
+    __syncthreads();
+    if (threadIdx.x == 0) { lo_s = 0; hi_s = maxc; }
+    __syncthreads();
+    // Threshold search: smallest x with |{cnt > x}| < k (at most the largest
+    // count) by bisection -> tiles with cnt > x are hot, ties at cnt == x
+    // filled by lowest index.
     while (true) {
         const int lo = lo_s, hi = hi_s;
         if (lo >= hi) break;
@@ -2035,8 +2054,9 @@ __global__ __launch_bounds__(1024) void k_hot_select(const SlotView* __restrict_
         if (threadIdx.x == 0) cnt_gt = 0;
         __syncthreads();
         int c = 0;
-        for (int i = threadIdx.x; i < T; i += blockDim.x) c += (S.tile_cnt[i] > mid) ? 1 : 0;
-        atomicAdd(&cnt_gt, c);
+        for (int i = threadIdx.x; i < T; i += blockDim.x) c += (tc(i) > mid) ? 1 : 0;
+        c = wave_sum_i32(c);
+        if (lane_id() == 0 && c) atomicAdd(&cnt_gt, c);
         __syncthreads();
         if (threadIdx.x == 0) {
             if (cnt_gt < k) hi_s = mid; else lo_s = mid + 1;
@@ -2048,8 +2068,9 @@ __global__ __launch_bounds__(1024) void k_hot_select(const SlotView* __restrict_
     if (threadIdx.x == 0) { above = 0; ties = 0; }
     __syncthreads();
     int c = 0;
-    for (int i = threadIdx.x; i < T; i += blockDim.x) c += (S.tile_cnt[i] > x) ? 1 : 0;
-    atomicAdd(&above, c);
+    for (int i = threadIdx.x; i < T; i += blockDim.x) c += (tc(i) > x) ? 1 : 0;
+    c = wave_sum_i32(c);
+    if (lane_id() == 0 && c) atomicAdd(&above, c);
     __syncthreads();
     const int room = k - above;
     // Ties at cnt == x (x >= 1): lowest indices first; serial over chunks keeps order.
@@ -2058,21 +2079,21 @@ __global__ __launch_bounds__(1024) void k_hot_select(const SlotView* __restrict_
         int hot = 0;
         bool tie = false;
         if (i < T) {
-            const int ci = S.tile_cnt[i];
+            const int ci = tc(i);
             hot = (ci > x) ? 1 : 0;
             tie = (ci == x && x >= 1);
         }
         // rank ties within the chunk in index order
         __shared__ int tie_base;
         __shared__ int wave_ties[1024 / WAVE];
-        const unsigned long long m = __ballot(tie);
+        const unsigned long long mk = __ballot(tie);
         const int w = threadIdx.x / WAVE;
-        if (lane_id() == 0) wave_ties[w] = __popcll(m);
+        if (lane_id() == 0) wave_ties[w] = __popcll(mk);
         if (threadIdx.x == 0) tie_base = ties;
         __syncthreads();
         int before = tie_base;
         for (int j = 0; j < w; ++j) before += wave_ties[j];
-        before += __popcll(m & ((1ull << lane_id()) - 1ull));
+        before += __popcll(mk & ((1ull << lane_id()) - 1ull));
         if (tie && before < room) hot = 1;
         if (i < T) S.tile_hot[i] = hot;
         __syncthreads();

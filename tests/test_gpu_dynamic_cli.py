@@ -142,6 +142,29 @@ def test_cli_static_matches_oracle(tmp_path, slots):
     assert "Total time:" in r.stderr
 
 
+def test_cli_static_graph_cache(tmp_path):
+    """MSCMF_GRAPH_CACHE: the first run writes the prepared graph, the next
+    reads it back; both answer like the oracle.  A rewritten graph.txt makes
+    the cache stale: the run rebuilds it from the text and answers for the
+    new graph."""
+    cache = tmp_path / "graph.mfg"
+    env = {"MSCMF_GRAPH_CACHE": str(cache), "MSCMF_R": "64"}
+    for seed, expect in ((5301, "[GraphCache] wrote"), (5301, "[GraphCache] loaded"), (5311, "graph cache is stale")):
+        e = M.Edges.rmat(11, 10**6, seed)
+        qs, qt = e.hub_queries(6, seed + 1)
+        if "loaded" not in expect:
+            write_inputs(tmp_path, e, qs, qt)
+        log(f"cli with graph cache: seed {seed}, expect {expect}")
+        r, out = run_cli(tmp_path, "static", env)
+        assert r.returncode == 0, r.stderr[-2000:]
+        assert expect in r.stderr, r.stderr[-2000:]
+        n, us, vs, cs = O.parse_graph(str(tmp_path / "in" / "graph.txt"))
+        og = O.build_csr(n, us, vs, cs)
+        lines = (out / "res_mt.txt").read_text().splitlines()
+        assert [int(l.split()[0]) for l in lines] == [O.maxflow(og, a, b) for a, b in zip(qs, qt)]
+    assert cache.exists()
+
+
 def test_cli_dynamic_with_updates(tmp_path):
     rng = np.random.default_rng(3)
     e = M.Edges.rmat(11, 10**6, 5201)

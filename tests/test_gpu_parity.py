@@ -204,3 +204,27 @@ def test_local_relabel_bit_exact(which, request):
                     assert np.array_equal(h1[m], np.maximum(h0[m], np.minimum(lo[m], lcap)))
                     if act == "all":  # one region: the exact global relabel, capped
                         assert np.array_equal(h1[m], np.minimum(hgr[m], lcap))
+
+
+@pytest.mark.parametrize("n,tile_max", [(4096, 64), (20000, 2)])
+def test_hot_select_is_the_sorted_choice(n, tile_max):
+    """The hot sweep's region choice (BC:1931-1946 sorts the region counts
+    and takes the top 32) equals the first k tiles in (count descending,
+    index ascending) order among nonzero counts; the bisection kernel must
+    give exactly that set, ties included.  The second shape has more tiles
+    than the kernel keeps in LDS."""
+    case = Case(M.Edges.uniform(n, 4 * n, 100, 1401), R=16, tile_max=tile_max)
+    eng, gid = engine_with(case)
+    T = case.dg.info()["ntiles"]
+    log(f"hot select: {T} tiles")
+    rng = np.random.default_rng(1402)
+    trials = [np.zeros(T, np.int64), np.full(T, 3)]
+    trials += [rng.integers(0, 6, T) * (rng.random(T) < 0.5) for _ in range(3)]
+    trials += [rng.integers(0, 4097, T) for _ in range(2)]
+    for cnt in trials:
+        order = sorted(np.nonzero(cnt)[0].tolist(), key=lambda i: (-int(cnt[i]), i))
+        for k in (1, 4, 32, T + 3):
+            want = np.zeros(T, np.int32)
+            want[order[:k]] = 1
+            got = eng.dbg_hot_select(gid, cnt, k)
+            assert np.array_equal(got, want), (k, int(cnt.max()))
