@@ -243,6 +243,40 @@ int mf_engine_graph_stats(const mf_engine* e, int64_t* captures, int64_t* replay
 int mf_engine_profile(mf_engine* e, int on);
 int mf_engine_profile_read(const mf_engine* e, double* ms, int64_t* launches, uint64_t* work);
 
+/* Per-step log, the information behind the reference's stderr lines
+ * ("[Init] kernel_1(...)=... ms, kernel_7(...)=... ms" BC:2188, the
+ * per-iteration timing line BC:2106): after every engine step the callback
+ * gets one record per slot that ran something.  ms[] are HIP-event times of
+ * the step's stages for all slots of the step, while profiling is on
+ * (mf_engine_profile), else 0. */
+#define MF_STEP_INIT 1u      /* residual initialisation (BC:151-170) */
+#define MF_STEP_PREFLOW 2u   /* preflow from s (BC:173-200) */
+#define MF_STEP_SEED 4u      /* relabel seeding (BC:203-233) */
+#define MF_STEP_BFS 8u       /* relabel BFS levels (BC:322-348) */
+#define MF_STEP_DISCHARGE 16u
+#define MF_STEP_HOT 32u      /* hot-region sweep (BC:1898-1984) */
+#define MF_STEP_RM 64u       /* RM repair (BC:605-657) */
+#define MF_STEP_CERT 128u    /* min-cut certificate */
+#define MF_STEP_PRUNE 256u   /* reachability pruning BFS (BC:1391-1445) */
+#define MF_STEP_COUNT 512u   /* active count */
+enum { MF_LOG_INIT = 0, MF_LOG_PREFLOW, MF_LOG_SEED, MF_LOG_BFS, MF_LOG_PR, MF_LOG_RM, MF_LOG_CERT, MF_LOG_FIELDS };
+typedef struct {
+    int64_t tag;           /* the slot's query */
+    int32_t slot;
+    uint32_t steps;        /* MF_STEP_* bits of what the slot ran */
+    int64_t iters;         /* discharge iterations of the query so far */
+    int32_t active, active_high;  /* after the step's count (MF_STEP_COUNT) */
+    int64_t fixed_edges;   /* RM repairs of the step (MF_STEP_RM) */
+    int32_t K, gr_freq;    /* the scheduler's current K and GR_FREQ */
+    int32_t path;          /* 1 = the discharge was a path push */
+    int32_t cycles;        /* discharge + count cycles of the step */
+    double ms[MF_LOG_FIELDS];  /* init, preflow, seeding, BFS (incl. pruning), push-relabel (discharge, hot
+                                  sweep, count), RM, certificate */
+} mf_step_log;
+typedef void (*mf_step_fn)(void* user, const mf_step_log* log);
+/* fn = NULL turns the log off. */
+int mf_engine_set_step_log(mf_engine* e, mf_step_fn fn, void* user);
+
 /* ------------------------------------------------------------- autotuner
  * The reference uses R = 128, K = max(4, E/V) and GR_FREQ0 = 128 for every
  * graph (BC:2583, BC:1465-1466).  mf_autotune times the plain solver on

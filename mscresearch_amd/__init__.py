@@ -48,6 +48,21 @@ class Params(C.Structure):
     ]
 
 
+class StepLog(C.Structure):
+    """mf_step_log: one slot's engine step (MF_STEP_* bits, counters, the
+    step's stage times init / preflow / seeding / BFS / push-relabel / RM /
+    certificate in ms while profiling)."""
+    _fields_ = [("tag", C.c_int64), ("slot", C.c_int32), ("steps", C.c_uint32), ("iters", C.c_int64),
+                ("active", C.c_int32), ("active_high", C.c_int32), ("fixed_edges", C.c_int64),
+                ("K", C.c_int32), ("gr_freq", C.c_int32), ("path", C.c_int32), ("cycles", C.c_int32),
+                ("ms", C.c_double * 7)]
+
+
+STEP_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(StepLog))
+MF_STEP_INIT, MF_STEP_PREFLOW, MF_STEP_SEED, MF_STEP_BFS, MF_STEP_DISCHARGE = 1, 2, 4, 8, 16
+MF_STEP_HOT, MF_STEP_RM, MF_STEP_CERT, MF_STEP_PRUNE, MF_STEP_COUNT = 32, 64, 128, 256, 512
+
+
 class TuneResult(C.Structure):
     _fields_ = [("R", C.c_int), ("tile_max", C.c_int), ("K", C.c_int), ("gr_freq", C.c_int),
                 ("ms_per_query", C.c_double), ("default_ms_per_query", C.c_double), ("evaluations", C.c_int),
@@ -125,6 +140,7 @@ SIGNATURES = {
     "mf_engine_now": (C.c_double, [_P]),
     "mf_engine_cache_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mf_engine_profile": (C.c_int, [_P, C.c_int]),
+    "mf_engine_set_step_log": (C.c_int, [_P, STEP_FN, C.c_void_p]),
     "mf_engine_graph_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mf_autotune": (C.c_int, [_P, C.c_int, _P, _P, C.POINTER(Params), C.c_int, C.c_int, C.c_char_p, C.POINTER(TuneResult)]),
     "mf_engine_profile_read": (C.c_int, [_P, _P, _P, _P]),
@@ -485,6 +501,12 @@ class Engine:
 
     def profile(self, on: bool):
         _check(lib().mf_engine_profile(self._h, 1 if on else 0))
+
+    def set_step_log(self, fn):
+        """fn(StepLog) after every engine step, per slot that ran something;
+        None turns it off."""
+        self._step_cb = None if fn is None else STEP_FN(lambda _u, p: fn(p.contents))
+        _check(lib().mf_engine_set_step_log(self._h, self._step_cb if fn is not None else STEP_FN(), None))
 
     def profile_read(self):
         ms = np.zeros(7, np.float64)

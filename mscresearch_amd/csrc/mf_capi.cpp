@@ -665,6 +665,13 @@ int mf_engine_profile(mf_engine* e, int on) {
     return MF_OK;
 }
 
+int mf_engine_set_step_log(mf_engine* e, mf_step_fn fn, void* user) {
+    if (!e) return fail(MF_EINVAL, "null engine");
+    e->eng.step_fn = fn;
+    e->eng.step_user = user;
+    return MF_OK;
+}
+
 int mf_engine_profile_read(const mf_engine* e, double* ms, int64_t* launches, uint64_t* work) {
     if (!e) return fail(MF_EINVAL, "null engine");
     for (int f = 0; f < kFamilies; ++f) {

@@ -13,7 +13,8 @@
 // (concurrent queries, default 1 = one query at a time like the reference),
 // MSCMF_TILE (tile size), MSCMF_PRUNE (0/1), MSCMF_GRAPH_CACHE (path of a
 // binary cache of the prepared graph: loaded while it matches graph.txt and
-// MSCMF_R, else rebuilt and written).
+// MSCMF_R, else rebuilt and written), MSCMF_LOG=1 (static mode with one
+// slot: the reference's per-query and per-iteration stderr lines, timed).
 #include <sys/stat.h>
 
 #include <algorithm>
@@ -55,6 +56,47 @@ void write_metrics(const std::string& path, const char* mode, const std::vector<
     for (size_t i = 0; i < lat_us.size(); ++i) fprintf(f, "%s%.1f", i ? ", " : "", lat_us[i]);
     fprintf(f, "]}\n");
     fclose(f);
+}
+
+// MSCMF_LOG=1 (static mode, one slot): the reference's per-query stderr
+// lines, in the format its plotting scripts parse ("[Init] kernel_1(...)"
+// BC:2188, the per-iteration timing line of the predecessor solver
+// Region_Aware_PR_without_Pruning.cu:1802-1807, "[QueryPrune+RegionBoundary]"
+// BC:2815-2819), from the engine's step log with HIP-event timing.  One
+// "Iteration" line per discharge step: the relabel and count time since the
+// previous one goes into it, as the reference's iteration starts with its
+// global relabel.
+struct RefLog {
+    int path_len = 256;
+    int64_t iteration = 0;
+    double k1 = 0, k7 = 0, init_hl = 0, bfs = 0, pr = 0, prune = 0;
+    bool init_printed = false;
+    void start() { *this = RefLog{path_len}; }
+};
+
+void ref_step_log(void* user, const mf_step_log* L) {
+    RefLog& R = *(RefLog*)user;
+    const uint32_t s = L->steps;
+    R.k1 += L->ms[MF_LOG_INIT];
+    R.k7 += L->ms[MF_LOG_PREFLOW];
+    R.init_hl += L->ms[MF_LOG_SEED];
+    if (s & MF_STEP_PRUNE) R.prune += L->ms[MF_LOG_BFS];
+    else R.bfs += L->ms[MF_LOG_BFS];
+    R.pr += L->ms[MF_LOG_PR];
+    if ((s & MF_STEP_PREFLOW) && !R.init_printed) {
+        fprintf(stderr, "[Init] kernel_1(init residual/excess)=%.3f ms, kernel_7(preflow)=%.3f ms\n", R.k1, R.k7);
+        R.init_printed = true;
+    }
+    if (s & MF_STEP_DISCHARGE) {
+        fprintf(stderr,
+                "Iteration %lld: init_hl=%.3f ms, bfs=%.3f ms, push_relabel=%.3f ms (K=%d, mode=%s, deeppath=%d), "
+                "remove_invalid=%.3f ms, active=%d, active_high=%d, fixed_edges=%lld, GR_FREQ0=%d\n\n",
+                (long long)R.iteration, R.init_hl, R.bfs, R.pr, L->K, L->path ? "active-aware" : "region-aware",
+                R.path_len, L->ms[MF_LOG_RM], L->active, L->active_high,
+                (long long)((s & MF_STEP_RM) ? L->fixed_edges : 0), L->gr_freq);
+        R.iteration += 1;
+        R.init_hl = R.bfs = R.pr = 0;
+    }
 }
 
 }  // namespace
@@ -144,6 +186,13 @@ int main(int argc, char** argv) {
         return 2;
     }
 
+    RefLog reflog;
+    reflog.path_len = P.path_len;
+    const bool ref_log = mode == "static" && slots == 1 && env_int("MSCMF_LOG", 0) != 0;
+    if (ref_log) {
+        mf_engine_profile(eng, 1);
+        mf_engine_set_step_log(eng, ref_step_log, &reflog);
+    }
     std::vector<long long> flows(nq, 0), times(nq, 0);
     std::vector<double> lat_us;
     int64_t edges_total = 0;
@@ -156,6 +205,12 @@ int main(int argc, char** argv) {
     if (mode == "static") {
         for (int64_t i = 0; i < nq; ++i) {
             const int s = qs[i], t = qt[i];
+            if (ref_log) {
+                const int os = (s >= 0) ? old_id[s] : -1, ot = (t >= 0) ? old_id[t] : -1;
+                fprintf(stderr, "[Static] Query %lld: s=%d, t=%d... ", (long long)i + 1, os + 1, ot + 1);
+                if (s < 0 || t < 0 || s == t) fprintf(stderr, "Invalid.\n");
+                reflog.start();
+            }
             if (s < 0 || t < 0 || s == t) continue;
             if (mf_engine_submit(eng, gid, s, t, i, 0.0) != MF_OK) {
                 fprintf(stderr, "%s\n", mf_last_error());
@@ -177,6 +232,20 @@ int main(int argc, char** argv) {
                     times[r.tag] = (long long)ms;
                     lat_us.push_back(ms * 1e3);
                     edges_total += r.E_run;
+                    if (ref_log) {
+                        if (r.pruned)
+                            fprintf(stderr,
+                                    "[QueryPrune+RegionBoundary] prune_time=%lld ms, pruned_n=%lld/%d, pruned_m=%lld, "
+                                    "run_n=%d, run_m=%lld, Flow=%lld, Time=%lld ms\n",
+                                    (long long)reflog.prune, (long long)r.kept, n, (long long)r.E_run, r.n_run,
+                                    (long long)r.E_run, (long long)r.flow, times[r.tag]);
+                        else
+                            fprintf(stderr,
+                                    "[QueryPrune+RegionBoundary] prune_time=%lld ms, pruned_n=%lld/%d, pruned_m=%lld, "
+                                    "fallback=full, run_n=%d, run_m=%lld, Flow=%lld, Time=%lld ms\n",
+                                    (long long)reflog.prune, (long long)r.kept, n, (long long)r.E_run, r.n_run,
+                                    (long long)r.E_run, (long long)r.flow, times[r.tag]);
+                    }
                 }
             }
         }
@@ -198,6 +267,10 @@ int main(int argc, char** argv) {
         for (int64_t i = 0; i < nq; ++i) {
             const int s = qs[i], t = qt[i];
             const int os = (s >= 0) ? old_id[s] : -1, ot = (t >= 0) ? old_id[t] : -1;
+            if (ref_log) {  // printed per query above
+                if (s >= 0 && t >= 0 && s != t) total += times[i];
+                continue;
+            }
             if (s < 0 || t < 0 || s == t) {
                 fprintf(stderr, "[Static] Query %lld: s=%d, t=%d... Invalid.\n", (long long)i + 1, os + 1, ot + 1);
                 continue;

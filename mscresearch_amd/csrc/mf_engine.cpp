@@ -1041,6 +1041,35 @@ int Engine::step(std::vector<Result>& out) {
     }
     int rc = launch_plans(plans);
     if (rc < 0) return rc;
+    if (step_fn) {
+        for (int i = 0; i < nslots_; ++i) {
+            const SlotState& S = slots_[i];
+            const Plan& p = plans[i];
+            if (!S.live || p.stages == 0) continue;
+            mf_step_log L;
+            memset(&L, 0, sizeof(L));
+            L.tag = S.q.tag;
+            L.slot = i;
+            const unsigned s = p.stages;
+            const bool gr_bfs = (s & ST_BFS) && p.bfs_mode == BFS_GR;
+            L.steps = ((s & ST_INIT) ? MF_STEP_INIT : 0u) | ((s & ST_PREFLOW) ? MF_STEP_PREFLOW : 0u) |
+                      ((s & ST_GR_SEED) ? MF_STEP_SEED : 0u) | ((gr_bfs || (s & ST_LOCAL)) ? MF_STEP_BFS : 0u) |
+                      (((s & ST_BFS) && !gr_bfs) || (s & (ST_BFS_SEED1 | ST_MASK)) ? MF_STEP_PRUNE : 0u) |
+                      ((s & ST_DISCHARGE) ? MF_STEP_DISCHARGE : 0u) | ((s & ST_HOT) ? MF_STEP_HOT : 0u) |
+                      ((s & ST_RM) ? MF_STEP_RM : 0u) | ((s & ST_CERT) ? MF_STEP_CERT : 0u) |
+                      ((s & ST_ACTIVE) ? MF_STEP_COUNT : 0u);
+            L.iters = S.sched.stats.iters;
+            L.active = h_ctr_[i].active;
+            L.active_high = h_ctr_[i].active_high;
+            L.fixed_edges = (int64_t)h_ctr_[i].fixed_edges;
+            L.K = S.sched.K();
+            L.gr_freq = S.sched.gr_freq();
+            L.path = (s & ST_DISCHARGE) && p.dis_kind == DK_PATH ? 1 : 0;
+            L.cycles = (s & ST_DISCHARGE) ? std::max(1, std::min(p.cycles, std::max(1, P_.fuse))) : 0;
+            for (int f = 0; f < MF_LOG_FIELDS; ++f) L.ms[f] = step_ms_[f];
+            step_fn(step_user, &L);
+        }
+    }
     return live();
 }
 
@@ -1182,19 +1211,29 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     };
     hipStream_t st = stream_;
     // profiling: an event pair around each kernel family's launches
-    std::vector<std::pair<int, int>> timed;  // (family, event pair index)
-    auto tbs = [&](int fam, hipStream_t s) {
+    struct Timed { int fam, k, bucket; };
+    std::vector<Timed> timed;  // (family, event pair index, step-log bucket)
+    auto bucket_of = [](int fam) {
+        switch (fam) {
+        case WF_INIT: return (int)MF_LOG_INIT;
+        case WF_ACTIVE: case WF_DISCHARGE: return (int)MF_LOG_PR;
+        case WF_RM: return (int)MF_LOG_RM;
+        case WF_CERT: return (int)MF_LOG_CERT;
+        default: return (int)MF_LOG_BFS;
+        }
+    };
+    auto tbs = [&](int fam, hipStream_t s, int bucket = -1) {
         if (!profile) return -1;
         const int k = (int)timed.size();
         if (2 * k + 1 >= (int)ev_.size()) return -1;
         (void)hipEventRecord(ev_[2 * k], s);
-        timed.push_back({fam, k});
+        timed.push_back({fam, k, bucket >= 0 ? bucket : bucket_of(fam)});
         return k;
     };
     auto tes = [&](int k, hipStream_t s) {
         if (k >= 0) (void)hipEventRecord(ev_[2 * k + 1], s);
     };
-    auto tb = [&](int fam) { return tbs(fam, st); };
+    auto tb = [&](int fam, int bucket = -1) { return tbs(fam, st, bucket); };
     auto te = [&](int k) { tes(k, st); };
     auto gridN = [&](int i) { return clampi(cdiv(slots_[i].view.g.n, 256), 1, 4096); };
     auto gridE = [&](int i) { return clampi(cdiv(slots_[i].view.g.E, 1024), 1, 8192); };
@@ -1229,8 +1268,8 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         launch_init_marked(st, d_views_, L(g_init_m), g_init_m.cnt, maxof(g_init_m, gridN));
         te(k);
     }
-    if (g_pre.cnt) { const int k = tb(WF_INIT); launch_preflow(st, d_views_, L(g_pre), g_pre.cnt); te(k); }
-    if (g_seed.cnt) { const int k = tb(WF_BFS); launch_gr_seed(st, d_views_, L(g_seed), g_seed.cnt, maxof(g_seed, gridN)); te(k); }
+    if (g_pre.cnt) { const int k = tb(WF_INIT, MF_LOG_PREFLOW); launch_preflow(st, d_views_, L(g_pre), g_pre.cnt); te(k); }
+    if (g_seed.cnt) { const int k = tb(WF_BFS, MF_LOG_SEED); launch_gr_seed(st, d_views_, L(g_seed), g_seed.cnt, maxof(g_seed, gridN)); te(k); }
     if (!local_groups.empty()) {
         const int k = tb(WF_BFS);
         for (auto& lg : local_groups) {
@@ -1392,11 +1431,13 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     if (g_mask.cnt)
         HIPCHK(hipMemcpyAsync(h_hash_, d_hash_, sizeof(unsigned long long) * nslots_, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    for (auto& pr : timed) {
+    for (double& x : step_ms_) x = 0.0;
+    for (auto& tm : timed) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, ev_[2 * pr.second], ev_[2 * pr.second + 1]) == hipSuccess) {
-            fam_ms[pr.first] += ms;
-            fam_launches[pr.first] += 1;
+        if (hipEventElapsedTime(&ms, ev_[2 * tm.k], ev_[2 * tm.k + 1]) == hipSuccess) {
+            fam_ms[tm.fam] += ms;
+            fam_launches[tm.fam] += 1;
+            step_ms_[tm.bucket] += ms;
         }
     }
     return MF_OK;

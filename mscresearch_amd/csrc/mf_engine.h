@@ -237,6 +237,7 @@ public:
     const std::string& error() const { return err_; }
     int phase() const { return phase_; }
     int K() const { return K_; }
+    int gr_freq() const { return GR_FREQ_; }
     QueryStats stats;
     // pruning hand-off: after S_MASK the engine swaps graphs and calls resume()
     bool wants_prune_decision() const { return st_ == S_PRUNE_DECIDE; }
@@ -427,6 +428,12 @@ public:
     // Per kernel family timing with HIP events on the engine stream, and the
     // algorithmic work of retired queries.
     bool profile = false;
+    // per-step log (mf_engine_set_step_log): step_ms_ holds the last step's
+    // stage times in the log's buckets
+    using StepFn = void (*)(void*, const mf_step_log*);
+    StepFn step_fn = nullptr;
+    void* step_user = nullptr;
+    double step_ms_[MF_LOG_FIELDS] = {};
     double fam_ms[kFamilies] = {};
     int64_t fam_launches[kFamilies] = {};
     uint64_t fam_work[kFamilies][4] = {};

@@ -165,6 +165,66 @@ def test_cli_static_graph_cache(tmp_path):
     assert cache.exists()
 
 
+def parse_ref_log(text):
+    """Per-query records from stderr as the reference's comparison scripts
+    read them (COMPARISON/ECL_RA/plot_three_datasets_ecl_labels.py:149-215):
+    a "Query i:" opens a record, the "[Init] kernel_1(...)=x ms, kernel_7(...)=y ms"
+    line gives k1/k7, each "Iteration j: init_hl=a ms, bfs=b ms, push_relabel=c ms
+    ... remove_invalid=d ms" line one iteration."""
+    import re
+    text = re.sub(r"(?=Query \d+:)", "\n", text)
+    text = re.sub(r"(?=\[Init\])", "\n", text)
+    text = re.sub(r"(?=Iteration \d+:)", "\n", text)
+    q, cur = {}, None
+    for line in text.splitlines():
+        m = re.search(r"Query\s+(\d+):", line)
+        if m:
+            cur = int(m.group(1))
+            q.setdefault(cur, {"iters": [], "k1": None, "k7": None, "flow": None})
+        if cur is None:
+            continue
+        m = re.search(r"kernel_1\(init residual/excess\)=([0-9.]+)\s*ms,\s*kernel_7\(preflow\)=([0-9.]+)\s*ms", line)
+        if m:
+            q[cur]["k1"], q[cur]["k7"] = float(m.group(1)), float(m.group(2))
+        m = re.search(r"Iteration\s+(\d+):\s*init_hl=([0-9.]+)\s*ms,\s*bfs=([0-9.]+)\s*ms,\s*"
+                      r"push_relabel=([0-9.]+)\s*ms.*?remove_invalid=([0-9.]+)\s*ms", line)
+        if m:
+            q[cur]["iters"].append(tuple(float(m.group(k)) for k in range(2, 6)))
+        m = re.search(r"\[QueryPrune\+RegionBoundary\].*Flow=(\d+), Time=(\d+) ms", line)
+        if m:
+            q[cur]["flow"] = int(m.group(1))
+    return q
+
+
+@pytest.mark.parametrize("gen", ["rmat", "grid"])
+def test_cli_reference_log_lines(tmp_path, gen):
+    """MSCMF_LOG=1: every valid query gets a kernel_1/kernel_7 line and
+    timed iteration lines the reference's scripts parse, and a closing line
+    whose flow is the res_mt.txt flow; invalid queries get no record."""
+    if gen == "rmat":
+        e = M.Edges.rmat(11, 10**6, 5401)
+        qs, qt = e.hub_queries(5, 5402)
+    else:
+        e = M.Edges.grid(48, 48, 1000, 5403)
+        qs, qt = M.grid_queries(48, 48, 3, 5404)
+    write_inputs(tmp_path, e, qs, qt, extra="7 7\n")
+    r, out = run_cli(tmp_path, "static", {"MSCMF_LOG": "1"})
+    assert r.returncode == 0, r.stderr[-2000:]
+    flows = [int(l.split()[0]) for l in (out / "res_mt.txt").read_text().splitlines()]
+    recs = parse_ref_log(r.stderr)
+    assert sorted(recs) == list(range(1, len(qs) + 2))
+    for i in range(1, len(qs) + 1):
+        rec = recs[i]
+        assert rec["k1"] is not None and rec["k1"] >= 0 and rec["k7"] >= 0, (i, r.stderr[-3000:])
+        assert rec["iters"], i
+        assert all(min(x) >= 0 for x in rec["iters"])
+        assert sum(x[0] + x[1] + x[2] for x in rec["iters"]) > 0  # timed
+        assert rec["flow"] == flows[i - 1]
+    last = recs[len(qs) + 1]
+    assert last["k1"] is None and not last["iters"] and "Invalid" in r.stderr
+    assert "Total time:" in r.stderr
+
+
 def test_cli_dynamic_with_updates(tmp_path):
     rng = np.random.default_rng(3)
     e = M.Edges.rmat(11, 10**6, 5201)

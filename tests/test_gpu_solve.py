@@ -313,3 +313,37 @@ def test_fused_cycles_replay_captured_graphs():
         eng.submit(gid, s, t, i)
     res = {r.tag: r.flow for r in eng.run()}
     assert [res[i] for i in range(4)] == want
+
+
+def test_step_log_reports_each_step(c1):
+    """The per-step log: one record per slot and step with the query's tag,
+    what the step ran (pruning, init + preflow + relabel seeding first, then
+    relabel levels, counts and discharges, the certificate last), counters
+    and, while profiling, the step's stage times."""
+    eng, gid = engine_with(c1, nslots=2)
+    seen = []
+    eng.set_step_log(lambda L: seen.append((L.tag, L.slot, L.steps, L.iters, L.active, L.K, L.gr_freq,
+                                            [L.ms[i] for i in range(7)])))
+    eng.profile(True)
+    qs, qt = M.uniform_queries(c1.n, 2, 1011)
+    for k, (a, b) in enumerate(zip(qs, qt)):
+        eng.submit(gid, *c1.q(a, b), 70 + k)
+    res = {r.tag: r for r in eng.run()}
+    eng.set_step_log(None)
+    eng.profile(False)
+    assert sorted(res) == [70, 71]
+    for tag in (70, 71):
+        mine = [x for x in seen if x[0] == tag]
+        bits = [x[2] for x in mine]
+        first = next(i for i, b in enumerate(bits) if b & M.MF_STEP_PREFLOW)
+        assert bits[first] & M.MF_STEP_INIT and bits[first] & M.MF_STEP_SEED
+        assert all(b & (M.MF_STEP_PRUNE | M.MF_STEP_PREFLOW) for b in bits[:first + 1])
+        assert any(b & M.MF_STEP_DISCHARGE for b in bits)
+        assert bits[-1] & M.MF_STEP_CERT
+        assert len({x[1] for x in mine}) == 1 and mine[-1][3] == res[tag].iters
+        assert all(x[5] >= 1 and x[6] >= 1 for x in mine)
+        assert all(min(x[7]) >= 0 for x in mine) and sum(sum(x[7]) for x in mine) > 0
+        assert res[tag].flow == c1.oracle_flow(*c1.q(qs[tag - 70], qt[tag - 70]))
+    seen.clear()
+    eng.solve(gid, *c1.q(qs[0], qt[0]))
+    assert not seen  # log off
