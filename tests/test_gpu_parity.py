@@ -44,10 +44,14 @@ def test_init_preflow_bit_exact(c1):
         assert np.array_equal(ex, oex)
 
 
+@pytest.mark.parametrize("direction", ["auto", "bottom_up"])
 @pytest.mark.parametrize("which", ["c1", "rmat12", "grid"])
-def test_global_relabel_heights_bit_exact(which, request):
+def test_global_relabel_heights_bit_exact(which, direction, request):
+    """Relabel heights equal the oracle's BFS distances.  "bottom_up" forces
+    every level bottom-up (switch at once, never back), so runs of bottom-up
+    levels read the unvisited lists of the levels before them."""
     case = request.getfixturevalue(which)
-    e, gid = engine_with(case)
+    e, gid = engine_with(case, **({} if direction == "auto" else dict(bu_alpha=1e12, bu_beta=1e12)))
     rng = np.random.default_rng(77)
     if which == "rmat12":
         s_, t_ = case.edges.hub_queries(4, 1202)
@@ -57,7 +61,7 @@ def test_global_relabel_heights_bit_exact(which, request):
         s_, t_ = M.uniform_queries(case.n, 4, 1003)
     for a, b in zip(s_, t_):
         s, t = case.q(a, b)
-        log(f"GR {which} s={s} t={t}")
+        log(f"GR {which} {direction} s={s} t={t}")
         res, ex = O.init_preflow(case.og, s)
         h = e.dbg_global_relabel(gid, s, t, res, ex)
         assert np.array_equal(h, O.bfs_heights(case.og, res, ex, s, t))
