@@ -108,6 +108,7 @@ Params from_c(const mf_params* p) {
     P.graphs = p->graphs;
     P.spec_cert = p->spec_cert;
     P.sparse_count = p->sparse_count;
+    P.small_wgs = p->small_wgs;
     return P;
 }
 
@@ -447,6 +448,7 @@ void mf_params_default(mf_params* p) {
     p->graphs = P.graphs;
     p->spec_cert = P.spec_cert;
     p->sparse_count = P.sparse_count;
+    p->small_wgs = P.small_wgs;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

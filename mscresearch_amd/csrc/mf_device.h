@@ -46,6 +46,10 @@ struct Ctr {
     int32_t nheavy;          // heavy entries (back of the list)
     int32_t ngiant;          // rows longer than kGiantDeg in the giant list
     int32_t ngchunks;        // kGiantChunk-entry tasks over those rows
+    // barrier of the workgroups of a multi-workgroup BFS launch: arrivals,
+    // generation
+    int32_t bar_count;
+    uint32_t bar_gen;
     // sparse counting of fused cycles: vertices pushed into since the last
     // touch reset are listed in hl (deduplicated by stamps in aux)
     int32_t ntouched;

@@ -259,6 +259,7 @@ void SlotSched::begin(const Params* P, const DGraph& g, int s, int t, bool prune
     iter_ = 0;
     last_gr_iter_ = 0;
     gr_depth_ = 0;
+    depth_hint_ = 0;
     spec_ = false;
     spec_failed_ = false;
     stag_ = no_stag_ = 0;
@@ -345,6 +346,7 @@ Plan SlotSched::bfs_step(const Ctr& c, int mode) {
         p.bfs_grid = clampi(cdiv(g_.n, 256), 1, 8192);
     } else if (f <= kSmallFrontier && mf <= kSmallEdges) {
         p.bfs_kind = BK_SMALL;
+        if (gr_depth_ >= 64 || depth_hint_ >= 64 || L >= 64) p.small_wgs = std::max(1, P_->small_wgs);
     } else {
         p.bfs_kind = BK_TD;
         p.bfs_grid = std::max(1, cdiv(f, 256));
@@ -354,6 +356,12 @@ Plan SlotSched::bfs_step(const Ctr& c, int mode) {
 
 Plan SlotSched::plan(const Ctr& c) {
     Plan p;
+    // a kernel flagged the slot (a multi-workgroup BFS barrier gave up): the
+    // counters of the last step cannot be trusted
+    if (c.err && st_ != S_START && st_ != S_WARM_START && st_ != S_PRUNE_FWD && st_ != S_DONE) {
+        fail(MF_EHIP, "device error flag " + std::to_string(c.err));
+        return p;
+    }
     switch (st_) {
     case S_PRUNE_FWD: {
         // forward BFS from s over cap > 0 (BC:1402-1416)
@@ -655,12 +663,21 @@ Engine::~Engine() {
 
 double Engine::now() const { return wall_seconds() - t0_; }
 
+long long barrier_spin_ticks() {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+        khz <= 0)
+        khz = 100000;  // 100 MHz
+    return 2000LL * khz;
+}
+
 int Engine::init(int nslots, const Params& P) {
     if (nslots < 1) { set_error("nslots must be >= 1"); return MF_EINVAL; }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { set_error("no HIP device"); return MF_ENODEV; }
     P_ = P;
     nslots_ = nslots;
+    spin_ticks_ = barrier_spin_ticks();
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&ev_up_, hipEventDisableTiming));
@@ -812,6 +829,7 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
     v.Kh = std::max(1, P_.K_heavy);
     const bool prune = P_.prune != 0 && prune_ok && !warm;
     S.sched.begin(&P_, G.g, q.s, q.t, prune, warm);
+    S.sched.set_depth_hint(G.depth_hint.load(std::memory_order_relaxed));
     S.sched.stats.t_submit = q.t_arrive;
     S.sched.stats.t_start = now();
     v.K = S.sched.K();
@@ -898,6 +916,7 @@ int Engine::class_decide(int i) {
             S.view.t = t;
             S.sched.stats.pruned = 1;
             S.sched.resume_after_prune(run->g, s, t);
+            S.sched.set_depth_hint(run->depth_hint.load(std::memory_order_relaxed));
             S.view.K = S.sched.K();
             return MF_OK;
         }
@@ -954,6 +973,7 @@ int Engine::prune_decide(int i) {
             S.view.t = ids[1];
             S.sched.stats.pruned = 1;
             S.sched.resume_after_prune(run->g, ids[0], ids[1]);
+            S.sched.set_depth_hint(run->depth_hint.load(std::memory_order_relaxed));
             S.view.K = S.sched.K();
             return MF_OK;
         }
@@ -1008,6 +1028,10 @@ int Engine::step(std::vector<Result>& out) {
             if (rc != MF_OK) return rc;
         }
         plans[i] = S.sched.plan(h_ctr_[i]);
+        if (S.sched.gr_depth() > 0) {
+            DeviceGraph& RG = S.run_graph ? *S.run_graph : *graphs_[S.q.graph];
+            RG.depth_hint.store(S.sched.gr_depth(), std::memory_order_relaxed);
+        }
         S.view.phase = S.sched.phase();
         S.view.K = S.sched.K();
         S.sched.stats.steps++;
@@ -1113,7 +1137,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     auto has = [&](int i, unsigned bit) { return !side[i] && (plans[i].stages & bit) != 0; };
     auto hs = [&](int i, unsigned bit) { return side[i] && (plans[i].stages & bit) != 0; };
     std::vector<std::pair<unsigned, Group>> side_pre;
-    Group s_seed, s_seed1[3], s_bfs[3];
+    Group s_seed, s_seed1[3], s_bfs[3][2];  // [mode][multi-workgroup]
     {
         std::vector<unsigned> vals;
         for (int i = 0; i < nslots_; ++i)
@@ -1125,7 +1149,8 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         s_seed = make_list([&](int i) { return hs(i, ST_GR_SEED); });
         for (int m = 0; m < 3; ++m) {
             s_seed1[m] = make_list([&](int i) { return hs(i, ST_BFS_SEED1) && plans[i].seed_mode == m; });
-            s_bfs[m] = make_list([&](int i) { return hs(i, ST_BFS) && plans[i].bfs_mode == m; });
+            for (int w = 0; w < 2; ++w)
+                s_bfs[m][w] = make_list([&](int i) { return hs(i, ST_BFS) && plans[i].bfs_mode == m && (plans[i].small_wgs > 1) == (w == 1); });
         }
     }
     // reset_pre grouped by distinct mask values
@@ -1158,10 +1183,16 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     Group g_seed = make_list([&](int i) { return has(i, ST_GR_SEED); });
     Group g_seed1[3];
     for (int m = 0; m < 3; ++m) g_seed1[m] = make_list([&](int i) { return has(i, ST_BFS_SEED1) && plans[i].seed_mode == m; });
-    Group g_bfs[3][3];
+    Group g_bfs[3][3], g_bfs_multi[3];  // [kind][mode]; small-frontier slots with several workgroups apart
     for (int k = 0; k < 3; ++k)
         for (int m = 0; m < 3; ++m)
-            g_bfs[k][m] = make_list([&](int i) { return has(i, ST_BFS) && plans[i].bfs_kind == k && plans[i].bfs_mode == m; });
+            g_bfs[k][m] = make_list([&](int i) {
+                return has(i, ST_BFS) && plans[i].bfs_kind == k && plans[i].bfs_mode == m && (k != BK_SMALL || plans[i].small_wgs <= 1);
+            });
+    for (int m = 0; m < 3; ++m)
+        g_bfs_multi[m] = make_list([&](int i) {
+            return has(i, ST_BFS) && plans[i].bfs_kind == BK_SMALL && plans[i].bfs_mode == m && plans[i].small_wgs > 1;
+        });
     // local relabels, one group per distinct level cap
     std::vector<std::pair<int, Group>> local_groups;
     {
@@ -1243,7 +1274,9 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         return 0;
     };
     // side stream: starts after the view / list upload on the main stream
-    const bool use_side = !side_pre.empty() || s_seed.cnt || s_bfs[0].cnt || s_bfs[1].cnt || s_bfs[2].cnt;
+    bool side_bfs = false;
+    for (int m = 0; m < 3; ++m) side_bfs = side_bfs || s_bfs[m][0].cnt || s_bfs[m][1].cnt;
+    const bool use_side = !side_pre.empty() || s_seed.cnt || side_bfs;
     if (use_side) {
         HIPCHK(hipEventRecord(ev_up_, st));
         HIPCHK(hipStreamWaitEvent(side_, ev_up_, 0));
@@ -1253,7 +1286,9 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         for (int m = 0; m < 3; ++m)
             if (s_seed1[m].cnt) launch_bfs_seed1(side_, d_views_, L(s_seed1[m]), s_seed1[m].cnt, maxof(s_seed1[m], gridN), m);
         for (int m = 0; m < 3; ++m)
-            if (s_bfs[m].cnt) launch_bfs_small(side_, d_views_, L(s_bfs[m]), s_bfs[m].cnt, m, P_.small_levels);
+            for (int w = 0; w < 2; ++w)
+                if (s_bfs[m][w].cnt)
+                    launch_bfs_small(side_, d_views_, L(s_bfs[m][w]), s_bfs[m][w].cnt, m, P_.small_levels, w ? P_.small_wgs : 1, spin_ticks_);
         tes(k, side_);
         HIPCHK(hipEventRecord(ev_side_, side_));
     }
@@ -1292,7 +1327,12 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     for (int m = 0; m < 3; ++m) {
         if (g_bfs[BK_SMALL][m].cnt) {
             const int k = tb(WF_BFS);
-            launch_bfs_small(st, d_views_, L(g_bfs[BK_SMALL][m]), g_bfs[BK_SMALL][m].cnt, m, P_.small_levels);
+            launch_bfs_small(st, d_views_, L(g_bfs[BK_SMALL][m]), g_bfs[BK_SMALL][m].cnt, m, P_.small_levels, 1, spin_ticks_);
+            te(k);
+        }
+        if (g_bfs_multi[m].cnt) {
+            const int k = tb(WF_BFS);
+            launch_bfs_small(st, d_views_, L(g_bfs_multi[m]), g_bfs_multi[m].cnt, m, P_.small_levels, P_.small_wgs, spin_ticks_);
             te(k);
         }
         if (g_bfs[BK_TD][m].cnt) {

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <cstdint>
 #include <deque>
 #include <map>
@@ -46,6 +47,12 @@ struct Params {
     // levels one single-workgroup BFS launch may run: bounds a step's length
     // on deep graphs so batched slots keep moving while one relabels
     int small_levels = 256;
+    // workgroups per slot of those multi-level BFS launches on deep graphs
+    // (last relabel, or the running BFS, at least 64 levels deep); 1 = one
+    // 1024-thread workgroup.  More meet at a barrier in global memory after
+    // each level, so a level's scattered loads use several CUs; a shallow
+    // BFS keeps one workgroup, its few levels not worth a barrier each.
+    int small_wgs = 8;
     double bu_alpha = 14.0, bu_beta = 24.0;  // direction-optimizing BFS thresholds
     int prune = 1;             // per-query reachability pruning (BC:2678)
     double prune_ratio = 0.995;  // BC:2647
@@ -77,6 +84,10 @@ struct Params {
     int sparse_count = 1;
 };
 
+// Wall-clock ticks a multi-workgroup BFS barrier may wait before it flags
+// the slot's error word (about 2 s).
+long long barrier_spin_ticks();
+
 // ---------------------------------------------------------- device memory
 uint64_t next_graph_uid();
 
@@ -95,6 +106,9 @@ struct DeviceGraph {
     uint8_t* d_mask = nullptr;     // pruned run graphs: parent mask (cache check)
     size_t bytes = 0;
     int tile_max = kTileMax;
+    // levels of the last exact relabel any query saw on this graph (engines
+    // on several threads may share the graph)
+    std::atomic<int> depth_hint{0};
     // Reachability classes for pruning: scc (over cap > 0 entries) and wcc
     // ids on the host, wcc ids on the device, and the forward-reach set of
     // each SCC once some query has needed it.
@@ -202,6 +216,7 @@ struct Plan {
     int local_cap = 0;        // ST_LOCAL: level cap of the local relabel
     int cycles = 1;           // ST_DISCHARGE: discharge + count cycles in this step
     bool sparse = false;      // ST_ACTIVE: count the previous list and the touched vertices only
+    int small_wgs = 1;        // BK_SMALL: workgroups per slot (1 or Params::small_wgs)
 };
 
 // Per-query statistics (host side).
@@ -233,6 +248,10 @@ public:
     // Decide the next step given the counters of the last one.
     Plan plan(const Ctr& c);
     bool done() const { return st_ == S_DONE || st_ == S_ERROR; }
+    // depth of this graph's relabels seen by earlier queries: picks the
+    // multi-workgroup small-frontier BFS from a query's first relabel on
+    void set_depth_hint(int d) { depth_hint_ = d; }
+    int gr_depth() const { return gr_depth_; }
     int status() const { return status_; }
     const std::string& error() const { return err_; }
     int phase() const { return phase_; }
@@ -267,6 +286,7 @@ private:
     bool just_gr_ = false;
     int64_t iter_ = 0, last_gr_iter_ = 0;
     int gr_depth_ = 0;  // levels of the last exact global relabel
+    int depth_hint_ = 0;
     int GR_FREQ_ = 128, K_ = 4, RM_FREQ_ = 64;
     int stag_ = 0, no_stag_ = 0;
     int64_t last_pos_ = INT64_MAX;
@@ -381,6 +401,7 @@ private:
 
     Params P_;
     int nslots_ = 0;
+    long long spin_ticks_ = 0;  // multi-workgroup BFS barrier wait limit
     hipStream_t stream_ = nullptr;
     hipStream_t side_ = nullptr;          // single-workgroup BFS steps, overlapped with the rest
     hipEvent_t ev_up_ = nullptr, ev_side_ = nullptr;

@@ -61,9 +61,12 @@ void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int n
 // One bottom-up level.
 void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
                    int mode);
-// Single-workgroup multi-level top-down BFS while the frontier stays small.
+// Multi-level top-down BFS while the frontier stays small: one workgroup
+// (wgs == 1), or wgs workgroups per slot meeting at a barrier in global
+// memory after each level (spin_ticks: wall-clock ticks a barrier may wait
+// before it gives up and flags the slot's error word).
 void launch_bfs_small(hipStream_t st, const SlotView* slots, const int* list, int nlist, int mode,
-                      int max_levels);
+                      int max_levels, int wgs, long long spin_ticks);
 
 // Active list (light entries at fa[0..), heavy at fa[n-1..]) and counters.
 void launch_active(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);

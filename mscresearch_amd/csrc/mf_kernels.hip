@@ -276,6 +276,7 @@ __global__ void k_reset(const SlotView* __restrict__ slots, const int* __restric
             c->nhl = 0;
             c->nhg = 0;
             c->ngiant = 0;
+            c->bar_count = 0;
             c->visited = 0;
         }
         if (what & (RST_ACTIVE | RST_SPARSE)) {
@@ -938,11 +939,149 @@ __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__
     work_flush(S, wk, red, WF_BFS);
 }
 
+// Barrier of the gridDim.x workgroups of one slot in a multi-workgroup BFS.
+// Every thread publishes its writes (agent-scope release) before the
+// workgroup arrives, and takes the others' (acquire) after the generation
+// moves; the last arrival resets the count before it moves the generation,
+// so no workgroup can arrive at the next barrier early.  The wait is bounded:
+// past spin_ticks of the wall clock it gives up, flags the slot's error word
+// (bit 4) and returns false.  All workgroups of the launch fit on the device
+// at once, also beside such a launch on the other stream (at most 512 of 256
+// threads each), and no other kernel waits on them, so a missing workgroup
+// is at worst still being dispatched.
+__device__ __forceinline__ bool slot_barrier(GAS Ctr* c, uint32_t& gen, long long spin_ticks, int* ok_s) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const int G = (int)gridDim.x;
+        const int arrived = __hip_atomic_fetch_add(&c->bar_count, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+        if (arrived == G) {
+            __hip_atomic_store(&c->bar_count, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&c->bar_gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const long long t0 = wall_clock64();
+            while (__hip_atomic_load(&c->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(1);
+                if (wall_clock64() - t0 > spin_ticks) {
+                    __hip_atomic_fetch_or(&c->err, 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        *ok_s = ok;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    gen += 1u;
+    return *ok_s != 0;
+}
+
+// Multi-workgroup version of k_bfs_small: gridDim.x workgroups of 256
+// threads per slot share each level's frontier (grid-stride) and append the
+// next one through LDS staging; the level counts are read after the barrier,
+// so every workgroup takes the same decision to go on.  The stopping rules
+// and the ring left behind are k_bfs_small's.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bfs_multi(const SlotView* __restrict__ slots,
+                                                   const int* __restrict__ list, int max_levels,
+                                                   long long spin_ticks) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ Stage<1024> st;
+    __shared__ int heavy[256];
+    __shared__ int nheavy;
+    __shared__ int ok_s;
+    __shared__ int next_fsz;
+    __shared__ int64_t next_fdeg;
+    __shared__ uint64_t red[256 / WAVE];
+    GAS int32_t* lvl = bfs_lvl<MODE>(S);
+    const int G = (int)gridDim.x;
+    int L = S.ctr->level;
+    int fsz = S.ctr->fsize[L & 3];
+    int64_t fdeg = S.ctr->fedges[L & 3];
+    // every workgroup reads the same generation: it only moves once all arrived
+    uint32_t gen = __hip_atomic_load(&S.ctr->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    Work wk;
+    int done_levels = 0;
+    int visited = 0;
+    while (fsz > 0 && fsz <= kSmallFrontier && fdeg <= kSmallEdges && done_levels < max_levels) {
+        const GAS int32_t* in = (L & 1) ? S.fb : S.fa;
+        GAS int32_t* out = (L & 1) ? S.fa : S.fb;
+        GAS int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
+        GAS int64_t* odeg = &S.ctr->fedges[(L + 1) & 3];
+        stage_init(st);
+        if (threadIdx.x == 0) nheavy = 0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            // level L + 2's counts start at zero (nobody reads or appends them
+            // before the next barrier)
+            S.ctr->fsize[(L + 2) & 3] = 0;
+            S.ctr->fedges[(L + 2) & 3] = 0;
+        }
+        __syncthreads();
+        for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < fsz; i += G * (int)blockDim.x) {
+            const int v = in[i];
+            wk.verts++;
+            const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+            if (re - rb > kLightDeg) {
+                const int k = atomicAdd(&nheavy, 1);
+                if (k < 256) { heavy[k] = v; continue; }
+            }
+            wk.edges += (uint64_t)(re - rb);
+            td_entries<MODE, 4>(S, rb, re, 1, L, lvl, [&](int w) {
+                stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
+                wk.relabels++;
+            });
+        }
+        __syncthreads();
+        const int nh = min(nheavy, 256);
+        for (int k = 0; k < nh; ++k) {
+            const int v = heavy[k];
+            const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+            if (threadIdx.x == 0) wk.edges += (uint64_t)(re - rb);
+            td_entries<MODE, 2>(S, rb + (int)threadIdx.x, re, (int)blockDim.x, L, lvl, [&](int w) {
+                stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
+                wk.relabels++;
+            });
+        }
+        stage_flush(st, out, ocnt, odeg);
+        if (!slot_barrier(S.ctr, gen, spin_ticks, &ok_s)) break;
+        // the next level's counts: nobody changes them from here on
+        if (threadIdx.x == 0) {
+            next_fsz = __hip_atomic_load(ocnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            next_fdeg = __hip_atomic_load(odeg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        fsz = next_fsz;
+        fdeg = next_fdeg;
+        ++L;
+        ++done_levels;
+        visited += fsz;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && done_levels > 0) {
+        // the ring as a multi-workgroup continuation expects it: level L's
+        // frontier in slot L (appended there), slot L+1 zeroed by the last level
+        S.ctr->visited = S.ctr->visited + visited;
+        S.ctr->level = L;
+    }
+    work_flush(S, wk, red, WF_BFS);
+}
+
 void launch_bfs_small(hipStream_t st, const SlotView* slots, const int* list, int nlist, int mode,
-                      int max_levels) {
-    if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_small<BFS_GR>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
-    else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_small<BFS_FWD>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
-    else hipLaunchKernelGGL(k_bfs_small<BFS_UND>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
+                      int max_levels, int wgs, long long spin_ticks) {
+    if (wgs <= 1) {
+        if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_small<BFS_GR>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
+        else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_small<BFS_FWD>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
+        else hipLaunchKernelGGL(k_bfs_small<BFS_UND>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
+        return;
+    }
+    // at most 512 workgroups per launch: this launch and one on the other
+    // stream fit on the device at once
+    int g = wgs;
+    while (g > 1 && (long long)g * nlist > 512) g >>= 1;
+    if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_multi<BFS_GR>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, spin_ticks);
+    else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_multi<BFS_FWD>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, spin_ticks);
+    else hipLaunchKernelGGL(k_bfs_multi<BFS_UND>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, spin_ticks);
 }
 
 // ------------------------------------------------------- local relabel (a14)

@@ -67,7 +67,7 @@ def test_params_defaults_are_the_reference_constants():
     assert (p.local_relabel, p.local_band) == (0, 128)  # band limit of BC:1611; off by default
     # additions without a reference constant: fused cycles, graph replay,
     # speculative certificate, sparse counts
-    assert (p.fuse, p.graphs, p.spec_cert, p.sparse_count) == (8, 1, 1, 1)
+    assert (p.fuse, p.graphs, p.spec_cert, p.sparse_count, p.small_wgs) == (8, 1, 1, 1, 8)
 
 
 def test_cli_usage_and_errors(tmp_path):

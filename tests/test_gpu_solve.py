@@ -55,12 +55,18 @@ def test_c1_flows_and_cpu_certificate(c1):
         assert np.all(T[hb < c1.n])
 
 
-@pytest.mark.parametrize("rows,cols,fuse", [(64, 64, 8), (128, 96, 8), (128, 96, 1)])
-def test_grid_flows(rows, cols, fuse):
-    """Grids are deep (BFS depth ~ rows + cols): fused discharge cycles run."""
-    log(f"grid {rows}x{cols} fuse {fuse}")
+@pytest.mark.parametrize("rows,cols,fuse,extra", [
+    (64, 64, 8, {}), (128, 96, 8, {}), (128, 96, 1, {}),
+    (128, 96, 8, dict(small_wgs=1)),                   # deep relabels in one workgroup
+    (128, 96, 8, dict(small_wgs=2, small_levels=3)),   # two workgroups, a launch every 3 levels
+])
+def test_grid_flows(rows, cols, fuse, extra):
+    """Grids are deep (BFS depth ~ rows + cols): fused discharge cycles run,
+    and relabel BFS levels past depth 64 run on several workgroups that meet
+    at a barrier after each level."""
+    log(f"grid {rows}x{cols} fuse {fuse} {extra}")
     case = Case(M.Edges.grid(rows, cols, 1000, 2001), R=64, tile_max=512)
-    eng, gid = engine_with(case, fuse=fuse)
+    eng, gid = engine_with(case, fuse=fuse, **extra)
     qs, qt = M.grid_queries(rows, cols, 8, 2002)
     for a, b in zip(qs, qt):
         s, t = case.q(a, b)
