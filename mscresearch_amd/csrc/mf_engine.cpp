@@ -1267,6 +1267,12 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     auto tb = [&](int fam, int bucket = -1) { return tbs(fam, st, bucket); };
     auto te = [&](int k) { tes(k, st); };
     auto gridN = [&](int i) { return clampi(cdiv(slots_[i].view.g.n, 256), 1, 4096); };
+    // vertex- and entry-parallel kernels are grid-stride: at most about
+    // kLaunchBlocks workgroups per launch over all its slots (several times
+    // what the device holds at once), so each workgroup's fixed costs
+    // (staging, block reductions, barriers) spread over several chunks
+    constexpr int kLaunchBlocks = 16384;
+    auto fit = [&](const Group& g, int gx) { return std::max(1, std::min(gx, std::max(64, kLaunchBlocks / std::max(1, g.cnt)))); };
     auto gridE = [&](int i) { return clampi(cdiv(slots_[i].view.g.E, 1024), 1, 8192); };
     auto giants = [&](const Group& g) {
         for (int k = 0; k < g.cnt; ++k)
@@ -1282,9 +1288,9 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         HIPCHK(hipStreamWaitEvent(side_, ev_up_, 0));
         const int k = tbs(WF_BFS, side_);
         for (auto& pg : side_pre) launch_reset(side_, d_views_, L(pg.second), pg.second.cnt, pg.first);
-        if (s_seed.cnt) launch_gr_seed(side_, d_views_, L(s_seed), s_seed.cnt, maxof(s_seed, gridN));
+        if (s_seed.cnt) launch_gr_seed(side_, d_views_, L(s_seed), s_seed.cnt, fit(s_seed, maxof(s_seed, gridN)));
         for (int m = 0; m < 3; ++m)
-            if (s_seed1[m].cnt) launch_bfs_seed1(side_, d_views_, L(s_seed1[m]), s_seed1[m].cnt, maxof(s_seed1[m], gridN), m);
+            if (s_seed1[m].cnt) launch_bfs_seed1(side_, d_views_, L(s_seed1[m]), s_seed1[m].cnt, fit(s_seed1[m], maxof(s_seed1[m], gridN)), m);
         for (int m = 0; m < 3; ++m)
             for (int w = 0; w < 2; ++w)
                 if (s_bfs[m][w].cnt)
@@ -1295,16 +1301,16 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     for (auto& pg : pre_groups) launch_reset(st, d_views_, L(pg.second), pg.second.cnt, pg.first);
     if (g_init.cnt) {
         const int k = tb(WF_INIT);
-        launch_init(st, d_views_, L(g_init), g_init.cnt, maxof(g_init, [&](int i) { return std::max(gridE(i), gridN(i)); }));
+        launch_init(st, d_views_, L(g_init), g_init.cnt, fit(g_init, maxof(g_init, [&](int i) { return std::max(gridE(i), gridN(i)); })));
         te(k);
     }
     if (g_init_m.cnt) {
         const int k = tb(WF_INIT);
-        launch_init_marked(st, d_views_, L(g_init_m), g_init_m.cnt, maxof(g_init_m, gridN));
+        launch_init_marked(st, d_views_, L(g_init_m), g_init_m.cnt, fit(g_init_m, maxof(g_init_m, gridN)));
         te(k);
     }
     if (g_pre.cnt) { const int k = tb(WF_INIT, MF_LOG_PREFLOW); launch_preflow(st, d_views_, L(g_pre), g_pre.cnt); te(k); }
-    if (g_seed.cnt) { const int k = tb(WF_BFS, MF_LOG_SEED); launch_gr_seed(st, d_views_, L(g_seed), g_seed.cnt, maxof(g_seed, gridN)); te(k); }
+    if (g_seed.cnt) { const int k = tb(WF_BFS, MF_LOG_SEED); launch_gr_seed(st, d_views_, L(g_seed), g_seed.cnt, fit(g_seed, maxof(g_seed, gridN))); te(k); }
     if (!local_groups.empty()) {
         const int k = tb(WF_BFS);
         for (auto& lg : local_groups) {
@@ -1321,7 +1327,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     for (int m = 0; m < 3; ++m)
         if (g_seed1[m].cnt) {
             const int k = tb(WF_BFS);
-            launch_bfs_seed1(st, d_views_, L(g_seed1[m]), g_seed1[m].cnt, maxof(g_seed1[m], gridN), m);
+            launch_bfs_seed1(st, d_views_, L(g_seed1[m]), g_seed1[m].cnt, fit(g_seed1[m], maxof(g_seed1[m], gridN)), m);
             te(k);
         }
     for (int m = 0; m < 3; ++m) {
@@ -1337,19 +1343,19 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         }
         if (g_bfs[BK_TD][m].cnt) {
             const int k = tb(WF_BFS);
-            launch_bfs_td(st, d_views_, L(g_bfs[BK_TD][m]), g_bfs[BK_TD][m].cnt, maxof(g_bfs[BK_TD][m], [&](int i) { return plans[i].bfs_grid; }), m);
+            launch_bfs_td(st, d_views_, L(g_bfs[BK_TD][m]), g_bfs[BK_TD][m].cnt, fit(g_bfs[BK_TD][m], maxof(g_bfs[BK_TD][m], [&](int i) { return plans[i].bfs_grid; })), m);
             te(k);
         }
         if (g_bfs[BK_BU][m].cnt) {
             const int k = tb(WF_BFS_BU);
-            launch_bfs_bu(st, d_views_, L(g_bfs[BK_BU][m]), g_bfs[BK_BU][m].cnt, maxof(g_bfs[BK_BU][m], [&](int i) { return plans[i].bfs_grid; }), m);
+            launch_bfs_bu(st, d_views_, L(g_bfs[BK_BU][m]), g_bfs[BK_BU][m].cnt, fit(g_bfs[BK_BU][m], maxof(g_bfs[BK_BU][m], [&](int i) { return plans[i].bfs_grid; })), m);
             te(k);
         }
     }
     if (g_mask.cnt) {
         HIPCHK(hipMemsetAsync(d_hash_, 0, sizeof(unsigned long long) * nslots_, st));
         const int k = tb(WF_BFS);
-        launch_mask_combine(st, d_views_, L(g_mask), g_mask.cnt, maxof(g_mask, gridN), d_hash_);
+        launch_mask_combine(st, d_views_, L(g_mask), g_mask.cnt, fit(g_mask, maxof(g_mask, gridN)), d_hash_);
         te(k);
     }
     if (g_dis[DK_TILE].cnt) {
@@ -1359,7 +1365,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     }
     if (g_dis[DK_LIST].cnt) {
         const int k = tb(WF_DISCHARGE);
-        launch_discharge_list(st, d_views_, L(g_dis[DK_LIST]), g_dis[DK_LIST].cnt, maxof(g_dis[DK_LIST], [&](int i) { return plans[i].dis_grid; }), giants(g_dis[DK_LIST]));
+        launch_discharge_list(st, d_views_, L(g_dis[DK_LIST]), g_dis[DK_LIST].cnt, fit(g_dis[DK_LIST], maxof(g_dis[DK_LIST], [&](int i) { return plans[i].dis_grid; })), giants(g_dis[DK_LIST]));
         te(k);
     }
     if (g_dis[DK_PATH].cnt) {
@@ -1379,7 +1385,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         te(k);
     }
     for (auto& ag : act_groups) launch_reset(st, d_views_, L(ag.second), ag.second.cnt, ag.first);
-    if (g_act.cnt) { const int k = tb(WF_ACTIVE); launch_active(st, d_views_, L(g_act), g_act.cnt, maxof(g_act, gridN)); te(k); }
+    if (g_act.cnt) { const int k = tb(WF_ACTIVE); launch_active(st, d_views_, L(g_act), g_act.cnt, fit(g_act, maxof(g_act, gridN))); te(k); }
     if (g_act_sp.cnt) { const int k = tb(WF_ACTIVE); launch_active_sparse(st, d_views_, L(g_act_sp), g_act_sp.cnt, 512); te(k); }
     // further discharge + count cycles of fused steps (deep graphs): the
     // count after each only looks at the previous list and the vertices the
@@ -1464,7 +1470,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
             emit(true);
         }
     }
-    if (g_cert.cnt) { const int k = tb(WF_CERT); launch_cert(st, d_views_, L(g_cert), g_cert.cnt, maxof(g_cert, gridN), P_.cert); te(k); }
+    if (g_cert.cnt) { const int k = tb(WF_CERT); launch_cert(st, d_views_, L(g_cert), g_cert.cnt, fit(g_cert, maxof(g_cert, gridN)), P_.cert); te(k); }
     HIPCHK(hipGetLastError());
     if (use_side) HIPCHK(hipStreamWaitEvent(st, ev_side_, 0));
     HIPCHK(hipMemcpyAsync(h_ctr_, d_ctrs_, sizeof(Ctr) * nslots_, hipMemcpyDeviceToHost, st));

@@ -641,8 +641,7 @@ __global__ __launch_bounds__(256) void k_bfs_td(const SlotView* __restrict__ slo
     }
     __syncthreads();
     Work wk;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < fsz) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < fsz; i += gridDim.x * blockDim.x) {
         const int v = in[i];
         wk.verts++;
         const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
@@ -1963,8 +1962,9 @@ __global__ __launch_bounds__(256) void k_bfs_td_giant(const SlotView* __restrict
 
 void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
                    int mode) {
-    // gridx = frontier / 256 workgroups; medium rows get up to 2048
-    // workgroups of 4 waves, giant rows 1024 workgroups of chunk tasks
+    // gridx: up to frontier / 256 workgroups (k_bfs_td is grid-stride);
+    // medium rows get up to 2048 workgroups of 4 waves, giant rows 1024
+    // workgroups of chunk tasks
     const int gh = gridx * 64 < 2048 ? gridx * 64 : 2048;
     if (mode == BFS_GR) {
         hipLaunchKernelGGL(k_bfs_td<BFS_GR>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
