@@ -1267,12 +1267,16 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     auto tb = [&](int fam, int bucket = -1) { return tbs(fam, st, bucket); };
     auto te = [&](int k) { tes(k, st); };
     auto gridN = [&](int i) { return clampi(cdiv(slots_[i].view.g.n, 256), 1, 4096); };
-    // vertex- and entry-parallel kernels are grid-stride: at most about
-    // kLaunchBlocks workgroups per launch over all its slots (several times
-    // what the device holds at once), so each workgroup's fixed costs
-    // (staging, block reductions, barriers) spread over several chunks
-    constexpr int kLaunchBlocks = 16384;
-    auto fit = [&](const Group& g, int gx) { return std::max(1, std::min(gx, std::max(64, kLaunchBlocks / std::max(1, g.cnt)))); };
+    // vertex- and entry-parallel kernels are grid-stride: at most 4096
+    // workgroups per slot and about 16K per launch over all its slots
+    // (several times what the device holds at once), so each workgroup's
+    // fixed costs (staging, block reductions, barriers) spread over several
+    // chunks
+    constexpr int kLaunchBlocks = 16384, kSlotBlocks = 4096;
+    auto fit = [&](const Group& g, int gx) {
+        const int cap = std::min(kSlotBlocks, std::max(64, kLaunchBlocks / std::max(1, g.cnt)));
+        return std::max(1, std::min(gx, cap));
+    };
     auto gridE = [&](int i) { return clampi(cdiv(slots_[i].view.g.E, 1024), 1, 8192); };
     auto giants = [&](const Group& g) {
         for (int k = 0; k < g.cnt; ++k)
