@@ -847,9 +847,17 @@ __global__ void k_bfs_advance(const SlotView* __restrict__ slots, const int* __r
     }
 }
 
+// Workgroups per slot for a grid-stride kernel that would like `want`: at
+// most about 16K over the nlist slots of the launch (fewer, fuller
+// workgroups; every one of them is dispatched even when it has no work).
+static int per_slot_grid(int want, int nlist) {
+    const int cap = 16384 / (nlist > 0 ? nlist : 1);
+    return want < cap ? want : (cap > 16 ? cap : 16);
+}
+
 void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
                    int mode) {
-    const int gh = gridx < 2048 ? gridx : 2048;
+    const int gh = per_slot_grid(gridx < 2048 ? gridx : 2048, nlist);
     if (mode == BFS_GR) {
         hipLaunchKernelGGL(k_bfs_bu<BFS_GR>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
         hipLaunchKernelGGL(k_bfs_bu_heavy<BFS_GR>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
@@ -1965,21 +1973,22 @@ void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int n
     // gridx: up to frontier / 256 workgroups (k_bfs_td is grid-stride);
     // medium rows get up to 2048 workgroups of 4 waves, giant rows 1024
     // workgroups of chunk tasks
-    const int gh = gridx * 64 < 2048 ? gridx * 64 : 2048;
+    const int gh = per_slot_grid(gridx * 64 < 2048 ? gridx * 64 : 2048, nlist);
+    const int gg = per_slot_grid(1024, nlist);
     if (mode == BFS_GR) {
         hipLaunchKernelGGL(k_bfs_td<BFS_GR>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
         hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
-        hipLaunchKernelGGL(k_bfs_td_giant<BFS_GR>, dim3(1024, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_td_giant<BFS_GR>, dim3(gg, nlist), dim3(256), 0, st, slots, list);
         hipLaunchKernelGGL(k_bfs_td_heavy<BFS_GR>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
     } else if (mode == BFS_FWD) {
         hipLaunchKernelGGL(k_bfs_td<BFS_FWD>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
         hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
-        hipLaunchKernelGGL(k_bfs_td_giant<BFS_FWD>, dim3(1024, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_td_giant<BFS_FWD>, dim3(gg, nlist), dim3(256), 0, st, slots, list);
         hipLaunchKernelGGL(k_bfs_td_heavy<BFS_FWD>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
     } else {
         hipLaunchKernelGGL(k_bfs_td<BFS_UND>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
         hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
-        hipLaunchKernelGGL(k_bfs_td_giant<BFS_UND>, dim3(1024, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_td_giant<BFS_UND>, dim3(gg, nlist), dim3(256), 0, st, slots, list);
         hipLaunchKernelGGL(k_bfs_td_heavy<BFS_UND>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
     }
     hipLaunchKernelGGL(k_bfs_advance, dim3(1, nlist), dim3(64), 0, st, slots, list);
@@ -1987,7 +1996,7 @@ void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int n
 
 void launch_giant(hipStream_t st, const SlotView* slots, const int* list, int nlist) {
     hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
-    hipLaunchKernelGGL(k_giant_push, dim3(512, nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_giant_push, dim3(per_slot_grid(512, nlist), nlist), dim3(256), 0, st, slots, list);
     hipLaunchKernelGGL(k_giant_relabel, dim3(1, nlist), dim3(256), 0, st, slots, list);
 }
 
@@ -2576,7 +2585,7 @@ void launch_cert(hipStream_t st, const SlotView* slots, const int* list, int nli
     }
     hipLaunchKernelGGL(k_cert, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
     hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
-    hipLaunchKernelGGL(k_cert_chunks, dim3(1024, nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_cert_chunks, dim3(per_slot_grid(1024, nlist), nlist), dim3(256), 0, st, slots, list);
     hipLaunchKernelGGL(k_cert_fin, dim3(4, nlist), dim3(256), 0, st, slots, list);
 }
 
