@@ -95,7 +95,8 @@ struct SlotView {
     int32_t* lbkt = nullptr;     // 2 * kLocalLevels local-relabel bucket counts and cursors
     // giant-row list (rows longer than kGiantDeg; at most E / (kGiantDeg+1)
     // of them, gcap): ids [0, gcap), per-row minimum [gcap, 2 gcap), chunk
-    // prefix [2 gcap, 3 gcap]; gacc[gcap] per-row int64 sums
+    // prefix [2 gcap, 3 gcap], then the row of each chunk task (at most
+    // E / kGiantChunk + gcap tasks); gacc[gcap] per-row int64 sums
     int32_t* giant = nullptr;
     int64_t* gacc = nullptr;
     int32_t gcap = 0;

@@ -1744,6 +1744,7 @@ __global__ __launch_bounds__(1024) void k_giant_prep(const SlotView* __restrict_
     __shared__ int carry;
     const int ng = S.ctr->ngiant;
     GAS int32_t* pre = S.giant + 2 * S.gcap;
+    GAS int32_t* tmap = pre + S.gcap + 1;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (int base = 0; base < ng; base += blockDim.x) {
@@ -1770,7 +1771,11 @@ __global__ __launch_bounds__(1024) void k_giant_prep(const SlotView* __restrict_
             if (j < w) before += wsum[j];
             all += wsum[j];
         }
-        if (g < ng) pre[g] = before + x - c;
+        if (g < ng) {
+            const int p = before + x - c;
+            pre[g] = p;
+            for (int j = 0; j < c; ++j) tmap[p + j] = g;  // row of each of its tasks
+        }
         __syncthreads();
         if (threadIdx.x == 0) carry += all;
         __syncthreads();
@@ -1781,22 +1786,15 @@ __global__ __launch_bounds__(1024) void k_giant_prep(const SlotView* __restrict_
     }
 }
 
-// Task -> (row k, chunk c) of the slot's giant list, found by thread 0 and
-// shared through LDS.
-__device__ __forceinline__ void giant_task(const GSlot& S, int task, int ng, int* sh) {
-    __syncthreads();  // the previous task's reads of sh are done
-    if (threadIdx.x == 0) {
-        const GAS int32_t* pre = S.giant + 2 * S.gcap;
-        int lo = 0, hi = ng - 1;  // last k with pre[k] <= task
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (pre[mid] <= task) lo = mid; else hi = mid - 1;
-        }
-        sh[0] = lo;
-        sh[1] = task - pre[lo];
-        sh[2] = S.giant[lo];
-    }
-    __syncthreads();
+// Task -> (row k, chunk c, vertex v) of the slot's giant list, from the task
+// map k_giant_prep wrote: every thread reads the same three words, so no
+// barrier is needed.
+struct GTask { int k, c, v; };
+__device__ __forceinline__ GTask giant_task(const GSlot& S, int task) {
+    const GAS int32_t* pre = S.giant + 2 * S.gcap;
+    const GAS int32_t* tmap = pre + S.gcap + 1;
+    const int k = tmap[task];
+    return GTask{k, task - pre[k], S.giant[k]};
 }
 
 // k_giant_push: one pass over each chunk of a giant row.  The chunk's
@@ -1824,14 +1822,13 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
     __shared__ uint64_t red[256 / WAVE];
     __shared__ int sh[4];
     constexpr int PER = kGiantChunk / 256;
-    const int ng = S.ctr->ngiant;
     const int T = S.ctr->ngchunks;
     if ((int)blockIdx.x >= T) return;  // uniform per block
     const int n = S.g.n;
     Work wk;
     for (int task = blockIdx.x; task < T; task += gridDim.x) {
-        giant_task(S, task, ng, sh);
-        const int k = sh[0], c = sh[1], v = sh[2];
+        const GTask gt = giant_task(S, task);
+        const int k = gt.k, c = gt.c, v = gt.v;
         if (threadIdx.x == 0) {
             bcast = ld64(&S.excess[v]);
             sh[3] = S.h[v];
@@ -1939,8 +1936,6 @@ __global__ __launch_bounds__(256) void k_bfs_td_giant(const SlotView* __restrict
     const GSlot S = load_slot(slots, list);
     __shared__ Stage<2048> st;
     __shared__ uint64_t red[256 / WAVE];
-    __shared__ int sh[4];
-    const int ng = S.ctr->ngiant;
     const int T = S.ctr->ngchunks;
     if ((int)blockIdx.x >= T) return;  // uniform per block
     const int L = S.ctr->level;
@@ -1951,8 +1946,8 @@ __global__ __launch_bounds__(256) void k_bfs_td_giant(const SlotView* __restrict
     stage_init(st);
     Work wk;
     for (int task = blockIdx.x; task < T; task += gridDim.x) {
-        giant_task(S, task, ng, sh);
-        const int c = sh[1], v = sh[2];
+        const GTask gt = giant_task(S, task);
+        const int c = gt.c, v = gt.v;
         const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
         const int e0 = rb + c * kGiantChunk;
         const int e1 = min(re, e0 + kGiantChunk);
@@ -2537,16 +2532,14 @@ __global__ __launch_bounds__(256) void k_cert_chunks(const SlotView* __restrict_
     const GSlot S = load_slot(slots, list);
     __shared__ int64_t red64[256 / WAVE];
     __shared__ uint64_t redu[256 / WAVE];
-    __shared__ int sh[4];
-    const int ng = S.ctr->ngiant;
     const int T = S.ctr->ngchunks;
     if ((int)blockIdx.x >= T) return;  // uniform per block
     const int n = S.g.n;
     int64_t cut = 0;
     uint64_t bad = 0, edges = 0;
     for (int task = blockIdx.x; task < T; task += gridDim.x) {
-        giant_task(S, task, ng, sh);
-        const int k = sh[0], c = sh[1], v = sh[2];
+        const GTask gt = giant_task(S, task);
+        const int k = gt.k, c = gt.c, v = gt.v;
         const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
         const int e0 = rb + c * kGiantChunk;
         const int e1 = min(re, e0 + kGiantChunk);
