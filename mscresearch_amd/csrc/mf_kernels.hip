@@ -740,7 +740,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
                 // the level gather only for entries that pass the residual
                 // test (most reverse entries carry no flow)
                 int scanned = 0;
-                const bool hit = bu_row_lane<MODE, 4>(S, rb, re, L, lvl, scanned);
+                const bool hit = bu_row_lane<MODE, 2>(S, rb, re, L, lvl, scanned);
                 wk.edges += (uint64_t)scanned;
                 if (hit) {
                     lvl[v] = L + 1;  // only this lane writes v
