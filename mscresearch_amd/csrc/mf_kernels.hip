@@ -2493,11 +2493,15 @@ __global__ __launch_bounds__(256) void k_cert_fp(const SlotView* __restrict__ sl
 #pragma unroll
             for (int j = 0; j < U; ++j) {
                 const int e = eb + j * 256;
-                int rr = lo;  // last row with rp[row] <= e, within lo + 255
+                // last row with rp[row] <= e, within lo + 255: binary lifting
+                // without branches (a probe past the last row reads the last
+                // row, which is then the answer exactly when it starts at or
+                // before e; row starts never decrease)
+                int rr = lo;
 #pragma unroll
                 for (int step = 128; step > 0; step >>= 1) {
-                    const int c = rr + step;
-                    if (c < nv && rp[c] <= e) rr = c;
+                    const int c = min(rr + step, nv - 1);
+                    rr = rp[c] <= e ? c : rr;
                 }
                 r[j] = rr;
                 x[j] = 0;
