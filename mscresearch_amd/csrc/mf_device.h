@@ -31,6 +31,9 @@ struct DGraph {
     uint64_t cert_seed = 0;
 };
 
+// Offset (in int32 words, 16-byte aligned) of the giant list's task records.
+inline __host__ __device__ int64_t giant_task_offset(int64_t gcap) { return (3 * gcap + 1 + 3) / 4 * 4; }
+
 // Per-slot device counters.  Written by kernels, copied to the host once per
 // scheduler step.  Field order is part of the host/device contract.
 struct Ctr {
@@ -95,7 +98,8 @@ struct SlotView {
     int32_t* lbkt = nullptr;     // 2 * kLocalLevels local-relabel bucket counts and cursors
     // giant-row list (rows longer than kGiantDeg; at most E / (kGiantDeg+1)
     // of them, gcap): ids [0, gcap), per-row minimum [gcap, 2 gcap), chunk
-    // prefix [2 gcap, 3 gcap], then the row of each chunk task (at most
+    // prefix [2 gcap, 3 gcap], then from giant_task_offset(gcap) one int4
+    // per chunk task (row, vertex, first entry, end entry; at most
     // E / kGiantChunk + gcap tasks); gacc[gcap] per-row int64 sums
     int32_t* giant = nullptr;
     int64_t* gacc = nullptr;

@@ -157,7 +157,7 @@ size_t Slab::layout(int n, int64_t E, int ntiles, char* base, Slab* sl) {
     char* p_tc = take(4 * tt);
     char* p_th = take(4 * tt);
     char* p_lb = take(sizeof(int32_t) * 2 * kLocalLevels);
-    char* p_gi = take(sizeof(int32_t) * (3 * gcap + 1 + ee / kGiantChunk + gcap + 1));
+    char* p_gi = take(sizeof(int32_t) * (giant_task_offset((int64_t)gcap) + 4 * (ee / kGiantChunk + gcap + 1)));
     char* p_ga = take(sizeof(int64_t) * gcap);
     char* p_em = take(ee);
     if (sl) {

@@ -1737,6 +1737,10 @@ __global__ __launch_bounds__(256) void k_discharge_list(const SlotView* __restri
 // kGiantChunk entries.  k_giant_prep turns the slot's giant list into a
 // prefix of task counts (one workgroup per slot); a workgroup maps a task to
 // (row, chunk) by binary search.
+// 16-byte task record of the giant list (a vector type keeps the load one
+// instruction): row in the list, vertex, first entry, end entry.
+typedef int TaskRec __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(1024) void k_giant_prep(const SlotView* __restrict__ slots,
                                                      const int* __restrict__ list) {
     const GSlot S = load_slot(slots, list);
@@ -1744,15 +1748,17 @@ __global__ __launch_bounds__(1024) void k_giant_prep(const SlotView* __restrict_
     __shared__ int carry;
     const int ng = S.ctr->ngiant;
     GAS int32_t* pre = S.giant + 2 * S.gcap;
-    GAS int32_t* tmap = pre + S.gcap + 1;
+    GAS TaskRec* rec = (GAS TaskRec*)(S.giant + giant_task_offset(S.gcap));
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (int base = 0; base < ng; base += blockDim.x) {
         const int g = base + (int)threadIdx.x;
-        int c = 0;
+        int c = 0, v = 0, rb = 0, re = 0;
         if (g < ng) {
-            const int v = S.giant[g];
-            c = (S.g.row_ptr[v + 1] - S.g.row_ptr[v] + kGiantChunk - 1) / kGiantChunk;
+            v = S.giant[g];
+            rb = S.g.row_ptr[v];
+            re = S.g.row_ptr[v + 1];
+            c = (re - rb + kGiantChunk - 1) / kGiantChunk;
             S.gacc[g] = 0;
         }
         // block exclusive scan of c
@@ -1774,7 +1780,10 @@ __global__ __launch_bounds__(1024) void k_giant_prep(const SlotView* __restrict_
         if (g < ng) {
             const int p = before + x - c;
             pre[g] = p;
-            for (int j = 0; j < c; ++j) tmap[p + j] = g;  // row of each of its tasks
+            for (int j = 0; j < c; ++j) {  // one record per task: row, vertex, entry range
+                const int e0 = rb + j * kGiantChunk;
+                rec[p + j] = TaskRec{g, v, e0, min(re, e0 + kGiantChunk)};
+            }
         }
         __syncthreads();
         if (threadIdx.x == 0) carry += all;
@@ -1786,15 +1795,13 @@ __global__ __launch_bounds__(1024) void k_giant_prep(const SlotView* __restrict_
     }
 }
 
-// Task -> (row k, chunk c, vertex v) of the slot's giant list, from the task
-// map k_giant_prep wrote: every thread reads the same three words, so no
-// barrier is needed.
-struct GTask { int k, c, v; };
+// Task -> (row k in the giant list, vertex v, entries [e0, e1)), the record
+// k_giant_prep wrote: every thread reads the same 16 bytes, so no barrier is
+// needed.
+struct GTask { int k, v, e0, e1; };
 __device__ __forceinline__ GTask giant_task(const GSlot& S, int task) {
-    const GAS int32_t* pre = S.giant + 2 * S.gcap;
-    const GAS int32_t* tmap = pre + S.gcap + 1;
-    const int k = tmap[task];
-    return GTask{k, task - pre[k], S.giant[k]};
+    const TaskRec r = ((const GAS TaskRec*)(S.giant + giant_task_offset(S.gcap)))[task];
+    return GTask{r.x, r.y, r.z, r.w};
 }
 
 // k_giant_push: one pass over each chunk of a giant row.  The chunk's
@@ -1828,7 +1835,7 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
     Work wk;
     for (int task = blockIdx.x; task < T; task += gridDim.x) {
         const GTask gt = giant_task(S, task);
-        const int k = gt.k, c = gt.c, v = gt.v;
+        const int k = gt.k, v = gt.v;
         if (threadIdx.x == 0) {
             bcast = ld64(&S.excess[v]);
             sh[3] = S.h[v];
@@ -1838,9 +1845,7 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
         const int hv = sh[3];
         __syncthreads();
         if (ex0 <= 0 || hv >= n) continue;  // dry or cut off: no report, no relabel
-        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
-        const int e0 = rb + c * kGiantChunk;
-        const int e1 = min(re, e0 + kGiantChunk);
+        const int e0 = gt.e0, e1 = gt.e1;
         int64_t r[PER];
         int hw[PER], cw[PER];
         int64_t tsum = 0;
@@ -1947,10 +1952,7 @@ __global__ __launch_bounds__(256) void k_bfs_td_giant(const SlotView* __restrict
     Work wk;
     for (int task = blockIdx.x; task < T; task += gridDim.x) {
         const GTask gt = giant_task(S, task);
-        const int c = gt.c, v = gt.v;
-        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
-        const int e0 = rb + c * kGiantChunk;
-        const int e1 = min(re, e0 + kGiantChunk);
+        const int e0 = gt.e0, e1 = gt.e1;
         if (threadIdx.x == 0) wk.edges += (uint64_t)(e1 - e0);
         td_entries<MODE, 4>(S, e0 + (int)threadIdx.x, e1, (int)blockDim.x, L, lvl, [&](int w) {
             stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
@@ -2543,10 +2545,8 @@ __global__ __launch_bounds__(256) void k_cert_chunks(const SlotView* __restrict_
     uint64_t bad = 0, edges = 0;
     for (int task = blockIdx.x; task < T; task += gridDim.x) {
         const GTask gt = giant_task(S, task);
-        const int k = gt.k, c = gt.c, v = gt.v;
-        const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
-        const int e0 = rb + c * kGiantChunk;
-        const int e1 = min(re, e0 + kGiantChunk);
+        const int k = gt.k, v = gt.v;
+        const int e0 = gt.e0, e1 = gt.e1;
         const bool inT = S.h[v] < n;
         int64_t ex = 0;
         for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) cert_entry(S, e, inT, n, ex, bad, cut);
