@@ -1836,6 +1836,21 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
     for (int task = blockIdx.x; task < T; task += gridDim.x) {
         const GTask gt = giant_task(S, task);
         const int k = gt.k, v = gt.v;
+        const int e0 = gt.e0, e1 = gt.e1;
+        // the chunk's residuals and columns, in flight while the row's
+        // excess and height come to every thread
+        int64_t r[PER];
+        int hw[PER], cw[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
+            r[j] = 0;
+            cw[j] = 0;
+            if (e < e1) {
+                r[j] = ld64(&S.res[e]);
+                cw[j] = S.g.col[e];
+            }
+        }
         if (threadIdx.x == 0) {
             bcast = ld64(&S.excess[v]);
             sh[3] = S.h[v];
@@ -1845,24 +1860,15 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
         const int hv = sh[3];
         __syncthreads();
         if (ex0 <= 0 || hv >= n) continue;  // dry or cut off: no report, no relabel
-        const int e0 = gt.e0, e1 = gt.e1;
-        int64_t r[PER];
-        int hw[PER], cw[PER];
         int64_t tsum = 0;
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
-            r[j] = 0;
             hw[j] = INT_MAX;
-            cw[j] = 0;
-            if (e < e1) {
-                const int64_t x = ld64(&S.res[e]);
-                if (x > 0) {
-                    r[j] = x;
-                    cw[j] = S.g.col[e];
-                    hw[j] = S.h[cw[j]];
-                    if (hw[j] < hv) tsum += x;
-                }
+            if (r[j] > 0) {
+                hw[j] = S.h[cw[j]];
+                if (hw[j] < hv) tsum += r[j];
+            } else {
+                r[j] = 0;
             }
         }
         int64_t tot = 0;
