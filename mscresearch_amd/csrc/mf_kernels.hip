@@ -699,6 +699,11 @@ __global__ __launch_bounds__(256) void k_bfs_td_heavy(const SlotView* __restrict
     work_flush(S, wk, red, WF_BFS);
 }
 
+// Bottom-up rows up to this many entries are scanned by one lane (a row
+// stops at its first parent, so a wave's 64-entry steps would mostly load
+// entries past it); longer ones by a wave.
+constexpr int kBuLightDeg = 128;
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slots,
                                                 const int* __restrict__ list) {
@@ -734,7 +739,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
             const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
             if (re - rb > kGiantDeg) {
                 S.hl[add32(&S.ctr->nhl, 1)] = v;
-            } else if (re - rb > kLightDeg) {
+            } else if (re - rb > kBuLightDeg) {
                 med[atomicAdd(&nmed, 1)] = v;
             } else {
                 // the level gather only for entries that pass the residual
