@@ -30,7 +30,7 @@ struct Params {
     int K = 0;                 // cycles per vertex per discharge pass; 0 = max(4, E/V)
     int gr_freq = 128;         // GR every gr_freq iterations (BC:1466)
     int gr_freq_min = 128, gr_freq_max = 256;
-    int gr_stall = 3;          // also GR after excess[t] stalls this many iterations (0 = off)
+    int gr_stall = 1;          // also GR after excess[t] stalls this many iterations (0 = off)
     int h_cut = 100;           // BC:1469
     int stagnate_rounds = 128; // BC:1472
     int path_switch = 8;       // BC:1480
