@@ -122,7 +122,7 @@ def serve_bench(dg22, hg22, e22, args):
     hg20 = M.HostGraph.from_edgelist(e20).partition(512)
     dg20 = M.DeviceGraph(hg20)
     log(f"serving graphs ready in {time.time() - t0:.1f}s")
-    lanes = [("grid1024", dgg, hgg, 32, M.default_params(fuse=32)),
+    lanes = [("grid1024", dgg, hgg, args.serve_grid_slots, M.default_params(fuse=32, small_wgs=args.serve_grid_wgs)),
              ("rmat20", dg20, hg20, 32, M.default_params()),
              ("rmat22", dg22, hg22, 48, M.default_params())]
     engines, budgets = [], []
@@ -280,6 +280,9 @@ def main():
     ap.add_argument("--serve-qps", type=float, nargs="+", default=[50.0, 200.0, 800.0])
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-max", type=int, default=1200)
+    ap.add_argument("--serve-grid-slots", type=int, default=32, help="slots of the grid lane")
+    ap.add_argument("--serve-grid-wgs", type=int, default=8,
+                    help="workgroups per slot of the grid lane's deep relabel BFS")
     ap.add_argument("--serve-slabs", type=float, default=24.0,
                     help="slab arena budget in full R-MAT 22 slabs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
