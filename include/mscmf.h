@@ -164,6 +164,7 @@ typedef struct mf_params {
     int spec_cert;       /* 1 = certify on the discharge's labels before a terminating exact GR */
     int sparse_count;    /* 1 = count actives from the previous list and the vertices pushed into */
     int small_wgs;       /* workgroups per slot of the multi-level small-frontier BFS (1 = one) */
+    long long small_edges; /* ... while the frontier's rows sum to at most this many entries */
 } mf_params;
 
 void mf_params_default(mf_params* p);

@@ -45,6 +45,7 @@ class Params(C.Structure):
         ("prune", C.c_int), ("prune_ratio", C.c_double), ("gr_work", C.c_double), ("K_heavy", C.c_int),
         ("local_relabel", C.c_int), ("local_band", C.c_int), ("fuse", C.c_int), ("graphs", C.c_int),
         ("spec_cert", C.c_int), ("sparse_count", C.c_int), ("small_wgs", C.c_int),
+        ("small_edges", C.c_longlong),
     ]
 
 

@@ -109,6 +109,7 @@ Params from_c(const mf_params* p) {
     P.spec_cert = p->spec_cert;
     P.sparse_count = p->sparse_count;
     P.small_wgs = p->small_wgs;
+    P.small_edges = p->small_edges;
     return P;
 }
 
@@ -449,6 +450,7 @@ void mf_params_default(mf_params* p) {
     p->spec_cert = P.spec_cert;
     p->sparse_count = P.sparse_count;
     p->small_wgs = P.small_wgs;
+    p->small_edges = P.small_edges;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {
@@ -756,7 +758,7 @@ static int dbg_bfs(mf_engine* e, Plan first, int mode) {
         if (!bu && (double)c.fedges[L & 3] > (double)mu / P.bu_alpha) bu = true;
         else if (bu && (double)f < (double)g.n / P.bu_beta) bu = false;
         if (bu) { p.bfs_kind = BK_BU; p.bfs_grid = std::max(1, std::min(8192, (g.n + 255) / 256)); }
-        else if (f <= kSmallFrontier && c.fedges[L & 3] <= kSmallEdges) p.bfs_kind = BK_SMALL;
+        else if (f <= kSmallFrontier && c.fedges[L & 3] <= P.small_edges) p.bfs_kind = BK_SMALL;
         else { p.bfs_kind = BK_TD; p.bfs_grid = (f + 255) / 256; }
         rc = e->eng.dbg_plan(p);
         if (rc != MF_OK) return rc;

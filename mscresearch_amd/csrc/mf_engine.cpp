@@ -344,7 +344,7 @@ Plan SlotSched::bfs_step(const Ctr& c, int mode) {
     if (bfs_bottom_up_) {
         p.bfs_kind = BK_BU;
         p.bfs_grid = clampi(cdiv(g_.n, 256), 1, 8192);
-    } else if (f <= kSmallFrontier && mf <= kSmallEdges) {
+    } else if (f <= kSmallFrontier && mf <= P_->small_edges) {
         p.bfs_kind = BK_SMALL;
         if (gr_depth_ >= 64 || depth_hint_ >= 64 || L >= 64) p.small_wgs = std::max(1, P_->small_wgs);
     } else {
@@ -1298,7 +1298,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         for (int m = 0; m < 3; ++m)
             for (int w = 0; w < 2; ++w)
                 if (s_bfs[m][w].cnt)
-                    launch_bfs_small(side_, d_views_, L(s_bfs[m][w]), s_bfs[m][w].cnt, m, P_.small_levels, w ? P_.small_wgs : 1, spin_ticks_);
+                    launch_bfs_small(side_, d_views_, L(s_bfs[m][w]), s_bfs[m][w].cnt, m, P_.small_levels, P_.small_edges, w ? P_.small_wgs : 1, spin_ticks_);
         tes(k, side_);
         HIPCHK(hipEventRecord(ev_side_, side_));
     }
@@ -1337,12 +1337,12 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     for (int m = 0; m < 3; ++m) {
         if (g_bfs[BK_SMALL][m].cnt) {
             const int k = tb(WF_BFS);
-            launch_bfs_small(st, d_views_, L(g_bfs[BK_SMALL][m]), g_bfs[BK_SMALL][m].cnt, m, P_.small_levels, 1, spin_ticks_);
+            launch_bfs_small(st, d_views_, L(g_bfs[BK_SMALL][m]), g_bfs[BK_SMALL][m].cnt, m, P_.small_levels, P_.small_edges, 1, spin_ticks_);
             te(k);
         }
         if (g_bfs_multi[m].cnt) {
             const int k = tb(WF_BFS);
-            launch_bfs_small(st, d_views_, L(g_bfs_multi[m]), g_bfs_multi[m].cnt, m, P_.small_levels, P_.small_wgs, spin_ticks_);
+            launch_bfs_small(st, d_views_, L(g_bfs_multi[m]), g_bfs_multi[m].cnt, m, P_.small_levels, P_.small_edges, P_.small_wgs, spin_ticks_);
             te(k);
         }
         if (g_bfs[BK_TD][m].cnt) {

@@ -53,6 +53,12 @@ struct Params {
     // each level, so a level's scattered loads use several CUs; a shallow
     // BFS keeps one workgroup, its few levels not worth a barrier each.
     int small_wgs = 8;
+    // a frontier whose rows sum to more entries than this leaves the
+    // single-workgroup BFS for the grid-wide top-down or bottom-up levels:
+    // a hub's row is spread over the device instead of one CU (+4% on R-MAT
+    // 22 hub queries against 2^18; grid frontiers of up to kSmallFrontier
+    // degree-4 vertices stay inside)
+    long long small_edges = 1 << 16;
     double bu_alpha = 14.0, bu_beta = 24.0;  // direction-optimizing BFS thresholds
     int prune = 1;             // per-query reachability pruning (BC:2678)
     double prune_ratio = 0.995;  // BC:2647

@@ -64,9 +64,10 @@ void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int n
 // Multi-level top-down BFS while the frontier stays small: one workgroup
 // (wgs == 1), or wgs workgroups per slot meeting at a barrier in global
 // memory after each level (spin_ticks: wall-clock ticks a barrier may wait
-// before it gives up and flags the slot's error word).
+// before it gives up and flags the slot's error word).  It stops above
+// kSmallFrontier vertices or max_edges frontier row entries.
 void launch_bfs_small(hipStream_t st, const SlotView* slots, const int* list, int nlist, int mode,
-                      int max_levels, int wgs, long long spin_ticks);
+                      int max_levels, long long max_edges, int wgs, long long spin_ticks);
 
 // Active list (light entries at fa[0..), heavy at fa[n-1..]) and counters.
 void launch_active(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);

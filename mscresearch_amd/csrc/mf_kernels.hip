@@ -882,7 +882,7 @@ void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int n
 // launch per level would cost more than the level itself.
 template <int MODE>
 __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__ slots,
-                                                    const int* __restrict__ list, int max_levels) {
+                                                    const int* __restrict__ list, int max_levels, long long max_edges) {
     const GSlot S = load_slot(slots, list);
     __shared__ int heavy[1024];
     __shared__ int nheavy;
@@ -896,7 +896,7 @@ __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__
     int done_levels = 0;
     int visited = 0;
     int64_t fdeg = S.ctr->fedges[L & 3];
-    while (fsz > 0 && fsz <= kSmallFrontier && fdeg <= kSmallEdges && done_levels < max_levels) {
+    while (fsz > 0 && fsz <= kSmallFrontier && fdeg <= max_edges && done_levels < max_levels) {
         const GAS int32_t* in = (L & 1) ? S.fb : S.fa;
         GAS int32_t* out = (L & 1) ? S.fa : S.fb;
         if (threadIdx.x == 0) { nheavy = 0; next_cnt = 0; next_deg = 0; }
@@ -998,7 +998,7 @@ __device__ __forceinline__ bool slot_barrier(GAS Ctr* c, uint32_t& gen, long lon
 template <int MODE>
 __global__ __launch_bounds__(256) void k_bfs_multi(const SlotView* __restrict__ slots,
                                                    const int* __restrict__ list, int max_levels,
-                                                   long long spin_ticks) {
+                                                   long long max_edges, long long spin_ticks) {
     const GSlot S = load_slot(slots, list);
     __shared__ Stage<1024> st;
     __shared__ int heavy[256];
@@ -1017,7 +1017,7 @@ __global__ __launch_bounds__(256) void k_bfs_multi(const SlotView* __restrict__ 
     Work wk;
     int done_levels = 0;
     int visited = 0;
-    while (fsz > 0 && fsz <= kSmallFrontier && fdeg <= kSmallEdges && done_levels < max_levels) {
+    while (fsz > 0 && fsz <= kSmallFrontier && fdeg <= max_edges && done_levels < max_levels) {
         const GAS int32_t* in = (L & 1) ? S.fb : S.fa;
         GAS int32_t* out = (L & 1) ? S.fa : S.fb;
         GAS int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
@@ -1080,20 +1080,20 @@ __global__ __launch_bounds__(256) void k_bfs_multi(const SlotView* __restrict__ 
 }
 
 void launch_bfs_small(hipStream_t st, const SlotView* slots, const int* list, int nlist, int mode,
-                      int max_levels, int wgs, long long spin_ticks) {
+                      int max_levels, long long max_edges, int wgs, long long spin_ticks) {
     if (wgs <= 1) {
-        if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_small<BFS_GR>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
-        else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_small<BFS_FWD>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
-        else hipLaunchKernelGGL(k_bfs_small<BFS_UND>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
+        if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_small<BFS_GR>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels, max_edges);
+        else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_small<BFS_FWD>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels, max_edges);
+        else hipLaunchKernelGGL(k_bfs_small<BFS_UND>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels, max_edges);
         return;
     }
     // at most 512 workgroups per launch: this launch and one on the other
     // stream fit on the device at once
     int g = wgs;
     while (g > 1 && (long long)g * nlist > 512) g >>= 1;
-    if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_multi<BFS_GR>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, spin_ticks);
-    else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_multi<BFS_FWD>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, spin_ticks);
-    else hipLaunchKernelGGL(k_bfs_multi<BFS_UND>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, spin_ticks);
+    if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_multi<BFS_GR>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, max_edges, spin_ticks);
+    else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_multi<BFS_FWD>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, max_edges, spin_ticks);
+    else hipLaunchKernelGGL(k_bfs_multi<BFS_UND>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, max_edges, spin_ticks);
 }
 
 // ------------------------------------------------------- local relabel (a14)
