@@ -150,7 +150,8 @@ typedef struct mf_params {
     double rm_tau;
     int tile_rounds;
     double tile_frac;    /* region mode when active >= tile_frac * n */
-    int cert;            /* certify every answer on the GPU */
+    int cert;            /* certify every answer on the GPU: 3 (default) fingerprint over the pairs
+                            pushes changed, 1 over every pair, 2 pair sums entry by entry, 0 off */
     long long iter_max;
     double bu_alpha, bu_beta;
     int prune;           /* per-query reachability pruning + induced subgraph */
@@ -342,9 +343,10 @@ int mf_dbg_induced(mf_engine* e, int gid, const int32_t* mask, int* n_out, int64
 int mf_dbg_local_relabel(mf_engine* e, int gid, int s, int t, const int64_t* res,
                          const int64_t* excess, int32_t* height, const int32_t* tile_active,
                          int lcap, int* level);
-/* Min-cut certificate (mode 1 fingerprint, 2 entry by entry) of a given
- * residual, excess and heights (T = {h < n}) on slot 0: violation count,
- * cut(V\T, T), excess(t) and whether the pair-sum fingerprint is 0. */
+/* Min-cut certificate (mode 1 fingerprint, 2 entry by entry, 3 fingerprint
+ * over the pairs whose residual left their capacities) of a given residual,
+ * excess and heights (T = {h < n}) on slot 0: violation count, cut(V\T, T),
+ * excess(t) and whether the pair-sum fingerprint is 0. */
 int mf_dbg_certify(mf_engine* e, int gid, int s, int t, const int64_t* res, const int64_t* excess,
                    const int32_t* height, int mode, int64_t* bad, int64_t* cut, int64_t* excess_t,
                    int* fp_zero);

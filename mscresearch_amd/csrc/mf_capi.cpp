@@ -777,6 +777,7 @@ int mf_dbg_global_relabel(mf_engine* e, int gid, int s, int t, const int64_t* re
     const DGraph& g = v.g;
     HIPCK(hipMemcpy(v.res, res, sizeof(int64_t) * g.E, hipMemcpyHostToDevice));
     e->eng.slab(0)->res_uid = 0;  // res no longer follows the slab's marks
+    e->eng.slab(0)->marks_ok = false;
     HIPCK(hipMemcpy(v.excess, excess, sizeof(int64_t) * g.n, hipMemcpyHostToDevice));
     Plan p;
     p.stages = ST_RESET_PRE | ST_GR_SEED | ST_BFS;
@@ -801,6 +802,7 @@ int mf_dbg_local_relabel(mf_engine* e, int gid, int s, int t, const int64_t* res
     const DGraph& g = v.g;
     HIPCK(hipMemcpy(v.res, res, sizeof(int64_t) * g.E, hipMemcpyHostToDevice));
     e->eng.slab(0)->res_uid = 0;  // res no longer follows the slab's marks
+    e->eng.slab(0)->marks_ok = false;
     HIPCK(hipMemcpy(v.excess, excess, sizeof(int64_t) * g.n, hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(v.h, height, sizeof(int32_t) * g.n, hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(v.tile_cnt, tile_active, sizeof(int32_t) * g.ntiles, hipMemcpyHostToDevice));
@@ -823,13 +825,23 @@ int mf_dbg_certify(mf_engine* e, int gid, int s, int t, const int64_t* res, cons
     int rc = check_query(e, gid, s, t);
     if (rc != MF_OK) return rc;
     if (!res || !excess || !height || !bad || !cut || !excess_t || !fp_zero) return fail(MF_EINVAL, "null argument");
-    if (mode != 1 && mode != 2) return fail(MF_EINVAL, "mode must be 1 or 2");
+    if (mode < 1 || mode > 3) return fail(MF_EINVAL, "mode must be 1, 2 or 3");
     rc = e->eng.dbg_begin(gid, s, t);
     if (rc != MF_OK) return rc;
     SlotView& v = e->eng.view(0);
     const DGraph& g = v.g;
     HIPCK(hipMemcpy(v.res, res, sizeof(int64_t) * g.E, hipMemcpyHostToDevice));
     e->eng.slab(0)->res_uid = 0;  // res no longer follows the slab's marks
+    e->eng.slab(0)->marks_ok = false;
+    if (mode == 3) {
+        // sparse mode: mark every entry whose residual left its capacity
+        std::vector<int64_t> cap((size_t)g.E);
+        HIPCK(hipMemcpy(cap.data(), g.cap, sizeof(int64_t) * g.E, hipMemcpyDeviceToHost));
+        std::vector<uint8_t> mk((size_t)g.E);
+        for (int64_t i = 0; i < g.E; ++i) mk[i] = res[i] != cap[i] ? 1 : 0;
+        HIPCK(hipMemcpy(v.emark, mk.data(), (size_t)g.E, hipMemcpyHostToDevice));
+        e->eng.slab(0)->marks_ok = true;
+    }
     HIPCK(hipMemcpy(v.excess, excess, sizeof(int64_t) * g.n, hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(v.h, height, sizeof(int32_t) * g.n, hipMemcpyHostToDevice));
     const int saved = e->eng.params().cert;

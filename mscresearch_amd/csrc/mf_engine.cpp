@@ -176,6 +176,7 @@ size_t Slab::layout(int n, int64_t E, int ntiles, char* base, Slab* sl) {
         sl->gacc = (int64_t*)p_ga;
         sl->emark = (uint8_t*)p_em;
         sl->res_uid = 0;
+        sl->marks_ok = false;
         sl->gcap = (int)gcap;
         sl->n_cap = n;
         sl->E_cap = (int)E;
@@ -1178,6 +1179,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         if (slots_[i].live && has(i, ST_INIT)) {
             slots_[i].slab.res_uid = run_g(i)->uid;
             slots_[i].slab.res_version = run_g(i)->cap_version;
+            slots_[i].slab.marks_ok = true;
         }
     Group g_pre = make_list([&](int i) { return has(i, ST_PREFLOW); });
     Group g_seed = make_list([&](int i) { return has(i, ST_GR_SEED); });
@@ -1221,7 +1223,11 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     }
     Group g_act = make_list([&](int i) { return has(i, ST_ACTIVE) && !plans[i].sparse; });
     Group g_act_sp = make_list([&](int i) { return has(i, ST_ACTIVE) && plans[i].sparse; });
-    Group g_cert = make_list([&](int i) { return has(i, ST_CERT); });
+    // the sparse certificate where the slab's marks cover its residual, the
+    // dense fingerprint elsewhere
+    const bool cert_sp = P_.cert == 3;
+    Group g_cert = make_list([&](int i) { return has(i, ST_CERT) && !(cert_sp && slots_[i].slab.marks_ok); });
+    Group g_cert_sp = make_list([&](int i) { return has(i, ST_CERT) && cert_sp && slots_[i].slab.marks_ok; });
     // slots of fused steps (see the fused-cycle launches below)
     int max_cycles = 1;
     for (int i = 0; i < nslots_; ++i)
@@ -1474,7 +1480,8 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
             emit(true);
         }
     }
-    if (g_cert.cnt) { const int k = tb(WF_CERT); launch_cert(st, d_views_, L(g_cert), g_cert.cnt, fit(g_cert, maxof(g_cert, gridN)), P_.cert); te(k); }
+    if (g_cert.cnt) { const int k = tb(WF_CERT); launch_cert(st, d_views_, L(g_cert), g_cert.cnt, fit(g_cert, maxof(g_cert, gridN)), cert_sp ? 1 : P_.cert); te(k); }
+    if (g_cert_sp.cnt) { const int k = tb(WF_CERT); launch_cert(st, d_views_, L(g_cert_sp), g_cert_sp.cnt, fit(g_cert_sp, maxof(g_cert_sp, gridN)), 3); te(k); }
     HIPCHK(hipGetLastError());
     if (use_side) HIPCHK(hipStreamWaitEvent(st, ev_side_, 0));
     HIPCHK(hipMemcpyAsync(h_ctr_, d_ctrs_, sizeof(Ctr) * nslots_, hipMemcpyDeviceToHost, st));

@@ -42,7 +42,10 @@ struct Params {
     double rm_tau = 1e-5;
     int tile_rounds = 1;       // local cycles per region pass (LDS-staged)
     double tile_frac = 1.0 / 64.0;  // region mode when active >= tile_frac * n
-    int cert = 1;              // certify every answer on the GPU: 1 pair sums by fingerprint, 2 entry by entry, 0 off
+    // certify every answer on the GPU: 3 fingerprint over the pairs pushes
+    // changed (1 where the slab's marks do not cover its residual), 1
+    // fingerprint over every pair, 2 pair sums entry by entry, 0 off
+    int cert = 3;
     long long iter_max = 20000000;  // BC:1476, but reported as an error
     // levels one single-workgroup BFS launch may run: bounds a step's length
     // on deep graphs so batched slots keep moving while one relabels
@@ -169,6 +172,10 @@ struct Slab {
     int gcap = 0;
     uint8_t* emark = nullptr;   // E marks of changed entry pairs
     uint64_t res_uid = 0;       // graph whose residual res holds (0 = none)
+    // every entry pair whose residual differs from its capacities has a
+    // marked entry (res reset from the capacities, since changed by marking
+    // pushes and updates only): the sparse certificate may run
+    bool marks_ok = false;
     uint64_t res_version = 0;   // ... at this capacity version
     size_t off = 0, bytes = 0;  // arena block
     bool held = false;

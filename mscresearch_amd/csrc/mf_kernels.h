@@ -91,7 +91,9 @@ void launch_hot_select(hipStream_t st, const SlotView* slots, const int* list, i
 void launch_rm(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
                int margin);
 // Min-cut certificate over the final residual and heights.  mode 1: pair
-// sums by fingerprint (coalesced); mode 2: pair sums entry by entry.
+// sums by fingerprint (coalesced); mode 2: pair sums entry by entry; mode 3:
+// the fingerprint over the marked pairs only (the slab's marks must cover
+// every pair whose residual left its capacities).
 void launch_cert(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx, int mode);
 
 // ---------------------------------------------------------------- pruning

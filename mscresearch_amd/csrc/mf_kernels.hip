@@ -2326,7 +2326,7 @@ void launch_rm(hipStream_t st, const SlotView* slots, const int* list, int nlist
 // from outside T into T.
 
 // Every entry pair has exactly one entry with capacity (forward) and one
-// without (reverse).  Two modes:
+// without (reverse).  Three modes:
 //  FULL (k_cert, k_cert_chunks, k_cert_fin): pair sums from the forward
 //        side, res[e] + res[rev e] == cap[e] (a gather), and each stored
 //        excess against its row's recomputed sum, exactly;
@@ -2334,9 +2334,19 @@ void launch_rm(hipStream_t st, const SlotView* slots, const int* list, int nlist
 //        fingerprint of odd weights (w per pair, u per vertex); the cap
 //        terms are graph constants (g.cert_fp, g.capsum), so an entry costs
 //        res + rev, both coalesced.
+//  SPARSE (k_cert_sp_pairs, k_cert_sp_rows): the flow certified is the one
+//        the marked pairs hold; every other pair carries none (res = cap).
+//        The same fingerprint over the marked pairs' entries only, with
+//        res - cap in place of res, so an unmarked pair adds nothing; the
+//        mark map is read 8 entries per load.  A valid flow of value
+//        excess(t) beside a closed cut of that capacity proves the value
+//        maximal whatever unmarked residuals hold.  Used where every pair
+//        whose residual left its capacities has a marked entry (residual
+//        reset from the capacities, then changed by pushes and capacity
+//        updates only, both of which mark).
 // Rows of T also check closedness and sum the cut from T's side: entry
 // e = w -> u with u outside T closes pair u -> w.
-enum CertMode : int { CERT_FP = 1, CERT_FULL = 2 };
+enum CertMode : int { CERT_FP = 1, CERT_FULL = 2, CERT_SPARSE = 3 };
 
 __device__ __forceinline__ uint64_t pair_weight(uint64_t pair, uint64_t seed) {
     uint64_t x = pair + seed + 0x9e3779b97f4a7c15ULL;
@@ -2541,6 +2551,117 @@ __global__ __launch_bounds__(256) void k_cert_fp(const SlotView* __restrict__ sl
     cert_flush(S, cut, bad, edges, fp, red64, redu);
 }
 
+// Sparse mode, pair side: each marked pair once (from its lower marked
+// entry), both entries' terms (w(pair) + u(row)) * (res - cap); the row of
+// an entry is the column of its pair entry.  The mark map's E bytes count
+// as E / 16 vertex-scan units (16 bytes each) in the work counters.
+__device__ __forceinline__ void cert_sp_pair(const GSlot& S, int e, uint64_t seed, uint64_t& bad,
+                                             uint64_t& fp, uint64_t& pairs) {
+    const int r = S.g.rev[e];
+    if (r < e && S.emark[r]) return;  // taken from r
+    const int64_t xe = S.res[e], xr = S.res[r];
+    const int64_t ce = S.g.cap[e], cr = S.g.cap[r];
+    if (xe < 0) ++bad;
+    if (xr < 0) ++bad;
+    const uint64_t wp = pair_weight((uint64_t)min(e, r), seed);
+    fp += (wp + vertex_weight(S.g.col[r], seed)) * (uint64_t)(xe - ce) +
+          (wp + vertex_weight(S.g.col[e], seed)) * (uint64_t)(xr - cr);
+    ++pairs;
+}
+
+__global__ __launch_bounds__(256) void k_cert_sp_pairs(const SlotView* __restrict__ slots,
+                                                       const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ int64_t red64[256 / WAVE];
+    __shared__ uint64_t redu[256 / WAVE];
+    const uint64_t seed = S.g.cert_seed;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t E = S.g.E;
+    const int64_t nw = E / 8;
+    const GAS uint64_t* m8 = (const GAS uint64_t*)S.emark;  // slab arrays are 256-byte aligned
+    uint64_t bad = 0, fp = 0, pairs = 0;
+    for (int64_t w = i0; w < nw; w += stride) {
+        uint64_t m = m8[w];
+        while (m) {
+            const int b = __builtin_ctzll(m) >> 3;
+            m &= ~(0xffULL << (8 * b));
+            cert_sp_pair(S, (int)(w * 8 + b), seed, bad, fp, pairs);
+        }
+    }
+    for (int64_t e = nw * 8 + i0; e < E; e += stride)  // tail
+        if (S.emark[e]) cert_sp_pair(S, (int)e, seed, bad, fp, pairs);
+    cert_flush(S, 0, bad, 2 * pairs, fp, red64, redu);
+    const uint64_t units = block_sum<uint64_t>(i0 == 0 ? (uint64_t)(E / 16) : 0, redu);
+    if (threadIdx.x == 0 && units) addu64(&S.ctr->wk[WF_CERT][1], units);
+}
+
+// Sparse mode, vertex side: -u(v) * stored excess(v) and the vertex checks
+// for every vertex; then, in workgroups holding a vertex of T, the entries
+// of T's rows (found as in k_cert_fp) for the cut and closedness, the
+// residual of an unmarked pair being its capacity.
+__global__ __launch_bounds__(256) void k_cert_sp_rows(const SlotView* __restrict__ slots,
+                                                      const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ int64_t red64[256 / WAVE];
+    __shared__ uint64_t redu[256 / WAVE];
+    __shared__ int rp[257];
+    __shared__ int tin[256];
+    const int n = S.g.n;
+    const uint64_t seed = S.g.cert_seed;
+    int64_t cut = 0;
+    uint64_t bad = 0, edges = 0, fp = 0, verts = 0;
+    const int nblk = (n + 255) / 256;
+    const int i = (int)threadIdx.x;
+    for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
+        const int v0 = b * 256;
+        const int nv = min(256, n - v0);
+        __syncthreads();  // the previous block's LDS reads are done
+        bool inT = false;
+        if (i < nv) {
+            const int v = v0 + i;
+            inT = S.h[v] < n;
+            const int64_t x = S.excess[v];
+            fp -= vertex_weight(v, seed) * (uint64_t)x;
+            if (v != S.s && x < 0) ++bad;
+            if (inT && v != S.t && x != 0) ++bad;
+            if (v == S.t) {
+                S.ctr->ex_t = x;
+                if (!inT) ++bad;
+            }
+            if (v == S.s && inT) ++bad;
+            ++verts;
+            tin[i] = inT ? 1 : 0;
+            rp[i] = S.g.row_ptr[v];
+            if (i == nv - 1) rp[nv] = S.g.row_ptr[v + 1];
+        }
+        if (!__syncthreads_or(inT)) continue;  // uniform per block
+        const int E1 = rp[nv];
+        int lo = 0;  // rows only grow along e
+        for (int e = rp[0] + i; e < E1; e += 256) {
+            int rr = lo;
+#pragma unroll
+            for (int step = 128; step > 0; step >>= 1) {
+                const int c = min(rr + step, nv - 1);
+                rr = rp[c] <= e ? c : rr;
+            }
+            lo = rr;
+            if (!tin[rr]) continue;
+            ++edges;
+            if (S.h[S.g.col[e]] >= n) {  // pair col[e] -> row enters T
+                const int rv = S.g.rev[e];
+                const int64_t c = S.g.cap[rv];
+                cut += c;
+                const int64_t x = (S.emark[e] | S.emark[rv]) ? S.res[rv] : c;
+                if (x > 0) ++bad;
+            }
+        }
+    }
+    cert_flush(S, cut, bad, edges, fp, red64, redu);
+    const uint64_t bv = block_sum<uint64_t>(verts, redu);
+    if (threadIdx.x == 0 && bv) addu64(&S.ctr->wk[WF_CERT][1], bv);
+}
+
 // Giant rows of the certificate in kGiantChunk-entry tasks over the grid:
 // each chunk adds its part of the row's recomputed excess into gacc[row];
 // k_cert_fin then checks each giant vertex.
@@ -2587,6 +2708,11 @@ __global__ __launch_bounds__(256) void k_cert_fin(const SlotView* __restrict__ s
 }
 
 void launch_cert(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx, int mode) {
+    if (mode == CERT_SPARSE) {
+        hipLaunchKernelGGL(k_cert_sp_pairs, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_cert_sp_rows, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        return;
+    }
     if (mode != CERT_FULL) {
         hipLaunchKernelGGL(k_cert_fp, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
         return;
@@ -2664,6 +2790,7 @@ __global__ __launch_bounds__(256) void k_apply_updates(const SlotView* __restric
             S.res[e] = res + (c2 - c);
         } else {
             const int64_t back = f - c2;  // flow that no longer fits
+            S.emark[e] = 1;
             S.res[e] = 0;
             S.res[r] = S.res[r] - back;
             // u = owner of entry e gets the flow back, w loses it

@@ -63,7 +63,7 @@ def test_params_defaults_are_the_reference_constants():
     p = M.default_params()
     assert (p.gr_freq, p.h_cut, p.stagnate_rounds, p.path_switch, p.path_len) == (128, 100, 128, 8, 256)
     assert (p.hot_k, p.hot_min_active, p.rm_freq0, p.rm_margin, p.iter_max) == (32, 64, 64, 4, 20000000)
-    assert p.prune == 1 and abs(p.prune_ratio - 0.995) < 1e-12 and p.cert == 1
+    assert p.prune == 1 and abs(p.prune_ratio - 0.995) < 1e-12 and p.cert == 3
     assert (p.local_relabel, p.local_band) == (0, 128)  # band limit of BC:1611; off by default
     # additions without a reference constant: fused cycles, graph replay,
     # speculative certificate, sparse counts

@@ -133,6 +133,7 @@ def test_edge_cases():
     dict(local_relabel=1, local_band=1 << 30, gr_stall=0, gr_work=0.0),  # local relabel between all discharges
     dict(local_relabel=1, h_cut=0, rm_freq0=1, rm_freq_max=1),       # ... and RM on every iteration
     dict(cert=2),                               # certificate pair sums entry by entry
+    dict(cert=1),                               # fingerprint over every pair (not only the marked ones)
     dict(spec_cert=0),                          # terminate through an exact global relabel only
     dict(sparse_count=0),                       # count actives over every vertex after each discharge
     dict(sparse_count=1, hot_min_active=1, hot_k=4, tile_frac=0.0),  # sparse counts after tile + hot sweeps
@@ -213,7 +214,7 @@ def test_giant_rows():
     sa, tb = int(d["new_id"][s]), int(d["new_id"][t])
     hb = int(d["new_id"][hub])
     r, res, ex, h = eng.dbg_solve_state(gid, sa, tb, n, len(d["col"]))
-    for mode in (1, 2):
+    for mode in (1, 2, 3):
         assert eng.dbg_certify(gid, sa, tb, res, ex, h, mode) == (0, r.flow, r.flow, True)
     rp = d["row_ptr"]
     assert rp[hb + 1] - rp[hb] > 2048
@@ -226,18 +227,21 @@ def test_giant_rows():
         bad_res, bad_ex = res.copy(), ex.copy()
         bad_res[e_bad] += 1
         bad_ex[row] += 1
-        b1 = eng.dbg_certify(gid, sa, tb, bad_res, bad_ex, h, 1)
-        assert b1[0] > 0 or not b1[3], row
+        for mode in (1, 3):
+            b1 = eng.dbg_certify(gid, sa, tb, bad_res, bad_ex, h, mode)
+            assert b1[0] > 0 or not b1[3], (row, mode)
         # stored excess off alone
         bad_ex = ex.copy()
         bad_ex[row] -= 1
-        b1 = eng.dbg_certify(gid, sa, tb, res, bad_ex, h, 1)
-        assert b1[0] > 0 or not b1[3], row
+        for mode in (1, 3):
+            b1 = eng.dbg_certify(gid, sa, tb, res, bad_ex, h, mode)
+            assert b1[0] > 0 or not b1[3], (row, mode)
 
 
 def test_gpu_certificate_catches_planted_mistakes(c1):
-    """The GPU certificate (fingerprint and entry-by-entry pair sums) accepts
-    a correct final state and rejects planted mistakes, like the oracle's."""
+    """The GPU certificate (fingerprint over every pair or over the changed
+    pairs, and entry-by-entry pair sums) accepts a correct final state and
+    rejects planted mistakes, like the oracle's."""
     eng, gid = engine_with(c1)
     qs, qt = M.uniform_queries(c1.n, 4, 1007)
     og = c1.og
@@ -248,7 +252,7 @@ def test_gpu_certificate_catches_planted_mistakes(c1):
         s, t = c1.q(a, b)
         r, res, ex, h = eng.dbg_solve_state(gid, s, t, c1.n, c1.E)
         assert O.certify(og, res, s, t, r.flow, excess=ex) == 0
-        for mode in (1, 2):
+        for mode in (1, 2, 3):
             bad, cut, ext, fz = eng.dbg_certify(gid, s, t, res, ex, h, mode)
             assert (bad, fz, cut, ext) == (0, True, r.flow, r.flow), mode
         # (a) two pair sums off by +1 / -1 inside one row: the recomputed
@@ -263,6 +267,7 @@ def test_gpu_certificate_catches_planted_mistakes(c1):
         assert O.certify(og, bad_res, s, t, r.flow, excess=ex) != 0
         assert not eng.dbg_certify(gid, s, t, bad_res, ex, h, 1)[3]
         assert eng.dbg_certify(gid, s, t, bad_res, ex, h, 2)[0] > 0
+        assert not eng.dbg_certify(gid, s, t, bad_res, ex, h, 3)[3]
         # (b) stored excess off by one at an inner vertex: the full mode
         # counts it; the fingerprint mode's vertex terms leave a nonzero sum
         # (or count it, for a vertex of T or of a giant row)
@@ -271,8 +276,9 @@ def test_gpu_certificate_catches_planted_mistakes(c1):
         bad_ex[w] += 1
         assert O.certify(og, res, s, t, r.flow, excess=bad_ex) != 0
         assert eng.dbg_certify(gid, s, t, res, bad_ex, h, 2)[0] > 0
-        b1 = eng.dbg_certify(gid, s, t, res, bad_ex, h, 1)
-        assert b1[0] > 0 or not b1[3]
+        for mode in (1, 3):
+            b1 = eng.dbg_certify(gid, s, t, res, bad_ex, h, mode)
+            assert b1[0] > 0 or not b1[3], mode
         # (b') a residual off by one with the stored excess of its row
         # following it: pair sums and row sums disagree, only the pair side
         # of the fingerprint sees it
@@ -281,19 +287,35 @@ def test_gpu_certificate_catches_planted_mistakes(c1):
         bad_res[e3] += 1
         bad_ex = ex.copy()
         bad_ex[owner[e3]] += 1
-        b1 = eng.dbg_certify(gid, s, t, bad_res, bad_ex, h, 1)
-        assert b1[0] > 0 or not b1[3]
+        for mode in (1, 3):
+            b1 = eng.dbg_certify(gid, s, t, bad_res, bad_ex, h, mode)
+            assert b1[0] > 0 or not b1[3], mode
         assert eng.dbg_certify(gid, s, t, bad_res, bad_ex, h, 2)[0] > 0
         # (c) a negative residual
         bad_res = res.copy()
         bad_res[int(rng.choice(np.nonzero(res > 0)[0]))] = -1
-        for mode in (1, 2):
+        for mode in (1, 2, 3):
             assert eng.dbg_certify(gid, s, t, bad_res, ex, h, mode)[0] > 0
         # (d) heights claiming s reaches t (s inside T)
         bad_h = h.copy()
         bad_h[s] = 0
-        for mode in (1, 2):
+        for mode in (1, 2, 3):
             assert eng.dbg_certify(gid, s, t, res, ex, bad_h, mode)[0] > 0
+        # (e) one unit pushed back across the cut (pair sum and stored
+        # excesses follow it): a residual entry now enters T
+        T = h < n
+        back = np.nonzero(T[owner] & ~T[og.col] & (res > 0) & (res[og.rev] == 0))[0]
+        if len(back):
+            e = int(back[0])
+            rv = int(og.rev[e])
+            bad_res, bad_ex = res.copy(), ex.copy()
+            bad_res[e] -= 1
+            bad_res[rv] += 1
+            bad_ex[owner[e]] -= 1
+            bad_ex[og.col[e]] += 1
+            for mode in (1, 2, 3):
+                got = eng.dbg_certify(gid, s, t, bad_res, bad_ex, h, mode)
+                assert got != (0, r.flow, r.flow, True), mode
 
 
 def test_fused_cycles_replay_captured_graphs():
