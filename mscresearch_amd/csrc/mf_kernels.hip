@@ -704,14 +704,16 @@ __global__ __launch_bounds__(256) void k_bfs_td_heavy(const SlotView* __restrict
 // entries past it); longer ones by a wave.
 constexpr int kBuLightDeg = 128;
 
+// One thread per vertex, no workgroup barrier inside the loop (a wave that
+// meets a long scan holds back only itself): light rows are scanned by
+// their lane; medium rows go to hl (k_bfs_bu_med, a wave each), giant rows
+// to the giant list (k_bfs_bu_heavy, a workgroup each).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slots,
                                                 const int* __restrict__ list) {
     const GSlot S = load_slot(slots, list);
     __shared__ Stage<2048> st;
     __shared__ uint64_t red[256 / WAVE];
-    __shared__ int med[256];
-    __shared__ int nmed;
     const int L = S.ctr->level;
     GAS int32_t* lvl = bfs_lvl<MODE>(S);
     GAS int32_t* out = (L & 1) ? S.fa : S.fb;
@@ -725,22 +727,15 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
     __syncthreads();
     const int n = S.g.n;
     Work wk;
-    const int lane = lane_id();
-    const int wpb = (int)(blockDim.x / WAVE);
-    // chunks of blockDim vertices: rows up to kLightDeg by their lane, longer
-    // rows by a wave of the block (ballot, stop at the first parent), giant
-    // rows by k_bfs_bu_heavy (a whole workgroup)
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
-        if (threadIdx.x == 0) nmed = 0;
-        __syncthreads();
-        const int v = (int)(base + threadIdx.x);
-        if (v < n && lvl[v] == n && bfs_vertex_ok<MODE>(S, v)) {
+    for (int64_t vi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; vi < n; vi += (int64_t)gridDim.x * blockDim.x) {
+        const int v = (int)vi;
+        if (lvl[v] == n && bfs_vertex_ok<MODE>(S, v)) {
             wk.verts++;
             const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
             if (re - rb > kGiantDeg) {
-                S.hl[add32(&S.ctr->nhl, 1)] = v;
+                S.giant[add32(&S.ctr->ngiant, 1)] = v;
             } else if (re - rb > kBuLightDeg) {
-                med[atomicAdd(&nmed, 1)] = v;
+                S.hl[add32(&S.ctr->nhl, 1)] = v;
             } else {
                 // the level gather only for entries that pass the residual
                 // test (most reverse entries carry no flow)
@@ -754,34 +749,58 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
                 }
             }
         }
-        __syncthreads();
-        const int nm = nmed;
-        for (int k = (int)(threadIdx.x / WAVE); k < nm; k += wpb) {
-            const int u = med[k];
-            const int rb = S.g.row_ptr[u], re = S.g.row_ptr[u + 1];
-            bool hit = false;
-            int scanned = 0;
-            for (int e0 = rb; e0 < re; e0 += WAVE) {
-                const int e = e0 + lane;
-                bool ok = false;
-                if (e < re) {
-                    const bool r = bfs_bu_edge_ok<MODE>(S, e);
-                    const int c = S.g.col[e];  // issued with the residual load
-                    ok = r && lvl[c] == L;
-                }
-                scanned += min(WAVE, re - e0);
-                if (__ballot(ok)) { hit = true; break; }
+    }
+    stage_flush(st, out, ocnt, odeg);
+    const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
+    if (threadIdx.x == 0 && nv) add32(&S.ctr->visited, (int32_t)nv);
+    work_flush(S, wk, red, WF_BFS_BU);
+}
+
+// Medium unvisited rows of a bottom-up level (kBuLightDeg < length <=
+// kGiantDeg), listed in hl by k_bfs_bu: a wave each, grid-stride, 64
+// entries per step (ballot), stopping at the first step holding a parent.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bfs_bu_med(const SlotView* __restrict__ slots,
+                                                    const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ Stage<256> st;
+    __shared__ uint64_t red[256 / WAVE];
+    const int nm = S.ctr->nhl;
+    const int wpb = (int)(blockDim.x / WAVE);
+    if ((int)blockIdx.x * wpb >= nm) return;  // uniform per block
+    const int L = S.ctr->level;
+    GAS int32_t* lvl = bfs_lvl<MODE>(S);
+    GAS int32_t* out = (L & 1) ? S.fa : S.fb;
+    GAS int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
+    GAS int64_t* odeg = &S.ctr->fedges[(L + 1) & 3];
+    stage_init(st);
+    __syncthreads();
+    Work wk;
+    const int lane = lane_id();
+    for (int k = (int)blockIdx.x * wpb + (int)(threadIdx.x / WAVE); k < nm; k += (int)gridDim.x * wpb) {
+        const int u = S.hl[k];
+        const int rb = S.g.row_ptr[u], re = S.g.row_ptr[u + 1];
+        bool hit = false;
+        int scanned = 0;
+        for (int e0 = rb; e0 < re; e0 += WAVE) {
+            const int e = e0 + lane;
+            bool ok = false;
+            if (e < re) {
+                const bool r = bfs_bu_edge_ok<MODE>(S, e);
+                const int c = S.g.col[e];  // issued with the residual load
+                ok = r && lvl[c] == L;
             }
-            if (lane == 0) {
-                wk.edges += (uint64_t)scanned;
-                if (hit) {
-                    lvl[u] = L + 1;  // only this wave writes u
-                    stage_push(st, u, re - rb, out, ocnt, odeg);
-                    wk.relabels++;
-                }
+            scanned += min(WAVE, re - e0);
+            if (__ballot(ok)) { hit = true; break; }
+        }
+        if (lane == 0) {
+            wk.edges += (uint64_t)scanned;
+            if (hit) {
+                lvl[u] = L + 1;  // only this wave writes u
+                stage_push(st, u, re - rb, out, ocnt, odeg);
+                wk.relabels++;
             }
         }
-        __syncthreads();  // med / nmed are reused by the next chunk
     }
     stage_flush(st, out, ocnt, odeg);
     const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
@@ -798,7 +817,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu_heavy(const SlotView* __restrict
     __shared__ Stage<256> st;
     __shared__ uint64_t red[256 / WAVE];
     __shared__ int found;
-    const int nh = S.ctr->nhl;
+    const int nh = S.ctr->ngiant;
     if ((int)blockIdx.x >= nh) return;  // uniform per block
     const int L = S.ctr->level;
     GAS int32_t* lvl = bfs_lvl<MODE>(S);
@@ -808,7 +827,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu_heavy(const SlotView* __restrict
     stage_init(st);
     Work wk;
     for (int k = blockIdx.x; k < nh; k += gridDim.x) {
-        const int v = S.hl[k];
+        const int v = S.giant[k];
         const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
         if (threadIdx.x == 0) found = 0;
         __syncthreads();
@@ -865,12 +884,15 @@ void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int n
     const int gh = per_slot_grid(gridx < 2048 ? gridx : 2048, nlist);
     if (mode == BFS_GR) {
         hipLaunchKernelGGL(k_bfs_bu<BFS_GR>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_bu_med<BFS_GR>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
         hipLaunchKernelGGL(k_bfs_bu_heavy<BFS_GR>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
     } else if (mode == BFS_FWD) {
         hipLaunchKernelGGL(k_bfs_bu<BFS_FWD>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_bu_med<BFS_FWD>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
         hipLaunchKernelGGL(k_bfs_bu_heavy<BFS_FWD>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
     } else {
         hipLaunchKernelGGL(k_bfs_bu<BFS_UND>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL(k_bfs_bu_med<BFS_UND>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
         hipLaunchKernelGGL(k_bfs_bu_heavy<BFS_UND>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
     }
     hipLaunchKernelGGL(k_bfs_advance, dim3(1, nlist), dim3(64), 0, st, slots, list);
