@@ -263,7 +263,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16, help="timed steps of --batch queries each")
     ap.add_argument("--warmup", type=int, default=1, help="untimed steps (distinct queries)")
-    ap.add_argument("--batch", type=int, default=96, help="engine slots = queries per step")
+    ap.add_argument("--batch", type=int, default=160, help="engine slots = queries per step")
     ap.add_argument("--workload", choices=["rmat", "grid", "uniform", "dynamic"], default="rmat")
     ap.add_argument("--rounds", type=int, default=20, help="dynamic: update rounds")
     ap.add_argument("--oracle-rounds", type=int, default=2, help="dynamic: rounds also checked on the CPU oracle")
