@@ -30,6 +30,15 @@ template <typename P>
 __device__ __forceinline__ int64_t ld64(P p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Residual read in a step that pushes nothing on the slot (BFS levels,
+// local relabel, certificate): the last writer was an earlier kernel, so a
+// plain load sees its value and may hit in L2.  The agent-scope ld64 is
+// for discharge kernels, where other workgroups of the same launch add to
+// residuals.
+template <typename P>
+__device__ __forceinline__ int64_t ld_res(P p) {
+    return *p;
+}
 template <typename P>
 __device__ __forceinline__ int64_t add64(P p, int64_t v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -431,7 +440,7 @@ __device__ __forceinline__ GAS int32_t* bfs_lvl(const GSlot& S) {
 // have cap > 0 and reverse entries cap 0).  UND: any entry.
 template <int MODE>
 __device__ __forceinline__ bool bfs_bu_edge_ok(const GSlot& S, int e) {
-    if (MODE == BFS_GR) return ld64(&S.res[e]) > 0;
+    if (MODE == BFS_GR) return ld_res(&S.res[e]) > 0;
     if (MODE == BFS_FWD) return S.g.cap[e] == 0;
     return true;
 }
@@ -462,7 +471,7 @@ __device__ __forceinline__ bool edge_ok_aux(const GSlot& S, int aux, int w) {
     if (MODE == BFS_GR) {
         if (w == S.s) return false;
         if (S.mask && S.mask[w] == 0) return false;
-        return ld64(&S.res[aux]) > 0;
+        return ld_res(&S.res[aux]) > 0;
     }
     return aux != 0;
 }
@@ -1195,7 +1204,7 @@ __global__ __launch_bounds__(256) void k_local_bound(const SlotView* __restrict_
                 int b = INT_MAX;
                 wk.edges += (uint64_t)(re - rb);
                 for (int e = rb; e < re; ++e) {
-                    if (ld64(&S.res[e]) <= 0) continue;
+                    if (ld_res(&S.res[e]) <= 0) continue;
                     const int hx = S.h[S.g.col[e]];
                     if (hx != mark) b = min(b, hx + 1);
                 }
@@ -1212,7 +1221,7 @@ __global__ __launch_bounds__(256) void k_local_bound(const SlotView* __restrict_
             if (lane == 0) wk.edges += (uint64_t)(re - rb);
             int b = INT_MAX;
             for (int e = rb + lane; e < re; e += WAVE) {
-                if (ld64(&S.res[e]) <= 0) continue;
+                if (ld_res(&S.res[e]) <= 0) continue;
                 const int hx = S.h[S.g.col[e]];
                 if (hx != mark) b = min(b, hx + 1);
             }
@@ -1321,7 +1330,7 @@ __global__ __launch_bounds__(1024) void k_local_bfs(const SlotView* __restrict__
             for (int e = rb; e < re; ++e) {
                 const int w = S.g.col[e];
                 if (S.h[w] != mark) continue;
-                if (ld64(&S.res[S.g.rev[e]]) <= 0) continue;
+                if (ld_res(&S.res[S.g.rev[e]]) <= 0) continue;
                 if (cas32(&S.h[w], mark, L + 1)) {
                     out[atomicAdd(&next_cnt, 1)] = w;
                     add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
@@ -1338,7 +1347,7 @@ __global__ __launch_bounds__(1024) void k_local_bfs(const SlotView* __restrict__
             for (int e = rb + (int)threadIdx.x; e < re; e += blockDim.x) {
                 const int w = S.g.col[e];
                 if (S.h[w] != mark) continue;
-                if (ld64(&S.res[S.g.rev[e]]) <= 0) continue;
+                if (ld_res(&S.res[S.g.rev[e]]) <= 0) continue;
                 if (cas32(&S.h[w], mark, L + 1)) {
                     out[atomicAdd(&next_cnt, 1)] = w;
                     add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
@@ -2518,7 +2527,7 @@ __global__ __launch_bounds__(256) void k_cert_fp(const SlotView* __restrict__ sl
             const uint64_t u = vertex_weight(v, seed);
             uw[i] = u;
             const bool inT = hv < n;
-            const int64_t x = ld64(&S.excess[v]);
+            const int64_t x = S.excess[v];
             fp -= u * (uint64_t)(S.g.capsum[v] + x);
             if (v != S.s && x < 0) ++bad;
             if (inT && v != S.t && x != 0) ++bad;
