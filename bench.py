@@ -2,10 +2,11 @@
 
 Workload (default): R-MAT scale 22 (a,b,c,d = .57,.19,.19,.05; 16*2^22
 samples, duplicates merged by summing log-uniform [1,1e6] capacities, graph
-seed 3001), 1,024 queries with s from the top 1% of vertices by out-degree
-and t uniform over vertices with in-degree > 0 (query seed 3002).  The timed
-region submits --steps x --batch queries at once to an engine with --batch
-concurrent slots, which refills a slot as soon as its query finishes
+seed 3001), --steps x --batch queries with s from the top 1% of vertices by
+out-degree and t uniform over vertices with in-degree > 0 (query seed 3002).
+The timed region submits them all at once to --engines engines sharing the
+--batch slots (each engine on its own host thread and HIP stream, through
+mf_serve_lanes), which refill a slot as soon as its query finishes
 (continuous batching); ms_per_step is the wall time / --steps.
 Every GPU flow carries the GPU's min-cut certificate; a sample is also
 compared bit for bit with the CPU oracle (Dinic), whose time on the same
@@ -78,19 +79,23 @@ def build_workload(args):
     return e, hg, d, qs2, qt2, name, desc
 
 
-def run_continuous(eng, gid, qs, qt, lo, hi):
-    """Queries lo..hi-1 submitted at once; the engine keeps its slots full
-    (continuous batching) until all are done."""
-    for i in range(lo, hi):
-        eng.submit(gid, qs[i], qt[i], i)
-    res_all = eng.run()
+def run_engines(engs, qs, qt, lo, hi):
+    """Queries lo..hi-1 over several engines, query i on engine (i - lo) % k,
+    all arriving at once, through mf_serve_lanes (one host thread and HIP
+    stream per engine, each keeping its slots full)."""
+    k = len(engs)
+    idx = np.arange(lo, hi)
+    lane = (idx - lo) % k
+    gids = np.array([engs[j][1] for j in lane], np.int32)
+    res_all = M.serve_lanes([e for e, _ in engs], lane, gids, np.asarray(qs)[idx], np.asarray(qt)[idx],
+                            np.zeros(len(idx)))
     lat, svc, flows = [], [], {}
     for r in res_all:
         if r.status != 0 or r.cert_ok != 1:
-            raise RuntimeError(f"query {r.tag}: status {r.status} cert {r.cert_ok}: {M.lib().mf_last_error().decode()}")
+            raise RuntimeError(f"query {lo + r.tag}: status {r.status} cert {r.cert_ok}: {M.lib().mf_last_error().decode()}")
         lat.append(r.t_end - r.t_arrive)
         svc.append(r.t_end - r.t_start)
-        flows[r.tag] = r.flow
+        flows[lo + int(r.tag)] = r.flow
     return res_all, lat, svc, flows
 
 
@@ -264,6 +269,8 @@ def main():
     ap.add_argument("--steps", type=int, default=16, help="timed steps of --batch queries each")
     ap.add_argument("--warmup", type=int, default=1, help="untimed steps (distinct queries)")
     ap.add_argument("--batch", type=int, default=160, help="engine slots = queries per step")
+    ap.add_argument("--engines", type=int, default=2,
+                    help="engines sharing the batch slots, one host thread and HIP stream each")
     ap.add_argument("--workload", choices=["rmat", "grid", "uniform", "dynamic"], default="rmat")
     ap.add_argument("--rounds", type=int, default=20, help="dynamic: update rounds")
     ap.add_argument("--oracle-rounds", type=int, default=2, help="dynamic: rounds also checked on the CPU oracle")
@@ -323,21 +330,32 @@ def main():
     timed_q = slice(args.batch * args.warmup, args.batch * args.warmup + args.queries)
     tq_s, tq_t = qs[timed_q], qt[timed_q]
 
-    # ---- batched (the measured value)
-    eng = M.Engine(args.batch)
-    gid = eng.add_graph(dg)
+    # ---- batched (the measured value): --engines engines share the --batch
+    # slots, each driven by its own host thread on its own HIP stream
+    if args.batch % args.engines:
+        raise SystemExit("--batch must be a multiple of --engines")
+    engs = []
+    for _ in range(args.engines):
+        eng = M.Engine(args.batch // args.engines)
+        engs.append((eng, eng.add_graph(dg)))
     if args.warmup:
-        run_continuous(eng, gid, qs, qt, 0, args.batch * args.warmup)
-    eng.profile(True)
+        run_engines(engs, qs, qt, 0, args.batch * args.warmup)
+    for eng, _ in engs:
+        eng.profile(True)
     t0 = time.perf_counter()
-    res, lat, svc, flows = run_continuous(eng, gid, qs, qt, args.batch * args.warmup,
-                                          args.batch * args.warmup + args.queries)
+    res, lat, svc, flows = run_engines(engs, qs, qt, args.batch * args.warmup,
+                                       args.batch * args.warmup + args.queries)
     wall = time.perf_counter() - t0
-    ms, launches, work = eng.profile_read()
-    eng.profile(False)
+    ms, launches, work = 0.0, 0, 0
+    hits = misses = 0
+    for eng, _ in engs:
+        m_, l_, w_ = eng.profile_read()
+        eng.profile(False)
+        ms, launches, work = ms + m_, launches + l_, work + w_
+        h_, mi_ = eng.cache_stats()
+        hits, misses = hits + h_, misses + mi_
     qps = args.queries / wall
     e_run = sum(r.E_run for r in res)
-    hits, misses = eng.cache_stats()
     log(f"batched: {args.queries} queries in {wall:.3f}s = {qps:.1f} q/s; cache hits {hits} misses {misses}")
     st = {k: float(np.mean([getattr(r, k) for r in res])) for k in
           ("iters", "steps", "grs", "bfs_levels", "drains", "pruned", "n_run", "E_run", "kept")}
@@ -382,7 +400,7 @@ def main():
     else:
         plain_qps = None
 
-    del eng  # its slab arena is not needed any more
+    del engs, eng  # their slab arenas are not needed any more
     serving = None
     if not args.no_serve and args.workload == "rmat" and args.scale == 22:
         serving = serve_bench(dg, hg, e, args)
@@ -419,7 +437,9 @@ def main():
         "dtype": "int64",
         "data": "synthetic",
         "config": {"workload": name, "graph": desc, "n": n, "csr_entries": E, "queries": args.queries,
-                   "batch": args.batch, "regions": args.R, "parallelism": "one GPU, batched slots"},
+                   "batch": args.batch, "engines": args.engines, "regions": args.R,
+                   "parallelism": f"one GPU, {args.engines} engine(s) x {args.batch // args.engines} batched slots, "
+                                  "one host thread and HIP stream each"},
         "plain_qps": round(plain_qps, 3) if plain_qps else None,
         "speedup_vs_plain": round(qps / plain_qps, 3) if plain_qps else None,
         "p50_ms": round(pct(lat, .5) * 1e3, 3),
