@@ -268,7 +268,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16, help="timed steps of --batch queries each")
     ap.add_argument("--warmup", type=int, default=1, help="untimed steps (distinct queries)")
-    ap.add_argument("--batch", type=int, default=160, help="engine slots = queries per step")
+    ap.add_argument("--batch", type=int, default=192, help="engine slots = queries per step")
     ap.add_argument("--engines", type=int, default=2,
                     help="engines sharing the batch slots, one host thread and HIP stream each")
     ap.add_argument("--workload", choices=["rmat", "grid", "uniform", "dynamic"], default="rmat")
