@@ -134,8 +134,7 @@ def serve_bench(dg22, hg22, e22, args):
     for name, dg, hg, slots, prm in lanes:
         eng = M.Engine(slots, prm)
         eng.add_graph(dg)
-        info = dg.info()
-        slab = 8 * info["E"] + 32 * info["n"]
+        slab = dg.slab_bytes()
         nslab = args.serve_slabs if name == "rmat22" else slots
         eng.set_budget(int(nslab * slab))
         engines.append(eng)

@@ -131,6 +131,10 @@ int mf_device_count(int* count);
  * the handle, never by (V,E,R) sizes. */
 int mf_graph_upload(const mf_hgraph* h, int tile_max, mf_graph** out);
 int mf_graph_info(const mf_graph* g, int* n, int64_t* E, int* ntiles, int64_t* bytes);
+/* Device bytes of one query slab for this graph: what an engine's slab
+ * arena (mf_engine_set_budget) holds per concurrent query on it.  No
+ * reference counterpart (the reference sizes its one state per run). */
+int mf_graph_slab_bytes(const mf_graph* g, int64_t* bytes);
 void mf_graph_free(mf_graph* g);
 
 /* ---------------------------------------------------------------- engine */
@@ -166,6 +170,7 @@ typedef struct mf_params {
     int sparse_count;    /* 1 = count actives from the previous list and the vertices pushed into */
     int small_wgs;       /* workgroups per slot of the multi-level small-frontier BFS (1 = one) */
     long long small_edges; /* ... while the frontier's rows sum to at most this many entries */
+    int mark_list;       /* marked entries listed per slab for resets and the certificate (0 = map only) */
 } mf_params;
 
 void mf_params_default(mf_params* p);

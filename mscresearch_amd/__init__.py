@@ -45,7 +45,7 @@ class Params(C.Structure):
         ("prune", C.c_int), ("prune_ratio", C.c_double), ("gr_work", C.c_double), ("K_heavy", C.c_int),
         ("local_relabel", C.c_int), ("local_band", C.c_int), ("fuse", C.c_int), ("graphs", C.c_int),
         ("spec_cert", C.c_int), ("sparse_count", C.c_int), ("small_wgs", C.c_int),
-        ("small_edges", C.c_longlong),
+        ("small_edges", C.c_longlong), ("mark_list", C.c_int),
     ]
 
 
@@ -125,6 +125,7 @@ SIGNATURES = {
     "mf_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "mf_graph_upload": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
     "mf_graph_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
+    "mf_graph_slab_bytes": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "mf_graph_free": (None, [_P]),
     "mf_params_default": (None, [C.POINTER(Params)]),
     "mf_engine_create": (C.c_int, [C.c_int, C.POINTER(Params), C.POINTER(_P)]),
@@ -421,6 +422,12 @@ class DeviceGraph:
         n, E, T, b = C.c_int(0), C.c_int64(0), C.c_int(0), C.c_int64(0)
         _check(lib().mf_graph_info(self._h, C.byref(n), C.byref(E), C.byref(T), C.byref(b)))
         return {"n": n.value, "E": E.value, "ntiles": T.value, "bytes": b.value}
+
+    def slab_bytes(self) -> int:
+        """Device bytes of one query slab on this graph (engine arena unit)."""
+        b = C.c_int64(0)
+        _check(lib().mf_graph_slab_bytes(self._h, C.byref(b)))
+        return b.value
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -110,6 +110,7 @@ Params from_c(const mf_params* p) {
     P.sparse_count = p->sparse_count;
     P.small_wgs = p->small_wgs;
     P.small_edges = p->small_edges;
+    P.mark_list = p->mark_list;
     return P;
 }
 
@@ -405,6 +406,12 @@ int mf_graph_info(const mf_graph* g, int* n, int64_t* E, int* ntiles, int64_t* b
     return MF_OK;
 }
 
+int mf_graph_slab_bytes(const mf_graph* g, int64_t* bytes) {
+    if (!g || !bytes) return fail(MF_EINVAL, "null argument");
+    *bytes = (int64_t)Slab::bytes_for(g->dg->g.n, g->dg->g.E, g->dg->g.ntiles);
+    return MF_OK;
+}
+
 void mf_graph_free(mf_graph* g) { delete g; }
 
 // ------------------------------------------------------------------ engine
@@ -451,6 +458,7 @@ void mf_params_default(mf_params* p) {
     p->sparse_count = P.sparse_count;
     p->small_wgs = P.small_wgs;
     p->small_edges = P.small_edges;
+    p->mark_list = P.mark_list;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {
@@ -840,6 +848,8 @@ int mf_dbg_certify(mf_engine* e, int gid, int s, int t, const int64_t* res, cons
         std::vector<uint8_t> mk((size_t)g.E);
         for (int64_t i = 0; i < g.E; ++i) mk[i] = res[i] != cap[i] ? 1 : 0;
         HIPCK(hipMemcpy(v.emark, mk.data(), (size_t)g.E, hipMemcpyHostToDevice));
+        const int32_t over = v.mcap + 1;  // no list: the certificate reads the map
+        HIPCK(hipMemcpy(v.mlist, &over, sizeof(int32_t), hipMemcpyHostToDevice));
         e->eng.slab(0)->marks_ok = true;
     }
     HIPCK(hipMemcpy(v.excess, excess, sizeof(int64_t) * g.n, hipMemcpyHostToDevice));

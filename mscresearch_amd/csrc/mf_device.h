@@ -105,6 +105,10 @@ struct SlotView {
     int64_t* gacc = nullptr;
     int32_t gcap = 0;
     uint8_t* emark = nullptr;    // E: entry's pair changed since the residual was last reset
+    // the marked entries in marking order: mlist[0] = count, entries from
+    // mlist[1]; past mcap entries only the map is kept (count > mcap)
+    int32_t* mlist = nullptr;
+    int32_t mcap = 0;
     const int32_t* mask = nullptr;  // optional vertex mask (nullptr = all)
     Ctr* ctr = nullptr;
     int32_t s = 0, t = 0;
@@ -121,5 +125,6 @@ constexpr int kSmallFrontier = 16384;  // single-workgroup BFS keeps going below
 constexpr int64_t kSmallEdges = 1 << 18; // ... and while the frontier's rows stay below this
 constexpr int kLocalLevels = 256;   // level cap of the local relabel
 constexpr int kGiantChunk = 2048;   // entries per workgroup task on a giant row
+constexpr int kMarkListMax = 1 << 20;  // marked entries listed per slab (beyond: the map alone)
 
 }  // namespace mf

@@ -160,6 +160,8 @@ size_t Slab::layout(int n, int64_t E, int ntiles, char* base, Slab* sl) {
     char* p_gi = take(sizeof(int32_t) * (giant_task_offset((int64_t)gcap) + 4 * (ee / kGiantChunk + gcap + 1)));
     char* p_ga = take(sizeof(int64_t) * gcap);
     char* p_em = take(ee);
+    const size_t mcap = std::min(ee, (size_t)kMarkListMax);
+    char* p_ml = take(sizeof(int32_t) * (mcap + 1));
     if (sl) {
         sl->res = (int64_t*)p_res;
         sl->excess = (int64_t*)p_ex;
@@ -175,6 +177,8 @@ size_t Slab::layout(int n, int64_t E, int ntiles, char* base, Slab* sl) {
         sl->giant = (int32_t*)p_gi;
         sl->gacc = (int64_t*)p_ga;
         sl->emark = (uint8_t*)p_em;
+        sl->mlist = (int32_t*)p_ml;
+        sl->mcap = (int)mcap;
         sl->res_uid = 0;
         sl->marks_ok = false;
         sl->gcap = (int)gcap;
@@ -824,6 +828,8 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
     v.lbkt = sl.lbkt;
     v.giant = sl.giant; v.gacc = sl.gacc; v.gcap = sl.gcap;
     v.emark = sl.emark;
+    v.mlist = sl.mlist;
+    v.mcap = std::max(0, std::min(sl.mcap, P_.mark_list));
     v.ctr = d_ctrs_ + i;
     v.s = q.s; v.t = q.t;
     v.H_cut = P_.h_cut;

@@ -62,6 +62,10 @@ struct Params {
     // 22 hub queries against 2^18; grid frontiers of up to kSmallFrontier
     // degree-4 vertices stay inside)
     long long small_edges = 1 << 16;
+    // entries a slab lists as they are first marked (at most kMarkListMax);
+    // resets and the sparse certificate read the list instead of the E-byte
+    // mark map while it holds every marked entry (0 = always the map)
+    int mark_list = kMarkListMax;
     double bu_alpha = 14.0, bu_beta = 24.0;  // direction-optimizing BFS thresholds
     int prune = 1;             // per-query reachability pruning (BC:2678)
     double prune_ratio = 0.995;  // BC:2647
@@ -171,6 +175,8 @@ struct Slab {
     int64_t* gacc = nullptr;
     int gcap = 0;
     uint8_t* emark = nullptr;   // E marks of changed entry pairs
+    int32_t* mlist = nullptr;   // count + the marked entries (SlotView::mlist)
+    int mcap = 0;
     uint64_t res_uid = 0;       // graph whose residual res holds (0 = none)
     // every entry pair whose residual differs from its capacities has a
     // marked entry (res reset from the capacities, since changed by marking

@@ -146,6 +146,8 @@ struct GSlot {
     GAS int64_t* gacc;
     int32_t gcap;
     GAS uint8_t* emark;
+    GAS int32_t* mlist;
+    int32_t mcap;
     const GAS int32_t* mask;
     GAS Ctr* ctr;
     int32_t s, t, K, Kh, H_cut, phase;
@@ -188,6 +190,8 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.gacc = as_global(v.gacc);
     S.gcap = v.gcap;
     S.emark = as_global(v.emark);
+    S.mlist = as_global(v.mlist);
+    S.mcap = v.mcap;
     S.mask = as_global(v.mask);
     S.ctr = as_global(v.ctr);
     S.s = v.s;
@@ -247,12 +251,27 @@ struct Work {
     uint64_t edges = 0, verts = 0, pushes = 0, relabels = 0;
 };
 
+// Mark entry e (its pair changed since the residual was last reset) and,
+// the first time, append it to the slab's list, so resets and certificates
+// visit the marked entries only.  The mark byte is set through an atomic OR
+// on its 32-bit word, whose old value says who marked it first; past mcap
+// entries only the map is kept (the count goes on past mcap).
+__device__ __forceinline__ void mark_entry(const GSlot& S, int e) {
+    if (S.emark[e]) return;
+    GAS uint32_t* w = (GAS uint32_t*)(S.emark + (e & ~3));  // slab arrays are 256-byte aligned
+    const uint32_t bit = 1u << (8 * (e & 3));
+    const uint32_t old = __hip_atomic_fetch_or(w, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old & bit) return;
+    const int k = add32(&S.mlist[0], 1);
+    if (k < S.mcap) S.mlist[1 + k] = e;
+}
+
 // Move d units of residual across entry e (its pair rev[e] gains them) and
 // mark the entry, so the next query on this slab resets only marked pairs.
 __device__ __forceinline__ void move_res(const GSlot& S, int e, int64_t d) {
     __hip_atomic_fetch_add(&S.res[e], -d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_fetch_add(&S.res[S.g.rev[e]], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    S.emark[e] = 1;
+    mark_entry(S, e);
 }
 
 __device__ __forceinline__ void work_flush(const GSlot& S, Work w, uint64_t* red, int fam) {
@@ -334,6 +353,7 @@ __global__ __launch_bounds__(256) void k_init(const SlotView* __restrict__ slots
         S.lock[v] = -1;
     }
     if (i0 == 0) {
+        S.mlist[0] = 0;  // the marks are all cleared above
         addu64(&S.ctr->wk[WF_INIT][0], (uint64_t)S.g.E);
         addu64(&S.ctr->wk[WF_INIT][1], (uint64_t)S.g.n);
     }
@@ -344,8 +364,10 @@ void launch_init(hipStream_t st, const SlotView* slots, const int* list, int nli
 }
 
 // Reset of a slab that last held a residual of the same graph: only entry
-// pairs marked by a push (or an update) since go back to their capacities;
-// the mark map is read 8 entries per load.
+// pairs marked by a push (or an update) since go back to their capacities.
+// The marked entries come from the slab's list, or, when it overflowed,
+// from the mark map read 8 entries per load.  The list count is cleared by
+// k_marks_clear after every workgroup has read it.
 __global__ __launch_bounds__(256) void k_init_marked(const SlotView* __restrict__ slots,
                                                      const int* __restrict__ list) {
     const GSlot S = load_slot(slots, list);
@@ -353,9 +375,19 @@ __global__ __launch_bounds__(256) void k_init_marked(const SlotView* __restrict_
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t E = S.g.E;
-    const int64_t nw = E / 8;
+    const int cnt = S.mlist[0];
+    const int64_t nw = cnt <= S.mcap ? 0 : E / 8;
     const GAS uint64_t* m8 = (const GAS uint64_t*)S.emark;  // slab arrays are 256-byte aligned
     Work wk;
+    if (cnt <= S.mcap)
+        for (int64_t k = i0; k < cnt; k += stride) {
+            const int e = S.mlist[1 + k];
+            const int r = S.g.rev[e];
+            S.res[e] = S.g.cap[e];
+            S.res[r] = S.g.cap[r];
+            S.emark[e] = 0;
+            wk.edges++;
+        }
     for (int64_t w = i0; w < nw; w += stride) {
         const uint64_t m = m8[w];
         if (m == 0) continue;
@@ -369,14 +401,15 @@ __global__ __launch_bounds__(256) void k_init_marked(const SlotView* __restrict_
                 wk.edges++;
             }
     }
-    for (int64_t e = nw * 8 + i0; e < E; e += stride)  // tail
-        if (S.emark[e]) {
-            const int r = S.g.rev[(int)e];
-            S.res[e] = S.g.cap[e];
-            S.res[r] = S.g.cap[r];
-            S.emark[e] = 0;
-            wk.edges++;
-        }
+    if (cnt > S.mcap)
+        for (int64_t e = nw * 8 + i0; e < E; e += stride)  // tail
+            if (S.emark[e]) {
+                const int r = S.g.rev[(int)e];
+                S.res[e] = S.g.cap[e];
+                S.res[r] = S.g.cap[r];
+                S.emark[e] = 0;
+                wk.edges++;
+            }
     for (int64_t v = i0; v < S.g.n; v += stride) {
         S.excess[v] = 0;
         S.h[v] = 0;
@@ -386,8 +419,15 @@ __global__ __launch_bounds__(256) void k_init_marked(const SlotView* __restrict_
     work_flush(S, wk, red, WF_INIT);
 }
 
+// After k_init_marked: the marks are all cleared, so the list starts empty.
+__global__ void k_marks_clear(const SlotView* __restrict__ slots, const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    if (threadIdx.x == 0) S.mlist[0] = 0;
+}
+
 void launch_init_marked(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx) {
     hipLaunchKernelGGL(k_init_marked, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_marks_clear, dim3(1, nlist), dim3(64), 0, st, slots, list);
 }
 
 // ---------------------------------------------------------------- preflow (a11)
@@ -2584,8 +2624,10 @@ __global__ __launch_bounds__(256) void k_cert_fp(const SlotView* __restrict__ sl
 
 // Sparse mode, pair side: each marked pair once (from its lower marked
 // entry), both entries' terms (w(pair) + u(row)) * (res - cap); the row of
-// an entry is the column of its pair entry.  The mark map's E bytes count
-// as E / 16 vertex-scan units (16 bytes each) in the work counters.
+// an entry is the column of its pair entry.  The marked entries come from
+// the slab's list, or from the mark map when the list overflowed; the
+// bytes read for that (4 per listed entry, or the map's E) count as
+// 16-byte vertex-scan units in the work counters.
 __device__ __forceinline__ void cert_sp_pair(const GSlot& S, int e, uint64_t seed, uint64_t& bad,
                                              uint64_t& fp, uint64_t& pairs) {
     const int r = S.g.rev[e];
@@ -2609,21 +2651,28 @@ __global__ __launch_bounds__(256) void k_cert_sp_pairs(const SlotView* __restric
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t E = S.g.E;
-    const int64_t nw = E / 8;
-    const GAS uint64_t* m8 = (const GAS uint64_t*)S.emark;  // slab arrays are 256-byte aligned
+    const int cnt = S.mlist[0];
+    const bool listed = cnt <= S.mcap;
     uint64_t bad = 0, fp = 0, pairs = 0;
-    for (int64_t w = i0; w < nw; w += stride) {
-        uint64_t m = m8[w];
-        while (m) {
-            const int b = __builtin_ctzll(m) >> 3;
-            m &= ~(0xffULL << (8 * b));
-            cert_sp_pair(S, (int)(w * 8 + b), seed, bad, fp, pairs);
+    if (listed) {
+        for (int64_t k = i0; k < cnt; k += stride) cert_sp_pair(S, S.mlist[1 + k], seed, bad, fp, pairs);
+    } else {
+        const int64_t nw = E / 8;
+        const GAS uint64_t* m8 = (const GAS uint64_t*)S.emark;  // slab arrays are 256-byte aligned
+        for (int64_t w = i0; w < nw; w += stride) {
+            uint64_t m = m8[w];
+            while (m) {
+                const int b = __builtin_ctzll(m) >> 3;
+                m &= ~(0xffULL << (8 * b));
+                cert_sp_pair(S, (int)(w * 8 + b), seed, bad, fp, pairs);
+            }
         }
+        for (int64_t e = nw * 8 + i0; e < E; e += stride)  // tail
+            if (S.emark[e]) cert_sp_pair(S, (int)e, seed, bad, fp, pairs);
     }
-    for (int64_t e = nw * 8 + i0; e < E; e += stride)  // tail
-        if (S.emark[e]) cert_sp_pair(S, (int)e, seed, bad, fp, pairs);
     cert_flush(S, 0, bad, 2 * pairs, fp, red64, redu);
-    const uint64_t units = block_sum<uint64_t>(i0 == 0 ? (uint64_t)(E / 16) : 0, redu);
+    // list or map bytes read, in 16-byte vertex-scan units
+    const uint64_t units = block_sum<uint64_t>(i0 == 0 ? (uint64_t)(listed ? cnt / 4 : E / 16) : 0, redu);
     if (threadIdx.x == 0 && units) addu64(&S.ctr->wk[WF_CERT][1], units);
 }
 
@@ -2817,11 +2866,11 @@ __global__ __launch_bounds__(256) void k_apply_updates(const SlotView* __restric
         const int64_t res = S.res[e];
         const int64_t f = c - res;  // flow u -> w on this pair (cap[rev] stays)
         if (f <= c2) {
-            S.emark[e] = 1;
+            mark_entry(S, e);
             S.res[e] = res + (c2 - c);
         } else {
             const int64_t back = f - c2;  // flow that no longer fits
-            S.emark[e] = 1;
+            mark_entry(S, e);
             S.res[e] = 0;
             S.res[r] = S.res[r] - back;
             // u = owner of entry e gets the flow back, w loses it

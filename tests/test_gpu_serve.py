@@ -40,9 +40,7 @@ def stream(graphs, nq, qps, seed):
 
 
 def slab_bytes(case):
-    d = case.dg.info()
-    n, E, T = d["n"], d["E"], d["ntiles"]
-    return 8 * E + 32 * n + 8 * T + 2 * 1024 + (E // 2049 + 1) * 20 + 14 * 256
+    return case.dg.slab_bytes()
 
 
 @pytest.mark.parametrize("budget_slabs", [0, 2.5])

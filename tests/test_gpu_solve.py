@@ -139,6 +139,8 @@ def test_edge_cases():
     dict(sparse_count=1, hot_min_active=1, hot_k=4, tile_frac=0.0),  # sparse counts after tile + hot sweeps
     dict(small_edges=0),                        # every BFS level grid-wide (top-down or bottom-up)
     dict(small_edges=1 << 30),                  # single-workgroup levels up to kSmallFrontier vertices
+    dict(mark_list=0),                          # resets and certificates from the mark map
+    dict(mark_list=16),                         # ... once the list of marked entries overflows
 ])
 def test_scheduler_variants_stay_exact(c1, variant):
     log(f"variant {variant}")
