@@ -71,9 +71,9 @@ struct Params {
     double prune_ratio = 0.995;  // BC:2647
     // also GR once the discharge has scanned gr_work x E entries since the
     // last GR (a GR costs about E scans; 0 = off).  On R-MAT 22 hub queries
-    // 0.25 beat 0.5, 1 and 2 (hub rows are rescanned between relabels);
-    // grids never reach it
-    double gr_work = 0.25;
+    // 0.125 beat 0.0625, 0.25, 0.5, 1 and 2 (hub rows are rescanned between
+    // relabels); grids never reach it
+    double gr_work = 0.125;
     // discharge cycles per pass for rows scanned by a whole wave (light rows
     // use K): one full scan of a hub row is already a long pass
     int K_heavy = 1;
