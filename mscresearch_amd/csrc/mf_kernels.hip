@@ -1913,8 +1913,17 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
         const GTask gt = giant_task(S, task);
         const int k = gt.k, v = gt.v;
         const int e0 = gt.e0, e1 = gt.e1;
-        // the chunk's residuals and columns, in flight while the row's
-        // excess and height come to every thread
+        // the row's excess and height first: a chunk that finds its row dry
+        // (the row's other chunks took the excess) reads none of its entries
+        if (threadIdx.x == 0) {
+            bcast = ld64(&S.excess[v]);
+            sh[3] = S.h[v];
+        }
+        __syncthreads();
+        const int64_t ex0 = bcast;
+        const int hv = sh[3];
+        __syncthreads();
+        if (ex0 <= 0 || hv >= n) continue;  // dry or cut off: no report, no relabel
         int64_t r[PER];
         int hw[PER], cw[PER];
 #pragma unroll
@@ -1927,15 +1936,6 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
                 cw[j] = S.g.col[e];
             }
         }
-        if (threadIdx.x == 0) {
-            bcast = ld64(&S.excess[v]);
-            sh[3] = S.h[v];
-        }
-        __syncthreads();
-        const int64_t ex0 = bcast;
-        const int hv = sh[3];
-        __syncthreads();
-        if (ex0 <= 0 || hv >= n) continue;  // dry or cut off: no report, no relabel
         int64_t tsum = 0;
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
