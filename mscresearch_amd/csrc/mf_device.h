@@ -29,6 +29,10 @@ struct DGraph {
     int64_t* capsum = nullptr;
     uint64_t* cert_fp = nullptr;
     uint64_t cert_seed = 0;
+    // pcap[e] = cap[e] + cap[rev e], kept current by capacity updates: the
+    // pair's residuals sum to it, so res[rev e] = pcap[e] - res[e] is read
+    // along e's row instead of gathered from rev e's
+    int64_t* pcap = nullptr;
 };
 
 // Offset (in int32 words, 16-byte aligned) of the giant list's task records.

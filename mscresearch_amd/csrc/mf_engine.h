@@ -132,6 +132,7 @@ struct DeviceGraph {
     int32_t* d_wcc = nullptr;
     int64_t* d_capsum = nullptr;   // certificate constants (DGraph::capsum, cert_fp)
     uint64_t* d_cert_fp = nullptr;
+    int64_t* d_pcap = nullptr;     // pair capacities (DGraph::pcap)
     struct FwdSet {
         std::vector<uint8_t> mask;  // host copy
         uint8_t* d_mask = nullptr;

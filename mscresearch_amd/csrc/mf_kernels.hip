@@ -127,6 +127,7 @@ struct GGraph {
     GAS int64_t* capsum;
     GAS uint64_t* cert_fp;
     uint64_t cert_seed;
+    const GAS int64_t* pcap;
 };
 
 struct GSlot {
@@ -175,6 +176,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.g.capsum = as_global(v.g.capsum);
     S.g.cert_fp = as_global(v.g.cert_fp);
     S.g.cert_seed = v.g.cert_seed;
+    S.g.pcap = as_global(v.g.pcap);
     S.res = as_global(v.res);
     S.excess = as_global(v.excess);
     S.h = as_global(v.h);
@@ -496,12 +498,13 @@ __device__ __forceinline__ bool bfs_vertex_ok(const GSlot& S, int v) {
 
 // May frontier vertex v (entry e: v -> w) discover w?  GR: w != s, w in
 // the mask, and w can push to v (res[rev e] > 0); FWD: e carries capacity;
-// UND: always.  Split in two so that the entry-side load is issued with the
-// column load: aux = edge_aux(e) (GR: rev[e]; FWD: cap[e] > 0), then
+// UND: always.  Split in two: aux = edge_aux(e), read along e's row with
+// the column (GR: res[rev e] = pcap[e] - res[e] > 0; FWD: cap[e] > 0), so
+// w's level is gathered only for entries that pass; then
 // edge_ok_aux(aux, w).
 template <int MODE>
 __device__ __forceinline__ int edge_aux(const GSlot& S, int e) {
-    if (MODE == BFS_GR) return S.g.rev[e];
+    if (MODE == BFS_GR) return S.g.pcap[e] - ld_res(&S.res[e]) > 0 ? 1 : 0;
     if (MODE == BFS_FWD) return S.g.cap[e] > 0 ? 1 : 0;
     return 1;
 }
@@ -511,16 +514,15 @@ __device__ __forceinline__ bool edge_ok_aux(const GSlot& S, int aux, int w) {
     if (MODE == BFS_GR) {
         if (w == S.s) return false;
         if (S.mask && S.mask[w] == 0) return false;
-        return ld_res(&S.res[aux]) > 0;
     }
     return aux != 0;
 }
 
 // Top-down expansion of entries e0, e0 + step, ... below e1 by one thread,
-// U entries at a time: their column and entry-side loads, then their level
-// gathers, then their residual gathers, each issued together (one memory
-// round trip per stage instead of one per entry and stage).  claim(w) runs
-// for each vertex this thread moved from n to level L + 1.
+// U entries at a time: their column and entry-side loads, then the level
+// gathers of the entries that pass, each issued together (one memory round
+// trip per stage instead of one per entry and stage).  claim(w) runs for
+// each vertex this thread moved from n to level L + 1.
 template <int MODE, int U, typename Claim>
 __device__ __forceinline__ void td_entries(const GSlot& S, int e0, int e1, int step, int L,
                                            GAS int32_t* lvl, Claim claim) {
@@ -539,7 +541,7 @@ __device__ __forceinline__ void td_entries(const GSlot& S, int e0, int e1, int s
             }
         }
 #pragma unroll
-        for (int k = 0; k < U; ++k) l[k] = w[k] >= 0 ? lvl[w[k]] : -1;
+        for (int k = 0; k < U; ++k) l[k] = (w[k] >= 0 && ax[k] != 0) ? lvl[w[k]] : -1;
 #pragma unroll
         for (int k = 0; k < U; ++k) ok[k] = l[k] == n && edge_ok_aux<MODE>(S, ax[k], w[k]);
 #pragma unroll
@@ -1370,7 +1372,7 @@ __global__ __launch_bounds__(1024) void k_local_bfs(const SlotView* __restrict__
             for (int e = rb; e < re; ++e) {
                 const int w = S.g.col[e];
                 if (S.h[w] != mark) continue;
-                if (ld_res(&S.res[S.g.rev[e]]) <= 0) continue;
+                if (S.g.pcap[e] - ld_res(&S.res[e]) <= 0) continue;  // res[rev e]
                 if (cas32(&S.h[w], mark, L + 1)) {
                     out[atomicAdd(&next_cnt, 1)] = w;
                     add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
@@ -1387,7 +1389,7 @@ __global__ __launch_bounds__(1024) void k_local_bfs(const SlotView* __restrict__
             for (int e = rb + (int)threadIdx.x; e < re; e += blockDim.x) {
                 const int w = S.g.col[e];
                 if (S.h[w] != mark) continue;
-                if (ld_res(&S.res[S.g.rev[e]]) <= 0) continue;
+                if (S.g.pcap[e] - ld_res(&S.res[e]) <= 0) continue;  // res[rev e]
                 if (cas32(&S.h[w], mark, L + 1)) {
                     out[atomicAdd(&next_cnt, 1)] = w;
                     add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
@@ -2904,13 +2906,27 @@ __global__ void k_cap_scatter(DGraph g, int64_t* __restrict__ cap, const int32_t
     }
 }
 
+// After k_cap_scatter (an entry and its pair may both be in one batch):
+// the updated pairs' capacity sums.
+__global__ void k_pcap_update(DGraph g, const int32_t* __restrict__ entries, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int e = entries[i];
+        const int rv = g.rev[e];
+        const int64_t p = g.cap[e] + g.cap[rv];
+        g.pcap[e] = p;
+        g.pcap[rv] = p;
+    }
+}
+
 void launch_cap_scatter(hipStream_t st, const DGraph& g, int64_t* cap, const int32_t* entries,
                         const int64_t* new_cap, int64_t* old_cap, int n) {
     const int gx = std::max(1, std::min(4096, (n + 255) / 256));
     hipLaunchKernelGGL(k_cap_scatter, dim3(gx), dim3(256), 0, st, g, cap, entries, new_cap, old_cap, n);
+    hipLaunchKernelGGL(k_pcap_update, dim3(gx), dim3(256), 0, st, g, entries, n);
 }
 
-// Certificate constants of a graph: capsum[v] and cert_fp (see DGraph).
+// Certificate constants of a graph: capsum[v] and cert_fp (see DGraph),
+// and the pair capacities pcap.
 // One wave per row (rows of any length stay balanced), rows grid-stride.
 __global__ __launch_bounds__(256) void k_cert_consts(DGraph g, int64_t* __restrict__ capsum,
                                                      uint64_t* __restrict__ fp_out) {
@@ -2922,8 +2938,10 @@ __global__ __launch_bounds__(256) void k_cert_consts(DGraph g, int64_t* __restri
         int64_t cs = 0;
         for (int e = g.row_ptr[v] + lane; e < g.row_ptr[v + 1]; e += WAVE) {
             const int64_t c = g.cap[e];
+            const int rv = g.rev[e];
             cs += c;
-            fp += pair_weight((uint64_t)min(e, g.rev[e]), g.cert_seed) * (uint64_t)c;
+            fp += pair_weight((uint64_t)min(e, rv), g.cert_seed) * (uint64_t)c;
+            g.pcap[e] = c + g.cap[rv];
         }
         cs = wave_sum_i64(cs);
         if (lane == 0) capsum[v] = cs;
