@@ -307,8 +307,8 @@ def main():
         print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy, "copies": 3}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box here has one MI355X and this path shards by query "
-              "(see DESIGN.md); run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU (one MI355X); across GPUs this path shards by query, each GPU "
+              "running its own engines on its own share of the queries; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     args.queries = args.steps * args.batch
     if M.device_count() < 1:
