@@ -256,16 +256,29 @@ struct Work {
 // Mark entry e (its pair changed since the residual was last reset) and,
 // the first time, append it to the slab's list, so resets and certificates
 // visit the marked entries only.  The mark byte is set through an atomic OR
-// on its 32-bit word, whose old value says who marked it first; past mcap
-// entries only the map is kept (the count goes on past mcap).
+// on its 32-bit word, whose old value says who marked it first.  The
+// lanes of a wave that mark first take their list places with one atomic
+// add (the count is one hot word per slab); past mcap entries only the map
+// is kept (the count goes on past mcap).  Every lane that reaches the call
+// takes part in the ballot.
 __device__ __forceinline__ void mark_entry(const GSlot& S, int e) {
-    if (S.emark[e]) return;
-    GAS uint32_t* w = (GAS uint32_t*)(S.emark + (e & ~3));  // slab arrays are 256-byte aligned
-    const uint32_t bit = 1u << (8 * (e & 3));
-    const uint32_t old = __hip_atomic_fetch_or(w, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (old & bit) return;
-    const int k = add32(&S.mlist[0], 1);
-    if (k < S.mcap) S.mlist[1 + k] = e;
+    bool first = false;
+    if (!S.emark[e]) {
+        GAS uint32_t* w = (GAS uint32_t*)(S.emark + (e & ~3));  // slab arrays are 256-byte aligned
+        const uint32_t bit = 1u << (8 * (e & 3));
+        first = !(__hip_atomic_fetch_or(w, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit);
+    }
+    const uint64_t m = __ballot(first);
+    if (m == 0) return;
+    const int lane = lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = add32(&S.mlist[0], __popcll(m));
+    base = __shfl(base, leader, WAVE);
+    if (first) {
+        const int k = base + __popcll(m & ((1ULL << lane) - 1));
+        if (k < S.mcap) S.mlist[1 + k] = e;
+    }
 }
 
 // Move d units of residual across entry e (its pair rev[e] gains them) and
@@ -498,22 +511,26 @@ __device__ __forceinline__ bool bfs_vertex_ok(const GSlot& S, int v) {
 
 // May frontier vertex v (entry e: v -> w) discover w?  GR: w != s, w in
 // the mask, and w can push to v (res[rev e] > 0); FWD: e carries capacity;
-// UND: always.  Split in two: aux = edge_aux(e), read along e's row with
-// the column (GR: res[rev e] = pcap[e] - res[e] > 0; FWD: cap[e] > 0), so
-// w's level is gathered only for entries that pass; then
-// edge_ok_aux(aux, w).
-template <int MODE>
+// UND: always.  Split in two: aux = edge_aux(e), loaded with the column,
+// then edge_ok_aux(aux, w).  GR with ROW: res[rev e] = pcap[e] - res[e] is
+// read along e's row (16 coalesced bytes), so w's level is gathered only
+// for entries that pass: the wide frontiers of the grid-wide top-down
+// levels.  GR without ROW: aux = rev[e] and res[rev e] is gathered only for
+// unvisited w: the single-workgroup levels, whose narrow frontiers meet
+// mostly visited neighbours and wait on each load's latency.
+template <int MODE, bool ROW>
 __device__ __forceinline__ int edge_aux(const GSlot& S, int e) {
-    if (MODE == BFS_GR) return S.g.pcap[e] - ld_res(&S.res[e]) > 0 ? 1 : 0;
+    if (MODE == BFS_GR) return ROW ? (S.g.pcap[e] - ld_res(&S.res[e]) > 0 ? 1 : 0) : S.g.rev[e];
     if (MODE == BFS_FWD) return S.g.cap[e] > 0 ? 1 : 0;
     return 1;
 }
 
-template <int MODE>
+template <int MODE, bool ROW>
 __device__ __forceinline__ bool edge_ok_aux(const GSlot& S, int aux, int w) {
     if (MODE == BFS_GR) {
         if (w == S.s) return false;
         if (S.mask && S.mask[w] == 0) return false;
+        if (!ROW) return ld_res(&S.res[aux]) > 0;
     }
     return aux != 0;
 }
@@ -523,7 +540,7 @@ __device__ __forceinline__ bool edge_ok_aux(const GSlot& S, int aux, int w) {
 // gathers of the entries that pass, each issued together (one memory round
 // trip per stage instead of one per entry and stage).  claim(w) runs for
 // each vertex this thread moved from n to level L + 1.
-template <int MODE, int U, typename Claim>
+template <int MODE, int U, bool ROW = true, typename Claim>
 __device__ __forceinline__ void td_entries(const GSlot& S, int e0, int e1, int step, int L,
                                            GAS int32_t* lvl, Claim claim) {
     const int n = S.g.n;
@@ -537,13 +554,17 @@ __device__ __forceinline__ void td_entries(const GSlot& S, int e0, int e1, int s
             ax[k] = 0;
             if (ek < e1) {
                 w[k] = S.g.col[ek];
-                ax[k] = edge_aux<MODE>(S, ek);
+                ax[k] = edge_aux<MODE, ROW>(S, ek);
             }
         }
 #pragma unroll
-        for (int k = 0; k < U; ++k) l[k] = (w[k] >= 0 && ax[k] != 0) ? lvl[w[k]] : -1;
+        for (int k = 0; k < U; ++k) {
+            // without ROW a GR aux is an entry index, not a flag
+            const bool pre = (MODE == BFS_GR && !ROW) || ax[k] != 0;
+            l[k] = (w[k] >= 0 && pre) ? lvl[w[k]] : -1;
+        }
 #pragma unroll
-        for (int k = 0; k < U; ++k) ok[k] = l[k] == n && edge_ok_aux<MODE>(S, ax[k], w[k]);
+        for (int k = 0; k < U; ++k) ok[k] = l[k] == n && edge_ok_aux<MODE, ROW>(S, ax[k], w[k]);
 #pragma unroll
         for (int k = 0; k < U; ++k)
             if (ok[k] && cas32(&lvl[w[k]], n, L + 1)) claim(w[k]);
@@ -983,7 +1004,7 @@ __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__
                 if (k < 1024) { heavy[k] = v; continue; }
             }
             wk.edges += (uint64_t)(re - rb);
-            td_entries<MODE, 4>(S, rb, re, 1, L, lvl, [&](int w) {
+            td_entries<MODE, 4, false>(S, rb, re, 1, L, lvl, [&](int w) {
                 const int p = atomicAdd(&next_cnt, 1);
                 out[p] = w;
                 add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
@@ -996,7 +1017,7 @@ __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__
             const int v = heavy[k];
             const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
             if (threadIdx.x == 0) wk.edges += (uint64_t)(re - rb);
-            td_entries<MODE, 2>(S, rb + (int)threadIdx.x, re, (int)blockDim.x, L, lvl, [&](int w) {
+            td_entries<MODE, 2, false>(S, rb + (int)threadIdx.x, re, (int)blockDim.x, L, lvl, [&](int w) {
                 const int p = atomicAdd(&next_cnt, 1);
                 out[p] = w;
                 add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
@@ -1113,7 +1134,7 @@ __global__ __launch_bounds__(256) void k_bfs_multi(const SlotView* __restrict__ 
                 if (k < 256) { heavy[k] = v; continue; }
             }
             wk.edges += (uint64_t)(re - rb);
-            td_entries<MODE, 4>(S, rb, re, 1, L, lvl, [&](int w) {
+            td_entries<MODE, 4, false>(S, rb, re, 1, L, lvl, [&](int w) {
                 stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
                 wk.relabels++;
             });
@@ -1124,7 +1145,7 @@ __global__ __launch_bounds__(256) void k_bfs_multi(const SlotView* __restrict__ 
             const int v = heavy[k];
             const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
             if (threadIdx.x == 0) wk.edges += (uint64_t)(re - rb);
-            td_entries<MODE, 2>(S, rb + (int)threadIdx.x, re, (int)blockDim.x, L, lvl, [&](int w) {
+            td_entries<MODE, 2, false>(S, rb + (int)threadIdx.x, re, (int)blockDim.x, L, lvl, [&](int w) {
                 stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
                 wk.relabels++;
             });
