@@ -120,6 +120,7 @@ struct SlotView {
     int32_t Kh = 1;              // ... per heavy (wave-scanned) vertex per pass
     int32_t H_cut = 100;         // "high" label threshold (BC:1469)
     int32_t phase = 0;           // 0 = normal (seeds t + deficits), 1 = drain (seeds deficits, t active)
+    int32_t seed_t_only = 0;     // phase 0 with no deficit (last count): the relabel seeds t alone
 };
 
 constexpr int kTileMax = 4096;      // vertices staged in LDS per workgroup

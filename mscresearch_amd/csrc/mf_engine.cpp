@@ -317,6 +317,9 @@ void SlotSched::fail(int code, const std::string& msg) {
 Plan SlotSched::start_gr(const Ctr* c) {
     Plan p;
     scans_at_gr_ = c ? c->wk[WF_DISCHARGE][0] : 0;
+    // deficits come only from RM repairs and capacity updates, and a count
+    // follows both: with none in the last count, t is the one seed
+    p.seed_t_only = c && phase_ == 0 && c->neg_count == 0;
     p.stages = ST_RESET_PRE | ST_GR_SEED | ST_BFS;
     p.reset_pre = RST_BFS;
     p.bfs_kind = BK_SMALL;
@@ -1042,6 +1045,7 @@ int Engine::step(std::vector<Result>& out) {
             RG.depth_hint.store(S.sched.gr_depth(), std::memory_order_relaxed);
         }
         S.view.phase = S.sched.phase();
+        S.view.seed_t_only = plans[i].seed_t_only ? 1 : 0;
         S.view.K = S.sched.K();
         S.sched.stats.steps++;
     }

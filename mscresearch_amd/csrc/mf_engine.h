@@ -244,6 +244,7 @@ struct Plan {
     int cycles = 1;           // ST_DISCHARGE: discharge + count cycles in this step
     bool sparse = false;      // ST_ACTIVE: count the previous list and the touched vertices only
     int small_wgs = 1;        // BK_SMALL: workgroups per slot (1 or Params::small_wgs)
+    bool seed_t_only = false; // ST_GR_SEED in phase 0 after a count that saw no deficit: t is the one seed
 };
 
 // Per-query statistics (host side).

@@ -151,7 +151,7 @@ struct GSlot {
     int32_t mcap;
     const GAS int32_t* mask;
     GAS Ctr* ctr;
-    int32_t s, t, K, Kh, H_cut, phase;
+    int32_t s, t, K, Kh, H_cut, phase, seed_t_only;
     int32_t tstamp;  // this cycle's touch stamp (negative)
 };
 
@@ -202,6 +202,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.Kh = v.Kh;
     S.H_cut = v.H_cut;
     S.phase = v.phase;
+    S.seed_t_only = v.seed_t_only;
     S.tstamp = -(2 * S.ctr->epoch + 1);
     return S;
 }
@@ -621,7 +622,7 @@ __global__ __launch_bounds__(256) void k_gr_seed(const SlotView* __restrict__ sl
         const int v = (int)i;
         bool seed = false;
         if (v != S.s && (S.mask == nullptr || S.mask[v] != 0 || v == S.t)) {
-            if (S.phase == 0) seed = (v == S.t) || (S.excess[v] < 0);
+            if (S.phase == 0) seed = (v == S.t) || (!S.seed_t_only && S.excess[v] < 0);
             else seed = (v != S.t) && (S.excess[v] < 0);
         }
         S.h[v] = seed ? 0 : n;
