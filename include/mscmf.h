@@ -91,6 +91,15 @@ void mf_hgraph_free(mf_hgraph* h);
 int mf_hgraph_save(const mf_hgraph* h, const char* path, const char* source_path);
 int mf_hgraph_open(const char* path, const char* source_path, int R, mf_hgraph** out);
 
+/* CSR entry of each accepted input line's forward edge, in line order:
+ * us/vs are the lines the graph was built from (mf_hgraph_from_edges or the
+ * generator's list), in the ids they were given in; the entries are those of
+ * the graph's current (region-ordered) CSR.  MF_EINVAL if the lines are not
+ * the graph's.  The bulk form of the reference's per-update search for the
+ * entry src->dest (OnAdd_kernel, BC:979-987), for mf_graph_update_caps. */
+int mf_hgraph_line_entries(const mf_hgraph* h, int64_t cnt, const int32_t* us, const int32_t* vs,
+                           int32_t* entry);
+
 /* Reachability classes behind the per-query pruning mask (BC:1391-1445):
  * scc[n] = strongly connected component over entries with cap > 0, wcc[n] =
  * weakly connected component.  The mask of (s,t) with t reachable from s is
@@ -317,7 +326,9 @@ int mf_engine_pin(mf_engine* e, int slot, int gid, int s, int t, int64_t tag);
  * fails its certificate is re-solved cold (counted in mf_result.drains < 0). */
 int mf_engine_solve_pinned(mf_engine* e, int warm, mf_result* out, int cap, int* nout);
 /* Capacity updates on graph gid.  entries are CSR entry ids of forward
- * edges (cap > 0), new_cap >= 1, no repeats.  Every pinned residual on gid
+ * edges (cap > 0), new_cap >= 1, no repeats; a batch breaking any of these
+ * returns MF_EINVAL and changes nothing.  Engines sharing the graph must not
+ * be solving on it during the call.  Every pinned residual on gid
  * is adjusted: an increase adds residual; a decrease below the edge's flow
  * sends the surplus back (excess at the tail, deficit at the head).
  * Replaces OnAdd_kernel (BC:959-1000), which handled increases only. */
@@ -337,6 +348,11 @@ int mf_dbg_global_relabel(mf_engine* e, int gid, int s, int t, const int64_t* re
                           const int64_t* excess, int32_t* height);
 /* Pruning mask on the GPU (BC:1391-1445); mask[n] 0/1, *kept. */
 int mf_dbg_prune_mask(mf_engine* e, int gid, int s, int t, int32_t* mask, int* kept);
+/* The pruning mask the product builds (BC:1391-1445 by reachability class):
+ * the forward set of s's SCC (from a forward BFS on the GPU, or the graph's
+ * cache) AND the WCC of t, mask[n] 0/1 and *kept, when t is reachable from s
+ * (*reachable = 1); *reachable = 0 means the flow is 0 and no mask is made. */
+int mf_dbg_class_mask(mf_engine* e, int gid, int s, int t, int32_t* mask, int64_t* kept, int* reachable);
 /* Induced subgraph on the GPU for a given mask (BC:2477-2551). */
 int mf_dbg_induced(mf_engine* e, int gid, const int32_t* mask, int* n_out, int64_t* E_out,
                    int32_t* row_ptr, int32_t* col, int64_t* cap, int32_t* rev,

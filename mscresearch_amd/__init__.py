@@ -109,6 +109,7 @@ SIGNATURES = {
     "mf_hgraph_free": (None, [_P]),
     "mf_hgraph_find_entry": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_int32)]),
     "mf_hgraph_components": (C.c_int, [_P, _P, _P]),
+    "mf_hgraph_line_entries": (C.c_int, [_P, C.c_int64, _P, _P, _P]),
     "mf_partition": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     "mf_read_queries": (C.c_int, [C.c_char_p, C.c_int64, _P, _P, C.POINTER(C.c_int64)]),
     "mf_gen_uniform": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_uint64, C.POINTER(_P)]),
@@ -155,6 +156,7 @@ SIGNATURES = {
     "mf_dbg_certify": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, C.c_int, _P, _P, _P, _P]),
     "mf_dbg_prune_mask": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.POINTER(C.c_int)]),
     "mf_dbg_hot_select": (C.c_int, [_P, C.c_int, _P, C.c_int, _P]),
+    "mf_dbg_class_mask": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "mf_dbg_induced": (C.c_int, [_P, C.c_int, _P, C.POINTER(C.c_int), C.POINTER(C.c_int64), _P, _P, _P, _P, _P]),
     "mf_dbg_solve_state": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(Result), _P, _P, _P]),
 }
@@ -386,6 +388,15 @@ class HostGraph:
                                       _ptr(d.get("rstart")), _ptr(d.get("region_id"))))
         return d
 
+    def new_ids(self):
+        """Original id -> region-ordered id (after partition)."""
+        n, _E, R, _ = self.info()
+        if R <= 0:
+            return np.arange(n, dtype=np.int32)
+        out = np.empty(n, np.int32)
+        _check(lib().mf_hgraph_export(self._h, None, None, None, None, _ptr(out), None, None, None))
+        return out
+
     def stats(self):
         n = self.info()[0]
         so, si = np.empty(n, np.int64), np.empty(n, np.int64)
@@ -399,6 +410,15 @@ class HostGraph:
         wcc = np.empty(n, np.int32)
         _check(lib().mf_hgraph_components(self._h, _ptr(scc), _ptr(wcc)))
         return scc, wcc
+
+    def line_entries(self, us, vs):
+        """Forward CSR entry (current ids) of every line the graph was built
+        from, given those lines in their original ids."""
+        us = np.ascontiguousarray(us, np.int32)
+        vs = np.ascontiguousarray(vs, np.int32)
+        out = np.empty(len(us), np.int32)
+        _check(lib().mf_hgraph_line_entries(self._h, len(us), _ptr(us), _ptr(vs), _ptr(out)))
+        return out
 
     def find_entry(self, u, v):
         e = C.c_int32(-1)
@@ -584,6 +604,13 @@ class Engine:
         k = C.c_int(0)
         _check(lib().mf_dbg_prune_mask(self._h, gid, int(s), int(t), _ptr(mask), C.byref(k)))
         return mask, k.value
+
+    def dbg_class_mask(self, gid, s, t, n):
+        """The product's class-path pruning mask: (mask or None, kept, reachable)."""
+        mask = np.zeros(n, np.int32)
+        kept, reach = C.c_int64(0), C.c_int(0)
+        _check(lib().mf_dbg_class_mask(self._h, gid, int(s), int(t), _ptr(mask), C.byref(kept), C.byref(reach)))
+        return (mask if reach.value else None), kept.value, bool(reach.value)
 
     def dbg_induced(self, gid, mask, E_parent):
         mask = np.ascontiguousarray(mask, np.int32)

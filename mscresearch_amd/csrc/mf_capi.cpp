@@ -12,26 +12,15 @@
 #include "../../include/mscmf.h"
 #include "host_graph.h"
 #include "mf_engine.h"
+#include "mf_handles.h"
 #include "mf_kernels.h"
 #include "mf_tune.h"
 
 using namespace mf;
 
-struct mf_hgraph {
-    HostGraph g;
-};
-struct mf_edges {
-    int n = 0;
-    std::vector<int32_t> us, vs;
-    std::vector<int64_t> caps;
-};
-struct mf_graph {
-    std::shared_ptr<DeviceGraph> dg;
-};
-struct mf_engine {
-    Engine eng;
-    std::vector<Result> done;  // results not yet handed out
-};
+namespace mf {
+void result_to_c(const Result& r, mf_result* o);
+}
 
 namespace {
 
@@ -42,7 +31,11 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
-void to_c(const Result& r, mf_result* o) {
+void to_c(const Result& r, mf_result* o) { mf::result_to_c(r, o); }
+
+}  // namespace
+
+void mf::result_to_c(const Result& r, mf_result* o) {
     memset(o, 0, sizeof(*o));
     o->tag = r.tag;
     o->flow = r.st.flow;
@@ -68,6 +61,8 @@ void to_c(const Result& r, mf_result* o) {
     for (int f = 0; f < kFamilies; ++f)
         for (int k = 0; k < 4; ++k) o->work[f][k] = r.st.wk[f][k];
 }
+
+namespace {
 
 Params from_c(const mf_params* p) {
     Params P;
@@ -254,6 +249,27 @@ int mf_hgraph_find_entry(const mf_hgraph* h, int u, int v, int32_t* entry) {
     if (u < 0 || u >= h->g.n || v < 0 || v >= h->g.n) return fail(MF_EINVAL, "vertex out of range");
     for (int e = h->g.row_ptr[u]; e < h->g.row_ptr[u + 1]; ++e)
         if (h->g.col[e] == v && h->g.cap[e] > 0) { *entry = e; break; }
+    return MF_OK;
+}
+
+int mf_hgraph_line_entries(const mf_hgraph* h, int64_t cnt, const int32_t* us, const int32_t* vs, int32_t* entry) {
+    if (!h || (cnt > 0 && (!us || !vs || !entry))) return fail(MF_EINVAL, "null argument");
+    const HostGraph& g = h->g;
+    if (cnt != g.lines) return fail(MF_EINVAL, "line count differs from the graph's accepted lines");
+    const int n = g.n;
+    const bool reord = (int)g.new_id.size() == n && n > 0;
+    // row lengths of the original order are those of the current rows
+    std::vector<int32_t> cur(n, 0);
+    for (int64_t i = 0; i < cnt; ++i) {
+        const int u = us[i], v = vs[i];
+        if (u < 0 || u >= n || v < 0 || v >= n) return fail(MF_EINVAL, "line endpoint out of range");
+        const int ru = reord ? g.new_id[u] : u, rv = reord ? g.new_id[v] : v;
+        const int64_t e = (int64_t)g.row_ptr[ru] + cur[u]++;
+        cur[v]++;
+        if (e >= g.row_ptr[ru + 1] || g.col[e] != rv || g.cap[e] <= 0)
+            return fail(MF_EINVAL, "line " + std::to_string(i) + " is not the graph's line (order or endpoints differ)");
+        entry[i] = (int32_t)e;
+    }
     return MF_OK;
 }
 
@@ -923,6 +939,28 @@ int mf_dbg_prune_mask(mf_engine* e, int gid, int s, int t, int32_t* mask, int* k
         if (kept) *kept = e->eng.ctr(0).visited;
         if (mask) HIPCK(hipMemcpy(mask, v.aux, sizeof(int32_t) * v.g.n, hipMemcpyDeviceToHost));
     }
+    e->eng.dbg_end();
+    return rc;
+}
+
+int mf_dbg_class_mask(mf_engine* e, int gid, int s, int t, int32_t* mask, int64_t* kept, int* reachable) {
+    int rc = check_query(e, gid, s, t);
+    if (rc != MF_OK) return rc;
+    if (!mask || !kept || !reachable) return fail(MF_EINVAL, "null argument");
+    rc = e->eng.dbg_begin(gid, s, t);
+    if (rc != MF_OK) return rc;
+    DeviceGraph* G = e->eng.graph(gid);
+    if (G->classes && !G->fwd_find(G->scc[s])) {
+        // the forward BFS from s the scheduler's S_PRUNE_FWD step runs
+        Plan p;
+        p.stages = ST_RESET_PRE | ST_BFS_SEED1 | ST_BFS;
+        p.reset_pre = RST_BFS | RST_WORK | RST_ERR;
+        p.seed_mode = BFS_FWD;
+        p.bfs_kind = BK_SMALL;
+        p.bfs_mode = BFS_FWD;
+        rc = dbg_bfs(e, p, BFS_FWD);
+    }
+    if (rc == MF_OK) rc = e->eng.dbg_class_mask(gid, s, t, mask, kept, reachable);
     e->eng.dbg_end();
     return rc;
 }

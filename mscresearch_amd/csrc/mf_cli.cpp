@@ -289,8 +289,10 @@ int main(int argc, char** argv) {
         fclose(f);
         write_metrics(out + "/metrics.json", "static", lat_us, wall, edges_total, (int)lat_us.size());
         fprintf(stderr, "Total time: %lld ms\n", total);
+    } else if (nq == 0) {
+        // BC:2845-2847: no queries, no res_dynamic.txt
+        fprintf(stderr, "[Dynamic] No queries found.\n");
     } else {
-        if (nq == 0) fprintf(stderr, "[Dynamic] No queries found.\n");
         // rounds of capacity updates, if any
         std::map<int, std::vector<std::pair<int32_t, int64_t>>> rounds;
         {

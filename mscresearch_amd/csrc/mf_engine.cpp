@@ -44,6 +44,19 @@ static double clock_epoch() {
     return t0;
 }
 
+// Engines alive in this process.  Each may run a multi-workgroup BFS on its
+// main and its side stream at once, and every workgroup of such a launch must
+// be resident together (they meet at barriers): the launches share a budget
+// of kMultiBudget workgroups of 256 threads, half of what 256 CUs hold at 8
+// such workgroups (32 waves) each, so other kernels always keep room to
+// finish and free the slots a launch still waits for.
+namespace {
+std::atomic<int> g_engines{0};
+constexpr int kMultiBudget = 1024;
+}  // namespace
+
+int multi_wg_cap() { return std::max(32, kMultiBudget / (2 * std::max(1, g_engines.load()))); }
+
 // ----------------------------------------------------------- device graph
 uint64_t next_graph_uid() {
     static std::atomic<uint64_t> next{1};
@@ -59,6 +72,18 @@ DeviceGraph::~DeviceGraph() {
 }
 
 DeviceGraph::FwdSet::~FwdSet() { (void)hipFree(d_mask); }
+
+std::shared_ptr<DeviceGraph::FwdSet> DeviceGraph::fwd_find(int32_t cls) const {
+    std::lock_guard<std::mutex> lk(fwd_mu);
+    auto it = fwd_cache.find(cls);
+    return it == fwd_cache.end() ? nullptr : it->second;
+}
+
+std::shared_ptr<DeviceGraph::FwdSet> DeviceGraph::fwd_insert(int32_t cls, std::shared_ptr<FwdSet> set) {
+    std::lock_guard<std::mutex> lk(fwd_mu);
+    auto ins = fwd_cache.emplace(cls, std::move(set));
+    return ins.first->second;
+}
 
 void make_tiles(const std::vector<int32_t>& rstart, int tile_max, std::vector<int32_t>& ts) {
     ts.clear();
@@ -115,6 +140,9 @@ int upload_graph(const HostGraph& hg, int tile_max, DeviceGraph& dg) {
     if (n > 0) HIPCHK(hipMemcpy(dg.d_wcc, dg.wcc.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     dg.classes = true;
     dg.bytes += sizeof(int32_t) * (size_t)n;
+    dg.fwd_bits.assign(((size_t)E + 63) / 64, 0);
+    for (int64_t e = 0; e < E; ++e)
+        if (hg.cap[e] > 0) dg.fwd_bits[(size_t)e >> 6] |= 1ull << (e & 63);
     return init_cert_consts(dg, nullptr);
 }
 
@@ -285,6 +313,8 @@ void SlotSched::begin(const Params* P, const DGraph& g, int s, int t, bool prune
     after_local_ = false;
     work_reset_ = false;
     fwd_only_ = false;
+    deficits_ = warm;
+    no_multi_ = false;
     st_ = prune_first ? S_PRUNE_FWD : (warm ? S_WARM_START : S_START);
 }
 
@@ -317,9 +347,9 @@ void SlotSched::fail(int code, const std::string& msg) {
 Plan SlotSched::start_gr(const Ctr* c) {
     Plan p;
     scans_at_gr_ = c ? c->wk[WF_DISCHARGE][0] : 0;
-    // deficits come only from RM repairs and capacity updates, and a count
-    // follows both: with none in the last count, t is the one seed
-    p.seed_t_only = c && phase_ == 0 && c->neg_count == 0;
+    // deficits come only from RM repairs and capacity updates (warm starts):
+    // when neither can have left one, t is the one seed
+    p.seed_t_only = c && phase_ == 0 && !deficits_;
     p.stages = ST_RESET_PRE | ST_GR_SEED | ST_BFS;
     p.reset_pre = RST_BFS;
     p.bfs_kind = BK_SMALL;
@@ -356,7 +386,7 @@ Plan SlotSched::bfs_step(const Ctr& c, int mode) {
         p.bfs_grid = clampi(cdiv(g_.n, 256), 1, 8192);
     } else if (f <= kSmallFrontier && mf <= P_->small_edges) {
         p.bfs_kind = BK_SMALL;
-        if (gr_depth_ >= 64 || depth_hint_ >= 64 || L >= 64) p.small_wgs = std::max(1, P_->small_wgs);
+        if (!no_multi_ && (gr_depth_ >= 64 || depth_hint_ >= 64 || L >= 64)) p.small_wgs = std::max(1, P_->small_wgs);
     } else {
         p.bfs_kind = BK_TD;
         p.bfs_grid = std::max(1, cdiv(f, 256));
@@ -364,11 +394,30 @@ Plan SlotSched::bfs_step(const Ctr& c, int mode) {
     return p;
 }
 
+// A multi-workgroup BFS whose barrier timed out left a partial level: the
+// BFS starts again from its seeds, single-workgroup.  A global relabel and
+// the pruning BFS both recompute everything they write from the seeds.
+Plan SlotSched::restart_bfs(const Ctr& c) {
+    no_multi_ = true;
+    stats.bfs_restarts++;
+    if (st_ == S_BFS && bfs_mode_ == BFS_GR) {
+        stats.grs--;  // the restart is the same relabel
+        Plan p = start_gr(&c);
+        p.reset_pre |= RST_ERR;
+        return p;
+    }
+    st_ = S_PRUNE_FWD;
+    Plan p = plan(c);  // S_PRUNE_FWD resets the error word
+    return p;
+}
+
 Plan SlotSched::plan(const Ctr& c) {
     Plan p;
-    // a kernel flagged the slot (a multi-workgroup BFS barrier gave up): the
-    // counters of the last step cannot be trusted
+    // a kernel flagged the slot: bit 4 = a multi-workgroup BFS barrier gave
+    // up (retried single-workgroup); anything else fails the query
     if (c.err && st_ != S_START && st_ != S_WARM_START && st_ != S_PRUNE_FWD && st_ != S_DONE) {
+        if (c.err == 4 && (st_ == S_BFS || st_ == S_PRUNE_UND || st_ == S_PRUNE_MASK) && stats.bfs_restarts < 4)
+            return restart_bfs(c);
         fail(MF_EHIP, "device error flag " + std::to_string(c.err));
         return p;
     }
@@ -457,6 +506,7 @@ Plan SlotSched::plan(const Ctr& c) {
         } else {
             pos_full_ = c.pos_excess;
             ext_full_ = c.ex_t;
+            deficits_ = c.neg_count > 0;
         }
         // counts right after a local relabel follow no discharge: they do not
         // feed the stagnation and stall monitors
@@ -468,7 +518,7 @@ Plan SlotSched::plan(const Ctr& c) {
                 // The certificate proves optimality for any closed T = {h < n}:
                 // try it on the discharge's labels first; a failure falls back
                 // to the exact global relabel.
-                if (P_->cert && P_->spec_cert && phase_ == 0 && !spec_failed_ && c.neg_count == 0) {
+                if (P_->cert && P_->spec_cert && phase_ == 0 && !spec_failed_ && !deficits_) {
                     p.stages = ST_RESET_PRE | ST_CERT;
                     p.reset_pre = RST_CERT;
                     spec_ = true;
@@ -597,11 +647,12 @@ Plan SlotSched::plan(const Ctr& c) {
             p.stages |= ST_RM;
             p.reset_act |= RST_RM;
             rm_last_ = true;
+            deficits_ = true;
         }
         // sparse count: every kernel of this step that pushes records the
         // vertices it pushes into, so those and the previous active list hold
         // every vertex that can be active after it
-        if (P_->sparse_count && !use_path && !(p.stages & ST_RM) && phase_ == 0 && c.neg_count == 0) {
+        if (P_->sparse_count && !use_path && !(p.stages & ST_RM) && phase_ == 0 && !deficits_) {
             p.sparse = true;
             p.stages |= ST_RESET_PRE;
             p.reset_pre |= RST_TOUCH;
@@ -609,8 +660,10 @@ Plan SlotSched::plan(const Ctr& c) {
             counted_sparse_ = true;
         }
         // deep graphs: several cycles per host round trip, up to the next
-        // count-due global relabel
-        if (!use_path && P_->fuse > 1 && gr_depth_ >= 64) {
+        // count-due global relabel.  The extra cycles count sparsely, so only
+        // a step whose own count is sparse (no RM, no deficit) fuses: the
+        // counters the host reads then always come from a sparse count.
+        if (!use_path && p.sparse && P_->fuse > 1 && gr_depth_ >= 64) {
             const int64_t left = std::max<int64_t>(1, std::max<int64_t>(GR_FREQ_, gr_depth_ / 4) - (iter_ - last_gr_iter_));
             p.cycles = (int)std::min<int64_t>(P_->fuse, left);
         }
@@ -646,6 +699,7 @@ Plan SlotSched::plan(const Ctr& c) {
 
 // ------------------------------------------------------------------ engine
 Engine::~Engine() {
+    if (counted_) g_engines.fetch_sub(1);
     if (stream_) (void)hipStreamSynchronize(stream_);
     slots_.clear();
     arena_.destroy();
@@ -673,6 +727,7 @@ Engine::~Engine() {
 
 double Engine::now() const { return wall_seconds() - t0_; }
 
+
 long long barrier_spin_ticks() {
     int dev = 0, khz = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
@@ -687,6 +742,7 @@ int Engine::init(int nslots, const Params& P) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { set_error("no HIP device"); return MF_ENODEV; }
     P_ = P;
     nslots_ = nslots;
+    HIPCHK(hipGetDevice(&device_));
     spin_ticks_ = barrier_spin_ticks();
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
@@ -712,6 +768,8 @@ int Engine::init(int nslots, const Params& P) {
     for (auto& ev : ev_) HIPCHK(hipEventCreate(&ev));
     slots_.resize(nslots);
     t0_ = clock_epoch();
+    g_engines.fetch_add(1);
+    counted_ = true;
     return MF_OK;
 }
 
@@ -792,11 +850,11 @@ void Engine::run_dims(const Query& q, int* n, int64_t* E, int* ntiles) const {
     *ntiles = G.g.ntiles;
     if (!P_.prune || !G.classes) return;
     const int cls = G.scc[q.s];
-    auto it = G.fwd_cache.find(cls);
-    if (it == G.fwd_cache.end() || !it->second->mask[q.t]) return;
+    auto fwd = G.fwd_find(cls);
+    if (!fwd || !fwd->mask[q.t]) return;
     const std::string key = std::to_string(q.graph) + ":c" + std::to_string(cls) + ":" + std::to_string(G.wcc[q.t]);
     auto ct = cache_.find(key);
-    if (ct == cache_.end() || !ct->second.g) return;
+    if (ct == cache_.end() || !ct->second.g || ct->second.version != G.cap_version.load()) return;
     const DGraph& r = ct->second.g->g;
     *n = r.n;
     *E = r.E;
@@ -847,7 +905,7 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
     v.K = S.sched.K();
     if (prune && G.classes) {
         // with the forward set of s's SCC at hand the prune needs no BFS
-        if (G.fwd_cache.count(G.scc[q.s])) return class_decide(i);
+        if (G.fwd_find(G.scc[q.s])) return class_decide(i);
         S.sched.set_fwd_only();
     }
     return MF_OK;
@@ -857,28 +915,57 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
 // is the mask of BC:1391-1445 whenever t is reachable from s (then s and t
 // are in it); when t is not reachable the flow is 0 (fwd(s) is a closed set
 // holding s but not t: an s-t cut of capacity 0).
+int Engine::fwd_set(DeviceGraph& G, int cls, const int32_t* d_h, std::shared_ptr<DeviceGraph::FwdSet>& out) {
+    out = G.fwd_find(cls);
+    if (out) return MF_OK;
+    // d_h holds a BFS_FWD from a vertex of the class: h < n is its forward set
+    auto fwd = std::make_shared<DeviceGraph::FwdSet>();
+    const int n = G.g.n;
+    HIPCHK(hipMalloc(&fwd->d_mask, std::max(n, 1)));
+    launch_fwd_from_h(stream_, d_h, n, fwd->d_mask);
+    fwd->mask.resize(n);
+    if (n > 0) HIPCHK(hipMemcpyAsync(fwd->mask.data(), fwd->d_mask, n, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    int64_t cnt = 0;
+    for (int v = 0; v < n; ++v) cnt += fwd->mask[v];
+    fwd->count = cnt;
+    // another engine may have cached the same class meanwhile: keep one
+    out = G.fwd_insert(cls, fwd);
+    return MF_OK;
+}
+
+int64_t Engine::class_kept(const DeviceGraph& G, const DeviceGraph::FwdSet& fwd, int wt) {
+    int64_t kept = 0;
+    for (int v = 0; v < G.g.n; ++v) kept += (fwd.mask[v] && G.wcc[v] == wt) ? 1 : 0;
+    return kept;
+}
+
+int Engine::dbg_class_mask(int gid, int s, int t, int32_t* mask, int64_t* kept, int* reachable) {
+    DeviceGraph& G = *graphs_[gid];
+    if (!G.classes) { set_error("graph has no reachability classes"); return MF_EINVAL; }
+    std::shared_ptr<DeviceGraph::FwdSet> fwd;
+    int rc = fwd_set(G, G.scc[s], slots_[0].view.h, fwd);
+    if (rc != MF_OK) return rc;
+    *reachable = fwd->mask[t] ? 1 : 0;
+    *kept = 0;
+    if (!*reachable) return MF_OK;
+    const int wt = G.wcc[t];
+    *kept = class_kept(G, *fwd, wt);
+    launch_class_mask(stream_, fwd->d_mask, G.d_wcc, wt, G.g.n, slots_[0].view.aux);
+    HIPCHK(hipMemcpyAsync(mask, slots_[0].view.aux, sizeof(int32_t) * G.g.n, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return MF_OK;
+}
+
 int Engine::class_decide(int i) {
     SlotState& S = slots_[i];
     DeviceGraph& G = *graphs_[S.q.graph];
     const int cls = G.scc[S.q.s];
-    auto it = G.fwd_cache.find(cls);
     std::shared_ptr<DeviceGraph::FwdSet> fwd;
-    if (it == G.fwd_cache.end()) {
-        // the slot has just run BFS_FWD from s: h < n is fwd(s)
-        fwd = std::make_shared<DeviceGraph::FwdSet>();
-        const int n = G.g.n;
-        HIPCHK(hipMalloc(&fwd->d_mask, std::max(n, 1)));
-        launch_fwd_from_h(stream_, S.view.h, n, fwd->d_mask);
-        fwd->mask.resize(n);
-        if (n > 0) HIPCHK(hipMemcpyAsync(fwd->mask.data(), fwd->d_mask, n, hipMemcpyDeviceToHost, stream_));
-        HIPCHK(hipStreamSynchronize(stream_));
-        int64_t cnt = 0;
-        for (int v = 0; v < n; ++v) cnt += fwd->mask[v];
-        fwd->count = cnt;
-        G.fwd_cache[cls] = fwd;
-    } else {
-        fwd = it->second;
-    }
+    // after S_PRUNE_FWD (when the class was not cached) the slot's h holds
+    // the forward BFS from s
+    int frc = fwd_set(G, cls, S.view.h, fwd);
+    if (frc != MF_OK) return frc;
     if (!fwd->mask[S.q.t]) {
         S.sched.finish_trivial(0);
         return MF_OK;
@@ -888,18 +975,25 @@ int Engine::class_decide(int i) {
     std::shared_ptr<DeviceGraph> run;
     bool known = false;
     auto ct = cache_.find(key);
+    const uint64_t ver = G.cap_version.load();
+    if (ct != cache_.end() && ct->second.version != ver) {
+        // built before a capacity update (by any engine sharing the graph)
+        if (ct->second.g) cache_bytes_ -= ct->second.g->bytes;
+        cache_.erase(ct);
+        ct = cache_.end();
+    }
     if (ct != cache_.end()) {
         run = ct->second.g;
         ct->second.last_use = ++use_clock_;
         known = true;
         ++cache_hits;
-        S.sched.stats.kept = run ? run->g.n : G.g.n;
+        S.sched.stats.kept = (int)ct->second.kept;
     }
+    int64_t kept = 0;
     if (!known) {
         ++cache_misses;
         const int n = G.g.n;
-        int64_t kept = 0;
-        for (int v = 0; v < n; ++v) kept += (fwd->mask[v] && G.wcc[v] == wt) ? 1 : 0;
+        kept = class_kept(G, *fwd, wt);
         S.sched.stats.kept = (int)kept;
         if (kept > 0 && (double)kept < P_.prune_ratio * (double)n) {
             launch_class_mask(stream_, fwd->d_mask, G.d_wcc, wt, n, S.view.aux);
@@ -916,7 +1010,7 @@ int Engine::class_decide(int i) {
             if (lru->second.g) cache_bytes_ -= lru->second.g->bytes;
             cache_.erase(lru);
         }
-        cache_[key] = CacheEntry{run, ++use_clock_};
+        cache_[key] = CacheEntry{run, ++use_clock_, kept, ver};
     }
     if (run) {
         const int s = run->old2sub_host.empty() ? -1 : run->old2sub_host[S.q.s];
@@ -952,6 +1046,12 @@ int Engine::prune_decide(int i) {
         const std::string key = std::to_string(S.q.graph) + ":" + std::to_string(hsh) + ":" + std::to_string(kept);
         std::shared_ptr<DeviceGraph> run;
         auto it = cache_.find(key);
+        const uint64_t ver = G.cap_version.load();
+        if (it != cache_.end() && it->second.version != ver) {
+            cache_bytes_ -= it->second.g->bytes;
+            cache_.erase(it);
+            it = cache_.end();
+        }
         if (it != cache_.end()) {
             run = it->second.g;
             it->second.last_use = ++use_clock_;
@@ -969,7 +1069,7 @@ int Engine::prune_decide(int i) {
                 cache_bytes_ -= lru->second.g->bytes;
                 cache_.erase(lru);
             }
-            cache_[key] = CacheEntry{ng, ++use_clock_};
+            cache_[key] = CacheEntry{ng, ++use_clock_, kept, ver};
             run = ng;
             ++cache_misses;
         }
@@ -1183,14 +1283,14 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     auto run_g = [&](int i) { return slots_[i].run_graph ? slots_[i].run_graph.get() : graphs_[slots_[i].q.graph].get(); };
     auto marked_ok = [&](int i) {
         const Slab& sl = slots_[i].slab;
-        return sl.res_uid == run_g(i)->uid && sl.res_version == run_g(i)->cap_version;
+        return sl.res_uid == run_g(i)->uid && sl.res_version == run_g(i)->cap_version.load();
     };
     Group g_init = make_list([&](int i) { return has(i, ST_INIT) && !marked_ok(i); });
     Group g_init_m = make_list([&](int i) { return has(i, ST_INIT) && marked_ok(i); });
     for (int i = 0; i < nslots_; ++i)
         if (slots_[i].live && has(i, ST_INIT)) {
             slots_[i].slab.res_uid = run_g(i)->uid;
-            slots_[i].slab.res_version = run_g(i)->cap_version;
+            slots_[i].slab.res_version = run_g(i)->cap_version.load();
             slots_[i].slab.marks_ok = true;
         }
     Group g_pre = make_list([&](int i) { return has(i, ST_PREFLOW); });
@@ -1316,7 +1416,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         for (int m = 0; m < 3; ++m)
             for (int w = 0; w < 2; ++w)
                 if (s_bfs[m][w].cnt)
-                    launch_bfs_small(side_, d_views_, L(s_bfs[m][w]), s_bfs[m][w].cnt, m, P_.small_levels, P_.small_edges, w ? P_.small_wgs : 1, spin_ticks_);
+                    launch_bfs_small(side_, d_views_, L(s_bfs[m][w]), s_bfs[m][w].cnt, m, P_.small_levels, P_.small_edges, w ? P_.small_wgs : 1, spin_ticks_, multi_wg_cap());
         tes(k, side_);
         HIPCHK(hipEventRecord(ev_side_, side_));
     }
@@ -1355,12 +1455,12 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     for (int m = 0; m < 3; ++m) {
         if (g_bfs[BK_SMALL][m].cnt) {
             const int k = tb(WF_BFS);
-            launch_bfs_small(st, d_views_, L(g_bfs[BK_SMALL][m]), g_bfs[BK_SMALL][m].cnt, m, P_.small_levels, P_.small_edges, 1, spin_ticks_);
+            launch_bfs_small(st, d_views_, L(g_bfs[BK_SMALL][m]), g_bfs[BK_SMALL][m].cnt, m, P_.small_levels, P_.small_edges, 1, spin_ticks_, multi_wg_cap());
             te(k);
         }
         if (g_bfs_multi[m].cnt) {
             const int k = tb(WF_BFS);
-            launch_bfs_small(st, d_views_, L(g_bfs_multi[m]), g_bfs_multi[m].cnt, m, P_.small_levels, P_.small_edges, P_.small_wgs, spin_ticks_);
+            launch_bfs_small(st, d_views_, L(g_bfs_multi[m]), g_bfs_multi[m].cnt, m, P_.small_levels, P_.small_edges, P_.small_wgs, spin_ticks_, multi_wg_cap());
             te(k);
         }
         if (g_bfs[BK_TD][m].cnt) {
@@ -1586,9 +1686,27 @@ int Engine::update_caps(int gid, const int32_t* entries, const int64_t* new_cap,
     if (gid < 0 || gid >= (int)graphs_.size()) { set_error("bad graph index"); return MF_EINVAL; }
     if (n <= 0) return MF_OK;
     DeviceGraph& G = *graphs_[gid];
-    for (int i = 0; i < n; ++i) {
-        if (entries[i] < 0 || entries[i] >= G.g.E) { set_error("update entry out of range"); return MF_EINVAL; }
-        if (new_cap[i] < 1) { set_error("updated capacity must be >= 1"); return MF_EINVAL; }
+    // Only forward entries (cap > 0) carry a capacity: a reverse entry would
+    // change the topology the reachability classes and cached forward sets
+    // were computed on.  Entries must not repeat (one scatter per entry).
+    // Nothing is changed unless the whole batch is valid.
+    {
+        std::vector<uint64_t> seen((size_t)(G.g.E + 63) / 64, 0);
+        for (int i = 0; i < n; ++i) {
+            const int e = entries[i];
+            if (e < 0 || e >= G.g.E) { set_error("update entry out of range"); return MF_EINVAL; }
+            if (new_cap[i] < 1) { set_error("updated capacity must be >= 1"); return MF_EINVAL; }
+            const uint64_t bit = 1ull << (e & 63);
+            if (!G.fwd_bits.empty() && !(G.fwd_bits[(size_t)e >> 6] & bit)) {
+                set_error("update entry " + std::to_string(e) + " is a reverse entry (capacity 0)");
+                return MF_EINVAL;
+            }
+            if (seen[(size_t)e >> 6] & bit) {
+                set_error("update entry " + std::to_string(e) + " repeats");
+                return MF_EINVAL;
+            }
+            seen[(size_t)e >> 6] |= bit;
+        }
     }
     int32_t* d_e = nullptr;
     int64_t *d_new = nullptr, *d_old = nullptr;
@@ -1618,10 +1736,10 @@ int Engine::update_caps(int gid, const int32_t* entries, const int64_t* new_cap,
     // residuals of this graph kept elsewhere predate the new capacities: the
     // next query on it (in any engine) resets every entry; pinned residuals
     // were adjusted above and stay current
-    ++G.cap_version;
+    const uint64_t ver = ++G.cap_version;
     for (int i = 0; i < nslots_; ++i)
         if (slots_[i].pinned && slots_[i].solved && slots_[i].q.graph == gid && slots_[i].slab.res_uid == G.uid)
-            slots_[i].slab.res_version = G.cap_version;
+            slots_[i].slab.res_version = ver;
     // cached induced subgraphs of this graph carry old capacities
     const std::string pre = std::to_string(gid) + ":";
     for (auto it = cache_.begin(); it != cache_.end();) {
@@ -1629,6 +1747,43 @@ int Engine::update_caps(int gid, const int32_t* entries, const int64_t* new_cap,
         else ++it;
     }
     return MF_OK;
+}
+
+int Engine::solve_in(const Query& q, char* base, size_t cap_bytes, std::vector<Result>& out) {
+    if (q.graph < 0 || q.graph >= (int)graphs_.size()) { set_error("bad graph index"); return MF_EINVAL; }
+    if (live() || !queue_.empty()) { set_error("engine busy"); return MF_EBUSY; }
+    for (auto& S : slots_)
+        if (S.pinned) { set_error("engine has pinned slots"); return MF_EBUSY; }
+    const DGraph& g = graphs_[q.graph]->g;
+    const size_t need = Slab::bytes_for(g.n, g.E, g.ntiles);
+    if (need > cap_bytes) {
+        set_error("the graph's query slab needs " + std::to_string(need) + " bytes, more than the " +
+                  std::to_string(cap_bytes) + " bytes available");
+        return MF_ENOMEM;
+    }
+    // slot 0 runs the query on a slab laid over the caller's memory; the
+    // arena block it may hold is put back afterwards
+    Slab saved = slots_[0].slab;
+    Slab& sl = slots_[0].slab;
+    sl = Slab();
+    Slab::layout(g.n, g.E, g.ntiles, base, &sl);
+    sl.held = true;
+    sl.bytes = need;
+    const int graphs_saved = P_.graphs;
+    P_.graphs = 0;  // instantiating a captured graph would allocate
+    int rc = admit(0, q, false, false);
+    if (rc == MF_OK) {
+        memset(&h_ctr_[0], 0, sizeof(Ctr));
+        while (live() > 0) {
+            rc = step(out);
+            if (rc < 0) break;
+        }
+    }
+    P_.graphs = graphs_saved;
+    slots_[0].live = false;
+    slots_[0].run_graph.reset();
+    slots_[0].slab = saved;
+    return rc < 0 ? rc : MF_OK;
 }
 
 int Engine::dbg_begin(int gid, int s, int t) {
@@ -1640,7 +1795,9 @@ int Engine::dbg_begin(int gid, int s, int t) {
     int rc = ensure_slab(0, g.n, g.E, g.ntiles);
     if (rc == kNoRoom) { set_error("no room in the slab arena"); return MF_ENOMEM; }
     if (rc != MF_OK) return rc;
-    rc = admit(0, q);
+    // the hooks work on the full graph: no pruning, whatever the graph's
+    // reachability-class cache already holds
+    rc = admit(0, q, false, false);
     if (rc != MF_OK) return rc;
     memset(&h_ctr_[0], 0, sizeof(Ctr));
     return MF_OK;

@@ -7,8 +7,9 @@
 #include <cstdint>
 #include <deque>
 #include <map>
-#include <set>
 #include <memory>
+#include <mutex>
+#include <set>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -107,6 +108,9 @@ struct Params {
 // Wall-clock ticks a multi-workgroup BFS barrier may wait before it flags
 // the slot's error word (about 2 s).
 long long barrier_spin_ticks();
+// Workgroups one multi-workgroup BFS launch may hold: a process-wide budget
+// split over the live engines' two streams each.
+int multi_wg_cap();
 
 // ---------------------------------------------------------- device memory
 uint64_t next_graph_uid();
@@ -114,7 +118,9 @@ uint64_t next_graph_uid();
 struct DeviceGraph {
     DGraph g;
     uint64_t uid = next_graph_uid();  // identifies the topology a residual belongs to
-    uint64_t cap_version = 0;         // bumped by every capacity update
+    // bumped by every capacity update; engines on other threads read it when
+    // deciding whether a slab's residual is still current
+    std::atomic<uint64_t> cap_version{0};
     int32_t* d_row_ptr = nullptr;
     int32_t* d_col = nullptr;
     int64_t* d_cap = nullptr;
@@ -144,7 +150,16 @@ struct DeviceGraph {
         int64_t count = 0;
         ~FwdSet();
     };
+    // Engines on several host threads share the graph: every access to
+    // fwd_cache holds fwd_mu (the sets themselves are immutable once cached).
     std::unordered_map<int32_t, std::shared_ptr<FwdSet>> fwd_cache;
+    mutable std::mutex fwd_mu;
+    std::shared_ptr<FwdSet> fwd_find(int32_t cls) const;
+    // Inserts set unless another engine cached the class first; returns the
+    // cached one.
+    std::shared_ptr<FwdSet> fwd_insert(int32_t cls, std::shared_ptr<FwdSet> set);
+    // forward entries (cap > 0) as a bitmap: capacity updates may only name these
+    std::vector<uint64_t> fwd_bits;
     std::vector<int32_t> old2sub_host;  // pruned run graphs: host copy of d_old2sub
     ~DeviceGraph();
 };
@@ -262,6 +277,7 @@ struct QueryStats {
     int64_t drains = 0;       // deficit drain phases
     int64_t local_relabels = 0;
     int64_t spec_fails = 0;   // certificates on discharge labels that failed
+    int64_t bfs_restarts = 0; // BFS runs restarted single-workgroup after a barrier time-out
     int64_t rm_fixed = 0;
     int cert_ok = 0;
     uint64_t edge_scans = 0, vertex_scans = 0, pushes = 0, relabels = 0;
@@ -336,6 +352,15 @@ private:
     bool spec_ = false;         // the certificate under way follows discharge labels, not a GR
     bool spec_failed_ = false;
     bool after_local_ = false;
+    // Deficits (negative excess off s) may exist: set by RM repairs and warm
+    // starts (capacity decreases), cleared only by a full count that sees
+    // none.  Sparse counts do not see deficits, so they never clear it.
+    bool deficits_ = false;
+    // a multi-workgroup BFS barrier gave up (the device was too busy to hold
+    // all its workgroups): this query's BFS runs single-workgroup from here on
+    bool no_multi_ = false;
+    // Restart the BFS under way after a barrier time-out.
+    Plan restart_bfs(const Ctr& c);
 };
 
 // ------------------------------------------------------------------ engine
@@ -392,6 +417,11 @@ public:
     int dbg_begin(int gid, int s, int t);
     int dbg_plan(const Plan& p);
     void dbg_end();
+    // The class-path pruning mask of (s, t) (fwd(scc(s)) AND wcc == wcc(t),
+    // built by the same calls as class_decide); slot 0 must hold a BFS_FWD
+    // from s in h unless s's class is cached.  reachable = 0: t is not
+    // reachable from s (the mask is not built).
+    int dbg_class_mask(int gid, int s, int t, int32_t* mask, int64_t* kept, int* reachable);
     const Ctr& ctr(int i) const;
     SlotView& view(int i);
     DeviceGraph* graph(int gid) { return graphs_[gid].get(); }
@@ -400,6 +430,11 @@ public:
     int pin(int slot, const Query& q);
     int solve_pinned(bool warm, std::vector<Result>& out);
     int update_caps(int gid, const int32_t* entries, const int64_t* new_cap, int n);
+    // One query on its full graph (no pruning, no captured graphs), its slab
+    // carved from caller memory [base, base + cap_bytes) instead of the
+    // arena: no device allocation on this path.  The engine must be idle.
+    int solve_in(const Query& q, char* base, size_t cap_bytes, std::vector<Result>& out);
+    int device() const { return device_; }
 
 private:
     struct SlotState {
@@ -425,10 +460,16 @@ private:
     int make_arena();
     int prune_decide(int slot);
     int class_decide(int slot);
+    // The forward set of SCC cls: cached, or made from d_h holding a BFS_FWD
+    // from a vertex of cls (then cached).
+    int fwd_set(DeviceGraph& G, int cls, const int32_t* d_h, std::shared_ptr<DeviceGraph::FwdSet>& out);
+    static int64_t class_kept(const DeviceGraph& G, const DeviceGraph::FwdSet& fwd, int wt);
     int launch_plans(std::vector<Plan>& plans);
 
     Params P_;
     int nslots_ = 0;
+    int device_ = 0;            // HIP device current at init
+    bool counted_ = false;      // counted in the live-engine total
     long long spin_ticks_ = 0;  // multi-workgroup BFS barrier wait limit
     hipStream_t stream_ = nullptr;
     hipStream_t side_ = nullptr;          // single-workgroup BFS steps, overlapped with the rest
@@ -469,6 +510,8 @@ private:
     struct CacheEntry {
         std::shared_ptr<DeviceGraph> g;
         uint64_t last_use = 0;
+        int64_t kept = 0;  // vertices of the pruning mask
+        uint64_t version = 0;  // the parent's capacity version the copy was built at
     };
     std::unordered_map<std::string, CacheEntry> cache_;
     uint64_t use_clock_ = 0;

@@ -352,7 +352,9 @@ void launch_reset(hipStream_t st, const SlotView* slots, const int* list, int nl
 }
 
 // -------------------------------------------------------- residual init (a10)
-// BC:151-170: res = cap, excess = 0.  Edge-parallel copy; locks freed.
+// BC:151-170: res = cap, excess = 0.  Edge-parallel copy; locks freed and
+// touch stamps cleared (the slab's memory may have served another slot,
+// whose touch epochs differ from this one's).
 
 __global__ __launch_bounds__(256) void k_init(const SlotView* __restrict__ slots,
                                               const int* __restrict__ list) {
@@ -367,6 +369,7 @@ __global__ __launch_bounds__(256) void k_init(const SlotView* __restrict__ slots
         S.excess[v] = 0;
         S.h[v] = 0;
         S.lock[v] = -1;
+        S.aux[v] = 0;  // no touch stamp of an earlier owner of this memory survives
     }
     if (i0 == 0) {
         S.mlist[0] = 0;  // the marks are all cleared above
@@ -430,6 +433,7 @@ __global__ __launch_bounds__(256) void k_init_marked(const SlotView* __restrict_
         S.excess[v] = 0;
         S.h[v] = 0;
         S.lock[v] = -1;
+        S.aux[v] = 0;
     }
     if (i0 == 0) wk.verts += (uint64_t)S.g.n;
     work_flush(S, wk, red, WF_INIT);
@@ -1052,10 +1056,12 @@ __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__
 // moves; the last arrival resets the count before it moves the generation,
 // so no workgroup can arrive at the next barrier early.  The wait is bounded:
 // past spin_ticks of the wall clock it gives up, flags the slot's error word
-// (bit 4) and returns false.  All workgroups of the launch fit on the device
-// at once, also beside such a launch on the other stream (at most 512 of 256
-// threads each), and no other kernel waits on them, so a missing workgroup
-// is at worst still being dispatched.
+// (bit 4) and returns false; the scheduler then runs that BFS again with a
+// single workgroup.  The multi-workgroup launches of all engines of the
+// process together hold at most 1024 workgroups of 256 threads (each launch
+// at most its share, launch_bfs_small's cap_total), half of what the device
+// holds, and no other kernel waits on them, so a missing workgroup is at
+// worst still being dispatched.
 __device__ __forceinline__ bool slot_barrier(GAS Ctr* c, uint32_t& gen, long long spin_ticks, int* ok_s) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
@@ -1175,17 +1181,18 @@ __global__ __launch_bounds__(256) void k_bfs_multi(const SlotView* __restrict__ 
 }
 
 void launch_bfs_small(hipStream_t st, const SlotView* slots, const int* list, int nlist, int mode,
-                      int max_levels, long long max_edges, int wgs, long long spin_ticks) {
+                      int max_levels, long long max_edges, int wgs, long long spin_ticks, int cap_total) {
     if (wgs <= 1) {
         if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_small<BFS_GR>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels, max_edges);
         else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_small<BFS_FWD>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels, max_edges);
         else hipLaunchKernelGGL(k_bfs_small<BFS_UND>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels, max_edges);
         return;
     }
-    // at most 512 workgroups per launch: this launch and one on the other
-    // stream fit on the device at once
+    // at most cap_total workgroups per launch (at most 512): every multi-
+    // workgroup launch of every engine fits on the device at once
+    const long long cap = std::min(512, std::max(1, cap_total));
     int g = wgs;
-    while (g > 1 && (long long)g * nlist > 512) g >>= 1;
+    while (g > 1 && (long long)g * nlist > cap) g >>= 1;
     if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_multi<BFS_GR>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, max_edges, spin_ticks);
     else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_multi<BFS_FWD>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, max_edges, spin_ticks);
     else hipLaunchKernelGGL(k_bfs_multi<BFS_UND>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, max_edges, spin_ticks);

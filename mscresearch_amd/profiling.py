@@ -5,8 +5,8 @@ Inputs: the counter_collection.csv files of four ``rocprofv3 --pmc`` runs
 ``bench.py --calibrate-copy BYTES``) and the bench's JSON line from the
 counter runs.  Each counter is scaled by known bytes / counted bytes of the
 calibration copy, summed over the kernels of a family and divided by that
-family's launches in the same run (the bench run profiled with its whole GPU
-work inside the timed region: --warmup 0 --plain 0).
+family's launches in the same run (the bench run with --profile-all, so the family
+launch counts cover every pass the counters saw, warm-up included).
 
 Output: a JSON file per workload that bench.py reads for roofline.traffic.
 """
@@ -97,8 +97,10 @@ def summarize(fetch_csv, write_csv, cfetch_csv, cwrite_csv, copy_bytes, bench_js
     return out
 
 
-def traffic_for(workload: str, family: str, root: str | None = None):
-    """Measured HBM bytes per launch of `family` for `workload`, or None."""
+def traffic_for(workload: str, family: str, root: str | None = None, full: bool = False):
+    """Measured HBM bytes per launch of `family` for `workload` (with
+    full=True the family's whole record of the counter runs: HBM and
+    algorithmic bytes per launch, launches), or None."""
     root = root or os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "profiles")
     path = os.path.join(root, f"{workload}_hbm_traffic.json")
     if not os.path.exists(path):
@@ -106,4 +108,6 @@ def traffic_for(workload: str, family: str, root: str | None = None):
     with open(path) as f:
         d = json.load(f)
     fam = d.get("families", {}).get(family)
-    return None if fam is None else fam["hbm_bytes_per_launch"]
+    if fam is None:
+        return None
+    return fam if full else fam["hbm_bytes_per_launch"]

@@ -40,6 +40,7 @@ def lib():
         L.orc_parse_queries.argtypes = [C.c_char_p, C.c_int64, P, P]
         L.orc_parse_queries.restype = C.c_int64
         L.orc_build_csr.argtypes = [C.c_int, C.c_int64, P, P, P, P, P, P, P, P, P, P, P]
+        L.orc_line_entries.argtypes = [C.c_int, C.c_int64, P, P, P, P]
         L.orc_csr_stats.argtypes = [C.c_int, P, P, P, P, P, P, P]
         L.orc_partition.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P]
         L.orc_reorder.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P, P, P, P, P]
@@ -57,6 +58,8 @@ def lib():
         L.orc_certify.restype = C.c_int
         L.orc_dinic_many.argtypes = [C.c_int, P, P, P, P, C.c_int, P, P, C.c_int, P]
         L.orc_dinic_many.restype = C.c_double
+        L.orc_dinic_many_ex.argtypes = [C.c_int, P, P, P, P, C.c_int, P, P, C.c_int, C.c_int, P]
+        L.orc_dinic_many_ex.restype = C.c_double
         _lib = L
     return _lib
 
@@ -123,6 +126,14 @@ def build_csr(n, us, vs, caps) -> Graph:
     lib().orc_build_csr(n, cnt, _p(us), _p(vs), _p(caps), _p(rp), _p(col), _p(cap), _p(rev), _p(so), _p(si),
                         _p(do), _p(di))
     return Graph(n, rp, col, cap, rev, stats=(so, si, do, di))
+
+
+def line_entries(g: Graph, us, vs):
+    """Forward entry of every accepted line in build_csr's layout."""
+    us, vs = _c(us, np.int32), _c(vs, np.int32)
+    out = np.empty(len(us), np.int32)
+    lib().orc_line_entries(g.n, len(us), _p(us), _p(vs), _p(g.row_ptr), _p(out))
+    return out
 
 
 def csr_stats(g: Graph):
@@ -216,10 +227,11 @@ def certify(g: Graph, res, s, t, flow, excess=None) -> int:
                              _p(None if excess is None else _c(excess, np.int64)), int(s), int(t), int(flow))
 
 
-def maxflow_many(g: Graph, s, t, threads):
-    """Dinic over many queries, one query per thread.  Returns (flows, seconds)."""
+def maxflow_many(g: Graph, s, t, threads, prune=False):
+    """Dinic over many queries, one query per thread; prune=True restricts each
+    query to its pruning mask (BC:1391-1445) first.  Returns (flows, seconds)."""
     s, t = _c(s, np.int32), _c(t, np.int32)
     flows = np.empty(len(s), np.int64)
-    sec = lib().orc_dinic_many(g.n, _p(g.row_ptr), _p(g.col), _p(g.cap), _p(g.rev), len(s), _p(s), _p(t),
-                               int(threads), _p(flows))
+    sec = lib().orc_dinic_many_ex(g.n, _p(g.row_ptr), _p(g.col), _p(g.cap), _p(g.rev), len(s), _p(s), _p(t),
+                                  int(threads), 1 if prune else 0, _p(flows))
     return flows, sec

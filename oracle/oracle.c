@@ -97,6 +97,20 @@ void orc_build_csr(int n, int64_t cnt, const int32_t* us, const int32_t* vs, con
     free(deg);
 }
 
+/* Forward entry of each accepted line in the CSR orc_build_csr makes: line i
+ * takes the next free place of row us[i] (BC:1078-1125), so replaying the
+ * row cursors gives its entry. */
+void orc_line_entries(int n, int64_t cnt, const int32_t* us, const int32_t* vs, const int32_t* row_ptr,
+                      int32_t* entry) {
+    int32_t* cur = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    for (int v = 0; v < n; ++v) cur[v] = row_ptr[v];
+    for (int64_t i = 0; i < cnt; ++i) {
+        entry[i] = cur[us[i]]++;
+        cur[vs[i]]++;
+    }
+    free(cur);
+}
+
 /* BC:2707-2720: stats over CSR entries with cap>0. */
 void orc_csr_stats(int n, const int32_t* row_ptr, const int32_t* col, const int64_t* cap,
                    int64_t* sum_out, int64_t* sum_in, int32_t* deg_out, int32_t* deg_in) {
@@ -391,11 +405,21 @@ typedef struct {
     const int32_t *row_ptr, *col, *rev;
     int64_t* res;
     int32_t *level, *it, *q, *path;
+    /* optional: vertices outside the mask are never entered (level -2) */
+    const uint8_t* mask;
+    /* optional: entries whose residual changed, so that a caller holding
+     * res == cap between queries restores only those (touched[e] = 1) */
+    uint8_t* touched;
+    int32_t* tlist;
+    int64_t ntouched;
 } dinic_ws;
 
 static int dinic_bfs(dinic_ws* w, int s, int t) {
     int n = w->n;
-    for (int v = 0; v < n; ++v) w->level[v] = -1;
+    if (w->mask)
+        for (int v = 0; v < n; ++v) w->level[v] = w->mask[v] ? -1 : -2;
+    else
+        for (int v = 0; v < n; ++v) w->level[v] = -1;
     int qh = 0, qt = 0;
     w->level[s] = 0; w->q[qt++] = s;
     int tl = -1;
@@ -404,7 +428,7 @@ static int dinic_bfs(dinic_ws* w, int s, int t) {
         if (tl >= 0 && w->level[v] >= tl) break;
         for (int e = w->row_ptr[v]; e < w->row_ptr[v + 1]; ++e) {
             int u = w->col[e];
-            if (w->res[e] > 0 && w->level[u] < 0) {
+            if (w->res[e] > 0 && w->level[u] == -1) {
                 w->level[u] = w->level[v] + 1;
                 if (u == t) tl = w->level[u];
                 w->q[qt++] = u;
@@ -428,6 +452,10 @@ static int64_t dinic_run(dinic_ws* w, int s, int t) {
                     int e = w->path[i];
                     w->res[e] -= b;
                     w->res[w->rev[e]] += b;
+                    if (w->touched && !w->touched[e]) {
+                        w->touched[e] = 1;
+                        w->tlist[w->ntouched++] = e;
+                    }
                 }
                 total += b;
                 int k = 0;
@@ -459,6 +487,7 @@ int64_t orc_dinic(int n, const int32_t* row_ptr, const int32_t* col, const int64
     if (s < 0 || t < 0 || s >= n || t >= n || s == t) return 0;
     int64_t E = row_ptr[n];
     dinic_ws w;
+    memset(&w, 0, sizeof(w));
     w.n = n; w.row_ptr = row_ptr; w.col = col; w.rev = rev;
     w.res = res_out ? res_out : (int64_t*)malloc(sizeof(int64_t) * (size_t)(E > 0 ? E : 1));
     for (int64_t e = 0; e < E; ++e) w.res[e] = cap[e];
@@ -525,37 +554,87 @@ typedef struct {
     int nq;
     const int32_t *s, *t;
     int64_t* flows;
+    int prune;
     atomic_int next;
 } many_ctx;
 
+/* The pruning mask of BC:1391-1445 (fwd(s) over cap > 0, AND the undirected
+ * component of t, plus s and t) into mask8, using q as the BFS queue.
+ * Returns 0 when t is not reachable from s over cap > 0 (the flow is 0). */
+static int prune_mask8(int n, const int32_t* row_ptr, const int32_t* col, const int64_t* cap, int s, int t,
+                       uint8_t* fwd, uint8_t* mask8, int32_t* q) {
+    memset(fwd, 0, (size_t)n);
+    memset(mask8, 0, (size_t)n);
+    int qh = 0, qt = 0;
+    fwd[s] = 1; q[qt++] = s;
+    while (qh < qt) {
+        int v = q[qh++];
+        for (int e = row_ptr[v]; e < row_ptr[v + 1]; ++e)
+            if (cap[e] > 0 && !fwd[col[e]]) { fwd[col[e]] = 1; q[qt++] = col[e]; }
+    }
+    if (!fwd[t]) return 0;
+    qh = qt = 0;
+    mask8[t] = 1; q[qt++] = t;
+    while (qh < qt) {
+        int v = q[qh++];
+        for (int e = row_ptr[v]; e < row_ptr[v + 1]; ++e)
+            if (!mask8[col[e]]) { mask8[col[e]] = 1; q[qt++] = col[e]; }
+    }
+    for (int v = 0; v < n; ++v) mask8[v] = (uint8_t)(mask8[v] & fwd[v]);
+    mask8[s] = mask8[t] = 1;
+    return 1;
+}
+
+/* One thread: the residual is copied from cap once and, after each query,
+ * only the entries Dinic changed are restored. */
 static void* many_worker(void* arg) {
     many_ctx* c = (many_ctx*)arg;
     int64_t E = c->row_ptr[c->n];
+    const size_t ee = (size_t)(E > 0 ? E : 1), nn = (size_t)(c->n > 0 ? c->n : 1);
     dinic_ws w;
+    memset(&w, 0, sizeof(w));
     w.n = c->n; w.row_ptr = c->row_ptr; w.col = c->col; w.rev = c->rev;
-    w.res = (int64_t*)malloc(sizeof(int64_t) * (size_t)(E > 0 ? E : 1));
-    w.level = (int32_t*)malloc(sizeof(int32_t) * (size_t)c->n);
-    w.it = (int32_t*)malloc(sizeof(int32_t) * (size_t)c->n);
-    w.q = (int32_t*)malloc(sizeof(int32_t) * (size_t)c->n);
-    w.path = (int32_t*)malloc(sizeof(int32_t) * ((size_t)c->n + 1));
+    w.res = (int64_t*)malloc(sizeof(int64_t) * ee);
+    w.level = (int32_t*)malloc(sizeof(int32_t) * nn);
+    w.it = (int32_t*)malloc(sizeof(int32_t) * nn);
+    w.q = (int32_t*)malloc(sizeof(int32_t) * nn);
+    w.path = (int32_t*)malloc(sizeof(int32_t) * (nn + 1));
+    w.touched = (uint8_t*)calloc(ee, 1);
+    w.tlist = (int32_t*)malloc(sizeof(int32_t) * ee);
+    uint8_t* fwd = c->prune ? (uint8_t*)malloc(nn) : NULL;
+    uint8_t* mask8 = c->prune ? (uint8_t*)malloc(nn) : NULL;
+    memcpy(w.res, c->cap, sizeof(int64_t) * (size_t)E);
     for (;;) {
         int q = atomic_fetch_add(&c->next, 1);
         if (q >= c->nq) break;
         int s = c->s[q], t = c->t[q];
         if (s < 0 || t < 0 || s >= c->n || t >= c->n || s == t) { c->flows[q] = 0; continue; }
-        for (int64_t e = 0; e < E; ++e) w.res[e] = c->cap[e];
+        w.mask = NULL;
+        if (c->prune) {
+            if (!prune_mask8(c->n, c->row_ptr, c->col, c->cap, s, t, fwd, mask8, w.q)) { c->flows[q] = 0; continue; }
+            w.mask = mask8;
+        }
+        w.ntouched = 0;
         c->flows[q] = dinic_run(&w, s, t);
+        for (int64_t k = 0; k < w.ntouched; ++k) {
+            const int e = w.tlist[k];
+            w.res[e] = c->cap[e];
+            w.res[c->rev[e]] = c->cap[c->rev[e]];
+            w.touched[e] = 0;
+        }
     }
+    free(mask8); free(fwd);
+    free(w.tlist); free(w.touched);
     free(w.path); free(w.q); free(w.it); free(w.level); free(w.res);
     return NULL;
 }
 
-double orc_dinic_many(int n, const int32_t* row_ptr, const int32_t* col, const int64_t* cap,
-                      const int32_t* rev, int nq, const int32_t* s, const int32_t* t,
-                      int threads, int64_t* flows) {
+double orc_dinic_many_ex(int n, const int32_t* row_ptr, const int32_t* col, const int64_t* cap,
+                         const int32_t* rev, int nq, const int32_t* s, const int32_t* t,
+                         int threads, int prune, int64_t* flows) {
     many_ctx c;
     c.n = n; c.row_ptr = row_ptr; c.col = col; c.rev = rev; c.cap = cap;
-    c.nq = nq; c.s = s; c.t = t; c.flows = flows;
+    c.nq = nq; c.s = s; c.t = t; c.flows = flows; c.prune = prune;
     atomic_init(&c.next, 0);
     if (threads < 1) threads = 1;
     struct timespec a, b;
@@ -566,4 +645,10 @@ double orc_dinic_many(int n, const int32_t* row_ptr, const int32_t* col, const i
     clock_gettime(CLOCK_MONOTONIC, &b);
     free(th);
     return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
+}
+
+double orc_dinic_many(int n, const int32_t* row_ptr, const int32_t* col, const int64_t* cap,
+                      const int32_t* rev, int nq, const int32_t* s, const int32_t* t,
+                      int threads, int64_t* flows) {
+    return orc_dinic_many_ex(n, row_ptr, col, cap, rev, nq, s, t, threads, 0, flows);
 }

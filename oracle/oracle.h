@@ -50,6 +50,11 @@ void orc_build_csr(int n, int64_t cnt, const int32_t* us, const int32_t* vs, con
                    int32_t* row_ptr, int32_t* col, int64_t* cap, int32_t* rev,
                    int64_t* sum_out, int64_t* sum_in, int32_t* deg_out, int32_t* deg_in);
 
+/* Forward CSR entry (of orc_build_csr's layout, row_ptr given) of every
+ * accepted line, in line order. */
+void orc_line_entries(int n, int64_t cnt, const int32_t* us, const int32_t* vs, const int32_t* row_ptr,
+                      int32_t* entry);
+
 /* Per-vertex stats recomputed from a CSR, counting entries with cap>0
  * (the per-query variant, BC:2707-2720). */
 void orc_csr_stats(int n, const int32_t* row_ptr, const int32_t* col, const int64_t* cap,
@@ -121,6 +126,12 @@ int orc_certify(int n, const int32_t* row_ptr, const int32_t* col, const int64_t
 double orc_dinic_many(int n, const int32_t* row_ptr, const int32_t* col, const int64_t* cap,
                       const int32_t* rev, int nq, const int32_t* s, const int32_t* t,
                       int threads, int64_t* flows);
+/* The same; prune != 0 first restricts each query to the pruning mask of
+ * orc_prune_mask (BC:1391-1445), answering 0 when t is not reachable from s.
+ * Each thread restores only the residual entries its last query changed. */
+double orc_dinic_many_ex(int n, const int32_t* row_ptr, const int32_t* col, const int64_t* cap,
+                         const int32_t* rev, int nq, const int32_t* s, const int32_t* t,
+                         int threads, int prune, int64_t* flows);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,7 @@ import pytest
 
 import mscresearch_amd as M
 import oracle as O
-from gpu_util import log
+from gpu_util import Case, engine_with, log
 
 pytestmark = pytest.mark.gpu
 
@@ -285,3 +285,59 @@ def test_engines_sharing_a_graph_see_capacity_updates():
             r = b.solve(gb, s, t)
             assert r.flow == O.maxflow(og, s, t), (rnd, s, t)
             assert r.cert_ok == 1
+
+
+def test_update_caps_rejects_reverse_and_repeated_entries():
+    """A capacity update may only name forward entries (cap > 0), each once;
+    a bad batch is refused whole and leaves every capacity as it was."""
+    case = Case(M.Edges.rmat(11, 10**6, 7101), R=16)
+    og = case.og
+    eng, gid = engine_with(case)
+    fwd = np.nonzero(og.cap > 0)[0]
+    rev = np.nonzero(og.cap == 0)[0]
+    s_, t_ = case.edges.hub_queries(2, 7102)
+    pairs = [case.q(a, b) for a, b in zip(s_, t_)]
+    before = [eng.solve(gid, s, t).flow for s, t in pairs]
+    for bad in (np.array([fwd[0], rev[0]], np.int32), np.array([fwd[1], fwd[2], fwd[1]], np.int32)):
+        with pytest.raises(M.MaxFlowError) as ei:
+            eng.update_caps(gid, bad, np.full(len(bad), 7, np.int64))
+        assert ei.value.code == M.MF_EINVAL
+    # nothing changed: the same flows, equal to the oracle's on the old capacities
+    for (s, t), f in zip(pairs, before):
+        r = eng.solve(gid, s, t)
+        assert r.flow == f == O.maxflow(og, s, t)
+    # a valid batch still applies
+    pick = fwd[:50]
+    eng.update_caps(gid, pick, np.full(50, 3, np.int64))
+    cap2 = og.cap.copy()
+    cap2[pick] = 3
+    og2 = O.Graph(og.n, og.row_ptr, og.col, cap2, og.rev)
+    for s, t in pairs:
+        assert eng.solve(gid, s, t).flow == O.maxflow(og2, s, t)
+
+
+def test_pruned_subgraphs_of_another_engine_follow_capacity_updates():
+    """Engine C prunes (R-MAT: the mask drops the isolated vertices) and caches
+    induced subgraphs holding copies of the capacities; engine A updates the
+    shared graph.  C's next solves must not run on its stale copies."""
+    case = Case(M.Edges.rmat(12, 10**6, 7201), R=32)
+    og = case.og
+    cap = og.cap.copy()
+    a, ga = engine_with(case)
+    c, gc = engine_with(case)
+    s_, t_ = case.edges.hub_queries(6, 7202)
+    pairs = [case.q(x, y) for x, y in zip(s_, t_)]
+    for s, t in pairs:
+        r = c.solve(gc, s, t)
+        assert r.flow == O.maxflow(og, s, t)
+    assert any(c.solve(gc, s, t).pruned for s, t in pairs)
+    rng = np.random.default_rng(7203)
+    for rnd in range(2):
+        ent, new = updated_round(rng, cap, np.nonzero(cap > 0)[0], 0.3)
+        a.update_caps(ga, ent, new)
+        cap[ent] = new
+        og2 = O.Graph(og.n, og.row_ptr, og.col, cap, og.rev)
+        for s, t in pairs:
+            r = c.solve(gc, s, t)
+            assert r.cert_ok == 1
+            assert r.flow == O.maxflow(og2, s, t), (rnd, s, t)

@@ -92,6 +92,63 @@ def test_prune_mask_bit_exact(which, request):
         assert kept == int(om.sum())
 
 
+def unreachable_pairs(case, k, seed):
+    """(s, t) in region-ordered ids with t not reachable from s over cap > 0:
+    s without a cap > 0 out-entry, or t without a cap > 0 in-entry."""
+    og = case.og
+    owner = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
+    fwd = og.cap > 0
+    outdeg = np.bincount(owner[fwd], minlength=og.n)
+    indeg = np.bincount(og.col[fwd], minlength=og.n)
+    rng = np.random.default_rng(seed)
+    pairs = []
+    sinks = np.nonzero((outdeg == 0) & (indeg > 0))[0]
+    sources = np.nonzero((indeg == 0) & (outdeg > 0))[0]
+    hub = int(np.argmax(outdeg))
+    for v in rng.permutation(sinks)[:k]:
+        pairs.append((int(v), hub if hub != v else int((v + 1) % og.n)))
+    for v in rng.permutation(sources)[:k]:
+        pairs.append((hub if hub != v else int((v + 1) % og.n), int(v)))
+    return pairs
+
+
+@pytest.mark.parametrize("which", ["c1", "rmat12", "grid"])
+def test_class_mask_bit_exact(which, request):
+    """The mask the product prunes with (the forward set of s's SCC AND the
+    WCC of t) equals the reference mask of BC:1391-1445 whenever t is
+    reachable from s; when it is not, the product answers 0 without a mask,
+    and the oracle's flow is 0.  Each pair runs twice: the second call finds
+    s's forward set in the graph's cache.  A solve through the scheduler
+    reports the same kept count."""
+    case = request.getfixturevalue(which)
+    e, gid = engine_with(case)
+    if which == "rmat12":
+        s_, t_ = case.edges.hub_queries(6, 1205)
+    elif which == "grid":
+        s_, t_ = M.grid_queries(48, 64, 4, 1305)
+    else:
+        s_, t_ = M.uniform_queries(case.n, 4, 1005)
+    pairs = [case.q(a, b) for a, b in zip(s_, t_)] + unreachable_pairs(case, 3, 1206)
+    n_unreach = 0
+    for s, t in pairs:
+        om = O.prune_mask(case.og, s, t)
+        f = O.maxflow(case.og, s, t)
+        for rep in range(2):
+            log(f"class mask {which} s={s} t={t} pass {rep}")
+            mask, kept, reach = e.dbg_class_mask(gid, s, t, case.n)
+            assert reach == (f > 0), (s, t, f)
+            if reach:
+                assert np.array_equal(mask, om)
+                assert kept == int(om.sum())
+        n_unreach += 0 if f > 0 else 1
+        r = e.solve(gid, s, t)
+        assert r.flow == f and r.cert_ok == 1
+        if f > 0:
+            assert r.kept == int(om.sum())
+    if which == "rmat12":
+        assert n_unreach > 0  # the unreachable branch was exercised
+
+
 def test_induced_subgraph_bit_exact(rmat12):
     e, gid = engine_with(rmat12)
     s_, t_ = rmat12.edges.hub_queries(3, 1204)

@@ -112,3 +112,43 @@ def test_serve_lanes_one_engine_per_graph(graphs):
     for e in engines:
         st = e.mem_stats()
         assert st["peak"] <= st["budget"]
+
+
+def test_lanes_share_one_graph_across_many_classes():
+    """Three engines, each on its own host thread, share one freshly uploaded
+    graph, so its per-SCC forward-set cache starts empty; their first queries
+    come from many SCC classes at once, and the engines build and insert
+    forward sets concurrently.  Every flow equals the oracle's."""
+    case = Case(M.Edges.rmat(13, 10**6, 6301), R=32)
+    scc, _wcc = case.hg.components()
+    n, E, _R, _l = case.hg.info()
+    og = case.og
+    owner = np.repeat(np.arange(n), np.diff(og.row_ptr))
+    outdeg = np.bincount(owner[og.cap > 0], minlength=n)
+    indeg = np.bincount(og.col[og.cap > 0], minlength=n)
+    rng = np.random.default_rng(6302)
+    # one source per distinct SCC among vertices with out-edges, up to 72
+    cand = rng.permutation(np.nonzero(outdeg > 0)[0])
+    seen, srcs = set(), []
+    for v in cand:
+        if scc[v] not in seen:
+            seen.add(scc[v])
+            srcs.append(int(v))
+        if len(srcs) == 72:
+            break
+    tgt = np.nonzero(indeg > 0)[0]
+    s = np.array(srcs, np.int32)
+    t = rng.choice(tgt, len(s)).astype(np.int32)
+    t = np.where(t == s, tgt[0] if tgt[0] != s[0] else tgt[1], t).astype(np.int32)
+    log(f"shared graph: {len(s)} queries from {len(set(scc[s].tolist()))} SCC classes over 3 lanes")
+    dg = M.DeviceGraph(case.hg)
+    engines = [M.Engine(6) for _ in range(3)]
+    gids = [e.add_graph(dg) for e in engines]
+    assert gids == [0, 0, 0]
+    lane = (np.arange(len(s)) % 3).astype(np.int32)
+    res = M.serve_lanes(engines, lane, np.zeros(len(s), np.int32), s, t, np.zeros(len(s)))
+    assert len(res) == len(s)
+    flows, _sec = O.maxflow_many(og, s, t, threads=4)
+    for r in res:
+        assert r.status == 0 and r.cert_ok == 1, (r.tag, r.status)
+        assert r.flow == flows[r.tag], (r.tag, r.flow, flows[r.tag])

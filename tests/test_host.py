@@ -222,3 +222,61 @@ def test_class_mask_equals_prune_mask():
             continue
         cls = fwd & (wcc == wcc[t])
         assert np.array_equal(cls.astype(np.int32), mask), (s, t)
+
+
+@pytest.mark.parametrize("gen", ["uniform", "rmat", "grid"])
+def test_line_entries_after_partition(gen):
+    """mf_hgraph_line_entries gives, in the region-ordered CSR, the forward
+    entry of every input line: the oracle's entry of the same line (its CSR
+    in input order) mapped through the relabel."""
+    if gen == "uniform":
+        e = M.Edges.uniform(500, 4000, 60, 91)   # repeated (u, v) pairs included
+    elif gen == "rmat":
+        e = M.Edges.rmat(10, 10**6, 92)
+    else:
+        e = M.Edges.grid(12, 17, 1000, 93)
+    n, us, vs, cs = e.arrays()
+    hg = M.HostGraph.from_edgelist(e).partition(16)
+    d = hg.export()
+    ent = hg.line_entries(us, vs)
+    nid = d["new_id"]
+    owner = np.repeat(np.arange(n), np.diff(d["row_ptr"]))
+    assert np.array_equal(owner[ent], nid[us])
+    assert np.array_equal(d["col"][ent], nid[vs])
+    assert np.array_equal(d["cap"][ent], cs)
+    assert len(np.unique(ent)) == len(ent)
+    # same rank within the row as the oracle's entry in input order
+    og = O.build_csr(n, us, vs, cs)
+    oent = O.line_entries(og, us, vs)
+    assert np.array_equal(ent - d["row_ptr"][nid[us]], oent - og.row_ptr[us])
+    with pytest.raises(M.MaxFlowError):
+        hg.line_entries(us[::-1].copy(), vs[::-1].copy())  # not the graph's line order
+
+
+def test_capacity_update_rounds_shape_and_determinism():
+    from mscresearch_amd import workloads as W
+
+    caps = np.random.default_rng(3).integers(1, 10**6, 5000)
+    a = list(W.capacity_update_rounds(caps, 4, 77, 0.01))
+    b = list(W.capacity_update_rounds(caps, 4, 77, 0.01))
+    cur = caps.copy()
+    for (la, na), (lb, nb) in zip(a, b):
+        assert np.array_equal(la, lb) and np.array_equal(na, nb)
+        assert len(la) == 50 and len(np.unique(la)) == 50 and np.all(np.diff(la) > 0)
+        assert np.all(na >= 1)
+        ratio = na / cur[la]
+        assert np.all(ratio >= 0.5 - 0.5 / cur[la]) and np.all(ratio <= 2.0 + 0.5 / cur[la])
+        cur[la] = na
+
+
+def test_edge_hash_sees_any_change():
+    from mscresearch_amd import workloads as W
+
+    us = np.array([0, 1, 2], np.int32)
+    vs = np.array([1, 2, 0], np.int32)
+    cs = np.array([5, 6, 7], np.int64)
+    h = W.edge_hash(us, vs, cs)
+    assert h == W.edge_hash(us.copy(), vs.copy(), cs.copy())
+    assert h != W.edge_hash(us, vs, np.array([5, 6, 8]))
+    assert h != W.edge_hash(vs, us, cs)
+    assert h != W.edge_hash(us[[1, 0, 2]], vs[[1, 0, 2]], cs[[1, 0, 2]])
