@@ -216,6 +216,20 @@ typedef struct mf_result {
  * pruning (BC:2666-2698): the int64 flow is exact and certified. */
 int mf_solve(mf_engine* e, int gid, int s, int t, mf_result* r);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture elsewhere in the process, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same plain solve as
+ * mf_solve, on the full graph (no pruning, no captured cycles), with the
+ * query's slab carved from a static per-device array of
+ * MF_NOALLOC_SCRATCH_BYTES instead of the engine's arena; the call makes no
+ * device allocation.  The graph's slab (mf_graph_slab_bytes) must fit, else
+ * MF_ENOMEM and nothing runs.  The engine must be idle (nothing queued,
+ * live or pinned).  Calls on one GPU share its array, which keeps the last
+ * call's contents (everything a solve reads is reset by the solve itself);
+ * they would race, so the library runs them one at a time per device. */
+#define MF_NOALLOC_SCRATCH_BYTES (256ll << 20)
+int mf_solve_noalloc(mf_engine* e, int gid, int s, int t, mf_result* r);
+
 /* Batched / served use: queue queries, then drive the engine.  Freed slots
  * are refilled from the queue every step (continuous batching). */
 int mf_engine_submit(mf_engine* e, int gid, int s, int t, int64_t tag, double t_arrive);

@@ -23,6 +23,7 @@ CLI_PATH = os.path.join(_PKG, "bin", "mscmf")
 
 MF_OK, MF_EINVAL, MF_ENOMEM, MF_EHIP, MF_EIO, MF_EITER, MF_ECERT, MF_ENODEV, MF_EBUSY, MF_ESTALE = (
     0, -1, -2, -3, -4, -5, -6, -7, -8, -9)
+NOALLOC_SCRATCH_BYTES = 256 << 20  # MF_NOALLOC_SCRATCH_BYTES
 
 _lib = None
 
@@ -133,6 +134,7 @@ SIGNATURES = {
     "mf_engine_add_graph": (C.c_int, [_P, _P, C.POINTER(C.c_int)]),
     "mf_engine_free": (None, [_P]),
     "mf_solve": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(Result)]),
+    "mf_solve_noalloc": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(Result)]),
     "mf_engine_submit": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_double]),
     "mf_engine_run": (C.c_int, [_P, C.c_int64, C.POINTER(Result), C.c_int, C.POINTER(C.c_int)]),
     "mf_engine_live": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
@@ -475,6 +477,13 @@ class Engine:
         r = Result()
         rc = lib().mf_solve(self._h, gid, int(s), int(t), C.byref(r))
         _check(rc)
+        return r
+
+    def solve_noalloc(self, gid, s, t) -> Result:
+        """mf_solve without device allocation (slab in the per-device static
+        scratch array; full graph)."""
+        r = Result()
+        _check(lib().mf_solve_noalloc(self._h, gid, int(s), int(t), C.byref(r)))
         return r
 
     def submit(self, gid, s, t, tag, t_arrive=0.0):

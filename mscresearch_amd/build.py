@@ -26,7 +26,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + 
 # Packed FP32 instructions are kept out of every device build.
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["host_graph.cpp", "mf_kernels.hip", "mf_induced.hip", "mf_engine.cpp", "mf_tune.cpp", "mf_capi.cpp"]
+LIB_SOURCES = ["host_graph.cpp", "mf_kernels.hip", "mf_induced.hip", "mf_engine.cpp", "mf_tune.cpp", "mf_capi.cpp",
+               "mf_noalloc.hip"]
 CLI_SOURCES = ["mf_cli.cpp"]
 
 LIB = os.path.join(PKG, "libmscmf.so")
