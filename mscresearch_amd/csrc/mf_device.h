@@ -33,6 +33,12 @@ struct DGraph {
     // pair's residuals sum to it, so res[rev e] = pcap[e] - res[e] is read
     // along e's row instead of gathered from rev e's
     int64_t* pcap = nullptr;
+    // fstart[v]: the first forward entry (cap > 0) of v's row, or its row
+    // start when it has none.  Bottom-up relabel levels scan a row from here
+    // and wrap: a parent is reached over residual out-entries, and rows built
+    // from (u, v)-sorted input lines open with the reverse entries of lower
+    // ids, which carry no residual until flow crosses them.
+    const int32_t* fstart = nullptr;
 };
 
 // Offset (in int32 words, 16-byte aligned) of the giant list's task records.

@@ -69,6 +69,7 @@ DeviceGraph::~DeviceGraph() {
     (void)hipFree(d_sub2old); (void)hipFree(d_old2sub); (void)hipFree(d_mask);
     (void)hipFree(d_wcc);
     (void)hipFree(d_capsum); (void)hipFree(d_cert_fp); (void)hipFree(d_pcap);
+    (void)hipFree(d_fstart);
 }
 
 DeviceGraph::FwdSet::~FwdSet() { (void)hipFree(d_mask); }
@@ -152,15 +153,18 @@ int init_cert_consts(DeviceGraph& dg, hipStream_t st) {
     HIPCHK(hipMalloc(&dg.d_capsum, sizeof(int64_t) * std::max(n, 1)));
     HIPCHK(hipMalloc(&dg.d_cert_fp, sizeof(uint64_t)));
     HIPCHK(hipMalloc(&dg.d_pcap, sizeof(int64_t) * std::max(dg.g.E, 1)));
+    HIPCHK(hipMalloc(&dg.d_fstart, sizeof(int32_t) * std::max(n, 1)));
     HIPCHK(hipMemsetAsync(dg.d_cert_fp, 0, sizeof(uint64_t), st));
     dg.g.capsum = dg.d_capsum;
     dg.g.cert_fp = dg.d_cert_fp;
     dg.g.pcap = dg.d_pcap;
+    launch_first_forward(st, dg.g, dg.d_fstart);
+    dg.g.fstart = dg.d_fstart;
     dg.g.cert_seed = 0x9e3779b97f4a7c15ULL * (++graphs_made) ^ ((uint64_t)(uint32_t)n << 32) ^ (uint64_t)(uint32_t)dg.g.E;
     launch_cert_consts(st, dg.g, dg.d_capsum, dg.d_cert_fp);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
-    dg.bytes += sizeof(int64_t) * ((size_t)n + (size_t)dg.g.E) + sizeof(uint64_t);
+    dg.bytes += sizeof(int64_t) * ((size_t)n + (size_t)dg.g.E) + sizeof(uint64_t) + sizeof(int32_t) * (size_t)n;
     return MF_OK;
 }
 

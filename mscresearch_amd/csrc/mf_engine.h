@@ -144,6 +144,7 @@ struct DeviceGraph {
     int64_t* d_capsum = nullptr;   // certificate constants (DGraph::capsum, cert_fp)
     uint64_t* d_cert_fp = nullptr;
     int64_t* d_pcap = nullptr;     // pair capacities (DGraph::pcap)
+    int32_t* d_fstart = nullptr;   // first forward entry per row (DGraph::fstart)
     struct FwdSet {
         std::vector<uint8_t> mask;  // host copy
         uint8_t* d_mask = nullptr;

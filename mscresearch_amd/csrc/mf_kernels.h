@@ -117,5 +117,7 @@ void launch_cap_scatter(hipStream_t st, const DGraph& g, int64_t* cap, const int
                         const int64_t* new_cap, int64_t* old_cap, int n);
 // g.capsum (n) and *fp_out (+=, zero it first) from g's capacities and seed.
 void launch_cert_consts(hipStream_t st, const DGraph& g, int64_t* capsum, uint64_t* fp_out);
+// fstart[v] = first entry of v's row with cap > 0, row_ptr[v] if none.
+void launch_first_forward(hipStream_t st, const DGraph& g, int32_t* fstart);
 
 }  // namespace mf

@@ -128,6 +128,7 @@ struct GGraph {
     GAS uint64_t* cert_fp;
     uint64_t cert_seed;
     const GAS int64_t* pcap;
+    const GAS int32_t* fstart;
 };
 
 struct GSlot {
@@ -177,6 +178,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.g.cert_fp = as_global(v.g.cert_fp);
     S.g.cert_seed = v.g.cert_seed;
     S.g.pcap = as_global(v.g.pcap);
+    S.g.fstart = as_global(v.g.fstart);
     S.res = as_global(v.res);
     S.excess = as_global(v.excess);
     S.h = as_global(v.h);
@@ -576,13 +578,13 @@ __device__ __forceinline__ void td_entries(const GSlot& S, int e0, int e1, int s
     }
 }
 
-// Bottom-up test of row [rb, re) by one thread, U entries at a time (their
-// residual/capacity and column loads together, then their level gathers):
-// does an entry lead to level L?  `scanned` = entries up to the end of the
-// group holding the first such entry.
+// Bottom-up test of entries [rb, re) by one thread, U entries at a time
+// (their residual/capacity and column loads together, then their level
+// gathers): does an entry lead to level L?  `scanned` = entries up to the end
+// of the group holding the first such entry.
 template <int MODE, int U>
-__device__ __forceinline__ bool bu_row_lane(const GSlot& S, int rb, int re, int L, const GAS int32_t* lvl,
-                                            int& scanned) {
+__device__ __forceinline__ bool bu_range_lane(const GSlot& S, int rb, int re, int L, const GAS int32_t* lvl,
+                                              int& scanned) {
     for (int e = rb; e < re; e += U) {
         bool ok[U];
         int c[U], l[U];
@@ -608,6 +610,31 @@ __device__ __forceinline__ bool bu_row_lane(const GSlot& S, int rb, int re, int 
     }
     scanned = re - rb;
     return false;
+}
+
+// First entry of v's row a bottom-up level looks at: the relabel (GR) reaches
+// a parent over a residual out-entry, so it starts at the first forward
+// entry (DGraph::fstart) and wraps; the pruning BFS modes start at the row
+// start.
+template <int MODE>
+__device__ __forceinline__ int bu_first(const GSlot& S, int v, int rb) {
+    return (MODE == BFS_GR && S.g.fstart) ? S.g.fstart[v] : rb;
+}
+
+// Bottom-up test of v's row [rb, re) from fs = bu_first(v), wrapping to
+// [rb, fs).
+template <int MODE, int U>
+__device__ __forceinline__ bool bu_row_lane(const GSlot& S, int rb, int fs, int re, int L, const GAS int32_t* lvl,
+                                            int& scanned) {
+    int a = 0;
+    if (bu_range_lane<MODE, U>(S, fs, re, L, lvl, a)) {
+        scanned = a;
+        return true;
+    }
+    int b = 0;
+    const bool hit = fs > rb && bu_range_lane<MODE, U>(S, rb, fs, L, lvl, b);
+    scanned = a + b;
+    return hit;
 }
 
 // GR seeding (a12): h = n everywhere, level 0 = t plus deficits (phase 0) or
@@ -817,7 +844,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
                 // the level gather only for entries that pass the residual
                 // test (most reverse entries carry no flow)
                 int scanned = 0;
-                const bool hit = bu_row_lane<MODE, 2>(S, rb, re, L, lvl, scanned);
+                const bool hit = bu_row_lane<MODE, 2>(S, rb, bu_first<MODE>(S, v, rb), re, L, lvl, scanned);
                 wk.edges += (uint64_t)scanned;
                 if (hit) {
                     lvl[v] = L + 1;  // only this lane writes v
@@ -857,17 +884,21 @@ __global__ __launch_bounds__(256) void k_bfs_bu_med(const SlotView* __restrict__
     for (int k = (int)blockIdx.x * wpb + (int)(threadIdx.x / WAVE); k < nm; k += (int)gridDim.x * wpb) {
         const int u = S.hl[k];
         const int rb = S.g.row_ptr[u], re = S.g.row_ptr[u + 1];
+        const int fs = bu_first<MODE>(S, u, rb);
+        const int d = re - rb;
         bool hit = false;
         int scanned = 0;
-        for (int e0 = rb; e0 < re; e0 += WAVE) {
-            const int e = e0 + lane;
+        // positions fs, fs + 1, ... wrapping from re to rb
+        for (int k0 = 0; k0 < d; k0 += WAVE) {
+            int e = fs + k0 + lane;
+            if (e >= re) e -= d;
             bool ok = false;
-            if (e < re) {
+            if (k0 + lane < d) {
                 const bool r = bfs_bu_edge_ok<MODE>(S, e);
                 const int c = S.g.col[e];  // issued with the residual load
                 ok = r && lvl[c] == L;
             }
-            scanned += min(WAVE, re - e0);
+            scanned += min(WAVE, d - k0);
             if (__ballot(ok)) { hit = true; break; }
         }
         if (lane == 0) {
@@ -906,12 +937,16 @@ __global__ __launch_bounds__(256) void k_bfs_bu_heavy(const SlotView* __restrict
     for (int k = blockIdx.x; k < nh; k += gridDim.x) {
         const int v = S.giant[k];
         const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+        const int fs = bu_first<MODE>(S, v, rb);
+        const int d = re - rb;
         if (threadIdx.x == 0) found = 0;
         __syncthreads();
-        int base = rb;
-        for (; base < re; base += blockDim.x) {
-            const int e = base + (int)threadIdx.x;
-            if (e < re) {
+        int k0 = 0;
+        // positions fs, fs + 1, ... wrapping from re to rb
+        for (; k0 < d; k0 += blockDim.x) {
+            const int kk = k0 + (int)threadIdx.x;
+            if (kk < d) {
+                const int e = fs + kk < re ? fs + kk : fs + kk - d;
                 const bool r = bfs_bu_edge_ok<MODE>(S, e);
                 const int c = S.g.col[e];  // issued with the residual load
                 if (r && lvl[c] == L) found = 1;
@@ -922,7 +957,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu_heavy(const SlotView* __restrict
             if (f) break;
         }
         if (threadIdx.x == 0) {
-            wk.edges += (uint64_t)(min(base + (int)blockDim.x, re) - rb);
+            wk.edges += (uint64_t)min(k0 + (int)blockDim.x, d);
             if (found) {
                 lvl[v] = L + 1;
                 stage_push(st, v, re - rb, out, ocnt, odeg);
@@ -2977,6 +3012,31 @@ __global__ __launch_bounds__(256) void k_cert_consts(DGraph g, int64_t* __restri
     }
     const uint64_t b = block_sum<uint64_t>(fp, red);
     if (threadIdx.x == 0 && b) addu64(fp_out, b);
+}
+
+// One wave per row, 64 entries per step, stopping at the first step that
+// holds a forward entry.
+__global__ __launch_bounds__(256) void k_first_forward(DGraph g, int32_t* __restrict__ fstart) {
+    const int lane = lane_id();
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / WAVE);
+    for (int64_t v = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; v < g.n; v += nw) {
+        const int rb = g.row_ptr[v], re = g.row_ptr[v + 1];
+        int f = rb;
+        for (int base = rb; base < re; base += WAVE) {
+            const int e = base + lane;
+            const uint64_t m = __ballot(e < re && g.cap[e] > 0);
+            if (m) {
+                f = base + __ffsll((unsigned long long)m) - 1;
+                break;
+            }
+        }
+        if (lane == 0) fstart[v] = f;
+    }
+}
+
+void launch_first_forward(hipStream_t st, const DGraph& g, int32_t* fstart) {
+    const int gx = std::max(1, std::min(16384, (g.n + 3) / 4));
+    hipLaunchKernelGGL(k_first_forward, dim3(gx), dim3(256), 0, st, g, fstart);
 }
 
 void launch_cert_consts(hipStream_t st, const DGraph& g, int64_t* capsum, uint64_t* fp_out) {
