@@ -69,10 +69,10 @@ class Workload:
     """A generated workload, its host graph at R, its device graph, and its
     queries (region-ordered ids) with the oracle's flows when committed."""
 
-    def __init__(self, name, R, tile_max=4096):
+    def __init__(self, name, R, tile_max=4096, e=None):
         t0 = time.time()
         self.name = name
-        e = W.edges(name)
+        e = W.edges(name) if e is None else e
         self.edges = e
         self.n, self.lines = e.info()
         fx = fixture(name)
@@ -149,11 +149,28 @@ def roofline_of(fam_table, workload_name):
 
 # ------------------------------------------------------------------ c3
 def headline(args, out):
-    c3 = Workload("c3", args.R)
+    e3 = W.edges("c3")
+    # the autotuner's recorded choice for this graph class and engine size
+    # (mscresearch_amd/tuned_params.txt, written by python -m mscresearch_amd.tune)
+    tv = None
+    if not args.no_tune:
+        from mscresearch_amd.tune import cached
+
+        tv = cached(M.HostGraph.from_edgelist(e3), args.batch // args.engines)
+    R, tile, slots = args.R, 4096, args.batch // args.engines
+    prm = M.default_params()
+    if tv is not None:
+        # the tuner times one engine; its slots are shared by the bench's engines
+        R, tile, slots = tv["R"], tv["tile_max"], max(1, tv["slots"] // args.engines)
+        prm = M.default_params(K=tv["K"], gr_freq=tv["gr_freq"], gr_freq_min=min(tv["gr_freq"], prm.gr_freq_min),
+                               gr_freq_max=max(2 * tv["gr_freq"], prm.gr_freq_max), tile_threads=tv["tile_threads"])
+        log(f"tuned parameters for {tv['graph_class']}: {tv}")
+    args.R = R
+    c3 = Workload("c3", R, tile, e=e3)
     q = len(c3.s)
     engs = []
     for _ in range(args.engines):
-        e = M.Engine(args.batch // args.engines)
+        e = M.Engine(slots, prm)
         e.add_graph(c3.dg)
         engs.append(e)
     bad_total = checked = 0
@@ -197,8 +214,8 @@ def headline(args, out):
     misses = sum(e.cache_stats()[1] for e in engs)
     del engs
 
-    # plain: one slot, one query at a time, profiling off
-    peng = M.Engine(1)
+    # plain: one slot, one query at a time, profiling off (the same parameters)
+    peng = M.Engine(1, prm)
     pgid = peng.add_graph(c3.dg)
     kq = min(args.plain, q)
     peng.solve(pgid, c3.s[0], c3.t[0])  # the reachability classes, as the batched warm-up
@@ -223,10 +240,10 @@ def headline(args, out):
                  "target_p99_ms": round(2 * pct(plain_lat, .5) * 1e3, 3), "levels": []}
         engs = []
         for _ in range(args.engines):
-            e = M.Engine(args.batch // args.engines)
+            e = M.Engine(slots, prm)
             e.add_graph(c3.dg)
             engs.append(e)
-        pass_over(engs, c3.s[:args.batch], c3.t[:args.batch])  # slabs in place
+        pass_over(engs, c3.s[:args.engines * slots], c3.t[:args.engines * slots])  # slabs in place
         rng = np.random.default_rng(3003)
         for frac in args.sweep:
             rate = frac * qps
@@ -252,9 +269,13 @@ def headline(args, out):
         "config": {"workload": "rmat22_hub_queries", "graph": "R-MAT scale 22 (a=.57,b=.19,c=.19,d=.05), 16*2^22 "
                    "samples, duplicates merged by summing log-uniform [1,1e6] caps, seed 3001",
                    "queries_per_step": q, "query_rule": "s uniform over the top 1% by out-degree, t uniform over "
-                   "in-degree > 0, seed 3002", "n": c3.n, "csr_entries": c3.E, "batch": args.batch,
-                   "engines": args.engines, "regions": args.R,
-                   "parallelism": f"one GPU, {args.engines} engine(s) x {args.batch // args.engines} batched slots, "
+                   "in-degree > 0, seed 3002", "n": c3.n, "csr_entries": c3.E, "batch": args.engines * slots,
+                   "engines": args.engines, "regions": R, "tile_max": tile, "K": prm.K, "gr_freq": prm.gr_freq,
+                   "tile_threads": prm.tile_threads,
+                   "tuned": None if tv is None else {k: tv[k] for k in ("graph_class", "R", "tile_max", "K", "gr_freq",
+                                                                        "tile_threads", "slots", "ms_per_query",
+                                                                        "default_ms_per_query")},
+                   "parallelism": f"one GPU, {args.engines} engine(s) x {slots} batched slots, "
                                   "one host thread and HIP stream each"},
         "plain_qps": round(plain_qps, 3) if plain_qps else None,
         "plain_queries": kq,
@@ -483,7 +504,8 @@ def main():
     ap.add_argument("--batch", type=int, default=192, help="engine slots over all engines")
     ap.add_argument("--engines", type=int, default=2,
                     help="engines sharing the batch slots, one host thread and HIP stream each")
-    ap.add_argument("--R", type=int, default=512)
+    ap.add_argument("--R", type=int, default=512, help="regions when no tuned value is recorded")
+    ap.add_argument("--no-tune", action="store_true", help="ignore the autotuner's recorded parameters")
     ap.add_argument("--plain", type=int, default=256, help="queries timed on the plain (one-at-a-time) solver")
     ap.add_argument("--sweep", type=float, nargs="*", default=[0.1, 0.25, 0.5, 0.75],
                     help="open-loop Poisson loads on R-MAT 22, as fractions of the batched rate")

@@ -106,6 +106,17 @@ int mf_hgraph_line_entries(const mf_hgraph* h, int64_t cnt, const int32_t* us, c
  * {v : v reachable from scc[s]} AND {v : wcc[v] == wcc[t]}. */
 int mf_hgraph_components(const mf_hgraph* h, int32_t* scc, int32_t* wcc);
 
+/* The reference driver's decisions for (s, t) (ids of the partitioned h),
+ * reported by the CLI's log only (hazard: the reference's boundary core drops
+ * interior vertices that can carry s-t flow, so this library never takes
+ * it): the pruning mask keeps *pruned_n vertices (BC:1391-1445); when that
+ * is below 0.995 n (BC:2690) the induced subgraph (*run_n vertices) is
+ * re-partitioned and its boundary-aware core (BC:2340-2475, hop limit 2)
+ * keeps *core_kept; *would_take = 1 when the reference takes that core
+ * (*core_kept < 0.98 *run_n, BC:2743). */
+int mf_hgraph_core_decision(const mf_hgraph* h, int s, int t, int64_t* pruned_n, int64_t* run_n,
+                            int64_t* core_kept, int* would_take);
+
 /* Stand-alone partition on given stats (BC:1130-1178), for parity tests. */
 int mf_partition(int n, int R, const int64_t* sum_out, const int64_t* sum_in,
                  const int32_t* deg_out, const int32_t* deg_in, int32_t* part);
@@ -180,6 +191,11 @@ typedef struct mf_params {
     int small_wgs;       /* workgroups per slot of the multi-level small-frontier BFS (1 = one) */
     long long small_edges; /* ... while the frontier's rows sum to at most this many entries */
     int mark_list;       /* marked entries listed per slab for resets and the certificate (0 = map only) */
+    int inject;          /* 1 = bounded injection: the preflow puts min(out-capacity of s, in-capacity
+                            of t) on s, which is then an ordinary vertex, instead of saturating the row
+                            of s (BC:173-200, 0); pinned (dynamic) queries always use 0 */
+    int tile_threads;    /* threads of the tile (region) discharge workgroup: 256, 512 (BC:1725) or 1024 */
+    int path_switch_deep, p_count_deep; /* path_switch and p_count0 once the last exact relabel is 64+ levels deep */
 } mf_params;
 
 void mf_params_default(mf_params* p);
@@ -309,24 +325,32 @@ typedef void (*mf_step_fn)(void* user, const mf_step_log* log);
 int mf_engine_set_step_log(mf_engine* e, mf_step_fn fn, void* user);
 
 /* ------------------------------------------------------------- autotuner
- * The reference uses R = 128, K = max(4, E/V) and GR_FREQ0 = 128 for every
- * graph (BC:2583, BC:1465-1466).  mf_autotune times the plain solver on
- * sample queries over R in {128, 256, 512, 1024}, tile size in {1024, 2048,
- * 4096} (vertices one discharge workgroup stages in LDS), K in {auto, 1, 2,
- * 4, 8} and GR_FREQ in {64, 128, 256, 512}, one coordinate-descent pass from
- * (R0, tile0, base->K, base->gr_freq); every configuration must reproduce the
- * first one's flows.  h: the host graph before partitioning; s, t: sample
- * queries in its ids.  cache_path (or NULL): per graph class
- * ("n<log2 n>-d<log2 avg degree>-s<log2 skew>-k<log2 BFS depth>") results
- * are read from and appended to this text file. */
+ * The reference uses R = 128, K = max(4, E/V), GR_FREQ0 = 128 and a
+ * 512-thread region kernel for every graph (BC:2583, BC:1465-1466,
+ * BC:1725).  mf_autotune runs one coordinate-descent pass over R in {128,
+ * 256, 512, 1024}, tile size in {1024, 2048, 4096} (vertices one discharge
+ * workgroup stages in LDS), K in {auto, 1, 2, 4, 8}, GR_FREQ in {64, 128,
+ * 256, 512}, the tile discharge workgroup size in {256, 512, 1024} and, for
+ * batch > 0, slots per engine in {batch/2, batch, 2 batch}, from (R0, tile0,
+ * base->K, base->gr_freq, base->tile_threads, batch).  batch = 0 times the
+ * plain solver (one query at a time); batch > 0 times the batched engine
+ * with every sample query submitted at once.  Every configuration must
+ * reproduce the first one's flows.  h: the host graph before partitioning;
+ * s, t: sample queries in its ids.  cache_path (or NULL): results per graph
+ * class ("n<log2 n>-d<log2 avg degree>-s<log2 skew>-k<log2 BFS depth>") and
+ * mode are read from and appended to this text file. */
 typedef struct mf_tune_result {
     int R, tile_max, K, gr_freq;
+    int tile_threads;  /* tile discharge workgroup size */
+    int slots;         /* batched tuning: slots per engine; 0 for the plain solver */
     double ms_per_query, default_ms_per_query;
     int evaluations; /* 0 = found in the cache */
     char graph_class[64];
 } mf_tune_result;
 int mf_autotune(const mf_hgraph* h, int nq, const int32_t* s, const int32_t* t, const mf_params* base, int R0,
-                int tile0, const char* cache_path, mf_tune_result* out);
+                int tile0, int batch, const char* cache_path, mf_tune_result* out);
+/* The graph class mf_autotune keys its cache by, into out (cap bytes). */
+int mf_graph_class(const mf_hgraph* h, char* out, int cap);
 
 /* ------------------------------------------------------- dynamic mode
  * Pinned slots keep their residual across capacity updates (the reference's

@@ -46,7 +46,8 @@ class Params(C.Structure):
         ("prune", C.c_int), ("prune_ratio", C.c_double), ("gr_work", C.c_double), ("K_heavy", C.c_int),
         ("local_relabel", C.c_int), ("local_band", C.c_int), ("fuse", C.c_int), ("graphs", C.c_int),
         ("spec_cert", C.c_int), ("sparse_count", C.c_int), ("small_wgs", C.c_int),
-        ("small_edges", C.c_longlong), ("mark_list", C.c_int),
+        ("small_edges", C.c_longlong), ("mark_list", C.c_int), ("inject", C.c_int),
+        ("tile_threads", C.c_int), ("path_switch_deep", C.c_int), ("p_count_deep", C.c_int),
     ]
 
 
@@ -67,6 +68,7 @@ MF_STEP_HOT, MF_STEP_RM, MF_STEP_CERT, MF_STEP_PRUNE, MF_STEP_COUNT = 32, 64, 12
 
 class TuneResult(C.Structure):
     _fields_ = [("R", C.c_int), ("tile_max", C.c_int), ("K", C.c_int), ("gr_freq", C.c_int),
+                ("tile_threads", C.c_int), ("slots", C.c_int),
                 ("ms_per_query", C.c_double), ("default_ms_per_query", C.c_double), ("evaluations", C.c_int),
                 ("graph_class", C.c_char * 64)]
 
@@ -111,6 +113,8 @@ SIGNATURES = {
     "mf_hgraph_find_entry": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_int32)]),
     "mf_hgraph_components": (C.c_int, [_P, _P, _P]),
     "mf_hgraph_line_entries": (C.c_int, [_P, C.c_int64, _P, _P, _P]),
+    "mf_hgraph_core_decision": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                          C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "mf_partition": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     "mf_read_queries": (C.c_int, [C.c_char_p, C.c_int64, _P, _P, C.POINTER(C.c_int64)]),
     "mf_gen_uniform": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_uint64, C.POINTER(_P)]),
@@ -147,7 +151,9 @@ SIGNATURES = {
     "mf_engine_profile": (C.c_int, [_P, C.c_int]),
     "mf_engine_set_step_log": (C.c_int, [_P, STEP_FN, C.c_void_p]),
     "mf_engine_graph_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
-    "mf_autotune": (C.c_int, [_P, C.c_int, _P, _P, C.POINTER(Params), C.c_int, C.c_int, C.c_char_p, C.POINTER(TuneResult)]),
+    "mf_graph_class": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "mf_autotune": (C.c_int, [_P, C.c_int, _P, _P, C.POINTER(Params), C.c_int, C.c_int, C.c_int, C.c_char_p,
+                              C.POINTER(TuneResult)]),
     "mf_engine_profile_read": (C.c_int, [_P, _P, _P, _P]),
     "mf_engine_pin": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64]),
     "mf_engine_solve_pinned": (C.c_int, [_P, C.c_int, C.POINTER(Result), C.c_int, C.POINTER(C.c_int)]),
@@ -255,17 +261,20 @@ class Edges:
             self._h = None
 
 
-def autotune(hg_unpartitioned, s, t, params=None, R=128, tile=4096, cache=None) -> dict:
-    """Pick R, tile size, K and GR_FREQ for this graph's class by timing the
-    plain solver on sample queries (ids of the unpartitioned graph); every
-    configuration must give the same flows.  Returns the chosen values."""
+def autotune(hg_unpartitioned, s, t, params=None, R=128, tile=4096, cache=None, batch=0) -> dict:
+    """Pick R, tile size, K, GR_FREQ, the tile workgroup size and (batch > 0)
+    slots per engine for this graph's class by timing the plain solver
+    (batch = 0) or the batched engine on sample queries (ids of the
+    unpartitioned graph); every configuration must give the same flows.
+    Returns the chosen values."""
     s = np.ascontiguousarray(s, np.int32)
     t = np.ascontiguousarray(t, np.int32)
     p = params if params is not None else default_params()
     out = TuneResult()
-    _check(lib().mf_autotune(hg_unpartitioned._h, len(s), _ptr(s), _ptr(t), C.byref(p), int(R), int(tile),
+    _check(lib().mf_autotune(hg_unpartitioned._h, len(s), _ptr(s), _ptr(t), C.byref(p), int(R), int(tile), int(batch),
                              cache.encode() if cache else None, C.byref(out)))
     return {"R": out.R, "tile_max": out.tile_max, "K": out.K, "gr_freq": out.gr_freq,
+            "tile_threads": out.tile_threads, "slots": out.slots,
             "ms_per_query": out.ms_per_query, "default_ms_per_query": out.default_ms_per_query,
             "evaluations": out.evaluations, "graph_class": out.graph_class.decode()}
 
@@ -412,6 +421,19 @@ class HostGraph:
         wcc = np.empty(n, np.int32)
         _check(lib().mf_hgraph_components(self._h, _ptr(scc), _ptr(wcc)))
         return scc, wcc
+
+    def core_decision(self, s, t):
+        """The reference driver's pruning / boundary-core decision for (s, t)
+        (region-ordered ids): pruned_n, run_n, core_kept, would_take."""
+        a, b, c, w = C.c_int64(0), C.c_int64(0), C.c_int64(0), C.c_int(0)
+        _check(lib().mf_hgraph_core_decision(self._h, int(s), int(t), C.byref(a), C.byref(b), C.byref(c), C.byref(w)))
+        return {"pruned_n": a.value, "run_n": b.value, "core_kept": c.value, "would_take": bool(w.value)}
+
+    def graph_class(self) -> str:
+        """The class the autotuner keys its cache by."""
+        buf = C.create_string_buffer(64)
+        _check(lib().mf_graph_class(self._h, buf, 64))
+        return buf.value.decode()
 
     def line_entries(self, us, vs):
         """Forward CSR entry (current ids) of every line the graph was built

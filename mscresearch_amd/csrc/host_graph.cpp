@@ -339,6 +339,137 @@ void weak_components(const HostGraph& g, std::vector<int32_t>& comp) {
     for (int v = 0; v < n; ++v) comp[v] = find(v);
 }
 
+// ------------------------------------------- the reference's core decision
+
+CoreDecision reference_core_decision(const HostGraph& g, int s, int t, int hop_limit) {
+    CoreDecision d;
+    const int n = g.n;
+    if (s < 0 || t < 0 || s >= n || t >= n || g.region_id.size() != (size_t)n) return d;
+    // pruning mask (BC:1391-1445)
+    std::vector<char> fwd(n, 0), und(n, 0);
+    std::vector<int32_t> q;
+    q.reserve(n);
+    fwd[s] = 1;
+    q.push_back(s);
+    for (size_t h = 0; h < q.size(); ++h) {
+        const int v = q[h];
+        for (int e = g.row_ptr[v]; e < g.row_ptr[v + 1]; ++e)
+            if (g.cap[e] > 0 && !fwd[g.col[e]]) { fwd[g.col[e]] = 1; q.push_back(g.col[e]); }
+    }
+    q.clear();
+    und[t] = 1;
+    q.push_back(t);
+    for (size_t h = 0; h < q.size(); ++h) {
+        const int v = q[h];
+        for (int e = g.row_ptr[v]; e < g.row_ptr[v + 1]; ++e)
+            if (!und[g.col[e]]) { und[g.col[e]] = 1; q.push_back(g.col[e]); }
+    }
+    std::vector<int32_t> old2sub(n, -1), sub2old;
+    for (int v = 0; v < n; ++v)
+        if ((fwd[v] && und[v]) || v == s || v == t) {
+            old2sub[v] = (int32_t)sub2old.size();
+            sub2old.push_back(v);
+        }
+    d.pruned_n = (int64_t)sub2old.size();
+    d.run_n = n;
+    if (!(d.pruned_n > 0 && (double)d.pruned_n < 0.995 * (double)n)) return d;
+    // induced subgraph (BC:2477-2551): kept entries in row order, rev remapped
+    HostGraph sub;
+    const int k = (int)sub2old.size();
+    sub.n = k;
+    sub.row_ptr.assign((size_t)k + 1, 0);
+    std::vector<int32_t> e2s(g.E(), -1);
+    for (int i = 0; i < k; ++i) {
+        const int v = sub2old[i];
+        int cnt = 0;
+        for (int e = g.row_ptr[v]; e < g.row_ptr[v + 1]; ++e)
+            if (old2sub[g.col[e]] >= 0) e2s[e] = sub.row_ptr[i] + cnt++;
+        sub.row_ptr[i + 1] = sub.row_ptr[i] + cnt;
+    }
+    sub.col.resize(sub.row_ptr[k]);
+    sub.cap.resize(sub.row_ptr[k]);
+    sub.rev.resize(sub.row_ptr[k]);
+    for (int i = 0; i < k; ++i) {
+        const int v = sub2old[i];
+        for (int e = g.row_ptr[v]; e < g.row_ptr[v + 1]; ++e)
+            if (e2s[e] >= 0) {
+                sub.col[e2s[e]] = old2sub[g.col[e]];
+                sub.cap[e2s[e]] = g.cap[e];
+                sub.rev[e2s[e]] = e2s[g.rev[e]];
+            }
+    }
+    d.pruned = true;
+    d.run_n = k;
+    // re-partition on stats over cap > 0 entries (BC:2705-2731)
+    const int runR = std::min(std::max(1, g.R), std::max(1, k));
+    std::vector<int64_t> so, si;
+    std::vector<int32_t> dout, din, part;
+    csr_stats(sub, so, si, dout, din);
+    capacity_degree_partition(k, runR, so, si, dout, din, part);
+    partition_and_reorder(sub, runR, part);
+    const int rs_v = sub.new_id[old2sub[s]], rt_v = sub.new_id[old2sub[t]];
+    // boundary-aware core (BC:2340-2475)
+    const int R = runR;
+    std::vector<std::vector<int32_t>> adj(R), radj(R);
+    std::vector<char> bnd(k, 0);
+    for (int v = 0; v < k; ++v) {
+        const int rv = sub.region_id[v];
+        for (int e = sub.row_ptr[v]; e < sub.row_ptr[v + 1]; ++e) {
+            const int ru = sub.region_id[sub.col[e]];
+            if (ru != rv) {
+                adj[rv].push_back(ru);
+                radj[ru].push_back(rv);
+                bnd[v] = 1;
+            }
+        }
+    }
+    std::vector<char> from_s(R, 0), to_t(R, 0);
+    std::vector<int32_t> rq;
+    from_s[sub.region_id[rs_v]] = 1;
+    rq.push_back(sub.region_id[rs_v]);
+    for (size_t h = 0; h < rq.size(); ++h)
+        for (int x : adj[rq[h]])
+            if (!from_s[x]) { from_s[x] = 1; rq.push_back(x); }
+    rq.clear();
+    to_t[sub.region_id[rt_v]] = 1;
+    rq.push_back(sub.region_id[rt_v]);
+    for (size_t h = 0; h < rq.size(); ++h)
+        for (int x : radj[rq[h]])
+            if (!to_t[x]) { to_t[x] = 1; rq.push_back(x); }
+    const int rs = sub.region_id[rs_v], rt = sub.region_id[rt_v];
+    std::vector<int32_t> dist(k, -1);
+    std::vector<char> mask(k, 0);
+    q.clear();
+    for (int v = 0; v < k; ++v) {
+        const int r = sub.region_id[v];
+        if (!(from_s[r] && to_t[r])) continue;
+        if (r == rs || r == rt || bnd[v]) {
+            mask[v] = 1;
+            dist[v] = 0;
+            q.push_back(v);
+        }
+    }
+    for (size_t h = 0; h < q.size(); ++h) {
+        const int v = q[h];
+        if (dist[v] >= hop_limit) continue;
+        const int rv = sub.region_id[v];
+        for (int e = sub.row_ptr[v]; e < sub.row_ptr[v + 1]; ++e) {
+            const int u = sub.col[e];
+            if (sub.region_id[u] != rv || dist[u] != -1) continue;
+            dist[u] = dist[v] + 1;
+            mask[u] = 1;
+            q.push_back(u);
+        }
+    }
+    mask[rs_v] = 1;
+    mask[rt_v] = 1;
+    int64_t kept = 0;
+    for (int v = 0; v < k; ++v) kept += mask[v];
+    d.core_kept = kept;
+    d.would_take = kept > 0 && (double)kept < 0.98 * (double)k;
+    return d;
+}
+
 // --------------------------------------------------------------- generators
 
 namespace {

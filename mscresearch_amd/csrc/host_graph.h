@@ -68,6 +68,22 @@ void partition_and_reorder(HostGraph& g, int R, const std::vector<int32_t>& part
 void strong_components(const HostGraph& g, std::vector<int32_t>& comp);
 void weak_components(const HostGraph& g, std::vector<int32_t>& comp);
 
+// What the reference's driver would do with (s, t) (BC:2666-2753), for a
+// diagnostic only: its pruning mask (forward reach of s over cap > 0 AND the
+// undirected component of t, plus s and t, BC:1391-1445) keeps pruned_n
+// vertices; when that is below 0.995 n it builds the induced subgraph
+// (BC:2477-2551, run_n vertices), re-partitions it into min(R, run_n) regions
+// (BC:2705-2731) and computes its boundary-aware core (regions on an s-t
+// region path, their boundary vertices and the regions of s and t, grown by
+// hop_limit in-region hops, BC:2340-2475), which it takes when core_kept <
+// 0.98 run_n.  The build never takes that core: it drops interior vertices
+// that can carry s-t flow.  s, t: ids of g (partitioned).
+struct CoreDecision {
+    int64_t pruned_n = 0, run_n = 0, core_kept = 0;
+    bool pruned = false, would_take = false;
+};
+CoreDecision reference_core_decision(const HostGraph& g, int s, int t, int hop_limit = 2);
+
 // ------------------------------------------------------------ generators
 // Seeded, platform-independent generators (splitmix64 -> xoshiro256**).
 // Each writes an edge list with 0-based ids in emission order.

@@ -106,6 +106,10 @@ Params from_c(const mf_params* p) {
     P.small_wgs = p->small_wgs;
     P.small_edges = p->small_edges;
     P.mark_list = p->mark_list;
+    P.inject = p->inject;
+    P.tile_threads = p->tile_threads;
+    P.path_switch_deep = p->path_switch_deep;
+    P.p_count_deep = p->p_count_deep;
     return P;
 }
 
@@ -278,6 +282,19 @@ int mf_hgraph_components(const mf_hgraph* h, int32_t* scc, int32_t* wcc) {
     std::vector<int32_t> a;
     if (scc) { strong_components(h->g, a); std::copy(a.begin(), a.end(), scc); }
     if (wcc) { weak_components(h->g, a); std::copy(a.begin(), a.end(), wcc); }
+    return MF_OK;
+}
+
+int mf_hgraph_core_decision(const mf_hgraph* h, int s, int t, int64_t* pruned_n, int64_t* run_n,
+                            int64_t* core_kept, int* would_take) {
+    if (!h) return fail(MF_EINVAL, "null graph");
+    if (h->g.rstart.empty()) return fail(MF_EINVAL, "graph is not partitioned");
+    if (s < 0 || t < 0 || s >= h->g.n || t >= h->g.n || s == t) return fail(MF_EINVAL, "bad query");
+    const CoreDecision d = reference_core_decision(h->g, s, t);
+    if (pruned_n) *pruned_n = d.pruned_n;
+    if (run_n) *run_n = d.run_n;
+    if (core_kept) *core_kept = d.core_kept;
+    if (would_take) *would_take = d.would_take ? 1 : 0;
     return MF_OK;
 }
 
@@ -475,6 +492,10 @@ void mf_params_default(mf_params* p) {
     p->small_wgs = P.small_wgs;
     p->small_edges = P.small_edges;
     p->mark_list = P.mark_list;
+    p->inject = P.inject;
+    p->tile_threads = P.tile_threads;
+    p->path_switch_deep = P.path_switch_deep;
+    p->p_count_deep = P.p_count_deep;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {
@@ -661,21 +682,29 @@ int mf_engine_cache_stats(const mf_engine* e, int64_t* hits, int64_t* misses) {
 }
 
 int mf_autotune(const mf_hgraph* h, int nq, const int32_t* s, const int32_t* t, const mf_params* base, int R0,
-                int tile0, const char* cache_path, mf_tune_result* out) {
+                int tile0, int batch, const char* cache_path, mf_tune_result* out) {
     if (!h || nq < 1 || !s || !t || !out) return fail(MF_EINVAL, "bad argument");
     if (R0 < 1 || tile0 < 1 || tile0 > kTileMax) return fail(MF_EINVAL, "bad R0 or tile0");
     std::vector<int32_t> vs(s, s + nq), vt(t, t + nq);
     TuneResult r;
-    const int rc = autotune(h->g, vs, vt, from_c(base), R0, tile0, cache_path ? cache_path : "", r);
+    const int rc = autotune(h->g, vs, vt, from_c(base), R0, tile0, batch, cache_path ? cache_path : "", r);
     if (rc != MF_OK) return rc;
     out->R = r.R;
     out->tile_max = r.tile_max;
     out->K = r.K;
     out->gr_freq = r.gr_freq;
+    out->tile_threads = r.tile_threads;
+    out->slots = r.slots;
     out->ms_per_query = r.ms;
     out->default_ms_per_query = r.ms_default;
     out->evaluations = r.evaluations;
     std::snprintf(out->graph_class, sizeof(out->graph_class), "%s", r.cls.c_str());
+    return MF_OK;
+}
+
+int mf_graph_class(const mf_hgraph* h, char* out, int cap) {
+    if (!h || !out || cap < 1) return fail(MF_EINVAL, "bad argument");
+    std::snprintf(out, (size_t)cap, "%s", graph_class(h->g).c_str());
     return MF_OK;
 }
 
@@ -994,11 +1023,15 @@ int mf_dbg_solve_state(mf_engine* e, int gid, int s, int t, mf_result* r, int64_
                        int32_t* height) {
     int rc = check_query(e, gid, s, t);
     if (rc != MF_OK) return rc;
-    const int saved = e->eng.params().prune;
+    // the reference's preflow, on the full graph: the state is the
+    // reference algorithm's and an independent CPU certificate reads it
+    const int saved = e->eng.params().prune, saved_inj = e->eng.params().inject;
     e->eng.params_mut().prune = 0;
+    e->eng.params_mut().inject = 0;
     mf_result tmp;
     rc = mf_solve(e, gid, s, t, &tmp);
     e->eng.params_mut().prune = saved;
+    e->eng.params_mut().inject = saved_inj;
     if (r) *r = tmp;
     if (rc != MF_OK) return rc;
     // the plain solve ran in slot 0 on the full graph

@@ -245,6 +245,16 @@ int main(int argc, char** argv) {
                                     "fallback=full, run_n=%d, run_m=%lld, Flow=%lld, Time=%lld ms\n",
                                     (long long)reflog.prune, (long long)r.kept, n, (long long)r.E_run, r.n_run,
                                     (long long)r.E_run, (long long)r.flow, times[r.tag]);
+                        // the reference would go on to its boundary-aware core
+                        // (BC:2733-2753), which can cut s-t paths; say so
+                        int64_t pn = 0, rn = 0, ck = 0;
+                        int take = 0;
+                        if (mf_hgraph_core_decision(h, s, t, &pn, &rn, &ck, &take) == MF_OK && take)
+                            fprintf(stderr,
+                                    "[RegionBoundary] the reference would run this query on its boundary core: "
+                                    "core_n=%lld of run_n=%lld (< 0.98); not taken here, it can drop vertices that "
+                                    "carry s-t flow\n",
+                                    (long long)ck, (long long)rn);
                     }
                 }
             }

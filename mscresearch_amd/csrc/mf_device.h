@@ -78,6 +78,9 @@ struct Ctr {
     int64_t cut;             // certificate: cut(V\T,T)
     uint64_t fp;             // certificate: pair-sum fingerprint (0 when every pair sums to its capacity)
     int64_t ex_t;            // excess of t seen by the last active / certificate pass
+    // units the preflow put on s (bounded injection, SlotView::inject), 0 when
+    // it saturated the row of s: the excess s holds beyond its row's flow
+    int64_t inject;
     // algorithmic work counters (roofline accounting), per kernel family:
     // wk[family][0] edge scans, [1] vertex scans, [2] pushes,
     // [3] relabels + BFS discoveries
@@ -127,6 +130,11 @@ struct SlotView {
     int32_t H_cut = 100;         // "high" label threshold (BC:1469)
     int32_t phase = 0;           // 0 = normal (seeds t + deficits), 1 = drain (seeds deficits, t active)
     int32_t seed_t_only = 0;     // phase 0 with no deficit (last count): the relabel seeds t alone
+    // 1 = bounded injection: the preflow puts B = min(out-capacity of s,
+    // in-capacity of t) on s instead of saturating its row, and s is then an
+    // ordinary vertex (labelled by the relabel, active, discharged); 0 = the
+    // reference's preflow (BC:173-200), s never labelled
+    int32_t inject = 0;
 };
 
 constexpr int kTileMax = 4096;      // vertices staged in LDS per workgroup

@@ -623,9 +623,16 @@ Plan SlotSched::plan(const Ctr& c) {
         }
         after_local_ = false;
         // discharge step
-        if (active > 0 && active <= P_->path_switch) ++path_stable_;
+        // deep graphs (last exact relabel >= 64 levels): excess crosses the
+        // graph about one hop per discharge iteration, so path pushes start
+        // earlier and with more active vertices
+        const bool deep = gr_depth_ >= 64;
+        const int pswitch = deep ? std::max(P_->path_switch, P_->path_switch_deep) : P_->path_switch;
+        const int pc0 = deep ? std::min(P_->p_count0, P_->p_count_deep) : P_->p_count0;
+        if (active > 0 && active <= pswitch) ++path_stable_;
         else { path_stable_ = 0; p_count_ = P_->p_count0; }
-        const bool use_path = active > 0 && active <= P_->path_switch && path_stable_ >= p_count_;
+        const int pneed = deep ? std::min(p_count_, pc0) : p_count_;
+        const bool use_path = active > 0 && active <= pswitch && path_stable_ >= pneed;
         p.stages = ST_DISCHARGE | ST_RESET_ACT | ST_ACTIVE;
         p.reset_act = RST_ACTIVE;
         if (use_path) {
@@ -901,6 +908,7 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
     v.s = q.s; v.t = q.t;
     v.H_cut = P_.h_cut;
     v.Kh = std::max(1, P_.K_heavy);
+    v.inject = (P_.inject && !S.pinned && !warm) ? 1 : 0;
     const bool prune = P_.prune != 0 && prune_ok && !warm;
     S.sched.begin(&P_, G.g, q.s, q.t, prune, warm);
     S.sched.set_depth_hint(G.depth_hint.load(std::memory_order_relaxed));
@@ -1486,7 +1494,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     }
     if (g_dis[DK_TILE].cnt) {
         const int k = tb(WF_DISCHARGE);
-        launch_discharge_tile(st, d_views_, L(g_dis[DK_TILE]), g_dis[DK_TILE].cnt, maxof(g_dis[DK_TILE], [&](int i) { return plans[i].dis_grid; }), 0, P_.tile_rounds, giants(g_dis[DK_TILE]));
+        launch_discharge_tile(st, d_views_, L(g_dis[DK_TILE]), g_dis[DK_TILE].cnt, maxof(g_dis[DK_TILE], [&](int i) { return plans[i].dis_grid; }), 0, P_.tile_rounds, giants(g_dis[DK_TILE]), P_.tile_threads);
         te(k);
     }
     if (g_dis[DK_LIST].cnt) {
@@ -1502,7 +1510,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     if (g_hot.cnt) {
         const int k = tb(WF_DISCHARGE);
         launch_hot_select(st, d_views_, L(g_hot), g_hot.cnt, P_.hot_k);
-        launch_discharge_tile(st, d_views_, L(g_hot), g_hot.cnt, maxof(g_hot, [&](int i) { return slots_[i].view.g.ntiles; }), 1, P_.hot_rounds, 0);
+        launch_discharge_tile(st, d_views_, L(g_hot), g_hot.cnt, maxof(g_hot, [&](int i) { return slots_[i].view.g.ntiles; }), 1, P_.hot_rounds, 0, P_.tile_threads);
         te(k);
     }
     if (g_rm.cnt) {
@@ -1547,7 +1555,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
                 launch_reset(st, d_views_, LC(c, 2), cnt[c][2], RST_TOUCH);
                 if (cnt[c][DK_TILE]) {
                     const int k = timed ? tb(WF_DISCHARGE) : -1;
-                    launch_discharge_tile(st, d_views_, LC(c, DK_TILE), cnt[c][DK_TILE], tiles, 0, P_.tile_rounds, gi);
+                    launch_discharge_tile(st, d_views_, LC(c, DK_TILE), cnt[c][DK_TILE], tiles, 0, P_.tile_rounds, gi, P_.tile_threads);
                     te(k);
                 }
                 if (cnt[c][DK_LIST]) {
@@ -1800,9 +1808,10 @@ int Engine::dbg_begin(int gid, int s, int t) {
     if (rc == kNoRoom) { set_error("no room in the slab arena"); return MF_ENOMEM; }
     if (rc != MF_OK) return rc;
     // the hooks work on the full graph: no pruning, whatever the graph's
-    // reachability-class cache already holds
+    // reachability-class cache already holds; the reference's preflow
     rc = admit(0, q, false, false);
     if (rc != MF_OK) return rc;
+    slots_[0].view.inject = 0;
     memset(&h_ctr_[0], 0, sizeof(Ctr));
     return MF_OK;
 }

@@ -103,6 +103,23 @@ struct Params {
     // list and the vertices the discharge pushed into, instead of a pass
     // over every vertex (not after path pushes or RM, which record nothing)
     int sparse_count = 1;
+    // Bounded injection (1): the preflow puts B = min(out-capacity of s,
+    // in-capacity of t) on s, an upper bound on the flow, instead of
+    // saturating the row of s (0, BC:173-200), and s is then an ordinary
+    // vertex.  Off by default: on R-MAT 22 hub queries the excess then leaves
+    // s only along shortest paths and finds the longer ones one relabel at a
+    // time (6.2 global relabels per query against 2.7, 400 against 1298
+    // queries/s batched).  Pinned (dynamic) queries always use 0.
+    int inject = 0;
+    // threads of the tile (region) discharge workgroup: 256, 512 or 1024; the
+    // reference's region kernel uses 512 (BC:1725); an autotuner dimension
+    int tile_threads = 512;
+    // path pushes on deep graphs (the last exact relabel at least 64 levels
+    // deep): up to path_switch_deep active vertices, after p_count_deep
+    // iterations with that few (grids: 252 -> 178 ms per plain query, 4.2
+    // global relabels instead of 7.3)
+    int path_switch_deep = 64;
+    int p_count_deep = 1;
 };
 
 // Wall-clock ticks a multi-workgroup BFS barrier may wait before it flags

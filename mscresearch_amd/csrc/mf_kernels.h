@@ -80,9 +80,10 @@ void launch_active_sparse(hipStream_t st, const SlotView* slots, const int* list
 // giants != 0: rows longer than kGiantDeg over the whole grid after it.
 void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* list, int nlist,
                            int gridx, int giants);
-// One workgroup per tile, tile heights and pending excess staged in LDS.
+// One workgroup of `threads` (256, 512 or 1024) per tile, tile heights and
+// pending excess staged in LDS.
 void launch_discharge_tile(hipStream_t st, const SlotView* slots, const int* list, int nlist,
-                           int gridx, int hot_only, int rounds, int giants);
+                           int gridx, int hot_only, int rounds, int giants, int threads);
 // Owner-exclusive path push from the active list.
 void launch_path_push(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
                       int max_hops);

@@ -152,7 +152,7 @@ struct GSlot {
     int32_t mcap;
     const GAS int32_t* mask;
     GAS Ctr* ctr;
-    int32_t s, t, K, Kh, H_cut, phase, seed_t_only;
+    int32_t s, t, K, Kh, H_cut, phase, seed_t_only, inj;
     int32_t tstamp;  // this cycle's touch stamp (negative)
 };
 
@@ -205,6 +205,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.H_cut = v.H_cut;
     S.phase = v.phase;
     S.seed_t_only = v.seed_t_only;
+    S.inj = v.inject;
     S.tstamp = -(2 * S.ctr->epoch + 1);
     return S;
 }
@@ -461,6 +462,24 @@ __global__ __launch_bounds__(256) void k_preflow(const SlotView* __restrict__ sl
     const GSlot S = load_slot(slots, list);
     __shared__ int64_t red[256 / WAVE];
     const int s = S.s;
+    if (S.inj) {
+        // Bounded injection: no flow can exceed the out-capacity of s or the
+        // in-capacity of t (each an s-t cut), so s starts with B units of
+        // excess and an unsaturated row.  In-capacity of t: over t's row,
+        // pcap - cap is the capacity of each entry's pair into t.
+        const int tb = S.g.row_ptr[S.t], te = S.g.row_ptr[S.t + 1];
+        int64_t in_t = 0;
+        for (int e = tb + threadIdx.x; e < te; e += blockDim.x) in_t += S.g.pcap[e] - S.g.cap[e];
+        in_t = block_sum<int64_t>(in_t, red);
+        if (threadIdx.x == 0) {
+            const int64_t out_s = S.g.capsum[s];
+            const int64_t B = out_s < in_t ? out_s : in_t;
+            S.excess[s] = B;
+            S.ctr->inject = B;
+        }
+        return;
+    }
+    if (threadIdx.x == 0) S.ctr->inject = 0;
     const int rb = S.g.row_ptr[s], re = S.g.row_ptr[s + 1];
     int64_t sum = 0;
     for (int e = rb + threadIdx.x; e < re; e += blockDim.x) {
@@ -510,7 +529,7 @@ __device__ __forceinline__ bool bfs_bu_edge_ok(const GSlot& S, int e) {
 template <int MODE>
 __device__ __forceinline__ bool bfs_vertex_ok(const GSlot& S, int v) {
     if (MODE == BFS_GR) {
-        if (v == S.s) return false;
+        if (v == S.s && !S.inj) return false;
         if (S.mask && S.mask[v] == 0) return false;
     }
     return true;
@@ -535,7 +554,7 @@ __device__ __forceinline__ int edge_aux(const GSlot& S, int e) {
 template <int MODE, bool ROW>
 __device__ __forceinline__ bool edge_ok_aux(const GSlot& S, int aux, int w) {
     if (MODE == BFS_GR) {
-        if (w == S.s) return false;
+        if (w == S.s && !S.inj) return false;
         if (S.mask && S.mask[w] == 0) return false;
         if (!ROW) return ld_res(&S.res[aux]) > 0;
     }
@@ -575,6 +594,50 @@ __device__ __forceinline__ void td_entries(const GSlot& S, int e0, int e1, int s
 #pragma unroll
         for (int k = 0; k < U; ++k)
             if (ok[k] && cas32(&lvl[w[k]], n, L + 1)) claim(w[k]);
+    }
+}
+
+// Top-down expansion for deep, narrow BFS levels (grids), where every level
+// is a chain of dependent loads and the frontier's neighbours are mostly
+// visited already: the column and pair entry of U entries, then, all issued
+// together, each neighbour's level, its pair residual (GR) and its row
+// bounds (the degree its frontier entry carries), so a level waits on two
+// gathers instead of four; claim(w, deg) for each vertex moved from n to
+// level L + 1.
+template <int MODE, int U, typename Claim>
+__device__ __forceinline__ void td_entries_deep(const GSlot& S, int e0, int e1, int step, int L,
+                                                GAS int32_t* lvl, Claim claim) {
+    const int n = S.g.n;
+    for (int e = e0; e < e1; e += U * step) {
+        int w[U], ax[U], l[U], d[U];
+        bool okx[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int ek = e + k * step;
+            w[k] = -1;
+            ax[k] = 0;
+            if (ek < e1) {
+                w[k] = S.g.col[ek];
+                ax[k] = MODE == BFS_GR ? S.g.rev[ek] : edge_aux<MODE, true>(S, ek);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            l[k] = -1;
+            okx[k] = false;
+            d[k] = 0;
+            if (w[k] >= 0) {
+                l[k] = lvl[w[k]];
+                okx[k] = MODE == BFS_GR ? ld_res(&S.res[ax[k]]) > 0 : ax[k] != 0;
+                d[k] = S.g.row_ptr[w[k] + 1] - S.g.row_ptr[w[k]];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            bool ok = l[k] == n && okx[k];
+            if (ok && MODE == BFS_GR) ok = (w[k] != S.s || S.inj) && (S.mask == nullptr || S.mask[w[k]] != 0);
+            if (ok && cas32(&lvl[w[k]], n, L + 1)) claim(w[k], d[k]);
+        }
     }
 }
 
@@ -1176,8 +1239,8 @@ __global__ __launch_bounds__(256) void k_bfs_multi(const SlotView* __restrict__ 
                 if (k < 256) { heavy[k] = v; continue; }
             }
             wk.edges += (uint64_t)(re - rb);
-            td_entries<MODE, 4, false>(S, rb, re, 1, L, lvl, [&](int w) {
-                stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
+            td_entries_deep<MODE, 4>(S, rb, re, 1, L, lvl, [&](int w, int deg) {
+                stage_push(st, w, deg, out, ocnt, odeg);
                 wk.relabels++;
             });
         }
@@ -1187,8 +1250,8 @@ __global__ __launch_bounds__(256) void k_bfs_multi(const SlotView* __restrict__ 
             const int v = heavy[k];
             const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
             if (threadIdx.x == 0) wk.edges += (uint64_t)(re - rb);
-            td_entries<MODE, 2, false>(S, rb + (int)threadIdx.x, re, (int)blockDim.x, L, lvl, [&](int w) {
-                stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
+            td_entries_deep<MODE, 2>(S, rb + (int)threadIdx.x, re, (int)blockDim.x, L, lvl, [&](int w, int deg) {
+                stage_push(st, w, deg, out, ocnt, odeg);
                 wk.relabels++;
             });
         }
@@ -1521,7 +1584,7 @@ __global__ __launch_bounds__(256) void k_active(const SlotView* __restrict__ slo
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const int v = (int)i;
         ++nv;
-        if (v == S.s) continue;
+        if (v == S.s && !S.inj) continue;
         const int64_t ex = S.excess[v];
         if (v == S.t) S.ctr->ex_t = ex;
         if (v == S.t && S.phase == 0) continue;
@@ -1601,7 +1664,7 @@ __global__ __launch_bounds__(256) void k_active_sparse(const SlotView* __restric
         const int v = i < pl ? S.fa[i] : (i < pl + ph ? S.fa[n - 1 - (i - pl)] : S.hl[i - pl - ph]);
         if (__hip_atomic_exchange(&S.aux[v], cstamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == cstamp) continue;
         ++nv;
-        if (v == S.s) continue;
+        if (v == S.s && !S.inj) continue;
         const int64_t ex = ld64(&S.excess[v]);
         if (v == S.t) S.ctr->ex_t = ex;
         if (v == S.t && S.phase == 0) continue;
@@ -2155,9 +2218,10 @@ void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* lis
 // heights and pending excess live in LDS for the whole launch; pushes to
 // vertices of the same tile are LDS atomics, other pushes global atomics.
 // `rounds` rebuilds of the tile's active list, one cycle per vertex each.
-__global__ __launch_bounds__(512) void k_discharge_tile(const SlotView* __restrict__ slots,
-                                                        const int* __restrict__ list, int hot_only,
-                                                        int rounds) {
+template <int NT>
+__global__ __launch_bounds__(NT) void k_discharge_tile(const SlotView* __restrict__ slots,
+                                                       const int* __restrict__ list, int hot_only,
+                                                       int rounds) {
     const GSlot S = load_slot(slots, list);
     const int tile = blockIdx.x;
     if (tile >= S.g.ntiles) return;
@@ -2169,7 +2233,7 @@ __global__ __launch_bounds__(512) void k_discharge_tile(const SlotView* __restri
     __shared__ int64_t ldx[kTileMax];
     __shared__ int32_t alist[kTileMax];
     __shared__ int nlight, nheavy;
-    __shared__ uint64_t red[512 / WAVE];
+    __shared__ uint64_t red[NT / WAVE];
     const int a = S.g.tile_start[tile], b = S.g.tile_start[tile + 1];
     const int size = b - a;  // <= kTileMax by construction of tile_start
     const int n = S.g.n;
@@ -2182,7 +2246,7 @@ __global__ __launch_bounds__(512) void k_discharge_tile(const SlotView* __restri
         __syncthreads();
         for (int i = threadIdx.x; i < size; i += blockDim.x) {
             const int v = a + i;
-            if (v == S.s) continue;
+            if (v == S.s && !S.inj) continue;
             if (v == S.t && S.phase == 0) continue;
             if (lh[i] >= n) continue;
             const int64_t ex = ld64(&S.excess[v]) + ldx[i];
@@ -2213,9 +2277,14 @@ __global__ __launch_bounds__(512) void k_discharge_tile(const SlotView* __restri
 }
 
 void launch_discharge_tile(hipStream_t st, const SlotView* slots, const int* list, int nlist,
-                           int gridx, int hot_only, int rounds, int giants) {
-    hipLaunchKernelGGL(k_discharge_tile, dim3(gridx, nlist), dim3(512), 0, st, slots, list, hot_only,
-                       rounds);
+                           int gridx, int hot_only, int rounds, int giants, int threads) {
+    if (threads <= 256)
+        hipLaunchKernelGGL(k_discharge_tile<256>, dim3(gridx, nlist), dim3(256), 0, st, slots, list, hot_only, rounds);
+    else if (threads <= 512)
+        hipLaunchKernelGGL(k_discharge_tile<512>, dim3(gridx, nlist), dim3(512), 0, st, slots, list, hot_only, rounds);
+    else
+        hipLaunchKernelGGL(k_discharge_tile<1024>, dim3(gridx, nlist), dim3(1024), 0, st, slots, list, hot_only,
+                           rounds);
     if (!hot_only && giants) launch_giant(st, slots, list, nlist);
 }
 
@@ -2512,7 +2581,8 @@ __device__ __forceinline__ void cert_entry(const GSlot& S, int e, bool inT, int 
 }
 
 __device__ __forceinline__ void cert_vertex(const GSlot& S, int v, int64_t ex, bool inT, uint64_t& bad) {
-    if (ex != S.excess[v]) ++bad;
+    // s holds the injected units beyond its row's flow
+    if (ex != S.excess[v] - (v == S.s ? S.ctr->inject : 0)) ++bad;
     if (v != S.s && ex < 0) ++bad;
     if (inT && v != S.t && ex != 0) ++bad;
     if (v == S.t) S.ctr->ex_t = S.excess[v];
@@ -2634,7 +2704,7 @@ __global__ __launch_bounds__(256) void k_cert_fp(const SlotView* __restrict__ sl
             uw[i] = u;
             const bool inT = hv < n;
             const int64_t x = S.excess[v];
-            fp -= u * (uint64_t)(S.g.capsum[v] + x);
+            fp -= u * (uint64_t)(S.g.capsum[v] + x - (v == S.s ? S.ctr->inject : 0));
             if (v != S.s && x < 0) ++bad;
             if (inT && v != S.t && x != 0) ++bad;
             if (v == S.t) {
@@ -2768,7 +2838,7 @@ __global__ __launch_bounds__(256) void k_cert_sp_rows(const SlotView* __restrict
             const int v = v0 + i;
             inT = S.h[v] < n;
             const int64_t x = S.excess[v];
-            fp -= vertex_weight(v, seed) * (uint64_t)x;
+            fp -= vertex_weight(v, seed) * (uint64_t)(x - (v == S.s ? S.ctr->inject : 0));
             if (v != S.s && x < 0) ++bad;
             if (inT && v != S.t && x != 0) ++bad;
             if (v == S.t) {

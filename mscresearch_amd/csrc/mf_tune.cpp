@@ -1,13 +1,16 @@
 // mf_tune.cpp -- autotuner: region count R, tile size (the vertex range one
-// discharge workgroup stages in LDS), local-cycle bound K and GR_FREQ per
-// graph class.
+// discharge workgroup stages in LDS), local-cycle bound K, GR_FREQ, the
+// tile discharge workgroup size and, for the batched engine, slots per
+// engine, per graph class.
 //
-// The reference fixes R = 128 (BC:2583), K = max(4, E/V) (BC:1465) and
-// GR_FREQ0 = 128 (BC:1466) for every graph.  Here one coordinate-descent pass
-// over the four knobs times the plain solver on sample queries of the graph;
-// every candidate must reproduce the flows of the first (default)
-// configuration, so tuning never changes an answer.  Results are kept per
-// graph class (sizes, degree skew and BFS depth, bucketed) in a text file.
+// The reference fixes R = 128 (BC:2583), K = max(4, E/V) (BC:1465),
+// GR_FREQ0 = 128 (BC:1466) and 512 threads for its region kernel (BC:1725)
+// for every graph.  Here one coordinate-descent pass over the knobs times
+// the plain solver (one query at a time) or the batched engine (all sample
+// queries at once) on sample queries of the graph; every candidate must
+// reproduce the flows of the first (default) configuration, so tuning never
+// changes an answer.  Results are kept per graph class (sizes, degree skew
+// and BFS depth, bucketed) and mode in a text file.
 #include "mf_tune.h"
 
 #include <algorithm>
@@ -95,47 +98,58 @@ int prepare(const HostGraph& g0, int R, int tile, const std::vector<int32_t>& s0
     return MF_OK;
 }
 
-// ms per query of the plain solver (one slot), after one untimed solve that
-// warms the graph's reachability caches; flows into `flows`.
-int measure(const Setup& su, const Params& P, std::vector<int64_t>& flows, double& ms) {
+// ms per sample query, flows into `flows` (by sample index).  slots == 0:
+// the plain solver, one query at a time on one slot; slots >= 1: every
+// sample query submitted at once to an engine of that many slots (the
+// batched engine's continuous refill), wall time / queries.  One untimed
+// round first warms the graph's reachability caches and the slabs.
+int measure(const Setup& su, const Params& P, int slots, std::vector<int64_t>& flows, double& ms) {
     Engine eng;
-    int rc = eng.init(1, P);
+    int rc = eng.init(std::max(1, slots), P);
     if (rc != MF_OK) return rc;
     const int gid = eng.add_graph(su.dg);
+    const int nq = (int)su.s.size();
     std::vector<Result> out;
-    auto solve = [&](int i) -> int {
-        Query q;
-        q.graph = gid;
-        q.s = su.s[i];
-        q.t = su.t[i];
-        q.tag = i;
+    auto run = [&](int lo, int hi) -> int {
         out.clear();
-        int r = eng.submit(q);
-        if (r != MF_OK) return r;
-        r = eng.run(out);
-        if (r != MF_OK) return r;
-        if (out.size() != 1 || out[0].st.status != MF_OK) {
-            set_error("autotune: a sample query failed: " + last_error());
-            return out.empty() ? MF_EINVAL : out[0].st.status;
+        for (int i = lo; i < hi; ++i) {
+            Query q;
+            q.graph = gid;
+            q.s = su.s[i];
+            q.t = su.t[i];
+            q.tag = i;
+            const int r = eng.submit(q);
+            if (r != MF_OK) return r;
         }
+        const int r = eng.run(out);
+        if (r != MF_OK) return r;
+        for (auto& x : out)
+            if (x.st.status != MF_OK) {
+                set_error("autotune: a sample query failed: " + last_error());
+                return x.st.status;
+            }
+        if ((int)out.size() != hi - lo) { set_error("autotune: a sample query gave no result"); return MF_EHIP; }
+        for (auto& x : out) flows[x.tag] = x.st.flow;
         return MF_OK;
     };
-    if (!su.s.empty()) {
-        rc = solve(0);
-        if (rc != MF_OK) return rc;
-    }
-    flows.assign(su.s.size(), 0);
+    flows.assign(nq, 0);
+    // warm-up: plain, the first query; batched, a round of min(slots, nq) queries
+    rc = run(0, slots == 0 ? std::min(1, nq) : std::min(slots, nq));
+    if (rc != MF_OK) return rc;
     const double t0 = seconds_now();
-    for (size_t i = 0; i < su.s.size(); ++i) {
-        rc = solve((int)i);
-        if (rc != MF_OK) return rc;
-        flows[i] = out[0].st.flow;
+    if (slots == 0) {
+        for (int i = 0; i < nq; ++i)
+            if ((rc = run(i, i + 1)) != MF_OK) return rc;
+    } else if ((rc = run(0, nq)) != MF_OK) {
+        return rc;
     }
-    ms = (seconds_now() - t0) * 1e3 / std::max<size_t>(1, su.s.size());
+    ms = (seconds_now() - t0) * 1e3 / std::max(1, nq);
     return MF_OK;
 }
 
-bool read_cache(const std::string& path, const std::string& cls, TuneResult& r) {
+std::string cache_key(const std::string& cls, int batch) { return cls + (batch > 0 ? ":b" + std::to_string(batch) : ":plain"); }
+
+bool read_cache(const std::string& path, const std::string& key, TuneResult& r) {
     std::ifstream f(path);
     std::string line;
     bool found = false;
@@ -143,8 +157,9 @@ bool read_cache(const std::string& path, const std::string& cls, TuneResult& r) 
         std::istringstream in(line);
         std::string c;
         TuneResult x;
-        if (!(in >> c >> x.R >> x.tile_max >> x.K >> x.gr_freq >> x.ms >> x.ms_default)) continue;
-        if (c == cls) { r = x; found = true; }  // the last entry of a class wins
+        if (!(in >> c >> x.R >> x.tile_max >> x.K >> x.gr_freq >> x.tile_threads >> x.slots >> x.ms >> x.ms_default))
+            continue;
+        if (c == key) { r = x; found = true; }  // the last entry of a key wins
     }
     return found;
 }
@@ -152,30 +167,32 @@ bool read_cache(const std::string& path, const std::string& cls, TuneResult& r) 
 }  // namespace
 
 int autotune(const HostGraph& g0, const std::vector<int32_t>& s, const std::vector<int32_t>& t, const Params& base,
-             int R0, int tile0, const std::string& cache_path, TuneResult& out) {
+             int R0, int tile0, int batch, const std::string& cache_path, TuneResult& out) {
     if (!g0.rstart.empty()) { set_error("autotune needs the graph before partitioning"); return MF_EINVAL; }
     if (s.empty() || s.size() != t.size()) { set_error("autotune needs sample queries"); return MF_EINVAL; }
+    if (batch < 0) { set_error("autotune: batch must be >= 0"); return MF_EINVAL; }
     for (size_t i = 0; i < s.size(); ++i)
         if (s[i] < 0 || t[i] < 0 || s[i] >= g0.n || t[i] >= g0.n || s[i] == t[i]) {
             set_error("autotune: bad sample query");
             return MF_EINVAL;
         }
     const std::string cls = graph_class(g0);
+    const std::string key = cache_key(cls, batch);
     if (!cache_path.empty()) {
         TuneResult c;
-        if (read_cache(cache_path, cls, c)) {
+        if (read_cache(cache_path, key, c)) {
             out = c;
             out.cls = cls;
             out.evaluations = 0;
             return MF_OK;
         }
     }
-    struct Cand { int R, tile, K, gr; };
-    Cand best{R0, tile0, base.K, base.gr_freq};
+    struct Cand { int R, tile, K, gr, threads, slots; };
+    Cand best{R0, tile0, base.K, base.gr_freq, base.tile_threads, batch};
     std::map<std::pair<int, int>, Setup> setups;
     auto setup_for = [&](int R, int tile, Setup** su) -> int {
-        auto key = std::make_pair(R, tile);
-        auto it = setups.find(key);
+        auto key2 = std::make_pair(R, tile);
+        auto it = setups.find(key2);
         if (it == setups.end()) {
             // keep at most the incumbent's setup and this one resident
             for (auto jt = setups.begin(); jt != setups.end();)
@@ -183,7 +200,7 @@ int autotune(const HostGraph& g0, const std::vector<int32_t>& s, const std::vect
             Setup nsu;
             const int rc = prepare(g0, R, tile, s, t, nsu);
             if (rc != MF_OK) return rc;
-            it = setups.emplace(key, std::move(nsu)).first;
+            it = setups.emplace(key2, std::move(nsu)).first;
         }
         *su = &it->second;
         return MF_OK;
@@ -199,8 +216,9 @@ int autotune(const HostGraph& g0, const std::vector<int32_t>& s, const std::vect
         P.gr_freq = c.gr;
         P.gr_freq_min = std::min(c.gr, base.gr_freq_min);
         P.gr_freq_max = std::max(2 * c.gr, base.gr_freq_max);
+        P.tile_threads = c.threads;
         std::vector<int64_t> flows;
-        rc = measure(*su, P, flows, ms);
+        rc = measure(*su, P, c.slots, flows, ms);
         if (rc != MF_OK) return rc;
         ++evals;
         if (ref.empty()) {
@@ -219,6 +237,11 @@ int autotune(const HostGraph& g0, const std::vector<int32_t>& s, const std::vect
     const std::vector<int> tiles = {1024, 2048, 4096};
     const std::vector<int> Ks = {0, 1, 2, 4, 8};
     const std::vector<int> grs = {64, 128, 256, 512};
+    const std::vector<int> threads = {256, 512, 1024};
+    std::vector<int> slots;
+    if (batch > 0)
+        for (int k : {batch / 2, batch, 2 * batch})
+            if (k >= 1 && std::find(slots.begin(), slots.end(), k) == slots.end()) slots.push_back(k);
     auto sweep = [&](auto field, const std::vector<int>& vals) -> int {
         for (int v : vals) {
             Cand c = best;
@@ -235,18 +258,22 @@ int autotune(const HostGraph& g0, const std::vector<int32_t>& s, const std::vect
     if ((rc = sweep([](Cand& c) -> int& { return c.tile; }, tiles)) != MF_OK) return rc;
     if ((rc = sweep([](Cand& c) -> int& { return c.K; }, Ks)) != MF_OK) return rc;
     if ((rc = sweep([](Cand& c) -> int& { return c.gr; }, grs)) != MF_OK) return rc;
+    if ((rc = sweep([](Cand& c) -> int& { return c.threads; }, threads)) != MF_OK) return rc;
+    if ((rc = sweep([](Cand& c) -> int& { return c.slots; }, slots)) != MF_OK) return rc;
     out.R = best.R;
     out.tile_max = best.tile;
     out.K = best.K;
     out.gr_freq = best.gr;
+    out.tile_threads = best.threads;
+    out.slots = best.slots;
     out.ms = best_ms;
     out.ms_default = default_ms;
     out.evaluations = evals;
     out.cls = cls;
     if (!cache_path.empty()) {
         std::ofstream f(cache_path, std::ios::app);
-        f << cls << " " << out.R << " " << out.tile_max << " " << out.K << " " << out.gr_freq << " " << out.ms << " "
-          << out.ms_default << "\n";
+        f << key << " " << out.R << " " << out.tile_max << " " << out.K << " " << out.gr_freq << " " << out.tile_threads
+          << " " << out.slots << " " << out.ms << " " << out.ms_default << "\n";
     }
     return MF_OK;
 }

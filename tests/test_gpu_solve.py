@@ -141,6 +141,11 @@ def test_edge_cases():
     dict(small_edges=1 << 30),                  # single-workgroup levels up to kSmallFrontier vertices
     dict(mark_list=0),                          # resets and certificates from the mark map
     dict(mark_list=16),                         # ... once the list of marked entries overflows
+    dict(inject=1),                             # bounded injection at s instead of a saturated row
+    dict(inject=1, cert=1),                     # ... with the fingerprint over every pair
+    dict(inject=1, cert=2, sparse_count=0),     # ... and entry-by-entry pair sums
+    dict(tile_threads=256, tile_frac=0.0),      # tile discharge in 256-thread workgroups, every step
+    dict(tile_threads=1024, tile_frac=0.0),     # ... and in 1024-thread workgroups
 ])
 def test_scheduler_variants_stay_exact(c1, variant):
     log(f"variant {variant}")
@@ -328,8 +333,11 @@ def test_fused_cycles_replay_captured_graphs():
     qs, qt = M.grid_queries(96, 96, 4, 2102)
     q = [case.q(a, b) for a, b in zip(qs, qt)]
     want = [case.oracle_flow(s, t) for s, t in q]
+    # the reference's path-push switch on deep graphs too, so that the
+    # discharge runs fused cycles long enough to repeat
+    ref_paths = dict(path_switch_deep=0, p_count_deep=64)
     for prm in (dict(), dict(graphs=0), dict(fuse=1)):
-        eng, gid = engine_with(case, **prm)
+        eng, gid = engine_with(case, **ref_paths, **prm)
         got = [eng.solve(gid, s, t) for s, t in q]
         assert [r.flow for r in got] == want, prm
         assert all(r.cert_ok == 1 for r in got)

@@ -280,3 +280,45 @@ def test_edge_hash_sees_any_change():
     assert h != W.edge_hash(us, vs, np.array([5, 6, 8]))
     assert h != W.edge_hash(vs, us, cs)
     assert h != W.edge_hash(us[[1, 0, 2]], vs[[1, 0, 2]], cs[[1, 0, 2]])
+
+
+@pytest.mark.parametrize("gen,R", [("rmat", 16), ("rmat", 4), ("uniform", 8), ("sparse", 8)])
+def test_reference_core_decision_matches_oracle(gen, R):
+    """The CLI's diagnostic of the reference's boundary-core step
+    (BC:2666-2753): prune mask, induced subgraph, re-partition into
+    min(R, n') regions and the boundary-aware core, each restated by the
+    oracle; the kept counts and the take decision must agree."""
+    if gen == "rmat":
+        e = M.Edges.rmat(9, 10**6, 95)
+        qs, qt = e.hub_queries(10, 96)
+    elif gen == "uniform":
+        e = M.Edges.uniform(600, 3000, 40, 97)
+        qs, qt = M.uniform_queries(600, 10, 98)
+    else:  # sparse: many small components, so pruning bites
+        e = M.Edges.uniform(2000, 1500, 40, 99)
+        qs, qt = M.uniform_queries(2000, 10, 100)
+    n, us, vs, cs = e.arrays()
+    hg = M.HostGraph.from_edgelist(e).partition(R)
+    d = hg.export()
+    og = O.Graph(n, d["row_ptr"], d["col"], d["cap"], d["rev"])
+    takes = 0
+    for a, b in zip(qs, qt):
+        s, t = int(d["new_id"][a]), int(d["new_id"][b])
+        got = hg.core_decision(s, t)
+        mask = O.prune_mask(og, s, t)
+        pn = int(mask.sum())
+        assert got["pruned_n"] == pn
+        if not (0 < pn < 0.995 * n):
+            assert got["run_n"] == n and not got["would_take"]
+            continue
+        sub, o2s, _ = O.induced(og, mask)
+        runR = min(R, sub.n)
+        so, si, do, di = O.csr_stats(sub)
+        part, _ = O.partition(sub.n, runR, so, si, do, di)
+        g2, nid, _oid, _rs, _re, rid = O.reorder(sub, runR, part)
+        core, _kr = O.core_mask(g2, runR, rid, int(nid[o2s[s]]), int(nid[o2s[t]]), 2)
+        kept = int(core.sum())
+        assert got["run_n"] == sub.n
+        assert got["core_kept"] == kept
+        assert got["would_take"] == (0 < kept < 0.98 * sub.n)
+        takes += int(got["would_take"])
