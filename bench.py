@@ -163,7 +163,8 @@ def headline(args, out):
         # the tuner times one engine; its slots are shared by the bench's engines
         R, tile, slots = tv["R"], tv["tile_max"], max(1, tv["slots"] // args.engines)
         prm = M.default_params(K=tv["K"], gr_freq=tv["gr_freq"], gr_freq_min=min(tv["gr_freq"], prm.gr_freq_min),
-                               gr_freq_max=max(2 * tv["gr_freq"], prm.gr_freq_max), tile_threads=tv["tile_threads"])
+                               gr_freq_max=max(2 * tv["gr_freq"], prm.gr_freq_max), tile_threads=tv["tile_threads"],
+                               tile_split=tv.get("tile_split", 1))
         log(f"tuned parameters for {tv['graph_class']}: {tv}")
     args.R = R
     c3 = Workload("c3", R, tile, e=e3)
@@ -271,9 +272,10 @@ def headline(args, out):
                    "queries_per_step": q, "query_rule": "s uniform over the top 1% by out-degree, t uniform over "
                    "in-degree > 0, seed 3002", "n": c3.n, "csr_entries": c3.E, "batch": args.engines * slots,
                    "engines": args.engines, "regions": R, "tile_max": tile, "K": prm.K, "gr_freq": prm.gr_freq,
-                   "tile_threads": prm.tile_threads,
+                   "tile_threads": prm.tile_threads, "tile_split": prm.tile_split,
                    "tuned": None if tv is None else {k: tv[k] for k in ("graph_class", "R", "tile_max", "K", "gr_freq",
-                                                                        "tile_threads", "slots", "ms_per_query",
+                                                                        "tile_threads", "slots", "tile_split",
+                                                                        "ms_per_query",
                                                                         "default_ms_per_query")},
                    "parallelism": f"one GPU, {args.engines} engine(s) x {slots} batched slots, "
                                   "one host thread and HIP stream each"},

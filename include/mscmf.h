@@ -196,6 +196,7 @@ typedef struct mf_params {
                             of s (BC:173-200, 0); pinned (dynamic) queries always use 0 */
     int tile_threads;    /* threads of the tile (region) discharge workgroup: 256, 512 (BC:1725) or 1024 */
     int path_switch_deep, p_count_deep; /* path_switch and p_count0 once the last exact relabel is 64+ levels deep */
+    int tile_split;      /* tile discharge: up to this many workgroups per tile by excess score (1..8, 1 = off) */
 } mf_params;
 
 void mf_params_default(mf_params* p);
@@ -332,7 +333,7 @@ int mf_engine_set_step_log(mf_engine* e, mf_step_fn fn, void* user);
  * workgroup stages in LDS), K in {auto, 1, 2, 4, 8}, GR_FREQ in {64, 128,
  * 256, 512}, the tile discharge workgroup size in {256, 512, 1024} and, for
  * batch > 0, slots per engine in {batch/2, batch, 2 batch}, from (R0, tile0,
- * base->K, base->gr_freq, base->tile_threads, batch).  batch = 0 times the
+ * base->K, base->gr_freq, base->tile_threads, batch, base->tile_split).  batch = 0 times the
  * plain solver (one query at a time); batch > 0 times the batched engine
  * with every sample query submitted at once.  Every configuration must
  * reproduce the first one's flows.  h: the host graph before partitioning;
@@ -343,6 +344,7 @@ typedef struct mf_tune_result {
     int R, tile_max, K, gr_freq;
     int tile_threads;  /* tile discharge workgroup size */
     int slots;         /* batched tuning: slots per engine; 0 for the plain solver */
+    int tile_split;    /* tile discharge workgroups per tile by excess score */
     double ms_per_query, default_ms_per_query;
     int evaluations; /* 0 = found in the cache */
     char graph_class[64];

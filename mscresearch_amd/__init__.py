@@ -48,6 +48,7 @@ class Params(C.Structure):
         ("spec_cert", C.c_int), ("sparse_count", C.c_int), ("small_wgs", C.c_int),
         ("small_edges", C.c_longlong), ("mark_list", C.c_int), ("inject", C.c_int),
         ("tile_threads", C.c_int), ("path_switch_deep", C.c_int), ("p_count_deep", C.c_int),
+        ("tile_split", C.c_int),
     ]
 
 
@@ -68,7 +69,7 @@ MF_STEP_HOT, MF_STEP_RM, MF_STEP_CERT, MF_STEP_PRUNE, MF_STEP_COUNT = 32, 64, 12
 
 class TuneResult(C.Structure):
     _fields_ = [("R", C.c_int), ("tile_max", C.c_int), ("K", C.c_int), ("gr_freq", C.c_int),
-                ("tile_threads", C.c_int), ("slots", C.c_int),
+                ("tile_threads", C.c_int), ("slots", C.c_int), ("tile_split", C.c_int),
                 ("ms_per_query", C.c_double), ("default_ms_per_query", C.c_double), ("evaluations", C.c_int),
                 ("graph_class", C.c_char * 64)]
 
@@ -262,8 +263,8 @@ class Edges:
 
 
 def autotune(hg_unpartitioned, s, t, params=None, R=128, tile=4096, cache=None, batch=0) -> dict:
-    """Pick R, tile size, K, GR_FREQ, the tile workgroup size and (batch > 0)
-    slots per engine for this graph's class by timing the plain solver
+    """Pick R, tile size, K, GR_FREQ, the tile workgroup size, workgroups per
+    tile by excess score and (batch > 0) slots per engine for this graph's class by timing the plain solver
     (batch = 0) or the batched engine on sample queries (ids of the
     unpartitioned graph); every configuration must give the same flows.
     Returns the chosen values."""
@@ -274,7 +275,7 @@ def autotune(hg_unpartitioned, s, t, params=None, R=128, tile=4096, cache=None, 
     _check(lib().mf_autotune(hg_unpartitioned._h, len(s), _ptr(s), _ptr(t), C.byref(p), int(R), int(tile), int(batch),
                              cache.encode() if cache else None, C.byref(out)))
     return {"R": out.R, "tile_max": out.tile_max, "K": out.K, "gr_freq": out.gr_freq,
-            "tile_threads": out.tile_threads, "slots": out.slots,
+            "tile_threads": out.tile_threads, "slots": out.slots, "tile_split": out.tile_split,
             "ms_per_query": out.ms_per_query, "default_ms_per_query": out.default_ms_per_query,
             "evaluations": out.evaluations, "graph_class": out.graph_class.decode()}
 

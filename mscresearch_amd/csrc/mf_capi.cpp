@@ -110,6 +110,7 @@ Params from_c(const mf_params* p) {
     P.tile_threads = p->tile_threads;
     P.path_switch_deep = p->path_switch_deep;
     P.p_count_deep = p->p_count_deep;
+    P.tile_split = p->tile_split;
     return P;
 }
 
@@ -496,6 +497,7 @@ void mf_params_default(mf_params* p) {
     p->tile_threads = P.tile_threads;
     p->path_switch_deep = P.path_switch_deep;
     p->p_count_deep = P.p_count_deep;
+    p->tile_split = P.tile_split;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {
@@ -694,6 +696,7 @@ int mf_autotune(const mf_hgraph* h, int nq, const int32_t* s, const int32_t* t, 
     out->K = r.K;
     out->gr_freq = r.gr_freq;
     out->tile_threads = r.tile_threads;
+    out->tile_split = r.tile_split;
     out->slots = r.slots;
     out->ms_per_query = r.ms;
     out->default_ms_per_query = r.ms_default;

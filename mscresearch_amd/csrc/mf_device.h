@@ -108,6 +108,10 @@ struct SlotView {
     int32_t* hl = nullptr;       // n heavy-row work list (BFS levels, certificate)
     int32_t* tile_cnt = nullptr; // ntiles active counts
     int32_t* tile_hot = nullptr; // ntiles hot flags
+    int64_t* tile_ex = nullptr;  // ntiles positive excess of the tile's active vertices (last count)
+    // workgroups per tile by excess score: [0] = count, then (tile, part,
+    // parts) per workgroup, at most kSplitMax per tile
+    int32_t* tsplit = nullptr;
     int32_t* lbkt = nullptr;     // 2 * kLocalLevels local-relabel bucket counts and cursors
     // giant-row list (rows longer than kGiantDeg; at most E / (kGiantDeg+1)
     // of them, gcap): ids [0, gcap), per-row minimum [gcap, 2 gcap), chunk
@@ -145,5 +149,6 @@ constexpr int64_t kSmallEdges = 1 << 18; // ... and while the frontier's rows st
 constexpr int kLocalLevels = 256;   // level cap of the local relabel
 constexpr int kGiantChunk = 2048;   // entries per workgroup task on a giant row
 constexpr int kMarkListMax = 1 << 20;  // marked entries listed per slab (beyond: the map alone)
+constexpr int kSplitMax = 8;        // workgroups one tile may get by excess score
 
 }  // namespace mf

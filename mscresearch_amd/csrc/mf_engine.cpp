@@ -190,6 +190,8 @@ size_t Slab::layout(int n, int64_t E, int ntiles, char* base, Slab* sl) {
     char* p_hl = take(4 * nn);
     char* p_tc = take(4 * tt);
     char* p_th = take(4 * tt);
+    char* p_tx = take(8 * tt);
+    char* p_ts = take(sizeof(int32_t) * (1 + 3 * (size_t)kSplitMax * tt));
     char* p_lb = take(sizeof(int32_t) * 2 * kLocalLevels);
     char* p_gi = take(sizeof(int32_t) * (giant_task_offset((int64_t)gcap) + 4 * (ee / kGiantChunk + gcap + 1)));
     char* p_ga = take(sizeof(int64_t) * gcap);
@@ -207,6 +209,8 @@ size_t Slab::layout(int n, int64_t E, int ntiles, char* base, Slab* sl) {
         sl->hl = (int32_t*)p_hl;
         sl->tile_cnt = (int32_t*)p_tc;
         sl->tile_hot = (int32_t*)p_th;
+        sl->tile_ex = (int64_t*)p_tx;
+        sl->tsplit = (int32_t*)p_ts;
         sl->lbkt = (int32_t*)p_lb;
         sl->giant = (int32_t*)p_gi;
         sl->gacc = (int64_t*)p_ga;
@@ -899,6 +903,7 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
     v.g = G.g;
     v.res = sl.res; v.excess = sl.excess; v.h = sl.h; v.fa = sl.fa; v.fb = sl.fb;
     v.lock = sl.lock; v.aux = sl.aux; v.hl = sl.hl; v.tile_cnt = sl.tile_cnt; v.tile_hot = sl.tile_hot;
+    v.tile_ex = sl.tile_ex; v.tsplit = sl.tsplit;
     v.lbkt = sl.lbkt;
     v.giant = sl.giant; v.gacc = sl.gacc; v.gcap = sl.gcap;
     v.emark = sl.emark;
@@ -1494,7 +1499,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     }
     if (g_dis[DK_TILE].cnt) {
         const int k = tb(WF_DISCHARGE);
-        launch_discharge_tile(st, d_views_, L(g_dis[DK_TILE]), g_dis[DK_TILE].cnt, maxof(g_dis[DK_TILE], [&](int i) { return plans[i].dis_grid; }), 0, P_.tile_rounds, giants(g_dis[DK_TILE]), P_.tile_threads);
+        launch_discharge_tile(st, d_views_, L(g_dis[DK_TILE]), g_dis[DK_TILE].cnt, maxof(g_dis[DK_TILE], [&](int i) { return plans[i].dis_grid; }), 0, P_.tile_rounds, giants(g_dis[DK_TILE]), P_.tile_threads, P_.tile_split);
         te(k);
     }
     if (g_dis[DK_LIST].cnt) {
@@ -1510,7 +1515,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     if (g_hot.cnt) {
         const int k = tb(WF_DISCHARGE);
         launch_hot_select(st, d_views_, L(g_hot), g_hot.cnt, P_.hot_k);
-        launch_discharge_tile(st, d_views_, L(g_hot), g_hot.cnt, maxof(g_hot, [&](int i) { return slots_[i].view.g.ntiles; }), 1, P_.hot_rounds, 0, P_.tile_threads);
+        launch_discharge_tile(st, d_views_, L(g_hot), g_hot.cnt, maxof(g_hot, [&](int i) { return slots_[i].view.g.ntiles; }), 1, P_.hot_rounds, 0, P_.tile_threads, P_.tile_split);
         te(k);
     }
     if (g_rm.cnt) {
@@ -1555,7 +1560,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
                 launch_reset(st, d_views_, LC(c, 2), cnt[c][2], RST_TOUCH);
                 if (cnt[c][DK_TILE]) {
                     const int k = timed ? tb(WF_DISCHARGE) : -1;
-                    launch_discharge_tile(st, d_views_, LC(c, DK_TILE), cnt[c][DK_TILE], tiles, 0, P_.tile_rounds, gi, P_.tile_threads);
+                    launch_discharge_tile(st, d_views_, LC(c, DK_TILE), cnt[c][DK_TILE], tiles, 0, P_.tile_rounds, gi, P_.tile_threads, P_.tile_split);
                     te(k);
                 }
                 if (cnt[c][DK_LIST]) {

@@ -120,6 +120,10 @@ struct Params {
     // global relabels instead of 7.3)
     int path_switch_deep = 64;
     int p_count_deep = 1;
+    // tile discharge (region mode and the hot sweep): up to this many
+    // workgroups per tile, dealt by the tile's excess score as the
+    // predecessor dealt blocks to regions; 1 = one workgroup per tile
+    int tile_split = 1;
 };
 
 // Wall-clock ticks a multi-workgroup BFS barrier may wait before it flags
@@ -209,6 +213,8 @@ struct Slab {
     int32_t* hl = nullptr;
     int32_t* tile_cnt = nullptr;
     int32_t* tile_hot = nullptr;
+    int64_t* tile_ex = nullptr;
+    int32_t* tsplit = nullptr;
     int32_t* lbkt = nullptr;
     int32_t* giant = nullptr;
     int64_t* gacc = nullptr;

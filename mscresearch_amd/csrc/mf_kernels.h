@@ -81,9 +81,10 @@ void launch_active_sparse(hipStream_t st, const SlotView* slots, const int* list
 void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* list, int nlist,
                            int gridx, int giants);
 // One workgroup of `threads` (256, 512 or 1024) per tile, tile heights and
-// pending excess staged in LDS.
+// pending excess staged in LDS; split > 1: up to `split` workgroups per tile
+// by excess score (k_tile_alloc), each staging its part of the tile.
 void launch_discharge_tile(hipStream_t st, const SlotView* slots, const int* list, int nlist,
-                           int gridx, int hot_only, int rounds, int giants, int threads);
+                           int gridx, int hot_only, int rounds, int giants, int threads, int split);
 // Owner-exclusive path push from the active list.
 void launch_path_push(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
                       int max_hops);

@@ -143,6 +143,8 @@ struct GSlot {
     GAS int32_t* hl;
     GAS int32_t* tile_cnt;
     GAS int32_t* tile_hot;
+    GAS int64_t* tile_ex;
+    GAS int32_t* tsplit;
     GAS int32_t* lbkt;
     GAS int32_t* giant;
     GAS int64_t* gacc;
@@ -189,6 +191,8 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.hl = as_global(v.hl);
     S.tile_cnt = as_global(v.tile_cnt);
     S.tile_hot = as_global(v.tile_hot);
+    S.tile_ex = as_global(v.tile_ex);
+    S.tsplit = as_global(v.tsplit);
     S.lbkt = as_global(v.lbkt);
     S.giant = as_global(v.giant);
     S.gacc = as_global(v.gacc);
@@ -347,7 +351,11 @@ __global__ void k_reset(const SlotView* __restrict__ slots, const int* __restric
         if (what & RST_ERR) c->err = 0;
     }
     if (what & (RST_ACTIVE | RST_SPARSE))
-        for (int i = threadIdx.x; i < S.g.ntiles; i += blockDim.x) { S.tile_cnt[i] = 0; S.tile_hot[i] = 0; }
+        for (int i = threadIdx.x; i < S.g.ntiles; i += blockDim.x) {
+            S.tile_cnt[i] = 0;
+            S.tile_hot[i] = 0;
+            S.tile_ex[i] = 0;
+        }
 }
 
 void launch_reset(hipStream_t st, const SlotView* slots, const int* list, int nlist, unsigned what) {
@@ -1597,7 +1605,11 @@ __global__ __launch_bounds__(256) void k_active(const SlotView* __restrict__ slo
         if (hv > S.H_cut) ++high;
         hmin = min(hmin, hv);
         hmax = max(hmax, hv);
-        add32(&S.tile_cnt[S.g.tile_of[v]], 1);
+        {
+            const int tl = S.g.tile_of[v];
+            add32(&S.tile_cnt[tl], 1);
+            add64(&S.tile_ex[tl], ex);
+        }
         const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
         if (deg <= kLightDeg) {
             stage_push(st, v, 0, S.fa, &S.ctr->nlight, (GAS int64_t*)nullptr);
@@ -1676,7 +1688,11 @@ __global__ __launch_bounds__(256) void k_active_sparse(const SlotView* __restric
         if (hv > S.H_cut) ++high;
         hmin = min(hmin, hv);
         hmax = max(hmax, hv);
-        add32(&S.tile_cnt[S.g.tile_of[v]], 1);
+        {
+            const int tl = S.g.tile_of[v];
+            add32(&S.tile_cnt[tl], 1);
+            add64(&S.tile_ex[tl], ex);
+        }
         const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
         if (deg <= kLightDeg) {
             stage_push(st, v, 0, S.fb, &S.ctr->nlight, (GAS int64_t*)nullptr);
@@ -2221,9 +2237,18 @@ void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* lis
 template <int NT>
 __global__ __launch_bounds__(NT) void k_discharge_tile(const SlotView* __restrict__ slots,
                                                        const int* __restrict__ list, int hot_only,
-                                                       int rounds) {
+                                                       int rounds, int split) {
     const GSlot S = load_slot(slots, list);
-    const int tile = blockIdx.x;
+    // split: the workgroup takes part `part` of `parts` of its tile's vertex
+    // range, as k_tile_alloc dealt them out by excess score; else one
+    // workgroup per tile
+    int tile = blockIdx.x, part = 0, parts = 1;
+    if (split) {
+        if ((int)blockIdx.x >= S.tsplit[0]) return;
+        tile = S.tsplit[1 + 3 * blockIdx.x];
+        part = S.tsplit[2 + 3 * blockIdx.x];
+        parts = S.tsplit[3 + 3 * blockIdx.x];
+    }
     if (tile >= S.g.ntiles) return;
     if (hot_only && S.tile_hot[tile] == 0) return;
     // no active vertex at the last count: nothing to do (vertices pushed into
@@ -2234,7 +2259,8 @@ __global__ __launch_bounds__(NT) void k_discharge_tile(const SlotView* __restric
     __shared__ int32_t alist[kTileMax];
     __shared__ int nlight, nheavy;
     __shared__ uint64_t red[NT / WAVE];
-    const int a = S.g.tile_start[tile], b = S.g.tile_start[tile + 1];
+    const int ta = S.g.tile_start[tile], tsz = S.g.tile_start[tile + 1] - ta;
+    const int a = ta + (int)((int64_t)tsz * part / parts), b = ta + (int)((int64_t)tsz * (part + 1) / parts);
     const int size = b - a;  // <= kTileMax by construction of tile_start
     const int n = S.g.n;
     for (int i = threadIdx.x; i < size; i += blockDim.x) { lh[i] = S.h[a + i]; ldx[i] = 0; }
@@ -2276,15 +2302,100 @@ __global__ __launch_bounds__(NT) void k_discharge_tile(const SlotView* __restric
     work_flush(S, wk, red, WF_DISCHARGE);
 }
 
+// Workgroups per tile by excess score (the predecessor's region blocks,
+// Region_Aware_PR_without_Pruning.cu:173-215 and :247-338): a tile with
+// active vertices (hot ones only, for the hot sweep) scores its active
+// vertices' positive excess plus (its active count / n) x the slot's
+// positive excess, and gets 1 + floor((kmax - 1) x score / top score)
+// workgroups, each a contiguous part of the tile's vertex range.  One
+// workgroup per slot writes the (tile, part, parts) records in tile order.
+__global__ __launch_bounds__(1024) void k_tile_alloc(const SlotView* __restrict__ slots,
+                                                     const int* __restrict__ list, int kmax, int hot_only) {
+    const GSlot S = load_slot(slots, list);
+    __shared__ uint64_t redu[1024 / WAVE];
+    __shared__ int wsum[1024 / WAVE];
+    __shared__ int carry;
+    const int T = S.g.ntiles;
+    const int n = S.g.n;
+    const int64_t pos = S.ctr->pos_excess > 0 ? S.ctr->pos_excess : 0;
+    auto score = [&](int i) -> int64_t {
+        const int c = S.tile_cnt[i];
+        if (c <= 0 || (hot_only && S.tile_hot[i] == 0)) return -1;
+        const double ratio = (double)c / (double)(n > 0 ? n : 1);
+        return S.tile_ex[i] + (int64_t)(ratio * (double)pos);
+    };
+    int64_t mx = 0;
+    for (int i = threadIdx.x; i < T; i += blockDim.x) mx = max(mx, score(i));
+    // block max through the unsigned sum's reduction slots
+    {
+        const int lane = lane_id();
+        for (int o = WAVE / 2; o > 0; o >>= 1) mx = max(mx, (int64_t)__shfl_down((long long)mx, o, WAVE));
+        if (lane == 0) redu[threadIdx.x / WAVE] = (uint64_t)mx;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t m = 0;
+            for (int w = 0; w < (int)(blockDim.x / WAVE); ++w) m = max(m, (int64_t)redu[w]);
+            redu[0] = (uint64_t)m;
+            carry = 0;
+        }
+        __syncthreads();
+        mx = (int64_t)redu[0];
+    }
+    for (int base = 0; base < T; base += blockDim.x) {
+        const int i = base + (int)threadIdx.x;
+        int b = 0;
+        if (i < T) {
+            const int64_t sc = score(i);
+            if (sc >= 0) {
+                const double f = mx > 0 ? (double)sc / (double)mx : 0.0;
+                b = 1 + (int)((double)(kmax - 1) * f);
+                b = min(b, kmax);
+            }
+        }
+        // block exclusive scan of b
+        int x = b;
+        const int lane = lane_id();
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const int y = __shfl_up(x, o, WAVE);
+            if (lane >= o) x += y;
+        }
+        const int w = threadIdx.x / WAVE;
+        if (lane == WAVE - 1) wsum[w] = x;
+        __syncthreads();
+        int before = carry, all = 0;
+        for (int j = 0; j < (int)(blockDim.x / WAVE); ++j) {
+            if (j < w) before += wsum[j];
+            all += wsum[j];
+        }
+        const int off = before + x - b;
+        for (int j = 0; j < b; ++j) {
+            S.tsplit[1 + 3 * (off + j)] = i;
+            S.tsplit[2 + 3 * (off + j)] = j;
+            S.tsplit[3 + 3 * (off + j)] = b;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) carry += all;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) S.tsplit[0] = carry;
+}
+
 void launch_discharge_tile(hipStream_t st, const SlotView* slots, const int* list, int nlist,
-                           int gridx, int hot_only, int rounds, int giants, int threads) {
+                           int gridx, int hot_only, int rounds, int giants, int threads, int split) {
+    int sp = 0;
+    if (split > 1) {
+        hipLaunchKernelGGL(k_tile_alloc, dim3(1, nlist), dim3(1024), 0, st, slots, list, std::min(split, kSplitMax),
+                           hot_only);
+        gridx *= std::min(split, kSplitMax);
+        sp = 1;
+    }
     if (threads <= 256)
-        hipLaunchKernelGGL(k_discharge_tile<256>, dim3(gridx, nlist), dim3(256), 0, st, slots, list, hot_only, rounds);
+        hipLaunchKernelGGL(k_discharge_tile<256>, dim3(gridx, nlist), dim3(256), 0, st, slots, list, hot_only, rounds, sp);
     else if (threads <= 512)
-        hipLaunchKernelGGL(k_discharge_tile<512>, dim3(gridx, nlist), dim3(512), 0, st, slots, list, hot_only, rounds);
+        hipLaunchKernelGGL(k_discharge_tile<512>, dim3(gridx, nlist), dim3(512), 0, st, slots, list, hot_only, rounds, sp);
     else
         hipLaunchKernelGGL(k_discharge_tile<1024>, dim3(gridx, nlist), dim3(1024), 0, st, slots, list, hot_only,
-                           rounds);
+                           rounds, sp);
     if (!hot_only && giants) launch_giant(st, slots, list, nlist);
 }
 

@@ -159,6 +159,7 @@ bool read_cache(const std::string& path, const std::string& key, TuneResult& r) 
         TuneResult x;
         if (!(in >> c >> x.R >> x.tile_max >> x.K >> x.gr_freq >> x.tile_threads >> x.slots >> x.ms >> x.ms_default))
             continue;
+        if (!(in >> x.tile_split)) x.tile_split = 1;
         if (c == key) { r = x; found = true; }  // the last entry of a key wins
     }
     return found;
@@ -187,8 +188,8 @@ int autotune(const HostGraph& g0, const std::vector<int32_t>& s, const std::vect
             return MF_OK;
         }
     }
-    struct Cand { int R, tile, K, gr, threads, slots; };
-    Cand best{R0, tile0, base.K, base.gr_freq, base.tile_threads, batch};
+    struct Cand { int R, tile, K, gr, threads, slots, split; };
+    Cand best{R0, tile0, base.K, base.gr_freq, base.tile_threads, batch, std::max(1, base.tile_split)};
     std::map<std::pair<int, int>, Setup> setups;
     auto setup_for = [&](int R, int tile, Setup** su) -> int {
         auto key2 = std::make_pair(R, tile);
@@ -217,6 +218,7 @@ int autotune(const HostGraph& g0, const std::vector<int32_t>& s, const std::vect
         P.gr_freq_min = std::min(c.gr, base.gr_freq_min);
         P.gr_freq_max = std::max(2 * c.gr, base.gr_freq_max);
         P.tile_threads = c.threads;
+        P.tile_split = c.split;
         std::vector<int64_t> flows;
         rc = measure(*su, P, c.slots, flows, ms);
         if (rc != MF_OK) return rc;
@@ -238,6 +240,7 @@ int autotune(const HostGraph& g0, const std::vector<int32_t>& s, const std::vect
     const std::vector<int> Ks = {0, 1, 2, 4, 8};
     const std::vector<int> grs = {64, 128, 256, 512};
     const std::vector<int> threads = {256, 512, 1024};
+    const std::vector<int> splits = {1, 2, 4, 8};
     std::vector<int> slots;
     if (batch > 0)
         for (int k : {batch / 2, batch, 2 * batch})
@@ -259,6 +262,7 @@ int autotune(const HostGraph& g0, const std::vector<int32_t>& s, const std::vect
     if ((rc = sweep([](Cand& c) -> int& { return c.K; }, Ks)) != MF_OK) return rc;
     if ((rc = sweep([](Cand& c) -> int& { return c.gr; }, grs)) != MF_OK) return rc;
     if ((rc = sweep([](Cand& c) -> int& { return c.threads; }, threads)) != MF_OK) return rc;
+    if ((rc = sweep([](Cand& c) -> int& { return c.split; }, splits)) != MF_OK) return rc;
     if ((rc = sweep([](Cand& c) -> int& { return c.slots; }, slots)) != MF_OK) return rc;
     out.R = best.R;
     out.tile_max = best.tile;
@@ -266,6 +270,7 @@ int autotune(const HostGraph& g0, const std::vector<int32_t>& s, const std::vect
     out.gr_freq = best.gr;
     out.tile_threads = best.threads;
     out.slots = best.slots;
+    out.tile_split = best.split;
     out.ms = best_ms;
     out.ms_default = default_ms;
     out.evaluations = evals;
@@ -273,7 +278,7 @@ int autotune(const HostGraph& g0, const std::vector<int32_t>& s, const std::vect
     if (!cache_path.empty()) {
         std::ofstream f(cache_path, std::ios::app);
         f << key << " " << out.R << " " << out.tile_max << " " << out.K << " " << out.gr_freq << " " << out.tile_threads
-          << " " << out.slots << " " << out.ms << " " << out.ms_default << "\n";
+          << " " << out.slots << " " << out.ms << " " << out.ms_default << " " << out.tile_split << "\n";
     }
     return MF_OK;
 }

@@ -4,8 +4,8 @@
         --cache mscresearch_amd/tuned_params.txt
 
 Runs mf_autotune (the library's coordinate descent over R, tile size, K,
-GR_FREQ, the tile discharge workgroup size and, with --batch, slots per
-engine) on the workload's first --queries queries, and appends the chosen
+GR_FREQ, the tile discharge workgroup size, workgroups per tile by excess
+score and, with --batch, slots per engine) on the workload's first --queries queries, and appends the chosen
 values for the graph's class to the cache.  The bench reads that file: a
 class found there is not timed again.
 """
@@ -27,8 +27,8 @@ DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_
 
 def cached(hg_unpartitioned, batch, cache=DEFAULT_CACHE):
     """The cache's entry for this graph's class and mode (the last one), or
-    None: {"R", "tile_max", "K", "gr_freq", "tile_threads", "slots", "ms_per_query",
-    "default_ms_per_query", "graph_class"}."""
+    None: {"R", "tile_max", "K", "gr_freq", "tile_threads", "slots", "tile_split",
+    "ms_per_query", "default_ms_per_query", "graph_class"}."""
     if not cache or not os.path.exists(cache):
         return None
     cls = hg_unpartitioned.graph_class()
@@ -37,13 +37,14 @@ def cached(hg_unpartitioned, batch, cache=DEFAULT_CACHE):
     with open(cache) as f:
         for line in f:
             p = line.split()
-            if len(p) == 9 and p[0] == key:
+            if len(p) in (9, 10) and p[0] == key:
                 found = p
     if found is None:
         return None
     names = ["R", "tile_max", "K", "gr_freq", "tile_threads", "slots"]
     out = {k: int(v) for k, v in zip(names, found[1:7])}
-    out.update(ms_per_query=float(found[7]), default_ms_per_query=float(found[8]), graph_class=cls, evaluations=0)
+    out.update(tile_split=int(found[9]) if len(found) == 10 else 1, ms_per_query=float(found[7]),
+               default_ms_per_query=float(found[8]), graph_class=cls, evaluations=0)
     return out
 
 

@@ -146,6 +146,9 @@ def test_edge_cases():
     dict(inject=1, cert=2, sparse_count=0),     # ... and entry-by-entry pair sums
     dict(tile_threads=256, tile_frac=0.0),      # tile discharge in 256-thread workgroups, every step
     dict(tile_threads=1024, tile_frac=0.0),     # ... and in 1024-thread workgroups
+    dict(tile_split=4, tile_frac=0.0),          # up to 4 workgroups per tile by excess score, every step
+    dict(tile_split=8, tile_threads=256, tile_frac=0.0, tile_rounds=1),  # ... 8 of 256 threads, one round each
+    dict(tile_split=2, sparse_count=1, hot_min_active=1, hot_k=4, tile_frac=0.0),  # ... and in the hot sweep
 ])
 def test_scheduler_variants_stay_exact(c1, variant):
     log(f"variant {variant}")

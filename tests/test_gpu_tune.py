@@ -23,13 +23,15 @@ def test_autotune_keeps_flows_and_caches(tmp_path):
     assert r["R"] in (128, 256, 512, 1024) and r["tile_max"] in (1024, 2048, 4096)
     assert r["K"] in (0, 1, 2, 4, 8) and r["gr_freq"] in (64, 128, 256, 512)
     assert r["tile_threads"] in (256, 512, 1024) and r["slots"] == 0
+    assert r["tile_split"] in (1, 2, 4, 8)
     assert r["ms_per_query"] <= r["default_ms_per_query"]
     assert r["graph_class"].startswith("n12-")
     # the tuned configuration solves exactly
     hg = M.HostGraph.from_edgelist(e).partition(r["R"])
     d = hg.export()
     og = O.Graph(len(d["new_id"]), d["row_ptr"], d["col"], d["cap"], d["rev"])
-    eng = M.Engine(1, M.default_params(K=r["K"], gr_freq=r["gr_freq"], tile_threads=r["tile_threads"]))
+    eng = M.Engine(1, M.default_params(K=r["K"], gr_freq=r["gr_freq"], tile_threads=r["tile_threads"],
+                                       tile_split=r["tile_split"]))
     gid = eng.add_graph(M.DeviceGraph(hg, r["tile_max"]))
     qs, qt = e.hub_queries(8, 7003)
     for a, b in zip(qs, qt):
@@ -38,8 +40,8 @@ def test_autotune_keeps_flows_and_caches(tmp_path):
     # same class again: from the cache, no timing
     r2 = M.autotune(M.HostGraph.from_edgelist(e), s, t, cache=cache)
     assert r2["evaluations"] == 0
-    assert (r2["R"], r2["tile_max"], r2["K"], r2["gr_freq"], r2["tile_threads"]) == \
-        (r["R"], r["tile_max"], r["K"], r["gr_freq"], r["tile_threads"])
+    assert (r2["R"], r2["tile_max"], r2["K"], r2["gr_freq"], r2["tile_threads"], r2["tile_split"]) == \
+        (r["R"], r["tile_max"], r["K"], r["gr_freq"], r["tile_threads"], r["tile_split"])
 
 
 def test_autotune_batched_engine(tmp_path):
@@ -61,7 +63,8 @@ def test_autotune_batched_engine(tmp_path):
     hg = M.HostGraph.from_edgelist(e).partition(rb["R"])
     d = hg.export()
     og = O.Graph(len(d["new_id"]), d["row_ptr"], d["col"], d["cap"], d["rev"])
-    eng = M.Engine(rb["slots"], M.default_params(K=rb["K"], gr_freq=rb["gr_freq"], tile_threads=rb["tile_threads"]))
+    eng = M.Engine(rb["slots"], M.default_params(K=rb["K"], gr_freq=rb["gr_freq"], tile_threads=rb["tile_threads"],
+                                                 tile_split=rb["tile_split"]))
     gid = eng.add_graph(M.DeviceGraph(hg, rb["tile_max"]))
     qs, qt = e.hub_queries(16, 7013)
     pairs = [(int(d["new_id"][a]), int(d["new_id"][b])) for a, b in zip(qs, qt)]
