@@ -117,6 +117,35 @@ int mf_hgraph_components(const mf_hgraph* h, int32_t* scc, int32_t* wcc);
 int mf_hgraph_core_decision(const mf_hgraph* h, int s, int t, int64_t* pruned_n, int64_t* run_n,
                             int64_t* core_kept, int* would_take);
 
+/* ----------------------------------------------- SCC contraction and routing
+ * Replaces the reference's offline tools SCC_CONTRACT/scc_contract.cpp
+ * (Kosaraju ids, scc_contract.cpp:44-105; contraction with capacities summed
+ * per component pair, :107-134), split_queries_by_scc.cpp:49-76 and
+ * convert_test_to_scc.cpp:47-60.  Component ids are the reference's, so
+ * maps, contracted graphs and split files match it byte for byte (bin/
+ * mscmf_scc writes them).  The contracted graph's flows are NOT exact: the
+ * inside of a component is taken as uncapacitated.  mf_scc_route is the
+ * exact use of the components. */
+typedef struct mf_scc mf_scc;
+/* From 0-based lines as the graph reader accepts them (out-of-range ids,
+ * self-loops and cap <= 0 dropped; line order matters for the ids). */
+int mf_scc_build(int n, int64_t m, const int32_t* us, const int32_t* vs, const int64_t* caps, mf_scc** out);
+/* From graph.txt, read as mf_hgraph_load reads it (scc_contract.cpp:18-40). */
+int mf_scc_load(const char* graph_path, mf_scc** out);
+int mf_scc_info(const mf_scc* h, int* n, int32_t* ncomp, int64_t* m_contracted);
+/* comp[n]: 0-based component id per vertex (the reference's .map is v+1, comp+1). */
+int mf_scc_comp(const mf_scc* h, int32_t* comp);
+/* The contracted graph, m_contracted entries sorted by (cu, cv), 0-based. */
+int mf_scc_edges(const mf_scc* h, int32_t* cu, int32_t* cv, int64_t* cap);
+/* Exact routing (0-based ids): route[i] = 0 when t is not reachable from s
+ * (max flow 0), 1 when s and t share a component C (every s-t path stays in
+ * C: solve on C's induced subgraph), 2 when t is reachable in another
+ * component (solve on the original graph), -1 for an id out of range or
+ * s == t.  The reference's split sends same-component queries to the
+ * original graph and the rest to the contracted one. */
+int mf_scc_route(const mf_scc* h, int nq, const int32_t* s, const int32_t* t, int32_t* route);
+void mf_scc_free(mf_scc* h);
+
 /* Stand-alone partition on given stats (BC:1130-1178), for parity tests. */
 int mf_partition(int n, int R, const int64_t* sum_out, const int64_t* sum_in,
                  const int32_t* deg_out, const int32_t* deg_in, int32_t* part);
@@ -197,6 +226,7 @@ typedef struct mf_params {
     int tile_threads;    /* threads of the tile (region) discharge workgroup: 256, 512 (BC:1725) or 1024 */
     int path_switch_deep, p_count_deep; /* path_switch and p_count0 once the last exact relabel is 64+ levels deep */
     int tile_split;      /* tile discharge: up to this many workgroups per tile by excess score (1..8, 1 = off) */
+    int scc_route;       /* 1: a query with s and t in one SCC runs on that SCC's induced subgraph (exact) */
 } mf_params;
 
 void mf_params_default(mf_params* p);

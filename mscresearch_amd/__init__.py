@@ -20,6 +20,7 @@ import numpy as np
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libmscmf.so")
 CLI_PATH = os.path.join(_PKG, "bin", "mscmf")
+SCC_CLI_PATH = os.path.join(_PKG, "bin", "mscmf_scc")
 
 MF_OK, MF_EINVAL, MF_ENOMEM, MF_EHIP, MF_EIO, MF_EITER, MF_ECERT, MF_ENODEV, MF_EBUSY, MF_ESTALE = (
     0, -1, -2, -3, -4, -5, -6, -7, -8, -9)
@@ -48,7 +49,7 @@ class Params(C.Structure):
         ("spec_cert", C.c_int), ("sparse_count", C.c_int), ("small_wgs", C.c_int),
         ("small_edges", C.c_longlong), ("mark_list", C.c_int), ("inject", C.c_int),
         ("tile_threads", C.c_int), ("path_switch_deep", C.c_int), ("p_count_deep", C.c_int),
-        ("tile_split", C.c_int),
+        ("tile_split", C.c_int), ("scc_route", C.c_int),
     ]
 
 
@@ -117,6 +118,13 @@ SIGNATURES = {
     "mf_hgraph_core_decision": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                           C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "mf_partition": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _P]),
+    "mf_scc_build": (C.c_int, [C.c_int, C.c_int64, _P, _P, _P, C.POINTER(_P)]),
+    "mf_scc_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "mf_scc_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    "mf_scc_comp": (C.c_int, [_P, _P]),
+    "mf_scc_edges": (C.c_int, [_P, _P, _P, _P]),
+    "mf_scc_route": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "mf_scc_free": (None, [_P]),
     "mf_read_queries": (C.c_int, [C.c_char_p, C.c_int64, _P, _P, C.POINTER(C.c_int64)]),
     "mf_gen_uniform": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_uint64, C.POINTER(_P)]),
     "mf_gen_grid": (C.c_int, [C.c_int, C.c_int, C.c_int64, C.c_uint64, C.POINTER(_P)]),
@@ -264,10 +272,10 @@ class Edges:
 
 def autotune(hg_unpartitioned, s, t, params=None, R=128, tile=4096, cache=None, batch=0) -> dict:
     """Pick R, tile size, K, GR_FREQ, the tile workgroup size, workgroups per
-    tile by excess score and (batch > 0) slots per engine for this graph's class by timing the plain solver
-    (batch = 0) or the batched engine on sample queries (ids of the
-    unpartitioned graph); every configuration must give the same flows.
-    Returns the chosen values."""
+    tile by excess score and (batch > 0) slots per engine for this graph's
+    class by timing the plain solver (batch = 0) or the batched engine on
+    sample queries (ids of the unpartitioned graph); every configuration must
+    give the same flows.  Returns the chosen values."""
     s = np.ascontiguousarray(s, np.int32)
     t = np.ascontiguousarray(t, np.int32)
     p = params if params is not None else default_params()
@@ -335,6 +343,60 @@ def partition(R, so, si, dout, din):
     dout, din = np.ascontiguousarray(dout, np.int32), np.ascontiguousarray(din, np.int32)
     _check(lib().mf_partition(n, R, _ptr(so), _ptr(si), _ptr(dout), _ptr(din), _ptr(part)))
     return part
+
+
+class SccContract:
+    """Strongly connected components of input lines in the reference's ids
+    (SCC_CONTRACT/scc_contract.cpp), the contracted graph and exact routing.
+
+    ``comp`` holds 0-based ids (the reference's .map is v+1, comp+1); the
+    contracted graph (``edges()``) sums capacities per component pair and is
+    not flow-exact.  ``route(s, t)``: 0 = t unreachable from s (max flow 0),
+    1 = same component (solve on it alone), 2 = other component, reachable,
+    -1 = bad query."""
+
+    def __init__(self, h):
+        self._h = h
+        n, c, m = C.c_int(0), C.c_int32(0), C.c_int64(0)
+        _check(lib().mf_scc_info(h, C.byref(n), C.byref(c), C.byref(m)))
+        self.n, self.ncomp, self.m = n.value, c.value, m.value
+
+    @classmethod
+    def from_lines(cls, n, us, vs, caps):
+        us = np.ascontiguousarray(us, np.int32)
+        vs = np.ascontiguousarray(vs, np.int32)
+        caps = np.ascontiguousarray(caps, np.int64)
+        h = _P()
+        _check(lib().mf_scc_build(int(n), len(us), _ptr(us), _ptr(vs), _ptr(caps), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load(cls, path):
+        h = _P()
+        _check(lib().mf_scc_load(str(path).encode(), C.byref(h)))
+        return cls(h)
+
+    def comp(self):
+        out = np.empty(self.n, np.int32)
+        _check(lib().mf_scc_comp(self._h, _ptr(out)))
+        return out
+
+    def edges(self):
+        cu, cv, cap = np.empty(self.m, np.int32), np.empty(self.m, np.int32), np.empty(self.m, np.int64)
+        _check(lib().mf_scc_edges(self._h, _ptr(cu), _ptr(cv), _ptr(cap)))
+        return cu, cv, cap
+
+    def route(self, s, t):
+        s = np.ascontiguousarray(s, np.int32)
+        t = np.ascontiguousarray(t, np.int32)
+        out = np.empty(len(s), np.int32)
+        _check(lib().mf_scc_route(self._h, len(s), _ptr(s), _ptr(t), _ptr(out)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mf_scc_free(self._h)
+            self._h = None
 
 
 class HostGraph:

@@ -111,6 +111,7 @@ Params from_c(const mf_params* p) {
     P.path_switch_deep = p->path_switch_deep;
     P.p_count_deep = p->p_count_deep;
     P.tile_split = p->tile_split;
+    P.scc_route = p->scc_route;
     return P;
 }
 
@@ -498,6 +499,7 @@ void mf_params_default(mf_params* p) {
     p->path_switch_deep = P.path_switch_deep;
     p->p_count_deep = P.p_count_deep;
     p->tile_split = P.tile_split;
+    p->scc_route = P.scc_route;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {
@@ -704,6 +706,58 @@ int mf_autotune(const mf_hgraph* h, int nq, const int32_t* s, const int32_t* t, 
     std::snprintf(out->graph_class, sizeof(out->graph_class), "%s", r.cls.c_str());
     return MF_OK;
 }
+
+int mf_scc_build(int n, int64_t m, const int32_t* us, const int32_t* vs, const int64_t* caps, mf_scc** out) {
+    if (n < 0 || m < 0 || !out || (m > 0 && (!us || !vs || !caps))) return fail(MF_EINVAL, "bad argument");
+    for (int64_t i = 0; i < m; ++i)
+        if (us[i] < 0 || us[i] >= n || vs[i] < 0 || vs[i] >= n || us[i] == vs[i] || caps[i] <= 0)
+            return fail(MF_EINVAL, "line " + std::to_string(i) + " out of range, self-loop or cap <= 0");
+    mf_scc* h = new (std::nothrow) mf_scc();
+    if (!h) return fail(MF_ENOMEM, "out of host memory");
+    scc_contract(n, m, us, vs, caps, h->c);
+    *out = h;
+    return MF_OK;
+}
+
+int mf_scc_load(const char* graph_path, mf_scc** out) {
+    if (!graph_path || !out) return fail(MF_EINVAL, "null argument");
+    std::vector<int32_t> us, vs;
+    std::vector<int64_t> caps;
+    int n = 0;
+    std::string err;
+    if (parse_graph_file(graph_path, us, vs, caps, n, err) != 0) return fail(MF_EIO, err);
+    return mf_scc_build(n, (int64_t)us.size(), us.data(), vs.data(), caps.data(), out);
+}
+
+int mf_scc_info(const mf_scc* h, int* n, int32_t* ncomp, int64_t* m_contracted) {
+    if (!h) return fail(MF_EINVAL, "null handle");
+    if (n) *n = h->c.n;
+    if (ncomp) *ncomp = h->c.ncomp;
+    if (m_contracted) *m_contracted = (int64_t)h->c.cu.size();
+    return MF_OK;
+}
+
+int mf_scc_comp(const mf_scc* h, int32_t* comp) {
+    if (!h || (!comp && h->c.n > 0)) return fail(MF_EINVAL, "bad argument");
+    std::copy(h->c.comp.begin(), h->c.comp.end(), comp);
+    return MF_OK;
+}
+
+int mf_scc_edges(const mf_scc* h, int32_t* cu, int32_t* cv, int64_t* cap) {
+    if (!h) return fail(MF_EINVAL, "null handle");
+    if (cu) std::copy(h->c.cu.begin(), h->c.cu.end(), cu);
+    if (cv) std::copy(h->c.cv.begin(), h->c.cv.end(), cv);
+    if (cap) std::copy(h->c.ccap.begin(), h->c.ccap.end(), cap);
+    return MF_OK;
+}
+
+int mf_scc_route(const mf_scc* h, int nq, const int32_t* s, const int32_t* t, int32_t* route) {
+    if (!h || nq < 0 || (nq > 0 && (!s || !t || !route))) return fail(MF_EINVAL, "bad argument");
+    scc_route(h->c, nq, s, t, route);
+    return MF_OK;
+}
+
+void mf_scc_free(mf_scc* h) { delete h; }
 
 int mf_graph_class(const mf_hgraph* h, char* out, int cap) {
     if (!h || !out || cap < 1) return fail(MF_EINVAL, "bad argument");

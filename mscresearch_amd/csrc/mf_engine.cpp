@@ -68,6 +68,7 @@ DeviceGraph::~DeviceGraph() {
     (void)hipFree(d_tile_start); (void)hipFree(d_tile_of);
     (void)hipFree(d_sub2old); (void)hipFree(d_old2sub); (void)hipFree(d_mask);
     (void)hipFree(d_wcc);
+    (void)hipFree(d_scc);
     (void)hipFree(d_capsum); (void)hipFree(d_cert_fp); (void)hipFree(d_pcap);
     (void)hipFree(d_fstart);
 }
@@ -139,8 +140,12 @@ int upload_graph(const HostGraph& hg, int tile_max, DeviceGraph& dg) {
     weak_components(hg, dg.wcc);
     HIPCHK(hipMalloc(&dg.d_wcc, sizeof(int32_t) * std::max(n, 1)));
     if (n > 0) HIPCHK(hipMemcpy(dg.d_wcc, dg.wcc.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&dg.d_scc, sizeof(int32_t) * std::max(n, 1)));
+    if (n > 0) HIPCHK(hipMemcpy(dg.d_scc, dg.scc.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    dg.scc_size.assign(n, 0);
+    for (int v = 0; v < n; ++v) ++dg.scc_size[dg.scc[v]];
     dg.classes = true;
-    dg.bytes += sizeof(int32_t) * (size_t)n;
+    dg.bytes += 2 * sizeof(int32_t) * (size_t)n;
     dg.fwd_bits.assign(((size_t)E + 63) / 64, 0);
     for (int64_t e = 0; e < E; ++e)
         if (hg.cap[e] > 0) dg.fwd_bits[(size_t)e >> 6] |= 1ull << (e & 63);
@@ -865,10 +870,11 @@ void Engine::run_dims(const Query& q, int* n, int64_t* E, int* ntiles) const {
     *ntiles = G.g.ntiles;
     if (!P_.prune || !G.classes) return;
     const int cls = G.scc[q.s];
-    auto fwd = G.fwd_find(cls);
-    if (!fwd || !fwd->mask[q.t]) return;
-    const std::string key = std::to_string(q.graph) + ":c" + std::to_string(cls) + ":" + std::to_string(G.wcc[q.t]);
-    auto ct = cache_.find(key);
+    if (!intra_route(G, q)) {
+        auto fwd = G.fwd_find(cls);
+        if (!fwd || !fwd->mask[q.t]) return;
+    }
+    auto ct = cache_.find(route_key(G, q));
     if (ct == cache_.end() || !ct->second.g || ct->second.version != G.cap_version.load()) return;
     const DGraph& r = ct->second.g->g;
     *n = r.n;
@@ -921,8 +927,9 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
     S.sched.stats.t_start = now();
     v.K = S.sched.K();
     if (prune && G.classes) {
-        // with the forward set of s's SCC at hand the prune needs no BFS
-        if (G.fwd_find(G.scc[q.s])) return class_decide(i);
+        // with the forward set of s's SCC at hand (or s and t in one SCC
+        // under SCC routing) the prune needs no BFS
+        if (intra_route(G, q) || G.fwd_find(G.scc[q.s])) return class_decide(i);
         S.sched.set_fwd_only();
     }
     return MF_OK;
@@ -974,21 +981,33 @@ int Engine::dbg_class_mask(int gid, int s, int t, int32_t* mask, int64_t* kept, 
     return MF_OK;
 }
 
+bool Engine::intra_route(const DeviceGraph& G, const Query& q) const {
+    return P_.scc_route != 0 && G.scc[q.s] == G.scc[q.t];
+}
+
+std::string Engine::route_key(const DeviceGraph& G, const Query& q) const {
+    if (intra_route(G, q)) return std::to_string(q.graph) + ":i" + std::to_string(G.scc[q.s]);
+    return std::to_string(q.graph) + ":c" + std::to_string(G.scc[q.s]) + ":" + std::to_string(G.wcc[q.t]);
+}
+
 int Engine::class_decide(int i) {
     SlotState& S = slots_[i];
     DeviceGraph& G = *graphs_[S.q.graph];
     const int cls = G.scc[S.q.s];
+    const bool intra = intra_route(G, S.q);
     std::shared_ptr<DeviceGraph::FwdSet> fwd;
-    // after S_PRUNE_FWD (when the class was not cached) the slot's h holds
-    // the forward BFS from s
-    int frc = fwd_set(G, cls, S.view.h, fwd);
-    if (frc != MF_OK) return frc;
-    if (!fwd->mask[S.q.t]) {
-        S.sched.finish_trivial(0);
-        return MF_OK;
+    if (!intra) {
+        // after S_PRUNE_FWD (when the class was not cached) the slot's h
+        // holds the forward BFS from s
+        int frc = fwd_set(G, cls, S.view.h, fwd);
+        if (frc != MF_OK) return frc;
+        if (!fwd->mask[S.q.t]) {
+            S.sched.finish_trivial(0);
+            return MF_OK;
+        }
     }
     const int wt = G.wcc[S.q.t];
-    const std::string key = std::to_string(S.q.graph) + ":c" + std::to_string(cls) + ":" + std::to_string(wt);
+    const std::string key = route_key(G, S.q);
     std::shared_ptr<DeviceGraph> run;
     bool known = false;
     auto ct = cache_.find(key);
@@ -1010,10 +1029,12 @@ int Engine::class_decide(int i) {
     if (!known) {
         ++cache_misses;
         const int n = G.g.n;
-        kept = class_kept(G, *fwd, wt);
+        kept = intra ? G.scc_size[cls] : class_kept(G, *fwd, wt);
         S.sched.stats.kept = (int)kept;
         if (kept > 0 && (double)kept < P_.prune_ratio * (double)n) {
-            launch_class_mask(stream_, fwd->d_mask, G.d_wcc, wt, n, S.view.aux);
+            // an SCC is inside its own forward set: the intra mask is the SCC
+            if (intra) launch_eq_mask(stream_, G.d_scc, cls, n, S.view.aux);
+            else launch_class_mask(stream_, fwd->d_mask, G.d_wcc, wt, n, S.view.aux);
             auto ng = std::make_shared<DeviceGraph>();
             int rc = build_induced_gpu(G, S.view.aux, (int)kept, stream_, *ng);
             if (rc != MF_OK) return rc;

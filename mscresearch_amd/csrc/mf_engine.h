@@ -74,6 +74,11 @@ struct Params {
     double bu_alpha = 7.0, bu_beta = 24.0;
     int prune = 1;             // per-query reachability pruning (BC:2678)
     double prune_ratio = 0.995;  // BC:2647
+    // exact SCC routing (SCC_CONTRACT/split_queries_by_scc.cpp:49-76 made
+    // exact): a query whose s and t share a strongly connected component C
+    // runs on C's induced subgraph (every s-t path stays inside C); others
+    // keep the reachability mask.  0 = the reference's mask for every query.
+    int scc_route = 0;
     // also GR once the discharge has scanned gr_work x E entries since the
     // last GR (a GR costs about E scans; 0 = off).  On R-MAT 22 hub queries
     // 0.125 beat 0.0625, 0.25, 0.5, 1 and 2 (hub rows are rescanned between
@@ -161,7 +166,9 @@ struct DeviceGraph {
     // each SCC once some query has needed it.
     bool classes = false;
     std::vector<int32_t> scc, wcc;
+    std::vector<int32_t> scc_size;  // vertices per SCC id
     int32_t* d_wcc = nullptr;
+    int32_t* d_scc = nullptr;
     int64_t* d_capsum = nullptr;   // certificate constants (DGraph::capsum, cert_fp)
     uint64_t* d_cert_fp = nullptr;
     int64_t* d_pcap = nullptr;     // pair capacities (DGraph::pcap)
@@ -191,6 +198,8 @@ void launch_fwd_from_h(hipStream_t st, const int32_t* h, int n, uint8_t* mask8);
 // mask[v] = fwd[v] && wcc[v] == wt  (int32 0/1)
 void launch_class_mask(hipStream_t st, const uint8_t* fwd, const int32_t* wcc, int32_t wt, int n,
                        int32_t* mask);
+// mask[v] = ids[v] == want  (int32 0/1)
+void launch_eq_mask(hipStream_t st, const int32_t* ids, int32_t want, int n, int32_t* mask);
 
 // Tiles: each region [rstart[r], rstart[r+1]) cut into pieces of at most tile_max.
 void make_tiles(const std::vector<int32_t>& rstart, int tile_max, std::vector<int32_t>& tile_start);
@@ -484,6 +493,10 @@ private:
     int make_arena();
     int prune_decide(int slot);
     int class_decide(int slot);
+    // SCC routing: s and t share an SCC (the query runs on that SCC)
+    bool intra_route(const DeviceGraph& G, const Query& q) const;
+    // cache key of the subgraph q runs on: its SCC, or its reachability class
+    std::string route_key(const DeviceGraph& G, const Query& q) const;
     // The forward set of SCC cls: cached, or made from d_h holding a BFS_FWD
     // from a vertex of cls (then cached).
     int fwd_set(DeviceGraph& G, int cls, const int32_t* d_h, std::shared_ptr<DeviceGraph::FwdSet>& out);

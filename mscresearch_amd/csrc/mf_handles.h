@@ -7,6 +7,7 @@
 #include "../../include/mscmf.h"
 #include "host_graph.h"
 #include "mf_engine.h"
+#include "scc.h"
 
 struct mf_hgraph {
     mf::HostGraph g;
@@ -18,6 +19,9 @@ struct mf_edges {
 };
 struct mf_graph {
     std::shared_ptr<mf::DeviceGraph> dg;
+};
+struct mf_scc {
+    mf::SccContract c;
 };
 struct mf_engine {
     mf::Engine eng;

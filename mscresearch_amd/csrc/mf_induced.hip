@@ -130,6 +130,11 @@ __global__ void k_class_mask(const uint8_t* __restrict__ fwd, const int32_t* __r
         mask[v] = (fwd[v] && wcc[v] == wt) ? 1 : 0;
 }
 
+__global__ void k_eq_mask(const int32_t* __restrict__ ids, int32_t want, int n, int32_t* __restrict__ mask) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+        mask[v] = ids[v] == want ? 1 : 0;
+}
+
 inline int grid_for(int64_t items, int per_block) {
     int64_t g = (items + per_block - 1) / per_block;
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, 16384));
@@ -153,6 +158,10 @@ void launch_fwd_from_h(hipStream_t st, const int32_t* h, int n, uint8_t* mask8) 
 void launch_class_mask(hipStream_t st, const uint8_t* fwd, const int32_t* wcc, int32_t wt, int n,
                        int32_t* mask) {
     hipLaunchKernelGGL(k_class_mask, dim3(grid_for(n, 256)), dim3(256), 0, st, fwd, wcc, wt, n, mask);
+}
+
+void launch_eq_mask(hipStream_t st, const int32_t* ids, int32_t want, int n, int32_t* mask) {
+    hipLaunchKernelGGL(k_eq_mask, dim3(grid_for(n, 256)), dim3(256), 0, st, ids, want, n, mask);
 }
 
 int build_induced_gpu(const DeviceGraph& P, const int32_t* d_mask, int kept, hipStream_t st,

@@ -60,6 +60,10 @@ def lib():
         L.orc_dinic_many.restype = C.c_double
         L.orc_dinic_many_ex.argtypes = [C.c_int, P, P, P, P, C.c_int, P, P, C.c_int, C.c_int, P]
         L.orc_dinic_many_ex.restype = C.c_double
+        L.orc_scc_kosaraju.argtypes = [C.c_int, C.c_int64, P, P, P]
+        L.orc_scc_kosaraju.restype = C.c_int
+        L.orc_scc_contract.argtypes = [C.c_int64, P, P, P, P, P, P, P]
+        L.orc_scc_contract.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -235,3 +239,20 @@ def maxflow_many(g: Graph, s, t, threads, prune=False):
     sec = lib().orc_dinic_many_ex(g.n, _p(g.row_ptr), _p(g.col), _p(g.cap), _p(g.rev), len(s), _p(s), _p(t),
                                   int(threads), 1 if prune else 0, _p(flows))
     return flows, sec
+
+
+def scc_kosaraju(n, us, vs):
+    """SCC_CONTRACT/scc_contract.cpp:44-105 restated: (component ids, count)."""
+    us, vs = _c(us, np.int32), _c(vs, np.int32)
+    comp = np.empty(n, np.int32)
+    C_ = lib().orc_scc_kosaraju(int(n), len(us), _p(us), _p(vs), _p(comp))
+    return comp, C_
+
+
+def scc_contract(us, vs, caps, comp):
+    """scc_contract.cpp:107-125 restated: (cu, cv, cap) sorted by (cu, cv)."""
+    us, vs, caps = _c(us, np.int32), _c(vs, np.int32), _c(caps, np.int64)
+    m = len(us)
+    cu, cv, cc = np.empty(max(m, 1), np.int32), np.empty(max(m, 1), np.int32), np.empty(max(m, 1), np.int64)
+    k = lib().orc_scc_contract(m, _p(us), _p(vs), _p(caps), _p(_c(comp, np.int32)), _p(cu), _p(cv), _p(cc))
+    return cu[:k].copy(), cv[:k].copy(), cc[:k].copy()

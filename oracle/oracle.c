@@ -652,3 +652,116 @@ double orc_dinic_many(int n, const int32_t* row_ptr, const int32_t* col, const i
                       int threads, int64_t* flows) {
     return orc_dinic_many_ex(n, row_ptr, col, cap, rev, nq, s, t, threads, 0, flows);
 }
+
+/* ------------------------------------------------------- SCC contraction */
+
+/* SCC_CONTRACT/scc_contract.cpp:31-105 restated (Kosaraju).  Adjacency in
+ * line order.  Pass 1 (:44-81): for r = 0..n-1 not yet seen, a depth-first
+ * search over out-lines; a vertex is appended to `order` once every one of its
+ * out-lines has been scanned.  Pass 2 (:83-103): walking `order` from its last
+ * vertex to its first, a vertex without an id opens the next id, which then
+ * spreads over in-lines to every vertex without one. */
+int orc_scc_kosaraju(int n, int64_t cnt, const int32_t* us, const int32_t* vs, int32_t* comp) {
+    int64_t* op = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+    int64_t* ip = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+    int32_t* oa = (int32_t*)malloc(sizeof(int32_t) * (size_t)(cnt > 0 ? cnt : 1));
+    int32_t* ia = (int32_t*)malloc(sizeof(int32_t) * (size_t)(cnt > 0 ? cnt : 1));
+    for (int64_t i = 0; i < cnt; ++i) { op[us[i] + 1]++; ip[vs[i] + 1]++; }
+    for (int v = 0; v < n; ++v) { op[v + 1] += op[v]; ip[v + 1] += ip[v]; }
+    {
+        int64_t* fo = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+        int64_t* fi = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+        for (int v = 0; v < n; ++v) { fo[v] = op[v]; fi[v] = ip[v]; }
+        for (int64_t i = 0; i < cnt; ++i) { oa[fo[us[i]]++] = vs[i]; ia[fi[vs[i]]++] = us[i]; }
+        free(fo);
+        free(fi);
+    }
+    char* seen = (char*)calloc((size_t)n + 1, 1);
+    int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int32_t* sv = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int64_t* sk = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    int no = 0;
+    for (int r = 0; r < n; ++r) {
+        if (seen[r]) continue;
+        int top = 0;
+        seen[r] = 1;
+        sv[0] = r;
+        sk[0] = op[r];
+        while (top >= 0) {
+            int x = sv[top];
+            if (sk[top] < op[x + 1]) {
+                int y = oa[sk[top]++];
+                if (!seen[y]) {
+                    seen[y] = 1;
+                    ++top;
+                    sv[top] = y;
+                    sk[top] = op[y];
+                }
+            } else {
+                order[no++] = x;
+                --top;
+            }
+        }
+    }
+    for (int v = 0; v < n; ++v) comp[v] = -1;
+    int C = 0;
+    for (int i = no - 1; i >= 0; --i) {
+        int r = order[i];
+        if (comp[r] != -1) continue;
+        int sp = 0;
+        comp[r] = C;
+        sv[sp++] = r;  /* each vertex is pushed once, when it gets its id */
+        while (sp > 0) {
+            int x = sv[--sp];
+            for (int64_t k = ip[x]; k < ip[x + 1]; ++k) {
+                int y = ia[k];
+                if (comp[y] == -1) {
+                    comp[y] = C;
+                    sv[sp++] = y;
+                }
+            }
+        }
+        ++C;
+    }
+    free(op); free(ip); free(oa); free(ia); free(seen); free(order); free(sv); free(sk);
+    return C;
+}
+
+typedef struct { int32_t a, b; int64_t c; } scc_trip;
+
+static int trip_cmp(const void* x, const void* y) {
+    const scc_trip* p = (const scc_trip*)x;
+    const scc_trip* q = (const scc_trip*)y;
+    if (p->a != q->a) return p->a < q->a ? -1 : 1;
+    if (p->b != q->b) return p->b < q->b ? -1 : 1;
+    return 0;
+}
+
+/* scc_contract.cpp:107-125 restated: every line between two components adds
+ * its capacity to the entry (comp[u], comp[v]); entries in (cu, cv) order. */
+int64_t orc_scc_contract(int64_t cnt, const int32_t* us, const int32_t* vs, const int64_t* caps,
+                         const int32_t* comp, int32_t* cu, int32_t* cv, int64_t* ccap) {
+    scc_trip* tr = (scc_trip*)malloc(sizeof(scc_trip) * (size_t)(cnt > 0 ? cnt : 1));
+    int64_t k = 0;
+    for (int64_t i = 0; i < cnt; ++i) {
+        if (comp[us[i]] == comp[vs[i]]) continue;
+        tr[k].a = comp[us[i]];
+        tr[k].b = comp[vs[i]];
+        tr[k].c = caps[i];
+        ++k;
+    }
+    qsort(tr, (size_t)k, sizeof(scc_trip), trip_cmp);
+    int64_t m = 0;
+    for (int64_t i = 0; i < k; ++i) {
+        if (m > 0 && cu[m - 1] == tr[i].a && cv[m - 1] == tr[i].b) {
+            ccap[m - 1] += tr[i].c;
+        } else {
+            cu[m] = tr[i].a;
+            cv[m] = tr[i].b;
+            ccap[m] = tr[i].c;
+            ++m;
+        }
+    }
+    free(tr);
+    return m;
+}

@@ -20,6 +20,11 @@
  *   orc_dinic            networkx / scipy golden fixtures, closed forms,
  *                        brute-force min cut (2^n subsets) on tiny graphs
  *   orc_certify          planted mistakes (tests corrupt a residual on the CPU)
+ *   orc_scc_kosaraju     the reference tool built from its own source
+ *                        (oracle/_ref/scc_contract, byte-identical .map),
+ *                        scipy's strong components (same partition), ids in
+ *                        a topological order of the condensation
+ *   orc_scc_contract     the reference tool's graph_scc.txt, brute-force sums
  */
 #ifndef MSC_ORACLE_H
 #define MSC_ORACLE_H
@@ -132,6 +137,14 @@ double orc_dinic_many(int n, const int32_t* row_ptr, const int32_t* col, const i
 double orc_dinic_many_ex(int n, const int32_t* row_ptr, const int32_t* col, const int64_t* cap,
                          const int32_t* rev, int nq, const int32_t* s, const int32_t* t,
                          int threads, int prune, int64_t* flows);
+
+/* SCC tools (SCC_CONTRACT/scc_contract.cpp).  Kosaraju component ids of the
+ * accepted lines (line order kept; it fixes the ids); returns the count. */
+int orc_scc_kosaraju(int n, int64_t cnt, const int32_t* us, const int32_t* vs, int32_t* comp);
+/* Lines between different components summed per (comp[u], comp[v]), sorted
+ * by that pair; cu/cv/ccap hold up to cnt entries; returns the entry count. */
+int64_t orc_scc_contract(int64_t cnt, const int32_t* us, const int32_t* vs, const int64_t* caps,
+                         const int32_t* comp, int32_t* cu, int32_t* cv, int64_t* ccap);
 
 #ifdef __cplusplus
 }
