@@ -175,13 +175,15 @@ def headline(args, out):
         e.add_graph(c3.dg)
         engs.append(e)
     bad_total = checked = 0
-    if args.profile_all:
-        for e in engs:
-            e.profile(True)
     for w in range(args.warmup):
         res = pass_over(engs, c3.s, c3.t)
         b, c = mismatches(res, c3.flows)
         bad_total, checked = bad_total + b, checked + c
+    # the timed and profiled passes lie between two trace markers
+    M.trace_marker()
+    if args.profile_all:
+        for e in engs:
+            e.profile(True)
     t0 = time.perf_counter()
     all_res = []
     for k in range(args.steps):
@@ -208,6 +210,7 @@ def headline(args, out):
     b, c = mismatches(pres, c3.flows)
     bad_total, checked = bad_total + b, checked + c
     fams = family_table(engs)
+    M.trace_marker()
     for e in engs:
         e.profile(False)
     log("family ms: " + ", ".join(f"{k}={v['ms']:.1f}/{v['launches']}" for k, v in fams.items()))
@@ -231,12 +234,13 @@ def headline(args, out):
     plain_qps = kq / plain_wall if kq else None
     bad_total += plain_bad
     checked += kq if c3.flows is not None else 0
-    log(f"plain: {kq} queries in {plain_wall:.3f}s = {plain_qps:.1f} q/s, p50 {pct(plain_lat, .5) * 1e3:.3f} ms")
+    if kq:
+        log(f"plain: {kq} queries in {plain_wall:.3f}s = {plain_qps:.1f} q/s, p50 {pct(plain_lat, .5) * 1e3:.3f} ms")
     del peng
 
     # open-loop Poisson sweep on R-MAT 22 alone, offered at fractions of the batched rate
     sweep = None
-    if not args.no_sweep:
+    if not args.no_sweep and kq:
         sweep = {"offered_fractions": args.sweep, "plain_p50_ms": round(pct(plain_lat, .5) * 1e3, 3),
                  "target_p99_ms": round(2 * pct(plain_lat, .5) * 1e3, 3), "levels": []}
         engs = []
@@ -305,20 +309,34 @@ def headline(args, out):
 def grid_record(args):
     rec = {"workload": "grid1024x1024_left_right", "queries": 0, "per_R": []}
     bad_total = checked = 0
-    for R in (128, 512):
+    for R in args.grid_R:
         g = Workload("c2", R)
         eng = M.Engine(args.grid_slots, M.default_params(fuse=32))
         gid = eng.add_graph(g.dg)
         q = min(args.grid_queries, len(g.s))
-        eng.profile(True)
+        if args.profile_all:
+            M.trace_marker()
+            eng.profile(True)
         t0 = time.perf_counter()
         for i in range(q):
             eng.submit(gid, g.s[i], g.t[i], i)
         res = eng.run()
         wall = time.perf_counter() - t0
-        fams = family_table([eng])
         b, c = mismatches(res, g.flows)
         bad_total, checked = bad_total + b, checked + c
+        if not args.profile_all:
+            # kernel families from a profiled pass over the first queries
+            M.trace_marker()
+            eng.profile(True)
+            pq = min(q, args.grid_slots)
+            for i in range(pq):
+                eng.submit(gid, g.s[i], g.t[i], i)
+            pres = eng.run()
+            b, c = mismatches(pres, g.flows)
+            bad_total, checked = bad_total + b, checked + c
+        fams = family_table([eng])
+        M.trace_marker()
+        eng.profile(False)
         lat = [r.t_end - r.t_arrive for r in res]
         e_run = sum(r.E_run for r in res)
         roof = roofline_of(fams, "grid1024x1024_left_right")
@@ -514,6 +532,7 @@ def main():
     ap.add_argument("--sweep-seconds", type=float, default=2.0)
     ap.add_argument("--grid-queries", type=int, default=256)
     ap.add_argument("--grid-slots", type=int, default=64)
+    ap.add_argument("--grid-R", type=int, nargs="+", default=[128, 512], help="region counts of the c2 sub-record")
     ap.add_argument("--serve-qps", type=float, nargs="+", default=[50.0, 200.0, 800.0])
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-max", type=int, default=1200)
@@ -530,7 +549,8 @@ def main():
     ap.add_argument("--only-headline", action="store_true", help="skip every sub-record and the CPU baseline")
     ap.add_argument("--only-grid", action="store_true", help="only the c2 sub-record (for its profiles)")
     ap.add_argument("--profile-all", action="store_true",
-                    help="kernel-family timing over warm-up and timed passes too (counter runs)")
+                    help="kernel-family timing over the timed passes too (counter runs: the families then "
+                         "cover every dispatch between the run's trace markers)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()

@@ -173,6 +173,10 @@ int mf_write_queries(const char* path, int q, const int32_t* s, const int32_t* t
 /* ---------------------------------------------------------- device graph */
 
 int mf_device_count(int* count);
+/* One empty kernel (k_trace_marker) on the current device, waited for: a
+ * mark by which a kernel trace or a counter run is split into phases (no
+ * counterpart in the reference). */
+int mf_trace_marker(void);
 
 /* Upload a (partitioned) host graph; regions are cut into tiles of at most
  * tile_max vertices (<= 4096), one workgroup per tile in region mode.

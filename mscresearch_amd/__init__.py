@@ -118,6 +118,7 @@ SIGNATURES = {
     "mf_hgraph_core_decision": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                           C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "mf_partition": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _P]),
+    "mf_trace_marker": (C.c_int, []),
     "mf_scc_build": (C.c_int, [C.c_int, C.c_int64, _P, _P, _P, C.POINTER(_P)]),
     "mf_scc_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
     "mf_scc_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
@@ -207,6 +208,12 @@ def device_count() -> int:
     c = C.c_int(0)
     _check(lib().mf_device_count(C.byref(c)))
     return c.value
+
+
+def trace_marker() -> None:
+    """One empty kernel (k_trace_marker), waited for: splits kernel traces
+    and counter runs into phases."""
+    _check(lib().mf_trace_marker())
 
 
 def default_params(**over) -> Params:

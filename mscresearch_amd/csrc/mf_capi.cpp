@@ -419,6 +419,14 @@ int mf_device_count(int* count) {
     return MF_OK;
 }
 
+int mf_trace_marker(void) {
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1) return fail(MF_ENODEV, "no HIP device");
+    launch_trace_marker(nullptr);
+    if (hipDeviceSynchronize() != hipSuccess) return fail(MF_EHIP, "trace marker failed");
+    return MF_OK;
+}
+
 int mf_graph_upload(const mf_hgraph* h, int tile_max, mf_graph** out) {
     if (!h || !out) return fail(MF_EINVAL, "null argument");
     int nd = 0;

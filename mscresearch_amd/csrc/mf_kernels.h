@@ -83,6 +83,9 @@ void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* lis
 // One workgroup of `threads` (256, 512 or 1024) per tile, tile heights and
 // pending excess staged in LDS; split > 1: up to `split` workgroups per tile
 // by excess score (k_tile_alloc), each staging its part of the tile.
+// An empty kernel: a phase mark in kernel traces and counter runs.
+void launch_trace_marker(hipStream_t st);
+
 void launch_discharge_tile(hipStream_t st, const SlotView* slots, const int* list, int nlist,
                            int gridx, int hot_only, int rounds, int giants, int threads, int split);
 // Owner-exclusive path push from the active list.

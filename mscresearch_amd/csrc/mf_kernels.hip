@@ -2380,6 +2380,10 @@ __global__ __launch_bounds__(1024) void k_tile_alloc(const SlotView* __restrict_
     if (threadIdx.x == 0) S.tsplit[0] = carry;
 }
 
+__global__ void k_trace_marker() {}
+
+void launch_trace_marker(hipStream_t st) { hipLaunchKernelGGL(k_trace_marker, dim3(1), dim3(64), 0, st); }
+
 void launch_discharge_tile(hipStream_t st, const SlotView* slots, const int* list, int nlist,
                            int gridx, int hot_only, int rounds, int giants, int threads, int split) {
     int sp = 0;
