@@ -332,8 +332,9 @@ def grid_record(args):
             for i in range(pq):
                 eng.submit(gid, g.s[i], g.t[i], i)
             pres = eng.run()
-            b, c = mismatches(pres, g.flows)
-            bad_total, checked = bad_total + b, checked + c
+            pb, pc = mismatches(pres, g.flows)
+            bad_total, checked = bad_total + pb, checked + pc
+            b, c = b + pb, c + pc
         fams = family_table([eng])
         M.trace_marker()
         eng.profile(False)

@@ -1174,13 +1174,19 @@ __device__ __forceinline__ bool slot_barrier(GAS Ctr* c, uint32_t& gen, long lon
     if (threadIdx.x == 0) {
         int ok = 1;
         const int G = (int)gridDim.x;
-        const int arrived = __hip_atomic_fetch_add(&c->bar_count, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+        // acq_rel: the last arrival's release of the generation then carries
+        // every workgroup's writes (each released them before arriving)
+        const int arrived = __hip_atomic_fetch_add(&c->bar_count, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
         if (arrived == G) {
             __hip_atomic_store(&c->bar_count, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&c->bar_gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const long long t0 = wall_clock64();
-            while (__hip_atomic_load(&c->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            // Relaxed polls: the acquire fence after the barrier pairs with
+            // the release store of the generation.  An acquire load here
+            // would invalidate this XCD's L2 on every poll, under the feet of
+            // every other workgroup on the XCD.
+            while (__hip_atomic_load(&c->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 __builtin_amdgcn_s_sleep(1);
                 if (wall_clock64() - t0 > spin_ticks) {
                     __hip_atomic_fetch_or(&c->err, 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
