@@ -231,6 +231,7 @@ typedef struct mf_params {
     int path_switch_deep, p_count_deep; /* path_switch and p_count0 once the last exact relabel is 64+ levels deep */
     int tile_split;      /* tile discharge: up to this many workgroups per tile by excess score (1..8, 1 = off) */
     int scc_route;       /* 1: a query with s and t in one SCC runs on that SCC's induced subgraph (exact) */
+    int bu_skip;         /* 1: bottom-up relabel levels skip vertices that cannot reach t's SCC when it is the largest */
 } mf_params;
 
 void mf_params_default(mf_params* p);

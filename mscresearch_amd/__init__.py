@@ -49,7 +49,7 @@ class Params(C.Structure):
         ("spec_cert", C.c_int), ("sparse_count", C.c_int), ("small_wgs", C.c_int),
         ("small_edges", C.c_longlong), ("mark_list", C.c_int), ("inject", C.c_int),
         ("tile_threads", C.c_int), ("path_switch_deep", C.c_int), ("p_count_deep", C.c_int),
-        ("tile_split", C.c_int), ("scc_route", C.c_int),
+        ("tile_split", C.c_int), ("scc_route", C.c_int), ("bu_skip", C.c_int),
     ]
 
 

@@ -112,6 +112,7 @@ Params from_c(const mf_params* p) {
     P.p_count_deep = p->p_count_deep;
     P.tile_split = p->tile_split;
     P.scc_route = p->scc_route;
+    P.bu_skip = p->bu_skip;
     return P;
 }
 
@@ -508,6 +509,7 @@ void mf_params_default(mf_params* p) {
     p->p_count_deep = P.p_count_deep;
     p->tile_split = P.tile_split;
     p->scc_route = P.scc_route;
+    p->bu_skip = P.bu_skip;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

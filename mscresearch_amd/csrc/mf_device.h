@@ -127,6 +127,11 @@ struct SlotView {
     int32_t* mlist = nullptr;
     int32_t mcap = 0;
     const int32_t* mask = nullptr;  // optional vertex mask (nullptr = all)
+    // relabel BFS, bottom-up levels: only vertices with live[v] != 0 look
+    // for a parent (nullptr = all).  live = can reach t's strongly connected
+    // component in the graph; the others never have a residual path to t
+    // (flow reaches them only from s, which the relabel never enters).
+    const uint8_t* live = nullptr;
     Ctr* ctr = nullptr;
     int32_t s = 0, t = 0;
     int32_t K = 4;               // discharge cycles per light vertex per pass

@@ -69,6 +69,7 @@ DeviceGraph::~DeviceGraph() {
     (void)hipFree(d_sub2old); (void)hipFree(d_old2sub); (void)hipFree(d_mask);
     (void)hipFree(d_wcc);
     (void)hipFree(d_scc);
+    (void)hipFree(d_live);
     (void)hipFree(d_capsum); (void)hipFree(d_cert_fp); (void)hipFree(d_pcap);
     (void)hipFree(d_fstart);
 }
@@ -144,6 +145,25 @@ int upload_graph(const HostGraph& hg, int tile_max, DeviceGraph& dg) {
     if (n > 0) HIPCHK(hipMemcpy(dg.d_scc, dg.scc.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     dg.scc_size.assign(n, 0);
     for (int v = 0; v < n; ++v) ++dg.scc_size[dg.scc[v]];
+    // vertices that can reach the largest SCC: a reverse search over the
+    // forward entries (an entry with cap 0 in w's row is the reverse of the
+    // forward entry col -> w)
+    if (n > 0) {
+        dg.giant_scc = (int32_t)(std::max_element(dg.scc_size.begin(), dg.scc_size.end()) - dg.scc_size.begin());
+        std::vector<uint8_t> live(n, 0);
+        std::vector<int32_t> q;
+        q.reserve(n);
+        for (int v = 0; v < n; ++v)
+            if (dg.scc[v] == dg.giant_scc) { live[v] = 1; q.push_back(v); }
+        for (size_t h = 0; h < q.size(); ++h) {
+            const int w = q[h];
+            for (int e = hg.row_ptr[w]; e < hg.row_ptr[w + 1]; ++e)
+                if (hg.cap[e] == 0 && !live[hg.col[e]]) { live[hg.col[e]] = 1; q.push_back(hg.col[e]); }
+        }
+        HIPCHK(hipMalloc(&dg.d_live, n));
+        HIPCHK(hipMemcpy(dg.d_live, live.data(), n, hipMemcpyHostToDevice));
+        dg.bytes += (size_t)n;
+    }
     dg.classes = true;
     dg.bytes += 2 * sizeof(int32_t) * (size_t)n;
     dg.fwd_bits.assign(((size_t)E + 63) / 64, 0);
@@ -1050,6 +1070,10 @@ int Engine::class_decide(int i) {
         }
         cache_[key] = CacheEntry{run, ++use_clock_, kept, ver};
     }
+    // t in the largest SCC: the relabel's bottom-up levels skip vertices
+    // that cannot reach it (no flow reaches them but from s, so they never
+    // have a residual path to t); not with injection, where s is relabelled
+    const bool skip = P_.bu_skip && !S.view.inject && G.giant_scc >= 0 && G.scc[S.q.t] == G.giant_scc;
     if (run) {
         const int s = run->old2sub_host.empty() ? -1 : run->old2sub_host[S.q.s];
         const int t = run->old2sub_host.empty() ? -1 : run->old2sub_host[S.q.t];
@@ -1058,6 +1082,7 @@ int Engine::class_decide(int i) {
             S.view.g = run->g;
             S.view.s = s;
             S.view.t = t;
+            S.view.live = skip ? run->d_live : nullptr;
             S.sched.stats.pruned = 1;
             S.sched.resume_after_prune(run->g, s, t);
             S.sched.set_depth_hint(run->depth_hint.load(std::memory_order_relaxed));
@@ -1065,6 +1090,7 @@ int Engine::class_decide(int i) {
             return MF_OK;
         }
     }
+    S.view.live = skip ? G.d_live : nullptr;
     S.sched.resume_after_prune(G.g, S.q.s, S.q.t);
     S.view.K = S.sched.K();
     return MF_OK;

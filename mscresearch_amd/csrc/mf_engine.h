@@ -79,6 +79,9 @@ struct Params {
     // runs on C's induced subgraph (every s-t path stays inside C); others
     // keep the reachability mask.  0 = the reference's mask for every query.
     int scc_route = 0;
+    // bottom-up relabel levels skip vertices that cannot reach t's SCC when
+    // that SCC is the graph's largest (DeviceGraph::d_live); 0 = scan all
+    int bu_skip = 1;
     // also GR once the discharge has scanned gr_work x E entries since the
     // last GR (a GR costs about E scans; 0 = off).  On R-MAT 22 hub queries
     // 0.125 beat 0.0625, 0.25, 0.5, 1 and 2 (hub rows are rescanned between
@@ -169,6 +172,12 @@ struct DeviceGraph {
     std::vector<int32_t> scc_size;  // vertices per SCC id
     int32_t* d_wcc = nullptr;
     int32_t* d_scc = nullptr;
+    // The largest SCC (id in the graph a query names) and, per vertex of
+    // this graph, whether it can reach that SCC over cap > 0 entries: the
+    // relabel's bottom-up levels skip the others for t in that SCC.  A
+    // pruned run graph carries its parent's (giant_scc is the parent's id).
+    int32_t giant_scc = -1;
+    uint8_t* d_live = nullptr;
     int64_t* d_capsum = nullptr;   // certificate constants (DGraph::capsum, cert_fp)
     uint64_t* d_cert_fp = nullptr;
     int64_t* d_pcap = nullptr;     // pair capacities (DGraph::pcap)

@@ -130,6 +130,12 @@ __global__ void k_class_mask(const uint8_t* __restrict__ fwd, const int32_t* __r
         mask[v] = (fwd[v] && wcc[v] == wt) ? 1 : 0;
 }
 
+__global__ void k_gather_u8(const uint8_t* __restrict__ src, const int32_t* __restrict__ idx, int k,
+                            uint8_t* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = src[idx[i]];
+}
+
 __global__ void k_eq_mask(const int32_t* __restrict__ ids, int32_t want, int n, int32_t* __restrict__ mask) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
         mask[v] = ids[v] == want ? 1 : 0;
@@ -216,6 +222,13 @@ int build_induced_gpu(const DeviceGraph& P, const int32_t* d_mask, int kept, hip
     hipLaunchKernelGGL(k_remap_rev, dim3(grid_for(E, 256)), dim3(256), 0, st, P.g.rev, e2s, E, out.d_rev);
     hipLaunchKernelGGL(k_tiles, dim3(grid_for(ntiles + 1, 256)), dim3(256), 0, st, P.g.tile_start, vscan,
                        ntiles, out.d_tile_start);
+    if (P.d_live) {
+        // the parent's reach of its largest SCC, in run-graph ids
+        HIPCHK(hipMalloc(&out.d_live, std::max(k, 1)));
+        hipLaunchKernelGGL(k_gather_u8, dim3(grid_for(k, 256)), dim3(256), 0, st, P.d_live, out.d_sub2old, k,
+                           out.d_live);
+        out.giant_scc = P.giant_scc;
+    }
     HIPCHK(hipGetLastError());
     out.old2sub_host.resize(n);
     if (n > 0)
@@ -234,7 +247,8 @@ int build_induced_gpu(const DeviceGraph& P, const int32_t* d_mask, int kept, hip
     out.g.tile_start = out.d_tile_start;
     out.g.tile_of = out.d_tile_of;
     out.tile_max = P.tile_max;
-    out.bytes = sizeof(int32_t) * ((size_t)k + 1) + (size_t)Ek * 16 + sizeof(int32_t) * (ntiles + 1 + 2 * (size_t)k + n) + n;
+    out.bytes = sizeof(int32_t) * ((size_t)k + 1) + (size_t)Ek * 16 + sizeof(int32_t) * (ntiles + 1 + 2 * (size_t)k + n) + n +
+                (out.d_live ? (size_t)k : 0);
     return init_cert_consts(out, st);
 }
 

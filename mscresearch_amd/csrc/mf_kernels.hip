@@ -153,6 +153,7 @@ struct GSlot {
     GAS int32_t* mlist;
     int32_t mcap;
     const GAS int32_t* mask;
+    const GAS uint8_t* live;
     GAS Ctr* ctr;
     int32_t s, t, K, Kh, H_cut, phase, seed_t_only, inj;
     int32_t tstamp;  // this cycle's touch stamp (negative)
@@ -201,6 +202,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.mlist = as_global(v.mlist);
     S.mcap = v.mcap;
     S.mask = as_global(v.mask);
+    S.live = as_global(v.live);
     S.ctr = as_global(v.ctr);
     S.s = v.s;
     S.t = v.t;
@@ -904,7 +906,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
     Work wk;
     for (int64_t vi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; vi < n; vi += (int64_t)gridDim.x * blockDim.x) {
         const int v = (int)vi;
-        if (lvl[v] == n && bfs_vertex_ok<MODE>(S, v)) {
+        if (lvl[v] == n && bfs_vertex_ok<MODE>(S, v) && (MODE != BFS_GR || S.live == nullptr || S.live[v])) {
             wk.verts++;
             const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
             if (re - rb > kGiantDeg) {

@@ -149,6 +149,8 @@ def test_edge_cases():
     dict(tile_split=4, tile_frac=0.0),          # up to 4 workgroups per tile by excess score, every step
     dict(tile_split=8, tile_threads=256, tile_frac=0.0, tile_rounds=1),  # ... 8 of 256 threads, one round each
     dict(tile_split=2, sparse_count=1, hot_min_active=1, hot_k=4, tile_frac=0.0),  # ... and in the hot sweep
+    dict(bu_skip=0),                            # bottom-up relabel levels scan every unvisited vertex
+    dict(bu_skip=1, small_edges=0),             # ... skip those that cannot reach t's SCC, every level grid-wide
 ])
 def test_scheduler_variants_stay_exact(c1, variant):
     log(f"variant {variant}")
