@@ -232,6 +232,7 @@ typedef struct mf_params {
     int tile_split;      /* tile discharge: up to this many workgroups per tile by excess score (1..8, 1 = off) */
     int scc_route;       /* 1: a query with s and t in one SCC runs on that SCC's induced subgraph (exact) */
     int bu_skip;         /* 1: bottom-up relabel levels skip vertices that cannot reach t's SCC when it is the largest */
+    int sink_cut;        /* 1: stop once excess(t) fills t's in-capacity, certified with T = {t} */
 } mf_params;
 
 void mf_params_default(mf_params* p);
@@ -261,6 +262,9 @@ typedef struct mf_result {
      * discoveries); bfs = seeding, single-workgroup and top-down levels,
      * bfs_bu = bottom-up levels */
     uint64_t work[MF_FAMILIES][4];
+    /* 1 when the answer was certified on the cut around t as soon as
+     * excess(t) reached t's in-capacity (mf_params.sink_cut), else 0 */
+    int64_t sink_cut;
 } mf_result;
 
 /* Plain single query: solve (s,t) (ids of graph gid) to completion.

@@ -49,7 +49,7 @@ class Params(C.Structure):
         ("spec_cert", C.c_int), ("sparse_count", C.c_int), ("small_wgs", C.c_int),
         ("small_edges", C.c_longlong), ("mark_list", C.c_int), ("inject", C.c_int),
         ("tile_threads", C.c_int), ("path_switch_deep", C.c_int), ("p_count_deep", C.c_int),
-        ("tile_split", C.c_int), ("scc_route", C.c_int), ("bu_skip", C.c_int),
+        ("tile_split", C.c_int), ("scc_route", C.c_int), ("bu_skip", C.c_int), ("sink_cut", C.c_int),
     ]
 
 
@@ -84,7 +84,7 @@ class Result(C.Structure):
         ("edge_scans", C.c_uint64), ("vertex_scans", C.c_uint64), ("pushes", C.c_uint64),
         ("relabels", C.c_uint64),
         ("t_arrive", C.c_double), ("t_start", C.c_double), ("t_end", C.c_double),
-        ("work", (C.c_uint64 * 4) * 7),
+        ("work", (C.c_uint64 * 4) * 7), ("sink_cut", C.c_int64),
     ]
 
     def as_dict(self) -> dict:

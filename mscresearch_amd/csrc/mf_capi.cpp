@@ -50,6 +50,7 @@ void mf::result_to_c(const Result& r, mf_result* o) {
     o->grs = r.st.grs;
     o->bfs_levels = r.st.bfs_levels;
     o->drains = r.st.drains;
+    o->sink_cut = (r.st.sink_cuts > 0 && r.st.spec_fails == 0) ? 1 : 0;
     o->rm_fixed = r.st.rm_fixed;
     o->edge_scans = r.st.edge_scans;
     o->vertex_scans = r.st.vertex_scans;
@@ -113,6 +114,7 @@ Params from_c(const mf_params* p) {
     P.tile_split = p->tile_split;
     P.scc_route = p->scc_route;
     P.bu_skip = p->bu_skip;
+    P.sink_cut = p->sink_cut;
     return P;
 }
 
@@ -510,6 +512,7 @@ void mf_params_default(mf_params* p) {
     p->tile_split = P.tile_split;
     p->scc_route = P.scc_route;
     p->bu_skip = P.bu_skip;
+    p->sink_cut = P.sink_cut;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

@@ -81,6 +81,12 @@ struct Ctr {
     // units the preflow put on s (bounded injection, SlotView::inject), 0 when
     // it saturated the row of s: the excess s holds beyond its row's flow
     int64_t inject;
+    // capacity into t (over t's row, pcap - cap of each entry), set by the
+    // preflow: once excess(t) reaches it the cut around t is saturated
+    int64_t incap_t;
+    // capacity out of s (capsum of s), set by the preflow: once excess(t)
+    // reaches it the cut around s is saturated
+    int64_t outcap_s;
     // algorithmic work counters (roofline accounting), per kernel family:
     // wk[family][0] edge scans, [1] vertex scans, [2] pushes,
     // [3] relabels + BFS discoveries

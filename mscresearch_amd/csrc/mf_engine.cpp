@@ -586,6 +586,23 @@ Plan SlotSched::plan(const Ctr& c) {
             st_ = S_DONE;
             return p;
         }
+        // excess(t) fills t's in-capacity: every entry into t is saturated,
+        // so the cut around t has capacity excess(t) and no flow can exceed
+        // it, whatever excess is still held elsewhere; certified with
+        // T = {t}.  Or it equals s's out-capacity: everything s sent has
+        // reached t, and the cut around s (saturated since the preflow)
+        // proves it, with T = V \ {s}.  A failure falls back to the global
+        // relabel.
+        const bool at_t = c.ex_t == c.incap_t, at_s = c.ex_t == c.outcap_s;
+        if (P_->sink_cut && P_->cert && phase_ == 0 && !deficits_ && !spec_failed_ && c.incap_t >= 0 &&
+            (at_t || at_s)) {
+            p.stages = ST_RESET_PRE | (at_t ? ST_SINK : ST_SOURCE) | ST_CERT;
+            p.reset_pre = RST_CERT;
+            spec_ = true;
+            stats.sink_cuts++;
+            st_ = S_CERT;
+            return p;
+        }
         if (iter_ > P_->iter_max) { fail(MF_EITER, "iteration watchdog fired"); return p; }
         // stagnation monitor on the positive excess sum (BC:1851-1891)
         bool hotspot_triggered = false;
@@ -1402,6 +1419,8 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     // the sparse certificate where the slab's marks cover its residual, the
     // dense fingerprint elsewhere
     const bool cert_sp = P_.cert == 3;
+    Group g_sink = make_list([&](int i) { return has(i, ST_SINK); });
+    Group g_src = make_list([&](int i) { return has(i, ST_SOURCE); });
     Group g_cert = make_list([&](int i) { return has(i, ST_CERT) && !(cert_sp && slots_[i].slab.marks_ok); });
     Group g_cert_sp = make_list([&](int i) { return has(i, ST_CERT) && cert_sp && slots_[i].slab.marks_ok; });
     // slots of fused steps (see the fused-cycle launches below)
@@ -1654,6 +1673,14 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
                 graph_seen_.insert(key);
             }
             emit(true);
+        }
+    }
+    {
+        if (g_sink.cnt || g_src.cnt) {
+            const int k = tb(WF_CERT);
+            if (g_sink.cnt) launch_cut_labels(st, d_views_, L(g_sink), g_sink.cnt, fit(g_sink, maxof(g_sink, gridN)), 0);
+            if (g_src.cnt) launch_cut_labels(st, d_views_, L(g_src), g_src.cnt, fit(g_src, maxof(g_src, gridN)), 1);
+            te(k);
         }
     }
     if (g_cert.cnt) { const int k = tb(WF_CERT); launch_cert(st, d_views_, L(g_cert), g_cert.cnt, fit(g_cert, maxof(g_cert, gridN)), cert_sp ? 1 : P_.cert); te(k); }

@@ -82,6 +82,11 @@ struct Params {
     // bottom-up relabel levels skip vertices that cannot reach t's SCC when
     // that SCC is the graph's largest (DeviceGraph::d_live); 0 = scan all
     int bu_skip = 1;
+    // stop as soon as excess(t) fills t's in-capacity or s's out-capacity
+    // (the cut around t or s is saturated: excess(t) is the maximum flow),
+    // certified with T = {t} or T = V \ {s}; 0 = only after no active
+    // vertex is left
+    int sink_cut = 1;
     // also GR once the discharge has scanned gr_work x E entries since the
     // last GR (a GR costs about E scans; 0 = off).  On R-MAT 22 hub queries
     // 0.125 beat 0.0625, 0.25, 0.5, 1 and 2 (hub rows are rescanned between
@@ -285,6 +290,8 @@ enum StageBit : unsigned {
     ST_ACTIVE = 1u << 11,
     ST_CERT = 1u << 12,
     ST_LOCAL = 1u << 13,
+    ST_SINK = 1u << 14,  // labels T = {t} before the certificate
+    ST_SOURCE = 1u << 15,  // labels T = V \ {s} before the certificate
 };
 
 enum BfsKind : int { BK_SMALL = 0, BK_TD = 1, BK_BU = 2 };
@@ -319,6 +326,7 @@ struct QueryStats {
     int64_t drains = 0;       // deficit drain phases
     int64_t local_relabels = 0;
     int64_t spec_fails = 0;   // certificates on discharge labels that failed
+    int64_t sink_cuts = 0;    // certificates tried on the cut around t or s (excess(t) = its capacity)
     int64_t bfs_restarts = 0; // BFS runs restarted single-workgroup after a barrier time-out
     int64_t rm_fixed = 0;
     int cert_ok = 0;

@@ -83,6 +83,10 @@ void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* lis
 // One workgroup of `threads` (256, 512 or 1024) per tile, tile heights and
 // pending excess staged in LDS; split > 1: up to `split` workgroups per tile
 // by excess score (k_tile_alloc), each staging its part of the tile.
+// Labels of a saturated terminal cut for the certificate: side 0, T = {t};
+// side 1, T = V \ {s}.
+void launch_cut_labels(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx, int side);
+
 // An empty kernel: a phase mark in kernel traces and counter runs.
 void launch_trace_marker(hipStream_t st);
 

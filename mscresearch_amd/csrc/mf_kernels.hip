@@ -472,15 +472,20 @@ __global__ __launch_bounds__(256) void k_preflow(const SlotView* __restrict__ sl
     const GSlot S = load_slot(slots, list);
     __shared__ int64_t red[256 / WAVE];
     const int s = S.s;
+    // In-capacity of t: over t's row, pcap - cap is the capacity of each
+    // entry's pair into t.
+    const int tb = S.g.row_ptr[S.t], te = S.g.row_ptr[S.t + 1];
+    int64_t in_t = 0;
+    for (int e = tb + threadIdx.x; e < te; e += blockDim.x) in_t += S.g.pcap[e] - S.g.cap[e];
+    in_t = block_sum<int64_t>(in_t, red);
+    if (threadIdx.x == 0) {
+        S.ctr->incap_t = in_t;
+        S.ctr->outcap_s = S.g.capsum[s];
+    }
     if (S.inj) {
         // Bounded injection: no flow can exceed the out-capacity of s or the
         // in-capacity of t (each an s-t cut), so s starts with B units of
-        // excess and an unsaturated row.  In-capacity of t: over t's row,
-        // pcap - cap is the capacity of each entry's pair into t.
-        const int tb = S.g.row_ptr[S.t], te = S.g.row_ptr[S.t + 1];
-        int64_t in_t = 0;
-        for (int e = tb + threadIdx.x; e < te; e += blockDim.x) in_t += S.g.pcap[e] - S.g.cap[e];
-        in_t = block_sum<int64_t>(in_t, red);
+        // excess and an unsaturated row.
         if (threadIdx.x == 0) {
             const int64_t out_s = S.g.capsum[s];
             const int64_t B = out_s < in_t ? out_s : in_t;
@@ -2386,6 +2391,21 @@ __global__ __launch_bounds__(1024) void k_tile_alloc(const SlotView* __restrict_
         __syncthreads();
     }
     if (threadIdx.x == 0) S.tsplit[0] = carry;
+}
+
+// Labels of a saturated terminal cut, for the certificate (T = {h < n}):
+// side 0, the cut around t: T = {t} (h = 0 at t, n elsewhere); side 1, the
+// cut around s: T = V \ {s} (h = n at s, 0 elsewhere).
+__global__ __launch_bounds__(256) void k_cut_labels(const SlotView* __restrict__ slots,
+                                                    const int* __restrict__ list, int side) {
+    const GSlot S = load_slot(slots, list);
+    const int n = S.g.n;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+        S.h[v] = side == 0 ? (v == S.t ? 0 : n) : (v == S.s ? n : 0);
+}
+
+void launch_cut_labels(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx, int side) {
+    hipLaunchKernelGGL(k_cut_labels, dim3(gridx, nlist), dim3(256), 0, st, slots, list, side);
 }
 
 __global__ void k_trace_marker() {}

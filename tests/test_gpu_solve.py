@@ -150,6 +150,7 @@ def test_edge_cases():
     dict(tile_split=8, tile_threads=256, tile_frac=0.0, tile_rounds=1),  # ... 8 of 256 threads, one round each
     dict(tile_split=2, sparse_count=1, hot_min_active=1, hot_k=4, tile_frac=0.0),  # ... and in the hot sweep
     dict(bu_skip=0),                            # bottom-up relabel levels scan every unvisited vertex
+    dict(sink_cut=0),                           # no early stop when excess(t) fills t's in-capacity
     dict(bu_skip=1, small_edges=0),             # ... skip those that cannot reach t's SCC, every level grid-wide
 ])
 def test_scheduler_variants_stay_exact(c1, variant):
@@ -161,6 +162,23 @@ def test_scheduler_variants_stay_exact(c1, variant):
         r = eng.solve(gid, s, t)
         assert r.flow == c1.oracle_flow(s, t), (variant, a, b)
         assert r.cert_ok == 1
+
+
+def test_sink_cut_stops_early_and_stays_exact(c1):
+    """Where t's in-entries are the minimum cut, the solve stops as soon as
+    excess(t) fills them, certified on T = {t}; flows equal the oracle's and
+    the full solve's."""
+    qs, qt = M.uniform_queries(c1.n, 24, 1021)
+    on, gid = engine_with(c1, sink_cut=1)
+    off, gid2 = engine_with(c1, sink_cut=0)
+    early = 0
+    for a, b in zip(qs, qt):
+        s, t = c1.q(a, b)
+        r1, r0 = on.solve(gid, s, t), off.solve(gid2, s, t)
+        assert r1.flow == r0.flow == c1.oracle_flow(s, t), (a, b)
+        assert r1.cert_ok == 1 and r0.sink_cut == 0
+        early += r1.sink_cut
+    assert early > 0
 
 
 def test_batched_equals_alone(c1):
