@@ -232,7 +232,8 @@ typedef struct mf_params {
     int tile_split;      /* tile discharge: up to this many workgroups per tile by excess score (1..8, 1 = off) */
     int scc_route;       /* 1: a query with s and t in one SCC runs on that SCC's induced subgraph (exact) */
     int bu_skip;         /* 1: bottom-up relabel levels skip vertices that cannot reach t's SCC when it is the largest */
-    int sink_cut;        /* 1: stop once excess(t) fills t's in-capacity, certified with T = {t} */
+    int sink_cut;        /* 1: stop once excess(t) fills t's in-capacity or s's out-capacity, certified on that cut */
+    int disc_band;       /* shallow graphs: discharge only active vertices within this many levels of the lowest (0 = all) */
 } mf_params;
 
 void mf_params_default(mf_params* p);

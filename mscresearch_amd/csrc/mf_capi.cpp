@@ -115,6 +115,7 @@ Params from_c(const mf_params* p) {
     P.scc_route = p->scc_route;
     P.bu_skip = p->bu_skip;
     P.sink_cut = p->sink_cut;
+    P.disc_band = p->disc_band;
     return P;
 }
 
@@ -513,6 +514,7 @@ void mf_params_default(mf_params* p) {
     p->scc_route = P.scc_route;
     p->bu_skip = P.bu_skip;
     p->sink_cut = P.sink_cut;
+    p->disc_band = P.disc_band;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

@@ -143,6 +143,9 @@ struct SlotView {
     int32_t K = 4;               // discharge cycles per light vertex per pass
     int32_t Kh = 1;              // ... per heavy (wave-scanned) vertex per pass
     int32_t H_cut = 100;         // "high" label threshold (BC:1469)
+    // discharge only vertices with h <= disc_cap this step (lowest band
+    // first); the others stay active and are counted as before
+    int32_t disc_cap = 0x7fffffff;
     int32_t phase = 0;           // 0 = normal (seeds t + deficits), 1 = drain (seeds deficits, t active)
     int32_t seed_t_only = 0;     // phase 0 with no deficit (last count): the relabel seeds t alone
     // 1 = bounded injection: the preflow puts B = min(out-capacity of s,

@@ -330,6 +330,7 @@ void SlotSched::begin(const Params* P, const DGraph& g, int s, int t, bool prune
     depth_hint_ = 0;
     spec_ = false;
     spec_failed_ = false;
+    disc_cap_ = 0x7fffffff;
     stag_ = no_stag_ = 0;
     last_pos_ = INT64_MAX;
     pos_full_ = ext_full_ = 0;
@@ -681,6 +682,10 @@ Plan SlotSched::plan(const Ctr& c) {
         const bool use_path = active > 0 && active <= pswitch && path_stable_ >= pneed;
         p.stages = ST_DISCHARGE | ST_RESET_ACT | ST_ACTIVE;
         p.reset_act = RST_ACTIVE;
+        // lowest band first on shallow graphs: the excess nearest t moves
+        // first, and with the terminal-cut stop the rest may never need to
+        disc_cap_ = (P_->disc_band > 0 && !deep && phase_ == 0 && c.hmin <= c.hmax) ? c.hmin + P_->disc_band
+                                                                                   : 0x7fffffff;
         if (use_path) {
             p.dis_kind = DK_PATH;
             p.dis_grid = std::max(1, cdiv(active, 4));
@@ -1228,6 +1233,7 @@ int Engine::step(std::vector<Result>& out) {
         S.view.phase = S.sched.phase();
         S.view.seed_t_only = plans[i].seed_t_only ? 1 : 0;
         S.view.K = S.sched.K();
+        S.view.disc_cap = S.sched.disc_cap();
         S.sched.stats.steps++;
     }
     // retire finished slots

@@ -87,6 +87,9 @@ struct Params {
     // certified with T = {t} or T = V \ {s}; 0 = only after no active
     // vertex is left
     int sink_cut = 1;
+    // shallow graphs: discharge only active vertices within disc_band
+    // levels of the lowest active height (0 = every active vertex)
+    int disc_band = 0;
     // also GR once the discharge has scanned gr_work x E entries since the
     // last GR (a GR costs about E scans; 0 = off).  On R-MAT 22 hub queries
     // 0.125 beat 0.0625, 0.25, 0.5, 1 and 2 (hub rows are rescanned between
@@ -346,6 +349,8 @@ public:
     // multi-workgroup small-frontier BFS from a query's first relabel on
     void set_depth_hint(int d) { depth_hint_ = d; }
     int gr_depth() const { return gr_depth_; }
+    // height cap of the next discharge step (INT_MAX: none)
+    int disc_cap() const { return disc_cap_; }
     int status() const { return status_; }
     const std::string& error() const { return err_; }
     int phase() const { return phase_; }
@@ -400,6 +405,7 @@ private:
     bool work_reset_ = false;
     bool fwd_only_ = false;
     bool spec_ = false;         // the certificate under way follows discharge labels, not a GR
+    int disc_cap_ = 0x7fffffff;
     bool spec_failed_ = false;
     bool after_local_ = false;
     // Deficits (negative excess off s) may exist: set by RM repairs and warm

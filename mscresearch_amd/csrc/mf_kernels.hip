@@ -155,7 +155,7 @@ struct GSlot {
     const GAS int32_t* mask;
     const GAS uint8_t* live;
     GAS Ctr* ctr;
-    int32_t s, t, K, Kh, H_cut, phase, seed_t_only, inj;
+    int32_t s, t, K, Kh, H_cut, phase, seed_t_only, inj, disc_cap;
     int32_t tstamp;  // this cycle's touch stamp (negative)
 };
 
@@ -209,6 +209,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.K = v.K;
     S.Kh = v.Kh;
     S.H_cut = v.H_cut;
+    S.disc_cap = v.disc_cap;
     S.phase = v.phase;
     S.seed_t_only = v.seed_t_only;
     S.inj = v.inject;
@@ -1817,6 +1818,7 @@ __device__ void lane_discharge(const GSlot& S, const TileCtx& tc, int v, int cyc
     const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
     int hv = own_h(S, tc, v);
     int64_t ex = own_excess(S, tc, v);
+    if (hv > S.disc_cap) return;  // above this step's band
     for (int c = 0; c < cycles; ++c) {
         if (ex <= 0 || hv >= n) break;
         int hmin = INT_MAX;
@@ -1861,6 +1863,7 @@ __device__ void wave_discharge(const GSlot& S, const TileCtx& tc, int v, int cyc
     const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
     int hv = __shfl(own_h(S, tc, v), 0, WAVE);
     int64_t ex = __shfl(own_excess(S, tc, v), 0, WAVE);
+    if (hv > S.disc_cap) return;  // above this step's band (wave-uniform)
     for (int c = 0; c < cycles; ++c) {
         if (ex <= 0 || hv >= n) break;
         int hmin = INT_MAX;
@@ -2081,7 +2084,8 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
         const int64_t ex0 = bcast;
         const int hv = sh[3];
         __syncthreads();
-        if (ex0 <= 0 || hv >= n) continue;  // dry or cut off: no report, no relabel
+        // dry, cut off or above this step's band: no report, no relabel
+        if (ex0 <= 0 || hv >= n || hv > S.disc_cap) continue;
         int64_t r[PER];
         int hw[PER], cw[PER];
 #pragma unroll
@@ -2287,7 +2291,7 @@ __global__ __launch_bounds__(NT) void k_discharge_tile(const SlotView* __restric
             const int v = a + i;
             if (v == S.s && !S.inj) continue;
             if (v == S.t && S.phase == 0) continue;
-            if (lh[i] >= n) continue;
+            if (lh[i] >= n || lh[i] > S.disc_cap) continue;
             const int64_t ex = ld64(&S.excess[v]) + ldx[i];
             if (ex <= 0) continue;
             const int deg = S.g.row_ptr[v + 1] - S.g.row_ptr[v];
