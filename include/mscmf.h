@@ -234,6 +234,7 @@ typedef struct mf_params {
     int bu_skip;         /* 1: bottom-up relabel levels skip vertices that cannot reach t's SCC when it is the largest */
     int sink_cut;        /* 1: stop once excess(t) fills t's in-capacity or s's out-capacity, certified on that cut */
     int disc_band;       /* shallow graphs: discharge only active vertices within this many levels of the lowest (0 = all) */
+    int bu_pick;         /* 1: bottom-up relabel levels try each vertex's entry to its longest-row head first */
 } mf_params;
 
 void mf_params_default(mf_params* p);

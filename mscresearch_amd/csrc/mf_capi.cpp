@@ -116,6 +116,7 @@ Params from_c(const mf_params* p) {
     P.bu_skip = p->bu_skip;
     P.sink_cut = p->sink_cut;
     P.disc_band = p->disc_band;
+    P.bu_pick = p->bu_pick;
     return P;
 }
 
@@ -515,6 +516,7 @@ void mf_params_default(mf_params* p) {
     p->bu_skip = P.bu_skip;
     p->sink_cut = P.sink_cut;
     p->disc_band = P.disc_band;
+    p->bu_pick = P.bu_pick;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

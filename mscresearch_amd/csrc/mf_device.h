@@ -39,6 +39,12 @@ struct DGraph {
     // from (u, v)-sorted input lines open with the reverse entries of lower
     // ids, which carry no residual until flow crosses them.
     const int32_t* fstart = nullptr;
+    // bu_pick[2v], bu_pick[2v+1]: the forward entry of v whose head has the
+    // longest row, and that head (-1, -1 without forward entries).  A
+    // bottom-up relabel level tries it before v's row: in skewed graphs the
+    // long-row heads are few, reached early and their levels stay cached,
+    // so most vertices find their parent without reading their own row.
+    const int32_t* bu_pick = nullptr;
 };
 
 // Offset (in int32 words, 16-byte aligned) of the giant list's task records.

@@ -72,6 +72,7 @@ DeviceGraph::~DeviceGraph() {
     (void)hipFree(d_live);
     (void)hipFree(d_capsum); (void)hipFree(d_cert_fp); (void)hipFree(d_pcap);
     (void)hipFree(d_fstart);
+    (void)hipFree(d_bu_pick);
 }
 
 DeviceGraph::FwdSet::~FwdSet() { (void)hipFree(d_mask); }
@@ -185,11 +186,14 @@ int init_cert_consts(DeviceGraph& dg, hipStream_t st) {
     dg.g.pcap = dg.d_pcap;
     launch_first_forward(st, dg.g, dg.d_fstart);
     dg.g.fstart = dg.d_fstart;
+    HIPCHK(hipMalloc(&dg.d_bu_pick, sizeof(int32_t) * 2 * (size_t)std::max(n, 1)));
+    launch_bu_pick(st, dg.g, dg.d_bu_pick);
+    dg.g.bu_pick = dg.d_bu_pick;
     dg.g.cert_seed = 0x9e3779b97f4a7c15ULL * (++graphs_made) ^ ((uint64_t)(uint32_t)n << 32) ^ (uint64_t)(uint32_t)dg.g.E;
     launch_cert_consts(st, dg.g, dg.d_capsum, dg.d_cert_fp);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
-    dg.bytes += sizeof(int64_t) * ((size_t)n + (size_t)dg.g.E) + sizeof(uint64_t) + sizeof(int32_t) * (size_t)n;
+    dg.bytes += sizeof(int64_t) * ((size_t)n + (size_t)dg.g.E) + sizeof(uint64_t) + 3 * sizeof(int32_t) * (size_t)n;
     return MF_OK;
 }
 
@@ -1304,7 +1308,10 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     // stage the views and lists
     SlotView* hv = (SlotView*)h_stage_;
     int* hl = (int*)((char*)h_stage_ + sizeof(SlotView) * nslots_);
-    for (int i = 0; i < nslots_; ++i) hv[i] = slots_[i].view;
+    for (int i = 0; i < nslots_; ++i) {
+        hv[i] = slots_[i].view;
+        if (!P_.bu_pick) hv[i].g.bu_pick = nullptr;
+    }
     size_t lpos = 0;
     struct Group { int off = 0, cnt = 0; };
     auto make_list = [&](auto pred) {

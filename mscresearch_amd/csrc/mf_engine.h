@@ -87,6 +87,9 @@ struct Params {
     // certified with T = {t} or T = V \ {s}; 0 = only after no active
     // vertex is left
     int sink_cut = 1;
+    // bottom-up relabel levels try each vertex's forward entry to its
+    // longest-row head before its row (DGraph::bu_pick); 0 = rows only
+    int bu_pick = 1;
     // shallow graphs: discharge only active vertices within disc_band
     // levels of the lowest active height (0 = every active vertex)
     int disc_band = 0;
@@ -190,6 +193,7 @@ struct DeviceGraph {
     uint64_t* d_cert_fp = nullptr;
     int64_t* d_pcap = nullptr;     // pair capacities (DGraph::pcap)
     int32_t* d_fstart = nullptr;   // first forward entry per row (DGraph::fstart)
+    int32_t* d_bu_pick = nullptr;  // DGraph::bu_pick
     struct FwdSet {
         std::vector<uint8_t> mask;  // host copy
         uint8_t* d_mask = nullptr;

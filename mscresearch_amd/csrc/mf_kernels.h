@@ -128,5 +128,7 @@ void launch_cap_scatter(hipStream_t st, const DGraph& g, int64_t* cap, const int
 void launch_cert_consts(hipStream_t st, const DGraph& g, int64_t* capsum, uint64_t* fp_out);
 // fstart[v] = first entry of v's row with cap > 0, row_ptr[v] if none.
 void launch_first_forward(hipStream_t st, const DGraph& g, int32_t* fstart);
+// DGraph::bu_pick (2n ints)
+void launch_bu_pick(hipStream_t st, const DGraph& g, int32_t* pick);
 
 }  // namespace mf

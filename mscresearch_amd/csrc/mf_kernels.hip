@@ -129,6 +129,7 @@ struct GGraph {
     uint64_t cert_seed;
     const GAS int64_t* pcap;
     const GAS int32_t* fstart;
+    const GAS int32_t* bu_pick;
 };
 
 struct GSlot {
@@ -182,6 +183,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.g.cert_seed = v.g.cert_seed;
     S.g.pcap = as_global(v.g.pcap);
     S.g.fstart = as_global(v.g.fstart);
+    S.g.bu_pick = as_global(v.g.bu_pick);
     S.res = as_global(v.res);
     S.excess = as_global(v.excess);
     S.h = as_global(v.h);
@@ -915,6 +917,17 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
         if (lvl[v] == n && bfs_vertex_ok<MODE>(S, v) && (MODE != BFS_GR || S.live == nullptr || S.live[v])) {
             wk.verts++;
             const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+            if (MODE == BFS_GR && S.g.bu_pick) {
+                // the picked entry first (its head's level is likely cached)
+                const int pe = S.g.bu_pick[2 * v], pw = S.g.bu_pick[2 * v + 1];
+                if (pe >= 0 && lvl[pw] == L && ld_res(&S.res[pe]) > 0) {
+                    wk.edges += 1;
+                    lvl[v] = L + 1;  // only this lane writes v
+                    stage_push(st, v, re - rb, out, ocnt, odeg);
+                    wk.relabels++;
+                    continue;
+                }
+            }
             if (re - rb > kGiantDeg) {
                 S.giant[add32(&S.ctr->ngiant, 1)] = v;
             } else if (re - rb > kBuLightDeg) {
@@ -3249,6 +3262,40 @@ __global__ __launch_bounds__(256) void k_first_forward(DGraph g, int32_t* __rest
         }
         if (lane == 0) fstart[v] = f;
     }
+}
+
+// One wave per row: the forward entry whose head has the longest row (ties:
+// the first such entry).
+__global__ __launch_bounds__(256) void k_bu_pick(DGraph g, int32_t* __restrict__ pick) {
+    const int lane = lane_id();
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / WAVE);
+    for (int64_t v = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; v < g.n; v += nw) {
+        const int rb = g.row_ptr[v], re = g.row_ptr[v + 1];
+        // (row length << 32) | (~entry): the max takes the longest head row,
+        // then the lowest entry
+        uint64_t best = 0;
+        for (int e = rb + lane; e < re; e += WAVE) {
+            if (g.cap[e] <= 0) continue;
+            const int w = g.col[e];
+            const uint64_t d = (uint64_t)(uint32_t)(g.row_ptr[w + 1] - g.row_ptr[w]) + 1;
+            const uint64_t key = (d << 32) | (uint64_t)(~(uint32_t)e);
+            best = key > best ? key : best;
+        }
+        for (int o = WAVE / 2; o > 0; o >>= 1) {
+            const uint64_t x = __shfl_xor(best, o, WAVE);
+            best = x > best ? x : best;
+        }
+        if (lane == 0) {
+            const int e = best ? (int)~(uint32_t)(best & 0xffffffffu) : -1;
+            pick[2 * v] = e;
+            pick[2 * v + 1] = e >= 0 ? g.col[e] : -1;
+        }
+    }
+}
+
+void launch_bu_pick(hipStream_t st, const DGraph& g, int32_t* pick) {
+    const int gx = std::max(1, std::min(16384, (g.n + 3) / 4));
+    hipLaunchKernelGGL(k_bu_pick, dim3(gx), dim3(256), 0, st, g, pick);
 }
 
 void launch_first_forward(hipStream_t st, const DGraph& g, int32_t* fstart) {
