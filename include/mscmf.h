@@ -235,6 +235,7 @@ typedef struct mf_params {
     int sink_cut;        /* 1: stop once excess(t) fills t's in-capacity or s's out-capacity, certified on that cut */
     int disc_band;       /* shallow graphs: discharge only active vertices within this many levels of the lowest (0 = all) */
     int bu_pick;         /* 1: bottom-up relabel levels try each vertex's entry to its longest-row head first */
+    int small_levels;    /* BFS levels one single-workgroup (or multi-workgroup) BFS launch runs at most */
 } mf_params;
 
 void mf_params_default(mf_params* p);

@@ -50,7 +50,7 @@ class Params(C.Structure):
         ("small_edges", C.c_longlong), ("mark_list", C.c_int), ("inject", C.c_int),
         ("tile_threads", C.c_int), ("path_switch_deep", C.c_int), ("p_count_deep", C.c_int),
         ("tile_split", C.c_int), ("scc_route", C.c_int), ("bu_skip", C.c_int), ("sink_cut", C.c_int),
-        ("disc_band", C.c_int), ("bu_pick", C.c_int),
+        ("disc_band", C.c_int), ("bu_pick", C.c_int), ("small_levels", C.c_int),
     ]
 
 
@@ -220,7 +220,10 @@ def trace_marker() -> None:
 def default_params(**over) -> Params:
     p = Params()
     lib().mf_params_default(C.byref(p))
+    names = {f for f, _ in Params._fields_}
     for k, v in over.items():
+        if k not in names:
+            raise KeyError(f"mf_params has no field {k!r}")
         setattr(p, k, v)
     return p
 

@@ -117,6 +117,7 @@ Params from_c(const mf_params* p) {
     P.sink_cut = p->sink_cut;
     P.disc_band = p->disc_band;
     P.bu_pick = p->bu_pick;
+    P.small_levels = p->small_levels;
     return P;
 }
 
@@ -517,6 +518,7 @@ void mf_params_default(mf_params* p) {
     p->sink_cut = P.sink_cut;
     p->disc_band = P.disc_band;
     p->bu_pick = P.bu_pick;
+    p->small_levels = P.small_levels;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

@@ -38,7 +38,10 @@ struct Params {
     int path_len = 256;        // BC:1481
     int path_attempts = 4;     // BC:895
     int p_count0 = 64;         // BC:1483
-    int hot = 1, hot_k = 32, hot_min_active = 64, hot_rounds = 1;  // BC:1899, 1936
+    // hot-region extra sweep (BC:1899, 1936): off by default; with the
+    // terminal-cut stop and the relabel's picked entries it cost 4% on R-MAT
+    // 22 hub queries (batched and plain) and 3% on the 1024^2 grid
+    int hot = 0, hot_k = 32, hot_min_active = 64, hot_rounds = 1;
     int rm = 1, rm_freq0 = 64, rm_freq_max = 256, rm_margin = 4;    // BC:1459, 2023, 2086
     double rm_tau = 1e-5;
     int tile_rounds = 1;       // local cycles per region pass (LDS-staged)

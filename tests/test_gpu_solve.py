@@ -125,7 +125,7 @@ def test_edge_cases():
     dict(tile_frac=1e9),                        # active-list discharge every iteration
     dict(path_switch=1 << 20, p_count0=1),      # path push whenever anything is active
     dict(h_cut=0, rm_freq0=1, rm_freq_max=1),   # RM repair on every iteration
-    dict(hot_min_active=1, hot_k=4),            # hot sweep on every even iteration
+    dict(hot=1, hot_min_active=1, hot_k=4),            # hot sweep on every even iteration
     dict(gr_freq=1, gr_freq_min=1, gr_freq_max=1),
     dict(K=1, gr_freq=1000, gr_freq_min=1000, gr_freq_max=1000),
     dict(prune=0),
@@ -136,7 +136,7 @@ def test_edge_cases():
     dict(cert=1),                               # fingerprint over every pair (not only the marked ones)
     dict(spec_cert=0),                          # terminate through an exact global relabel only
     dict(sparse_count=0),                       # count actives over every vertex after each discharge
-    dict(sparse_count=1, hot_min_active=1, hot_k=4, tile_frac=0.0),  # sparse counts after tile + hot sweeps
+    dict(hot=1, sparse_count=1, hot_min_active=1, hot_k=4, tile_frac=0.0),  # sparse counts after tile + hot sweeps
     dict(small_edges=0),                        # every BFS level grid-wide (top-down or bottom-up)
     dict(small_edges=1 << 30),                  # single-workgroup levels up to kSmallFrontier vertices
     dict(mark_list=0),                          # resets and certificates from the mark map
@@ -148,7 +148,7 @@ def test_edge_cases():
     dict(tile_threads=1024, tile_frac=0.0),     # ... and in 1024-thread workgroups
     dict(tile_split=4, tile_frac=0.0),          # up to 4 workgroups per tile by excess score, every step
     dict(tile_split=8, tile_threads=256, tile_frac=0.0, tile_rounds=1),  # ... 8 of 256 threads, one round each
-    dict(tile_split=2, sparse_count=1, hot_min_active=1, hot_k=4, tile_frac=0.0),  # ... and in the hot sweep
+    dict(hot=1, tile_split=2, sparse_count=1, hot_min_active=1, hot_k=4, tile_frac=0.0),  # ... and in the hot sweep
     dict(bu_skip=0),                            # bottom-up relabel levels scan every unvisited vertex
     dict(sink_cut=0),                           # no early stop when excess(t) fills t's in-capacity
     dict(bu_pick=0),                            # bottom-up relabel levels read rows only
