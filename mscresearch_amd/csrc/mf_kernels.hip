@@ -2070,8 +2070,14 @@ __device__ __forceinline__ bool cas64(P p, int64_t& expect, int64_t want) {
                                                 __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// phase 0: every chunk; 1: each row's lead chunk only (the one holding its
+// first forward entry, where a hub's out-entries with residual start);
+// 2: the other chunks, after the lead chunks have run.  A hub's excess
+// (one preflow entry's capacity, or what its neighbours pushed) usually
+// fits into the lower entries of its lead chunk: the other chunks then find
+// the row dry and read none of their entries.
 __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__ slots,
-                                                    const int* __restrict__ list) {
+                                                    const int* __restrict__ list, int phase) {
     const GSlot S = load_slot(slots, list);
     __shared__ int64_t lscr[256 / WAVE];
     __shared__ int iscr[256 / WAVE];
@@ -2087,6 +2093,11 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
         const GTask gt = giant_task(S, task);
         const int k = gt.k, v = gt.v;
         const int e0 = gt.e0, e1 = gt.e1;
+        if (phase) {
+            const int fs = S.g.fstart ? S.g.fstart[v] : S.g.row_ptr[v];
+            const bool lead = fs >= e0 && fs < e1;
+            if (lead != (phase == 1)) continue;  // uniform per block
+        }
         // the row's excess and height first: a chunk that finds its row dry
         // (the row's other chunks took the excess) reads none of its entries
         if (threadIdx.x == 0) {
@@ -2250,7 +2261,8 @@ void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int n
 
 void launch_giant(hipStream_t st, const SlotView* slots, const int* list, int nlist) {
     hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
-    hipLaunchKernelGGL(k_giant_push, dim3(per_slot_grid(512, nlist), nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_giant_push, dim3(per_slot_grid(512, nlist), nlist), dim3(256), 0, st, slots, list, 1);
+    hipLaunchKernelGGL(k_giant_push, dim3(per_slot_grid(512, nlist), nlist), dim3(256), 0, st, slots, list, 2);
     hipLaunchKernelGGL(k_giant_relabel, dim3(1, nlist), dim3(256), 0, st, slots, list);
 }
 
