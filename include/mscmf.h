@@ -236,6 +236,7 @@ typedef struct mf_params {
     int disc_band;       /* shallow graphs: discharge only active vertices within this many levels of the lowest (0 = all) */
     int bu_pick;         /* 1: bottom-up relabel levels try each vertex's entry to its longest-row head first */
     int small_levels;    /* BFS levels one single-workgroup (or multi-workgroup) BFS launch runs at most */
+    int push_first;      /* 1: lane/wave discharge pushes first and reads the row minimum only to relabel */
 } mf_params;
 
 void mf_params_default(mf_params* p);

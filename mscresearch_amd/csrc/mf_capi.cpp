@@ -118,6 +118,7 @@ Params from_c(const mf_params* p) {
     P.disc_band = p->disc_band;
     P.bu_pick = p->bu_pick;
     P.small_levels = p->small_levels;
+    P.push_first = p->push_first;
     return P;
 }
 
@@ -519,6 +520,7 @@ void mf_params_default(mf_params* p) {
     p->disc_band = P.disc_band;
     p->bu_pick = P.bu_pick;
     p->small_levels = P.small_levels;
+    p->push_first = P.push_first;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

@@ -152,6 +152,9 @@ struct SlotView {
     // discharge only vertices with h <= disc_cap this step (lowest band
     // first); the others stay active and are counted as before
     int32_t disc_cap = 0x7fffffff;
+    // lane and wave discharge: push first, read the row minimum only to
+    // relabel (1), or the minimum first, then push (0)
+    int32_t push_first = 0;
     int32_t phase = 0;           // 0 = normal (seeds t + deficits), 1 = drain (seeds deficits, t active)
     int32_t seed_t_only = 0;     // phase 0 with no deficit (last count): the relabel seeds t alone
     // 1 = bounded injection: the preflow puts B = min(out-capacity of s,

@@ -156,7 +156,7 @@ struct GSlot {
     const GAS int32_t* mask;
     const GAS uint8_t* live;
     GAS Ctr* ctr;
-    int32_t s, t, K, Kh, H_cut, phase, seed_t_only, inj, disc_cap;
+    int32_t s, t, K, Kh, H_cut, phase, seed_t_only, inj, disc_cap, push_first;
     int32_t tstamp;  // this cycle's touch stamp (negative)
 };
 
@@ -212,6 +212,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.Kh = v.Kh;
     S.H_cut = v.H_cut;
     S.disc_cap = v.disc_cap;
+    S.push_first = v.push_first;
     S.phase = v.phase;
     S.seed_t_only = v.seed_t_only;
     S.inj = v.inject;
@@ -1826,7 +1827,7 @@ __device__ __forceinline__ void set_h(const GSlot& S, const TileCtx& tc, int v, 
 }
 
 // Light row: one lane, `cycles` discharge cycles.
-__device__ void lane_discharge(const GSlot& S, const TileCtx& tc, int v, int cycles, Work& wk) {
+__device__ void lane_discharge_minfirst(const GSlot& S, const TileCtx& tc, int v, int cycles, Work& wk) {
     const int n = S.g.n;
     const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
     int hv = own_h(S, tc, v);
@@ -1870,7 +1871,7 @@ __device__ void lane_discharge(const GSlot& S, const TileCtx& tc, int v, int cyc
 }
 
 // Heavy row: one wave, `cycles` discharge cycles; wave-uniform control.
-__device__ void wave_discharge(const GSlot& S, const TileCtx& tc, int v, int cycles, Work& wk) {
+__device__ void wave_discharge_minfirst(const GSlot& S, const TileCtx& tc, int v, int cycles, Work& wk) {
     const int n = S.g.n;
     const int lane = lane_id();
     const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
@@ -1920,6 +1921,113 @@ __device__ void wave_discharge(const GSlot& S, const TileCtx& tc, int v, int cyc
                 if (lane == 0) nex = take_excess(S, tc, v, ex, tot);
                 ex = __shfl(nex, 0, WAVE);
             }
+        }
+    }
+}
+
+// Push-first discharge (mf_params.push_first): one pass pushes the excess into lower
+// neighbours (h < hv) as it meets them and stops once the excess is gone,
+// keeping the lowest height among the entries that still have residual; only
+// a vertex whose whole row was scanned with excess left is relabelled (to
+// that height + 1, n when nothing below n - 1 remains) and pushes again at
+// once.  Under valid labels the lower neighbours are exactly those at
+// hv - 1, so the pushes are the minimum-first ones; the full-row minimum is
+// read only when a relabel needs it.
+__device__ void lane_discharge(const GSlot& S, const TileCtx& tc, int v, int cycles, Work& wk) {
+    if (!S.push_first) { lane_discharge_minfirst(S, tc, v, cycles, wk); return; }
+    const int n = S.g.n;
+    const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+    int hv = own_h(S, tc, v);
+    int64_t ex = own_excess(S, tc, v);
+    if (hv > S.disc_cap) return;  // above this step's band
+    for (int c = 0; c < cycles; ++c) {
+        if (ex <= 0 || hv >= n) break;
+        for (int pass = 0; pass < 2; ++pass) {
+            int64_t rem = ex;
+            int m = INT_MAX;
+            int e = rb;
+            for (; e < re && rem > 0; ++e) {
+                const int64_t r = ld64(&S.res[e]);
+                if (r <= 0) continue;
+                const int w = S.g.col[e];
+                const int hw = nbr_h(S, tc, w);
+                if (hw < hv) {
+                    const int64_t d = rem < r ? rem : r;
+                    move_res(S, e, d);
+                    push_into(S, tc, w, d);
+                    rem -= d;
+                    wk.pushes++;
+                    if (d < r) m = min(m, hw);
+                } else {
+                    m = min(m, hw);
+                }
+            }
+            wk.edges += (uint64_t)(e - rb);
+            const int64_t tot = ex - rem;
+            if (tot > 0) ex = take_excess(S, tc, v, ex, tot);
+            if (rem <= 0 || pass == 1) break;
+            // the whole row is scanned and excess is left: relabel
+            hv = m >= n - 1 ? n : m + 1;
+            set_h(S, tc, v, hv);
+            wk.relabels++;
+            if (hv >= n || ex <= 0) break;
+        }
+    }
+}
+
+__device__ void wave_discharge(const GSlot& S, const TileCtx& tc, int v, int cycles, Work& wk) {
+    if (!S.push_first) { wave_discharge_minfirst(S, tc, v, cycles, wk); return; }
+    const int n = S.g.n;
+    const int lane = lane_id();
+    const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+    int hv = __shfl(own_h(S, tc, v), 0, WAVE);
+    int64_t ex = __shfl(own_excess(S, tc, v), 0, WAVE);
+    if (hv > S.disc_cap) return;  // above this step's band (wave-uniform)
+    for (int c = 0; c < cycles; ++c) {
+        if (ex <= 0 || hv >= n) break;
+        for (int pass = 0; pass < 2; ++pass) {
+            int64_t rem = ex;
+            int m = INT_MAX;
+            int base = rb;
+            for (; base < re && rem > 0; base += WAVE) {
+                const int e = base + lane;
+                int64_t r = 0;
+                int hw = INT_MAX, w = 0;
+                if (e < re) {
+                    r = ld64(&S.res[e]);
+                    if (r > 0) {
+                        w = S.g.col[e];
+                        hw = nbr_h(S, tc, w);
+                    } else {
+                        r = 0;
+                    }
+                }
+                const int64_t a = (r > 0 && hw < hv) ? r : 0;  // into a lower neighbour
+                const int64_t incl = wave_incl_scan_i64(a);
+                const int64_t excl = incl - a;
+                const int64_t d = (a > 0 && rem > excl) ? ((rem - excl) < a ? (rem - excl) : a) : 0;
+                if (d > 0) {
+                    move_res(S, e, d);
+                    push_into(S, tc, w, d);
+                    wk.pushes++;
+                }
+                if (r > d) m = min(m, hw);
+                const int64_t chunk = __shfl(incl, WAVE - 1, WAVE);
+                rem -= (chunk < rem ? chunk : rem);
+            }
+            if (lane == 0) wk.edges += (uint64_t)(min(base, re) - rb);
+            const int64_t tot = ex - rem;
+            if (tot > 0) {
+                int64_t nex = 0;
+                if (lane == 0) nex = take_excess(S, tc, v, ex, tot);
+                ex = __shfl(nex, 0, WAVE);
+            }
+            if (rem <= 0 || pass == 1) break;
+            // the whole row is scanned and excess is left: relabel
+            m = wave_min_i32(m);
+            hv = m >= n - 1 ? n : m + 1;
+            if (lane == 0) { set_h(S, tc, v, hv); wk.relabels++; }
+            if (hv >= n || ex <= 0) break;
         }
     }
 }

@@ -152,6 +152,8 @@ def test_edge_cases():
     dict(bu_skip=0),                            # bottom-up relabel levels scan every unvisited vertex
     dict(sink_cut=0),                           # no early stop when excess(t) fills t's in-capacity
     dict(bu_pick=0),                            # bottom-up relabel levels read rows only
+    dict(push_first=1),                         # lane/wave discharge: push first, minimum only to relabel
+    dict(push_first=1, tile_frac=0.0),          # ... in tiles
     dict(disc_band=1),                          # discharge the lowest two active levels only
     dict(disc_band=2, sink_cut=0, tile_frac=0.0),  # ... three, to the end, in tiles
     dict(bu_skip=1, small_edges=0),             # ... skip those that cannot reach t's SCC, every level grid-wide
