@@ -155,6 +155,10 @@ struct SlotView {
     // lane and wave discharge: push first, read the row minimum only to
     // relabel (1), or the minimum first, then push (0)
     int32_t push_first = 0;
+    // the slab's entry marks cover its residual (reset from the capacities,
+    // changed since only by marked pushes and updates): an unmarked entry
+    // holds at least its capacity
+    int32_t marks_ok = 0;
     int32_t phase = 0;           // 0 = normal (seeds t + deficits), 1 = drain (seeds deficits, t active)
     int32_t seed_t_only = 0;     // phase 0 with no deficit (last count): the relabel seeds t alone
     // 1 = bounded injection: the preflow puts B = min(out-capacity of s,

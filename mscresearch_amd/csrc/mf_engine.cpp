@@ -1312,6 +1312,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     for (int i = 0; i < nslots_; ++i) {
         hv[i] = slots_[i].view;
         if (!P_.bu_pick) hv[i].g.bu_pick = nullptr;
+        hv[i].marks_ok = slots_[i].slab.marks_ok ? 1 : 0;
     }
     size_t lpos = 0;
     struct Group { int off = 0, cnt = 0; };

@@ -156,7 +156,7 @@ struct GSlot {
     const GAS int32_t* mask;
     const GAS uint8_t* live;
     GAS Ctr* ctr;
-    int32_t s, t, K, Kh, H_cut, phase, seed_t_only, inj, disc_cap, push_first;
+    int32_t s, t, K, Kh, H_cut, phase, seed_t_only, inj, disc_cap, push_first, marks_ok;
     int32_t tstamp;  // this cycle's touch stamp (negative)
 };
 
@@ -213,6 +213,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.H_cut = v.H_cut;
     S.disc_cap = v.disc_cap;
     S.push_first = v.push_first;
+    S.marks_ok = v.marks_ok;
     S.phase = v.phase;
     S.seed_t_only = v.seed_t_only;
     S.inj = v.inject;
@@ -919,9 +920,12 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
             wk.verts++;
             const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
             if (MODE == BFS_GR && S.g.bu_pick) {
-                // the picked entry first (its head's level is likely cached)
+                // the picked entry first (its head's level is likely cached);
+                // a forward entry no push has marked still holds its
+                // capacity, so its mark byte (128 entries a line) answers
+                // for its residual (16 a line)
                 const int pe = S.g.bu_pick[2 * v], pw = S.g.bu_pick[2 * v + 1];
-                if (pe >= 0 && lvl[pw] == L && ld_res(&S.res[pe]) > 0) {
+                if (pe >= 0 && lvl[pw] == L && ((S.marks_ok && S.emark[pe] == 0) || ld_res(&S.res[pe]) > 0)) {
                     wk.edges += 1;
                     lvl[v] = L + 1;  // only this lane writes v
                     stage_push(st, v, re - rb, out, ocnt, odeg);
