@@ -237,6 +237,7 @@ typedef struct mf_params {
     int bu_pick;         /* 1: bottom-up relabel levels try each vertex's entry to its longest-row head first */
     int small_levels;    /* BFS levels one single-workgroup (or multi-workgroup) BFS launch runs at most */
     int push_first;      /* 1: lane/wave discharge pushes first and reads the row minimum only to relabel */
+    int slot_blocks, launch_blocks; /* grid-stride launches: workgroups per slot / per launch at most */
 } mf_params;
 
 void mf_params_default(mf_params* p);

@@ -51,7 +51,7 @@ class Params(C.Structure):
         ("tile_threads", C.c_int), ("path_switch_deep", C.c_int), ("p_count_deep", C.c_int),
         ("tile_split", C.c_int), ("scc_route", C.c_int), ("bu_skip", C.c_int), ("sink_cut", C.c_int),
         ("disc_band", C.c_int), ("bu_pick", C.c_int), ("small_levels", C.c_int),
-        ("push_first", C.c_int),
+        ("push_first", C.c_int), ("slot_blocks", C.c_int), ("launch_blocks", C.c_int),
     ]
 
 

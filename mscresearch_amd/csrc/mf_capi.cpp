@@ -119,6 +119,8 @@ Params from_c(const mf_params* p) {
     P.bu_pick = p->bu_pick;
     P.small_levels = p->small_levels;
     P.push_first = p->push_first;
+    P.slot_blocks = p->slot_blocks;
+    P.launch_blocks = p->launch_blocks;
     return P;
 }
 
@@ -521,6 +523,8 @@ void mf_params_default(mf_params* p) {
     p->bu_pick = P.bu_pick;
     p->small_levels = P.small_levels;
     p->push_first = P.push_first;
+    p->slot_blocks = P.slot_blocks;
+    p->launch_blocks = P.launch_blocks;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

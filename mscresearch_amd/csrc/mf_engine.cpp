@@ -1488,7 +1488,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     // (several times what the device holds at once), so each workgroup's
     // fixed costs (staging, block reductions, barriers) spread over several
     // chunks
-    constexpr int kLaunchBlocks = 16384, kSlotBlocks = 4096;
+    const int kLaunchBlocks = std::max(64, P_.launch_blocks), kSlotBlocks = std::max(16, P_.slot_blocks);
     auto fit = [&](const Group& g, int gx) {
         const int cap = std::min(kSlotBlocks, std::max(64, kLaunchBlocks / std::max(1, g.cnt)));
         return std::max(1, std::min(gx, cap));

@@ -97,6 +97,12 @@ struct Params {
     // read the row's minimum only to relabel (1); 0 = minimum pass, then
     // pushes.  Measured neutral on R-MAT 22 and 3% slower on the grid
     int push_first = 0;
+    // grid-stride launches: at most slot_blocks workgroups per slot and
+    // about launch_blocks per launch, so each workgroup's fixed costs
+    // (staging, block sums, counter atomics) spread over several chunks.
+    // On R-MAT 22 hub queries 768 per slot beat 256, 512, 1024 (-1 to -2%),
+    // 2048 and 4096 (+12% batched, +21% plain over 4096); the grid +4%
+    int slot_blocks = 768, launch_blocks = 16384;
     // shallow graphs: discharge only active vertices within disc_band
     // levels of the lowest active height (0 = every active vertex)
     int disc_band = 0;

@@ -305,18 +305,31 @@ __device__ __forceinline__ void move_res(const GSlot& S, int e, int64_t d) {
 }
 
 __device__ __forceinline__ void work_flush(const GSlot& S, Work w, uint64_t* red, int fam) {
-    // Four block sums, one atomic each from thread 0.
-    const uint64_t e = block_sum<uint64_t>(w.edges, red);
-    const uint64_t v = block_sum<uint64_t>(w.verts, red);
-    const uint64_t p = block_sum<uint64_t>(w.pushes, red);
-    const uint64_t r = block_sum<uint64_t>(w.relabels, red);
+    // The four counters in one block reduction (one barrier pair, the
+    // workgroup's fixed cost in a grid-stride kernel), one atomic each from
+    // thread 0.  Up to 1024 threads.
+    (void)red;
+    __shared__ uint64_t wred[4][1024 / WAVE];
+    uint64_t x[4] = {w.edges, w.verts, w.pushes, w.relabels};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        for (int o = WAVE / 2; o > 0; o >>= 1) x[k] += __shfl_xor(x[k], o, WAVE);
+    const int wv = threadIdx.x / WAVE;
+    if (lane_id() == 0)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wred[k][wv] = x[k];
+    __syncthreads();
     if (threadIdx.x == 0) {
+        uint64_t t[4] = {0, 0, 0, 0};
+        for (int i = 0; i < (int)(blockDim.x / WAVE); ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) t[k] += wred[k][i];
         GAS uint64_t* c = S.ctr->wk[fam];
-        if (e) addu64(&c[0], e);
-        if (v) addu64(&c[1], v);
-        if (p) addu64(&c[2], p);
-        if (r) addu64(&c[3], r);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (t[k]) addu64(&c[k], t[k]);
     }
+    __syncthreads();
 }
 
 }  // namespace
