@@ -238,6 +238,7 @@ typedef struct mf_params {
     int small_levels;    /* BFS levels one single-workgroup (or multi-workgroup) BFS launch runs at most */
     int push_first;      /* 1: lane/wave discharge pushes first and reads the row minimum only to relabel */
     int slot_blocks, launch_blocks; /* grid-stride launches: workgroups per slot / per launch at most */
+    int bu_light;        /* bottom-up rows up to this length are scanned by one lane, longer ones by a wave */
 } mf_params;
 
 void mf_params_default(mf_params* p);

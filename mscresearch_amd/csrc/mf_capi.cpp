@@ -121,6 +121,7 @@ Params from_c(const mf_params* p) {
     P.push_first = p->push_first;
     P.slot_blocks = p->slot_blocks;
     P.launch_blocks = p->launch_blocks;
+    P.bu_light = p->bu_light;
     return P;
 }
 
@@ -525,6 +526,7 @@ void mf_params_default(mf_params* p) {
     p->push_first = P.push_first;
     p->slot_blocks = P.slot_blocks;
     p->launch_blocks = P.launch_blocks;
+    p->bu_light = P.bu_light;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

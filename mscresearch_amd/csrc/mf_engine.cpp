@@ -967,6 +967,7 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
     v.Kh = std::max(1, P_.K_heavy);
     v.inject = (P_.inject && !S.pinned && !warm) ? 1 : 0;
     v.push_first = P_.push_first ? 1 : 0;
+    v.bu_light = std::max(1, std::min(P_.bu_light, kGiantDeg));
     const bool prune = P_.prune != 0 && prune_ok && !warm;
     S.sched.begin(&P_, G.g, q.s, q.t, prune, warm);
     S.sched.set_depth_hint(G.depth_hint.load(std::memory_order_relaxed));

@@ -103,6 +103,9 @@ struct Params {
     // On R-MAT 22 hub queries 768 per slot beat 256, 512, 1024 (-1 to -2%),
     // 2048 and 4096 (+12% batched, +21% plain over 4096); the grid +4%
     int slot_blocks = 768, launch_blocks = 16384;
+    // bottom-up rows up to bu_light entries go to one lane each, longer ones
+    // to a wave each (k_bfs_bu_med)
+    int bu_light = 128;
     // shallow graphs: discharge only active vertices within disc_band
     // levels of the lowest active height (0 = every active vertex)
     int disc_band = 0;

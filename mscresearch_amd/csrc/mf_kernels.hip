@@ -156,7 +156,7 @@ struct GSlot {
     const GAS int32_t* mask;
     const GAS uint8_t* live;
     GAS Ctr* ctr;
-    int32_t s, t, K, Kh, H_cut, phase, seed_t_only, inj, disc_cap, push_first, marks_ok;
+    int32_t s, t, K, Kh, H_cut, phase, seed_t_only, inj, disc_cap, push_first, marks_ok, bu_light;
     int32_t tstamp;  // this cycle's touch stamp (negative)
 };
 
@@ -214,6 +214,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.disc_cap = v.disc_cap;
     S.push_first = v.push_first;
     S.marks_ok = v.marks_ok;
+    S.bu_light = v.bu_light;
     S.phase = v.phase;
     S.seed_t_only = v.seed_t_only;
     S.inj = v.inject;
@@ -899,11 +900,6 @@ __global__ __launch_bounds__(256) void k_bfs_td_heavy(const SlotView* __restrict
     work_flush(S, wk, red, WF_BFS);
 }
 
-// Bottom-up rows up to this many entries are scanned by one lane (a row
-// stops at its first parent, so a wave's 64-entry steps would mostly load
-// entries past it); longer ones by a wave.
-constexpr int kBuLightDeg = 128;
-
 // One thread per vertex, no workgroup barrier inside the loop (a wave that
 // meets a long scan holds back only itself): light rows are scanned by
 // their lane; medium rows go to hl (k_bfs_bu_med, a wave each), giant rows
@@ -948,7 +944,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
             }
             if (re - rb > kGiantDeg) {
                 S.giant[add32(&S.ctr->ngiant, 1)] = v;
-            } else if (re - rb > kBuLightDeg) {
+            } else if (re - rb > S.bu_light) {
                 S.hl[add32(&S.ctr->nhl, 1)] = v;
             } else {
                 // the level gather only for entries that pass the residual
@@ -970,7 +966,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
     work_flush(S, wk, red, WF_BFS_BU);
 }
 
-// Medium unvisited rows of a bottom-up level (kBuLightDeg < length <=
+// Medium unvisited rows of a bottom-up level (SlotView::bu_light < length <=
 // kGiantDeg), listed in hl by k_bfs_bu: a wave each, grid-stride, 64
 // entries per step (ballot), stopping at the first step holding a parent.
 template <int MODE>
@@ -2201,8 +2197,10 @@ __device__ __forceinline__ bool cas64(P p, int64_t& expect, int64_t want) {
 // (one preflow entry's capacity, or what its neighbours pushed) usually
 // fits into the lower entries of its lead chunk: the other chunks then find
 // the row dry and read none of their entries.
+template <int PHASE>
 __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__ slots,
-                                                    const int* __restrict__ list, int phase) {
+                                                    const int* __restrict__ list) {
+    constexpr int phase = PHASE;
     const GSlot S = load_slot(slots, list);
     __shared__ int64_t lscr[256 / WAVE];
     __shared__ int iscr[256 / WAVE];
@@ -2218,7 +2216,7 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
         const GTask gt = giant_task(S, task);
         const int k = gt.k, v = gt.v;
         const int e0 = gt.e0, e1 = gt.e1;
-        if (phase) {
+        if (phase != 0) {
             const int fs = S.g.fstart ? S.g.fstart[v] : S.g.row_ptr[v];
             const bool lead = fs >= e0 && fs < e1;
             if (lead != (phase == 1)) continue;  // uniform per block
@@ -2386,8 +2384,8 @@ void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int n
 
 void launch_giant(hipStream_t st, const SlotView* slots, const int* list, int nlist) {
     hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
-    hipLaunchKernelGGL(k_giant_push, dim3(per_slot_grid(512, nlist), nlist), dim3(256), 0, st, slots, list, 1);
-    hipLaunchKernelGGL(k_giant_push, dim3(per_slot_grid(512, nlist), nlist), dim3(256), 0, st, slots, list, 2);
+    hipLaunchKernelGGL(k_giant_push<1>, dim3(per_slot_grid(512, nlist), nlist), dim3(256), 0, st, slots, list);
+    hipLaunchKernelGGL(k_giant_push<2>, dim3(per_slot_grid(512, nlist), nlist), dim3(256), 0, st, slots, list);
     hipLaunchKernelGGL(k_giant_relabel, dim3(1, nlist), dim3(256), 0, st, slots, list);
 }
 
