@@ -122,6 +122,7 @@ Params from_c(const mf_params* p) {
     P.slot_blocks = p->slot_blocks;
     P.launch_blocks = p->launch_blocks;
     P.bu_light = p->bu_light;
+    P.bu_bits = p->bu_bits;
     return P;
 }
 
@@ -527,6 +528,7 @@ void mf_params_default(mf_params* p) {
     p->slot_blocks = P.slot_blocks;
     p->launch_blocks = P.launch_blocks;
     p->bu_light = P.bu_light;
+    p->bu_bits = P.bu_bits;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {
@@ -892,7 +894,10 @@ static int dbg_bfs(mf_engine* e, Plan first, int mode) {
         p.stages = ST_BFS;
         p.bfs_mode = mode;
         const int64_t mu = std::max<int64_t>(0, (int64_t)g.E - medges);
-        if (!bu && (double)c.fedges[L & 3] > (double)mu / P.bu_alpha) bu = true;
+        if (!bu && (double)c.fedges[L & 3] > (double)mu / P.bu_alpha) {
+            bu = true;
+            p.bu_bits = true;
+        }
         else if (bu && (double)f < (double)g.n / P.bu_beta) bu = false;
         if (bu) { p.bfs_kind = BK_BU; p.bfs_grid = std::max(1, std::min(8192, (g.n + 255) / 256)); }
         else if (f <= kSmallFrontier && c.fedges[L & 3] <= P.small_edges) p.bfs_kind = BK_SMALL;

@@ -81,6 +81,9 @@ struct Ctr {
     int32_t hmin, hmax;      // active height band
     int32_t cert_bad;        // certificate violations
     int32_t err;             // device-side error flags
+    // level whose frontier bitmap SlotView::fbits holds (-1: none yet in
+    // this BFS); a bottom-up level uses the bitmap only when it is its own
+    int32_t fb_level;
     int64_t cut;             // certificate: cut(V\T,T)
     uint64_t fp;             // certificate: pair-sum fingerprint (0 when every pair sums to its capacity)
     int64_t ex_t;            // excess of t seen by the last active / certificate pass
@@ -133,6 +136,11 @@ struct SlotView {
     int32_t* giant = nullptr;
     int64_t* gacc = nullptr;
     int32_t gcap = 0;
+    // bottom-up BFS levels: two bitmaps of ceil(n / 64) words, the vertices
+    // on level L in fbits[(L & 1) * nw ...] (bit v & 63 of word v >> 6); a
+    // level tests its candidate parents here (n / 8 bytes, which stay in
+    // L2) instead of gathering their 4-byte levels
+    uint64_t* fbits = nullptr;
     uint8_t* emark = nullptr;    // E: entry's pair changed since the residual was last reset
     // the marked entries in marking order: mlist[0] = count, entries from
     // mlist[1]; past mcap entries only the map is kept (count > mcap)

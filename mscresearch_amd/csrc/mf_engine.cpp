@@ -224,6 +224,7 @@ size_t Slab::layout(int n, int64_t E, int ntiles, char* base, Slab* sl) {
     char* p_lb = take(sizeof(int32_t) * 2 * kLocalLevels);
     char* p_gi = take(sizeof(int32_t) * (giant_task_offset((int64_t)gcap) + 4 * (ee / kGiantChunk + gcap + 1)));
     char* p_ga = take(sizeof(int64_t) * gcap);
+    char* p_fb2 = take(16 * ((nn + 63) / 64));
     char* p_em = take(ee);
     const size_t mcap = std::min(ee, (size_t)kMarkListMax);
     char* p_ml = take(sizeof(int32_t) * (mcap + 1));
@@ -243,6 +244,7 @@ size_t Slab::layout(int n, int64_t E, int ntiles, char* base, Slab* sl) {
         sl->lbkt = (int32_t*)p_lb;
         sl->giant = (int32_t*)p_gi;
         sl->gacc = (int64_t*)p_ga;
+        sl->fbits = (uint64_t*)p_fb2;
         sl->emark = (uint8_t*)p_em;
         sl->mlist = (int32_t*)p_ml;
         sl->mcap = (int)mcap;
@@ -415,9 +417,10 @@ Plan SlotSched::bfs_step(const Ctr& c, int mode) {
     p.stages = ST_BFS;
     p.bfs_mode = mode;
     const int64_t mu = std::max<int64_t>(0, (int64_t)g_.E - bfs_medges_);
-    if (!bfs_bottom_up_ && (double)mf > (double)mu / P_->bu_alpha)
+    if (!bfs_bottom_up_ && (double)mf > (double)mu / P_->bu_alpha) {
         bfs_bottom_up_ = true;
-    else if (bfs_bottom_up_ && (double)f < (double)g_.n / P_->bu_beta)
+        p.bu_bits = true;  // level L came from a top-down or single-workgroup level
+    } else if (bfs_bottom_up_ && (double)f < (double)g_.n / P_->bu_beta)
         bfs_bottom_up_ = false;
     if (bfs_bottom_up_) {
         p.bfs_kind = BK_BU;
@@ -959,6 +962,7 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
     v.lbkt = sl.lbkt;
     v.giant = sl.giant; v.gacc = sl.gacc; v.gcap = sl.gcap;
     v.emark = sl.emark;
+    v.fbits = P_.bu_bits ? sl.fbits : nullptr;
     v.mlist = sl.mlist;
     v.mcap = std::max(0, std::min(sl.mcap, P_.mark_list));
     v.ctr = d_ctrs_ + i;
@@ -1400,6 +1404,11 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
             g_bfs[k][m] = make_list([&](int i) {
                 return has(i, ST_BFS) && plans[i].bfs_kind == k && plans[i].bfs_mode == m && (k != BK_SMALL || plans[i].small_wgs <= 1);
             });
+    Group g_bubits[3];  // first bottom-up level of a BFS: level L's bitmap from the levels
+    for (int m = 0; m < 3; ++m)
+        g_bubits[m] = make_list([&](int i) {
+            return has(i, ST_BFS) && plans[i].bfs_kind == BK_BU && plans[i].bfs_mode == m && plans[i].bu_bits && P_.bu_bits;
+        });
     for (int m = 0; m < 3; ++m)
         g_bfs_multi[m] = make_list([&](int i) {
             return has(i, ST_BFS) && plans[i].bfs_kind == BK_SMALL && plans[i].bfs_mode == m && plans[i].small_wgs > 1;
@@ -1569,6 +1578,8 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         }
         if (g_bfs[BK_BU][m].cnt) {
             const int k = tb(WF_BFS_BU);
+            if (g_bubits[m].cnt)
+                launch_bu_bits(st, d_views_, L(g_bubits[m]), g_bubits[m].cnt, fit(g_bubits[m], maxof(g_bubits[m], [&](int i) { return plans[i].bfs_grid; })), m);
             launch_bfs_bu(st, d_views_, L(g_bfs[BK_BU][m]), g_bfs[BK_BU][m].cnt, fit(g_bfs[BK_BU][m], maxof(g_bfs[BK_BU][m], [&](int i) { return plans[i].bfs_grid; })), m);
             te(k);
         }

@@ -59,7 +59,11 @@ struct Params {
     // 1024-thread workgroup.  More meet at a barrier in global memory after
     // each level, so a level's scattered loads use several CUs; a shallow
     // BFS keeps one workgroup, its few levels not worth a barrier each.
-    int small_wgs = 8;
+    // Grid 1024^2, 128 queries in 64 slots: 4 per slot 44 q/s against 1, 2
+    // and 8 at 40, 41 and 35 (every barrier's agent-scope fences and polls
+    // cost more with more workgroups); one slot alone is the same at 4, 8
+    // and 16.  A launch also holds at most 256 of them (launch_bfs_small).
+    int small_wgs = 4;
     // a frontier whose rows sum to more entries than this leaves the
     // single-workgroup BFS for the grid-wide top-down or bottom-up levels:
     // a hub's row is spread over the device instead of one CU (+4% on R-MAT
@@ -106,6 +110,9 @@ struct Params {
     // bottom-up rows up to bu_light entries go to one lane each, longer ones
     // to a wave each (k_bfs_bu_med)
     int bu_light = 128;
+    // bottom-up levels test candidate parents in a frontier bitmap (n / 8
+    // bytes per slot) instead of gathering their levels (0 = gather levels)
+    int bu_bits = 1;
     // shallow graphs: discharge only active vertices within disc_band
     // levels of the lowest active height (0 = every active vertex)
     int disc_band = 0;
@@ -265,6 +272,7 @@ struct Slab {
     int32_t* giant = nullptr;
     int64_t* gacc = nullptr;
     int gcap = 0;
+    uint64_t* fbits = nullptr;  // 2 x ceil(n / 64) frontier bitmap words (SlotView::fbits)
     uint8_t* emark = nullptr;   // E marks of changed entry pairs
     int32_t* mlist = nullptr;   // count + the marked entries (SlotView::mlist)
     int mcap = 0;
@@ -332,6 +340,7 @@ struct Plan {
     bool sparse = false;      // ST_ACTIVE: count the previous list and the touched vertices only
     int small_wgs = 1;        // BK_SMALL: workgroups per slot (1 or Params::small_wgs)
     bool seed_t_only = false; // ST_GR_SEED in phase 0 after a count that saw no deficit: t is the one seed
+    bool bu_bits = false;     // BK_BU: the first bottom-up level of this BFS builds level L's bitmap first
 };
 
 // Per-query statistics (host side).

@@ -59,6 +59,9 @@ void launch_bfs_seed1(hipStream_t st, const SlotView* slots, const int* list, in
 void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
                    int mode);
 // One bottom-up level.
+// First bottom-up level of a BFS: the level-L bitmap (SlotView::fbits) from
+// the levels.
+void launch_bu_bits(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx, int mode);
 void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
                    int mode);
 // Multi-level top-down BFS while the frontier stays small: one workgroup

@@ -150,6 +150,7 @@ struct GSlot {
     GAS int32_t* giant;
     GAS int64_t* gacc;
     int32_t gcap;
+    GAS uint64_t* fbits;
     GAS uint8_t* emark;
     GAS int32_t* mlist;
     int32_t mcap;
@@ -200,6 +201,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.giant = as_global(v.giant);
     S.gacc = as_global(v.gacc);
     S.gcap = v.gcap;
+    S.fbits = as_global(v.fbits);
     S.emark = as_global(v.emark);
     S.mlist = as_global(v.mlist);
     S.mcap = v.mcap;
@@ -350,6 +352,7 @@ __global__ void k_reset(const SlotView* __restrict__ slots, const int* __restric
             c->ngiant = 0;
             c->bar_count = 0;
             c->visited = 0;
+            c->fb_level = -1;
         }
         if (what & (RST_ACTIVE | RST_SPARSE)) {
             c->prev_nlight = c->nlight;
@@ -560,6 +563,29 @@ __device__ __forceinline__ bool bfs_bu_edge_ok(const GSlot& S, int e) {
     return true;
 }
 
+// Frontier bitmaps of the bottom-up levels (SlotView::fbits): words per
+// bitmap, level L's bitmap, and "is x on level L" from it (a 4-byte load
+// from n / 8 bytes) or, without bitmaps, from x's level.
+__device__ __forceinline__ int fb_words(int n) { return (n + 63) >> 6; }
+__device__ __forceinline__ GAS uint64_t* fb_level(const GSlot& S, int L) {
+    return S.fbits + (size_t)(L & 1) * (size_t)fb_words(S.g.n);
+}
+// Level L's bitmap as 32-bit words when it is complete (Ctr::fb_level),
+// else nullptr: candidate parents are then tested by their levels.
+__device__ __forceinline__ const GAS uint32_t* fb_valid(const GSlot& S, int L) {
+    return (S.fbits && S.ctr->fb_level == L) ? (const GAS uint32_t*)fb_level(S, L) : nullptr;
+}
+__device__ __forceinline__ bool on_level(const GAS uint32_t* bits, const GAS int32_t* lvl, int x, int L) {
+    if (bits) return (bits[x >> 5] >> (x & 31)) & 1u;
+    return lvl[x] == L;
+}
+// Vertex x joined level L + 1 outside the wave that owns its bitmap word.
+__device__ __forceinline__ void fb_set_next(const GSlot& S, int x, int L) {
+    if (S.fbits)
+        __hip_atomic_fetch_or(fb_level(S, L + 1) + (x >> 6), 1ull << (x & 63), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int MODE>
 __device__ __forceinline__ bool bfs_vertex_ok(const GSlot& S, int v) {
     if (MODE == BFS_GR) {
@@ -681,10 +707,10 @@ __device__ __forceinline__ void td_entries_deep(const GSlot& S, int e0, int e1, 
 // of the group holding the first such entry.
 template <int MODE, int U>
 __device__ __forceinline__ bool bu_range_lane(const GSlot& S, int rb, int re, int L, const GAS int32_t* lvl,
-                                              int& scanned) {
+                                              const GAS uint32_t* bits, int& scanned) {
     for (int e = rb; e < re; e += U) {
         bool ok[U];
-        int c[U], l[U];
+        int c[U];
 #pragma unroll
         for (int k = 0; k < U; ++k) {
             const int ek = e + k;
@@ -695,11 +721,12 @@ __device__ __forceinline__ bool bu_range_lane(const GSlot& S, int rb, int re, in
                 c[k] = S.g.col[ek];
             }
         }
+        bool on[U];
 #pragma unroll
-        for (int k = 0; k < U; ++k) l[k] = ok[k] ? lvl[c[k]] : -1;
+        for (int k = 0; k < U; ++k) on[k] = ok[k] && on_level(bits, lvl, c[k], L);
         bool hit = false;
 #pragma unroll
-        for (int k = 0; k < U; ++k) hit |= l[k] == L;
+        for (int k = 0; k < U; ++k) hit |= on[k];
         if (hit) {
             scanned = min(e + U, re) - rb;
             return true;
@@ -722,14 +749,14 @@ __device__ __forceinline__ int bu_first(const GSlot& S, int v, int rb) {
 // [rb, fs).
 template <int MODE, int U>
 __device__ __forceinline__ bool bu_row_lane(const GSlot& S, int rb, int fs, int re, int L, const GAS int32_t* lvl,
-                                            int& scanned) {
+                                            const GAS uint32_t* bits, int& scanned) {
     int a = 0;
-    if (bu_range_lane<MODE, U>(S, fs, re, L, lvl, a)) {
+    if (bu_range_lane<MODE, U>(S, fs, re, L, lvl, bits, a)) {
         scanned = a;
         return true;
     }
     int b = 0;
-    const bool hit = fs > rb && bu_range_lane<MODE, U>(S, rb, fs, L, lvl, b);
+    const bool hit = fs > rb && bu_range_lane<MODE, U>(S, rb, fs, L, lvl, bits, b);
     scanned = a + b;
     return hit;
 }
@@ -922,42 +949,54 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
     }
     __syncthreads();
     const int n = S.g.n;
+    // with bitmaps, level L + 1's bitmap is written here word by word: a
+    // wave's 64 vertices are one word (every word, so no clearing), and the
+    // medium and giant rows listed for the kernels after this one are OR-ed
+    // in by them
+    GAS uint64_t* nbits = S.fbits ? fb_level(S, L + 1) : nullptr;
+    const GAS uint32_t* bits = fb_valid(S, L);
     Work wk;
-    for (int64_t vi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; vi < n; vi += (int64_t)gridDim.x * blockDim.x) {
-        const int v = (int)vi;
-        if (lvl[v] == n && bfs_vertex_ok<MODE>(S, v) && (MODE != BFS_GR || S.live == nullptr || S.live[v])) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
+        const int v = (int)base + lane_id();
+        bool joined = false;
+        if (v < n && lvl[v] == n && bfs_vertex_ok<MODE>(S, v) && (MODE != BFS_GR || S.live == nullptr || S.live[v])) {
             wk.verts++;
             const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+            bool picked = false;
             if (MODE == BFS_GR && S.g.bu_pick) {
                 // the picked entry first (its head's level is likely cached);
                 // a forward entry no push has marked still holds its
                 // capacity, so its mark byte (128 entries a line) answers
                 // for its residual (16 a line)
                 const int pe = S.g.bu_pick[2 * v], pw = S.g.bu_pick[2 * v + 1];
-                if (pe >= 0 && lvl[pw] == L && ((S.marks_ok && S.emark[pe] == 0) || ld_res(&S.res[pe]) > 0)) {
+                picked = pe >= 0 && on_level(bits, lvl, pw, L) && ((S.marks_ok && S.emark[pe] == 0) || ld_res(&S.res[pe]) > 0);
+                if (picked) {
                     wk.edges += 1;
-                    lvl[v] = L + 1;  // only this lane writes v
-                    stage_push(st, v, re - rb, out, ocnt, odeg);
-                    wk.relabels++;
-                    continue;
+                    joined = true;
                 }
             }
-            if (re - rb > kGiantDeg) {
+            if (picked) {
+            } else if (re - rb > kGiantDeg) {
                 S.giant[add32(&S.ctr->ngiant, 1)] = v;
             } else if (re - rb > S.bu_light) {
                 S.hl[add32(&S.ctr->nhl, 1)] = v;
             } else {
-                // the level gather only for entries that pass the residual
+                // the parent test only for entries that pass the residual
                 // test (most reverse entries carry no flow)
                 int scanned = 0;
-                const bool hit = bu_row_lane<MODE, 2>(S, rb, bu_first<MODE>(S, v, rb), re, L, lvl, scanned);
+                joined = bu_row_lane<MODE, 2>(S, rb, bu_first<MODE>(S, v, rb), re, L, lvl, bits, scanned);
                 wk.edges += (uint64_t)scanned;
-                if (hit) {
-                    lvl[v] = L + 1;  // only this lane writes v
-                    stage_push(st, v, re - rb, out, ocnt, odeg);
-                    wk.relabels++;
-                }
             }
+            if (joined) {
+                lvl[v] = L + 1;  // only this lane writes v
+                stage_push(st, v, re - rb, out, ocnt, odeg);
+                wk.relabels++;
+            }
+        }
+        if (nbits) {
+            const uint64_t m = __ballot(joined);
+            if (lane_id() == 0) nbits[base >> 6] = m;
         }
     }
     stage_flush(st, out, ocnt, odeg);
@@ -980,6 +1019,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu_med(const SlotView* __restrict__
     if ((int)blockIdx.x * wpb >= nm) return;  // uniform per block
     const int L = S.ctr->level;
     GAS int32_t* lvl = bfs_lvl<MODE>(S);
+    const GAS uint32_t* bits = fb_valid(S, L);
     GAS int32_t* out = (L & 1) ? S.fa : S.fb;
     GAS int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
     GAS int64_t* odeg = &S.ctr->fedges[(L + 1) & 3];
@@ -1002,7 +1042,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu_med(const SlotView* __restrict__
             if (k0 + lane < d) {
                 const bool r = bfs_bu_edge_ok<MODE>(S, e);
                 const int c = S.g.col[e];  // issued with the residual load
-                ok = r && lvl[c] == L;
+                ok = r && on_level(bits, lvl, c, L);
             }
             scanned += min(WAVE, d - k0);
             if (__ballot(ok)) { hit = true; break; }
@@ -1011,6 +1051,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu_med(const SlotView* __restrict__
             wk.edges += (uint64_t)scanned;
             if (hit) {
                 lvl[u] = L + 1;  // only this wave writes u
+                fb_set_next(S, u, L);
                 stage_push(st, u, re - rb, out, ocnt, odeg);
                 wk.relabels++;
             }
@@ -1035,6 +1076,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu_heavy(const SlotView* __restrict
     if ((int)blockIdx.x >= nh) return;  // uniform per block
     const int L = S.ctr->level;
     GAS int32_t* lvl = bfs_lvl<MODE>(S);
+    const GAS uint32_t* bits = fb_valid(S, L);
     GAS int32_t* out = (L & 1) ? S.fa : S.fb;
     GAS int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
     GAS int64_t* odeg = &S.ctr->fedges[(L + 1) & 3];
@@ -1055,7 +1097,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu_heavy(const SlotView* __restrict
                 const int e = fs + kk < re ? fs + kk : fs + kk - d;
                 const bool r = bfs_bu_edge_ok<MODE>(S, e);
                 const int c = S.g.col[e];  // issued with the residual load
-                if (r && lvl[c] == L) found = 1;
+                if (r && on_level(bits, lvl, c, L)) found = 1;
             }
             __syncthreads();
             const int f = found;
@@ -1066,6 +1108,7 @@ __global__ __launch_bounds__(256) void k_bfs_bu_heavy(const SlotView* __restrict
             wk.edges += (uint64_t)min(k0 + (int)blockDim.x, d);
             if (found) {
                 lvl[v] = L + 1;
+                fb_set_next(S, v, L);
                 stage_push(st, v, re - rb, out, ocnt, odeg);
                 wk.relabels++;
             }
@@ -1078,10 +1121,12 @@ __global__ __launch_bounds__(256) void k_bfs_bu_heavy(const SlotView* __restrict
     work_flush(S, wk, red, WF_BFS_BU);
 }
 
-// Level bookkeeping after a multi-workgroup level: level += 1.
-__global__ void k_bfs_advance(const SlotView* __restrict__ slots, const int* __restrict__ list) {
+// Level bookkeeping after a multi-workgroup level: level += 1; after a
+// bottom-up level (bu = 1) the new level's bitmap is complete.
+__global__ void k_bfs_advance(const SlotView* __restrict__ slots, const int* __restrict__ list, int bu) {
     const GSlot S = load_slot(slots, list);
     if (threadIdx.x == 0) {
+        if (bu && S.fbits) S.ctr->fb_level = S.ctr->level + 1;
         S.ctr->level = S.ctr->level + 1;
         S.ctr->nhl = 0;
         S.ctr->nhg = 0;
@@ -1095,6 +1140,34 @@ __global__ void k_bfs_advance(const SlotView* __restrict__ slots, const int* __r
 static int per_slot_grid(int want, int nlist) {
     const int cap = 16384 / (nlist > 0 ? nlist : 1);
     return want < cap ? want : (cap > 16 ? cap : 16);
+}
+
+// Level L's bitmap from the levels (a wave per 64-vertex word), before the
+// first bottom-up level of a BFS; later bottom-up levels write their
+// successor's bitmap themselves.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bu_bits(const SlotView* __restrict__ slots,
+                                                 const int* __restrict__ list) {
+    const GSlot S = load_slot(slots, list);
+    if (!S.fbits) return;
+    const int L = S.ctr->level;
+    const GAS int32_t* lvl = bfs_lvl<MODE>(S);
+    GAS uint64_t* bits = fb_level(S, L);
+    const int n = S.g.n;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += stride) {
+        const int v = (int)base + lane_id();
+        const uint64_t m = __ballot(v < n && lvl[v] == L);
+        if (lane_id() == 0) bits[base >> 6] = m;
+    }
+    // read by the next kernel, once every workgroup of this one is done
+    if (blockIdx.x == 0 && threadIdx.x == 0) S.ctr->fb_level = L;
+}
+
+void launch_bu_bits(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx, int mode) {
+    if (mode == BFS_GR) hipLaunchKernelGGL(k_bu_bits<BFS_GR>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bu_bits<BFS_FWD>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
+    else hipLaunchKernelGGL(k_bu_bits<BFS_UND>, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
 }
 
 void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx,
@@ -1113,7 +1186,7 @@ void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int n
         hipLaunchKernelGGL(k_bfs_bu_med<BFS_UND>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
         hipLaunchKernelGGL(k_bfs_bu_heavy<BFS_UND>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
     }
-    hipLaunchKernelGGL(k_bfs_advance, dim3(1, nlist), dim3(64), 0, st, slots, list);
+    hipLaunchKernelGGL(k_bfs_advance, dim3(1, nlist), dim3(64), 0, st, slots, list, 1);
 }
 
 // Single-workgroup BFS: runs levels back to back with only workgroup
@@ -1335,9 +1408,11 @@ void launch_bfs_small(hipStream_t st, const SlotView* slots, const int* list, in
         else hipLaunchKernelGGL(k_bfs_small<BFS_UND>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels, max_edges);
         return;
     }
-    // at most cap_total workgroups per launch (at most 512): every multi-
-    // workgroup launch of every engine fits on the device at once
-    const long long cap = std::min(512, std::max(1, cap_total));
+    // at most cap_total workgroups per launch (at most 256): every multi-
+    // workgroup launch of every engine fits on the device at once, and many
+    // slots share the barriers' cost (grid 1024^2 with 128 slots: 2 per slot
+    // 57 q/s against 54 at 4)
+    const long long cap = std::min(256, std::max(1, cap_total));
     int g = wgs;
     while (g > 1 && (long long)g * nlist > cap) g >>= 1;
     if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_multi<BFS_GR>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, max_edges, spin_ticks);
@@ -2379,7 +2454,7 @@ void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int n
         hipLaunchKernelGGL(k_bfs_td_giant<BFS_UND>, dim3(gg, nlist), dim3(256), 0, st, slots, list);
         hipLaunchKernelGGL(k_bfs_td_heavy<BFS_UND>, dim3(gh, nlist), dim3(256), 0, st, slots, list);
     }
-    hipLaunchKernelGGL(k_bfs_advance, dim3(1, nlist), dim3(64), 0, st, slots, list);
+    hipLaunchKernelGGL(k_bfs_advance, dim3(1, nlist), dim3(64), 0, st, slots, list, 0);
 }
 
 void launch_giant(hipStream_t st, const SlotView* slots, const int* list, int nlist) {

@@ -67,7 +67,7 @@ def test_params_defaults_are_the_reference_constants():
     assert (p.local_relabel, p.local_band) == (0, 128)  # band limit of BC:1611; off by default
     # additions without a reference constant: fused cycles, graph replay,
     # speculative certificate, sparse counts
-    assert (p.fuse, p.graphs, p.spec_cert, p.sparse_count, p.small_wgs) == (8, 1, 1, 1, 8)
+    assert (p.fuse, p.graphs, p.spec_cert, p.sparse_count, p.small_wgs) == (8, 1, 1, 1, 4)
     assert p.small_edges == 1 << 16 and p.mark_list == 1 << 20
 
 

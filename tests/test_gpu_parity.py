@@ -44,14 +44,23 @@ def test_init_preflow_bit_exact(c1):
         assert np.array_equal(ex, oex)
 
 
-@pytest.mark.parametrize("direction", ["auto", "bottom_up"])
+DIRECTIONS = {
+    "auto": {},
+    # every level bottom-up (switch at once, never back): runs of bottom-up
+    # levels read the frontier bitmaps the levels before them wrote
+    "bottom_up": dict(bu_alpha=1e12, bu_beta=1e12),
+    # ... testing candidate parents by their levels instead of the bitmaps
+    "bottom_up_levels": dict(bu_alpha=1e12, bu_beta=1e12, bu_bits=0),
+}
+
+
+@pytest.mark.parametrize("direction", list(DIRECTIONS))
 @pytest.mark.parametrize("which", ["c1", "rmat12", "grid"])
 def test_global_relabel_heights_bit_exact(which, direction, request):
-    """Relabel heights equal the oracle's BFS distances.  "bottom_up" forces
-    every level bottom-up (switch at once, never back), so runs of bottom-up
-    levels read the unvisited lists of the levels before them."""
+    """Relabel heights equal the oracle's BFS distances, with the direction-
+    optimizing switch and with every level bottom-up."""
     case = request.getfixturevalue(which)
-    e, gid = engine_with(case, **({} if direction == "auto" else dict(bu_alpha=1e12, bu_beta=1e12)))
+    e, gid = engine_with(case, **DIRECTIONS[direction])
     rng = np.random.default_rng(77)
     if which == "rmat12":
         s_, t_ = case.edges.hub_queries(4, 1202)

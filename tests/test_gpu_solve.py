@@ -157,6 +157,9 @@ def test_edge_cases():
     dict(disc_band=1),                          # discharge the lowest two active levels only
     dict(disc_band=2, sink_cut=0, tile_frac=0.0),  # ... three, to the end, in tiles
     dict(bu_skip=1, small_edges=0),             # ... skip those that cannot reach t's SCC, every level grid-wide
+    dict(bu_bits=0),                            # bottom-up levels test parents by their levels, not a bitmap
+    dict(bu_bits=0, small_edges=0, bu_alpha=1.0),  # ... every level grid-wide, bottom-up early
+    dict(small_edges=0, bu_alpha=1.0, bu_beta=4.0),  # bitmaps over runs of bottom-up levels between top-down ones
 ])
 def test_scheduler_variants_stay_exact(c1, variant):
     log(f"variant {variant}")
