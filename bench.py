@@ -341,7 +341,25 @@ def grid_record(args):
         lat = [r.t_end - r.t_arrive for r in res]
         e_run = sum(r.E_run for r in res)
         roof = roofline_of(fams, "grid1024x1024_left_right")
-        rr = {"R": R, "slots": args.grid_slots, "queries": q, "qps": round(q / wall, 3),
+        # the plain solver (one slot, one query at a time) on the first queries
+        plain = {}
+        if args.grid_plain > 0:
+            pe = M.Engine(1, M.default_params(fuse=64))
+            pg = pe.add_graph(g.dg)
+            pq = min(args.grid_plain, len(g.s))
+            pe.solve(pg, g.s[0], g.t[0])  # warm: slab allocated, pruning cached
+            t1 = time.perf_counter()
+            pres = [pe.solve(pg, g.s[i], g.t[i]) for i in range(pq)]
+            pw = time.perf_counter() - t1
+            for i, r in enumerate(pres):
+                r.tag = i
+            pb, pc = mismatches(pres, g.flows)
+            bad_total, checked = bad_total + pb, checked + pc
+            b, c = b + pb, c + pc
+            plat = [r.t_end - r.t_start for r in pres]
+            plain = {"plain_queries": pq, "plain_qps": round(pq / pw, 3), "plain_p50_ms": round(pct(plat, .5) * 1e3, 3)}
+            del pe
+        rr = {"R": R, "slots": args.grid_slots, "queries": q, "qps": round(q / wall, 3), **plain,
               "p50_ms": round(pct(lat, .5) * 1e3, 3), "p99_ms": round(pct(lat, .99) * 1e3, 3),
               "residual_edges_per_s": round(e_run / wall, 1),
               "mean_bfs_levels": round(float(np.mean([r.bfs_levels for r in res])), 1),
@@ -532,12 +550,13 @@ def main():
                     help="open-loop Poisson loads on R-MAT 22, as fractions of the batched rate")
     ap.add_argument("--sweep-seconds", type=float, default=2.0)
     ap.add_argument("--grid-queries", type=int, default=256)
-    ap.add_argument("--grid-slots", type=int, default=128)
+    ap.add_argument("--grid-slots", type=int, default=256)
+    ap.add_argument("--grid-plain", type=int, default=16, help="grid queries timed on the plain (one-at-a-time) solver")
     ap.add_argument("--grid-R", type=int, nargs="+", default=[128, 512], help="region counts of the c2 sub-record")
     ap.add_argument("--serve-qps", type=float, nargs="+", default=[50.0, 200.0, 800.0])
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-max", type=int, default=1200)
-    ap.add_argument("--serve-grid-slots", type=int, default=128, help="slots of the grid lane")
+    ap.add_argument("--serve-grid-slots", type=int, default=256, help="slots of the grid lane")
     ap.add_argument("--serve-grid-wgs", type=int, default=4,
                     help="workgroups per slot of the grid lane's deep relabel BFS")
     ap.add_argument("--serve-slabs", type=float, default=24.0, help="slab arena budget in full R-MAT 22 slabs")

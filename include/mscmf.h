@@ -240,6 +240,7 @@ typedef struct mf_params {
     int slot_blocks, launch_blocks; /* grid-stride launches: workgroups per slot / per launch at most */
     int bu_light;        /* bottom-up rows up to this length are scanned by one lane, longer ones by a wave */
     int bu_bits;         /* 1: bottom-up levels test candidate parents in a frontier bitmap, 0: in their levels */
+    double gr_work_deep; /* gr_work once the last exact relabel was 64+ levels deep */
 } mf_params;
 
 void mf_params_default(mf_params* p);

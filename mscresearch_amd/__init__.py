@@ -52,7 +52,7 @@ class Params(C.Structure):
         ("tile_split", C.c_int), ("scc_route", C.c_int), ("bu_skip", C.c_int), ("sink_cut", C.c_int),
         ("disc_band", C.c_int), ("bu_pick", C.c_int), ("small_levels", C.c_int),
         ("push_first", C.c_int), ("slot_blocks", C.c_int), ("launch_blocks", C.c_int),
-        ("bu_light", C.c_int), ("bu_bits", C.c_int),
+        ("bu_light", C.c_int), ("bu_bits", C.c_int), ("gr_work_deep", C.c_double),
     ]
 
 

@@ -123,6 +123,7 @@ Params from_c(const mf_params* p) {
     P.launch_blocks = p->launch_blocks;
     P.bu_light = p->bu_light;
     P.bu_bits = p->bu_bits;
+    P.gr_work_deep = p->gr_work_deep;
     return P;
 }
 
@@ -529,6 +530,7 @@ void mf_params_default(mf_params* p) {
     p->launch_blocks = P.launch_blocks;
     p->bu_light = P.bu_light;
     p->bu_bits = P.bu_bits;
+    p->gr_work_deep = P.gr_work_deep;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

@@ -656,9 +656,10 @@ Plan SlotSched::plan(const Ctr& c) {
         // iterations as the last GR had levels: excess moves about one hop
         // per iteration, so on deep graphs (grids) it is still in transit.
         const uint64_t scans = c.wk[WF_DISCHARGE][0] - scans_at_gr_;
+        const double gr_work = gr_depth_ >= 64 ? P_->gr_work_deep : P_->gr_work;
         const bool gr_due = force_gr_ || (iter_ - last_gr_iter_ >= std::max<int64_t>(GR_FREQ_, gr_depth_ / 4)) ||
                             (P_->gr_stall > 0 && stall_ >= P_->gr_stall && 4 * (iter_ - last_gr_iter_) >= gr_depth_) ||
-                            (P_->gr_work > 0 && (double)scans >= P_->gr_work * (double)g_.E);
+                            (gr_work > 0 && (double)scans >= gr_work * (double)g_.E);
         if (gr_due && !exact) return start_gr(&c);
         // local relabel (BC:1582-1721): BFS from t and the deficits through
         // the tiles holding active vertices only, for at most band + 2 levels,

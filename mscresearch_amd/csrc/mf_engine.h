@@ -119,8 +119,13 @@ struct Params {
     // also GR once the discharge has scanned gr_work x E entries since the
     // last GR (a GR costs about E scans; 0 = off).  On R-MAT 22 hub queries
     // 0.125 beat 0.0625, 0.25, 0.5, 1 and 2 (hub rows are rescanned between
-    // relabels); grids never reach it
+    // relabels)
     double gr_work = 0.125;
+    // ... on deep graphs (the last exact relabel at least 64 levels deep),
+    // where path pushes scan whole paths: grid 1024^2, 512 queries in 256
+    // slots 170 q/s at 0.5 against 169 at 0.25, 146 at 1 and 149 at 0.125
+    // (more relabels, each about 1,700 levels)
+    double gr_work_deep = 0.5;
     // discharge cycles per pass for rows scanned by a whole wave (light rows
     // use K): a second cycle pushes what the first relabel made admissible
     // without another count step (2 beat 1, 3 and 4 on R-MAT 22 by 1.6%)
@@ -159,8 +164,11 @@ struct Params {
     // path pushes on deep graphs (the last exact relabel at least 64 levels
     // deep): up to path_switch_deep active vertices, after p_count_deep
     // iterations with that few (grids: 252 -> 178 ms per plain query, 4.2
-    // global relabels instead of 7.3)
-    int path_switch_deep = 64;
+    // global relabels instead of 7.3 at 64; grid 1024^2 with 64 slots 72 q/s
+    // at 256 against 45 at 64: with about 100 active vertices the list
+    // discharge moved excess one hop per iteration, 1,400 iterations a query
+    // instead of 26)
+    int path_switch_deep = 256;
     int p_count_deep = 1;
     // tile discharge (region mode and the hot sweep): up to this many
     // workgroups per tile, dealt by the tile's excess score as the

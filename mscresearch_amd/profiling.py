@@ -27,7 +27,7 @@ FAMILY_OF = {
     "k_gr_seed": "bfs", "k_bfs_small": "bfs", "k_bfs_multi": "bfs", "k_bfs_td": "bfs", "k_bfs_td_heavy": "bfs", "k_bfs_td_giant": "bfs",
     "k_bfs_advance": "bfs", "k_bfs_seed1": "bfs", "k_local_mark": "bfs", "k_local_bound": "bfs",
     "k_local_bucket": "bfs", "k_local_bfs": "bfs", "k_local_finish": "bfs", "k_mask_combine": "bfs",
-    "k_bfs_bu": "bfs_bu", "k_bfs_bu_med": "bfs_bu", "k_bfs_bu_heavy": "bfs_bu",
+    "k_bfs_bu": "bfs_bu", "k_bfs_bu_med": "bfs_bu", "k_bfs_bu_heavy": "bfs_bu", "k_bu_bits": "bfs_bu",
     "k_active": "active", "k_active_sparse": "active", "k_list_copy": "active",
     "k_discharge_list": "discharge", "k_discharge_tile": "discharge", "k_giant_prep": "discharge",
     "k_giant_push": "discharge", "k_giant_relabel": "discharge", "k_path_push": "discharge",
