@@ -156,7 +156,7 @@ def headline(args, out):
     if not args.no_tune:
         from mscresearch_amd.tune import cached
 
-        tv = cached(M.HostGraph.from_edgelist(e3), args.batch // args.engines)
+        tv = cached(M.HostGraph.from_edgelist(e3), args.tune_batch)
     R, tile, slots = args.R, 4096, args.batch // args.engines
     prm = M.default_params()
     if tv is not None:
@@ -541,9 +541,12 @@ def main():
     ap.add_argument("--steps", type=int, default=4, help="timed passes over the 1,024 R-MAT 22 queries")
     ap.add_argument("--warmup", type=int, default=1, help="untimed passes before them")
     ap.add_argument("--batch", type=int, default=192, help="engine slots over all engines")
-    ap.add_argument("--engines", type=int, default=2,
-                    help="engines sharing the batch slots, one host thread and HIP stream each")
+    ap.add_argument("--engines", type=int, default=4,
+                    help="engines sharing the batch slots, one host thread and HIP stream each "
+                         "(192 slots on R-MAT 22: 1, 2, 3, 4, 6, 8 engines gave 1712, 2089, 2163, 2206, 2010, 1850 q/s)")
     ap.add_argument("--R", type=int, default=512, help="regions when no tuned value is recorded")
+    ap.add_argument("--tune-batch", type=int, default=96,
+                    help="the autotuner's batched entry to use (its engine size; the tuned slots are shared by --engines)")
     ap.add_argument("--no-tune", action="store_true", help="ignore the autotuner's recorded parameters")
     ap.add_argument("--plain", type=int, default=256, help="queries timed on the plain (one-at-a-time) solver")
     ap.add_argument("--sweep", type=float, nargs="*", default=[0.1, 0.25, 0.5, 0.75],

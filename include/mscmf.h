@@ -241,6 +241,7 @@ typedef struct mf_params {
     int bu_light;        /* bottom-up rows up to this length are scanned by one lane, longer ones by a wave */
     int bu_bits;         /* 1: bottom-up levels test candidate parents in a frontier bitmap, 0: in their levels */
     double gr_work_deep; /* gr_work once the last exact relabel was 64+ levels deep */
+    long long small_edges_deep; /* small_edges on deep graphs (relabel, depth hint or BFS 64+ levels deep) */
 } mf_params;
 
 void mf_params_default(mf_params* p);

@@ -124,6 +124,7 @@ Params from_c(const mf_params* p) {
     P.bu_light = p->bu_light;
     P.bu_bits = p->bu_bits;
     P.gr_work_deep = p->gr_work_deep;
+    P.small_edges_deep = p->small_edges_deep;
     return P;
 }
 
@@ -531,6 +532,7 @@ void mf_params_default(mf_params* p) {
     p->bu_light = P.bu_light;
     p->bu_bits = P.bu_bits;
     p->gr_work_deep = P.gr_work_deep;
+    p->small_edges_deep = P.small_edges_deep;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

@@ -170,6 +170,9 @@ struct SlotView {
     // bottom-up rows up to this length are scanned by one lane, longer ones
     // (to kGiantDeg) by a wave
     int32_t bu_light = 128;
+    // multi-level single- or multi-workgroup BFS launches stop before a
+    // level whose frontier rows sum to more entries than this
+    int64_t bfs_edges = 1 << 16;
     int32_t phase = 0;           // 0 = normal (seeds t + deficits), 1 = drain (seeds deficits, t active)
     int32_t seed_t_only = 0;     // phase 0 with no deficit (last count): the relabel seeds t alone
     // 1 = bounded injection: the preflow puts B = min(out-capacity of s,

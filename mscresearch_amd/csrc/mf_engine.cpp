@@ -394,6 +394,7 @@ Plan SlotSched::start_gr(const Ctr* c) {
     p.reset_pre = RST_BFS;
     p.bfs_kind = BK_SMALL;
     p.bfs_mode = BFS_GR;
+    p.small_edges = deep_bfs(0) ? P_->small_edges_deep : P_->small_edges;
     bfs_mode_ = BFS_GR;
     bfs_medges_ = 0;
     bfs_bottom_up_ = false;
@@ -425,9 +426,10 @@ Plan SlotSched::bfs_step(const Ctr& c, int mode) {
     if (bfs_bottom_up_) {
         p.bfs_kind = BK_BU;
         p.bfs_grid = clampi(cdiv(g_.n, 256), 1, 8192);
-    } else if (f <= kSmallFrontier && mf <= P_->small_edges) {
+    } else if (f <= kSmallFrontier && mf <= (deep_bfs(L) ? P_->small_edges_deep : P_->small_edges)) {
         p.bfs_kind = BK_SMALL;
-        if (!no_multi_ && (gr_depth_ >= 64 || depth_hint_ >= 64 || L >= 64)) p.small_wgs = std::max(1, P_->small_wgs);
+        p.small_edges = deep_bfs(L) ? P_->small_edges_deep : P_->small_edges;
+        if (!no_multi_ && deep_bfs(L)) p.small_wgs = std::max(1, P_->small_wgs);
     } else {
         p.bfs_kind = BK_TD;
         p.bfs_grid = std::max(1, cdiv(f, 256));
@@ -1319,6 +1321,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         hv[i] = slots_[i].view;
         if (!P_.bu_pick) hv[i].g.bu_pick = nullptr;
         hv[i].marks_ok = slots_[i].slab.marks_ok ? 1 : 0;
+        hv[i].bfs_edges = plans[i].small_edges > 0 ? plans[i].small_edges : P_.small_edges;
     }
     size_t lpos = 0;
     struct Group { int off = 0, cnt = 0; };
@@ -1525,7 +1528,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         for (int m = 0; m < 3; ++m)
             for (int w = 0; w < 2; ++w)
                 if (s_bfs[m][w].cnt)
-                    launch_bfs_small(side_, d_views_, L(s_bfs[m][w]), s_bfs[m][w].cnt, m, P_.small_levels, P_.small_edges, w ? P_.small_wgs : 1, spin_ticks_, multi_wg_cap());
+                    launch_bfs_small(side_, d_views_, L(s_bfs[m][w]), s_bfs[m][w].cnt, m, P_.small_levels, w ? P_.small_wgs : 1, spin_ticks_, multi_wg_cap());
         tes(k, side_);
         HIPCHK(hipEventRecord(ev_side_, side_));
     }
@@ -1564,12 +1567,12 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     for (int m = 0; m < 3; ++m) {
         if (g_bfs[BK_SMALL][m].cnt) {
             const int k = tb(WF_BFS);
-            launch_bfs_small(st, d_views_, L(g_bfs[BK_SMALL][m]), g_bfs[BK_SMALL][m].cnt, m, P_.small_levels, P_.small_edges, 1, spin_ticks_, multi_wg_cap());
+            launch_bfs_small(st, d_views_, L(g_bfs[BK_SMALL][m]), g_bfs[BK_SMALL][m].cnt, m, P_.small_levels, 1, spin_ticks_, multi_wg_cap());
             te(k);
         }
         if (g_bfs_multi[m].cnt) {
             const int k = tb(WF_BFS);
-            launch_bfs_small(st, d_views_, L(g_bfs_multi[m]), g_bfs_multi[m].cnt, m, P_.small_levels, P_.small_edges, P_.small_wgs, spin_ticks_, multi_wg_cap());
+            launch_bfs_small(st, d_views_, L(g_bfs_multi[m]), g_bfs_multi[m].cnt, m, P_.small_levels, P_.small_wgs, spin_ticks_, multi_wg_cap());
             te(k);
         }
         if (g_bfs[BK_TD][m].cnt) {

@@ -66,10 +66,15 @@ struct Params {
     int small_wgs = 4;
     // a frontier whose rows sum to more entries than this leaves the
     // single-workgroup BFS for the grid-wide top-down or bottom-up levels:
-    // a hub's row is spread over the device instead of one CU (+4% on R-MAT
-    // 22 hub queries against 2^18; grid frontiers of up to kSmallFrontier
-    // degree-4 vertices stay inside)
-    long long small_edges = 1 << 16;
+    // a hub's row is spread over the device instead of one CU.  R-MAT 22
+    // hub queries, 4 engines x 48 slots: 2^13 2304 q/s against 2180-2200 at
+    // 2^16 and 2004 at 2^18 (plain 473 against 463)
+    long long small_edges = 1 << 13;
+    // ... on deep graphs (the last exact relabel, the graph's depth hint or
+    // the running BFS at least 64 levels deep), where each grid-wide level
+    // would be an engine step of its own: grid frontiers of up to
+    // kSmallFrontier degree-8 rows stay inside
+    long long small_edges_deep = 1 << 17;
     // entries a slab lists as they are first marked (at most kMarkListMax);
     // resets and the sparse certificate read the list instead of the E-byte
     // mark map while it holds every marked entry (0 = always the map)
@@ -349,6 +354,7 @@ struct Plan {
     int small_wgs = 1;        // BK_SMALL: workgroups per slot (1 or Params::small_wgs)
     bool seed_t_only = false; // ST_GR_SEED in phase 0 after a count that saw no deficit: t is the one seed
     bool bu_bits = false;     // BK_BU: the first bottom-up level of this BFS builds level L's bitmap first
+    long long small_edges = 0; // BK_SMALL: frontier-entry bound of the multi-level launch (0 = Params::small_edges)
 };
 
 // Per-query statistics (host side).
@@ -386,6 +392,9 @@ public:
     // multi-workgroup small-frontier BFS from a query's first relabel on
     void set_depth_hint(int d) { depth_hint_ = d; }
     int gr_depth() const { return gr_depth_; }
+    // a BFS at level L on a deep graph: multi-workgroup levels and the deep
+    // frontier-entry bound
+    bool deep_bfs(int L) const { return gr_depth_ >= 64 || depth_hint_ >= 64 || L >= 64; }
     // height cap of the next discharge step (INT_MAX: none)
     int disc_cap() const { return disc_cap_; }
     int status() const { return status_; }

@@ -68,11 +68,11 @@ void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int n
 // (wgs == 1), or wgs workgroups per slot meeting at a barrier in global
 // memory after each level (spin_ticks: wall-clock ticks a barrier may wait
 // before it gives up and flags the slot's error word).  It stops above
-// kSmallFrontier vertices or max_edges frontier row entries.
+// kSmallFrontier vertices or SlotView::bfs_edges frontier row entries.
 // cap_total bounds the workgroups of one multi-workgroup launch (all of them
 // must be resident at once).
 void launch_bfs_small(hipStream_t st, const SlotView* slots, const int* list, int nlist, int mode,
-                      int max_levels, long long max_edges, int wgs, long long spin_ticks, int cap_total);
+                      int max_levels, int wgs, long long spin_ticks, int cap_total);
 
 // Active list (light entries at fa[0..), heavy at fa[n-1..]) and counters.
 void launch_active(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);

@@ -158,6 +158,7 @@ struct GSlot {
     const GAS uint8_t* live;
     GAS Ctr* ctr;
     int32_t s, t, K, Kh, H_cut, phase, seed_t_only, inj, disc_cap, push_first, marks_ok, bu_light;
+    int64_t bfs_edges;
     int32_t tstamp;  // this cycle's touch stamp (negative)
 };
 
@@ -217,6 +218,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.push_first = v.push_first;
     S.marks_ok = v.marks_ok;
     S.bu_light = v.bu_light;
+    S.bfs_edges = v.bfs_edges;
     S.phase = v.phase;
     S.seed_t_only = v.seed_t_only;
     S.inj = v.inject;
@@ -1195,7 +1197,7 @@ void launch_bfs_bu(hipStream_t st, const SlotView* slots, const int* list, int n
 // launch per level would cost more than the level itself.
 template <int MODE>
 __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__ slots,
-                                                    const int* __restrict__ list, int max_levels, long long max_edges) {
+                                                    const int* __restrict__ list, int max_levels) {
     const GSlot S = load_slot(slots, list);
     __shared__ int heavy[1024];
     __shared__ int nheavy;
@@ -1209,7 +1211,7 @@ __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__
     int done_levels = 0;
     int visited = 0;
     int64_t fdeg = S.ctr->fedges[L & 3];
-    while (fsz > 0 && fsz <= kSmallFrontier && fdeg <= max_edges && done_levels < max_levels) {
+    while (fsz > 0 && fsz <= kSmallFrontier && fdeg <= S.bfs_edges && done_levels < max_levels) {
         const GAS int32_t* in = (L & 1) ? S.fb : S.fa;
         GAS int32_t* out = (L & 1) ? S.fa : S.fb;
         if (threadIdx.x == 0) { nheavy = 0; next_cnt = 0; next_deg = 0; }
@@ -1319,7 +1321,7 @@ __device__ __forceinline__ bool slot_barrier(GAS Ctr* c, uint32_t& gen, long lon
 template <int MODE>
 __global__ __launch_bounds__(256) void k_bfs_multi(const SlotView* __restrict__ slots,
                                                    const int* __restrict__ list, int max_levels,
-                                                   long long max_edges, long long spin_ticks) {
+                                                   long long spin_ticks) {
     const GSlot S = load_slot(slots, list);
     __shared__ Stage<1024> st;
     __shared__ int heavy[256];
@@ -1338,7 +1340,7 @@ __global__ __launch_bounds__(256) void k_bfs_multi(const SlotView* __restrict__ 
     Work wk;
     int done_levels = 0;
     int visited = 0;
-    while (fsz > 0 && fsz <= kSmallFrontier && fdeg <= max_edges && done_levels < max_levels) {
+    while (fsz > 0 && fsz <= kSmallFrontier && fdeg <= S.bfs_edges && done_levels < max_levels) {
         const GAS int32_t* in = (L & 1) ? S.fb : S.fa;
         GAS int32_t* out = (L & 1) ? S.fa : S.fb;
         GAS int32_t* ocnt = &S.ctr->fsize[(L + 1) & 3];
@@ -1401,11 +1403,11 @@ __global__ __launch_bounds__(256) void k_bfs_multi(const SlotView* __restrict__ 
 }
 
 void launch_bfs_small(hipStream_t st, const SlotView* slots, const int* list, int nlist, int mode,
-                      int max_levels, long long max_edges, int wgs, long long spin_ticks, int cap_total) {
+                      int max_levels, int wgs, long long spin_ticks, int cap_total) {
     if (wgs <= 1) {
-        if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_small<BFS_GR>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels, max_edges);
-        else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_small<BFS_FWD>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels, max_edges);
-        else hipLaunchKernelGGL(k_bfs_small<BFS_UND>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels, max_edges);
+        if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_small<BFS_GR>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
+        else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_small<BFS_FWD>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
+        else hipLaunchKernelGGL(k_bfs_small<BFS_UND>, dim3(1, nlist), dim3(1024), 0, st, slots, list, max_levels);
         return;
     }
     // at most cap_total workgroups per launch (at most 256): every multi-
@@ -1415,9 +1417,9 @@ void launch_bfs_small(hipStream_t st, const SlotView* slots, const int* list, in
     const long long cap = std::min(256, std::max(1, cap_total));
     int g = wgs;
     while (g > 1 && (long long)g * nlist > cap) g >>= 1;
-    if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_multi<BFS_GR>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, max_edges, spin_ticks);
-    else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_multi<BFS_FWD>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, max_edges, spin_ticks);
-    else hipLaunchKernelGGL(k_bfs_multi<BFS_UND>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, max_edges, spin_ticks);
+    if (mode == BFS_GR) hipLaunchKernelGGL(k_bfs_multi<BFS_GR>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, spin_ticks);
+    else if (mode == BFS_FWD) hipLaunchKernelGGL(k_bfs_multi<BFS_FWD>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, spin_ticks);
+    else hipLaunchKernelGGL(k_bfs_multi<BFS_UND>, dim3(g, nlist), dim3(256), 0, st, slots, list, max_levels, spin_ticks);
 }
 
 // ------------------------------------------------------- local relabel (a14)

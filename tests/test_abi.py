@@ -68,7 +68,7 @@ def test_params_defaults_are_the_reference_constants():
     # additions without a reference constant: fused cycles, graph replay,
     # speculative certificate, sparse counts
     assert (p.fuse, p.graphs, p.spec_cert, p.sparse_count, p.small_wgs) == (8, 1, 1, 1, 4)
-    assert p.small_edges == 1 << 16 and p.mark_list == 1 << 20
+    assert p.small_edges == 1 << 13 and p.small_edges_deep == 1 << 17 and p.mark_list == 1 << 20
 
 
 def test_cli_usage_and_errors(tmp_path):
