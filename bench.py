@@ -391,7 +391,7 @@ def serve_record(args, c3):
     r20 = Workload("r20", 512)
     ws = [gw, r20, c3]
     lanes = [("grid1024", args.serve_grid_slots, M.default_params(fuse=64, small_wgs=args.serve_grid_wgs)),
-             ("rmat20", 32, M.default_params()), ("rmat22", 48, M.default_params())]
+             ("rmat20", 96, M.default_params()), ("rmat22", 48, M.default_params())]
     engines, budgets = [], []
     for (nm, slots, prm), w in zip(lanes, ws):
         eng = M.Engine(slots, prm)
@@ -562,15 +562,15 @@ def main():
     ap.add_argument("--sweep-seconds", type=float, default=2.0)
     ap.add_argument("--grid-queries", type=int, default=256)
     ap.add_argument("--grid-slots", type=int, default=256)
-    ap.add_argument("--grid-plain", type=int, default=16, help="grid queries timed on the plain (one-at-a-time) solver")
+    ap.add_argument("--grid-plain", type=int, default=48, help="grid queries timed on the plain (one-at-a-time) solver")
     ap.add_argument("--grid-R", type=int, nargs="+", default=[128, 512], help="region counts of the c2 sub-record")
     ap.add_argument("--serve-qps", type=float, nargs="+", default=[50.0, 200.0, 800.0])
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-max", type=int, default=1200)
-    ap.add_argument("--serve-grid-slots", type=int, default=256, help="slots of the grid lane")
+    ap.add_argument("--serve-grid-slots", type=int, default=512, help="slots of the grid lane")
     ap.add_argument("--serve-grid-wgs", type=int, default=4,
                     help="workgroups per slot of the grid lane's deep relabel BFS")
-    ap.add_argument("--serve-slabs", type=float, default=24.0, help="slab arena budget in full R-MAT 22 slabs")
+    ap.add_argument("--serve-slabs", type=float, default=48.0, help="slab arena budget in full R-MAT 22 slabs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
