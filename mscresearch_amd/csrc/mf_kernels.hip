@@ -226,9 +226,23 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     return S;
 }
 
+// A place in a global list for each active lane of the wave: one atomic on
+// the list's count per wave instead of one per lane (the count is one hot
+// word).  Every lane that reaches the call takes part.
+__device__ __forceinline__ int wave_slot(GAS int32_t* cnt) {
+    const uint64_t m = __ballot(true);
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = add32(cnt, __popcll(m));
+    base = __shfl(base, leader, WAVE);
+    return base + __popcll(m & ((1ULL << lane_id()) - 1));
+}
+
 // LDS-staged append buffer: entries are collected with LDS atomics and
-// flushed with one global atomic per workgroup.  Overflow goes straight to
-// global memory.
+// flushed with one global atomic per workgroup.  Past CAP entries go
+// straight to global memory, their places taken with one global atomic per
+// wave (the count is one hot word per list); the row-length sum of every
+// entry stays in LDS until the flush.
 template <int CAP>
 struct Stage {
     int32_t buf[CAP];
@@ -246,13 +260,11 @@ template <int CAP>
 __device__ __forceinline__ void stage_push(Stage<CAP>& st, int32_t item, int deg, GAS int32_t* gbuf,
                                            GAS int32_t* gcnt, GAS int64_t* gdeg) {
     const int pos = atomicAdd(&st.cnt, 1);
+    add64_lds(&st.deg_sum, (int64_t)deg);
     if (pos < CAP) {
         st.buf[pos] = item;
-        add64_lds(&st.deg_sum, (int64_t)deg);
     } else {
-        const int g = add32(gcnt, 1);
-        gbuf[g] = item;
-        if (gdeg) add64(gdeg, (int64_t)deg);
+        gbuf[wave_slot(gcnt)] = item;  // the lanes of this call that overflow
     }
 }
 
@@ -879,8 +891,8 @@ __global__ __launch_bounds__(256) void k_bfs_td(const SlotView* __restrict__ slo
         // a wave each), giant rows to the giant list (k_bfs_td_giant, in
         // kGiantChunk-entry tasks)
         if (deg <= kLightDeg) td_lane<MODE>(S, v, L, lvl, st, out, ocnt, odeg, wk);
-        else if (deg <= kGiantDeg) S.hl[add32(&S.ctr->nhl, 1)] = v;
-        else S.giant[add32(&S.ctr->ngiant, 1)] = v;
+        else if (deg <= kGiantDeg) S.hl[wave_slot(&S.ctr->nhl)] = v;
+        else S.giant[wave_slot(&S.ctr->ngiant)] = v;
     }
     stage_flush(st, out, ocnt, odeg);
     const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
@@ -980,9 +992,9 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
             }
             if (picked) {
             } else if (re - rb > kGiantDeg) {
-                S.giant[add32(&S.ctr->ngiant, 1)] = v;
+                S.giant[wave_slot(&S.ctr->ngiant)] = v;
             } else if (re - rb > S.bu_light) {
-                S.hl[add32(&S.ctr->nhl, 1)] = v;
+                S.hl[wave_slot(&S.ctr->nhl)] = v;
             } else {
                 // the parent test only for entries that pass the residual
                 // test (most reverse entries carry no flow)
@@ -1878,7 +1890,7 @@ __device__ __forceinline__ int nbr_h(const GSlot& S, const TileCtx& tc, int w) {
 // w received excess this cycle: listed once per touch epoch for a sparse count.
 __device__ __forceinline__ void touch(const GSlot& S, int w) {
     if (__hip_atomic_exchange(&S.aux[w], S.tstamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != S.tstamp)
-        S.hl[add32(&S.ctr->ntouched, 1)] = w;
+        S.hl[wave_slot(&S.ctr->ntouched)] = w;
 }
 
 __device__ __forceinline__ void push_into(const GSlot& S, const TileCtx& tc, int w, int64_t d) {
