@@ -2264,10 +2264,11 @@ __device__ __forceinline__ GTask giant_task(const GSlot& S, int task) {
     return GTask{r.x, r.y, r.z, r.w};
 }
 
-// k_giant_push: one pass over each chunk of a giant row.  The chunk's
-// entries go to registers; the chunk claims min(excess, residual into lower
+// k_giant_push: one pass over each chunk of a giant row, in rounds of one
+// entry per thread.  A round claims min(excess, residual into its lower
 // entries) from the row's vertex with a compare-and-swap that never
-// overdraws and pushes it to the lower entries in prefix-sum order.  The
+// overdraws and pushes it to those entries in prefix-sum order; the chunk
+// stops at the round that finds less excess than it could take.  The
 // lowest height over the entries that keep residual goes to the row's
 // minimum slot, and gacc counts the chunks that reported; k_giant_relabel
 // raises a vertex that still has excess to that minimum + 1 (n when it is
@@ -2322,64 +2323,65 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
         __syncthreads();
         // dry, cut off or above this step's band: no report, no relabel
         if (ex0 <= 0 || hv >= n || hv > S.disc_cap) continue;
-        int64_t r[PER];
-        int hw[PER], cw[PER];
-#pragma unroll
+        // the chunk in rounds of blockDim.x entries, each claiming from the
+        // row what its lower entries can take: once a round claims less than
+        // that, the row is dry and the rest of the chunk is neither read nor
+        // gathered (a hub's excess usually fits into its first lower
+        // entries); a chunk that read every round reports its minimum
+        int m = INT_MAX - 1;
+        bool dry = false;
+        int rounds = 0;
         for (int j = 0; j < PER; ++j) {
             const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
-            r[j] = 0;
-            cw[j] = 0;
+            int64_t r = 0;
+            int cw = 0, hw = INT_MAX;
             if (e < e1) {
-                r[j] = ld64(&S.res[e]);
-                cw[j] = S.g.col[e];
+                r = ld64(&S.res[e]);
+                cw = S.g.col[e];
             }
-        }
-        int64_t tsum = 0;
-#pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            hw[j] = INT_MAX;
-            if (r[j] > 0) {
-                hw[j] = S.h[cw[j]];
-                if (hw[j] < hv) tsum += r[j];
+            int64_t a = 0;
+            if (r > 0) {
+                hw = S.h[cw];
+                if (hw < hv) a = r;
             } else {
-                r[j] = 0;
+                r = 0;
             }
-        }
-        int64_t tot = 0;
-        const int64_t incl = block_incl_scan_i64(tsum, lscr, &tot);
-        if (threadIdx.x == 0) wk.edges += (uint64_t)(e1 - e0);
-        int64_t got = 0;
-        if (tot > 0) {  // block-uniform
-            if (threadIdx.x == 0) {
-                int64_t cur = ld64(&S.excess[v]);
-                int64_t a = 0;
-                while (cur > 0) {
-                    a = cur < tot ? cur : tot;
-                    if (cas64(&S.excess[v], cur, cur - a)) break;
+            int64_t tot = 0;
+            const int64_t incl = block_incl_scan_i64(a, lscr, &tot);
+            ++rounds;
+            int64_t got = 0;
+            if (tot > 0) {  // block-uniform
+                if (threadIdx.x == 0) {
+                    int64_t cur = ld64(&S.excess[v]);
+                    int64_t take = 0;
+                    while (cur > 0) {
+                        take = cur < tot ? cur : tot;
+                        if (cas64(&S.excess[v], cur, cur - take)) break;
+                    }
+                    bcast = cur > 0 ? take : 0;
                 }
-                bcast = cur > 0 ? a : 0;
+                __syncthreads();
+                got = bcast;
+                __syncthreads();
             }
-            __syncthreads();
-            got = bcast;
-            __syncthreads();
-        }
-        int64_t rem = got - (incl - tsum);
-        int m = INT_MAX - 1;
-#pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            if (r[j] <= 0) continue;
-            if (hw[j] < hv && rem > 0) {
-                const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
-                const int64_t d = rem < r[j] ? rem : r[j];
+            const int64_t rem = got - (incl - a);
+            if (a > 0 && rem > 0) {
+                const int64_t d = rem < a ? rem : a;
                 move_res(S, e, d);
-                add64(&S.excess[cw[j]], d);
-                touch(S, cw[j]);
-                rem -= d;
-                r[j] -= d;
+                add64(&S.excess[cw], d);
+                touch(S, cw);
+                r -= d;
                 wk.pushes++;
             }
-            if (r[j] > 0) m = min(m, hw[j]);
+            if (r > 0) m = min(m, hw);
+            if (got < tot) {  // block-uniform: the row's excess ran out in this round
+                dry = true;
+                break;
+            }
+            if (e0 + (j + 1) * (int)blockDim.x >= e1) break;
         }
+        if (threadIdx.x == 0) wk.edges += (uint64_t)min(e1 - e0, rounds * (int)blockDim.x);
+        if (dry) continue;  // block-uniform
         m = block_min_i32(m, iscr);
         if (threadIdx.x == 0) {
             min32(&S.giant[S.gcap + k], m);
