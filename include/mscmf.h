@@ -242,6 +242,8 @@ typedef struct mf_params {
     int bu_bits;         /* 1: bottom-up levels test candidate parents in a frontier bitmap, 0: in their levels */
     double gr_work_deep; /* gr_work once the last exact relabel was 64+ levels deep */
     long long small_edges_deep; /* small_edges on deep graphs (relabel, depth hint or BFS 64+ levels deep) */
+    int giant_rounds;    /* giant-row discharge chunks in rounds that stop once the row is dry: 1 on, 0 off,
+                            2 = on in engines of more than one slot */
 } mf_params;
 
 void mf_params_default(mf_params* p);

@@ -125,6 +125,7 @@ Params from_c(const mf_params* p) {
     P.bu_bits = p->bu_bits;
     P.gr_work_deep = p->gr_work_deep;
     P.small_edges_deep = p->small_edges_deep;
+    P.giant_rounds = p->giant_rounds;
     return P;
 }
 
@@ -533,6 +534,7 @@ void mf_params_default(mf_params* p) {
     p->bu_bits = P.bu_bits;
     p->gr_work_deep = P.gr_work_deep;
     p->small_edges_deep = P.small_edges_deep;
+    p->giant_rounds = P.giant_rounds;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

@@ -1508,9 +1508,12 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
         return std::max(1, std::min(gx, cap));
     };
     auto gridE = [&](int i) { return clampi(cdiv(slots_[i].view.g.E, 1024), 1, 8192); };
+    // giant rows in the group: 0 = none, 1 = discharged in whole chunks,
+    // 2 = in rounds (Params::giant_rounds)
+    const int giant_mode = (P_.giant_rounds == 2 ? nslots_ > 1 : P_.giant_rounds != 0) ? 2 : 1;
     auto giants = [&](const Group& g) {
         for (int k = 0; k < g.cnt; ++k)
-            if (slots_[hl[g.off + k]].view.g.max_deg > kGiantDeg) return 1;
+            if (slots_[hl[g.off + k]].view.g.max_deg > kGiantDeg) return giant_mode;
         return 0;
     };
     // side stream: starts after the view / list upload on the main stream
@@ -1642,7 +1645,7 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
                     if (kind < 2 && plans[i].dis_kind != kind) continue;
                     lst[cnt[c][kind]++] = i;
                     tiles = std::max(tiles, slots_[i].view.g.ntiles);
-                    gi |= slots_[i].view.g.max_deg > kGiantDeg ? 1 : 0;
+                    if (slots_[i].view.g.max_deg > kGiantDeg) gi = giant_mode;
                 }
                 key.push_back(cnt[c][kind]);
                 key.insert(key.end(), lst, lst + cnt[c][kind]);

@@ -80,7 +80,8 @@ void launch_active(hipStream_t st, const SlotView* slots, const int* list, int n
 // RST_TOUCH only (after a RST_SPARSE reset); gridx any size.
 void launch_active_sparse(hipStream_t st, const SlotView* slots, const int* list, int nlist, int gridx);
 // Discharge of the active list: lanes for light rows, waves for heavy rows;
-// giants != 0: rows longer than kGiantDeg over the whole grid after it.
+// giants != 0: rows longer than kGiantDeg over the whole grid after it, in
+// whole 2048-entry chunks (1) or in rounds that stop once the row is dry (2).
 void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* list, int nlist,
                            int gridx, int giants);
 // One workgroup of `threads` (256, 512 or 1024) per tile, tile heights and

@@ -2287,7 +2287,7 @@ __device__ __forceinline__ bool cas64(P p, int64_t& expect, int64_t want) {
 // (one preflow entry's capacity, or what its neighbours pushed) usually
 // fits into the lower entries of its lead chunk: the other chunks then find
 // the row dry and read none of their entries.
-template <int PHASE>
+template <int PHASE, bool ROUNDS>
 __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__ slots,
                                                     const int* __restrict__ list) {
     constexpr int phase = PHASE;
@@ -2323,65 +2323,130 @@ __global__ __launch_bounds__(256) void k_giant_push(const SlotView* __restrict__
         __syncthreads();
         // dry, cut off or above this step's band: no report, no relabel
         if (ex0 <= 0 || hv >= n || hv > S.disc_cap) continue;
-        // the chunk in rounds of blockDim.x entries, each claiming from the
-        // row what its lower entries can take: once a round claims less than
-        // that, the row is dry and the rest of the chunk is neither read nor
-        // gathered (a hub's excess usually fits into its first lower
-        // entries); a chunk that read every round reports its minimum
         int m = INT_MAX - 1;
-        bool dry = false;
-        int rounds = 0;
-        for (int j = 0; j < PER; ++j) {
-            const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
-            int64_t r = 0;
-            int cw = 0, hw = INT_MAX;
-            if (e < e1) {
-                r = ld64(&S.res[e]);
-                cw = S.g.col[e];
+        if constexpr (ROUNDS) {
+            // the chunk in rounds of blockDim.x entries, each claiming from
+            // the row what its lower entries can take: once a round claims
+            // less than that, the row is dry and the rest of the chunk is
+            // neither read nor gathered (a hub's excess usually fits into its
+            // first lower entries); a chunk that read every round reports its
+            // minimum.  Fewer bytes per discharge, but the excess lands on
+            // the first lower entries of a chunk instead of all of them, which
+            // costs iterations: engines of many slots (bandwidth) use it, a
+            // one-slot solver (latency) does not
+            bool dry = false;
+            int rounds = 0;
+            for (int j = 0; j < PER; ++j) {
+                const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
+                int64_t r = 0;
+                int cw = 0, hw = INT_MAX;
+                if (e < e1) {
+                    r = ld64(&S.res[e]);
+                    cw = S.g.col[e];
+                }
+                int64_t a = 0;
+                if (r > 0) {
+                    hw = S.h[cw];
+                    if (hw < hv) a = r;
+                } else {
+                    r = 0;
+                }
+                int64_t tot = 0;
+                const int64_t incl = block_incl_scan_i64(a, lscr, &tot);
+                ++rounds;
+                int64_t got = 0;
+                if (tot > 0) {  // block-uniform
+                    if (threadIdx.x == 0) {
+                        int64_t cur = ld64(&S.excess[v]);
+                        int64_t take = 0;
+                        while (cur > 0) {
+                            take = cur < tot ? cur : tot;
+                            if (cas64(&S.excess[v], cur, cur - take)) break;
+                        }
+                        bcast = cur > 0 ? take : 0;
+                    }
+                    __syncthreads();
+                    got = bcast;
+                    __syncthreads();
+                }
+                const int64_t rem = got - (incl - a);
+                if (a > 0 && rem > 0) {
+                    const int64_t d = rem < a ? rem : a;
+                    move_res(S, e, d);
+                    add64(&S.excess[cw], d);
+                    touch(S, cw);
+                    r -= d;
+                    wk.pushes++;
+                }
+                if (r > 0) m = min(m, hw);
+                if (got < tot) {  // block-uniform: the row's excess ran out in this round
+                    dry = true;
+                    break;
+                }
+                if (e0 + (j + 1) * (int)blockDim.x >= e1) break;
             }
-            int64_t a = 0;
-            if (r > 0) {
-                hw = S.h[cw];
-                if (hw < hv) a = r;
-            } else {
-                r = 0;
+            if (threadIdx.x == 0) wk.edges += (uint64_t)min(e1 - e0, rounds * (int)blockDim.x);
+            if (dry) continue;  // block-uniform
+        } else {
+            // the whole chunk into registers, one claim, pushes in prefix order
+            int64_t r[PER];
+            int hw[PER], cw[PER];
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
+                r[j] = 0;
+                cw[j] = 0;
+                if (e < e1) {
+                    r[j] = ld64(&S.res[e]);
+                    cw[j] = S.g.col[e];
+                }
+            }
+            int64_t tsum = 0;
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                hw[j] = INT_MAX;
+                if (r[j] > 0) {
+                    hw[j] = S.h[cw[j]];
+                    if (hw[j] < hv) tsum += r[j];
+                } else {
+                    r[j] = 0;
+                }
             }
             int64_t tot = 0;
-            const int64_t incl = block_incl_scan_i64(a, lscr, &tot);
-            ++rounds;
+            const int64_t incl = block_incl_scan_i64(tsum, lscr, &tot);
+            if (threadIdx.x == 0) wk.edges += (uint64_t)(e1 - e0);
             int64_t got = 0;
             if (tot > 0) {  // block-uniform
                 if (threadIdx.x == 0) {
                     int64_t cur = ld64(&S.excess[v]);
-                    int64_t take = 0;
+                    int64_t a = 0;
                     while (cur > 0) {
-                        take = cur < tot ? cur : tot;
-                        if (cas64(&S.excess[v], cur, cur - take)) break;
+                        a = cur < tot ? cur : tot;
+                        if (cas64(&S.excess[v], cur, cur - a)) break;
                     }
-                    bcast = cur > 0 ? take : 0;
+                    bcast = cur > 0 ? a : 0;
                 }
                 __syncthreads();
                 got = bcast;
                 __syncthreads();
             }
-            const int64_t rem = got - (incl - a);
-            if (a > 0 && rem > 0) {
-                const int64_t d = rem < a ? rem : a;
-                move_res(S, e, d);
-                add64(&S.excess[cw], d);
-                touch(S, cw);
-                r -= d;
-                wk.pushes++;
+            int64_t rem = got - (incl - tsum);
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                if (r[j] <= 0) continue;
+                if (hw[j] < hv && rem > 0) {
+                    const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
+                    const int64_t d = rem < r[j] ? rem : r[j];
+                    move_res(S, e, d);
+                    add64(&S.excess[cw[j]], d);
+                    touch(S, cw[j]);
+                    rem -= d;
+                    r[j] -= d;
+                    wk.pushes++;
+                }
+                if (r[j] > 0) m = min(m, hw[j]);
             }
-            if (r > 0) m = min(m, hw);
-            if (got < tot) {  // block-uniform: the row's excess ran out in this round
-                dry = true;
-                break;
-            }
-            if (e0 + (j + 1) * (int)blockDim.x >= e1) break;
         }
-        if (threadIdx.x == 0) wk.edges += (uint64_t)min(e1 - e0, rounds * (int)blockDim.x);
-        if (dry) continue;  // block-uniform
         m = block_min_i32(m, iscr);
         if (threadIdx.x == 0) {
             min32(&S.giant[S.gcap + k], m);
@@ -2473,17 +2538,24 @@ void launch_bfs_td(hipStream_t st, const SlotView* slots, const int* list, int n
     hipLaunchKernelGGL(k_bfs_advance, dim3(1, nlist), dim3(64), 0, st, slots, list, 0);
 }
 
-void launch_giant(hipStream_t st, const SlotView* slots, const int* list, int nlist) {
+// giants: 1 = whole chunks, 2 = chunks in rounds (k_giant_push)
+void launch_giant(hipStream_t st, const SlotView* slots, const int* list, int nlist, int giants) {
     hipLaunchKernelGGL(k_giant_prep, dim3(1, nlist), dim3(1024), 0, st, slots, list);
-    hipLaunchKernelGGL(k_giant_push<1>, dim3(per_slot_grid(512, nlist), nlist), dim3(256), 0, st, slots, list);
-    hipLaunchKernelGGL(k_giant_push<2>, dim3(per_slot_grid(512, nlist), nlist), dim3(256), 0, st, slots, list);
+    const dim3 grid(per_slot_grid(512, nlist), nlist);
+    if (giants == 2) {
+        hipLaunchKernelGGL((k_giant_push<1, true>), grid, dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL((k_giant_push<2, true>), grid, dim3(256), 0, st, slots, list);
+    } else {
+        hipLaunchKernelGGL((k_giant_push<1, false>), grid, dim3(256), 0, st, slots, list);
+        hipLaunchKernelGGL((k_giant_push<2, false>), grid, dim3(256), 0, st, slots, list);
+    }
     hipLaunchKernelGGL(k_giant_relabel, dim3(1, nlist), dim3(256), 0, st, slots, list);
 }
 
 void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* list, int nlist,
                            int gridx, int giants) {
     hipLaunchKernelGGL(k_discharge_list, dim3(gridx, nlist), dim3(256), 0, st, slots, list);
-    if (giants) launch_giant(st, slots, list, nlist);
+    if (giants) launch_giant(st, slots, list, nlist, giants);
 }
 
 // Region (tile) discharge (BC:432-515, one block per region): the tile's
@@ -2671,7 +2743,7 @@ void launch_discharge_tile(hipStream_t st, const SlotView* slots, const int* lis
     else
         hipLaunchKernelGGL(k_discharge_tile<1024>, dim3(gridx, nlist), dim3(1024), 0, st, slots, list, hot_only,
                            rounds, sp);
-    if (!hot_only && giants) launch_giant(st, slots, list, nlist);
+    if (!hot_only && giants) launch_giant(st, slots, list, nlist, giants);
 }
 
 // ------------------------------------------------------------ path push (a17)

@@ -237,7 +237,8 @@ def test_giant_rows():
     d = hg.export()
     og = O.Graph(n, d["row_ptr"], d["col"], d["cap"], d["rev"])
     dg = M.DeviceGraph(hg, 1024)
-    for variant in (dict(), dict(tile_frac=0.0), dict(tile_frac=1e9), dict(prune=0)):
+    for variant in (dict(), dict(tile_frac=0.0), dict(tile_frac=1e9), dict(prune=0),
+                    dict(giant_rounds=1), dict(giant_rounds=1, tile_frac=0.0)):
         eng = M.Engine(1, M.default_params(**variant))
         gid = eng.add_graph(dg)
         for a, b in ((s, t), (hub, t), (s, int(leaves[5]))):
