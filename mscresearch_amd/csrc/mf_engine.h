@@ -52,8 +52,10 @@ struct Params {
     int cert = 3;
     long long iter_max = 20000000;  // BC:1476, but reported as an error
     // levels one single-workgroup BFS launch may run: bounds a step's length
-    // on deep graphs so batched slots keep moving while one relabels
-    int small_levels = 256;
+    // on deep graphs so batched slots keep moving while one relabels (grid
+    // 1024^2, 512 queries in 256 slots: 128 207 q/s against 166-172 at 256,
+    // 201 at 96, 203 at 64 and 124 at 512)
+    int small_levels = 128;
     // workgroups per slot of those multi-level BFS launches on deep graphs
     // (last relabel, or the running BFS, at least 64 levels deep); 1 = one
     // 1024-thread workgroup.  More meet at a barrier in global memory after
