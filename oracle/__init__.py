@@ -23,7 +23,10 @@ _lib = None
 
 
 def build() -> str:
+    """Build liboracle.so, and the reference's SCC tools into _ref/ when their sources are present."""
     subprocess.run(["make", "-s", "-C", _DIR], check=True)
+    if os.path.isdir("/root/reference/SCC_CONTRACT"):
+        subprocess.run(["make", "-s", "-C", _DIR, "ref"], check=True)
     return LIB_PATH
 
 
