@@ -466,6 +466,13 @@ int mf_dbg_certify(mf_engine* e, int gid, int s, int t, const int64_t* res, cons
  * graph): tile_hot[i] = 1 for the k tiles first in (count descending, index
  * ascending) order among tiles with a nonzero count. */
 int mf_dbg_hot_select(mf_engine* e, int gid, const int32_t* tile_cnt, int k, int32_t* tile_hot);
+/* Wait bound, in wall-clock ticks, of the engine's multi-workgroup BFS
+ * barriers (ticks <= 0: the default of 2 s).  A barrier that waits longer
+ * flags its slot and the BFS restarts single-workgroup; a tiny bound forces
+ * that path from the host.  *restarts (may be NULL) gets the restarts of the
+ * queries the engine has retired so far.  No counterpart in the reference,
+ * whose BFS is one launch per level (BC:1538-1572). */
+int mf_dbg_barrier_ticks(mf_engine* e, long long ticks, long long* restarts);
 /* Solve without pruning on slot 0 and copy the final residual, excess and
  * heights out (for an independent CPU certificate). */
 int mf_dbg_solve_state(mf_engine* e, int gid, int s, int t, mf_result* r, int64_t* res,

@@ -181,6 +181,7 @@ SIGNATURES = {
     "mf_dbg_class_mask": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "mf_dbg_induced": (C.c_int, [_P, C.c_int, _P, C.POINTER(C.c_int), C.POINTER(C.c_int64), _P, _P, _P, _P, _P]),
     "mf_dbg_solve_state": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(Result), _P, _P, _P]),
+    "mf_dbg_barrier_ticks": (C.c_int, [_P, C.c_longlong, C.POINTER(C.c_longlong)]),
 }
 
 
@@ -689,6 +690,13 @@ class Engine:
         _check(lib().mf_dbg_local_relabel(self._h, gid, int(s), int(t), _ptr(res), _ptr(excess), _ptr(h),
                                           _ptr(ta), int(lcap), C.byref(lvl)))
         return h, lvl.value
+
+    def dbg_barrier_ticks(self, ticks):
+        """Set the multi-workgroup BFS barrier's wait bound (ticks <= 0: the
+        default); returns the BFS restarts of the queries retired so far."""
+        r = C.c_longlong(0)
+        _check(lib().mf_dbg_barrier_ticks(self._h, int(ticks), C.byref(r)))
+        return r.value
 
     def dbg_hot_select(self, gid, tile_cnt, k):
         """The hot sweep's choice of k tiles on given per-tile active counts."""

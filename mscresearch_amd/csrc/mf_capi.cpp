@@ -1012,6 +1012,13 @@ int mf_dbg_certify(mf_engine* e, int gid, int s, int t, const int64_t* res, cons
     return rc;
 }
 
+int mf_dbg_barrier_ticks(mf_engine* e, long long ticks, long long* restarts) {
+    if (!e) return fail(MF_EINVAL, "null engine");
+    e->eng.set_barrier_ticks(ticks);
+    if (restarts) *restarts = e->eng.bfs_restarts;
+    return MF_OK;
+}
+
 int mf_dbg_hot_select(mf_engine* e, int gid, const int32_t* tile_cnt, int k, int32_t* tile_hot) {
     if (!e || !tile_cnt || !tile_hot || k < 1) return fail(MF_EINVAL, "bad argument");
     int rc = check_query(e, gid, 0, 1);

@@ -1257,6 +1257,7 @@ int Engine::step(std::vector<Result>& out) {
         r.tag = S.q.tag;
         r.st = S.sched.stats;
         r.st.status = S.sched.status();
+        bfs_restarts += r.st.bfs_restarts;
         const Ctr& c = h_ctr_[i];
         r.st.edge_scans = r.st.vertex_scans = r.st.pushes = r.st.relabels = 0;
         for (int f = 0; f < kFamilies; ++f) {

@@ -605,6 +605,11 @@ private:
     uint64_t graph_clock_ = 0;
 public:
     int64_t graph_replays = 0, graph_captures = 0;
+    // BFS runs of retired queries restarted single-workgroup after a
+    // multi-workgroup barrier gave up waiting
+    int64_t bfs_restarts = 0;
+    // the barrier's wait bound in wall-clock ticks (<= 0: the default, 2 s)
+    void set_barrier_ticks(long long ticks) { spin_ticks_ = ticks > 0 ? ticks : barrier_spin_ticks(); }
 private:
     size_t budget_ = 0;         // 0 = nslots full slabs of the largest graph
     bool arena_stale_ = false;  // default-sized arena smaller than the graphs now need

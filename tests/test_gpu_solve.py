@@ -75,6 +75,29 @@ def test_grid_flows(rows, cols, fuse, extra):
         assert r.cert_ok == 1
 
 
+@pytest.mark.parametrize("nslots", [1, 4])
+def test_barrier_timeout_restarts_single_workgroup(nslots):
+    """A multi-workgroup BFS barrier that waits past its bound flags the slot,
+    and the BFS starts again in one workgroup.  Forced from the host with a
+    bound of one clock tick (every wait ends at once, so every kernel exits):
+    the restart path runs and every flow stays exact and certified."""
+    rows, cols = 128, 96
+    log(f"grid {rows}x{cols}, {nslots} slot(s), barrier bound 1 tick")
+    case = Case(M.Edges.grid(rows, cols, 1000, 2001), R=64, tile_max=512)
+    eng, gid = engine_with(case, nslots=nslots, small_wgs=4)
+    assert eng.dbg_barrier_ticks(1) == 0
+    qs, qt = M.grid_queries(rows, cols, 8, 2002)
+    q = [case.q(a, b) for a, b in zip(qs, qt)]
+    for i, (s, t) in enumerate(q):
+        eng.submit(gid, s, t, i)
+    res = eng.run()
+    assert len(res) == len(q)
+    for r in res:
+        assert r.status == 0 and r.cert_ok == 1
+        assert r.flow == case.oracle_flow(*q[r.tag]), r.tag
+    assert eng.dbg_barrier_ticks(0) > 0  # the restart path ran
+
+
 @pytest.mark.parametrize("scale", [12, 14])
 def test_rmat_flows_pruned(scale):
     log(f"rmat scale {scale}")
