@@ -244,6 +244,8 @@ typedef struct mf_params {
     long long small_edges_deep; /* small_edges on deep graphs (relabel, depth hint or BFS 64+ levels deep) */
     int giant_rounds;    /* giant-row discharge chunks in rounds that stop once the row is dry: 1 on, 0 off,
                             2 = on in engines of more than one slot */
+    int bu_recv;         /* 1: bottom-up relabel levels test rows no flow has entered or left through the
+                            graph's forward-entry mask instead of their residuals */
 } mf_params;
 
 void mf_params_default(mf_params* p);

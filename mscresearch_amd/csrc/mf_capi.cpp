@@ -126,6 +126,7 @@ Params from_c(const mf_params* p) {
     P.gr_work_deep = p->gr_work_deep;
     P.small_edges_deep = p->small_edges_deep;
     P.giant_rounds = p->giant_rounds;
+    P.bu_recv = p->bu_recv;
     return P;
 }
 
@@ -535,6 +536,7 @@ void mf_params_default(mf_params* p) {
     p->gr_work_deep = P.gr_work_deep;
     p->small_edges_deep = P.small_edges_deep;
     p->giant_rounds = P.giant_rounds;
+    p->bu_recv = P.bu_recv;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

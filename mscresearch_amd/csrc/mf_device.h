@@ -45,6 +45,11 @@ struct DGraph {
     // long-row heads are few, reached early and their levels stay cached,
     // so most vertices find their parent without reading their own row.
     const int32_t* bu_pick = nullptr;
+    // fwdm: bit e & 31 of fwdm[e >> 5] is set when entry e is a forward
+    // entry (cap > 0; capacity updates keep forward entries forward).  A
+    // bottom-up relabel level tests a row whose residuals still equal its
+    // capacities through it: only its forward entries can lead to a parent.
+    const uint32_t* fwdm = nullptr;
 };
 
 // Offset (in int32 words, 16-byte aligned) of the giant list's task records.
@@ -175,6 +180,15 @@ struct SlotView {
     int64_t bfs_edges = 1 << 16;
     int32_t phase = 0;           // 0 = normal (seeds t + deficits), 1 = drain (seeds deficits, t active)
     int32_t seed_t_only = 0;     // phase 0 with no deficit (last count): the relabel seeds t alone
+    // recv[v] = 1 once flow entered v's row or left it: every push marks its
+    // head (move_res), the preflow s, a repair its pusher; the residual
+    // resets clear it.  A vertex never marked holds res = cap on its whole
+    // row (no vertex but s and a repaired one pushes without receiving
+    // first).  recv_ok: the slab's marks follow this rule (not after
+    // capacity updates of a warm start, nor for residuals a debug hook
+    // loaded), so bottom-up relabel levels may use them.
+    uint8_t* recv = nullptr;
+    int32_t recv_ok = 0;
     // 1 = bounded injection: the preflow puts B = min(out-capacity of s,
     // in-capacity of t) on s instead of saturating its row, and s is then an
     // ordinary vertex (labelled by the relabel, active, discharged); 0 = the

@@ -73,6 +73,7 @@ DeviceGraph::~DeviceGraph() {
     (void)hipFree(d_capsum); (void)hipFree(d_cert_fp); (void)hipFree(d_pcap);
     (void)hipFree(d_fstart);
     (void)hipFree(d_bu_pick);
+    (void)hipFree(d_fwdm);
 }
 
 DeviceGraph::FwdSet::~FwdSet() { (void)hipFree(d_mask); }
@@ -189,6 +190,9 @@ int init_cert_consts(DeviceGraph& dg, hipStream_t st) {
     HIPCHK(hipMalloc(&dg.d_bu_pick, sizeof(int32_t) * 2 * (size_t)std::max(n, 1)));
     launch_bu_pick(st, dg.g, dg.d_bu_pick);
     dg.g.bu_pick = dg.d_bu_pick;
+    HIPCHK(hipMalloc(&dg.d_fwdm, sizeof(uint64_t) * (((size_t)std::max(dg.g.E, 1) + 63) / 64)));
+    launch_fwd_mask(st, dg.g, dg.d_fwdm);
+    dg.g.fwdm = dg.d_fwdm;
     dg.g.cert_seed = 0x9e3779b97f4a7c15ULL * (++graphs_made) ^ ((uint64_t)(uint32_t)n << 32) ^ (uint64_t)(uint32_t)dg.g.E;
     launch_cert_consts(st, dg.g, dg.d_capsum, dg.d_cert_fp);
     HIPCHK(hipGetLastError());
@@ -225,6 +229,7 @@ size_t Slab::layout(int n, int64_t E, int ntiles, char* base, Slab* sl) {
     char* p_gi = take(sizeof(int32_t) * (giant_task_offset((int64_t)gcap) + 4 * (ee / kGiantChunk + gcap + 1)));
     char* p_ga = take(sizeof(int64_t) * gcap);
     char* p_fb2 = take(16 * ((nn + 63) / 64));
+    char* p_recv = take(nn);
     char* p_em = take(ee);
     const size_t mcap = std::min(ee, (size_t)kMarkListMax);
     char* p_ml = take(sizeof(int32_t) * (mcap + 1));
@@ -245,6 +250,7 @@ size_t Slab::layout(int n, int64_t E, int ntiles, char* base, Slab* sl) {
         sl->giant = (int32_t*)p_gi;
         sl->gacc = (int64_t*)p_ga;
         sl->fbits = (uint64_t*)p_fb2;
+        sl->recv = (uint8_t*)p_recv;
         sl->emark = (uint8_t*)p_em;
         sl->mlist = (int32_t*)p_ml;
         sl->mcap = (int)mcap;
@@ -966,6 +972,8 @@ int Engine::admit(int i, const Query& q, bool warm, bool prune_ok) {
     v.giant = sl.giant; v.gacc = sl.gacc; v.gcap = sl.gcap;
     v.emark = sl.emark;
     v.fbits = P_.bu_bits ? sl.fbits : nullptr;
+    v.recv = sl.recv;
+    v.recv_ok = (P_.bu_recv && !warm) ? 1 : 0;
     v.mlist = sl.mlist;
     v.mcap = std::max(0, std::min(sl.mcap, P_.mark_list));
     v.ctr = d_ctrs_ + i;
@@ -1926,6 +1934,7 @@ int Engine::dbg_begin(int gid, int s, int t) {
     rc = admit(0, q, false, false);
     if (rc != MF_OK) return rc;
     slots_[0].view.inject = 0;
+    slots_[0].view.recv_ok = 0;  // hooks load residuals the marks do not follow
     memset(&h_ctr_[0], 0, sizeof(Ctr));
     return MF_OK;
 }

@@ -117,6 +117,9 @@ struct Params {
     // bottom-up rows up to bu_light entries go to one lane each, longer ones
     // to a wave each (k_bfs_bu_med)
     int bu_light = 128;
+    // 1: bottom-up relabel levels test rows no flow has entered or left
+    // through the graph's forward mask, without reading their residuals
+    int bu_recv = 1;
     // bottom-up levels test candidate parents in a frontier bitmap (n / 8
     // bytes per slot) instead of gathering their levels (0 = gather levels)
     int bu_bits = 1;
@@ -238,6 +241,7 @@ struct DeviceGraph {
     int64_t* d_pcap = nullptr;     // pair capacities (DGraph::pcap)
     int32_t* d_fstart = nullptr;   // first forward entry per row (DGraph::fstart)
     int32_t* d_bu_pick = nullptr;  // DGraph::bu_pick
+    uint32_t* d_fwdm = nullptr;    // DGraph::fwdm
     struct FwdSet {
         std::vector<uint8_t> mask;  // host copy
         uint8_t* d_mask = nullptr;
@@ -294,6 +298,7 @@ struct Slab {
     int64_t* gacc = nullptr;
     int gcap = 0;
     uint64_t* fbits = nullptr;  // 2 x ceil(n / 64) frontier bitmap words (SlotView::fbits)
+    uint8_t* recv = nullptr;    // n bytes: the vertex's row left its capacities (SlotView::recv)
     uint8_t* emark = nullptr;   // E marks of changed entry pairs
     int32_t* mlist = nullptr;   // count + the marked entries (SlotView::mlist)
     int mcap = 0;

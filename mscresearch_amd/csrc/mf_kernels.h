@@ -134,5 +134,7 @@ void launch_cert_consts(hipStream_t st, const DGraph& g, int64_t* capsum, uint64
 void launch_first_forward(hipStream_t st, const DGraph& g, int32_t* fstart);
 // DGraph::bu_pick (2n ints)
 void launch_bu_pick(hipStream_t st, const DGraph& g, int32_t* pick);
+// DGraph::fwdm: one bit per entry, set for forward entries (cap > 0).
+void launch_fwd_mask(hipStream_t st, const DGraph& g, uint32_t* fwdm);
 
 }  // namespace mf

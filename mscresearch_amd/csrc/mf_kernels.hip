@@ -130,6 +130,7 @@ struct GGraph {
     const GAS int64_t* pcap;
     const GAS int32_t* fstart;
     const GAS int32_t* bu_pick;
+    const GAS uint32_t* fwdm;
 };
 
 struct GSlot {
@@ -151,6 +152,8 @@ struct GSlot {
     GAS int64_t* gacc;
     int32_t gcap;
     GAS uint64_t* fbits;
+    GAS uint8_t* recv;
+    int32_t recv_ok;
     GAS uint8_t* emark;
     GAS int32_t* mlist;
     int32_t mcap;
@@ -186,6 +189,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.g.pcap = as_global(v.g.pcap);
     S.g.fstart = as_global(v.g.fstart);
     S.g.bu_pick = as_global(v.g.bu_pick);
+    S.g.fwdm = as_global(v.g.fwdm);
     S.res = as_global(v.res);
     S.excess = as_global(v.excess);
     S.h = as_global(v.h);
@@ -203,6 +207,8 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.gacc = as_global(v.gacc);
     S.gcap = v.gcap;
     S.fbits = as_global(v.fbits);
+    S.recv = as_global(v.recv);
+    S.recv_ok = v.recv_ok;
     S.emark = as_global(v.emark);
     S.mlist = as_global(v.mlist);
     S.mcap = v.mcap;
@@ -318,6 +324,7 @@ __device__ __forceinline__ void mark_entry(const GSlot& S, int e) {
 __device__ __forceinline__ void move_res(const GSlot& S, int e, int64_t d) {
     __hip_atomic_fetch_add(&S.res[e], -d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_fetch_add(&S.res[S.g.rev[e]], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (S.recv) S.recv[S.g.col[e]] = 1;  // flow entered the head's row
     mark_entry(S, e);
 }
 
@@ -419,6 +426,7 @@ __global__ __launch_bounds__(256) void k_init(const SlotView* __restrict__ slots
         S.h[v] = 0;
         S.lock[v] = -1;
         S.aux[v] = 0;  // no touch stamp of an earlier owner of this memory survives
+        if (S.recv) S.recv[v] = 0;
     }
     if (i0 == 0) {
         S.mlist[0] = 0;  // the marks are all cleared above
@@ -483,6 +491,7 @@ __global__ __launch_bounds__(256) void k_init_marked(const SlotView* __restrict_
         S.h[v] = 0;
         S.lock[v] = -1;
         S.aux[v] = 0;
+        if (S.recv) S.recv[v] = 0;
     }
     if (i0 == 0) wk.verts += (uint64_t)S.g.n;
     work_flush(S, wk, red, WF_INIT);
@@ -517,6 +526,7 @@ __global__ __launch_bounds__(256) void k_preflow(const SlotView* __restrict__ sl
     if (threadIdx.x == 0) {
         S.ctr->incap_t = in_t;
         S.ctr->outcap_s = S.g.capsum[s];
+        if (S.recv) S.recv[s] = 1;  // s pushes without receiving
     }
     if (S.inj) {
         // Bounded injection: no flow can exceed the out-capacity of s or the
@@ -775,6 +785,50 @@ __device__ __forceinline__ bool bu_row_lane(const GSlot& S, int rb, int fs, int 
     return hit;
 }
 
+// Bottom-up test of a row whose residuals still equal its capacities
+// (SlotView::recv): a parent is a forward entry whose head is on level L, so
+// only the forward entries of [fs, re) are read (the graph's forward mask,
+// 32 entries per load; [rb, fs) holds reverse entries only), their columns
+// and level tests U at a time.  `scanned` = entries covered up to the group
+// holding the parent.
+template <int U>
+__device__ __forceinline__ bool bu_row_fresh(const GSlot& S, int fs, int re, int L, const GAS int32_t* lvl,
+                                             const GAS uint32_t* bits, int& scanned) {
+    for (int e = fs; e < re;) {
+        const int lo = e & 31;
+        const int span = min(32 - lo, re - e);
+        uint32_t m = S.g.fwdm[e >> 5] >> lo;
+        if (span < 32) m &= (1u << span) - 1u;
+        while (m) {
+            int ek[U], c[U];
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                ek[k] = -1;
+                if (m) {
+                    ek[k] = e + __builtin_ctz(m);
+                    m &= m - 1u;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < U; ++k) c[k] = ek[k] >= 0 ? S.g.col[ek[k]] : -1;
+            bool hit = false;
+            int last = e;
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                hit |= c[k] >= 0 && on_level(bits, lvl, c[k], L);
+                last = max(last, ek[k]);
+            }
+            if (hit) {
+                scanned = last + 1 - fs;
+                return true;
+            }
+        }
+        e += span;
+    }
+    scanned = re - fs;
+    return false;
+}
+
 // GR seeding (a12): h = n everywhere, level 0 = t plus deficits (phase 0) or
 // deficits only (phase 1, draining from t), under the mask; s stays n.
 __global__ __launch_bounds__(256) void k_gr_seed(const SlotView* __restrict__ slots,
@@ -977,14 +1031,17 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
         if (v < n && lvl[v] == n && bfs_vertex_ok<MODE>(S, v) && (MODE != BFS_GR || S.live == nullptr || S.live[v])) {
             wk.verts++;
             const int rb = S.g.row_ptr[v], re = S.g.row_ptr[v + 1];
+            // no flow entered or left v's row: its residuals are its capacities
+            const bool fresh = MODE == BFS_GR && S.recv_ok && S.g.fwdm && S.recv[v] == 0;
             bool picked = false;
             if (MODE == BFS_GR && S.g.bu_pick) {
                 // the picked entry first (its head's level is likely cached);
                 // a forward entry no push has marked still holds its
                 // capacity, so its mark byte (128 entries a line) answers
-                // for its residual (16 a line)
+                // for its residual (16 a line), and a fresh row needs neither
                 const int pe = S.g.bu_pick[2 * v], pw = S.g.bu_pick[2 * v + 1];
-                picked = pe >= 0 && on_level(bits, lvl, pw, L) && ((S.marks_ok && S.emark[pe] == 0) || ld_res(&S.res[pe]) > 0);
+                picked = pe >= 0 && on_level(bits, lvl, pw, L) &&
+                         (fresh || (S.marks_ok && S.emark[pe] == 0) || ld_res(&S.res[pe]) > 0);
                 if (picked) {
                     wk.edges += 1;
                     joined = true;
@@ -999,7 +1056,10 @@ __global__ __launch_bounds__(256) void k_bfs_bu(const SlotView* __restrict__ slo
                 // the parent test only for entries that pass the residual
                 // test (most reverse entries carry no flow)
                 int scanned = 0;
-                joined = bu_row_lane<MODE, 2>(S, rb, bu_first<MODE>(S, v, rb), re, L, lvl, bits, scanned);
+                if (fresh)
+                    joined = bu_row_fresh<4>(S, bu_first<MODE>(S, v, rb), re, L, lvl, bits, scanned);
+                else
+                    joined = bu_row_lane<MODE, 2>(S, rb, bu_first<MODE>(S, v, rb), re, L, lvl, bits, scanned);
                 wk.edges += (uint64_t)scanned;
             }
             if (joined) {
@@ -2965,6 +3025,7 @@ __global__ __launch_bounds__(256) void k_rm(const SlotView* __restrict__ slots,
             const int64_t d = ld64(&S.res[re_]);
             if (d <= 0) continue;
             move_res(S, re_, d);
+            if (S.recv) S.recv[w] = 1;  // w pushed without receiving
             add64(&S.excess[w], -d);
             gained += d;
             ++fixed;
@@ -3589,6 +3650,22 @@ __global__ __launch_bounds__(256) void k_bu_pick(DGraph g, int32_t* __restrict__
             pick[2 * v + 1] = e >= 0 ? g.col[e] : -1;
         }
     }
+}
+
+// One wave per 64 entries: their forward bits as one 64-bit word (two
+// 32-bit words of fwdm; the array is 8-byte aligned).
+__global__ __launch_bounds__(256) void k_fwd_mask(DGraph g, uint32_t* __restrict__ fwdm) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < g.E; base += stride) {
+        const int64_t e = base + lane_id();
+        const uint64_t m = __ballot(e < g.E && g.cap[e] > 0);
+        if (lane_id() == 0) *(uint64_t*)(fwdm + (base >> 5)) = m;
+    }
+}
+
+void launch_fwd_mask(hipStream_t st, const DGraph& g, uint32_t* fwdm) {
+    const int gx = std::max(1, (int)std::min<int64_t>(16384, ((int64_t)g.E + 255) / 256));
+    hipLaunchKernelGGL(k_fwd_mask, dim3(gx), dim3(256), 0, st, g, fwdm);
 }
 
 void launch_bu_pick(hipStream_t st, const DGraph& g, int32_t* pick) {
