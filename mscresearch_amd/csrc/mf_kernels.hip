@@ -645,14 +645,24 @@ __device__ __forceinline__ bool edge_ok_aux(const GSlot& S, int aux, int w) {
     return aux != 0;
 }
 
+// Whether frontier vertex v's row still holds its capacities (GR only).
+template <int MODE>
+__device__ __forceinline__ bool td_fresh(const GSlot& S, int v) {
+    return MODE == BFS_GR && S.recv_ok && S.g.fwdm && S.recv[v] == 0;
+}
+
 // Top-down expansion of entries e0, e0 + step, ... below e1 by one thread,
 // U entries at a time: their column and entry-side loads, then the level
 // gathers of the entries that pass, each issued together (one memory round
 // trip per stage instead of one per entry and stage).  claim(w) runs for
-// each vertex this thread moved from n to level L + 1.
+// each vertex this thread moved from n to level L + 1.  fresh (GR, ROW): no
+// flow has entered or left the row's vertex (SlotView::recv), so its
+// residuals are its capacities and w can push into it exactly when the
+// entry is a reverse one (its pair, in w's row, is forward with cap > 0):
+// the forward mask answers instead of the residuals.
 template <int MODE, int U, bool ROW = true, typename Claim>
 __device__ __forceinline__ void td_entries(const GSlot& S, int e0, int e1, int step, int L,
-                                           GAS int32_t* lvl, Claim claim) {
+                                           GAS int32_t* lvl, Claim claim, bool fresh = false) {
     const int n = S.g.n;
     for (int e = e0; e < e1; e += U * step) {
         int w[U], ax[U], l[U];
@@ -664,7 +674,8 @@ __device__ __forceinline__ void td_entries(const GSlot& S, int e0, int e1, int s
             ax[k] = 0;
             if (ek < e1) {
                 w[k] = S.g.col[ek];
-                ax[k] = edge_aux<MODE, ROW>(S, ek);
+                if (MODE == BFS_GR && ROW && fresh) ax[k] = ((S.g.fwdm[ek >> 5] >> (ek & 31)) & 1u) ? 0 : 1;
+                else ax[k] = edge_aux<MODE, ROW>(S, ek);
             }
         }
 #pragma unroll
@@ -900,7 +911,7 @@ __device__ __forceinline__ void td_lane(const GSlot& S, int v, int L, GAS int32_
     td_entries<MODE, 4>(S, rb, re, 1, L, lvl, [&](int w) {
         stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
         wk.relabels++;
-    });
+    }, td_fresh<MODE>(S, v));
 }
 
 // Top-down expansion of one (heavy) frontier vertex by the whole workgroup.
@@ -913,7 +924,7 @@ __device__ __forceinline__ void td_block(const GSlot& S, int v, int L, GAS int32
     td_entries<MODE, 2>(S, rb + (int)threadIdx.x, re, (int)blockDim.x, L, lvl, [&](int w) {
         stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
         wk.relabels++;
-    });
+    }, td_fresh<MODE>(S, v));
 }
 
 template <int MODE>
@@ -964,7 +975,7 @@ __device__ __forceinline__ void td_wave(const GSlot& S, int v, int L, GAS int32_
     td_entries<MODE, 2>(S, rb + lane_id(), re, WAVE, L, lvl, [&](int w) {
         stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
         wk.relabels++;
-    });
+    }, td_fresh<MODE>(S, v));
 }
 
 // Medium rows of a top-down level, a wave each, taken grid-stride from the
@@ -2564,7 +2575,7 @@ __global__ __launch_bounds__(256) void k_bfs_td_giant(const SlotView* __restrict
         td_entries<MODE, 4>(S, e0 + (int)threadIdx.x, e1, (int)blockDim.x, L, lvl, [&](int w) {
             stage_push(st, w, S.g.row_ptr[w + 1] - S.g.row_ptr[w], out, ocnt, odeg);
             wk.relabels++;
-        });
+        }, td_fresh<MODE>(S, gt.v));
     }
     stage_flush(st, out, ocnt, odeg);
     const uint64_t nv = block_sum<uint64_t>(wk.relabels, red);
