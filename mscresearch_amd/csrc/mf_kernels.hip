@@ -635,12 +635,17 @@ __device__ __forceinline__ int edge_aux(const GSlot& S, int e) {
     return 1;
 }
 
+// GR without ROW, a fresh row (td_entries): the pair's residual is known
+// from the forward mask, carried in aux as kFreshYes / kFreshNo instead of
+// the pair's entry index.
+constexpr int kFreshNo = -1, kFreshYes = -2;
+
 template <int MODE, bool ROW>
 __device__ __forceinline__ bool edge_ok_aux(const GSlot& S, int aux, int w) {
     if (MODE == BFS_GR) {
         if (w == S.s && !S.inj) return false;
         if (S.mask && S.mask[w] == 0) return false;
-        if (!ROW) return ld_res(&S.res[aux]) > 0;
+        if (!ROW) return aux == kFreshYes || (aux >= 0 && ld_res(&S.res[aux]) > 0);
     }
     return aux != 0;
 }
@@ -674,14 +679,18 @@ __device__ __forceinline__ void td_entries(const GSlot& S, int e0, int e1, int s
             ax[k] = 0;
             if (ek < e1) {
                 w[k] = S.g.col[ek];
-                if (MODE == BFS_GR && ROW && fresh) ax[k] = ((S.g.fwdm[ek >> 5] >> (ek & 31)) & 1u) ? 0 : 1;
-                else ax[k] = edge_aux<MODE, ROW>(S, ek);
+                if (MODE == BFS_GR && fresh) {
+                    const bool into = !((S.g.fwdm[ek >> 5] >> (ek & 31)) & 1u);  // a reverse entry
+                    ax[k] = ROW ? (into ? 1 : 0) : (into ? kFreshYes : kFreshNo);
+                } else {
+                    ax[k] = edge_aux<MODE, ROW>(S, ek);
+                }
             }
         }
 #pragma unroll
         for (int k = 0; k < U; ++k) {
-            // without ROW a GR aux is an entry index, not a flag
-            const bool pre = (MODE == BFS_GR && !ROW) || ax[k] != 0;
+            // without ROW a GR aux is an entry index (or a fresh answer), not a flag
+            const bool pre = (MODE == BFS_GR && !ROW) ? ax[k] != kFreshNo : ax[k] != 0;
             l[k] = (w[k] >= 0 && pre) ? lvl[w[k]] : -1;
         }
 #pragma unroll
@@ -701,7 +710,7 @@ __device__ __forceinline__ void td_entries(const GSlot& S, int e0, int e1, int s
 // level L + 1.
 template <int MODE, int U, typename Claim>
 __device__ __forceinline__ void td_entries_deep(const GSlot& S, int e0, int e1, int step, int L,
-                                                GAS int32_t* lvl, Claim claim) {
+                                                GAS int32_t* lvl, Claim claim, bool fresh = false) {
     const int n = S.g.n;
     for (int e = e0; e < e1; e += U * step) {
         int w[U], ax[U], l[U], d[U];
@@ -713,7 +722,10 @@ __device__ __forceinline__ void td_entries_deep(const GSlot& S, int e0, int e1, 
             ax[k] = 0;
             if (ek < e1) {
                 w[k] = S.g.col[ek];
-                ax[k] = MODE == BFS_GR ? S.g.rev[ek] : edge_aux<MODE, true>(S, ek);
+                if (MODE == BFS_GR && fresh)  // a reverse entry of a fresh row: its pair holds its capacity
+                    ax[k] = ((S.g.fwdm[ek >> 5] >> (ek & 31)) & 1u) ? kFreshNo : kFreshYes;
+                else
+                    ax[k] = MODE == BFS_GR ? S.g.rev[ek] : edge_aux<MODE, true>(S, ek);
             }
         }
 #pragma unroll
@@ -721,9 +733,10 @@ __device__ __forceinline__ void td_entries_deep(const GSlot& S, int e0, int e1, 
             l[k] = -1;
             okx[k] = false;
             d[k] = 0;
-            if (w[k] >= 0) {
+            if (w[k] >= 0 && !(MODE == BFS_GR && ax[k] == kFreshNo)) {
                 l[k] = lvl[w[k]];
-                okx[k] = MODE == BFS_GR ? ld_res(&S.res[ax[k]]) > 0 : ax[k] != 0;
+                if (MODE == BFS_GR) okx[k] = ax[k] == kFreshYes || (ax[k] >= 0 && ld_res(&S.res[ax[k]]) > 0);
+                else okx[k] = ax[k] != 0;
                 d[k] = S.g.row_ptr[w[k] + 1] - S.g.row_ptr[w[k]];
             }
         }
@@ -1313,7 +1326,7 @@ __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__
                 out[p] = w;
                 add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
                 wk.relabels++;
-            });
+            }, td_fresh<MODE>(S, v));
         }
         __syncthreads();
         const int nh = min(nheavy, 1024);
@@ -1326,7 +1339,7 @@ __global__ __launch_bounds__(1024) void k_bfs_small(const SlotView* __restrict__
                 out[p] = w;
                 add64_lds(&next_deg, (int64_t)(S.g.row_ptr[w + 1] - S.g.row_ptr[w]));
                 wk.relabels++;
-            });
+            }, td_fresh<MODE>(S, v));
         }
         __syncthreads();
         fsz = next_cnt;
@@ -1449,7 +1462,7 @@ __global__ __launch_bounds__(256) void k_bfs_multi(const SlotView* __restrict__ 
             td_entries_deep<MODE, 4>(S, rb, re, 1, L, lvl, [&](int w, int deg) {
                 stage_push(st, w, deg, out, ocnt, odeg);
                 wk.relabels++;
-            });
+            }, td_fresh<MODE>(S, v));
         }
         __syncthreads();
         const int nh = min(nheavy, 256);
@@ -1460,7 +1473,7 @@ __global__ __launch_bounds__(256) void k_bfs_multi(const SlotView* __restrict__ 
             td_entries_deep<MODE, 2>(S, rb + (int)threadIdx.x, re, (int)blockDim.x, L, lvl, [&](int w, int deg) {
                 stage_push(st, w, deg, out, ocnt, odeg);
                 wk.relabels++;
-            });
+            }, td_fresh<MODE>(S, v));
         }
         stage_flush(st, out, ocnt, odeg);
         if (!slot_barrier(S.ctr, gen, spin_ticks, &ok_s)) break;
