@@ -929,6 +929,14 @@ int mf_dbg_global_relabel(mf_engine* e, int gid, int s, int t, const int64_t* re
     e->eng.slab(0)->res_uid = 0;  // res no longer follows the slab's marks
     e->eng.slab(0)->marks_ok = false;
     HIPCK(hipMemcpy(v.excess, excess, sizeof(int64_t) * g.n, hipMemcpyHostToDevice));
+    // rows of the given residual that still hold their capacities take the
+    // relabel's fresh-row tests, as in a solve
+    if (e->eng.params().bu_recv && g.fwdm) {
+        launch_recv_from_res(nullptr, g, v.res, v.recv);
+        HIPCK(hipGetLastError());
+        HIPCK(hipDeviceSynchronize());
+        v.recv_ok = 1;
+    }
     Plan p;
     p.stages = ST_RESET_PRE | ST_GR_SEED | ST_BFS;
     p.reset_pre = RST_BFS | RST_WORK | RST_ERR;

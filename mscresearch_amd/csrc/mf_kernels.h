@@ -136,5 +136,8 @@ void launch_first_forward(hipStream_t st, const DGraph& g, int32_t* fstart);
 void launch_bu_pick(hipStream_t st, const DGraph& g, int32_t* pick);
 // DGraph::fwdm: one bit per entry, set for forward entries (cap > 0).
 void launch_fwd_mask(hipStream_t st, const DGraph& g, uint32_t* fwdm);
+// SlotView::recv from a residual loaded from the host (a row that differs
+// from its capacities anywhere is marked).
+void launch_recv_from_res(hipStream_t st, const DGraph& g, const int64_t* res, uint8_t* recv);
 
 }  // namespace mf

@@ -3687,6 +3687,24 @@ __global__ __launch_bounds__(256) void k_fwd_mask(DGraph g, uint32_t* __restrict
     }
 }
 
+// recv[v] from a given residual: 1 when some entry of v's row differs from
+// its capacity (the rule the pushes keep, for a residual loaded from the
+// host).
+__global__ __launch_bounds__(256) void k_recv_from_res(DGraph g, const int64_t* __restrict__ res,
+                                                      uint8_t* __restrict__ recv) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += stride) {
+        uint8_t r = 0;
+        for (int e = g.row_ptr[v]; e < g.row_ptr[v + 1]; ++e) r |= res[e] != g.cap[e] ? 1 : 0;
+        recv[v] = r;
+    }
+}
+
+void launch_recv_from_res(hipStream_t st, const DGraph& g, const int64_t* res, uint8_t* recv) {
+    const int gx = std::max(1, std::min(16384, (g.n + 255) / 256));
+    hipLaunchKernelGGL(k_recv_from_res, dim3(gx), dim3(256), 0, st, g, res, recv);
+}
+
 void launch_fwd_mask(hipStream_t st, const DGraph& g, uint32_t* fwdm) {
     const int gx = std::max(1, (int)std::min<int64_t>(16384, ((int64_t)g.E + 255) / 256));
     hipLaunchKernelGGL(k_fwd_mask, dim3(gx), dim3(256), 0, st, g, fwdm);
