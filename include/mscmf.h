@@ -457,8 +457,9 @@ int mf_dbg_local_relabel(mf_engine* e, int gid, int s, int t, const int64_t* res
                          const int64_t* excess, int32_t* height, const int32_t* tile_active,
                          int lcap, int* level);
 /* Min-cut certificate (mode 1 fingerprint, 2 entry by entry, 3 fingerprint
- * over the pairs whose residual left their capacities) of a given residual,
- * excess and heights (T = {h < n}) on slot 0: violation count, cut(V\T, T),
+ * over the pairs whose residual left their capacities, 4 = 3 reading the cut
+ * entries from the rows of V\T instead of T's) of a given residual, excess
+ * and heights (T = {h < n}) on slot 0: violation count, cut(V\T, T),
  * excess(t) and whether the pair-sum fingerprint is 0. */
 int mf_dbg_certify(mf_engine* e, int gid, int s, int t, const int64_t* res, const int64_t* excess,
                    const int32_t* height, int mode, int64_t* bad, int64_t* cut, int64_t* excess_t,

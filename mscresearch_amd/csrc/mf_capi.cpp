@@ -983,7 +983,10 @@ int mf_dbg_certify(mf_engine* e, int gid, int s, int t, const int64_t* res, cons
     int rc = check_query(e, gid, s, t);
     if (rc != MF_OK) return rc;
     if (!res || !excess || !height || !bad || !cut || !excess_t || !fp_zero) return fail(MF_EINVAL, "null argument");
-    if (mode < 1 || mode > 3) return fail(MF_EINVAL, "mode must be 1, 2 or 3");
+    if (mode < 1 || mode > 4) return fail(MF_EINVAL, "mode must be 1, 2, 3 or 4");
+    // mode 4: mode 3 finding the cut entries in the rows of V \ T
+    const bool sside = mode == 4;
+    if (sside) mode = 3;
     rc = e->eng.dbg_begin(gid, s, t);
     if (rc != MF_OK) return rc;
     SlotView& v = e->eng.view(0);
@@ -1006,6 +1009,7 @@ int mf_dbg_certify(mf_engine* e, int gid, int s, int t, const int64_t* res, cons
     HIPCK(hipMemcpy(v.h, height, sizeof(int32_t) * g.n, hipMemcpyHostToDevice));
     const int saved = e->eng.params().cert;
     e->eng.set_cert_mode(mode);
+    v.cert_sside = sside ? 1 : 0;
     Plan p;
     p.stages = ST_RESET_PRE | ST_CERT;
     p.reset_pre = RST_CERT | RST_WORK | RST_ERR;

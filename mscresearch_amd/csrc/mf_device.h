@@ -189,6 +189,10 @@ struct SlotView {
     // loaded), so bottom-up relabel levels may use them.
     uint8_t* recv = nullptr;
     int32_t recv_ok = 0;
+    // the sparse certificate finds the cut entries V \ T -> T from the rows
+    // of V \ T (1) or, through their pairs, from the rows of T (0): the
+    // same entries, read from the smaller side's rows
+    int32_t cert_sside = 0;
     // 1 = bounded injection: the preflow puts B = min(out-capacity of s,
     // in-capacity of t) on s instead of saturating its row, and s is then an
     // ordinary vertex (labelled by the relabel, active, discharged); 0 = the

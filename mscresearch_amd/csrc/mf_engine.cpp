@@ -537,7 +537,10 @@ Plan SlotSched::plan(const Ctr& c) {
             return bfs_step(c, bfs_mode_);
         }
         stats.bfs_levels += L;
-        if (bfs_mode_ == BFS_GR) gr_depth_ = L;
+        if (bfs_mode_ == BFS_GR) {
+            gr_depth_ = L;
+            gr_visited_ = c.visited;
+        }
         p.stages = ST_RESET_ACT | ST_ACTIVE;
         p.reset_act = RST_ACTIVE;
         just_gr_ = true;
@@ -570,6 +573,7 @@ Plan SlotSched::plan(const Ctr& c) {
                 if (P_->cert && P_->spec_cert && phase_ == 0 && !spec_failed_ && !deficits_) {
                     p.stages = ST_RESET_PRE | ST_CERT;
                     p.reset_pre = RST_CERT;
+                    p.cert_sside = 2 * gr_visited_ > g_.n;
                     spec_ = true;
                     st_ = S_CERT;
                     return p;
@@ -595,6 +599,7 @@ Plan SlotSched::plan(const Ctr& c) {
             if (P_->cert) {
                 p.stages = ST_RESET_PRE | ST_CERT;
                 p.reset_pre = RST_CERT;
+                p.cert_sside = 2 * (int64_t)c.visited > g_.n;
                 st_ = S_CERT;
                 return p;
             }
@@ -614,6 +619,7 @@ Plan SlotSched::plan(const Ctr& c) {
             (at_t || at_s)) {
             p.stages = ST_RESET_PRE | (at_t ? ST_SINK : ST_SOURCE) | ST_CERT;
             p.reset_pre = RST_CERT;
+            p.cert_sside = !at_t;  // T = V \ {s}: scan s's row
             spec_ = true;
             stats.sink_cuts++;
             st_ = S_CERT;
@@ -1253,6 +1259,7 @@ int Engine::step(std::vector<Result>& out) {
         }
         S.view.phase = S.sched.phase();
         S.view.seed_t_only = plans[i].seed_t_only ? 1 : 0;
+        S.view.cert_sside = plans[i].cert_sside ? 1 : 0;
         S.view.K = S.sched.K();
         S.view.disc_cap = S.sched.disc_cap();
         S.sched.stats.steps++;

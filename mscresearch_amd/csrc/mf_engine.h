@@ -367,6 +367,7 @@ struct Plan {
     int small_wgs = 1;        // BK_SMALL: workgroups per slot (1 or Params::small_wgs)
     bool seed_t_only = false; // ST_GR_SEED in phase 0 after a count that saw no deficit: t is the one seed
     bool bu_bits = false;     // BK_BU: the first bottom-up level of this BFS builds level L's bitmap first
+    bool cert_sside = false;  // ST_CERT: the sparse certificate scans the rows of V \ T (the smaller side)
     long long small_edges = 0; // BK_SMALL: frontier-entry bound of the multi-level launch (0 = Params::small_edges)
 };
 
@@ -471,6 +472,9 @@ private:
     // starts (capacity decreases), cleared only by a full count that sees
     // none.  Sparse counts do not see deficits, so they never clear it.
     bool deficits_ = false;
+    // vertices the last exact relabel labelled: the size of T = {h < n} a
+    // certificate on its labels (or the discharge's, which only shrink T) sees
+    int64_t gr_visited_ = 0;
     // a multi-workgroup BFS barrier gave up (the device was too busy to hold
     // all its workgroups): this query's BFS runs single-workgroup from here on
     bool no_multi_ = false;

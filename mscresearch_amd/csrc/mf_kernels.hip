@@ -153,7 +153,7 @@ struct GSlot {
     int32_t gcap;
     GAS uint64_t* fbits;
     GAS uint8_t* recv;
-    int32_t recv_ok;
+    int32_t recv_ok, cert_sside;
     GAS uint8_t* emark;
     GAS int32_t* mlist;
     int32_t mcap;
@@ -209,6 +209,7 @@ __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* lis
     S.fbits = as_global(v.fbits);
     S.recv = as_global(v.recv);
     S.recv_ok = v.recv_ok;
+    S.cert_sside = v.cert_sside;
     S.emark = as_global(v.emark);
     S.mlist = as_global(v.mlist);
     S.mcap = v.mcap;
@@ -3356,9 +3357,11 @@ __global__ __launch_bounds__(256) void k_cert_sp_pairs(const SlotView* __restric
 }
 
 // Sparse mode, vertex side: -u(v) * stored excess(v) and the vertex checks
-// for every vertex; then, in workgroups holding a vertex of T, the entries
-// of T's rows (found as in k_cert_fp) for the cut and closedness, the
-// residual of an unmarked pair being its capacity.
+// for every vertex; then the cut entries V\T -> T for the cut and
+// closedness, the residual of an unmarked pair being its capacity: found in
+// the rows of T through their pairs (workgroups holding a vertex of T), or,
+// with cert_sside, in the rows of V\T themselves (workgroups holding one),
+// whichever side the scheduler expects to be smaller.
 __global__ __launch_bounds__(256) void k_cert_sp_rows(const SlotView* __restrict__ slots,
                                                       const int* __restrict__ list) {
     const GSlot S = load_slot(slots, list);
@@ -3390,11 +3393,12 @@ __global__ __launch_bounds__(256) void k_cert_sp_rows(const SlotView* __restrict
             }
             if (v == S.s && inT) ++bad;
             ++verts;
-            tin[i] = inT ? 1 : 0;
             rp[i] = S.g.row_ptr[v];
             if (i == nv - 1) rp[nv] = S.g.row_ptr[v + 1];
         }
-        if (!__syncthreads_or(inT)) continue;  // uniform per block
+        const bool scan = i < nv && (S.cert_sside ? !inT : inT);
+        if (i < nv) tin[i] = scan ? 1 : 0;
+        if (!__syncthreads_or(scan)) continue;  // uniform per block
         const int E1 = rp[nv];
         int lo = 0;  // rows only grow along e
         for (int e = rp[0] + i; e < E1; e += 256) {
@@ -3407,7 +3411,16 @@ __global__ __launch_bounds__(256) void k_cert_sp_rows(const SlotView* __restrict
             lo = rr;
             if (!tin[rr]) continue;
             ++edges;
-            if (S.h[S.g.col[e]] >= n) {  // pair col[e] -> row enters T
+            const int w = S.g.col[e];
+            if (S.cert_sside) {
+                if (S.h[w] < n) {  // the row's entry row -> col[e] enters T
+                    const int rv = S.g.rev[e];
+                    const int64_t c = S.g.cap[e];
+                    cut += c;
+                    const int64_t x = (S.emark[e] | S.emark[rv]) ? S.res[e] : c;
+                    if (x > 0) ++bad;
+                }
+            } else if (S.h[w] >= n) {  // pair col[e] -> row enters T
                 const int rv = S.g.rev[e];
                 const int64_t c = S.g.cap[rv];
                 cut += c;

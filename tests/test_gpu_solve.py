@@ -314,7 +314,7 @@ def test_gpu_certificate_catches_planted_mistakes(c1):
         s, t = c1.q(a, b)
         r, res, ex, h = eng.dbg_solve_state(gid, s, t, c1.n, c1.E)
         assert O.certify(og, res, s, t, r.flow, excess=ex) == 0
-        for mode in (1, 2, 3):
+        for mode in (1, 2, 3, 4):
             bad, cut, ext, fz = eng.dbg_certify(gid, s, t, res, ex, h, mode)
             assert (bad, fz, cut, ext) == (0, True, r.flow, r.flow), mode
         # (a) two pair sums off by +1 / -1 inside one row: the recomputed
@@ -330,6 +330,7 @@ def test_gpu_certificate_catches_planted_mistakes(c1):
         assert not eng.dbg_certify(gid, s, t, bad_res, ex, h, 1)[3]
         assert eng.dbg_certify(gid, s, t, bad_res, ex, h, 2)[0] > 0
         assert not eng.dbg_certify(gid, s, t, bad_res, ex, h, 3)[3]
+        assert not eng.dbg_certify(gid, s, t, bad_res, ex, h, 4)[3]
         # (b) stored excess off by one at an inner vertex: the full mode
         # counts it; the fingerprint mode's vertex terms leave a nonzero sum
         # (or count it, for a vertex of T or of a giant row)
@@ -338,7 +339,7 @@ def test_gpu_certificate_catches_planted_mistakes(c1):
         bad_ex[w] += 1
         assert O.certify(og, res, s, t, r.flow, excess=bad_ex) != 0
         assert eng.dbg_certify(gid, s, t, res, bad_ex, h, 2)[0] > 0
-        for mode in (1, 3):
+        for mode in (1, 3, 4):
             b1 = eng.dbg_certify(gid, s, t, res, bad_ex, h, mode)
             assert b1[0] > 0 or not b1[3], mode
         # (b') a residual off by one with the stored excess of its row
@@ -349,19 +350,19 @@ def test_gpu_certificate_catches_planted_mistakes(c1):
         bad_res[e3] += 1
         bad_ex = ex.copy()
         bad_ex[owner[e3]] += 1
-        for mode in (1, 3):
+        for mode in (1, 3, 4):
             b1 = eng.dbg_certify(gid, s, t, bad_res, bad_ex, h, mode)
             assert b1[0] > 0 or not b1[3], mode
         assert eng.dbg_certify(gid, s, t, bad_res, bad_ex, h, 2)[0] > 0
         # (c) a negative residual
         bad_res = res.copy()
         bad_res[int(rng.choice(np.nonzero(res > 0)[0]))] = -1
-        for mode in (1, 2, 3):
+        for mode in (1, 2, 3, 4):
             assert eng.dbg_certify(gid, s, t, bad_res, ex, h, mode)[0] > 0
         # (d) heights claiming s reaches t (s inside T)
         bad_h = h.copy()
         bad_h[s] = 0
-        for mode in (1, 2, 3):
+        for mode in (1, 2, 3, 4):
             assert eng.dbg_certify(gid, s, t, res, ex, bad_h, mode)[0] > 0
         # (e) one unit pushed back across the cut (pair sum and stored
         # excesses follow it): a residual entry now enters T
@@ -375,7 +376,7 @@ def test_gpu_certificate_catches_planted_mistakes(c1):
             bad_res[rv] += 1
             bad_ex[owner[e]] -= 1
             bad_ex[og.col[e]] += 1
-            for mode in (1, 2, 3):
+            for mode in (1, 2, 3, 4):
                 got = eng.dbg_certify(gid, s, t, bad_res, bad_ex, h, mode)
                 assert got != (0, r.flow, r.flow, True), mode
 
