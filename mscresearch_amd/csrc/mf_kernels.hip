@@ -1808,6 +1808,8 @@ __global__ __launch_bounds__(256) void k_active(const SlotView* __restrict__ slo
         const int v = (int)i;
         ++nv;
         if (v == S.s && !S.inj) continue;
+        // no flow has entered or left v (SlotView::recv): its excess is 0
+        if (S.recv_ok && v != S.t && S.recv[v] == 0) continue;
         const int64_t ex = S.excess[v];
         if (v == S.t) S.ctr->ex_t = ex;
         if (v == S.t && S.phase == 0) continue;

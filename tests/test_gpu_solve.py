@@ -183,6 +183,9 @@ def test_edge_cases():
     dict(bu_bits=0),                            # bottom-up levels test parents by their levels, not a bitmap
     dict(bu_bits=0, small_edges=0, bu_alpha=1.0),  # ... every level grid-wide, bottom-up early
     dict(small_edges=0, bu_alpha=1.0, bu_beta=4.0),  # bitmaps over runs of bottom-up levels between top-down ones
+    dict(bu_recv=0),                            # relabel levels read every row's residuals (no fresh-row tests)
+    dict(bu_recv=0, small_edges=0, bu_alpha=1.0),  # ... every level grid-wide, bottom-up early
+    dict(small_edges=0, bu_alpha=1.0, h_cut=0, rm_freq0=1, rm_freq_max=1),  # fresh rows beside RM repairs
 ])
 def test_scheduler_variants_stay_exact(c1, variant):
     log(f"variant {variant}")
