@@ -246,6 +246,8 @@ typedef struct mf_params {
                             2 = on in engines of more than one slot */
     int bu_recv;         /* 1: bottom-up relabel levels test rows no flow has entered or left through the
                             graph's forward-entry mask instead of their residuals */
+    int dis_queue;       /* 1: a step's list discharge runs as one persistent launch taking every slot's
+                            blocks off a device work queue; 0 (default): a grid per slot */
 } mf_params;
 
 void mf_params_default(mf_params* p);

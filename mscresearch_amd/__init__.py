@@ -54,7 +54,7 @@ class Params(C.Structure):
         ("push_first", C.c_int), ("slot_blocks", C.c_int), ("launch_blocks", C.c_int),
         ("bu_light", C.c_int), ("bu_bits", C.c_int), ("gr_work_deep", C.c_double),
         ("small_edges_deep", C.c_longlong), ("giant_rounds", C.c_int),
-        ("bu_recv", C.c_int),
+        ("bu_recv", C.c_int), ("dis_queue", C.c_int),
     ]
 
 

@@ -127,6 +127,7 @@ Params from_c(const mf_params* p) {
     P.small_edges_deep = p->small_edges_deep;
     P.giant_rounds = p->giant_rounds;
     P.bu_recv = p->bu_recv;
+    P.dis_queue = p->dis_queue;
     return P;
 }
 
@@ -537,6 +538,7 @@ void mf_params_default(mf_params* p) {
     p->small_edges_deep = P.small_edges_deep;
     p->giant_rounds = P.giant_rounds;
     p->bu_recv = P.bu_recv;
+    p->dis_queue = P.dis_queue;
 }
 
 int mf_engine_create(int nslots, const mf_params* p, mf_engine** out) {

@@ -834,7 +834,8 @@ int Engine::init(int nslots, const Params& P) {
     HIPCHK(hipEventCreateWithFlags(&ev_side_, hipEventDisableTiming));
     // views followed by the launch lists: a slot sits in at most one group per
     // stage plus three per extra fused cycle
-    list_cap_ = (size_t)nslots * (32 + 3 * (size_t)std::max(1, P.fuse));
+    // (+ the list discharge's work queue: 3 + nslots words)
+    list_cap_ = (size_t)nslots * (33 + 3 * (size_t)std::max(1, P.fuse)) + 3;
     HIPCHK(hipMalloc(&d_views_, sizeof(SlotView) * nslots + sizeof(int) * list_cap_));
     HIPCHK(hipMalloc(&d_hash_, sizeof(unsigned long long) * nslots));
     HIPCHK(hipHostMalloc(&h_stage_, sizeof(SlotView) * nslots + sizeof(int) * list_cap_));
@@ -1473,6 +1474,25 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     for (int i = 0; i < nslots_; ++i)
         if (slots_[i].live && has(i, ST_DISCHARGE)) max_cycles = std::max(max_cycles, std::min(plans[i].cycles, std::max(1, P_.fuse)));
     max_cycles = std::min(max_cycles, 64);
+    // the list discharge's device work queue {next = 0, total, prefix of the
+    // slots' blocks}, uploaded with the lists: the blocks are the ones the
+    // slots' last counts give (Plan::dis_grid)
+    size_t dq_off = 0;
+    int64_t dq_total = 0;
+    if (P_.dis_queue && g_dis[DK_LIST].cnt) {
+        dq_off = lpos;
+        const int cnt = g_dis[DK_LIST].cnt;
+        if (lpos + 3 + (size_t)cnt <= list_cap_) {
+            for (int q = 0; q < cnt; ++q) {
+                hl[lpos + 2 + q] = (int)dq_total;
+                dq_total += plans[hl[g_dis[DK_LIST].off + q]].dis_grid;
+            }
+            hl[lpos] = 0;
+            hl[lpos + 1] = (int)dq_total;
+            hl[lpos + 2 + cnt] = (int)dq_total;
+        }
+        lpos += 3 + (size_t)cnt;
+    }
     if (lpos > list_cap_) { set_error("launch list overflow"); return MF_EINVAL; }
     const int nlive = live();
     if (nlive == 0) return 0;
@@ -1620,7 +1640,14 @@ int Engine::launch_plans(std::vector<Plan>& plans) {
     }
     if (g_dis[DK_LIST].cnt) {
         const int k = tb(WF_DISCHARGE);
-        launch_discharge_list(st, d_views_, L(g_dis[DK_LIST]), g_dis[DK_LIST].cnt, fit(g_dis[DK_LIST], maxof(g_dis[DK_LIST], [&](int i) { return plans[i].dis_grid; })), giants(g_dis[DK_LIST]));
+        if (P_.dis_queue) {
+            // one persistent launch: as many workgroups as the slots' blocks,
+            // at most the per-launch cap
+            const int wg = (int)std::max<int64_t>(1, std::min<int64_t>(dq_total, kLaunchBlocks));
+            launch_discharge_queue(st, d_views_, L(g_dis[DK_LIST]), g_dis[DK_LIST].cnt, d_list_base + dq_off, wg, giants(g_dis[DK_LIST]));
+        } else {
+            launch_discharge_list(st, d_views_, L(g_dis[DK_LIST]), g_dis[DK_LIST].cnt, fit(g_dis[DK_LIST], maxof(g_dis[DK_LIST], [&](int i) { return plans[i].dis_grid; })), giants(g_dis[DK_LIST]));
+        }
         te(k);
     }
     if (g_dis[DK_PATH].cnt) {

@@ -120,6 +120,12 @@ struct Params {
     // 1: bottom-up relabel levels test rows no flow has entered or left
     // through the graph's forward mask, without reading their residuals
     int bu_recv = 1;
+    // 1: the list discharge of a step runs as one persistent launch over a
+    // device work queue of every slot's blocks; 0: one grid per slot, whose
+    // workgroups past a slot's blocks exit at once (measured faster on
+    // R-MAT 22: the queue's atomics on one word cost more than the empty
+    // workgroups the hardware retires)
+    int dis_queue = 0;
     // bottom-up levels test candidate parents in a frontier bitmap (n / 8
     // bytes per slot) instead of gathering their levels (0 = gather levels)
     int bu_bits = 1;

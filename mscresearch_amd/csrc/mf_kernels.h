@@ -84,6 +84,12 @@ void launch_active_sparse(hipStream_t st, const SlotView* slots, const int* list
 // whole 2048-entry chunks (1) or in rounds that stop once the row is dry (2).
 void launch_discharge_list(hipStream_t st, const SlotView* slots, const int* list, int nlist,
                            int gridx, int giants);
+// The same discharge for every slot of the list in one persistent launch of
+// `workgroups` workgroups taking 256-row blocks off a device work queue
+// dq = {0, total, prefix of the slots' blocks [nlist + 1]} that the caller
+// uploads with the list; giants as above.
+void launch_discharge_queue(hipStream_t st, const SlotView* slots, const int* list, int nlist, int32_t* dq,
+                            int workgroups, int giants);
 // One workgroup of `threads` (256, 512 or 1024) per tile, tile heights and
 // pending excess staged in LDS; split > 1: up to `split` workgroups per tile
 // by excess score (k_tile_alloc), each staging its part of the tile.

@@ -170,8 +170,14 @@ __device__ __forceinline__ GAS T* as_global(T* p) {
     return (GAS T*)p;
 }
 
+__device__ __forceinline__ GSlot load_slot_at(const SlotView* slots, int slot);
+
 __device__ __forceinline__ GSlot load_slot(const SlotView* slots, const int* list) {
-    const SlotView v = slots[list[blockIdx.y]];
+    return load_slot_at(slots, list[blockIdx.y]);
+}
+
+__device__ __forceinline__ GSlot load_slot_at(const SlotView* slots, int slot) {
+    const SlotView v = slots[slot];
     GSlot S;
     S.g.n = v.g.n;
     S.g.E = v.g.E;
@@ -2278,6 +2284,73 @@ __global__ __launch_bounds__(256) void k_discharge_list(const SlotView* __restri
         }
     }
     work_flush(S, wk, red, WF_DISCHARGE);
+}
+
+// One persistent launch for the list discharge of every slot of a step, fed
+// by a device work queue: dq = {next, total, prefix[0..nlist]}, the prefix of
+// the slots' blocks (light rows 256 to a block, heavy rows 4) over the
+// launch list, written by the host from the counts it read and uploaded with
+// the list (so `next` starts at 0 every step).  The workgroups take blocks
+// off it with one atomic each until it is empty, so slots with uneven active
+// lists share one grid instead of each getting the largest one's.  A block is
+// k_discharge_list's block b of its slot; each wave adds its work counters
+// to its block's slot.
+__global__ __launch_bounds__(256) void k_discharge_queue(const SlotView* __restrict__ slots,
+                                                         const int* __restrict__ list, int nlist,
+                                                         int32_t* __restrict__ dq) {
+    __shared__ int s_item, s_j;
+    const int total = dq[1];
+    TileCtx tc;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const int item = atomicAdd(&dq[0], 1);
+            int j = 0;
+            if (item < total) {  // the last list entry whose prefix is <= item
+                int lo = 0, hi = nlist - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (dq[2 + mid] <= item) lo = mid;
+                    else hi = mid - 1;
+                }
+                j = lo;
+            }
+            s_item = item;
+            s_j = j;
+        }
+        __syncthreads();
+        const int item = s_item, j = s_j;
+        __syncthreads();  // s_item / s_j are rewritten by the next take
+        if (item >= total) break;  // uniform per block
+        const GSlot S = load_slot_at(slots, list[j]);
+        const int nl = S.ctr->nlight, nh = S.ctr->nheavy;
+        const int lb = (nl + 255) / 256;
+        const int b = item - dq[2 + j];
+        Work wk;
+        if (b < lb) {
+            const int i = b * 256 + threadIdx.x;
+            if (i < nl) lane_discharge(S, tc, S.fa[i], S.K, wk);
+        } else {
+            const int k = (b - lb) * 4 + (int)(threadIdx.x / WAVE);
+            if (k < nh) {
+                const int v = S.fa[S.g.n - 1 - k];
+                if (S.g.row_ptr[v + 1] - S.g.row_ptr[v] <= kGiantDeg) wave_discharge(S, tc, v, S.Kh, wk);
+            }
+        }
+        uint64_t x[4] = {wk.edges, wk.verts, wk.pushes, wk.relabels};
+        for (int q = 0; q < 4; ++q) {
+            for (int o = WAVE / 2; o > 0; o >>= 1) x[q] += __shfl_xor(x[q], o, WAVE);
+            if (lane_id() == 0 && x[q]) addu64(&S.ctr->wk[WF_DISCHARGE][q], x[q]);
+        }
+    }
+}
+
+void launch_giant(hipStream_t st, const SlotView* slots, const int* list, int nlist, int giants);
+
+void launch_discharge_queue(hipStream_t st, const SlotView* slots, const int* list, int nlist, int32_t* dq,
+                            int workgroups, int giants) {
+    hipLaunchKernelGGL(k_discharge_queue, dim3(workgroups > 0 ? workgroups : 1), dim3(256), 0, st, slots, list,
+                       nlist, dq);
+    if (giants) launch_giant(st, slots, list, nlist, giants);
 }
 
 // Giant rows (longer than kGiantDeg), spread over the whole grid in tasks of

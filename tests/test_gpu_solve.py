@@ -183,6 +183,7 @@ def test_edge_cases():
     dict(bu_bits=0),                            # bottom-up levels test parents by their levels, not a bitmap
     dict(bu_bits=0, small_edges=0, bu_alpha=1.0),  # ... every level grid-wide, bottom-up early
     dict(small_edges=0, bu_alpha=1.0, bu_beta=4.0),  # bitmaps over runs of bottom-up levels between top-down ones
+    dict(dis_queue=1),                          # list discharge as one launch over a device work queue
     dict(bu_recv=0),                            # relabel levels read every row's residuals (no fresh-row tests)
     dict(bu_recv=0, small_edges=0, bu_alpha=1.0),  # ... every level grid-wide, bottom-up early
     dict(small_edges=0, bu_alpha=1.0, h_cut=0, rm_freq0=1, rm_freq_max=1),  # fresh rows beside RM repairs
@@ -215,13 +216,16 @@ def test_sink_cut_stops_early_and_stays_exact(c1):
     assert early > 0
 
 
-def test_batched_equals_alone(c1):
-    """Each result is the same alone and in the mix (continuous batching)."""
+@pytest.mark.parametrize("extra", [{}, dict(dis_queue=1)])
+def test_batched_equals_alone(c1, extra):
+    """Each result is the same alone and in the mix (continuous batching);
+    with dis_queue=1 the mix's list discharge is one launch over a device work
+    queue of every slot's blocks."""
     qs, qt = M.uniform_queries(c1.n, 64, 1002)
     q = [c1.q(a, b) for a, b in zip(qs, qt)]
     alone_eng, gid = engine_with(c1)
     alone = [alone_eng.solve(gid, s, t).flow for s, t in q]
-    mix_eng, gid2 = engine_with(c1, nslots=24)
+    mix_eng, gid2 = engine_with(c1, nslots=24, **extra)
     for i, (s, t) in enumerate(q):
         mix_eng.submit(gid2, s, t, i)
     res = mix_eng.run()
