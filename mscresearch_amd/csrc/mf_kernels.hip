@@ -524,18 +524,22 @@ __global__ __launch_bounds__(256) void k_preflow(const SlotView* __restrict__ sl
     const GSlot S = load_slot(slots, list);
     __shared__ int64_t red[256 / WAVE];
     const int s = S.s;
-    // In-capacity of t: over t's row, pcap - cap is the capacity of each
-    // entry's pair into t.
-    const int tb = S.g.row_ptr[S.t], te = S.g.row_ptr[S.t + 1];
+    // workgroup 0: the counters; every workgroup: its share of s's row
     int64_t in_t = 0;
-    for (int e = tb + threadIdx.x; e < te; e += blockDim.x) in_t += S.g.pcap[e] - S.g.cap[e];
-    in_t = block_sum<int64_t>(in_t, red);
-    if (threadIdx.x == 0) {
-        S.ctr->incap_t = in_t;
-        S.ctr->outcap_s = S.g.capsum[s];
-        if (S.recv) S.recv[s] = 1;  // s pushes without receiving
+    if (blockIdx.x == 0) {
+        // In-capacity of t: over t's row, pcap - cap is the capacity of each
+        // entry's pair into t.
+        const int tb = S.g.row_ptr[S.t], te = S.g.row_ptr[S.t + 1];
+        for (int e = tb + threadIdx.x; e < te; e += blockDim.x) in_t += S.g.pcap[e] - S.g.cap[e];
+        in_t = block_sum<int64_t>(in_t, red);
+        if (threadIdx.x == 0) {
+            S.ctr->incap_t = in_t;
+            S.ctr->outcap_s = S.g.capsum[s];
+            if (S.recv) S.recv[s] = 1;  // s pushes without receiving
+        }
     }
     if (S.inj) {
+        if (blockIdx.x != 0) return;
         // Bounded injection: no flow can exceed the out-capacity of s or the
         // in-capacity of t (each an s-t cut), so s starts with B units of
         // excess and an unsaturated row.
@@ -547,10 +551,10 @@ __global__ __launch_bounds__(256) void k_preflow(const SlotView* __restrict__ sl
         }
         return;
     }
-    if (threadIdx.x == 0) S.ctr->inject = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) S.ctr->inject = 0;
     const int rb = S.g.row_ptr[s], re = S.g.row_ptr[s + 1];
     int64_t sum = 0;
-    for (int e = rb + threadIdx.x; e < re; e += blockDim.x) {
+    for (int e = rb + (int)(blockIdx.x * blockDim.x + threadIdx.x); e < re; e += (int)(gridDim.x * blockDim.x)) {
         const int64_t d = ld64(&S.res[e]);
         if (d > 0) {
             move_res(S, e, d);
@@ -563,7 +567,8 @@ __global__ __launch_bounds__(256) void k_preflow(const SlotView* __restrict__ sl
 }
 
 void launch_preflow(hipStream_t st, const SlotView* slots, const int* list, int nlist) {
-    hipLaunchKernelGGL(k_preflow, dim3(1, nlist), dim3(256), 0, st, slots, list);
+    // s is often a hub: its row over 16 workgroups per slot
+    hipLaunchKernelGGL(k_preflow, dim3(16, nlist), dim3(256), 0, st, slots, list);
 }
 
 // ------------------------------------------------------------- BFS (a12/a13)
