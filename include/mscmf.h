@@ -475,8 +475,9 @@ int mf_dbg_hot_select(mf_engine* e, int gid, const int32_t* tile_cnt, int k, int
  * barriers (ticks <= 0: the default of 2 s).  A barrier that waits longer
  * flags its slot and the BFS restarts single-workgroup; a tiny bound forces
  * that path from the host.  *restarts (may be NULL) gets the restarts of the
- * queries the engine has retired so far.  No counterpart in the reference,
- * whose BFS is one launch per level (BC:1538-1572). */
+ * queries the engine has retired so far.  Call it while the engine is idle
+ * (nothing live).  No counterpart in the reference, whose BFS is one launch
+ * per level (BC:1538-1572). */
 int mf_dbg_barrier_ticks(mf_engine* e, long long ticks, long long* restarts);
 /* Solve without pruning on slot 0 and copy the final residual, excess and
  * heights out (for an independent CPU certificate). */
