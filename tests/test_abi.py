@@ -69,6 +69,8 @@ def test_params_defaults_are_the_reference_constants():
     # speculative certificate, sparse counts
     assert (p.fuse, p.graphs, p.spec_cert, p.sparse_count, p.small_wgs) == (8, 1, 1, 1, 4)
     assert p.small_edges == 1 << 13 and p.small_edges_deep == 1 << 17 and p.mark_list == 1 << 20
+    # fresh-row relabel tests on, the list discharge as a grid per slot
+    assert (p.bu_recv, p.dis_queue, p.giant_rounds) == (1, 0, 2)
 
 
 def test_cli_usage_and_errors(tmp_path):
